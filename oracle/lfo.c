@@ -1,0 +1,718 @@
+/* Test infrastructure only: CPU restatement ("oracle") of the LightFEM Q4
+ * assembly + PCG path. Only tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg use it, and only as the checker. Each function restates the
+ * reference stage cited beside it, in the same loop and summation order, built
+ * with -ffp-contract=off so no multiply-add is fused (the reference is built for
+ * plain x86-64, which has no FMA either). */
+#include "lfo.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const double kGauss = 0.57735026918962576451; /* element.cpp:132 */
+
+static double max0(double v) { return (0.0 < v) ? v : 0.0; } /* std::max(0.0, v) */
+
+/* element.cpp:16-32 */
+static void constitutive(double e, double nu, int plane_mode, double d[9]) {
+    if (plane_mode == LFO_PLANE_STRESS) {
+        const double factor = e / (1.0 - nu * nu);
+        d[0] = factor; d[1] = factor * nu; d[2] = 0.0;
+        d[3] = factor * nu; d[4] = factor; d[5] = 0.0;
+        d[6] = 0.0; d[7] = 0.0; d[8] = factor * (1.0 - nu) * 0.5;
+    } else {
+        const double factor = e / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        d[0] = factor * (1.0 - nu); d[1] = factor * nu; d[2] = 0.0;
+        d[3] = factor * nu; d[4] = factor * (1.0 - nu); d[5] = 0.0;
+        d[6] = 0.0; d[7] = 0.0; d[8] = factor * (1.0 - 2.0 * nu) * 0.5;
+    }
+}
+
+/* element.cpp:34-73; returns 0, or 1 on a non-positive Jacobian. */
+static int kinematics(const double xy[8], double xi, double eta, double b[24], double* det_out) {
+    const double dxi[4] = {-0.25 * (1.0 - eta), +0.25 * (1.0 - eta), +0.25 * (1.0 + eta), -0.25 * (1.0 + eta)};
+    const double deta[4] = {-0.25 * (1.0 - xi), -0.25 * (1.0 + xi), +0.25 * (1.0 + xi), +0.25 * (1.0 - xi)};
+    double dx_dxi = 0.0, dx_deta = 0.0, dy_dxi = 0.0, dy_deta = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        dx_dxi += dxi[i] * xy[2 * i];
+        dx_deta += deta[i] * xy[2 * i];
+        dy_dxi += dxi[i] * xy[2 * i + 1];
+        dy_deta += deta[i] * xy[2 * i + 1];
+    }
+    const double det = dx_dxi * dy_deta - dx_deta * dy_dxi;
+    if (!(det > 1.0e-14) || !isfinite(det)) return 1;
+    const double inv00 = dy_deta / det;
+    const double inv01 = -dx_deta / det;
+    const double inv10 = -dy_dxi / det;
+    const double inv11 = dx_dxi / det;
+    for (int i = 0; i < 4; ++i) {
+        const double dnx = inv00 * dxi[i] + inv10 * deta[i];
+        const double dny = inv01 * dxi[i] + inv11 * deta[i];
+        b[2 * i] = dnx;
+        b[2 * i + 1] = 0.0;
+        b[8 + 2 * i] = 0.0;
+        b[8 + 2 * i + 1] = dny;
+        b[16 + 2 * i] = dny;
+        b[16 + 2 * i + 1] = dnx;
+    }
+    *det_out = det;
+    return 0;
+}
+
+/* element.cpp:130-161 */
+int lfo_q4_stiffness(const double xy[8], double young, double nu, double thickness, int plane_mode,
+                     double ke[64]) {
+    const double points[4][2] = {{-kGauss, -kGauss}, {kGauss, -kGauss}, {kGauss, kGauss}, {-kGauss, kGauss}};
+    double d[9];
+    constitutive(young, nu, plane_mode, d);
+    for (int k = 0; k < 64; ++k) ke[k] = 0.0;
+    for (int p = 0; p < 4; ++p) {
+        double b[24], det;
+        if (kinematics(xy, points[p][0], points[p][1], b, &det)) return 1;
+        double db[24];
+        for (int k = 0; k < 24; ++k) db[k] = 0.0;
+        for (int row = 0; row < 3; ++row)
+            for (int column = 0; column < 8; ++column)
+                for (int inner = 0; inner < 3; ++inner) db[8 * row + column] += d[3 * row + inner] * b[8 * inner + column];
+        const double weight = det * thickness;
+        for (int row = 0; row < 8; ++row)
+            for (int column = 0; column < 8; ++column) {
+                double contribution = 0.0;
+                for (int s = 0; s < 3; ++s) contribution += b[8 * s + row] * db[8 * s + column];
+                ke[8 * row + column] += contribution * weight;
+            }
+    }
+    return 0;
+}
+
+/* element.cpp:75-95 and 212-248 */
+static void field_at(const double b[24], const double d[9], const double u[8], double out[7]) {
+    double strain[3] = {0.0, 0.0, 0.0}, stress[3] = {0.0, 0.0, 0.0};
+    for (int row = 0; row < 3; ++row)
+        for (int column = 0; column < 8; ++column) strain[row] += b[8 * row + column] * u[column];
+    for (int row = 0; row < 3; ++row)
+        for (int column = 0; column < 3; ++column) stress[row] += d[3 * row + column] * strain[column];
+    const double sx = stress[0], sy = stress[1], txy = stress[2];
+    out[0] = strain[0]; out[1] = strain[1]; out[2] = strain[2];
+    out[3] = sx; out[4] = sy; out[5] = txy;
+    out[6] = sqrt(max0(sx * sx - sx * sy + sy * sy + 3.0 * txy * txy));
+}
+
+int lfo_q4_fields(const double xy[8], double young, double nu, double thickness, int plane_mode,
+                  const double u[8], double centre[7], double gp[36]) {
+    (void)thickness;
+    const double points[4][2] = {{-kGauss, -kGauss}, {kGauss, -kGauss}, {kGauss, kGauss}, {-kGauss, kGauss}};
+    double d[9], b[24], det;
+    constitutive(young, nu, plane_mode, d);
+    if (kinematics(xy, 0.0, 0.0, b, &det)) return 1;
+    field_at(b, d, u, centre);
+    for (int p = 0; p < 4; ++p) {
+        if (kinematics(xy, points[p][0], points[p][1], b, &det)) return 1;
+        gp[9 * p] = points[p][0];
+        gp[9 * p + 1] = points[p][1];
+        field_at(b, d, u, gp + 9 * p + 2);
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------- pattern */
+
+static int cmp_i64(const void* a, const void* b) {
+    const int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+    return (x > y) - (x < y);
+}
+
+/* sparse.cpp:15-41: per-row column set (diagonal always inserted), sorted. */
+static int64_t build_rows(int64_t ndofs, int64_t nelem, const int64_t* edofs, int64_t* row_ptr, int64_t* col) {
+    int64_t* count = calloc((size_t)ndofs + 1, sizeof(int64_t));
+    for (int64_t r = 0; r < ndofs; ++r) count[r] = 1;
+    for (int64_t e = 0; e < nelem; ++e)
+        for (int a = 0; a < 8; ++a) count[edofs[8 * e + a]] += 8;
+    int64_t* start = malloc(((size_t)ndofs + 1) * sizeof(int64_t));
+    start[0] = 0;
+    for (int64_t r = 0; r < ndofs; ++r) start[r + 1] = start[r] + count[r];
+    int64_t* raw = malloc((size_t)start[ndofs] * sizeof(int64_t));
+    int64_t* fill = malloc((size_t)ndofs * sizeof(int64_t));
+    for (int64_t r = 0; r < ndofs; ++r) {
+        fill[r] = start[r];
+        raw[fill[r]++] = r;
+    }
+    for (int64_t e = 0; e < nelem; ++e)
+        for (int a = 0; a < 8; ++a) {
+            const int64_t row = edofs[8 * e + a];
+            for (int c = 0; c < 8; ++c) raw[fill[row]++] = edofs[8 * e + c];
+        }
+    int64_t nnz = 0;
+    if (row_ptr) row_ptr[0] = 0;
+    for (int64_t r = 0; r < ndofs; ++r) {
+        int64_t* seg = raw + start[r];
+        const int64_t len = start[r + 1] - start[r];
+        qsort(seg, (size_t)len, sizeof(int64_t), cmp_i64);
+        for (int64_t i = 0; i < len; ++i) {
+            if (i > 0 && seg[i] == seg[i - 1]) continue;
+            if (col) col[nnz] = seg[i];
+            ++nnz;
+        }
+        if (row_ptr) row_ptr[r + 1] = nnz;
+    }
+    free(count); free(start); free(raw); free(fill);
+    return nnz;
+}
+
+int64_t lfo_pattern_nnz(int64_t ndofs, int64_t nelem, const int64_t* edofs) {
+    return build_rows(ndofs, nelem, edofs, NULL, NULL);
+}
+
+/* sparse.cpp:65-74 value_index (lower_bound in the row) */
+static int64_t value_index(const int64_t* row_ptr, const int64_t* col, int64_t row, int64_t column) {
+    int64_t lo = row_ptr[row], hi = row_ptr[row + 1];
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (col[mid] < column) lo = mid + 1; else hi = mid;
+    }
+    if (lo == row_ptr[row + 1] || col[lo] != column) return -1;
+    return lo;
+}
+
+/* sparse.cpp:15-58 */
+int lfo_pattern_build(int64_t ndofs, int64_t nelem, const int64_t* edofs, int64_t* row_ptr, int64_t* col,
+                      int64_t* emap) {
+    build_rows(ndofs, nelem, edofs, row_ptr, col);
+    for (int64_t e = 0; e < nelem; ++e)
+        for (int r = 0; r < 8; ++r)
+            for (int c = 0; c < 8; ++c) {
+                const int64_t idx = value_index(row_ptr, col, edofs[8 * e + r], edofs[8 * e + c]);
+                if (idx < 0) return 1;
+                emap[64 * e + 8 * r + c] = idx;
+            }
+    return 0;
+}
+
+/* sparse.cpp:227-242 */
+void lfo_csr_profile(int64_t n, const int64_t* row_ptr, const int64_t* col, int64_t out[3]) {
+    int64_t bandwidth = 0, profile = 0;
+    for (int64_t row = 0; row < n; ++row) {
+        int64_t minimum = row;
+        for (int64_t i = row_ptr[row]; i < row_ptr[row + 1]; ++i) {
+            const int64_t c = col[i];
+            const int64_t dist = row > c ? row - c : c - row;
+            if (bandwidth < dist) bandwidth = dist;
+            if (c <= row && c < minimum) minimum = c;
+        }
+        profile += row - minimum;
+    }
+    out[0] = row_ptr[n];
+    out[1] = bandwidth;
+    out[2] = profile;
+}
+
+/* --------------------------------------------------------------- ordering */
+
+typedef struct {
+    int64_t nnodes;
+    const int64_t* ids;    /* node ids, any order */
+    int64_t* sorted_ids;   /* ascending */
+    int64_t* sorted_pos;   /* position in ids[] */
+} id_index;
+
+static id_index* g_idx_for_sort;
+static int cmp_pos_by_id(const void* a, const void* b) {
+    const int64_t x = g_idx_for_sort->ids[*(const int64_t*)a], y = g_idx_for_sort->ids[*(const int64_t*)b];
+    return (x > y) - (x < y);
+}
+
+static void id_index_build(id_index* ix, int64_t nnodes, const int64_t* ids) {
+    ix->nnodes = nnodes;
+    ix->ids = ids;
+    ix->sorted_pos = malloc((size_t)nnodes * sizeof(int64_t));
+    ix->sorted_ids = malloc((size_t)nnodes * sizeof(int64_t));
+    for (int64_t i = 0; i < nnodes; ++i) ix->sorted_pos[i] = i;
+    g_idx_for_sort = ix;
+    qsort(ix->sorted_pos, (size_t)nnodes, sizeof(int64_t), cmp_pos_by_id);
+    for (int64_t i = 0; i < nnodes; ++i) ix->sorted_ids[i] = ids[ix->sorted_pos[i]];
+}
+
+static int64_t id_lookup(const id_index* ix, int64_t id) {
+    int64_t lo = 0, hi = ix->nnodes;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (ix->sorted_ids[mid] < id) lo = mid + 1; else hi = mid;
+    }
+    return (lo < ix->nnodes && ix->sorted_ids[lo] == id) ? ix->sorted_pos[lo] : -1;
+}
+
+static void id_index_free(id_index* ix) {
+    free(ix->sorted_pos);
+    free(ix->sorted_ids);
+}
+
+/* RCM key (degree, id): reordering.cpp:274-278 */
+static const int64_t* g_degree;
+static const int64_t* g_ids;
+static int cmp_degree_id(const void* a, const void* b) {
+    const int64_t x = *(const int64_t*)a, y = *(const int64_t*)b; /* positions */
+    if (g_degree[x] != g_degree[y]) return g_degree[x] < g_degree[y] ? -1 : 1;
+    return (g_ids[x] > g_ids[y]) - (g_ids[x] < g_ids[y]);
+}
+
+/* reordering.cpp:11-62; order_out receives node ids. */
+int lfo_rcm(int64_t nnodes, const int64_t* node_ids, int64_t nelem, const int64_t* enode_ids, int64_t* order_out) {
+    id_index ix;
+    id_index_build(&ix, nnodes, node_ids);
+    /* adjacency as sorted-unique neighbour position lists (set semantics) */
+    int64_t* cnt = calloc((size_t)nnodes + 1, sizeof(int64_t));
+    int64_t* en = malloc((size_t)(4 * nelem > 0 ? 4 * nelem : 1) * sizeof(int64_t));
+    for (int64_t k = 0; k < 4 * nelem; ++k) {
+        en[k] = id_lookup(&ix, enode_ids[k]);
+        if (en[k] < 0) { id_index_free(&ix); free(cnt); free(en); return 1; }
+    }
+    for (int64_t e = 0; e < nelem; ++e)
+        for (int i = 0; i < 4; ++i)
+            for (int j = i + 1; j < 4; ++j) { cnt[en[4 * e + i]]++; cnt[en[4 * e + j]]++; }
+    int64_t* start = malloc(((size_t)nnodes + 1) * sizeof(int64_t));
+    start[0] = 0;
+    for (int64_t i = 0; i < nnodes; ++i) start[i + 1] = start[i] + cnt[i];
+    int64_t* nb = malloc((size_t)(start[nnodes] > 0 ? start[nnodes] : 1) * sizeof(int64_t));
+    int64_t* fill = malloc((size_t)nnodes * sizeof(int64_t));
+    for (int64_t i = 0; i < nnodes; ++i) fill[i] = start[i];
+    for (int64_t e = 0; e < nelem; ++e)
+        for (int i = 0; i < 4; ++i)
+            for (int j = i + 1; j < 4; ++j) {
+                const int64_t a = en[4 * e + i], b = en[4 * e + j];
+                nb[fill[a]++] = b;
+                nb[fill[b]++] = a;
+            }
+    int64_t* degree = malloc((size_t)nnodes * sizeof(int64_t));
+    int64_t* ustart = malloc(((size_t)nnodes + 1) * sizeof(int64_t));
+    ustart[0] = 0;
+    int64_t w = 0;
+    for (int64_t i = 0; i < nnodes; ++i) {
+        int64_t* seg = nb + start[i];
+        const int64_t len = start[i + 1] - start[i];
+        qsort(seg, (size_t)len, sizeof(int64_t), cmp_i64);
+        int64_t d = 0;
+        for (int64_t k = 0; k < len; ++k) {
+            if (k > 0 && seg[k] == seg[k - 1]) continue;
+            nb[w++] = seg[k];
+            ++d;
+        }
+        degree[i] = d;
+        ustart[i + 1] = w;
+    }
+    g_degree = degree;
+    g_ids = node_ids;
+    int64_t* seeds = malloc((size_t)(nnodes > 0 ? nnodes : 1) * sizeof(int64_t));
+    for (int64_t i = 0; i < nnodes; ++i) seeds[i] = i; /* positions in model.nodes order */
+    qsort(seeds, (size_t)nnodes, sizeof(int64_t), cmp_degree_id);
+    char* visited = calloc((size_t)nnodes + 1, 1);
+    int64_t* queue = malloc((size_t)(nnodes > 0 ? nnodes : 1) * sizeof(int64_t));
+    int64_t* tmp = malloc((size_t)(start[nnodes] > 0 ? start[nnodes] : 1) * sizeof(int64_t));
+    int64_t out = 0;
+    for (int64_t s = 0; s < nnodes; ++s) {
+        const int64_t seed = seeds[s];
+        if (visited[seed]) continue;
+        visited[seed] = 1;
+        int64_t head = 0, tail = 0;
+        queue[tail++] = seed;
+        while (head < tail) {
+            const int64_t current = queue[head++];
+            order_out[out++] = node_ids[current];
+            const int64_t len = ustart[current + 1] - ustart[current];
+            memcpy(tmp, nb + ustart[current], (size_t)len * sizeof(int64_t));
+            qsort(tmp, (size_t)len, sizeof(int64_t), cmp_degree_id);
+            for (int64_t k = 0; k < len; ++k)
+                if (!visited[tmp[k]]) { visited[tmp[k]] = 1; queue[tail++] = tmp[k]; }
+        }
+    }
+    for (int64_t i = 0; i < out / 2; ++i) {
+        const int64_t t = order_out[i];
+        order_out[i] = order_out[out - 1 - i];
+        order_out[out - 1 - i] = t;
+    }
+    free(cnt); free(en); free(start); free(nb); free(fill); free(degree); free(ustart);
+    free(seeds); free(visited); free(queue); free(tmp);
+    id_index_free(&ix);
+    return 0;
+}
+
+/* solver.cpp:100-133 greedy first-fit colouring in element order; elements sharing
+ * a node conflict. Returns the number of colours. */
+int64_t lfo_colors(int64_t nelem, const int64_t* enode_ids, int64_t* color_out) {
+    if (nelem == 0) return 0;
+    /* node id -> element list (elements appended in ascending order) */
+    id_index ix;
+    int64_t* ids = malloc((size_t)(4 * nelem) * sizeof(int64_t));
+    memcpy(ids, enode_ids, (size_t)(4 * nelem) * sizeof(int64_t));
+    qsort(ids, (size_t)(4 * nelem), sizeof(int64_t), cmp_i64);
+    int64_t nu = 0;
+    for (int64_t k = 0; k < 4 * nelem; ++k)
+        if (k == 0 || ids[k] != ids[k - 1]) ids[nu++] = ids[k];
+    id_index_build(&ix, nu, ids);
+    int64_t* cnt = calloc((size_t)nu + 1, sizeof(int64_t));
+    int64_t* en = malloc((size_t)(4 * nelem) * sizeof(int64_t));
+    for (int64_t k = 0; k < 4 * nelem; ++k) { en[k] = id_lookup(&ix, enode_ids[k]); cnt[en[k]]++; }
+    int64_t* start = malloc(((size_t)nu + 1) * sizeof(int64_t));
+    start[0] = 0;
+    for (int64_t i = 0; i < nu; ++i) start[i + 1] = start[i] + cnt[i];
+    int64_t* list = malloc((size_t)start[nu] * sizeof(int64_t));
+    int64_t* fill = malloc((size_t)nu * sizeof(int64_t));
+    for (int64_t i = 0; i < nu; ++i) fill[i] = start[i];
+    for (int64_t e = 0; e < nelem; ++e)
+        for (int i = 0; i < 4; ++i) list[fill[en[4 * e + i]]++] = e;
+    int64_t* stamp_of = calloc((size_t)nelem, sizeof(int64_t));
+    int64_t stamp = 0, maximum = -1;
+    for (int64_t e = 0; e < nelem; ++e) color_out[e] = -1;
+    for (int64_t e = 0; e < nelem; ++e) {
+        ++stamp;
+        for (int i = 0; i < 4; ++i) {
+            const int64_t node = en[4 * e + i];
+            for (int64_t k = start[node]; k < start[node + 1]; ++k) {
+                const int64_t nbr = list[k];
+                if (color_out[nbr] >= 0) stamp_of[color_out[nbr]] = stamp;
+            }
+        }
+        int64_t color = 0;
+        while (stamp_of[color] == stamp) ++color;
+        color_out[e] = color;
+        if (maximum < color) maximum = color;
+    }
+    free(ids); free(cnt); free(en); free(start); free(list); free(fill); free(stamp_of);
+    id_index_free(&ix);
+    return maximum + 1;
+}
+
+/* -------------------------------------------------------------- assembly */
+
+/* solver.cpp:135-142 */
+void lfo_assemble_serial(int64_t nelem, const double* ke, const int64_t* emap, int64_t nnz, double* values) {
+    for (int64_t k = 0; k < nnz; ++k) values[k] = 0.0;
+    for (int64_t e = 0; e < nelem; ++e)
+        for (int k = 0; k < 64; ++k) values[emap[64 * e + k]] += ke[64 * e + k];
+}
+
+/* solver.cpp:355-413 (the O(nnz) sweep). */
+void lfo_dirichlet(int64_t n, const int64_t* row_ptr, const int64_t* col, double* values, double* rhs, int64_t nc,
+                   const int64_t* cdofs, const double* cvals) {
+    if (nc == 0) return;
+    char* constrained = calloc((size_t)n, 1);
+    double* prescribed = calloc((size_t)n, sizeof(double));
+    for (int64_t k = 0; k < nc; ++k) {
+        constrained[cdofs[k]] = 1;
+        prescribed[cdofs[k]] = cvals[k];
+    }
+    for (int64_t row = 0; row < n; ++row) {
+        const int64_t begin = row_ptr[row], end = row_ptr[row + 1];
+        if (constrained[row]) {
+            for (int64_t i = begin; i < end; ++i) values[i] = 0.0;
+            values[value_index(row_ptr, col, row, row)] = 1.0;
+            rhs[row] = prescribed[row];
+            continue;
+        }
+        for (int64_t i = begin; i < end; ++i) {
+            const int64_t c = col[i];
+            if (constrained[c] == 0 || values[i] == 0.0) continue;
+            rhs[row] -= values[i] * prescribed[c];
+            values[i] = 0.0;
+        }
+    }
+    free(constrained);
+    free(prescribed);
+}
+
+/* sparse.cpp:104-120 (row order is irrelevant; each row is summed serially). */
+void lfo_spmv(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values, const double* x,
+              double* y) {
+    for (int64_t row = 0; row < n; ++row) {
+        double sum = 0.0;
+        for (int64_t i = row_ptr[row]; i < row_ptr[row + 1]; ++i) sum += values[i] * x[col[i]];
+        y[row] = sum;
+    }
+}
+
+/* -------------------------------------------------------- preconditioners */
+
+int64_t lfo_ic0_factor_size(int64_t n, const int64_t* row_ptr, const int64_t* col) {
+    int64_t size = 0;
+    for (int64_t row = 0; row < n; ++row) {
+        int found = 0;
+        for (int64_t i = row_ptr[row]; i < row_ptr[row + 1]; ++i) {
+            if (col[i] > row) break;
+            ++size;
+            if (col[i] == row) found = 1;
+        }
+        if (!found) ++size;
+    }
+    return size;
+}
+
+/* precond.cpp:298-384 */
+int64_t lfo_ic0_factor(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values, int64_t* lrow,
+                       int64_t* lcol, double* lval, double* inv_diag) {
+    int64_t* diag = malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    int64_t m = 0;
+    lrow[0] = 0;
+    for (int64_t row = 0; row < n; ++row) {
+        int found = 0;
+        for (int64_t i = row_ptr[row]; i < row_ptr[row + 1]; ++i) {
+            const int64_t c = col[i];
+            if (c > row) break;
+            lcol[m] = c;
+            lval[m] = values[i];
+            ++m;
+            if (c == row) { diag[row] = m - 1; found = 1; }
+        }
+        if (!found) { lcol[m] = row; lval[m] = 1.0; ++m; diag[row] = m - 1; }
+        lrow[row + 1] = m;
+        inv_diag[row] = 1.0;
+    }
+    double scale = 0.0;
+    for (int64_t row = 0; row < n; ++row) {
+        const double a = fabs(lval[diag[row]]);
+        if (scale < a) scale = a;
+    }
+    const double floor_value = (1.0e-30 < 1.0e-14 * scale) ? 1.0e-14 * scale : 1.0e-30;
+    int64_t breakdowns = 0;
+    for (int64_t row = 0; row < n; ++row) {
+        const int64_t begin = lrow[row], d = diag[row];
+        for (int64_t index = begin; index < d; ++index) {
+            const int64_t column = lcol[index];
+            const int64_t other_begin = lrow[column], other_diag = diag[column];
+            double sum = lval[index];
+            int64_t left = begin, right = other_begin;
+            while (left < index && right < other_diag) {
+                const int64_t lc = lcol[left], rc = lcol[right];
+                if (lc < rc) ++left;
+                else if (lc > rc) ++right;
+                else { sum -= lval[left] * lval[right]; ++left; ++right; }
+            }
+            lval[index] = sum * inv_diag[column];
+        }
+        double sum = lval[d];
+        for (int64_t index = begin; index < d; ++index) sum -= lval[index] * lval[index];
+        if (!(sum > 0.0) || !isfinite(sum)) { ++breakdowns; sum = floor_value; }
+        const double pivot = sqrt(sum);
+        lval[d] = pivot;
+        inv_diag[row] = 1.0 / pivot;
+    }
+    free(diag);
+    return breakdowns;
+}
+
+/* precond.cpp:386-412 */
+void lfo_ic0_apply(int64_t n, const int64_t* lrow, const int64_t* lcol, const double* lval, const double* inv_diag,
+                   const double* in, double* out) {
+    if (out != in) memcpy(out, in, (size_t)n * sizeof(double));
+    for (int64_t row = 0; row < n; ++row) {
+        double sum = out[row];
+        for (int64_t i = lrow[row]; i + 1 < lrow[row + 1]; ++i) sum -= lval[i] * out[lcol[i]];
+        out[row] = sum * inv_diag[row];
+    }
+    for (int64_t row = n; row-- > 0;) {
+        out[row] *= inv_diag[row];
+        const double value = out[row];
+        for (int64_t i = lrow[row]; i + 1 < lrow[row + 1]; ++i) out[lcol[i]] -= lval[i] * value;
+    }
+}
+
+/* precond.cpp:178-232 for one-node subdomains (dofs 2I, 2I+1 in matrix order). */
+int64_t lfo_block1_factor(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values,
+                          double* factor) {
+    int64_t degenerate = 0;
+    for (int64_t node = 0; node < n / 2; ++node) {
+        double dense[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < 2; ++i) {
+            const int64_t dof = 2 * node + i;
+            for (int64_t k = row_ptr[dof]; k < row_ptr[dof + 1]; ++k) {
+                const int64_t c = col[k];
+                if (c / 2 != node) continue;
+                dense[i * 2 + (c - 2 * node)] = values[k];
+            }
+        }
+        int factorized = 1;
+        for (int i = 0; i < 2 && factorized; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double sum = dense[i * 2 + j];
+                for (int k = 0; k < j; ++k) sum -= dense[i * 2 + k] * dense[j * 2 + k];
+                if (i == j) {
+                    if (!(sum > 0.0) || !isfinite(sum)) { factorized = 0; break; }
+                    dense[i * 2 + i] = sqrt(sum);
+                } else {
+                    dense[i * 2 + j] = sum / dense[j * 2 + j];
+                }
+            }
+        if (!factorized) {
+            ++degenerate;
+            dense[0] = 1.0; dense[2] = 0.0; dense[3] = 1.0;
+        }
+        double* t = factor + 3 * node;
+        t[0] = dense[0];
+        t[0] = 1.0 / t[0];
+        t[1] = dense[2];
+        t[2] = dense[3];
+        t[2] = 1.0 / t[2];
+    }
+    return degenerate;
+}
+
+/* precond.cpp:242-291 for size-2 blocks. */
+void lfo_block1_apply(int64_t n, const double* factor, const double* in, double* out) {
+    for (int64_t node = 0; node < n / 2; ++node) {
+        const double* f = factor + 3 * node;
+        double s0 = in[2 * node], s1 = in[2 * node + 1];
+        /* forward */
+        s0 = s0 * f[0];
+        {
+            double sum = s1;
+            sum -= f[1] * s0;
+            s1 = sum * f[2];
+        }
+        /* backward */
+        s1 = s1 * f[2];
+        {
+            double sum = s0;
+            sum -= f[1] * s1;
+            s0 = sum * f[0];
+        }
+        out[2 * node] = s0;
+        out[2 * node + 1] = s1;
+    }
+}
+
+/* ------------------------------------------------------------------- PCG */
+
+typedef struct {
+    int kind;
+    int64_t n;
+    const double* inv_diag; /* jacobi */
+    double* factor;         /* block1 */
+    int64_t* lrow; int64_t* lcol; double* lval; double* ic_inv; /* ic0 */
+} precond_t;
+
+static void precond_apply(const precond_t* m, const double* in, double* out) {
+    if (m->kind == LFO_PRECOND_JACOBI) {
+        for (int64_t i = 0; i < m->n; ++i) out[i] = m->inv_diag[i] * in[i]; /* solver.cpp:502-505 */
+    } else if (m->kind == LFO_PRECOND_BLOCK1) {
+        lfo_block1_apply(m->n, m->factor, in, out);
+    } else {
+        lfo_ic0_apply(m->n, m->lrow, m->lcol, m->lval, m->ic_inv, in, out);
+    }
+}
+
+/* solver.cpp:532-627 with the preconditioners of solve_iterative (:675-747). */
+int lfo_pcg(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values, const double* rhs,
+            int precond, int stop_rule, double tolerance, int64_t max_iterations, double* x,
+            lfo_pcg_report* report) {
+    memset(report, 0, sizeof(*report));
+    precond_t m;
+    memset(&m, 0, sizeof(m));
+    m.kind = precond;
+    m.n = n;
+    double* inv_diag = NULL;
+    if (precond == LFO_PRECOND_JACOBI) {
+        inv_diag = malloc((size_t)n * sizeof(double));
+        for (int64_t row = 0; row < n; ++row) { /* solver.cpp:656-669 */
+            const double d = values[value_index(row_ptr, col, row, row)];
+            inv_diag[row] = fabs(d) > 1.0e-30 ? 1.0 / d : 1.0;
+        }
+        m.inv_diag = inv_diag;
+    } else if (precond == LFO_PRECOND_BLOCK1) {
+        m.factor = malloc((size_t)(3 * (n / 2) + 1) * sizeof(double));
+        lfo_block1_factor(n, row_ptr, col, values, m.factor);
+    } else {
+        const int64_t size = lfo_ic0_factor_size(n, row_ptr, col);
+        m.lrow = malloc(((size_t)n + 1) * sizeof(int64_t));
+        m.lcol = malloc((size_t)size * sizeof(int64_t));
+        m.lval = malloc((size_t)size * sizeof(double));
+        m.ic_inv = malloc((size_t)n * sizeof(double));
+        lfo_ic0_factor(n, row_ptr, col, values, m.lrow, m.lcol, m.lval, m.ic_inv);
+    }
+    double* r = malloc((size_t)n * sizeof(double));
+    double* z = malloc((size_t)n * sizeof(double));
+    double* p = malloc((size_t)n * sizeof(double));
+    double* q = malloc((size_t)n * sizeof(double));
+    for (int64_t i = 0; i < n; ++i) { x[i] = 0.0; r[i] = rhs[i]; }
+    precond_apply(&m, r, z);
+    double rr = 0.0, rho = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double v = r[i];
+        rr += v * v;
+        p[i] = z[i];
+        rho += v * z[i];
+    }
+    report->initial_residual = sqrt(max0(rr));
+    const double base = stop_rule == LFO_STOP_REFERENCE
+                            ? ((1.0 < report->initial_residual) ? report->initial_residual : 1.0)
+                            : report->initial_residual;
+    const double target = tolerance * base;
+    int status = 0;
+    if (report->initial_residual <= target) {
+        report->converged = 1;
+        report->final_residual = report->initial_residual;
+        goto done;
+    }
+    if (!(rho > 0.0) || !isfinite(rho)) { status = 1; goto done; }
+    for (int64_t it = 1; it <= max_iterations; ++it) {
+        lfo_spmv(n, row_ptr, col, values, p, q);
+        double curvature = 0.0;
+        for (int64_t i = 0; i < n; ++i) curvature += p[i] * q[i];
+        if (!(curvature > 0.0) || !isfinite(curvature)) { status = 2; goto done; }
+        const double alpha = rho / curvature;
+        rr = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            const double v = r[i] - alpha * q[i];
+            r[i] = v;
+            rr += v * v;
+        }
+        report->iterations = it;
+        report->final_residual = sqrt(max0(rr));
+        if (report->final_residual <= target) { report->converged = 1; goto done; }
+        precond_apply(&m, r, z);
+        double next = 0.0;
+        for (int64_t i = 0; i < n; ++i) next += r[i] * z[i];
+        if (!(next > 0.0) || !isfinite(next)) { status = 3; goto done; }
+        const double beta = next / rho;
+        for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        rho = next;
+    }
+    report->final_residual = sqrt(max0(rr));
+done:
+    report->status = status;
+    free(r); free(z); free(p); free(q); free(inv_diag);
+    free(m.factor); free(m.lrow); free(m.lcol); free(m.lval); free(m.ic_inv);
+    return status;
+}
+
+/* ------------------------------------------------------------ smoothing */
+
+/* solver.cpp:807-871 (Q4 part). gp layout per element: 4 x [xi, eta, e3, s3, vm]. */
+void lfo_smooth_nodal(int64_t nnodes, int64_t nelem, const int64_t* enode_idx, const double* gp, double* nodal) {
+    const double root_three = 1.7320508075688772935;
+    const double alpha = 0.5 * (1.0 + root_three);
+    const double beta = 0.5 * (1.0 - root_three);
+    const double ex[4][4] = {
+        {alpha * alpha, alpha * beta, beta * beta, alpha * beta},
+        {alpha * beta, alpha * alpha, alpha * beta, beta * beta},
+        {beta * beta, alpha * beta, alpha * alpha, alpha * beta},
+        {alpha * beta, beta * beta, alpha * beta, alpha * alpha},
+    };
+    double* sums = calloc((size_t)(3 * nnodes), sizeof(double));
+    int64_t* counts = calloc((size_t)nnodes, sizeof(int64_t));
+    for (int64_t e = 0; e < nelem; ++e)
+        for (int l = 0; l < 4; ++l) {
+            const int64_t node = enode_idx[4 * e + l];
+            for (int c = 0; c < 3; ++c)
+                for (int g = 0; g < 4; ++g) sums[3 * node + c] += ex[l][g] * gp[36 * e + 9 * g + 5 + c];
+            counts[node]++;
+        }
+    for (int64_t node = 0; node < nnodes; ++node) {
+        double* out = nodal + 4 * node;
+        if (counts[node] == 0) { out[0] = out[1] = out[2] = out[3] = 0.0; continue; }
+        for (int c = 0; c < 3; ++c) out[c] = sums[3 * node + c] / (double)counts[node];
+        const double sx = out[0], sy = out[1], txy = out[2];
+        out[3] = sqrt(max0(sx * sx - sx * sy + sy * sy + 3.0 * txy * txy));
+    }
+    free(sums);
+    free(counts);
+}

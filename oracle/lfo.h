@@ -1,0 +1,91 @@
+/* Test infrastructure only: CPU restatement ("oracle") of the LightFEM Q4
+ * assembly + PCG path. It is the checker for the HIP library in tests/, for
+ * __graft_entry__.smoke() and for bench.py's CPU baseline; the product never
+ * links it.
+ *
+ * Every function restates one reference stage literally, in the same loop and
+ * summation order, so that its results are bit-identical to the compiled
+ * reference (the golden vectors under tests/golden/ pin that). Indices are
+ * int64 throughout; ids are the model's node ids.
+ *
+ * Pinning (what checks each function; see tests/test_oracle.py):
+ *   lfo_q4_stiffness       golden ke (bit-exact), symmetry, rigid modes, patch test,
+ *                          closed-form rectangle, linearity in E
+ *   lfo_q4_fields          golden centre / Gauss-point fields (bit-exact), patch test
+ *   lfo_pattern_*          golden row_ptr/col/emap (exact), nnz = 4(3n+1)^2
+ *   lfo_rcm                golden RCM order (exact), 200x200 bandwidth/profile from README
+ *   lfo_colors             golden colours (exact), conflict-freedom
+ *   lfo_assemble_serial    golden K (bit-exact)
+ *   lfo_dirichlet          golden Kc/fc (bit-exact), dense brute-force restatement
+ *   lfo_pcg                golden u and iteration counts (bit-exact), scipy direct solve
+ *   lfo_ic0_* / block      golden preconditioner applications (bit-exact)
+ *   lfo_smooth_nodal       golden nodal fields (bit-exact)
+ */
+#ifndef LFO_H
+#define LFO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { LFO_PLANE_STRESS = 0, LFO_PLANE_STRAIN = 1 };
+enum { LFO_PRECOND_JACOBI = 0, LFO_PRECOND_BLOCK1 = 1, LFO_PRECOND_IC0 = 2 };
+enum { LFO_STOP_REFERENCE = 0, LFO_STOP_RELATIVE = 1 };
+
+/* 0 on success, 1 on a non-positive Jacobian. */
+int lfo_q4_stiffness(const double xy[8], double young, double nu, double thickness, int plane_mode,
+                     double ke[64]);
+int lfo_q4_fields(const double xy[8], double young, double nu, double thickness, int plane_mode,
+                  const double u[8], double centre[7], double gp[36]);
+
+int64_t lfo_pattern_nnz(int64_t ndofs, int64_t nelem, const int64_t* edofs);
+int lfo_pattern_build(int64_t ndofs, int64_t nelem, const int64_t* edofs, int64_t* row_ptr,
+                      int64_t* col, int64_t* emap);
+
+int lfo_rcm(int64_t nnodes, const int64_t* node_ids, int64_t nelem, const int64_t* enode_ids,
+            int64_t* order_out);
+int64_t lfo_colors(int64_t nelem, const int64_t* enode_ids, int64_t* color_out);
+void lfo_csr_profile(int64_t n, const int64_t* row_ptr, const int64_t* col, int64_t out[3]);
+
+void lfo_assemble_serial(int64_t nelem, const double* ke, const int64_t* emap, int64_t nnz,
+                         double* values);
+void lfo_dirichlet(int64_t n, const int64_t* row_ptr, const int64_t* col, double* values, double* rhs,
+                   int64_t nc, const int64_t* cdofs, const double* cvals);
+void lfo_spmv(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values,
+              const double* x, double* y);
+
+/* IC(0) factor of the lower triangle; returns breakdown count, -1 on error. */
+int64_t lfo_ic0_factor_size(int64_t n, const int64_t* row_ptr, const int64_t* col);
+int64_t lfo_ic0_factor(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values,
+                       int64_t* lrow, int64_t* lcol, double* lval, double* inv_diag);
+void lfo_ic0_apply(int64_t n, const int64_t* lrow, const int64_t* lcol, const double* lval,
+                   const double* inv_diag, const double* in, double* out);
+/* 2x2 nodal block-Jacobi (SubdomainBlockJacobi with one node per subdomain).
+ * factor: 3 doubles per node [1/l00, l10, 1/l11]; returns degenerate count. */
+int64_t lfo_block1_factor(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values,
+                          double* factor);
+void lfo_block1_apply(int64_t n, const double* factor, const double* in, double* out);
+
+typedef struct {
+    int64_t iterations;
+    int32_t converged;
+    int32_t status; /* 0 ok, 1 non-positive initial rho, 2 non-positive curvature, 3 rho breakdown */
+    double initial_residual;
+    double final_residual;
+} lfo_pcg_report;
+
+int lfo_pcg(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values, const double* rhs,
+            int precond, int stop_rule, double tolerance, int64_t max_iterations, double* x,
+            lfo_pcg_report* report);
+
+/* Nodal smoothing (solver.cpp:807-871); node index = dof_x / 2. */
+void lfo_smooth_nodal(int64_t nnodes, int64_t nelem, const int64_t* enode_idx, const double* gp,
+                      double* nodal);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,302 @@
+"""Pins the CPU oracle (oracle/lfo.c) before it is trusted as the checker.
+
+Two kinds of checks: bit-exact agreement with the golden vectors the compiled
+reference produced (tests/golden/), and properties the mathematics fixes
+(closed-form element matrices by exact quadrature, rigid-body modes, patch
+test, structural counts, textbook triangular solves, direct solves).
+"""
+import numpy as np
+import pytest
+import scipy.sparse
+import scipy.sparse.linalg
+
+from conftest import FULL_CASES, case_precond, load_golden
+from oracle import lfo
+from lightfem_amd import meshgen
+
+
+def _enode_idx(g):
+    return g["edofs"][:, 0::2] // 2
+
+
+def _plane(meta):
+    return lfo.PLANE_STRAIN if meta["plane_mode"] == "plane_strain" else lfo.PLANE_STRESS
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_oracle_matches_reference_bit_for_bit(name, golden_index):
+    g = load_golden(name)
+    meta = golden_index[name]["meta"]
+    enode = _enode_idx(g)
+    n = g["f"].shape[0]
+    # element stiffness, pattern + map, serial assembly
+    ke = lfo.element_stiffness(g["xy"], enode, g["elem_material"], _plane(meta))
+    assert np.array_equal(ke.view(np.int64), g["ke"].view(np.int64))
+    row_ptr, col, emap = lfo.pattern(n, g["edofs"])
+    assert np.array_equal(row_ptr, g["row_ptr"]) and np.array_equal(col, g["col"])
+    assert np.array_equal(emap, g["emap"])
+    K = lfo.assemble_serial(ke, emap, col.shape[0])
+    if meta["assembly"] == "serial":
+        assert np.array_equal(K.view(np.int64), g["K"].view(np.int64))
+    # colouring (in element order, node sharing)
+    enode_ids = g["node_ids"][enode]
+    colors, count = lfo.colors(enode_ids)
+    assert np.array_equal(colors, g["colors"])
+    # Dirichlet elimination on the reference's assembled matrix
+    Kc, fc = lfo.dirichlet(row_ptr, col, g["K"], g["f"], g["cdofs"], g["cvals"])
+    assert np.array_equal(Kc.view(np.int64), g["Kc"].view(np.int64))
+    assert np.array_equal(fc.view(np.int64), g["fc"].view(np.int64))
+    # preconditioner applications
+    f = lfo.ic0_factor(row_ptr, col, Kc)
+    assert np.array_equal(lfo.ic0_apply(f, g["probe"]).view(np.int64), g["ic0_apply_probe"].view(np.int64))
+    factor, _ = lfo.block1_factor(row_ptr, col, Kc)
+    assert np.array_equal(lfo.block1_apply(factor, g["probe"]).view(np.int64),
+                          g["block1_apply_probe"].view(np.int64))
+    # PCG with the configured preconditioner, reference stop rule
+    u, rep = lfo.pcg(row_ptr, col, Kc, fc, case_precond(meta), lfo.STOP_REFERENCE, meta["tolerance"])
+    assert rep["converged"] and rep["iterations"] == meta["iterations"]
+    assert rep["initial_residual"] == meta["initial_residual"]
+    assert np.array_equal(u.view(np.int64), g["u"].view(np.int64))
+    R = lfo.spmv(row_ptr, col, g["K"], u) - g["f"]
+    assert np.array_equal(R.view(np.int64), g["R"].view(np.int64))
+    centre, gp = lfo.element_fields(g["xy"], enode, g["elem_material"], _plane(meta), u)
+    assert np.array_equal(centre.view(np.int64), g["elem_centre"].view(np.int64))
+    assert np.array_equal(gp.view(np.int64), g["elem_gp"].view(np.int64))
+    nodal = lfo.smooth_nodal(g["xy"].shape[0], enode, gp.reshape(-1, 36))
+    assert np.array_equal(nodal.view(np.int64), g["node_fields"].view(np.int64))
+
+
+@pytest.mark.parametrize("name", ["scrambled_rcm", "grid16_rcm_coloring"])
+def test_oracle_rcm_matches_reference(name):
+    g = load_golden(name)
+    enode_ids = g["node_ids"][_enode_idx(g)]
+    order = lfo.rcm(np.sort(g["node_ids"]), enode_ids)
+    assert np.array_equal(order, g["rcm_order"])
+    assert np.array_equal(order, g["node_ids"])
+
+
+def test_oracle_assembly_variants_agree():
+    base = load_golden("scrambled_none")
+    for strategy in ("atomic", "coloring", "thread_local"):
+        g = load_golden(f"scrambled_{strategy}")
+        assert np.max(np.abs(g["K"] - base["K"])) <= 1e-13 * np.max(np.abs(base["K"]))
+        assert np.max(np.abs(g["u"] - base["u"])) <= 1e-11
+
+
+def test_oracle_c1_digests(golden_index):
+    """50x50 cantilever: every stage hashed bit-exactly."""
+    import hashlib
+    m = meshgen.c1_model()
+    xy = m.xy
+    enode = m.enodes - 1
+    n = 2 * xy.shape[0]
+    out = lfo.solve_model(xy, enode, m.element_young(), lfo.PLANE_STRESS, _load_vector(m), *_constraints(m))
+    ref = golden_index["c1"]
+    dig = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<i8" if a.dtype.kind in "iu" else "<f8").tobytes()).hexdigest()  # noqa
+    for key, ours in (("row_ptr", out["row_ptr"]), ("col", out["col"]), ("emap", out["emap"]), ("ke", out["ke"]),
+                      ("K", out["K"]), ("Kc", out["Kc"]), ("fc", out["fc"]), ("u", out["u"]), ("R", out["R"])):
+        assert dig(ours) == ref["sha256"][key], key
+    assert out["report"]["iterations"] == ref["meta"]["iterations"] == 313
+    assert n == ref["meta"]["dofs"] == 5202 and out["col"].shape[0] == 91204
+
+
+def _load_vector(m):
+    f = np.zeros(2 * m.n_nodes)
+    idx = m.load_nodes - 1
+    np.add.at(f, 2 * idx, m.load_values[:, 0])
+    np.add.at(f, 2 * idx + 1, m.load_values[:, 1])
+    return f
+
+
+def _constraints(m):
+    idx = m.bc_nodes - 1
+    dofs = np.stack([2 * idx, 2 * idx + 1], axis=1).reshape(-1)
+    vals = m.bc_values.reshape(-1)
+    order = np.argsort(dofs, kind="stable")
+    return dofs[order], vals[order]
+
+
+# ----------------------------------------------------------- mathematical pins
+
+def _exact_rect_ke(a, b, young, nu, t, plane_mode):
+    """Independent Q4 stiffness for an axis-aligned a x b rectangle by 5x5 Gauss
+    quadrature (the integrand is polynomial, so both rules are exact)."""
+    if plane_mode == lfo.PLANE_STRESS:
+        c = young / (1 - nu * nu)
+        D = c * np.array([[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]])
+    else:
+        c = young / ((1 + nu) * (1 - 2 * nu))
+        D = c * np.array([[1 - nu, nu, 0], [nu, 1 - nu, 0], [0, 0, (1 - 2 * nu) / 2]])
+    pts, wts = np.polynomial.legendre.leggauss(5)
+    sx = np.array([-1, 1, 1, -1])
+    sy = np.array([-1, -1, 1, 1])
+    K = np.zeros((8, 8))
+    for xi, wx in zip(pts, wts):
+        for eta, wy in zip(pts, wts):
+            dndx = 0.25 * sx * (1 + sy * eta) * (2 / a)
+            dndy = 0.25 * sy * (1 + sx * xi) * (2 / b)
+            B = np.zeros((3, 8))
+            B[0, 0::2] = dndx
+            B[1, 1::2] = dndy
+            B[2, 0::2] = dndy
+            B[2, 1::2] = dndx
+            K += B.T @ D @ B * (a * b / 4) * t * wx * wy
+    return K
+
+
+@pytest.mark.parametrize("plane_mode", [lfo.PLANE_STRESS, lfo.PLANE_STRAIN])
+def test_q4_stiffness_closed_form_rectangle(plane_mode):
+    a, b = 2.0, 0.75
+    xy = np.array([[0, 0], [a, 0], [a, b], [0, b]], dtype=float) + np.array([3.0, -1.0])
+    ke = lfo.q4_stiffness(xy, 210.0, 0.3, 0.5, plane_mode)
+    exact = _exact_rect_ke(a, b, 210.0, 0.3, 0.5, plane_mode)
+    assert np.allclose(ke, exact, rtol=0, atol=1e-12 * np.abs(exact).max())
+
+
+def test_q4_stiffness_symmetry_rigid_modes_linearity():
+    xy = np.array([[0.1, 0.0], [2.0, 0.3], [1.7, 1.9], [-0.2, 1.2]])
+    ke = lfo.q4_stiffness(xy, 1000.0, 0.25, 1.0, lfo.PLANE_STRESS)
+    scale = np.abs(ke).max()
+    assert np.allclose(ke, ke.T, atol=1e-10)
+    tx = np.tile([1.0, 0.0], 4)
+    ty = np.tile([0.0, 1.0], 4)
+    rot = np.stack([-xy[:, 1], xy[:, 0]], axis=1).reshape(8)
+    for mode in (tx, ty, rot):
+        assert np.abs(ke @ mode).max() <= 1e-12 * scale
+    ke2 = lfo.q4_stiffness(xy, 2000.0, 0.25, 1.0, lfo.PLANE_STRESS)
+    assert np.array_equal(ke2, 2.0 * ke)
+    eig = np.linalg.eigvalsh(ke)
+    assert np.sum(eig > 1e-9 * scale) == 5  # three rigid modes, five deformation modes
+    clockwise = xy[::-1]
+    with pytest.raises(RuntimeError):
+        lfo.q4_stiffness(clockwise, 1000.0, 0.25, 1.0, lfo.PLANE_STRESS)
+
+
+def test_patch_constant_strain():
+    """test_solver.cpp:64-85: u_x = x on a 2x1 element gives exx=1, sxx=E/(1-nu^2)."""
+    xy = np.array([[0.0, 0.0], [2.0, 0.0], [2.0, 1.0], [0.0, 1.0]])
+    u = np.array([0.0, 0.0, 2.0, 0.0, 2.0, 0.0, 0.0, 0.0])
+    centre, gp = lfo.q4_fields(xy, 1000.0, 0.25, 1.0, lfo.PLANE_STRESS, u)
+    assert abs(centre[0] - 1.0) <= 1e-12 and abs(centre[1]) <= 1e-12 and abs(centre[2]) <= 1e-12
+    assert abs(centre[3] - 1000.0 / (1 - 0.25 ** 2)) <= 1e-9
+    for g in gp:
+        assert abs(g[2] - 1.0) <= 1e-12 and abs(g[5] - centre[3]) <= 1e-9
+        sx, sy, t = g[5:8]
+        assert abs(g[8] - np.sqrt(sx * sx - sx * sy + sy * sy + 3 * t * t)) <= 1e-12 * abs(g[8])
+
+
+@pytest.mark.parametrize("n", [1, 3, 8, 20])
+def test_pattern_structure(n):
+    m = meshgen.structured_cantilever(n, n)
+    enode = m.enodes - 1
+    edofs = np.stack([2 * enode, 2 * enode + 1], axis=2).reshape(-1, 8)
+    row_ptr, col, emap = lfo.pattern(2 * m.n_nodes, edofs)
+    assert col.shape[0] == 4 * (3 * n + 1) ** 2
+    for r in range(row_ptr.shape[0] - 1):
+        seg = col[row_ptr[r]:row_ptr[r + 1]]
+        assert np.all(np.diff(seg) > 0) and r in seg
+    rows = np.searchsorted(row_ptr, emap, side="right") - 1
+    assert np.array_equal(rows, np.repeat(edofs, 8, axis=1))
+    assert np.array_equal(col[emap], np.tile(edofs, (1, 8)))
+
+
+def test_colors_conflict_free_and_four_on_grid():
+    m = meshgen.structured_cantilever(12, 9)
+    colors, count = lfo.colors(m.enodes)
+    assert count == 4
+    for c in range(count):
+        nodes = m.enodes[colors == c].reshape(-1)
+        assert np.unique(nodes).shape[0] == nodes.shape[0]
+    # first-fit: every element with colour k>0 touches some element of each colour < k
+    for e in range(m.n_elems):
+        touching = np.any(np.isin(m.enodes, m.enodes[e]), axis=1)
+        touching[e] = False
+        assert set(range(colors[e])) <= set(colors[touching].tolist())
+
+
+def test_rcm_readme_structure_200():
+    """README.md:125 (thesis Table 4-3): 200x200 bandwidth 405 -> 803, profile 32,522,001 -> 42,949,601."""
+    m = meshgen.structured_cantilever(200, 200)
+    enode = m.enodes - 1
+    edofs = np.stack([2 * enode, 2 * enode + 1], axis=2).reshape(-1, 8)
+    row_ptr, col, _ = lfo.pattern(2 * m.n_nodes, edofs)
+    before = lfo.csr_profile(row_ptr, col)
+    order = lfo.rcm(m.node_ids, m.enodes)
+    assert np.array_equal(np.sort(order), m.node_ids)
+    position = np.empty(m.n_nodes, dtype=np.int64)
+    position[order - 1] = np.arange(m.n_nodes)
+    enode_r = position[enode]
+    edofs_r = np.stack([2 * enode_r, 2 * enode_r + 1], axis=2).reshape(-1, 8)
+    row_ptr_r, col_r, _ = lfo.pattern(2 * m.n_nodes, edofs_r)
+    after = lfo.csr_profile(row_ptr_r, col_r)
+    assert (before["bandwidth"], before["profile"]) == (405, 32522001)
+    assert (after["bandwidth"], after["profile"]) == (803, 42949601)
+    assert before["nonzeros"] == after["nonzeros"] == 1444804
+
+
+def test_dirichlet_dense_restatement():
+    g = load_golden("grid6_jacobi")
+    n = g["f"].shape[0]
+    ubar = np.zeros(g["cdofs"].shape[0])
+    ubar[::3] = 1e-3  # nonzero prescribed values exercise the RHS transfer
+    Kc, fc = lfo.dirichlet(g["row_ptr"], g["col"], g["K"], g["f"], g["cdofs"], ubar)
+    K = scipy.sparse.csr_matrix((g["K"], g["col"], g["row_ptr"]), shape=(n, n)).toarray()
+    c = np.zeros(n, dtype=bool)
+    c[g["cdofs"]] = True
+    ub = np.zeros(n)
+    ub[g["cdofs"]] = ubar
+    expect_f = g["f"] - K[:, c] @ ub[c]
+    expect_f[c] = ub[c]
+    expect_K = K.copy()
+    expect_K[c, :] = 0
+    expect_K[:, c] = 0
+    expect_K[c, c] = 1
+    got = scipy.sparse.csr_matrix((Kc, g["col"], g["row_ptr"]), shape=(n, n)).toarray()
+    assert np.allclose(got, expect_K, rtol=0, atol=0)
+    assert np.allclose(fc, expect_f, rtol=1e-14, atol=1e-18)
+
+
+@pytest.mark.parametrize("precond", [lfo.JACOBI, lfo.BLOCK1, lfo.IC0])
+def test_pcg_matches_direct_solve(precond):
+    g = load_golden("grid16x12_strain_loge")
+    n = g["f"].shape[0]
+    A = scipy.sparse.csr_matrix((g["Kc"], g["col"], g["row_ptr"]), shape=(n, n)).tocsc()
+    exact = scipy.sparse.linalg.spsolve(A, g["fc"])
+    u, rep = lfo.pcg(g["row_ptr"], g["col"], g["Kc"], g["fc"], precond, lfo.STOP_RELATIVE, 1e-13)
+    assert rep["converged"]
+    assert np.linalg.norm(u - exact) <= 1e-9 * np.linalg.norm(exact)
+
+
+def test_ic0_is_exact_on_a_tridiagonal_matrix():
+    """With no fill-in possible IC(0) is the exact Cholesky factor, so applying it solves A z = b."""
+    n = 40
+    rng = np.random.default_rng(5)
+    main = 4.0 + rng.random(n)
+    off = -1.0 - rng.random(n - 1)
+    A = scipy.sparse.diags([off, main, off], [-1, 0, 1]).tocsr()
+    A.sort_indices()
+    f = lfo.ic0_factor(A.indptr, A.indices, A.data)
+    b = rng.random(n)
+    z = lfo.ic0_apply(f, b)
+    assert f.breakdowns == 0
+    assert np.allclose(A @ z, b, rtol=1e-13, atol=1e-13)
+
+
+def test_block1_solves_each_nodal_block():
+    g = load_golden("grid16_stress")
+    n = g["f"].shape[0]
+    factor, degenerate = lfo.block1_factor(g["row_ptr"], g["col"], g["Kc"])
+    assert degenerate == 0
+    z = lfo.block1_apply(factor, g["probe"])
+    K = scipy.sparse.csr_matrix((g["Kc"], g["col"], g["row_ptr"]), shape=(n, n))
+    for node in range(0, n // 2, 7):
+        blk = K[2 * node:2 * node + 2, 2 * node:2 * node + 2].toarray()
+        assert np.allclose(blk @ z[2 * node:2 * node + 2], g["probe"][2 * node:2 * node + 2], rtol=1e-12)
+
+
+def test_smoothing_reproduces_constant_stress():
+    m = meshgen.structured_cantilever(5, 4)
+    gp = np.zeros((m.n_elems, 4, 9))
+    gp[:, :, 5:8] = [3.0, -2.0, 0.5]
+    nodal = lfo.smooth_nodal(m.n_nodes, m.enodes - 1, gp.reshape(-1, 36))
+    assert np.allclose(nodal[:, :3], [3.0, -2.0, 0.5], rtol=1e-14)
