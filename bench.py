@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Headline benchmark: Jacobi-PCG time to relative residual 1e-10 on a 1000x1000
+Q4 plane-strain cantilever (2,004,002 dofs, 36,024,004 nnz, per-element E
+log-uniform in [1e-3, 1] from a fixed seed), FP32-value mixed precision vs FP64,
+plus the SpMV's achieved HBM bandwidth.
+
+A step is one full PCG solve on the resident, assembled and constrained system
+(x0 = 0, stop on the true relative residual ||r||/||b|| <= 1e-10). Prints one JSON
+line. The CPU baseline times the compiled reference (oracle/_ref) on a bounded
+number of PCG iterations of the same system on this host's cores and scales the
+per-iteration time to the iteration count of the full solve.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PCG time-to-1e-10 (ms) + SpMV HBM GB/s, 1000² Q4; FP32-mixed vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def _sync():
+    lib = ctypes.CDLL("libamdhip64.so")
+    lib.hipDeviceSynchronize()
+
+
+def spmv_bytes(n: int, nnz: int, value_bytes: int) -> int:
+    """Algorithmic bytes of one SpMV: values + int32 columns, int32 row offsets, x read once, y written once."""
+    return nnz * (value_bytes + 4) + 4 * (n + 1) + 16 * n
+
+
+def cpu_baseline(n_mesh: int, young: np.ndarray, iterations_full: int, sample_iters: int) -> dict:
+    ref = os.path.join(ROOT, "oracle", "_ref", "lfref")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if os.path.exists(ref):
+        with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as handle:
+            young.astype("<f8").tofile(handle)
+            path = handle.name
+        try:
+            out = subprocess.run([ref, "bench", str(n_mesh), str(n_mesh), path, "plane_strain", "0.3",
+                                  str(sample_iters), str(threads), "jacobi"], capture_output=True, text=True,
+                                 timeout=600, check=True)
+        finally:
+            os.unlink(path)
+        rec = json.loads(out.stdout.strip().splitlines()[-1])
+        per_iter = rec["pcg_ms"] / max(1, rec["iterations"])
+        return {"value": per_iter * iterations_full, "unit": "ms", "cores": rec["threads_used"],
+                "kind": "reference",
+                "sample": f"compiled reference pcg_solve, {rec['iterations']} Jacobi-PCG iterations of the same "
+                          f"1000x1000 system ({rec['pcg_ms']:.0f} ms, matvec {rec['matvec_ms']:.0f} ms) scaled to "
+                          f"{iterations_full} iterations",
+                "ms_per_iteration": per_iter, "element_stiffness_ms": rec["element_stiffness_ms"],
+                "assembly_ms": rec["assembly_ms"]}
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1000, help="mesh size (n x n elements)")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--cpu-iters", type=int, default=300)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--check-every", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the path is a single-device solve and --gpus {args.gpus} is not "
+              f"supported (multi-GPU sharding is design-only).", file=sys.stderr)
+        return 2
+
+    from lightfem_amd import meshgen, native, pipeline
+
+    n_mesh = args.n
+    t0 = time.perf_counter()
+    model = meshgen.c3_model(n_mesh)
+    dev = native.Device(0)
+    print(f"bench: {n_mesh}x{n_mesh} plane-strain log-uniform E, setup ...", file=sys.stderr, flush=True)
+    sys_ = pipeline.setup(model, device=dev, assembly="coloring")
+    setup_s = time.perf_counter() - t0
+    n, nnz = dev.n, dev.nnz
+    # assembly strategies (device times from hipEvents)
+    assembly = {}
+    for strategy in ("coloring", "atomic", "serial"):
+        rep = dev.assemble(strategy)
+        assembly[strategy] = {"element_stiffness_ms": rep["element_stiffness_ms"], "assembly_ms": rep["assembly_ms"]}
+    dev.set_load(sys_.indexed.load)
+    dev.dirichlet(sys_.indexed.cdofs, sys_.indexed.cvals)
+    # SpMV kernel: average launch duration over 50 launches (hipEvents on the library stream)
+    spmv = {}
+    for precision, vbytes in (("fp64", 8), ("mixed", 4)):
+        best = None
+        for variant in (1, 2, 3):
+            ms = dev.bench_spmv(precision, variant, 50)
+            if best is None or ms < best[1]:
+                best = (variant, ms)
+        b = spmv_bytes(n, nnz, vbytes)
+        spmv[precision] = {"variant": best[0], "ms": best[1], "bytes": b, "GBs": b / (best[1] * 1e-3) / 1e9}
+    print(f"bench: spmv {json.dumps(spmv)}", file=sys.stderr, flush=True)
+    forms = {}
+    for form in ("fp64", "mixed"):
+        variant = args.variant or spmv[form]["variant"]
+        for _ in range(args.warmup):
+            dev.pcg("jacobi", form, "relative", args.tol, 10 ** 6, args.check_every, spmv_variant=variant)
+        _sync()
+        t = time.perf_counter()
+        reps = []
+        for _ in range(args.steps):
+            u, rep = dev.pcg("jacobi", form, "relative", args.tol, 10 ** 6, args.check_every, spmv_variant=variant)
+            reps.append(rep)
+        _sync()
+        wall = (time.perf_counter() - t) * 1e3 / args.steps
+        forms[form] = {"ms_per_solve": wall, "device_solve_ms": float(np.median([r["solve_ms"] for r in reps])),
+                       "iterations": reps[-1]["iterations"], "replacements": reps[-1]["residual_replacements"],
+                       "true_relative_residual": reps[-1]["true_residual"] / reps[-1]["initial_residual"],
+                       "spmv_variant": variant, "u": u}
+        print(f"bench: {form} {wall:.1f} ms/solve, {reps[-1]['iterations']} iterations", file=sys.stderr, flush=True)
+    u64, umx = forms["fp64"].pop("u"), forms["mixed"].pop("u")
+    forms["mixed"]["rel_u_diff_vs_fp64"] = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
+    per_iter_ms = {k: v["device_solve_ms"] / max(1, v["iterations"]) for k, v in forms.items()}
+    iter_bytes64 = spmv_bytes(n, nnz, 8) + 112 * n  # SpMV + update (56n) + direction (32n) + p.q/q traffic
+    cpu = None
+    if not args.skip_cpu:
+        try:
+            cpu = cpu_baseline(n_mesh, sys_.indexed.materials[sys_.indexed.elem_mat][:, 0],
+                               forms["fp64"]["iterations"], args.cpu_iters)
+        except Exception as exc:  # reported, not fatal
+            print(f"bench: cpu baseline failed: {exc}", file=sys.stderr)
+    value = forms["mixed"]["ms_per_solve"]
+    dominant = spmv["mixed"]
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-values+f64", "data": "synthetic",
+        "config": {"workload": f"q4-cantilever-{n_mesh}x{n_mesh}-plane-strain-loguniformE-1e-3..1-seed"
+                               f"{meshgen.SEED_LOGE}", "dofs": n, "nnz": nnz, "preconditioner": "jacobi",
+                   "stop_rule": f"true relative residual <= {args.tol:g}", "assembly": "coloring"},
+        "fp64_ms": round(forms["fp64"]["ms_per_solve"], 3),
+        "mixed_speedup_vs_fp64": round(forms["fp64"]["ms_per_solve"] / value, 4),
+        "forms": forms, "per_iteration_ms": per_iter_ms,
+        "iteration_bytes_fp64_est": iter_bytes64,
+        "spmv": spmv, "assembly": assembly, "setup_s": round(setup_s, 2),
+        "roofline": {"bound": "hbm", "kernel": "pcg_spmv_kernel<float values>",
+                     "achieved": round(dominant["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dominant["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "fp64_spmv_GBs": round(spmv["fp64"]["GBs"], 1),
+                     "fp64_frac": round(spmv["fp64"]["GBs"] / HBM_PEAK_GBS, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    dev.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,153 @@
+/* lightfem_gpu.h - C ABI of the MI355X (gfx950) backend for LightFEM's Q4
+ * assembly + PCG path.
+ *
+ * The reference has no FFI: its "operator API" is the C++ library, and every
+ * stage below is a file-static function inside LinearElasticSolver::solve
+ * (src/solver.cpp:875-960). Each entry point replaces one of those stages; the
+ * host orchestrator (lightfem::LinearElasticSolver with the hip backend, or a
+ * ctypes caller) calls them in the reference's order.
+ *
+ * Conventions
+ *  - Status codes, never exceptions, cross the ABI; lfg_last_error() holds the
+ *    message. The C++ wrapper rethrows each code with the reference's text.
+ *  - The caller owns every host array. The library copies inputs into its own
+ *    device buffers (owned until lfg_destroy) and writes results into
+ *    caller-allocated host arrays. Calls are synchronous with respect to the
+ *    host arrays.
+ *  - One HIP stream per context; a context must not be shared between host
+ *    threads without external locking.
+ *  - Numbering is the reference's: node index i (after sort_and_index or RCM)
+ *    owns dofs 2i and 2i+1 (src/model.cpp:74-92). Element node lists are node
+ *    indices in the element's counter-clockwise order.
+ */
+#ifndef LIGHTFEM_GPU_H
+#define LIGHTFEM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    LFG_OK = 0,
+    LFG_ERR_ARG = 1,            /* std::invalid_argument / std::logic_error in the reference */
+    LFG_ERR_JACOBIAN = 2,       /* element.cpp:52-54 */
+    LFG_ERR_NOT_SPD = 3,        /* solver.cpp:591-593 non-positive curvature */
+    LFG_ERR_NONFINITE = 4,      /* solver.cpp:579-581 initial rho, :616-618 rho breakdown */
+    LFG_ERR_NOT_CONVERGED = 5,  /* solver.cpp:948-950 */
+    LFG_ERR_HIP = 6,            /* HIP runtime error (device, memory) */
+    LFG_ERR_STATE = 7,          /* call made before the stage it needs */
+    LFG_ERR_UNSUPPORTED = 8     /* option outside this backend's scope */
+} lfg_status;
+
+typedef struct lfg_ctx lfg_ctx;
+
+enum { LFG_PLANE_STRESS = 0, LFG_PLANE_STRAIN = 1 };
+/* `assembly` setting (model.hpp:99, solver.cpp:275-299). */
+enum { LFG_ASSEMBLY_SERIAL = 0, LFG_ASSEMBLY_ATOMIC = 1, LFG_ASSEMBLY_COLORING = 2, LFG_ASSEMBLY_THREAD_LOCAL = 3 };
+/* `preconditioner` setting (model.hpp:105); BLOCK2 = `block` with subdomain_nodes 1. */
+enum { LFG_PRECOND_JACOBI = 0, LFG_PRECOND_BLOCK2 = 1, LFG_PRECOND_IC0 = 2 };
+enum { LFG_PRECISION_FP64 = 0, LFG_PRECISION_MIXED = 1 };
+/* reference: ||r|| <= tol*max(1,||r0||) (solver.cpp:572); relative: ||r|| <= tol*||b||. */
+enum { LFG_STOP_REFERENCE = 0, LFG_STOP_RELATIVE = 1 };
+
+typedef struct {
+    int32_t preconditioner;  /* LFG_PRECOND_* */
+    int32_t precision;       /* LFG_PRECISION_* */
+    int32_t stop_rule;       /* LFG_STOP_* */
+    int32_t check_every;     /* iterations per captured graph between convergence reads; 0 = default */
+    double tolerance;        /* SolverSettings::tolerance */
+    int64_t max_iterations;  /* SolverSettings::max_iterations */
+    double replace_drop;     /* mixed: replace the residual in FP64 when it falls by this factor; 0 = default */
+    int32_t direct_launch;   /* 1: launch each kernel directly instead of replaying a captured hipGraph */
+    int32_t spmv_variant;    /* 0 = autotuned/default */
+} lfg_pcg_options;
+
+typedef struct {
+    int32_t converged;
+    int32_t status;               /* lfg_status of the solve */
+    int64_t iterations;
+    int64_t residual_replacements;  /* mixed precision: FP64 residual recomputations */
+    double initial_residual;      /* ||r0|| = ||b|| (x0 = 0) */
+    double final_residual;        /* ||r|| at stop: recurrence residual (FP64) or replaced residual (mixed) */
+    double true_residual;         /* ||b - K u|| recomputed in FP64 after the solve */
+    double solve_ms;              /* device time, first SpMV to convergence (hipEvent) */
+    double setup_ms;              /* preconditioner / precision setup (hipEvent) */
+    int64_t breakdowns;           /* IC(0) clamped pivots (precond.cpp:373-378) */
+    int64_t degenerate_blocks;    /* 2x2 blocks replaced by identity (precond.cpp:218-223) */
+    int64_t levels;               /* IC(0): forward + backward level count */
+} lfg_pcg_report;
+
+typedef struct {
+    double element_stiffness_ms;  /* Ke kernel (PerformanceReport::element_stiffness_ms) */
+    double assembly_ms;           /* scatter / gather (PerformanceReport::assembly_ms) */
+    int32_t coloring_count;
+    int32_t jacobian_failures;
+} lfg_assembly_report;
+
+/* Context (one device, one stream). */
+int lfg_create(int32_t device, lfg_ctx** out);
+void lfg_destroy(lfg_ctx* ctx);
+const char* lfg_last_error(const lfg_ctx* ctx);
+const char* lfg_status_string(int32_t status);
+
+/* Mesh and materials. Replaces q4_coordinates/find_material inside the Ke loop
+ * (solver.cpp:48-56, 243-270). xy: 2*n_nodes; enodes: 4*n_elems node indices;
+ * elem_material: n_elems material indices; materials: 3*n_materials (E, nu, t). */
+int lfg_set_mesh(lfg_ctx* ctx, int64_t n_nodes, const double* xy, int64_t n_elems, const int32_t* enodes,
+                 const int32_t* elem_material, int64_t n_materials, const double* materials, int32_t plane_mode);
+
+/* CSR pattern and element->CSR-value map of build_csr_assembly_pattern
+ * (sparse.cpp:15-58): row_ptr n_rows+1, col nnz, emap 64*n_elems (row-major local r*8+c). */
+int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* row_ptr, const int32_t* col,
+                    const int32_t* emap);
+
+/* Colour groups of element_colors (solver.cpp:100-133): color_ptr n_colors+1, elements ascending per colour. */
+int lfg_set_colors(lfg_ctx* ctx, int32_t n_colors, const int32_t* color_ptr, const int32_t* color_elems);
+
+/* Element stiffness alone (q4_stiffness, element.cpp:130-161) into ke_out (64*n_elems), for checks. */
+int lfg_element_stiffness(lfg_ctx* ctx, double* ke_out);
+
+/* Numeric assembly (solver.cpp:243-301): Ke kernel + strategy. serial = deterministic
+ * gather in element order (bit-identical to serial_assembly), coloring = conflict-free
+ * scatter per colour (bit-identical to coloring_assembly), atomic = FP64 hardware
+ * atomics, thread_local = the deterministic gather. */
+int lfg_assemble(lfg_ctx* ctx, int32_t strategy, lfg_assembly_report* report);
+
+/* Unconstrained stiffness values (AnalysisResult::stiffness, solver.cpp:906-909). */
+int lfg_get_stiffness(lfg_ctx* ctx, double* values_out);
+/* Overwrite the unconstrained values (tests, or a caller-assembled matrix). */
+int lfg_set_stiffness(lfg_ctx* ctx, const double* values);
+
+/* Load vector assembled on the host (solver.cpp:303-332). */
+int lfg_set_load(lfg_ctx* ctx, const double* f);
+
+/* Symmetric Dirichlet elimination (solver.cpp:355-413) into a constrained copy;
+ * the unconstrained values and load stay for reactions. dofs ascending. */
+int lfg_dirichlet(lfg_ctx* ctx, int64_t n_constraints, const int32_t* dofs, const double* values);
+int lfg_get_system(lfg_ctx* ctx, double* values_out, double* rhs_out);
+
+/* PCG on the constrained system (solve_iterative + pcg_solve, solver.cpp:532-747). */
+int lfg_pcg(lfg_ctx* ctx, const lfg_pcg_options* options, double* u_out, lfg_pcg_report* report);
+
+/* R = K_unconstrained u - f (solver.cpp:953-954) for the last solution. */
+int lfg_reactions(lfg_ctx* ctx, double* reaction_out);
+
+/* Element fields (postprocess, solver.cpp:756-805: centre 7 doubles and 4 Gauss
+ * points x 9 doubles [xi, eta, strain3, stress3, vm] per element) and nodal
+ * smoothing (solver.cpp:807-871: 4 doubles per node). Any output may be NULL. */
+int lfg_postprocess(lfg_ctx* ctx, double* centre, double* gauss_points, double* nodal);
+
+/* Checks and measurement hooks on the constrained system. */
+int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, double* out);
+int lfg_spmv(lfg_ctx* ctx, int32_t precision, const double* x, double* y);
+/* Average device time (ms) of `reps` SpMV launches of the given precision and
+ * variant on resident data. */
+int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIGHTFEM_GPU_H */

@@ -1,0 +1,45 @@
+/* lightfem_host.h - C ABI of the host-side stages that stay on the CPU by design
+ * (their outputs are integer structures the device kernels consume). Each one is
+ * bit-identical in value to the reference stage cited beside it; they are also
+ * used by lightfem::LinearElasticSolver (liblightfem.so) and by ctypes callers.
+ * Node indices are positions in the model's node order (node i owns dofs 2i,
+ * 2i+1, src/model.cpp:74-92). Return 0 on success, non-zero on invalid input.
+ */
+#ifndef LIGHTFEM_HOST_H
+#define LIGHTFEM_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Number of CSR nonzeros build_csr_assembly_pattern (sparse.cpp:15-58) produces
+ * for Q4 elements given as 4 node indices each; -1 on invalid input. */
+int64_t lfh_csr_pattern_nnz(int64_t n_nodes, int64_t n_elems, const int32_t* enodes);
+
+/* The pattern itself: row_ptr (2*n_nodes+1), sorted columns (nnz) and the
+ * element->value map (64 per element, local entry r*8+c), sparse.cpp:15-58. */
+int lfh_csr_pattern(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int32_t* row_ptr, int32_t* col,
+                    int32_t* emap);
+
+/* reverse_cuthill_mckee (reordering.cpp:11-62): node ids in RCM order. node_ids
+ * lists the model's nodes; enode_ids the 4 node ids of each Q4 element. */
+int lfh_rcm(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int32_t* enode_ids, int32_t* order_out);
+
+/* element_colors (solver.cpp:100-133): greedy first-fit colour per element in
+ * element order (elements sharing a node conflict). Returns the colour count via
+ * n_colors_out; color_ptr/color_elems (n_colors+1 / n_elems, ascending elements
+ * per colour) are filled when non-NULL and sized by the caller (color_ptr with
+ * room for n_elems+1 entries suffices). */
+int lfh_element_colors(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int32_t* color_out,
+                       int32_t* n_colors_out, int32_t* color_ptr, int32_t* color_elems);
+
+/* csr_profile (sparse.cpp:227-242): out = {nonzeros, bandwidth, profile}. */
+int lfh_csr_profile(int64_t n_rows, const int32_t* row_ptr, const int32_t* col, int64_t out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIGHTFEM_HOST_H */

@@ -1,0 +1,867 @@
+// C ABI of the gfx950 backend (include/lightfem_gpu.h): context, device buffers,
+// host-side set-up of the device layouts, and the graph-driven PCG loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lfg_kernels.hpp"
+#include "lightfem_gpu.h"
+
+namespace {
+
+struct LfgError : std::runtime_error {
+    int code;
+    LfgError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define LFG_HIP(call)                                                                                   \
+    do {                                                                                                \
+        const hipError_t err_ = (call);                                                                 \
+        if (err_ != hipSuccess)                                                                         \
+            throw LfgError(LFG_ERR_HIP, std::string(#call) + " failed: " + hipGetErrorString(err_));     \
+    } while (0)
+
+// Owning device allocation.
+template <typename T>
+struct DevArray {
+    T* p = nullptr;
+    size_t n = 0;
+    DevArray() = default;
+    DevArray(const DevArray&) = delete;
+    DevArray& operator=(const DevArray&) = delete;
+    ~DevArray() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        LFG_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T* host, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) LFG_HIP(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T* host, size_t count, hipStream_t s) const {
+        if (count) LFG_HIP(hipMemcpyAsync(host, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+struct Ic0Device {
+    DevArray<int> lrow, lcol, urow, ucol, fptr, frows, bptr, brows;
+    DevArray<double> lval, uval, inv_diag;
+    int fwd_levels = 0, bwd_levels = 0;
+    long long breakdowns = 0;
+    bool ready = false;
+};
+
+}  // namespace
+
+struct lfg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_ring[2] = {nullptr, nullptr};
+    std::string last_error;
+
+    // mesh
+    int n_nodes = 0, n_elems = 0, n_mats = 0, plane_mode = 0;
+    DevArray<double> xy, materials;
+    DevArray<int> enodes, elem_mat, node_elem_ptr, node_elem;
+    bool have_mesh = false;
+
+    // pattern
+    int n = 0;
+    long long nnz = 0;
+    std::vector<int> h_row_ptr, h_col;
+    DevArray<int> row_ptr, col, emap, cptr, contrib;
+    bool have_pattern = false;
+    std::vector<int> h_color_ptr;
+    DevArray<int> color_elems;
+
+    // values
+    DevArray<double> K, Kc, f, fc, ke_buf;
+    DevArray<float> K32;
+    bool have_K = false, have_f = false, have_system = false, k32_valid = false;
+    DevArray<int> constrained;
+    DevArray<double> prescribed;
+
+    // solver
+    DevArray<double> x, r, p, q, z, minv, factor, diag, partials, scratch;
+    DevArray<lfg::PcgState> state;
+    DevArray<int> flags;
+    lfg::PcgState* h_state = nullptr;  // pinned, 2 slots
+    bool have_solution = false;
+    Ic0Device ic0;
+    int precond_ready = -1;  // which preconditioner the device factors belong to
+
+    // fields
+    DevArray<double> centre, gp, nodal;
+};
+
+namespace {
+
+template <typename F>
+int guarded(lfg_ctx* ctx, F&& body) {
+    if (!ctx) return LFG_ERR_ARG;
+    try {
+        ctx->last_error.clear();
+        return body();
+    } catch (const LfgError& e) {
+        ctx->last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->last_error = e.what();
+        return LFG_ERR_ARG;
+    }
+}
+
+void require(bool cond, int code, const char* msg) {
+    if (!cond) throw LfgError(code, msg);
+}
+
+lfg::MeshView mesh_view(const lfg_ctx* c) {
+    return lfg::MeshView{c->n_nodes, c->n_elems, c->xy.p, c->enodes.p, c->elem_mat.p, c->materials.p, c->plane_mode};
+}
+
+lfg::CsrView csr_view(const lfg_ctx* c) { return lfg::CsrView{c->n, c->nnz, c->row_ptr.p, c->col.p}; }
+
+void sync(lfg_ctx* c) { LFG_HIP(hipStreamSynchronize(c->stream)); }
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    LFG_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+// Stable counting sort of the element->value map by target: for every CSR value,
+// the (element, local entry) pairs that add into it, in ascending element order.
+void build_gather_map(lfg_ctx* c, const int32_t* emap) {
+    const long long total = 64LL * c->n_elems;
+    std::vector<int> counts(static_cast<size_t>(c->nnz) + 1, 0);
+    for (long long k = 0; k < total; ++k) counts[static_cast<size_t>(emap[k]) + 1]++;
+    for (long long k = 0; k < c->nnz; ++k) counts[static_cast<size_t>(k) + 1] += counts[static_cast<size_t>(k)];
+    std::vector<int> contrib(static_cast<size_t>(total));
+    std::vector<int> cursor(counts.begin(), counts.end() - 1);
+    for (long long k = 0; k < total; ++k) contrib[static_cast<size_t>(cursor[static_cast<size_t>(emap[k])]++)] =
+        static_cast<int>(k);
+    c->cptr.upload(counts.data(), counts.size(), c->stream);
+    c->contrib.upload(contrib.data(), contrib.size(), c->stream);
+    sync(c);
+}
+
+// IC(0) of the lower triangle (precond.cpp:298-384), level sets for the device
+// sweeps, and the transposed strict lower part with rows in descending order.
+void build_ic0(lfg_ctx* c) {
+    const int n = c->n;
+    std::vector<double> kc(static_cast<size_t>(c->nnz));
+    c->Kc.download(kc.data(), kc.size(), c->stream);
+    sync(c);
+    std::vector<int> lrow(static_cast<size_t>(n) + 1, 0), lcol, diag_index(static_cast<size_t>(n), 0);
+    std::vector<double> lval, inv_diag(static_cast<size_t>(n), 1.0);
+    lcol.reserve(static_cast<size_t>(c->nnz / 2 + n));
+    lval.reserve(static_cast<size_t>(c->nnz / 2 + n));
+    for (int row = 0; row < n; ++row) {
+        bool found = false;
+        for (int k = c->h_row_ptr[row]; k < c->h_row_ptr[row + 1]; ++k) {
+            const int colk = c->h_col[k];
+            if (colk > row) break;
+            lcol.push_back(colk);
+            lval.push_back(kc[static_cast<size_t>(k)]);
+            if (colk == row) {
+                diag_index[row] = static_cast<int>(lval.size()) - 1;
+                found = true;
+            }
+        }
+        if (!found) {
+            lcol.push_back(row);
+            lval.push_back(1.0);
+            diag_index[row] = static_cast<int>(lval.size()) - 1;
+        }
+        lrow[row + 1] = static_cast<int>(lval.size());
+    }
+    double scale = 0.0;
+    for (int row = 0; row < n; ++row) scale = std::max(scale, std::abs(lval[diag_index[row]]));
+    const double floor_value = std::max(1.0e-30, 1.0e-14 * scale);
+    long long breakdowns = 0;
+    for (int row = 0; row < n; ++row) {
+        const int begin = lrow[row], d = diag_index[row];
+        for (int index = begin; index < d; ++index) {
+            const int column = lcol[index];
+            const int other_diag = diag_index[column];
+            double sum = lval[index];
+            int left = begin, right = lrow[column];
+            while (left < index && right < other_diag) {
+                const int lc = lcol[left], rc = lcol[right];
+                if (lc < rc) ++left;
+                else if (lc > rc) ++right;
+                else {
+                    sum -= lval[left] * lval[right];
+                    ++left;
+                    ++right;
+                }
+            }
+            lval[index] = sum * inv_diag[column];
+        }
+        double sum = lval[d];
+        for (int index = begin; index < d; ++index) sum -= lval[index] * lval[index];
+        if (!(sum > 0.0) || !std::isfinite(sum)) {
+            ++breakdowns;
+            sum = floor_value;
+        }
+        const double pivot = std::sqrt(sum);
+        lval[d] = pivot;
+        inv_diag[row] = 1.0 / pivot;
+    }
+    // forward levels
+    std::vector<int> level(static_cast<size_t>(n), 0);
+    int fwd_levels = 0;
+    for (int row = 0; row < n; ++row) {
+        int lv = 0;
+        for (int k = lrow[row]; k < lrow[row + 1] - 1; ++k) lv = std::max(lv, level[lcol[k]] + 1);
+        level[row] = lv;
+        fwd_levels = std::max(fwd_levels, lv + 1);
+    }
+    std::vector<int> fptr(static_cast<size_t>(fwd_levels) + 1, 0), frows(static_cast<size_t>(n));
+    for (int row = 0; row < n; ++row) fptr[level[row] + 1]++;
+    for (int l = 0; l < fwd_levels; ++l) fptr[l + 1] += fptr[l];
+    {
+        std::vector<int> cur(fptr.begin(), fptr.end() - 1);
+        for (int row = 0; row < n; ++row) frows[cur[level[row]]++] = row;
+    }
+    // transpose of the strict lower part, rows descending per column
+    std::vector<int> urow(static_cast<size_t>(n) + 1, 0);
+    for (int row = 0; row < n; ++row)
+        for (int k = lrow[row]; k < lrow[row + 1] - 1; ++k) urow[lcol[k] + 1]++;
+    for (int i = 0; i < n; ++i) urow[i + 1] += urow[i];
+    std::vector<int> ucol(static_cast<size_t>(urow[n]));
+    std::vector<double> uval(static_cast<size_t>(urow[n]));
+    {
+        std::vector<int> cur(urow.begin(), urow.end() - 1);
+        for (int row = n - 1; row >= 0; --row)
+            for (int k = lrow[row]; k < lrow[row + 1] - 1; ++k) {
+                const int cc = lcol[k];
+                ucol[cur[cc]] = row;
+                uval[cur[cc]] = lval[k];
+                ++cur[cc];
+            }
+    }
+    // backward levels
+    std::vector<int> blevel(static_cast<size_t>(n), 0);
+    int bwd_levels = 0;
+    for (int cc = n - 1; cc >= 0; --cc) {
+        int lv = 0;
+        for (int k = urow[cc]; k < urow[cc + 1]; ++k) lv = std::max(lv, blevel[ucol[k]] + 1);
+        blevel[cc] = lv;
+        bwd_levels = std::max(bwd_levels, lv + 1);
+    }
+    std::vector<int> bptr(static_cast<size_t>(bwd_levels) + 1, 0), brows(static_cast<size_t>(n));
+    for (int cc = 0; cc < n; ++cc) bptr[blevel[cc] + 1]++;
+    for (int l = 0; l < bwd_levels; ++l) bptr[l + 1] += bptr[l];
+    {
+        std::vector<int> cur(bptr.begin(), bptr.end() - 1);
+        for (int cc = 0; cc < n; ++cc) brows[cur[blevel[cc]]++] = cc;
+    }
+    Ic0Device& d = c->ic0;
+    d.lrow.upload(lrow.data(), lrow.size(), c->stream);
+    d.lcol.upload(lcol.data(), lcol.size(), c->stream);
+    d.lval.upload(lval.data(), lval.size(), c->stream);
+    d.inv_diag.upload(inv_diag.data(), inv_diag.size(), c->stream);
+    d.urow.upload(urow.data(), urow.size(), c->stream);
+    d.ucol.upload(ucol.data(), ucol.size(), c->stream);
+    d.uval.upload(uval.data(), uval.size(), c->stream);
+    d.fptr.upload(fptr.data(), fptr.size(), c->stream);
+    d.frows.upload(frows.data(), frows.size(), c->stream);
+    d.bptr.upload(bptr.data(), bptr.size(), c->stream);
+    d.brows.upload(brows.data(), brows.size(), c->stream);
+    d.fwd_levels = fwd_levels;
+    d.bwd_levels = bwd_levels;
+    d.breakdowns = breakdowns;
+    d.ready = true;
+    sync(c);
+}
+
+lfg::Ic0View ic0_view(const lfg_ctx* c) {
+    const Ic0Device& d = c->ic0;
+    return lfg::Ic0View{c->n,        d.lrow.p,  d.lcol.p,       d.lval.p,  d.inv_diag.p, d.urow.p,
+                        d.ucol.p,    d.uval.p,  d.fptr.p,       d.frows.p, d.fwd_levels, d.bptr.p,
+                        d.brows.p,   d.bwd_levels};
+}
+
+void ensure_vectors(lfg_ctx* c) {
+    const size_t n = static_cast<size_t>(c->n);
+    c->x.alloc(n);
+    c->r.alloc(n);
+    c->p.alloc(n);
+    c->q.alloc(n);
+    c->z.alloc(n);
+    c->partials.alloc(16384);
+    c->scratch.alloc(4096 + 8);
+    if (!c->state.p) {
+        c->state.alloc(1);
+        LFG_HIP(hipMemsetAsync(c->state.p, 0, sizeof(lfg::PcgState), c->stream));
+        c->flags.alloc(4);
+        LFG_HIP(hipMemsetAsync(c->scratch.p, 0, c->scratch.n * sizeof(double), c->stream));
+    }
+    if (!c->h_state) LFG_HIP(hipHostMalloc(&c->h_state, 2 * sizeof(lfg::PcgState), hipHostMallocDefault));
+}
+
+// Preconditioner factors on the constrained matrix. Returns set-up milliseconds.
+void setup_preconditioner(lfg_ctx* c, int kind, long long* degenerate) {
+    *degenerate = 0;
+    if (c->precond_ready == kind) return;
+    LFG_HIP(hipMemsetAsync(c->flags.p, 0, 4 * sizeof(int), c->stream));
+    const lfg::CsrView a = csr_view(c);
+    if (kind == LFG_PRECOND_JACOBI) {
+        c->diag.alloc(static_cast<size_t>(c->n));
+        c->minv.alloc(static_cast<size_t>(c->n));
+        lfg::launch_extract_diag(c->stream, a, c->Kc.p, c->diag.p, c->flags.p);
+        lfg::launch_jacobi_inv(c->stream, c->n, c->diag.p, c->minv.p, c->flags.p);
+        int flags = 0;
+        LFG_HIP(hipMemcpyAsync(&flags, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        if (flags & 1) throw LfgError(LFG_ERR_ARG, "requested entry is absent from preallocated CSR pattern");
+        if (flags & 2) throw LfgError(LFG_ERR_NONFINITE, "Jacobi preconditioner encountered a non-finite diagonal");
+    } else if (kind == LFG_PRECOND_BLOCK2) {
+        c->factor.alloc(3 * static_cast<size_t>(c->n / 2));
+        lfg::launch_block2_factor(c->stream, a, c->Kc.p, c->factor.p, c->flags.p + 1);
+        int deg = 0;
+        LFG_HIP(hipMemcpyAsync(&deg, c->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        *degenerate = deg;
+    } else {
+        build_ic0(c);
+    }
+    c->precond_ready = kind;
+}
+
+lfg::PcgBuffers pcg_buffers(lfg_ctx* c, int kind) {
+    lfg::PcgBuffers b{};
+    b.n = c->n;
+    b.x = c->x.p;
+    b.r = c->r.p;
+    b.p = c->p.p;
+    b.q = c->q.p;
+    b.z = c->z.p;
+    b.b = c->fc.p;
+    b.m = kind == LFG_PRECOND_JACOBI ? c->minv.p : (kind == LFG_PRECOND_BLOCK2 ? c->factor.p : nullptr);
+    b.precond = kind;
+    b.st = c->state.p;
+    b.partials = c->partials.p;
+    return b;
+}
+
+void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int variant) {
+    const lfg::CsrView a = csr_view(c);
+    lfg::launch_pcg_spmv(c->stream, b, a, mixed ? nullptr : c->Kc.p, mixed ? c->K32.p : nullptr, variant);
+    lfg::launch_pcg_update(c->stream, b, mixed);
+    if (mixed) lfg::launch_pcg_replace(c->stream, b, a, c->Kc.p);
+    if (b.precond == LFG_PRECOND_IC0) lfg::launch_pcg_ic0_step(c->stream, b, ic0_view(c));
+    lfg::launch_pcg_direction(c->stream, b);
+}
+
+const char* status_text(int s) {
+    switch (s) {
+        case LFG_OK: return "ok";
+        case LFG_ERR_ARG: return "invalid argument";
+        case LFG_ERR_JACOBIAN: return "Q4 stiffness evaluation received a non-positive Jacobian";
+        case LFG_ERR_NOT_SPD: return "PCG encountered non-positive curvature; system is not SPD";
+        case LFG_ERR_NONFINITE: return "PCG preconditioned residual became non-positive or non-finite";
+        case LFG_ERR_NOT_CONVERGED: return "maximum iterations reached without convergence";
+        case LFG_ERR_HIP: return "HIP runtime error";
+        case LFG_ERR_STATE: return "call order violated";
+        case LFG_ERR_UNSUPPORTED: return "unsupported option";
+        default: return "unknown status";
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lfg_status_string(int32_t status) { return status_text(status); }
+
+int lfg_create(int32_t device, lfg_ctx** out) {
+    if (!out) return LFG_ERR_ARG;
+    *out = nullptr;
+    auto ctx = std::make_unique<lfg_ctx>();
+    ctx->device = device;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return LFG_ERR_HIP;
+    if (hipSetDevice(device) != hipSuccess) return LFG_ERR_HIP;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return LFG_ERR_HIP;
+    if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess ||
+        hipEventCreate(&ctx->ev_ring[0]) != hipSuccess || hipEventCreate(&ctx->ev_ring[1]) != hipSuccess)
+        return LFG_ERR_HIP;
+    *out = ctx.release();
+    return LFG_OK;
+}
+
+void lfg_destroy(lfg_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->h_state) (void)hipHostFree(ctx->h_state);
+    for (hipEvent_t e : {ctx->ev_start, ctx->ev_stop, ctx->ev_ring[0], ctx->ev_ring[1]})
+        if (e) (void)hipEventDestroy(e);
+    hipStream_t s = ctx->stream;
+    delete ctx;  // device buffers free here
+    if (s) (void)hipStreamDestroy(s);
+}
+
+const char* lfg_last_error(const lfg_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int lfg_set_mesh(lfg_ctx* ctx, int64_t n_nodes, const double* xy, int64_t n_elems, const int32_t* enodes,
+                 const int32_t* elem_material, int64_t n_materials, const double* materials, int32_t plane_mode) {
+    return guarded(ctx, [&] {
+        require(n_nodes > 0 && n_elems > 0 && n_materials > 0 && xy && enodes && elem_material && materials,
+                LFG_ERR_ARG, "mesh arrays must be non-empty");
+        require(n_nodes < (1LL << 30) && n_elems < (1LL << 24), LFG_ERR_UNSUPPORTED,
+                "mesh exceeds the int32 index range of this backend");
+        require(plane_mode == LFG_PLANE_STRESS || plane_mode == LFG_PLANE_STRAIN, LFG_ERR_ARG, "unknown plane mode");
+        for (int64_t k = 0; k < 4 * n_elems; ++k)
+            require(enodes[k] >= 0 && enodes[k] < n_nodes, LFG_ERR_ARG, "validated element references an unknown node");
+        for (int64_t e = 0; e < n_elems; ++e)
+            require(elem_material[e] >= 0 && elem_material[e] < n_materials, LFG_ERR_ARG,
+                    "validated element references an unknown material");
+        LFG_HIP(hipSetDevice(ctx->device));
+        ctx->n_nodes = static_cast<int>(n_nodes);
+        ctx->n_elems = static_cast<int>(n_elems);
+        ctx->n_mats = static_cast<int>(n_materials);
+        ctx->plane_mode = plane_mode;
+        ctx->xy.upload(xy, 2 * static_cast<size_t>(n_nodes), ctx->stream);
+        ctx->enodes.upload(enodes, 4 * static_cast<size_t>(n_elems), ctx->stream);
+        ctx->elem_mat.upload(elem_material, static_cast<size_t>(n_elems), ctx->stream);
+        ctx->materials.upload(materials, 3 * static_cast<size_t>(n_materials), ctx->stream);
+        // node -> (element << 2 | local node) in ascending element order, for smoothing
+        std::vector<int> ptr(static_cast<size_t>(n_nodes) + 1, 0);
+        for (int64_t k = 0; k < 4 * n_elems; ++k) ptr[static_cast<size_t>(enodes[k]) + 1]++;
+        for (int64_t i = 0; i < n_nodes; ++i) ptr[i + 1] += ptr[i];
+        std::vector<int> list(static_cast<size_t>(4 * n_elems));
+        std::vector<int> cur(ptr.begin(), ptr.end() - 1);
+        for (int64_t e = 0; e < n_elems; ++e)
+            for (int l = 0; l < 4; ++l) list[cur[enodes[4 * e + l]]++] = static_cast<int>((e << 2) | l);
+        ctx->node_elem_ptr.upload(ptr.data(), ptr.size(), ctx->stream);
+        ctx->node_elem.upload(list.data(), list.size(), ctx->stream);
+        sync(ctx);
+        ctx->have_mesh = true;
+        ctx->have_K = ctx->have_system = ctx->have_solution = false;
+        ctx->precond_ready = -1;
+        return LFG_OK;
+    });
+}
+
+int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* row_ptr, const int32_t* col,
+                    const int32_t* emap) {
+    return guarded(ctx, [&] {
+        require(ctx->have_mesh, LFG_ERR_STATE, "lfg_set_mesh must precede lfg_set_pattern");
+        require(n_rows == 2LL * ctx->n_nodes, LFG_ERR_ARG, "CSR dimension must be two dofs per node");
+        require(nnz > 0 && nnz < (1LL << 31) && row_ptr && col && emap, LFG_ERR_ARG, "invalid CSR pattern");
+        require(row_ptr[0] == 0 && row_ptr[n_rows] == nnz, LFG_ERR_ARG, "row offsets do not span the nonzeros");
+        for (int64_t k = 0; k < 64LL * ctx->n_elems; ++k)
+            require(emap[k] >= 0 && emap[k] < nnz, LFG_ERR_ARG, "element map entry outside the CSR values");
+        LFG_HIP(hipSetDevice(ctx->device));
+        ctx->n = static_cast<int>(n_rows);
+        ctx->nnz = nnz;
+        ctx->h_row_ptr.assign(row_ptr, row_ptr + n_rows + 1);
+        ctx->h_col.assign(col, col + nnz);
+        ctx->row_ptr.upload(row_ptr, static_cast<size_t>(n_rows) + 1, ctx->stream);
+        ctx->col.upload(col, static_cast<size_t>(nnz), ctx->stream);
+        ctx->emap.upload(emap, 64 * static_cast<size_t>(ctx->n_elems), ctx->stream);
+        build_gather_map(ctx, emap);
+        ctx->K.alloc(static_cast<size_t>(nnz));
+        ctx->have_pattern = true;
+        ctx->have_K = ctx->have_system = ctx->have_solution = false;
+        ctx->precond_ready = -1;
+        return LFG_OK;
+    });
+}
+
+int lfg_set_colors(lfg_ctx* ctx, int32_t n_colors, const int32_t* color_ptr, const int32_t* color_elems) {
+    return guarded(ctx, [&] {
+        require(ctx->have_mesh, LFG_ERR_STATE, "lfg_set_mesh must precede lfg_set_colors");
+        require(n_colors > 0 && color_ptr && color_elems && color_ptr[0] == 0 && color_ptr[n_colors] == ctx->n_elems,
+                LFG_ERR_ARG, "colour groups must partition the elements");
+        for (int64_t k = 0; k < ctx->n_elems; ++k)
+            require(color_elems[k] >= 0 && color_elems[k] < ctx->n_elems, LFG_ERR_ARG, "colour lists an unknown element");
+        ctx->h_color_ptr.assign(color_ptr, color_ptr + n_colors + 1);
+        ctx->color_elems.upload(color_elems, static_cast<size_t>(ctx->n_elems), ctx->stream);
+        sync(ctx);
+        return LFG_OK;
+    });
+}
+
+int lfg_element_stiffness(lfg_ctx* ctx, double* ke_out) {
+    return guarded(ctx, [&] {
+        require(ctx->have_mesh && ke_out, LFG_ERR_STATE, "lfg_set_mesh must precede lfg_element_stiffness");
+        LFG_HIP(hipSetDevice(ctx->device));
+        ensure_vectors(ctx);
+        ctx->ke_buf.alloc(64 * static_cast<size_t>(ctx->n_elems));
+        LFG_HIP(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), ctx->stream));
+        lfg::launch_ke(ctx->stream, mesh_view(ctx), lfg::KE_STORE, nullptr, ctx->n_elems, nullptr, nullptr,
+                       ctx->ke_buf.p, ctx->flags.p);
+        int failures = 0;
+        LFG_HIP(hipMemcpyAsync(&failures, ctx->flags.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->ke_buf.download(ke_out, ctx->ke_buf.n, ctx->stream);
+        sync(ctx);
+        if (failures) throw LfgError(LFG_ERR_JACOBIAN, "Q4 stiffness evaluation received a non-positive Jacobian");
+        return LFG_OK;
+    });
+}
+
+int lfg_assemble(lfg_ctx* ctx, int32_t strategy, lfg_assembly_report* report) {
+    return guarded(ctx, [&] {
+        require(ctx->have_pattern, LFG_ERR_STATE, "lfg_set_pattern must precede lfg_assemble");
+        LFG_HIP(hipSetDevice(ctx->device));
+        ensure_vectors(ctx);
+        const lfg::MeshView m = mesh_view(ctx);
+        LFG_HIP(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), ctx->stream));
+        lfg_assembly_report rep{};
+        if (strategy == LFG_ASSEMBLY_SERIAL || strategy == LFG_ASSEMBLY_THREAD_LOCAL) {
+            ctx->ke_buf.alloc(64 * static_cast<size_t>(ctx->n_elems));
+            LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+            lfg::launch_ke(ctx->stream, m, lfg::KE_STORE, nullptr, ctx->n_elems, nullptr, nullptr, ctx->ke_buf.p,
+                           ctx->flags.p);
+            LFG_HIP(hipEventRecord(ctx->ev_ring[0], ctx->stream));
+            lfg::launch_gather(ctx->stream, ctx->nnz, ctx->cptr.p, ctx->contrib.p, ctx->ke_buf.p, ctx->K.p);
+            LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+            sync(ctx);
+            rep.element_stiffness_ms = elapsed(ctx->ev_start, ctx->ev_ring[0]);
+            rep.assembly_ms = elapsed(ctx->ev_ring[0], ctx->ev_stop);
+        } else if (strategy == LFG_ASSEMBLY_ATOMIC) {
+            LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+            lfg::launch_fill(ctx->stream, ctx->K.p, ctx->nnz, 0.0);
+            lfg::launch_ke(ctx->stream, m, lfg::KE_ATOMIC, nullptr, ctx->n_elems, ctx->emap.p, ctx->K.p, nullptr,
+                           ctx->flags.p);
+            LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+            sync(ctx);
+            rep.assembly_ms = elapsed(ctx->ev_start, ctx->ev_stop);  // Ke fused with the scatter
+        } else if (strategy == LFG_ASSEMBLY_COLORING) {
+            require(!ctx->h_color_ptr.empty(), LFG_ERR_STATE, "lfg_set_colors must precede coloring assembly");
+            LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+            lfg::launch_fill(ctx->stream, ctx->K.p, ctx->nnz, 0.0);
+            const int colors = static_cast<int>(ctx->h_color_ptr.size()) - 1;
+            for (int cidx = 0; cidx < colors; ++cidx) {
+                const int begin = ctx->h_color_ptr[cidx];
+                lfg::launch_ke(ctx->stream, m, lfg::KE_COLORED, ctx->color_elems.p + begin,
+                               ctx->h_color_ptr[cidx + 1] - begin, ctx->emap.p, ctx->K.p, nullptr, ctx->flags.p);
+            }
+            LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+            sync(ctx);
+            rep.assembly_ms = elapsed(ctx->ev_start, ctx->ev_stop);
+            rep.coloring_count = colors;
+        } else {
+            throw LfgError(LFG_ERR_ARG, "validated model has an unknown assembly strategy");
+        }
+        int failures = 0;
+        LFG_HIP(hipMemcpy(&failures, ctx->flags.p, sizeof(int), hipMemcpyDeviceToHost));
+        rep.jacobian_failures = failures;
+        if (report) *report = rep;
+        if (failures) throw LfgError(LFG_ERR_JACOBIAN, "Q4 stiffness evaluation received a non-positive Jacobian");
+        ctx->have_K = true;
+        ctx->have_system = ctx->have_solution = false;
+        ctx->precond_ready = -1;
+        return LFG_OK;
+    });
+}
+
+int lfg_get_stiffness(lfg_ctx* ctx, double* values_out) {
+    return guarded(ctx, [&] {
+        require(ctx->have_K && values_out, LFG_ERR_STATE, "no assembled stiffness");
+        ctx->K.download(values_out, static_cast<size_t>(ctx->nnz), ctx->stream);
+        sync(ctx);
+        return LFG_OK;
+    });
+}
+
+int lfg_set_stiffness(lfg_ctx* ctx, const double* values) {
+    return guarded(ctx, [&] {
+        require(ctx->have_pattern && values, LFG_ERR_STATE, "lfg_set_pattern must precede lfg_set_stiffness");
+        ctx->K.upload(values, static_cast<size_t>(ctx->nnz), ctx->stream);
+        sync(ctx);
+        ctx->have_K = true;
+        ctx->have_system = ctx->have_solution = false;
+        ctx->precond_ready = -1;
+        return LFG_OK;
+    });
+}
+
+int lfg_set_load(lfg_ctx* ctx, const double* f) {
+    return guarded(ctx, [&] {
+        require(ctx->have_pattern && f, LFG_ERR_STATE, "lfg_set_pattern must precede lfg_set_load");
+        ctx->f.upload(f, static_cast<size_t>(ctx->n), ctx->stream);
+        sync(ctx);
+        ctx->have_f = true;
+        ctx->have_system = ctx->have_solution = false;
+        return LFG_OK;
+    });
+}
+
+int lfg_dirichlet(lfg_ctx* ctx, int64_t n_constraints, const int32_t* dofs, const double* values) {
+    return guarded(ctx, [&] {
+        require(ctx->have_K && ctx->have_f, LFG_ERR_STATE, "assembly and load must precede lfg_dirichlet");
+        require(n_constraints >= 0 && (n_constraints == 0 || (dofs && values)), LFG_ERR_ARG, "invalid constraints");
+        std::vector<int> flag(static_cast<size_t>(ctx->n), 0);
+        std::vector<double> presc(static_cast<size_t>(ctx->n), 0.0);
+        for (int64_t k = 0; k < n_constraints; ++k) {
+            require(dofs[k] >= 0 && dofs[k] < ctx->n, LFG_ERR_ARG, "validated boundary references an unknown node");
+            flag[dofs[k]] = 1;
+            presc[dofs[k]] = values[k];
+        }
+        LFG_HIP(hipSetDevice(ctx->device));
+        ctx->constrained.upload(flag.data(), flag.size(), ctx->stream);
+        ctx->prescribed.upload(presc.data(), presc.size(), ctx->stream);
+        ctx->Kc.alloc(static_cast<size_t>(ctx->nnz));
+        ctx->fc.alloc(static_cast<size_t>(ctx->n));
+        if (n_constraints == 0) {  // solver.cpp:358: nothing to eliminate
+            LFG_HIP(hipMemcpyAsync(ctx->Kc.p, ctx->K.p, ctx->nnz * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            LFG_HIP(hipMemcpyAsync(ctx->fc.p, ctx->f.p, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        } else {
+            lfg::launch_dirichlet(ctx->stream, csr_view(ctx), ctx->K.p, ctx->f.p, ctx->constrained.p,
+                                  ctx->prescribed.p, ctx->Kc.p, ctx->fc.p);
+        }
+        sync(ctx);
+        ctx->have_system = true;
+        ctx->have_solution = false;
+        ctx->k32_valid = false;
+        ctx->precond_ready = -1;
+        return LFG_OK;
+    });
+}
+
+int lfg_get_system(lfg_ctx* ctx, double* values_out, double* rhs_out) {
+    return guarded(ctx, [&] {
+        require(ctx->have_system, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_get_system");
+        if (values_out) ctx->Kc.download(values_out, static_cast<size_t>(ctx->nnz), ctx->stream);
+        if (rhs_out) ctx->fc.download(rhs_out, static_cast<size_t>(ctx->n), ctx->stream);
+        sync(ctx);
+        return LFG_OK;
+    });
+}
+
+int lfg_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, double* u_out, lfg_pcg_report* report) {
+    return guarded(ctx, [&] {
+        require(ctx->have_system, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_pcg");
+        require(opt != nullptr, LFG_ERR_ARG, "missing PCG options");
+        const int kind = opt->preconditioner;
+        require(kind == LFG_PRECOND_JACOBI || kind == LFG_PRECOND_BLOCK2 || kind == LFG_PRECOND_IC0, LFG_ERR_ARG,
+                "preconditioner must be jacobi, block, or ic0");
+        const int mixed = opt->precision == LFG_PRECISION_MIXED;
+        require(!(mixed && kind == LFG_PRECOND_IC0), LFG_ERR_UNSUPPORTED,
+                "mixed precision is implemented for the jacobi and 2x2 block preconditioners");
+        require(opt->tolerance > 0.0 && opt->tolerance < 1.0, LFG_ERR_ARG, "solver tolerance must satisfy 0 < tolerance < 1");
+        require(opt->max_iterations > 0, LFG_ERR_ARG, "max_iterations must be greater than zero");
+        LFG_HIP(hipSetDevice(ctx->device));
+        ensure_vectors(ctx);
+        lfg_pcg_report rep{};
+        // set-up: preconditioner factors and FP32 values
+        LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+        long long degenerate = 0;
+        const int previous = ctx->precond_ready;
+        setup_preconditioner(ctx, kind, &degenerate);
+        if (previous == kind && kind == LFG_PRECOND_BLOCK2) degenerate = -1;  // factor reused; count unknown here
+        if (mixed && !ctx->k32_valid) {
+            ctx->K32.alloc(static_cast<size_t>(ctx->nnz));
+            lfg::launch_to_f32(ctx->stream, ctx->Kc.p, ctx->K32.p, ctx->nnz);
+            ctx->k32_valid = true;
+        }
+        LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+        sync(ctx);
+        rep.setup_ms = elapsed(ctx->ev_start, ctx->ev_stop);
+        rep.degenerate_blocks = degenerate;
+        if (kind == LFG_PRECOND_IC0) {
+            rep.breakdowns = ctx->ic0.breakdowns;
+            rep.levels = ctx->ic0.fwd_levels + ctx->ic0.bwd_levels;
+        }
+
+        const lfg::PcgBuffers b = pcg_buffers(ctx, kind);
+        const int variant = opt->spmv_variant;
+        const int per_graph = opt->check_every > 0 ? opt->check_every : 32;
+        const double drop = opt->replace_drop > 0.0 ? opt->replace_drop : 0.1;
+        // capture `per_graph` iterations once
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        const bool use_graph = opt->direct_launch == 0;
+        if (use_graph) {
+            LFG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant);
+            LFG_HIP(hipStreamEndCapture(ctx->stream, &graph));
+            LFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
+        struct GraphGuard {
+            hipGraph_t g;
+            hipGraphExec_t e;
+            ~GraphGuard() {
+                if (e) (void)hipGraphExecDestroy(e);
+                if (g) (void)hipGraphDestroy(g);
+            }
+        } guard{graph, exec};
+
+        LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+        lfg::launch_pcg_init(ctx->stream, b, opt->stop_rule, opt->tolerance, opt->max_iterations, drop);
+        if (kind == LFG_PRECOND_IC0) {
+            lfg::launch_pcg_ic0_step(ctx->stream, b, ic0_view(ctx));
+            lfg::launch_pcg_direction(ctx->stream, b);
+        }
+        const long long launch_cap = opt->max_iterations / per_graph + 2;
+        long long launched = 0;
+        auto launch_one = [&](int slot) {
+            if (use_graph) {
+                LFG_HIP(hipGraphLaunch(exec, ctx->stream));
+            } else {
+                for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant);
+            }
+            LFG_HIP(hipMemcpyAsync(ctx->h_state + slot, ctx->state.p, sizeof(lfg::PcgState), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            LFG_HIP(hipEventRecord(ctx->ev_ring[slot], ctx->stream));
+            ++launched;
+        };
+        launch_one(0);
+        long long k = 0;
+        float solve_ms = 0.0f;
+        while (true) {
+            const int slot = static_cast<int>(k & 1);
+            if (launched < launch_cap) launch_one(1 - slot);
+            LFG_HIP(hipEventSynchronize(ctx->ev_ring[slot]));
+            if (ctx->h_state[slot].done || launched >= launch_cap) {
+                solve_ms = elapsed(ctx->ev_start, ctx->ev_ring[slot]);
+                break;
+            }
+            ++k;
+        }
+        sync(ctx);
+        lfg::PcgState st{};
+        LFG_HIP(hipMemcpy(&st, ctx->state.p, sizeof(st), hipMemcpyDeviceToHost));
+        // FP64 true residual ||b - K u|| for the report
+        double r2 = 0.0;
+        lfg::launch_residual_norm(ctx->stream, csr_view(ctx), ctx->Kc.p, ctx->x.p, ctx->fc.p, ctx->scratch.p,
+                                  ctx->scratch.p + 4096 + 4);  // partials [0, 2048), ticket at 4096
+        LFG_HIP(hipMemcpyAsync(&r2, ctx->scratch.p + 4096 + 4, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        if (u_out) ctx->x.download(u_out, static_cast<size_t>(ctx->n), ctx->stream);
+        sync(ctx);
+        rep.converged = st.converged;
+        rep.iterations = st.iter;
+        rep.residual_replacements = st.replacements;
+        rep.initial_residual = st.rnorm0;
+        rep.final_residual = std::sqrt(std::max(0.0, st.rr));
+        rep.true_residual = std::sqrt(std::max(0.0, r2));
+        rep.solve_ms = solve_ms;
+        rep.status = st.status != LFG_OK ? st.status : (st.converged ? LFG_OK : LFG_ERR_NOT_CONVERGED);
+        if (report) *report = rep;
+        ctx->have_solution = true;
+        if (st.status != LFG_OK) throw LfgError(st.status, status_text(st.status));
+        return st.converged ? LFG_OK : LFG_ERR_NOT_CONVERGED;
+    });
+}
+
+int lfg_reactions(lfg_ctx* ctx, double* reaction_out) {
+    return guarded(ctx, [&] {
+        require(ctx->have_solution && reaction_out, LFG_ERR_STATE, "lfg_pcg must precede lfg_reactions");
+        LFG_HIP(hipSetDevice(ctx->device));
+        lfg::launch_spmv_serial_rows(ctx->stream, csr_view(ctx), ctx->K.p, ctx->x.p, ctx->f.p, ctx->q.p);
+        ctx->q.download(reaction_out, static_cast<size_t>(ctx->n), ctx->stream);
+        sync(ctx);
+        return LFG_OK;
+    });
+}
+
+int lfg_postprocess(lfg_ctx* ctx, double* centre, double* gauss_points, double* nodal) {
+    return guarded(ctx, [&] {
+        require(ctx->have_solution, LFG_ERR_STATE, "lfg_pcg must precede lfg_postprocess");
+        LFG_HIP(hipSetDevice(ctx->device));
+        const size_t e = static_cast<size_t>(ctx->n_elems);
+        ctx->centre.alloc(7 * e);
+        ctx->gp.alloc(36 * e);
+        ctx->nodal.alloc(4 * static_cast<size_t>(ctx->n_nodes));
+        lfg::launch_fields(ctx->stream, mesh_view(ctx), ctx->x.p, ctx->centre.p, ctx->gp.p);
+        lfg::launch_smooth(ctx->stream, ctx->n_nodes, ctx->node_elem_ptr.p, ctx->node_elem.p, ctx->gp.p,
+                           ctx->nodal.p);
+        if (centre) ctx->centre.download(centre, 7 * e, ctx->stream);
+        if (gauss_points) ctx->gp.download(gauss_points, 36 * e, ctx->stream);
+        if (nodal) ctx->nodal.download(nodal, 4 * static_cast<size_t>(ctx->n_nodes), ctx->stream);
+        sync(ctx);
+        return LFG_OK;
+    });
+}
+
+int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, double* out) {
+    return guarded(ctx, [&] {
+        require(ctx->have_system && in && out, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_precond_apply");
+        LFG_HIP(hipSetDevice(ctx->device));
+        ensure_vectors(ctx);
+        long long degenerate = 0;
+        setup_preconditioner(ctx, preconditioner, &degenerate);
+        LFG_HIP(hipMemcpyAsync(ctx->r.p, in, ctx->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        if (preconditioner == LFG_PRECOND_IC0) {
+            lfg::launch_ic0_apply(ctx->stream, ic0_view(ctx), ctx->r.p, ctx->z.p);
+        } else {
+            lfg::launch_precond_apply(ctx->stream, preconditioner, ctx->n,
+                                      preconditioner == LFG_PRECOND_JACOBI ? ctx->minv.p : ctx->factor.p, ctx->r.p,
+                                      ctx->z.p);
+        }
+        ctx->z.download(out, static_cast<size_t>(ctx->n), ctx->stream);
+        sync(ctx);
+        ctx->have_solution = false;
+        return LFG_OK;
+    });
+}
+
+static void run_spmv(lfg_ctx* ctx, int precision, int variant) {
+    lfg::PcgBuffers b = pcg_buffers(ctx, LFG_PRECOND_JACOBI);
+    b.st = ctx->state.p;
+    lfg::launch_pcg_spmv(ctx->stream, b, csr_view(ctx), precision == LFG_PRECISION_MIXED ? nullptr : ctx->Kc.p,
+                         precision == LFG_PRECISION_MIXED ? ctx->K32.p : nullptr, variant);
+}
+
+static void prepare_spmv(lfg_ctx* ctx, int precision) {
+    ensure_vectors(ctx);
+    lfg::PcgState zero{};
+    LFG_HIP(hipMemcpyAsync(ctx->state.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
+    if (precision == LFG_PRECISION_MIXED && !ctx->k32_valid) {
+        ctx->K32.alloc(static_cast<size_t>(ctx->nnz));
+        lfg::launch_to_f32(ctx->stream, ctx->Kc.p, ctx->K32.p, ctx->nnz);
+        ctx->k32_valid = true;
+    }
+}
+
+int lfg_spmv(lfg_ctx* ctx, int32_t precision, const double* x, double* y) {
+    return guarded(ctx, [&] {
+        require(ctx->have_system && x && y, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_spmv");
+        LFG_HIP(hipSetDevice(ctx->device));
+        prepare_spmv(ctx, precision);
+        LFG_HIP(hipMemcpyAsync(ctx->p.p, x, ctx->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        run_spmv(ctx, precision, 0);
+        ctx->q.download(y, static_cast<size_t>(ctx->n), ctx->stream);
+        sync(ctx);
+        ctx->have_solution = false;
+        return LFG_OK;
+    });
+}
+
+int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t reps, double* avg_ms) {
+    return guarded(ctx, [&] {
+        require(ctx->have_system && reps > 0 && avg_ms, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_bench_spmv");
+        LFG_HIP(hipSetDevice(ctx->device));
+        prepare_spmv(ctx, precision);
+        lfg::launch_fill(ctx->stream, ctx->p.p, ctx->n, 1.0);
+        run_spmv(ctx, precision, variant);  // warm-up
+        LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+        for (int i = 0; i < reps; ++i) run_spmv(ctx, precision, variant);
+        LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+        sync(ctx);
+        *avg_ms = elapsed(ctx->ev_start, ctx->ev_stop) / reps;
+        ctx->have_solution = false;
+        return LFG_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,322 @@
+// Q4 element kernels for gfx950: element stiffness with the three assembly
+// strategies, and centre / Gauss-point field recovery with nodal smoothing.
+//
+// Arithmetic follows the reference's operation order term by term
+// (element.cpp:16-161, 212-248; solver.cpp:135-179, 807-871) and the library is
+// built with -ffp-contract=off, so Ke, serial/coloured K and the fields are
+// bit-identical to the CPU reference. Ke is ~4.6 flop per byte written, so the
+// kernels are memory-bound: one element is spread over 8 lanes (lane r owns Ke
+// row r), which keeps registers low and makes each wave write 8 contiguous
+// 512-byte Ke blocks or 512 scattered map entries per step.
+#include "lfg_kernels.hpp"
+
+namespace lfg {
+namespace {
+
+constexpr double kGauss = 0.57735026918962576451;  // element.cpp:132
+
+struct Coords {
+    double x[4];
+    double y[4];
+};
+
+__device__ __forceinline__ void constitutive(double e, double nu, int plane_mode, double d[9]) {
+    // element.cpp:16-32
+    if (plane_mode == 0) {
+        const double factor = e / (1.0 - nu * nu);
+        d[0] = factor; d[1] = factor * nu; d[2] = 0.0;
+        d[3] = factor * nu; d[4] = factor; d[5] = 0.0;
+        d[6] = 0.0; d[7] = 0.0; d[8] = factor * (1.0 - nu) * 0.5;
+    } else {
+        const double factor = e / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        d[0] = factor * (1.0 - nu); d[1] = factor * nu; d[2] = 0.0;
+        d[3] = factor * nu; d[4] = factor * (1.0 - nu); d[5] = 0.0;
+        d[6] = 0.0; d[7] = 0.0; d[8] = factor * (1.0 - 2.0 * nu) * 0.5;
+    }
+}
+
+// element.cpp:34-73. b is 3x8 row-major; returns false on a non-positive Jacobian.
+__device__ __forceinline__ bool kinematics(const Coords& c, double xi, double eta, double b[24], double& det) {
+    const double dxi[4] = {-0.25 * (1.0 - eta), +0.25 * (1.0 - eta), +0.25 * (1.0 + eta), -0.25 * (1.0 + eta)};
+    const double deta[4] = {-0.25 * (1.0 - xi), -0.25 * (1.0 + xi), +0.25 * (1.0 + xi), +0.25 * (1.0 - xi)};
+    double dx_dxi = 0.0, dx_deta = 0.0, dy_dxi = 0.0, dy_deta = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        dx_dxi += dxi[i] * c.x[i];
+        dx_deta += deta[i] * c.x[i];
+        dy_dxi += dxi[i] * c.y[i];
+        dy_deta += deta[i] * c.y[i];
+    }
+    det = dx_dxi * dy_deta - dx_deta * dy_dxi;
+    if (!(det > 1.0e-14) || !isfinite(det)) return false;
+    const double inv00 = dy_deta / det;
+    const double inv01 = -dx_deta / det;
+    const double inv10 = -dy_dxi / det;
+    const double inv11 = dx_dxi / det;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double dnx = inv00 * dxi[i] + inv10 * deta[i];
+        const double dny = inv01 * dxi[i] + inv11 * deta[i];
+        b[2 * i] = dnx;
+        b[2 * i + 1] = 0.0;
+        b[8 + 2 * i] = 0.0;
+        b[8 + 2 * i + 1] = dny;
+        b[16 + 2 * i] = dny;
+        b[16 + 2 * i + 1] = dnx;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void load_element(const MeshView& m, int e, Coords& c, double& young, double& nu,
+                                             double& thick) {
+    const int4 nodes = reinterpret_cast<const int4*>(m.enodes)[e];
+    const int ids[4] = {nodes.x, nodes.y, nodes.z, nodes.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double2 p = reinterpret_cast<const double2*>(m.xy)[ids[i]];
+        c.x[i] = p.x;
+        c.y[i] = p.y;
+    }
+    const int mat = m.elem_mat[e];
+    young = m.materials[3 * mat];
+    nu = m.materials[3 * mat + 1];
+    thick = m.materials[3 * mat + 2];
+}
+
+// Row `row` of Ke (element.cpp:130-161): ke[c] = sum_gp (sum_s b[s][row] db[s][c]) * detJ * t.
+__device__ __forceinline__ bool ke_row(const Coords& c, double young, double nu, double thick, int plane_mode,
+                                       int row, double ke[8]) {
+    const double points[4][2] = {{-kGauss, -kGauss}, {kGauss, -kGauss}, {kGauss, kGauss}, {-kGauss, kGauss}};
+    double d[9];
+    constitutive(young, nu, plane_mode, d);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ke[k] = 0.0;
+    bool ok = true;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        double b[24], det;
+        ok = kinematics(c, points[p][0], points[p][1], b, det) && ok;
+        double db[24];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int col = 0; col < 8; ++col) {
+                double acc = 0.0;
+#pragma unroll
+                for (int inner = 0; inner < 3; ++inner) acc += d[3 * r + inner] * b[8 * inner + col];
+                db[8 * r + col] = acc;
+            }
+        const double weight = det * thick;
+        const double b0 = b[row], b1 = b[8 + row], b2 = b[16 + row];
+#pragma unroll
+        for (int col = 0; col < 8; ++col) {
+            double contribution = 0.0;
+            contribution += b0 * db[col];
+            contribution += b1 * db[8 + col];
+            contribution += b2 * db[16 + col];
+            ke[col] += contribution * weight;
+        }
+    }
+    return ok;
+}
+
+template <int Mode>
+__global__ __launch_bounds__(256) void ke_kernel(MeshView m, const int* __restrict__ elems, int count,
+                                                 const int* __restrict__ emap, double* __restrict__ values,
+                                                 double* __restrict__ ke_out, int* __restrict__ failures) {
+    const long long tid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int slot = static_cast<int>(tid >> 3);
+    const int row = static_cast<int>(tid & 7);
+    if (slot >= count) return;
+    const int e = elems ? elems[slot] : slot;
+    Coords c;
+    double young, nu, thick;
+    load_element(m, e, c, young, nu, thick);
+    double ke[8];
+    const bool ok = ke_row(c, young, nu, thick, m.plane_mode, row, ke);
+    if (!ok) {
+        if (row == 0) atomicAdd(failures, 1);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ke[k] = 0.0;
+    }
+    const long long base = 64LL * e + 8 * row;
+    if (Mode == KE_STORE) {
+        double2* out = reinterpret_cast<double2*>(ke_out + base);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = make_double2(ke[2 * k], ke[2 * k + 1]);
+    } else {
+        const int4* map4 = reinterpret_cast<const int4*>(emap + base);
+        const int4 m0 = map4[0], m1 = map4[1];
+        const int idx[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (Mode == KE_ATOMIC) {
+                unsafeAtomicAdd(values + idx[k], ke[k]);  // global_atomic_add_f64
+            } else {
+                values[idx[k]] += ke[k];  // conflict-free within one colour
+            }
+        }
+    }
+}
+
+// Serial-order assembly as a gather (solver.cpp:135-142): each CSR value sums its
+// contributions in ascending element order starting from 0.0.
+__global__ __launch_bounds__(256) void gather_kernel(long long nnz, const int* __restrict__ cptr,
+                                                     const int* __restrict__ contrib, const double* __restrict__ ke,
+                                                     double* __restrict__ values) {
+    const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    double sum = 0.0;
+    for (int j = cptr[k]; j < cptr[k + 1]; ++j) sum += ke[contrib[j]];
+    values[k] = sum;
+}
+
+__global__ void fill_kernel(double* p, long long n, double v) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// element.cpp:75-95, 212-248: strain = B u, stress = D strain, in-plane von Mises.
+__device__ __forceinline__ void field_at(const double b[24], const double d[9], const double u[8], double* out) {
+    double strain[3] = {0.0, 0.0, 0.0};
+    double stress[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) strain[r] += b[8 * r + c] * u[c];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) stress[r] += d[3 * r + c] * strain[c];
+    const double sx = stress[0], sy = stress[1], txy = stress[2];
+    out[0] = strain[0];
+    out[1] = strain[1];
+    out[2] = strain[2];
+    out[3] = sx;
+    out[4] = sy;
+    out[5] = txy;
+    out[6] = sqrt(max0(sx * sx - sx * sy + sy * sy + 3.0 * txy * txy));
+}
+
+// One thread per element: centre field (7 doubles) and 4 Gauss points (9 doubles each).
+__global__ __launch_bounds__(128) void fields_kernel(MeshView m, const double* __restrict__ u,
+                                                     double* __restrict__ centre, double* __restrict__ gp) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.n_elems) return;
+    Coords c;
+    double young, nu, thick;
+    load_element(m, e, c, young, nu, thick);
+    const int4 nodes = reinterpret_cast<const int4*>(m.enodes)[e];
+    const int ids[4] = {nodes.x, nodes.y, nodes.z, nodes.w};
+    double ue[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double2 v = reinterpret_cast<const double2*>(u)[ids[i]];
+        ue[2 * i] = v.x;
+        ue[2 * i + 1] = v.y;
+    }
+    double d[9], b[24], det;
+    constitutive(young, nu, m.plane_mode, d);
+    double out[7];
+    kinematics(c, 0.0, 0.0, b, det);
+    field_at(b, d, ue, out);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) centre[7LL * e + k] = out[k];
+    const double points[4][2] = {{-kGauss, -kGauss}, {kGauss, -kGauss}, {kGauss, kGauss}, {-kGauss, kGauss}};
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        kinematics(c, points[p][0], points[p][1], b, det);
+        double* g = gp + 36LL * e + 9 * p;
+        g[0] = points[p][0];
+        g[1] = points[p][1];
+        field_at(b, d, ue, out);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) g[2 + k] = out[k];
+    }
+}
+
+// solver.cpp:807-871 as a gather: node i sums extrapolated Gauss-point stresses of
+// its elements in ascending element order (the order the reference's element
+// loop adds them), then averages and forms the nodal von Mises stress.
+__global__ __launch_bounds__(256) void smooth_kernel(int n_nodes, const int* __restrict__ ptr,
+                                                     const int* __restrict__ list, const double* __restrict__ gp,
+                                                     double* __restrict__ nodal) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= n_nodes) return;
+    const double root_three = 1.7320508075688772935;
+    const double alpha = 0.5 * (1.0 + root_three);
+    const double beta = 0.5 * (1.0 - root_three);
+    const double ex[4][4] = {
+        {alpha * alpha, alpha * beta, beta * beta, alpha * beta},
+        {alpha * beta, alpha * alpha, alpha * beta, beta * beta},
+        {beta * beta, alpha * beta, alpha * alpha, alpha * beta},
+        {alpha * beta, beta * beta, alpha * beta, alpha * alpha},
+    };
+    double sums[3] = {0.0, 0.0, 0.0};
+    const int begin = ptr[node], end = ptr[node + 1];
+    for (int j = begin; j < end; ++j) {
+        const int entry = list[j];
+        const int e = entry >> 2;
+        const int local = entry & 3;
+        const double* g = gp + 36LL * e;
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sums[comp] += ex[local][q] * g[9 * q + 5 + comp];
+    }
+    double* out = nodal + 4LL * node;
+    const int count = end - begin;
+    if (count == 0) {
+        out[0] = out[1] = out[2] = out[3] = 0.0;
+        return;
+    }
+    for (int comp = 0; comp < 3; ++comp) out[comp] = sums[comp] / static_cast<double>(count);
+    const double sx = out[0], sy = out[1], txy = out[2];
+    out[3] = sqrt(max0(sx * sx - sx * sy + sy * sy + 3.0 * txy * txy));
+}
+
+inline unsigned int blocks_for(long long work, int per_block) {
+    return static_cast<unsigned int>((work + per_block - 1) / per_block);
+}
+
+}  // namespace
+
+void launch_ke(hipStream_t s, const MeshView& m, KeMode mode, const int* elems, int count, const int* emap,
+               double* values, double* ke_out, int* failures) {
+    if (count <= 0) return;
+    const unsigned int grid = blocks_for(8LL * count, 256);
+    switch (mode) {
+        case KE_STORE:
+            ke_kernel<KE_STORE><<<grid, 256, 0, s>>>(m, elems, count, emap, values, ke_out, failures);
+            break;
+        case KE_ATOMIC:
+            ke_kernel<KE_ATOMIC><<<grid, 256, 0, s>>>(m, elems, count, emap, values, ke_out, failures);
+            break;
+        case KE_COLORED:
+            ke_kernel<KE_COLORED><<<grid, 256, 0, s>>>(m, elems, count, emap, values, ke_out, failures);
+            break;
+    }
+}
+
+void launch_gather(hipStream_t s, long long nnz, const int* cptr, const int* contrib, const double* ke,
+                   double* values) {
+    if (nnz <= 0) return;
+    gather_kernel<<<blocks_for(nnz, 256), 256, 0, s>>>(nnz, cptr, contrib, ke, values);
+}
+
+void launch_fill(hipStream_t s, double* p, long long n, double v) {
+    if (n <= 0) return;
+    fill_kernel<<<blocks_for(n, 256), 256, 0, s>>>(p, n, v);
+}
+
+void launch_fields(hipStream_t s, const MeshView& m, const double* u, double* centre, double* gp) {
+    if (m.n_elems <= 0) return;
+    fields_kernel<<<blocks_for(m.n_elems, 128), 128, 0, s>>>(m, u, centre, gp);
+}
+
+void launch_smooth(hipStream_t s, int n_nodes, const int* ptr, const int* list, const double* gp, double* nodal) {
+    if (n_nodes <= 0) return;
+    smooth_kernel<<<blocks_for(n_nodes, 256), 256, 0, s>>>(n_nodes, ptr, list, gp, nodal);
+}
+
+}  // namespace lfg

@@ -1,0 +1,721 @@
+// Solver kernels for gfx950: symmetric Dirichlet elimination, reactions, the
+// fused preconditioned-CG iteration (FP64 and FP32-value mixed precision) and
+// the level-scheduled IC(0) sweeps.
+//
+// Layout: CSR with int32 row offsets and columns, FP64 (or FP32) values; node i
+// owns rows 2i and 2i+1, which share one column set. The SpMV therefore works
+// on node groups: 2W lanes per node, W lanes per row, each group loading its
+// row segments with consecutive lanes (coalesced) and reducing with wave
+// shuffles. Every reduction is finished in-kernel by the last block to arrive
+// (ticket pattern), so one CG iteration is three launches (SpMV + p.Ap, update
+// + r.z, direction) that a hipGraph replays without host round trips; each
+// kernel returns immediately once the device state says the solve is done.
+#include "lfg_kernels.hpp"
+
+#include "lightfem_gpu.h"
+
+namespace lfg {
+namespace {
+
+constexpr int kThreads = 256;
+
+inline unsigned int blocks_for(long long work, int per_block) {
+    return static_cast<unsigned int>((work + per_block - 1) / per_block);
+}
+
+inline unsigned int round8(unsigned int b) { return (b + 7u) & ~7u; }
+
+// ------------------------------------------------------------ Dirichlet
+
+// solver.cpp:355-413. One node group of 16 lanes per row pair would scatter the
+// RHS sum; the sum must run in column order to stay bit-exact, so lane 0 of each
+// 8-lane row group does it while all 8 lanes copy/zero the row coalesced.
+__global__ __launch_bounds__(256) void dirichlet_kernel(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                                        const double* __restrict__ values, const double* __restrict__ f,
+                                                        const int* __restrict__ constrained,
+                                                        const double* __restrict__ prescribed, double* __restrict__ kc,
+                                                        double* __restrict__ fc) {
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const int lane = threadIdx.x & 7;
+    if (row >= n) return;
+    const int begin = rp[row], end = rp[row + 1];
+    if (constrained[row]) {
+        for (int k = begin + lane; k < end; k += 8) kc[k] = (col[k] == row) ? 1.0 : 0.0;
+        if (lane == 0) fc[row] = prescribed[row];
+        return;
+    }
+    for (int k = begin + lane; k < end; k += 8) {
+        const double v = values[k];
+        kc[k] = (constrained[col[k]] != 0 && v != 0.0) ? 0.0 : v;
+    }
+    if (lane == 0) {
+        double rhs = f[row];
+        for (int k = begin; k < end; ++k) {
+            const int c = col[k];
+            const double v = values[k];
+            if (constrained[c] == 0 || v == 0.0) continue;
+            rhs -= v * prescribed[c];
+        }
+        fc[row] = rhs;
+    }
+}
+
+// y = A x - sub with each row summed serially in column order (sparse.cpp:104-120
+// followed by solver.cpp:953-954), so reactions are bit-identical to the CPU.
+__global__ __launch_bounds__(256) void spmv_serial_rows_kernel(int n, const int* __restrict__ rp,
+                                                               const int* __restrict__ col,
+                                                               const double* __restrict__ values,
+                                                               const double* __restrict__ x,
+                                                               const double* __restrict__ sub, double* __restrict__ y) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    double sum = 0.0;
+    for (int k = rp[row]; k < rp[row + 1]; ++k) sum += values[k] * x[col[k]];
+    y[row] = sub ? sum - sub[row] : sum;
+}
+
+__device__ __forceinline__ int find_in_row(const int* col, int begin, int end, int c) {
+    int lo = begin, hi = end;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (col[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    return (lo < end && col[lo] == c) ? lo : -1;
+}
+
+__global__ void extract_diag_kernel(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                    const double* __restrict__ values, double* __restrict__ diag, int* status) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    const int k = find_in_row(col, rp[row], rp[row + 1], row);
+    if (k < 0) {
+        atomicOr(status, 1);
+        diag[row] = 0.0;
+        return;
+    }
+    diag[row] = values[k];
+}
+
+// inverse_diagonal_of (solver.cpp:656-669).
+__global__ void jacobi_inv_kernel(int n, const double* __restrict__ diag, double* __restrict__ inv, int* status) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double d = diag[i];
+    if (!isfinite(d)) atomicOr(status, 2);
+    inv[i] = fabs(d) > 1.0e-30 ? 1.0 / d : 1.0;
+}
+
+// SubdomainBlockJacobi factor for one-node subdomains (precond.cpp:178-232):
+// packed [1/l00, l10, 1/l11] per node, identity when the block is not SPD.
+__global__ void block2_factor_kernel(int n_nodes, const int* __restrict__ rp, const int* __restrict__ col,
+                                     const double* __restrict__ values, double* __restrict__ factor, int* degenerate) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= n_nodes) return;
+    double dense[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < 2; ++i) {
+        const int dof = 2 * node + i;
+        for (int k = rp[dof]; k < rp[dof + 1]; ++k) {
+            const int c = col[k];
+            if ((c >> 1) != node) continue;
+            dense[i * 2 + (c - 2 * node)] = values[k];
+        }
+    }
+    bool factorized = true;
+    for (int i = 0; i < 2 && factorized; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double sum = dense[i * 2 + j];
+            for (int k = 0; k < j; ++k) sum -= dense[i * 2 + k] * dense[j * 2 + k];
+            if (i == j) {
+                if (!(sum > 0.0) || !isfinite(sum)) {
+                    factorized = false;
+                    break;
+                }
+                dense[i * 2 + i] = sqrt(sum);
+            } else {
+                dense[i * 2 + j] = sum / dense[j * 2 + j];
+            }
+        }
+    if (!factorized) {
+        atomicAdd(degenerate, 1);
+        dense[0] = 1.0;
+        dense[2] = 0.0;
+        dense[3] = 1.0;
+    }
+    double* t = factor + 3LL * node;
+    t[0] = 1.0 / dense[0];
+    t[1] = dense[2];
+    t[2] = 1.0 / dense[3];
+}
+
+// precond.cpp:266-287 for a size-2 block: forward then backward substitution with
+// the packed factor (reciprocal pivots).
+__device__ __forceinline__ void block2_solve(const double* f, double r0, double r1, double& z0, double& z1) {
+    const double f0 = f[0], f1 = f[1], f2 = f[2];
+    double s0 = r0 * f0;
+    double s1 = r1;
+    s1 -= f1 * s0;
+    s1 = s1 * f2;
+    s1 = s1 * f2;
+    double t = s0;
+    t -= f1 * s1;
+    z0 = t * f0;
+    z1 = s1;
+}
+
+// z = M^-1 r for node i (dofs 2i, 2i+1).
+template <int Prec>
+__device__ __forceinline__ void precond_node(const double* m, int node, double r0, double r1, double& z0, double& z1) {
+    if (Prec == LFG_PRECOND_JACOBI) {
+        const double2 d = reinterpret_cast<const double2*>(m)[node];
+        z0 = d.x * r0;  // solver.cpp:502-505
+        z1 = d.y * r1;
+    } else {
+        block2_solve(m + 3LL * node, r0, r1, z0, z1);
+    }
+}
+
+__global__ void precond_apply_kernel(int kind, int n_nodes, const double* __restrict__ m, const double* __restrict__ in,
+                                     double* __restrict__ out) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= n_nodes) return;
+    const double2 r = reinterpret_cast<const double2*>(in)[node];
+    double z0, z1;
+    if (kind == LFG_PRECOND_JACOBI) precond_node<LFG_PRECOND_JACOBI>(m, node, r.x, r.y, z0, z1);
+    else precond_node<LFG_PRECOND_BLOCK2>(m, node, r.x, r.y, z0, z1);
+    reinterpret_cast<double2*>(out)[node] = make_double2(z0, z1);
+}
+
+__global__ void to_f32_kernel(const double* __restrict__ in, float* __restrict__ out, long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = static_cast<float>(in[i]);
+}
+
+// ------------------------------------------------------------ PCG kernels
+
+// Row sum of a W-lane row group: lanes stride the row segment (coalesced loads of
+// values and columns), gather x, accumulate in FP64, reduce with shuffles. The
+// result is valid in every lane of the group.
+template <int W, typename VT>
+__device__ __forceinline__ double row_dot(const int* __restrict__ rp, const int* __restrict__ col,
+                                          const VT* __restrict__ val, const double* __restrict__ x, int row,
+                                          int lane) {
+    const int begin = rp[row], end = rp[row + 1];
+    double sum = 0.0;
+    for (int k = begin + lane; k < end; k += W) sum += static_cast<double>(val[k]) * x[col[k]];
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, W);
+    return sum;
+}
+
+// r0 = b, x0 = 0, z0 = M^-1 r0, p0 = z0; ||r0||^2 and r0.z0; threshold.
+template <int Prec>
+__global__ __launch_bounds__(kThreads) void pcg_init_kernel(PcgBuffers b, int stop_rule, double tol,
+                                                            long long max_iter, double drop) {
+    __shared__ double sh[kThreads / kWave];
+    const int n_nodes = b.n >> 1;
+    double rr = 0.0, rz = 0.0;
+    for (int node = blockIdx.x * kThreads + threadIdx.x; node < n_nodes; node += gridDim.x * kThreads) {
+        const double2 r = reinterpret_cast<const double2*>(b.b)[node];
+        reinterpret_cast<double2*>(b.r)[node] = r;
+        reinterpret_cast<double2*>(b.x)[node] = make_double2(0.0, 0.0);
+        rr += r.x * r.x + r.y * r.y;
+        if (Prec == LFG_PRECOND_IC0) {
+            reinterpret_cast<double2*>(b.p)[node] = make_double2(0.0, 0.0);
+        } else {
+            double z0, z1;
+            precond_node<Prec>(b.m, node, r.x, r.y, z0, z1);
+            reinterpret_cast<double2*>(b.p)[node] = make_double2(z0, z1);
+            rz += r.x * z0 + r.y * z1;
+        }
+    }
+    const double srr = block_sum<kThreads>(rr, sh);
+    const double srz = block_sum<kThreads>(rz, sh);
+    if (threadIdx.x == 0) {
+        b.partials[2 * blockIdx.x] = srr;
+        b.partials[2 * blockIdx.x + 1] = srz;
+    }
+    if (!ticket_last(&b.st->ticket[0], gridDim.x)) return;
+    const double trr = reduce_partials<kThreads>(b.partials, gridDim.x, 2, sh);
+    const double trz = reduce_partials<kThreads>(b.partials + 1, gridDim.x, 2, sh);
+    if (threadIdx.x == 0) {
+        PcgState* st = b.st;
+        st->ticket[0] = 0;
+        const double rnorm0 = sqrt(max0(trr));
+        const double base = stop_rule == LFG_STOP_REFERENCE ? ((1.0 < rnorm0) ? rnorm0 : 1.0) : rnorm0;
+        st->rnorm0 = rnorm0;
+        st->target = tol * base;
+        st->rr = trr;
+        st->rho = trz;
+        st->beta = 0.0;
+        st->alpha = 0.0;
+        st->iter = 0;
+        st->max_iter = max_iter;
+        st->replacements = 0;
+        st->maxr = rnorm0;
+        st->drop = drop;
+        st->need_replace = 0;
+        st->converged = 0;
+        st->status = LFG_OK;
+        st->done = 0;
+        if (rnorm0 <= st->target) {  // solver.cpp:573-578
+            st->converged = 1;
+            st->done = 1;
+        } else if (Prec != LFG_PRECOND_IC0 && (!(trz > 0.0) || !isfinite(trz))) {  // :579-581
+            st->status = LFG_ERR_NONFINITE;
+            st->done = 1;
+        }
+    }
+}
+
+// q = A p (node groups of 2W lanes), fused p.q, alpha = rho / p.q. Each block owns
+// one contiguous node range (XCD-remapped), so partials stay few and the rows a
+// XCD touches, with the p entries they gather, stay in that XCD's L2.
+template <int W, typename VT>
+__global__ __launch_bounds__(kThreads) void pcg_spmv_kernel(PcgBuffers b, const int* __restrict__ rp,
+                                                            const int* __restrict__ col, const VT* __restrict__ val) {
+    if (b.st->done) return;
+    __shared__ double sh[kThreads / kWave];
+    constexpr int kNodesPerStep = kThreads / (2 * W);
+    const int n_nodes = b.n >> 1;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = (n_nodes + gridDim.x - 1) / gridDim.x;
+    const int start = lb * per_block;
+    const int stop = min(start + per_block, n_nodes);
+    const int group = threadIdx.x / (2 * W);
+    const int half = (threadIdx.x / W) & 1;
+    const int lane = threadIdx.x & (W - 1);
+    double pq = 0.0;
+    for (int base = start; base < stop; base += kNodesPerStep) {
+        const int node = base + group;
+        if (node < stop) {
+            const int row = 2 * node + half;
+            const double s = row_dot<W, VT>(rp, col, val, b.p, row, lane);
+            if (lane == 0) {
+                b.q[row] = s;
+                pq += b.p[row] * s;
+            }
+        }
+    }
+    const double spq = block_sum<kThreads>(pq, sh);
+    if (threadIdx.x == 0) b.partials[blockIdx.x] = spq;
+    if (!ticket_last(&b.st->ticket[1], gridDim.x)) return;
+    const double tpq = reduce_partials<kThreads>(b.partials, gridDim.x, 1, sh);
+    if (threadIdx.x == 0) {
+        PcgState* st = b.st;
+        st->ticket[1] = 0;
+        st->pq = tpq;
+        if (!(tpq > 0.0) || !isfinite(tpq)) {  // solver.cpp:591-593
+            st->status = LFG_ERR_NOT_SPD;
+            st->done = 1;
+        } else {
+            st->alpha = st->rho / tpq;
+        }
+    }
+}
+
+// Finishes an iteration after r.z is known (solver.cpp:614-621 checks).
+__device__ __forceinline__ void accept_rz(PcgState* st, double rz) {
+    if (!(rz > 0.0) || !isfinite(rz)) {
+        st->status = LFG_ERR_NONFINITE;
+        st->done = 1;
+        return;
+    }
+    st->beta = rz / st->rho;
+    st->rho = rz;
+    if (st->iter >= st->max_iter) st->done = 1;
+}
+
+// x += alpha p, r -= alpha q (solver.cpp:599-604), ||r||^2, r.z with z = M^-1 r.
+template <int Prec, bool Mixed>
+__global__ __launch_bounds__(kThreads) void pcg_update_kernel(PcgBuffers b) {
+    if (b.st->done) return;
+    __shared__ double sh[kThreads / kWave];
+    const double alpha = b.st->alpha;
+    const int n_nodes = b.n >> 1;
+    double rr = 0.0, rz = 0.0;
+    for (int node = blockIdx.x * kThreads + threadIdx.x; node < n_nodes; node += gridDim.x * kThreads) {
+        const double2 p = reinterpret_cast<const double2*>(b.p)[node];
+        const double2 q = reinterpret_cast<const double2*>(b.q)[node];
+        double2 x = reinterpret_cast<double2*>(b.x)[node];
+        double2 r = reinterpret_cast<double2*>(b.r)[node];
+        x.x += alpha * p.x;
+        x.y += alpha * p.y;
+        r.x = r.x - alpha * q.x;
+        r.y = r.y - alpha * q.y;
+        reinterpret_cast<double2*>(b.x)[node] = x;
+        reinterpret_cast<double2*>(b.r)[node] = r;
+        rr += r.x * r.x + r.y * r.y;
+        if (Prec != LFG_PRECOND_IC0) {
+            double z0, z1;
+            precond_node<Prec>(b.m, node, r.x, r.y, z0, z1);
+            rz += r.x * z0 + r.y * z1;
+        }
+    }
+    const double srr = block_sum<kThreads>(rr, sh);
+    const double srz = block_sum<kThreads>(rz, sh);
+    if (threadIdx.x == 0) {
+        b.partials[2 * blockIdx.x] = srr;
+        b.partials[2 * blockIdx.x + 1] = srz;
+    }
+    if (!ticket_last(&b.st->ticket[2], gridDim.x)) return;
+    const double trr = reduce_partials<kThreads>(b.partials, gridDim.x, 2, sh);
+    const double trz = reduce_partials<kThreads>(b.partials + 1, gridDim.x, 2, sh);
+    if (threadIdx.x == 0) {
+        PcgState* st = b.st;
+        st->ticket[2] = 0;
+        st->iter += 1;
+        st->rr = trr;
+        st->rz_new = trz;
+        const double rnorm = sqrt(max0(trr));
+        if (Mixed) {
+            // Replace the recurrence residual by the FP64 residual when it has
+            // dropped far enough or claims convergence.
+            if (rnorm <= st->target || rnorm < st->drop * st->maxr) {
+                st->need_replace = 1;
+                return;
+            }
+        } else if (rnorm <= st->target) {  // solver.cpp:606-611
+            st->converged = 1;
+            st->done = 1;
+            return;
+        }
+        if (Prec == LFG_PRECOND_IC0) {
+            if (st->iter >= st->max_iter) st->done = 1;  // z and beta come from the IC(0) step
+        } else {
+            accept_rz(st, trz);
+        }
+    }
+}
+
+// Mixed precision: r = b - K x in FP64 when the update kernel asked for it, then
+// ||r||^2, r.z, the convergence test on the replaced residual, and beta.
+template <int W, int Prec>
+__global__ __launch_bounds__(kThreads) void pcg_replace_kernel(PcgBuffers b, const int* __restrict__ rp,
+                                                               const int* __restrict__ col,
+                                                               const double* __restrict__ val) {
+    if (b.st->done || !b.st->need_replace) return;
+    __shared__ double sh[kThreads / kWave];
+    constexpr int kNodesPerBlock = kThreads / (2 * W);
+    const int group = threadIdx.x / (2 * W);
+    const int half = (threadIdx.x / W) & 1;
+    const int lane = threadIdx.x & (W - 1);
+    const int n_nodes = b.n >> 1;
+    double rr = 0.0, rz = 0.0;
+    for (int base = blockIdx.x * kNodesPerBlock; base < n_nodes; base += gridDim.x * kNodesPerBlock) {
+        const int node = base + group;
+        const bool active = node < n_nodes;
+        const int row = 2 * min(node, n_nodes - 1) + half;
+        const double s = row_dot<W, double>(rp, col, val, b.x, row, lane);
+        const double ri = b.b[row] - s;
+        const double r_other = __shfl_xor(ri, W, 2 * W);
+        if (active && lane == 0) {
+            b.r[row] = ri;
+            if (half == 0) {
+                double z0, z1;
+                precond_node<Prec>(b.m, node, ri, r_other, z0, z1);
+                rr += ri * ri + r_other * r_other;
+                rz += ri * z0 + r_other * z1;
+            }
+        }
+    }
+    const double srr = block_sum<kThreads>(rr, sh);
+    const double srz = block_sum<kThreads>(rz, sh);
+    if (threadIdx.x == 0) {
+        b.partials[2 * blockIdx.x] = srr;
+        b.partials[2 * blockIdx.x + 1] = srz;
+    }
+    if (!ticket_last(&b.st->ticket[3], gridDim.x)) return;
+    const double trr = reduce_partials<kThreads>(b.partials, gridDim.x, 2, sh);
+    const double trz = reduce_partials<kThreads>(b.partials + 1, gridDim.x, 2, sh);
+    if (threadIdx.x == 0) {
+        PcgState* st = b.st;
+        st->ticket[3] = 0;
+        st->need_replace = 0;
+        st->replacements += 1;
+        st->rr = trr;
+        const double rnorm = sqrt(max0(trr));
+        st->maxr = rnorm;
+        if (rnorm <= st->target) {
+            st->converged = 1;
+            st->done = 1;
+            return;
+        }
+        accept_rz(st, trz);
+    }
+}
+
+// p = z + beta p (solver.cpp:620).
+template <int Prec>
+__global__ __launch_bounds__(kThreads) void pcg_direction_kernel(PcgBuffers b) {
+    if (b.st->done) return;
+    const double beta = b.st->beta;
+    const int n_nodes = b.n >> 1;
+    for (int node = blockIdx.x * kThreads + threadIdx.x; node < n_nodes; node += gridDim.x * kThreads) {
+        double z0, z1;
+        if (Prec == LFG_PRECOND_IC0) {
+            const double2 z = reinterpret_cast<const double2*>(b.z)[node];
+            z0 = z.x;
+            z1 = z.y;
+        } else {
+            const double2 r = reinterpret_cast<const double2*>(b.r)[node];
+            precond_node<Prec>(b.m, node, r.x, r.y, z0, z1);
+        }
+        double2 p = reinterpret_cast<double2*>(b.p)[node];
+        p.x = z0 + beta * p.x;
+        p.y = z1 + beta * p.y;
+        reinterpret_cast<double2*>(b.p)[node] = p;
+    }
+}
+
+// ------------------------------------------------------------ IC(0)
+
+constexpr int kIcThreads = 1024;
+
+// Level-scheduled sweeps (precond.cpp:386-412) in one workgroup. Forward: rows of
+// a level gather over their strict lower row. Backward: the reference scatters
+// column updates row by row in descending order; here column c gathers the same
+// terms from the transposed factor, in descending row order, and scales by its
+// reciprocal pivot. Per-row arithmetic order equals the reference, so the result
+// is bit-identical.
+__device__ void ic0_sweeps(const Ic0View& f, const double* __restrict__ in, double* __restrict__ z) {
+    for (int i = threadIdx.x; i < f.n; i += kIcThreads) z[i] = in[i];
+    __syncthreads();
+    for (int level = 0; level < f.fwd_levels; ++level) {
+        const int begin = f.fwd_level_ptr[level], end = f.fwd_level_ptr[level + 1];
+        for (int idx = begin + threadIdx.x; idx < end; idx += kIcThreads) {
+            const int row = f.fwd_rows[idx];
+            double sum = z[row];
+            const int last = f.lrow[row + 1] - 1;
+            for (int k = f.lrow[row]; k < last; ++k) sum -= f.lval[k] * z[f.lcol[k]];
+            z[row] = sum * f.inv_diag[row];
+        }
+        __syncthreads();
+    }
+    for (int level = 0; level < f.bwd_levels; ++level) {
+        const int begin = f.bwd_level_ptr[level], end = f.bwd_level_ptr[level + 1];
+        for (int idx = begin + threadIdx.x; idx < end; idx += kIcThreads) {
+            const int c = f.bwd_rows[idx];
+            double sum = z[c];
+            for (int k = f.urow[c]; k < f.urow[c + 1]; ++k) sum -= f.uval[k] * z[f.ucol[k]];
+            z[c] = sum * f.inv_diag[c];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kIcThreads) void ic0_apply_kernel(Ic0View f, const double* __restrict__ in,
+                                                               double* __restrict__ out) {
+    ic0_sweeps(f, in, out);
+}
+
+// z = IC0(r), r.z, then rho/beta (first call: rho = r.z, beta = 0).
+__global__ __launch_bounds__(kIcThreads) void pcg_ic0_step_kernel(PcgBuffers b, Ic0View f) {
+    if (b.st->done) return;
+    __shared__ double sh[kIcThreads / kWave];
+    ic0_sweeps(f, b.r, b.z);
+    double rz = 0.0;
+    for (int i = threadIdx.x; i < f.n; i += kIcThreads) rz += b.r[i] * b.z[i];
+    const double trz = block_sum<kIcThreads>(rz, sh);
+    if (threadIdx.x == 0) {
+        PcgState* st = b.st;
+        if (st->iter == 0) {
+            if (!(trz > 0.0) || !isfinite(trz)) {  // solver.cpp:579-581
+                st->status = LFG_ERR_NONFINITE;
+                st->done = 1;
+            } else {
+                st->rho = trz;
+                st->beta = 0.0;
+            }
+        } else if (!(trz > 0.0) || !isfinite(trz)) {  // :616-618
+            st->status = LFG_ERR_NONFINITE;
+            st->done = 1;
+        } else {
+            st->beta = trz / st->rho;
+            st->rho = trz;
+        }
+    }
+}
+
+// ||b - A x||^2 into out2; blocks own contiguous row ranges, the last block folds
+// the partials.
+template <int W>
+__global__ __launch_bounds__(kThreads) void residual_norm_kernel(int n, const int* __restrict__ rp,
+                                                                 const int* __restrict__ col,
+                                                                 const double* __restrict__ val,
+                                                                 const double* __restrict__ x,
+                                                                 const double* __restrict__ bvec,
+                                                                 double* __restrict__ partials,
+                                                                 unsigned int* ticket, double* out2) {
+    __shared__ double sh[kThreads / kWave];
+    constexpr int kRows = kThreads / W;
+    const int per_block = (n + gridDim.x - 1) / gridDim.x;
+    const int start = blockIdx.x * per_block;
+    const int stop = min(start + per_block, n);
+    const int lane = threadIdx.x & (W - 1);
+    double acc = 0.0;
+    for (int base = start; base < stop; base += kRows) {
+        const int row = base + threadIdx.x / W;
+        if (row < stop) {
+            const double s = row_dot<W, double>(rp, col, val, x, row, lane);
+            if (lane == 0) {
+                const double ri = bvec[row] - s;
+                acc += ri * ri;
+            }
+        }
+    }
+    const double sacc = block_sum<kThreads>(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = sacc;
+    if (!ticket_last(ticket, gridDim.x)) return;
+    const double total = reduce_partials<kThreads>(partials, gridDim.x, 1, sh);
+    if (threadIdx.x == 0) {
+        *ticket = 0;
+        *out2 = total;
+    }
+}
+
+// Node-group width per SpMV variant: 1 -> 4 lanes per row, 2 -> 8, 3 -> 16.
+inline int variant_width(int variant) {
+    switch (variant) {
+        case 1: return 4;
+        case 3: return 16;
+        default: return 8;
+    }
+}
+
+inline unsigned int update_blocks(int n) {
+    const unsigned int b = blocks_for(n >> 1, kThreads);
+    return round8(b < 4096u ? b : 4096u);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+
+void launch_dirichlet(hipStream_t s, const CsrView& a, const double* values, const double* f, const int* constrained,
+                      const double* prescribed, double* kc, double* fc) {
+    if (a.n <= 0) return;
+    dirichlet_kernel<<<blocks_for(8LL * a.n, 256), 256, 0, s>>>(a.n, a.row_ptr, a.col, values, f, constrained,
+                                                                 prescribed, kc, fc);
+}
+
+void launch_spmv_serial_rows(hipStream_t s, const CsrView& a, const double* values, const double* x,
+                             const double* sub, double* y) {
+    if (a.n <= 0) return;
+    spmv_serial_rows_kernel<<<blocks_for(a.n, 256), 256, 0, s>>>(a.n, a.row_ptr, a.col, values, x, sub, y);
+}
+
+void launch_extract_diag(hipStream_t s, const CsrView& a, const double* values, double* diag, int* status) {
+    extract_diag_kernel<<<blocks_for(a.n, 256), 256, 0, s>>>(a.n, a.row_ptr, a.col, values, diag, status);
+}
+
+void launch_block2_factor(hipStream_t s, const CsrView& a, const double* values, double* factor, int* degenerate) {
+    block2_factor_kernel<<<blocks_for(a.n / 2, 256), 256, 0, s>>>(a.n / 2, a.row_ptr, a.col, values, factor,
+                                                                   degenerate);
+}
+
+void launch_jacobi_inv(hipStream_t s, int n, const double* diag, double* inv, int* status) {
+    jacobi_inv_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, diag, inv, status);
+}
+
+void launch_precond_apply(hipStream_t s, int kind, int n, const double* m, const double* in, double* out) {
+    precond_apply_kernel<<<blocks_for(n / 2, 256), 256, 0, s>>>(kind, n / 2, m, in, out);
+}
+
+void launch_to_f32(hipStream_t s, const double* in, float* out, long long n) {
+    if (n <= 0) return;
+    to_f32_kernel<<<blocks_for(n, 256), 256, 0, s>>>(in, out, n);
+}
+
+// Fixed SpMV grid: at most kSpmvBlocks blocks, each looping over its node range.
+constexpr unsigned int kSpmvBlocks = 2048;
+
+int spmv_blocks(int n, int variant) {
+    const int w = variant_width(variant);
+    const int nodes_per_step = kThreads / (2 * w);
+    const unsigned int need = blocks_for(n >> 1, nodes_per_step);
+    return static_cast<int>(round8(need < kSpmvBlocks ? need : kSpmvBlocks));
+}
+
+void launch_pcg_init(hipStream_t s, const PcgBuffers& b, int stop_rule, double tol, long long max_iter, double drop) {
+    const unsigned int grid = update_blocks(b.n);
+    switch (b.precond) {
+        case LFG_PRECOND_JACOBI:
+            pcg_init_kernel<LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop);
+            break;
+        case LFG_PRECOND_BLOCK2:
+            pcg_init_kernel<LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop);
+            break;
+        default:
+            pcg_init_kernel<LFG_PRECOND_IC0><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop);
+            break;
+    }
+}
+
+template <int W>
+static void spmv_w(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* v64, const float* v32) {
+    const unsigned int grid = static_cast<unsigned int>(spmv_blocks(b.n, W == 4 ? 1 : (W == 16 ? 3 : 2)));
+    if (v32) pcg_spmv_kernel<W, float><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, v32);
+    else pcg_spmv_kernel<W, double><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, v64);
+}
+
+void launch_pcg_spmv(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* vals64, const float* vals32,
+                     int variant) {
+    switch (variant_width(variant)) {
+        case 4: spmv_w<4>(s, b, a, vals64, vals32); break;
+        case 16: spmv_w<16>(s, b, a, vals64, vals32); break;
+        default: spmv_w<8>(s, b, a, vals64, vals32); break;
+    }
+}
+
+void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed) {
+    const unsigned int grid = update_blocks(b.n);
+    if (mixed) {
+        if (b.precond == LFG_PRECOND_BLOCK2) pcg_update_kernel<LFG_PRECOND_BLOCK2, true><<<grid, kThreads, 0, s>>>(b);
+        else pcg_update_kernel<LFG_PRECOND_JACOBI, true><<<grid, kThreads, 0, s>>>(b);
+        return;
+    }
+    switch (b.precond) {
+        case LFG_PRECOND_JACOBI: pcg_update_kernel<LFG_PRECOND_JACOBI, false><<<grid, kThreads, 0, s>>>(b); break;
+        case LFG_PRECOND_BLOCK2: pcg_update_kernel<LFG_PRECOND_BLOCK2, false><<<grid, kThreads, 0, s>>>(b); break;
+        default: pcg_update_kernel<LFG_PRECOND_IC0, false><<<grid, kThreads, 0, s>>>(b); break;
+    }
+}
+
+void launch_pcg_replace(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* vals64) {
+    constexpr int W = 8;
+    const unsigned int grid = 2048;
+    if (b.precond == LFG_PRECOND_BLOCK2)
+        pcg_replace_kernel<W, LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, vals64);
+    else
+        pcg_replace_kernel<W, LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, vals64);
+}
+
+void launch_pcg_direction(hipStream_t s, const PcgBuffers& b) {
+    const unsigned int grid = update_blocks(b.n);
+    switch (b.precond) {
+        case LFG_PRECOND_JACOBI: pcg_direction_kernel<LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(b); break;
+        case LFG_PRECOND_BLOCK2: pcg_direction_kernel<LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(b); break;
+        default: pcg_direction_kernel<LFG_PRECOND_IC0><<<grid, kThreads, 0, s>>>(b); break;
+    }
+}
+
+void launch_ic0_apply(hipStream_t s, const Ic0View& f, const double* in, double* out) {
+    ic0_apply_kernel<<<1, kIcThreads, 0, s>>>(f, in, out);
+}
+
+void launch_pcg_ic0_step(hipStream_t s, const PcgBuffers& b, const Ic0View& f) {
+    pcg_ic0_step_kernel<<<1, kIcThreads, 0, s>>>(b, f);
+}
+
+void launch_residual_norm(hipStream_t s, const CsrView& a, const double* vals64, const double* x, const double* bvec,
+                          double* work, double* out2) {
+    constexpr int W = 8;
+    const unsigned int need = blocks_for(a.n, kThreads / W);
+    const unsigned int grid = need < 2048u ? need : 2048u;
+    // work: partials in [0, 2048), the ticket word at double slot 4096 (zeroed once,
+    // reset by the last block).
+    unsigned int* ticket = reinterpret_cast<unsigned int*>(work + 4096);
+    residual_norm_kernel<W><<<grid, kThreads, 0, s>>>(a.n, a.row_ptr, a.col, vals64, x, bvec, work, ticket, out2);
+}
+
+}  // namespace lfg

@@ -1,0 +1,243 @@
+// Host stages that produce the integer structures of the path: CSR pattern and
+// element->value map, RCM ordering, element colouring and the CSR profile.
+//
+// Each result is identical to the reference stage it replaces. The pattern is
+// built from node adjacency instead of one hash set per dof row: both dofs of a
+// node couple to exactly the dofs of the nodes sharing an element with it, so a
+// sorted, unique neighbour-node list per node yields every row in ascending
+// column order, and a map entry is a short binary search in that list. This is
+// O(nnz) with OpenMP over nodes, where the reference's unordered_set build takes
+// seconds at a million elements.
+#include "lightfem_host.h"
+
+#include <algorithm>
+#include <cstdint>
+#include <queue>
+#include <vector>
+
+namespace {
+
+struct NodeElements {
+    std::vector<int> ptr;
+    std::vector<int> elems;  // ascending element index per node
+};
+
+NodeElements node_elements(int64_t n_nodes, int64_t n_elems, const int32_t* enodes) {
+    NodeElements ne;
+    ne.ptr.assign(static_cast<size_t>(n_nodes) + 1, 0);
+    for (int64_t k = 0; k < 4 * n_elems; ++k) ne.ptr[static_cast<size_t>(enodes[k]) + 1]++;
+    for (int64_t i = 0; i < n_nodes; ++i) ne.ptr[i + 1] += ne.ptr[i];
+    ne.elems.resize(static_cast<size_t>(4 * n_elems));
+    std::vector<int> cur(ne.ptr.begin(), ne.ptr.end() - 1);
+    for (int64_t e = 0; e < n_elems; ++e)
+        for (int l = 0; l < 4; ++l) ne.elems[cur[enodes[4 * e + l]]++] = static_cast<int>(e);
+    return ne;
+}
+
+bool valid_elements(int64_t n_nodes, int64_t n_elems, const int32_t* enodes) {
+    if (n_nodes <= 0 || n_elems < 0 || (n_elems > 0 && !enodes)) return false;
+    for (int64_t k = 0; k < 4 * n_elems; ++k)
+        if (enodes[k] < 0 || enodes[k] >= n_nodes) return false;
+    return true;
+}
+
+// Sorted unique neighbour nodes (including the node itself when it has elements).
+void neighbour_lists(int64_t n_nodes, const int32_t* enodes, const NodeElements& ne, std::vector<int>& ptr,
+                     std::vector<int>& list) {
+    ptr.assign(static_cast<size_t>(n_nodes) + 1, 0);
+    std::vector<std::vector<int>> tmp(static_cast<size_t>(n_nodes));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        std::vector<int>& v = tmp[static_cast<size_t>(i)];
+        for (int j = ne.ptr[i]; j < ne.ptr[i + 1]; ++j) {
+            const int e = ne.elems[j];
+            for (int l = 0; l < 4; ++l) v.push_back(enodes[4 * e + l]);
+        }
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    for (int64_t i = 0; i < n_nodes; ++i) ptr[i + 1] = ptr[i] + static_cast<int>(tmp[static_cast<size_t>(i)].size());
+    list.resize(static_cast<size_t>(ptr[n_nodes]));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n_nodes; ++i)
+        std::copy(tmp[static_cast<size_t>(i)].begin(), tmp[static_cast<size_t>(i)].end(), list.begin() + ptr[i]);
+}
+
+int64_t row_length(const std::vector<int>& nptr, int64_t node) {
+    const int64_t m = nptr[node + 1] - nptr[node];
+    return m == 0 ? 1 : 2 * m;  // an unused node keeps only its diagonal
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t lfh_csr_pattern_nnz(int64_t n_nodes, int64_t n_elems, const int32_t* enodes) {
+    if (!valid_elements(n_nodes, n_elems, enodes)) return -1;
+    const NodeElements ne = node_elements(n_nodes, n_elems, enodes);
+    std::vector<int> nptr, nlist;
+    neighbour_lists(n_nodes, enodes, ne, nptr, nlist);
+    int64_t nnz = 0;
+    for (int64_t i = 0; i < n_nodes; ++i) nnz += 2 * row_length(nptr, i);
+    return nnz;
+}
+
+int lfh_csr_pattern(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int32_t* row_ptr, int32_t* col,
+                    int32_t* emap) {
+    if (!valid_elements(n_nodes, n_elems, enodes) || !row_ptr || !col || (n_elems > 0 && !emap)) return 1;
+    const NodeElements ne = node_elements(n_nodes, n_elems, enodes);
+    std::vector<int> nptr, nlist;
+    neighbour_lists(n_nodes, enodes, ne, nptr, nlist);
+    int64_t total = 0;
+    row_ptr[0] = 0;
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        const int64_t len = row_length(nptr, i);
+        if (total + 2 * len > INT32_MAX) return 2;
+        row_ptr[2 * i + 1] = static_cast<int32_t>(total + len);
+        row_ptr[2 * i + 2] = static_cast<int32_t>(total + 2 * len);
+        total += 2 * len;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        for (int comp = 0; comp < 2; ++comp) {
+            int32_t* out = col + row_ptr[2 * i + comp];
+            if (nptr[i + 1] == nptr[i]) {
+                out[0] = static_cast<int32_t>(2 * i + comp);
+                continue;
+            }
+            for (int k = nptr[i]; k < nptr[i + 1]; ++k) {
+                *out++ = 2 * nlist[k];
+                *out++ = 2 * nlist[k] + 1;
+            }
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < n_elems; ++e) {
+        for (int r = 0; r < 8; ++r) {
+            const int a = enodes[4 * e + r / 2];
+            const int* begin = nlist.data() + nptr[a];
+            const int* end = nlist.data() + nptr[a + 1];
+            const int32_t row_start = row_ptr[2 * a + (r & 1)];
+            for (int c = 0; c < 8; ++c) {
+                const int b = enodes[4 * e + c / 2];
+                const int pos = static_cast<int>(std::lower_bound(begin, end, b) - begin);
+                emap[64 * e + 8 * r + c] = row_start + 2 * pos + (c & 1);
+            }
+        }
+    }
+    return 0;
+}
+
+int lfh_rcm(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int32_t* enode_ids, int32_t* order_out) {
+    if (n_nodes <= 0 || !node_ids || !order_out || (n_elems > 0 && !enode_ids)) return 1;
+    // node id -> index in node_ids
+    std::vector<int> by_id(static_cast<size_t>(n_nodes));
+    for (int64_t i = 0; i < n_nodes; ++i) by_id[static_cast<size_t>(i)] = static_cast<int>(i);
+    std::sort(by_id.begin(), by_id.end(), [&](int a, int b) { return node_ids[a] < node_ids[b]; });
+    std::vector<int> sorted_ids(static_cast<size_t>(n_nodes));
+    for (int64_t i = 0; i < n_nodes; ++i) sorted_ids[static_cast<size_t>(i)] = node_ids[by_id[static_cast<size_t>(i)]];
+    auto index_of = [&](int id) -> int {
+        const auto it = std::lower_bound(sorted_ids.begin(), sorted_ids.end(), id);
+        if (it == sorted_ids.end() || *it != id) return -1;
+        return by_id[static_cast<size_t>(it - sorted_ids.begin())];
+    };
+    std::vector<int32_t> en(static_cast<size_t>(4 * n_elems));
+    for (int64_t k = 0; k < 4 * n_elems; ++k) {
+        en[static_cast<size_t>(k)] = index_of(enode_ids[k]);
+        if (en[static_cast<size_t>(k)] < 0) return 2;
+    }
+    // distinct neighbours over the 6 node pairs of each Q4 (reordering.cpp:17-24)
+    const NodeElements ne = node_elements(n_nodes, n_elems, en.data());
+    std::vector<int> nptr, nlist;
+    neighbour_lists(n_nodes, en.data(), ne, nptr, nlist);
+    std::vector<int> degree(static_cast<size_t>(n_nodes));
+    std::vector<int> adj_ptr(static_cast<size_t>(n_nodes) + 1, 0), adj;
+    adj.reserve(nlist.size());
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        for (int k = nptr[i]; k < nptr[i + 1]; ++k)
+            if (nlist[k] != i) adj.push_back(nlist[k]);
+        adj_ptr[i + 1] = static_cast<int>(adj.size());
+        degree[static_cast<size_t>(i)] = adj_ptr[i + 1] - adj_ptr[i];
+    }
+    const auto key_less = [&](int a, int b) {  // (degree, id), reordering.cpp:274-278
+        if (degree[a] != degree[b]) return degree[a] < degree[b];
+        return node_ids[a] < node_ids[b];
+    };
+    std::vector<int> seeds(static_cast<size_t>(n_nodes));
+    for (int64_t i = 0; i < n_nodes; ++i) seeds[static_cast<size_t>(i)] = static_cast<int>(i);
+    std::sort(seeds.begin(), seeds.end(), key_less);
+    std::vector<char> visited(static_cast<size_t>(n_nodes), 0);
+    std::vector<int> queue(static_cast<size_t>(n_nodes)), neigh;
+    int64_t out = 0;
+    for (const int seed : seeds) {
+        if (visited[seed]) continue;
+        visited[seed] = 1;
+        size_t head = 0, tail = 0;
+        queue[tail++] = seed;
+        while (head < tail) {
+            const int current = queue[head++];
+            order_out[out++] = node_ids[current];
+            neigh.assign(adj.begin() + adj_ptr[current], adj.begin() + adj_ptr[current + 1]);
+            std::sort(neigh.begin(), neigh.end(), key_less);
+            for (const int nb : neigh)
+                if (!visited[nb]) {
+                    visited[nb] = 1;
+                    queue[tail++] = nb;
+                }
+        }
+    }
+    std::reverse(order_out, order_out + out);
+    return 0;
+}
+
+int lfh_element_colors(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int32_t* color_out,
+                       int32_t* n_colors_out, int32_t* color_ptr, int32_t* color_elems) {
+    if (!valid_elements(n_nodes, n_elems, enodes) || !color_out || !n_colors_out) return 1;
+    const NodeElements ne = node_elements(n_nodes, n_elems, enodes);
+    std::vector<int64_t> stamp_of(static_cast<size_t>(n_elems) + 1, 0);
+    int max_color = -1;
+    for (int64_t e = 0; e < n_elems; ++e) color_out[e] = -1;
+    for (int64_t e = 0; e < n_elems; ++e) {
+        const int64_t stamp = e + 1;
+        for (int l = 0; l < 4; ++l) {
+            const int node = enodes[4 * e + l];
+            for (int j = ne.ptr[node]; j < ne.ptr[node + 1]; ++j) {
+                const int other = ne.elems[j];
+                if (color_out[other] >= 0) stamp_of[static_cast<size_t>(color_out[other])] = stamp;
+            }
+        }
+        int color = 0;
+        while (stamp_of[static_cast<size_t>(color)] == stamp) ++color;
+        color_out[e] = color;
+        max_color = std::max(max_color, color);
+    }
+    *n_colors_out = max_color + 1;
+    if (color_ptr && color_elems) {
+        std::fill(color_ptr, color_ptr + max_color + 2, 0);
+        for (int64_t e = 0; e < n_elems; ++e) color_ptr[color_out[e] + 1]++;
+        for (int c = 0; c <= max_color; ++c) color_ptr[c + 1] += color_ptr[c];
+        std::vector<int32_t> cur(color_ptr, color_ptr + max_color + 1);
+        for (int64_t e = 0; e < n_elems; ++e) color_elems[cur[color_out[e]]++] = static_cast<int32_t>(e);
+    }
+    return 0;
+}
+
+int lfh_csr_profile(int64_t n_rows, const int32_t* row_ptr, const int32_t* col, int64_t out[3]) {
+    if (n_rows < 0 || !row_ptr || !col || !out) return 1;
+    int64_t bandwidth = 0, profile = 0;
+    for (int64_t row = 0; row < n_rows; ++row) {
+        int64_t minimum = row;
+        for (int32_t k = row_ptr[row]; k < row_ptr[row + 1]; ++k) {
+            const int64_t c = col[k];
+            bandwidth = std::max(bandwidth, row > c ? row - c : c - row);
+            if (c <= row) minimum = std::min(minimum, c);
+        }
+        profile += row - minimum;
+    }
+    out[0] = row_ptr[n_rows];
+    out[1] = bandwidth;
+    out[2] = profile;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,320 @@
+"""ctypes binding of the two C ABIs: ``include/lightfem_gpu.h`` (the gfx950
+backend, liblightfem_gpu.so) and ``include/lightfem_host.h`` (host stages,
+liblightfem.so). Both libraries are built in-tree by ``lightfem_amd.build``;
+loading fails loudly when they are missing - there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_PKG, "lib")
+GPU_LIB_PATH = os.path.join(LIB_DIR, "liblightfem_gpu.so")
+HOST_LIB_PATH = os.path.join(LIB_DIR, "liblightfem.so")
+
+OK, ERR_ARG, ERR_JACOBIAN, ERR_NOT_SPD, ERR_NONFINITE, ERR_NOT_CONVERGED, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED = range(9)
+PLANE_STRESS, PLANE_STRAIN = 0, 1
+ASSEMBLY = {"serial": 0, "atomic": 1, "coloring": 2, "thread_local": 3}
+PRECOND = {"jacobi": 0, "block": 1, "block2": 1, "ic0": 2}
+PRECISION = {"fp64": 0, "mixed": 1}
+STOP = {"reference": 0, "relative": 1}
+
+GPU_SYMBOLS = ["lfg_create", "lfg_destroy", "lfg_last_error", "lfg_status_string", "lfg_set_mesh", "lfg_set_pattern",
+               "lfg_set_colors", "lfg_element_stiffness", "lfg_assemble", "lfg_get_stiffness", "lfg_set_stiffness",
+               "lfg_set_load", "lfg_dirichlet", "lfg_get_system", "lfg_pcg", "lfg_reactions", "lfg_postprocess",
+               "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv"]
+HOST_SYMBOLS = ["lfh_csr_pattern_nnz", "lfh_csr_pattern", "lfh_rcm", "lfh_element_colors", "lfh_csr_profile"]
+
+
+class LightFemError(RuntimeError):
+    """A failed stage; ``code`` is the lfg_status, the message the reference's text."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class PcgOptions(ctypes.Structure):
+    _fields_ = [("preconditioner", ctypes.c_int32), ("precision", ctypes.c_int32), ("stop_rule", ctypes.c_int32),
+                ("check_every", ctypes.c_int32), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int64),
+                ("replace_drop", ctypes.c_double), ("direct_launch", ctypes.c_int32),
+                ("spmv_variant", ctypes.c_int32)]
+
+
+class PcgReport(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int32), ("status", ctypes.c_int32), ("iterations", ctypes.c_int64),
+                ("residual_replacements", ctypes.c_int64), ("initial_residual", ctypes.c_double),
+                ("final_residual", ctypes.c_double), ("true_residual", ctypes.c_double),
+                ("solve_ms", ctypes.c_double), ("setup_ms", ctypes.c_double), ("breakdowns", ctypes.c_int64),
+                ("degenerate_blocks", ctypes.c_int64), ("levels", ctypes.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class AssemblyReport(ctypes.Structure):
+    _fields_ = [("element_stiffness_ms", ctypes.c_double), ("assembly_ms", ctypes.c_double),
+                ("coloring_count", ctypes.c_int32), ("jacobian_failures", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_gpu: Optional[ctypes.CDLL] = None
+_host: Optional[ctypes.CDLL] = None
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+
+
+def _load(path: str) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise RuntimeError(f"LightFEM native library is not built: {path} (run python -m lightfem_amd.build)")
+    return ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
+def gpu_lib() -> ctypes.CDLL:
+    global _gpu
+    if _gpu is None:
+        lib = _load(GPU_LIB_PATH)
+        lib.lfg_last_error.restype = ctypes.c_char_p
+        lib.lfg_last_error.argtypes = [_P]
+        lib.lfg_status_string.restype = ctypes.c_char_p
+        lib.lfg_create.argtypes = [ctypes.c_int32, ctypes.POINTER(_P)]
+        lib.lfg_destroy.argtypes = [_P]
+        lib.lfg_destroy.restype = None
+        lib.lfg_set_mesh.argtypes = [_P, _I64, _P, _I64, _P, _P, _I64, _P, ctypes.c_int32]
+        lib.lfg_set_pattern.argtypes = [_P, _I64, _I64, _P, _P, _P]
+        lib.lfg_set_colors.argtypes = [_P, ctypes.c_int32, _P, _P]
+        lib.lfg_element_stiffness.argtypes = [_P, _P]
+        lib.lfg_assemble.argtypes = [_P, ctypes.c_int32, ctypes.POINTER(AssemblyReport)]
+        lib.lfg_get_stiffness.argtypes = [_P, _P]
+        lib.lfg_set_stiffness.argtypes = [_P, _P]
+        lib.lfg_set_load.argtypes = [_P, _P]
+        lib.lfg_dirichlet.argtypes = [_P, _I64, _P, _P]
+        lib.lfg_get_system.argtypes = [_P, _P, _P]
+        lib.lfg_pcg.argtypes = [_P, ctypes.POINTER(PcgOptions), _P, ctypes.POINTER(PcgReport)]
+        lib.lfg_reactions.argtypes = [_P, _P]
+        lib.lfg_postprocess.argtypes = [_P, _P, _P, _P]
+        lib.lfg_precond_apply.argtypes = [_P, ctypes.c_int32, _P, _P]
+        lib.lfg_spmv.argtypes = [_P, ctypes.c_int32, _P, _P]
+        lib.lfg_bench_spmv.argtypes = [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_double)]
+        _gpu = lib
+    return _gpu
+
+
+def host_lib() -> ctypes.CDLL:
+    global _host
+    if _host is None:
+        gpu_lib()  # liblightfem.so links against it
+        lib = _load(HOST_LIB_PATH)
+        lib.lfh_csr_pattern_nnz.restype = ctypes.c_int64
+        lib.lfh_csr_pattern_nnz.argtypes = [_I64, _I64, _P]
+        lib.lfh_csr_pattern.argtypes = [_I64, _I64, _P, _P, _P, _P]
+        lib.lfh_rcm.argtypes = [_I64, _P, _I64, _P, _P]
+        lib.lfh_element_colors.argtypes = [_I64, _I64, _P, _P, _P, _P, _P]
+        lib.lfh_csr_profile.argtypes = [_I64, _P, _P, _P]
+        _host = lib
+    return _host
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------- host stages
+
+def csr_pattern(n_nodes: int, enodes: np.ndarray):
+    """(row_ptr, col, emap) of build_csr_assembly_pattern for Q4 node-index lists."""
+    enodes = _i32(enodes).reshape(-1, 4)
+    lib = host_lib()
+    nnz = lib.lfh_csr_pattern_nnz(n_nodes, enodes.shape[0], _ptr(enodes))
+    if nnz < 0:
+        raise LightFemError(ERR_ARG, "element DOF is outside CSR matrix dimension")
+    row_ptr = np.empty(2 * n_nodes + 1, dtype=np.int32)
+    col = np.empty(nnz, dtype=np.int32)
+    emap = np.empty((enodes.shape[0], 64), dtype=np.int32)
+    rc = lib.lfh_csr_pattern(n_nodes, enodes.shape[0], _ptr(enodes), _ptr(row_ptr), _ptr(col), _ptr(emap))
+    if rc:
+        raise LightFemError(ERR_ARG, "CSR pattern exceeds the int32 index range")
+    return row_ptr, col, emap
+
+
+def rcm(node_ids: np.ndarray, enode_ids: np.ndarray) -> np.ndarray:
+    node_ids = _i32(node_ids)
+    enode_ids = _i32(enode_ids).reshape(-1, 4)
+    out = np.empty(node_ids.shape[0], dtype=np.int32)
+    if host_lib().lfh_rcm(node_ids.shape[0], _ptr(node_ids), enode_ids.shape[0], _ptr(enode_ids), _ptr(out)):
+        raise LightFemError(ERR_ARG, "RCM: element references an unknown node")
+    return out
+
+
+def element_colors(n_nodes: int, enodes: np.ndarray):
+    """(color per element, color_ptr, color_elems)."""
+    enodes = _i32(enodes).reshape(-1, 4)
+    ne = enodes.shape[0]
+    colors = np.empty(ne, dtype=np.int32)
+    count = ctypes.c_int32(0)
+    ptr = np.empty(ne + 2, dtype=np.int32)
+    elems = np.empty(ne, dtype=np.int32)
+    if host_lib().lfh_element_colors(n_nodes, ne, _ptr(enodes), _ptr(colors), ctypes.byref(count), _ptr(ptr),
+                                     _ptr(elems)):
+        raise LightFemError(ERR_ARG, "colouring: element references an unknown node")
+    return colors, ptr[:count.value + 1].copy(), elems
+
+
+def csr_profile(row_ptr: np.ndarray, col: np.ndarray) -> dict:
+    row_ptr, col = _i32(row_ptr), _i32(col)
+    out = np.empty(3, dtype=np.int64)
+    host_lib().lfh_csr_profile(row_ptr.shape[0] - 1, _ptr(row_ptr), _ptr(col), _ptr(out))
+    return {"nonzeros": int(out[0]), "bandwidth": int(out[1]), "profile": int(out[2])}
+
+
+# ------------------------------------------------------------- device stages
+
+class Device:
+    """One lfg context: device buffers and one HIP stream on GPU `device`."""
+
+    def __init__(self, device: int = 0):
+        self._lib = gpu_lib()
+        handle = _P()
+        rc = self._lib.lfg_create(device, ctypes.byref(handle))
+        if rc != OK:
+            raise LightFemError(rc, f"cannot create a HIP context on device {device}")
+        self._ctx = handle
+        self._keep: dict = {}
+        self.n = 0
+        self.nnz = 0
+        self.n_nodes = 0
+        self.n_elems = 0
+
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            self._lib.lfg_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int) -> int:
+        if rc not in (OK,):
+            message = self._lib.lfg_last_error(self._ctx).decode()
+            raise LightFemError(rc, message)
+        return rc
+
+    def set_mesh(self, xy, enodes, elem_material, materials, plane_mode: int) -> None:
+        xy = _f64(xy).reshape(-1, 2)
+        enodes = _i32(enodes).reshape(-1, 4)
+        elem_material = _i32(elem_material)
+        materials = _f64(materials).reshape(-1, 3)
+        self._check(self._lib.lfg_set_mesh(self._ctx, xy.shape[0], _ptr(xy), enodes.shape[0], _ptr(enodes),
+                                           _ptr(elem_material), materials.shape[0], _ptr(materials), plane_mode))
+        self.n_nodes = xy.shape[0]
+        self.n_elems = enodes.shape[0]
+
+    def set_pattern(self, row_ptr, col, emap) -> None:
+        row_ptr, col, emap = _i32(row_ptr), _i32(col), _i32(emap)
+        self._check(self._lib.lfg_set_pattern(self._ctx, row_ptr.shape[0] - 1, col.shape[0], _ptr(row_ptr),
+                                              _ptr(col), _ptr(emap)))
+        self.n = row_ptr.shape[0] - 1
+        self.nnz = col.shape[0]
+
+    def set_colors(self, color_ptr, color_elems) -> None:
+        color_ptr, color_elems = _i32(color_ptr), _i32(color_elems)
+        self._check(self._lib.lfg_set_colors(self._ctx, color_ptr.shape[0] - 1, _ptr(color_ptr), _ptr(color_elems)))
+
+    def element_stiffness(self) -> np.ndarray:
+        out = np.empty((self.n_elems, 64))
+        self._check(self._lib.lfg_element_stiffness(self._ctx, _ptr(out)))
+        return out
+
+    def assemble(self, strategy: str = "serial") -> dict:
+        rep = AssemblyReport()
+        self._check(self._lib.lfg_assemble(self._ctx, ASSEMBLY[strategy], ctypes.byref(rep)))
+        return rep.as_dict()
+
+    def stiffness(self) -> np.ndarray:
+        out = np.empty(self.nnz)
+        self._check(self._lib.lfg_get_stiffness(self._ctx, _ptr(out)))
+        return out
+
+    def set_stiffness(self, values) -> None:
+        values = _f64(values)
+        self._check(self._lib.lfg_set_stiffness(self._ctx, _ptr(values)))
+
+    def set_load(self, f) -> None:
+        f = _f64(f)
+        self._check(self._lib.lfg_set_load(self._ctx, _ptr(f)))
+
+    def dirichlet(self, dofs, values) -> None:
+        dofs, values = _i32(dofs), _f64(values)
+        self._check(self._lib.lfg_dirichlet(self._ctx, dofs.shape[0], _ptr(dofs), _ptr(values)))
+
+    def system(self):
+        kc = np.empty(self.nnz)
+        fc = np.empty(self.n)
+        self._check(self._lib.lfg_get_system(self._ctx, _ptr(kc), _ptr(fc)))
+        return kc, fc
+
+    def pcg(self, preconditioner: str = "jacobi", precision: str = "fp64", stop_rule: str = "reference",
+            tolerance: float = 1e-10, max_iterations: int = 100000, check_every: int = 0,
+            replace_drop: float = 0.0, direct_launch: bool = False, spmv_variant: int = 0,
+            raise_on_error: bool = True):
+        opt = PcgOptions(PRECOND[preconditioner], PRECISION[precision], STOP[stop_rule], check_every, tolerance,
+                         max_iterations, replace_drop, int(direct_launch), spmv_variant)
+        rep = PcgReport()
+        u = np.empty(self.n)
+        rc = self._lib.lfg_pcg(self._ctx, ctypes.byref(opt), _ptr(u), ctypes.byref(rep))
+        if rc != OK and raise_on_error:
+            self._check(rc)
+        return u, rep.as_dict()
+
+    def reactions(self) -> np.ndarray:
+        out = np.empty(self.n)
+        self._check(self._lib.lfg_reactions(self._ctx, _ptr(out)))
+        return out
+
+    def postprocess(self):
+        centre = np.empty((self.n_elems, 7))
+        gp = np.empty((self.n_elems, 4, 9))
+        nodal = np.empty((self.n_nodes, 4))
+        self._check(self._lib.lfg_postprocess(self._ctx, _ptr(centre), _ptr(gp), _ptr(nodal)))
+        return centre, gp, nodal
+
+    def precond_apply(self, preconditioner: str, vec) -> np.ndarray:
+        vec = _f64(vec)
+        out = np.empty_like(vec)
+        self._check(self._lib.lfg_precond_apply(self._ctx, PRECOND[preconditioner], _ptr(vec), _ptr(out)))
+        return out
+
+    def spmv(self, x, precision: str = "fp64") -> np.ndarray:
+        x = _f64(x)
+        y = np.empty_like(x)
+        self._check(self._lib.lfg_spmv(self._ctx, PRECISION[precision], _ptr(x), _ptr(y)))
+        return y
+
+    def bench_spmv(self, precision: str = "fp64", variant: int = 0, reps: int = 20) -> float:
+        ms = ctypes.c_double(0.0)
+        self._check(self._lib.lfg_bench_spmv(self._ctx, PRECISION[precision], variant, reps, ctypes.byref(ms)))
+        return ms.value

@@ -1,0 +1,242 @@
+"""GPU parity: every device stage against the reference's golden vectors and the
+CPU oracle on the same inputs, through the C ABI.
+
+Bars: Ke, serial and coloured K, Dirichlet, preconditioner applications,
+reactions and fields are bit-identical (same operation order, no contraction);
+atomic K within 1e-13 of serial relative to max|K| (summation order is not
+fixed); PCG displacements within 1e-9 relative (norm-wise) of the reference,
+with iteration counts within 1% + 2 (dot products sum in a different order).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from conftest import FULL_CASES, case_precond, load_golden
+from lightfem_amd import meshgen, native, pipeline
+from oracle import lfo
+
+pytestmark = pytest.mark.gpu
+
+PRECOND_NAME = {0: "jacobi", 1: "block", 2: "ic0"}
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.int64)
+
+
+def digest(a):
+    dt = "<i8" if a.dtype.kind in "iu" else "<f8"
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=dt).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = native.Device(0)
+    yield d
+    d.close()
+
+
+def upload(dev, g, meta):
+    enode = g["edofs"][:, 0::2] // 2
+    n_nodes = g["xy"].shape[0]
+    plane = native.PLANE_STRAIN if meta["plane_mode"] == "plane_strain" else native.PLANE_STRESS
+    dev.set_mesh(g["xy"], enode, np.arange(enode.shape[0]), g["elem_material"], plane)
+    row_ptr, col, emap = native.csr_pattern(n_nodes, enode)
+    colors, ptr, elems = native.element_colors(n_nodes, enode)
+    dev.set_pattern(row_ptr, col, emap)
+    dev.set_colors(ptr, elems)
+    return enode, plane, (colors, ptr, elems)
+
+
+def colored_sum(ke, emap, ptr, elems, nnz):
+    values = np.zeros(nnz)
+    for c in range(ptr.shape[0] - 1):
+        group = elems[ptr[c]:ptr[c + 1]]
+        values[emap[group].ravel()] += ke[group].ravel()
+    return values
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_stages_match_reference(dev, golden_index, name):
+    g = load_golden(name)
+    meta = golden_index[name]["meta"]
+    enode, plane, (colors, ptr, elems) = upload(dev, g, meta)
+    nnz = g["col"].shape[0]
+    print(f"case {name}: dofs={g['f'].shape[0]} nnz={nnz}", flush=True)
+    # element stiffness
+    ke = dev.element_stiffness()
+    assert np.array_equal(bits(ke), bits(g["ke"]))
+    # colouring (bit-identical to the reference's colour-ordered sums)
+    dev.assemble("coloring")
+    k_col = dev.stiffness()
+    assert np.array_equal(bits(k_col), bits(colored_sum(g["ke"], g["emap"], ptr, elems, nnz)))
+    if meta["assembly"] == "coloring":
+        assert np.array_equal(bits(k_col), bits(g["K"]))
+    # atomic
+    dev.assemble("atomic")
+    k_serial_ref = lfo.assemble_serial(g["ke"], g["emap"], nnz)
+    assert np.max(np.abs(dev.stiffness() - k_serial_ref)) <= 1e-13 * np.max(np.abs(k_serial_ref))
+    # serial (gather in element order) last: the solve below uses it
+    rep = dev.assemble("serial")
+    assert rep["jacobian_failures"] == 0
+    K = dev.stiffness()
+    assert np.array_equal(bits(K), bits(k_serial_ref))
+    if meta["assembly"] == "serial":
+        assert np.array_equal(bits(K), bits(g["K"]))
+    # Dirichlet elimination (on the reference's matrix when it used another strategy)
+    dev.set_stiffness(g["K"])
+    dev.set_load(g["f"])
+    dev.dirichlet(g["cdofs"], g["cvals"])
+    kc, fc = dev.system()
+    assert np.array_equal(bits(kc), bits(g["Kc"])) and np.array_equal(bits(fc), bits(g["fc"]))
+    # preconditioner applications
+    assert np.array_equal(bits(dev.precond_apply("jacobi", g["probe"])), bits(g["jacobi_invdiag"] * g["probe"]))
+    assert np.array_equal(bits(dev.precond_apply("block", g["probe"])), bits(g["block1_apply_probe"]))
+    assert np.array_equal(bits(dev.precond_apply("ic0", g["probe"])), bits(g["ic0_apply_probe"]))
+    # PCG with the case's preconditioner and the reference stop rule
+    precond = PRECOND_NAME[case_precond(meta)]
+    u, rep = dev.pcg(precond, "fp64", "reference", meta["tolerance"])
+    assert rep["converged"] == 1
+    assert abs(rep["iterations"] - meta["iterations"]) <= 2 + 0.01 * meta["iterations"]
+    assert rep["initial_residual"] == pytest.approx(meta["initial_residual"], rel=1e-14)
+    assert np.linalg.norm(u - g["u"]) <= 1e-9 * np.linalg.norm(g["u"])
+    # reactions and fields from the device solution, against the oracle on that solution
+    R = dev.reactions()
+    assert np.array_equal(bits(R), bits(lfo.spmv(g["row_ptr"], g["col"], g["K"], u) - g["f"]))
+    centre, gp, nodal = dev.postprocess()
+    o_centre, o_gp = lfo.element_fields(g["xy"], enode, g["elem_material"], plane, u)
+    assert np.array_equal(bits(centre), bits(o_centre)) and np.array_equal(bits(gp), bits(o_gp))
+    assert np.array_equal(bits(nodal), bits(lfo.smooth_nodal(g["xy"].shape[0], enode, o_gp.reshape(-1, 36))))
+    # and against the reference's own outputs to solver accuracy
+    scale = np.max(np.abs(g["R"]))
+    assert np.max(np.abs(R - g["R"])) <= 1e-9 * scale
+    assert np.max(np.abs(centre - g["elem_centre"])) <= 1e-9 * np.max(np.abs(g["elem_centre"]))
+
+
+def _c1(dev, assembly="serial"):
+    return pipeline.solve(meshgen.c1_model(), assembly=assembly, device=dev)
+
+
+def test_c1_pipeline_matches_reference(dev, golden_index):
+    """50x50 cantilever: structures and K bit-exact, u and R to 1e-9."""
+    ref = golden_index["c1"]
+    print("case c1 50x50 serial jacobi", flush=True)
+    out = _c1(dev)
+    s = out["system"]
+    assert digest(s.row_ptr.astype(np.int64)) == ref["sha256"]["row_ptr"]
+    assert digest(s.emap.astype(np.int64)) == ref["sha256"]["emap"]
+    assert digest(s.device.stiffness()) == ref["sha256"]["K"]
+    kc, fc = s.device.system()
+    assert digest(kc) == ref["sha256"]["Kc"] and digest(fc) == ref["sha256"]["fc"]
+    gold = load_golden("c1_u")
+    assert abs(out["pcg"]["iterations"] - 313) <= 3
+    assert np.linalg.norm(out["u"] - gold["u"]) <= 1e-9 * np.linalg.norm(gold["u"])
+    assert np.max(np.abs(out["reaction"] - gold["R"])) <= 1e-9 * np.max(np.abs(gold["R"]))
+    assert abs(out["reaction"][1::2].sum() - 1.0) <= 1e-10  # reactions balance the load
+
+
+@pytest.mark.parametrize("precond,iters", [("jacobi", 1240), ("block", 1240), ("ic0", 359)])
+def test_c2_preconditioners(dev, golden_index, precond, iters):
+    """200x200, RCM, coloring: K digest exact, PCG vs the oracle."""
+    print(f"case c2 200x200 rcm coloring {precond}", flush=True)
+    sys_ = pipeline.setup(meshgen.c2_model(), device=dev, assembly="coloring", reordering="rcm")
+    ref = golden_index[f"c2_{'block1' if precond == 'block' else precond}"]
+    assert digest(dev.stiffness()) == ref["sha256"]["K"]
+    assert sys_.assembly_report["coloring_count"] == 4
+    kc, fc = dev.system()
+    assert digest(kc) == ref["sha256"]["Kc"]
+    u, rep = dev.pcg(precond, "fp64", "reference", 1e-10)
+    assert rep["converged"] == 1 and abs(rep["iterations"] - iters) <= 2 + 0.01 * iters
+    o_u, o_rep = lfo.pcg(sys_.row_ptr, sys_.col, kc, fc, {"jacobi": 0, "block": 1, "ic0": 2}[precond])
+    assert o_rep["iterations"] == iters
+    assert np.linalg.norm(u - o_u) <= 1e-9 * np.linalg.norm(o_u)
+    if precond == "ic0":
+        assert rep["breakdowns"] == 0 and rep["levels"] > 0
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "block"])
+def test_mixed_precision_matches_fp64(dev, precond):
+    """FP32 values, FP64 vectors/dots, FP64 residual replacement: same answer to 1e-9."""
+    n = 64
+    print(f"case mixed {n}x{n} log-uniform E plane strain {precond}", flush=True)
+    m = meshgen.c3_model(n)
+    sys_ = pipeline.setup(m, device=dev, assembly="coloring")
+    kc, fc = dev.system()
+    u64, r64 = dev.pcg(precond, "fp64", "relative", 1e-12)
+    um, rm = dev.pcg(precond, "mixed", "relative", 1e-12)
+    assert r64["converged"] and rm["converged"] and rm["residual_replacements"] >= 1
+    assert rm["true_residual"] <= 1.5e-12 * rm["initial_residual"]
+    A = __import__("scipy.sparse", fromlist=["csr_matrix"]).csr_matrix((kc, sys_.col, sys_.row_ptr))
+    exact = __import__("scipy.sparse.linalg", fromlist=["spsolve"]).spsolve(A.tocsc(), fc)
+    for u in (u64, um):
+        assert np.linalg.norm(u - exact) <= 1e-9 * np.linalg.norm(exact)
+
+
+def test_deterministic_and_graph_equals_direct(dev):
+    print("case determinism 50x50", flush=True)
+    pipeline.setup(meshgen.c1_model(), device=dev)
+    u1, r1 = dev.pcg("jacobi")
+    u2, r2 = dev.pcg("jacobi")
+    u3, r3 = dev.pcg("jacobi", direct_launch=True, check_every=7)
+    assert np.array_equal(bits(u1), bits(u2)) and np.array_equal(bits(u1), bits(u3))
+    assert r1["iterations"] == r2["iterations"] == r3["iterations"]
+    for variant in (1, 2, 3):
+        u4, r4 = dev.pcg("jacobi", spmv_variant=variant)
+        assert r4["converged"] and np.linalg.norm(u4 - u1) <= 1e-12 * np.linalg.norm(u1)
+
+
+def test_error_paths(dev):
+    print("case error paths", flush=True)
+    m = meshgen.structured_cantilever(6, 6)
+    pipeline.setup(m, device=dev)
+    with pytest.raises(native.LightFemError) as err:
+        dev.pcg("jacobi", max_iterations=1)
+    assert err.value.code == native.ERR_NOT_CONVERGED
+    bad = meshgen.structured_cantilever(3, 3)
+    bad.enodes[4] = bad.enodes[4][::-1].copy()  # clockwise element
+    with pytest.raises(native.LightFemError) as err:
+        pipeline.setup(bad, device=dev)
+    assert err.value.code == native.ERR_JACOBIAN
+    assert "non-positive Jacobian" in str(err.value)
+    neg = meshgen.structured_cantilever(3, 3, e_uniform=-1.0)
+    pipeline.setup(neg, device=dev)
+    with pytest.raises(native.LightFemError) as err:
+        dev.pcg("jacobi")
+    assert err.value.code in (native.ERR_NOT_SPD, native.ERR_NONFINITE)
+
+
+def test_full_size_assembly_properties(dev):
+    """1000x1000 (BASELINE config 3 size): strategies agree, Dirichlet rows are identity."""
+    print("case 1000x1000 assembly strategies", flush=True)
+    m = meshgen.c3_model(1000)
+    sys_ = pipeline.setup(m, device=dev, assembly="serial")
+    assert sys_.col.shape[0] == 36024004 and sys_.row_ptr.shape[0] - 1 == 2004002
+    k_serial = dev.stiffness()
+    dev.assemble("coloring")
+    k_col = dev.stiffness()
+    dev.assemble("atomic")
+    k_at = dev.stiffness()
+    scale = np.max(np.abs(k_serial))
+    assert np.max(np.abs(k_col - k_serial)) <= 1e-13 * scale
+    assert np.max(np.abs(k_at - k_serial)) <= 1e-13 * scale
+    # symmetry of the assembled matrix on a sample of rows
+    rows = np.arange(0, 2004002, 9973)
+    for r in rows:
+        for k in range(sys_.row_ptr[r], sys_.row_ptr[r + 1]):
+            c = sys_.col[k]
+            kk = sys_.row_ptr[c] + np.searchsorted(sys_.col[sys_.row_ptr[c]:sys_.row_ptr[c + 1]], r)
+            assert k_serial[k] == k_serial[kk]
+    dev.set_stiffness(k_serial)
+    dev.set_load(sys_.indexed.load)
+    dev.dirichlet(sys_.indexed.cdofs, sys_.indexed.cvals)
+    kc, fc = dev.system()
+    for d in sys_.indexed.cdofs[::50]:
+        seg = slice(sys_.row_ptr[d], sys_.row_ptr[d + 1])
+        assert np.array_equal(kc[seg], (sys_.col[seg] == d).astype(float))
+    x = np.random.default_rng(1).standard_normal(2004002)
+    y = dev.spmv(x)
+    import scipy.sparse
+    A = scipy.sparse.csr_matrix((kc, sys_.col, sys_.row_ptr), shape=(2004002, 2004002))
+    ref = A @ x
+    assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref))
