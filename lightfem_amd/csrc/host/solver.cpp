@@ -1,0 +1,350 @@
+// LinearElasticSolver::solve on the gfx950 backend. The stage order, results and
+// exception texts are those of src/solver.cpp:875-960; host stages (numbering,
+// RCM, pattern + map, colouring, loads) run here and every numeric stage runs on
+// the GPU through include/lightfem_gpu.h. There is no CPU fallback: a missing
+// device is an error.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "lightfem/lightfem.hpp"
+#include "lightfem_gpu.h"
+#include "lightfem_host.h"
+
+namespace lightfem {
+
+namespace {
+
+double ms_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+}
+
+struct DeviceContext {
+    lfg_ctx* ctx = nullptr;
+    DeviceContext() {
+        const int rc = lfg_create(0, &ctx);
+        if (rc != LFG_OK) throw std::runtime_error("no usable gfx950 device: lfg_create failed");
+    }
+    ~DeviceContext() { lfg_destroy(ctx); }
+    DeviceContext(const DeviceContext&) = delete;
+    DeviceContext& operator=(const DeviceContext&) = delete;
+
+    // Rethrows a failed stage with the reference's exception type and text.
+    void check(int rc) const {
+        if (rc == LFG_OK) return;
+        const std::string message = lfg_last_error(ctx);
+        switch (rc) {
+            case LFG_ERR_ARG: throw std::invalid_argument(message);
+            case LFG_ERR_STATE: throw std::logic_error(message);
+            default: throw std::runtime_error(message);
+        }
+    }
+};
+
+struct HostArrays {
+    std::vector<double> xy;
+    std::vector<int32_t> enodes;      // node indices
+    std::vector<int32_t> enode_ids;   // node ids (for RCM)
+    std::vector<int32_t> elem_mat;
+    std::vector<double> materials;
+};
+
+HostArrays host_arrays(const Model& model) {
+    HostArrays h;
+    h.xy.resize(2 * model.nodes.size());
+    for (std::size_t i = 0; i < model.nodes.size(); ++i) {
+        h.xy[2 * i] = model.nodes[i].position.x;
+        h.xy[2 * i + 1] = model.nodes[i].position.y;
+    }
+    h.materials.reserve(3 * model.materials.size());
+    for (const Material& m : model.materials) {
+        h.materials.push_back(m.young_modulus);
+        h.materials.push_back(m.poisson_ratio);
+        h.materials.push_back(m.thickness);
+    }
+    h.enodes.reserve(4 * model.elements.size());
+    h.elem_mat.reserve(model.elements.size());
+    for (const Q4Element& e : model.elements) {
+        for (const int id : e.node_ids) {
+            const Node* node = model.find_node(id);
+            if (!node) throw std::logic_error("validated element references an unknown node");
+            h.enodes.push_back(static_cast<int32_t>(node->dof_x / 2));
+            h.enode_ids.push_back(id);
+        }
+        const Material* m = model.find_material(e.material_id);
+        if (!m) throw std::logic_error("validated element references an unknown material");
+        h.elem_mat.push_back(static_cast<int32_t>(m - model.materials.data()));
+    }
+    return h;
+}
+
+struct Pattern {
+    std::vector<int32_t> row_ptr, col, emap;
+};
+
+Pattern build_pattern(const Model& model, const HostArrays& h) {
+    Pattern p;
+    const int64_t n_nodes = static_cast<int64_t>(model.nodes.size());
+    const int64_t n_elems = static_cast<int64_t>(model.elements.size());
+    const int64_t nnz = lfh_csr_pattern_nnz(n_nodes, n_elems, h.enodes.data());
+    if (nnz < 0) throw std::out_of_range("element DOF is outside CSR matrix dimension");
+    p.row_ptr.resize(static_cast<std::size_t>(2 * n_nodes + 1));
+    p.col.resize(static_cast<std::size_t>(nnz));
+    p.emap.resize(static_cast<std::size_t>(64 * n_elems));
+    if (lfh_csr_pattern(n_nodes, n_elems, h.enodes.data(), p.row_ptr.data(), p.col.data(), p.emap.data()) != 0)
+        throw std::overflow_error("CSR pattern exceeds the int32 index range of the gfx950 backend");
+    return p;
+}
+
+CsrProfile profile_of(const Pattern& p) {
+    int64_t out[3] = {0, 0, 0};
+    lfh_csr_profile(static_cast<int64_t>(p.row_ptr.size()) - 1, p.row_ptr.data(), p.col.data(), out);
+    CsrProfile c;
+    c.dimension = p.row_ptr.size() - 1;
+    c.nonzeros = static_cast<std::size_t>(out[0]);
+    c.bandwidth = static_cast<std::size_t>(out[1]);
+    c.profile = static_cast<std::size_t>(out[2]);
+    return c;
+}
+
+std::array<Vec2, 4> corners(const Model& model, const Q4Element& e) {
+    std::array<Vec2, 4> c{};
+    for (int i = 0; i < 4; ++i) {
+        const Node* node = model.find_node(e.node_ids[i]);
+        if (!node) throw std::logic_error("validated element references an unknown node");
+        c[i] = node->position;
+    }
+    return c;
+}
+
+// solver.cpp:303-332: nodal loads, then body forces, then edge loads, in model order.
+std::vector<double> load_vector(const Model& model) {
+    std::vector<double> f(model.summary().dofs, 0.0);
+    for (const NodalLoad& load : model.loads) {
+        const Node* node = model.find_node(load.node_id);
+        if (!node) throw std::logic_error("validated load references an unknown node");
+        f[node->dof_x] += load.fx;
+        f[node->dof_y] += load.fy;
+    }
+    const auto add = [&](const Q4Element& e, const std::array<double, 8>& local) {
+        const auto dofs = q4_dof_indices(model, e);
+        for (int i = 0; i < 8; ++i) f[dofs[i]] += local[i];
+    };
+    for (const BodyForce& force : model.body_forces) {
+        const Q4Element* e = model.find_element(force.element_id);
+        if (!e) throw std::logic_error("validated body force references an unknown element");
+        const Material* m = model.find_material(e->material_id);
+        if (!m) throw std::logic_error("validated element references an unknown material");
+        add(*e, q4_body_force(corners(model, *e), *m, force.bx, force.by));
+    }
+    for (const EdgeLoad& load : model.edge_loads) {
+        const Q4Element* e = model.find_element(load.element_id);
+        if (!e) throw std::logic_error("validated edge load references an unknown element");
+        const Material* m = model.find_material(e->material_id);
+        if (!m) throw std::logic_error("validated element references an unknown material");
+        const auto c = corners(model, *e);
+        add(*e, load.type == EdgeLoadType::Traction ? q4_edge_traction(c, *m, load.edge, load.first, load.second)
+                                                    : q4_edge_pressure(c, *m, load.edge, load.first));
+    }
+    return f;
+}
+
+int assembly_code(const std::string& name) {
+    if (name == "serial" || name == "scan") return LFG_ASSEMBLY_SERIAL;  // scan computes the same sums
+    if (name == "atomic") return LFG_ASSEMBLY_ATOMIC;
+    if (name == "coloring") return LFG_ASSEMBLY_COLORING;
+    if (name == "thread_local") return LFG_ASSEMBLY_THREAD_LOCAL;
+    throw std::logic_error("validated model has an unknown assembly strategy");
+}
+
+}  // namespace
+
+std::vector<int> reverse_cuthill_mckee(const Model& model) {
+    if (!model.is_indexed()) throw std::logic_error("model must be indexed before RCM ordering");
+    std::vector<int32_t> ids(model.nodes.size());
+    for (std::size_t i = 0; i < model.nodes.size(); ++i) ids[i] = model.nodes[i].id;
+    std::vector<int32_t> enode_ids;
+    enode_ids.reserve(4 * model.elements.size());
+    for (const Q4Element& e : model.elements)
+        for (const int id : e.node_ids) enode_ids.push_back(id);
+    std::vector<int32_t> order(ids.size());
+    if (lfh_rcm(static_cast<int64_t>(ids.size()), ids.data(), static_cast<int64_t>(model.elements.size()),
+                enode_ids.data(), order.data()) != 0)
+        throw std::logic_error("validated element references an unknown node");
+    return std::vector<int>(order.begin(), order.end());
+}
+
+AnalysisResult LinearElasticSolver::solve(Model& model) const {
+    if (!model.is_indexed()) throw std::logic_error("model must pass validation and sort_and_index before solving");
+    const SolverSettings& s = model.settings;
+    if (!model.trusses.empty())
+        throw std::runtime_error("truss members are outside the gfx950 backend's scope (Q4 models only)");
+    if (s.storage != "csr") throw std::runtime_error("storage bsr is not implemented by the gfx950 backend; use csr");
+    if (s.preconditioner == "block" && s.subdomain_nodes != 1)
+        throw std::runtime_error("the gfx950 backend implements preconditioner block with subdomain_nodes 1 "
+                                 "(2x2 nodal blocks)");
+    const auto solve_start = std::chrono::steady_clock::now();
+    AnalysisResult result;
+    result.performance.reordering = s.reordering;
+    const bool rcm = s.reordering == "rcm";
+    const auto profile_start = std::chrono::steady_clock::now();
+    if (rcm) {
+        result.performance.before = profile_of(build_pattern(model, host_arrays(model)));
+        model.index_with_node_order(reverse_cuthill_mckee(model));
+    }
+    result.performance.profile_ms = ms_since(profile_start);
+
+    DeviceContext dev;
+    const auto pattern_start = std::chrono::steady_clock::now();
+    const HostArrays h = host_arrays(model);
+    const Pattern pat = build_pattern(model, h);
+    result.performance.pattern_ms = ms_since(pattern_start);
+    const int64_t n_nodes = static_cast<int64_t>(model.nodes.size());
+    const int64_t n_elems = static_cast<int64_t>(model.elements.size());
+    const int64_t n = 2 * n_nodes;
+    const int64_t nnz = static_cast<int64_t>(pat.col.size());
+    dev.check(lfg_set_mesh(dev.ctx, n_nodes, h.xy.data(), n_elems, h.enodes.data(), h.elem_mat.data(),
+                           static_cast<int64_t>(model.materials.size()), h.materials.data(),
+                           s.plane_mode == PlaneMode::PlaneStrain ? LFG_PLANE_STRAIN : LFG_PLANE_STRESS));
+    dev.check(lfg_set_pattern(dev.ctx, n, nnz, pat.row_ptr.data(), pat.col.data(), pat.emap.data()));
+    const int strategy = assembly_code(s.assembly);
+    if (strategy == LFG_ASSEMBLY_COLORING) {
+        std::vector<int32_t> color(static_cast<std::size_t>(n_elems)), ptr(static_cast<std::size_t>(n_elems) + 2),
+            elems(static_cast<std::size_t>(n_elems));
+        int32_t count = 0;
+        lfh_element_colors(n_nodes, n_elems, h.enodes.data(), color.data(), &count, ptr.data(), elems.data());
+        dev.check(lfg_set_colors(dev.ctx, count, ptr.data(), elems.data()));
+    }
+    lfg_assembly_report asm_rep{};
+    dev.check(lfg_assemble(dev.ctx, strategy, &asm_rep));
+    const std::vector<double> load = load_vector(model);
+    dev.check(lfg_set_load(dev.ctx, load.data()));
+
+    const auto retain_start = std::chrono::steady_clock::now();
+    std::vector<double> values(static_cast<std::size_t>(nnz));
+    dev.check(lfg_get_stiffness(dev.ctx, values.data()));
+    result.stiffness = CsrMatrix(static_cast<std::size_t>(n),
+                                 std::vector<std::size_t>(pat.row_ptr.begin(), pat.row_ptr.end()),
+                                 std::vector<std::size_t>(pat.col.begin(), pat.col.end()), std::move(values));
+    result.load = load;
+    result.performance.retain_matrix_ms = ms_since(retain_start);
+    const auto after_start = std::chrono::steady_clock::now();
+    result.performance.after = profile_of(pat);
+    result.performance.profile_ms += ms_since(after_start);
+    if (!rcm) result.performance.before = result.performance.after;
+    result.performance.assembly = s.assembly;
+    result.performance.requested_threads = s.threads;
+    result.performance.threads_used = 1;
+    result.performance.coloring_count = static_cast<std::size_t>(asm_rep.coloring_count);
+    result.performance.element_stiffness_ms = asm_rep.element_stiffness_ms;
+    result.performance.assembly_ms = asm_rep.assembly_ms;
+
+    // solver.cpp:340-351, 355-413
+    const auto boundary_start = std::chrono::steady_clock::now();
+    std::vector<std::pair<int32_t, double>> constraints;
+    for (const DirichletBC& bc : model.boundaries) {
+        const Node* node = model.find_node(bc.node_id);
+        if (!node) throw std::logic_error("validated boundary references an unknown node");
+        if (bc.ux) constraints.emplace_back(static_cast<int32_t>(node->dof_x), *bc.ux);
+        if (bc.uy) constraints.emplace_back(static_cast<int32_t>(node->dof_y), *bc.uy);
+    }
+    std::sort(constraints.begin(), constraints.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    std::vector<int32_t> cdofs;
+    std::vector<double> cvals;
+    for (const auto& c : constraints) {
+        cdofs.push_back(c.first);
+        cvals.push_back(c.second);
+    }
+    dev.check(lfg_dirichlet(dev.ctx, static_cast<int64_t>(cdofs.size()), cdofs.data(), cvals.data()));
+    result.performance.boundary_ms = ms_since(boundary_start);
+
+    lfg_pcg_options opt{};
+    opt.preconditioner = s.preconditioner == "ic0" ? LFG_PRECOND_IC0
+                         : s.preconditioner == "block" ? LFG_PRECOND_BLOCK2
+                                                       : LFG_PRECOND_JACOBI;
+    opt.precision = s.precision == "mixed" ? LFG_PRECISION_MIXED : LFG_PRECISION_FP64;
+    opt.stop_rule = s.stop_rule == "relative" ? LFG_STOP_RELATIVE : LFG_STOP_REFERENCE;
+    opt.tolerance = s.tolerance;
+    opt.max_iterations = static_cast<int64_t>(s.max_iterations);
+    lfg_pcg_report rep{};
+    result.displacement.assign(static_cast<std::size_t>(n), 0.0);
+    const int rc = lfg_pcg(dev.ctx, &opt, result.displacement.data(), &rep);
+    LinearSolverReport& sr = result.solver;
+    sr.converged = rep.converged != 0;
+    sr.iterations = static_cast<std::size_t>(rep.iterations);
+    sr.initial_residual = rep.initial_residual;
+    sr.final_residual = rep.final_residual;
+    sr.solve_ms = rep.solve_ms;
+    sr.matvec_calls = static_cast<std::size_t>(rep.iterations);
+    sr.preconditioner = s.preconditioner;
+    sr.storage = s.storage;
+    sr.preconditioner_setup_ms = rep.setup_ms;
+    sr.precision = s.precision;
+    sr.stop_rule = s.stop_rule;
+    sr.true_residual = rep.true_residual;
+    sr.residual_replacements = static_cast<std::size_t>(rep.residual_replacements);
+    sr.matrix_bytes_per_matvec = static_cast<std::size_t>(nnz) * ((s.precision == "mixed" ? 4 : 8) + 4);
+    if (s.preconditioner == "block") {
+        sr.subdomains = static_cast<std::size_t>(n_nodes);
+        // one node per subdomain: the share of nonzeros inside the 2x2 nodal blocks
+        std::size_t inside = 0;
+        for (int64_t row = 0; row < n; ++row)
+            for (int32_t k = pat.row_ptr[row]; k < pat.row_ptr[row + 1]; ++k)
+                if (pat.col[k] / 2 == row / 2) ++inside;
+        sr.capture_ratio = nnz ? static_cast<double>(inside) / static_cast<double>(nnz) : 0.0;
+    }
+    if (rc == LFG_ERR_NOT_SPD) throw std::runtime_error("PCG encountered non-positive curvature; system is not SPD");
+    if (rc == LFG_ERR_NONFINITE) {
+        throw std::runtime_error(rep.iterations == 0
+                                     ? "PCG failed to form a positive initial preconditioned residual"
+                                     : "PCG preconditioned residual became non-positive or non-finite");
+    }
+    if (rc == LFG_ERR_NOT_CONVERGED || !sr.converged) {
+        sr.message = "maximum iterations reached without convergence";
+        throw std::runtime_error("PCG did not converge: " + sr.message);
+    }
+    dev.check(rc);
+    sr.message = rep.initial_residual <= (s.stop_rule == "relative" ? s.tolerance * rep.initial_residual
+                                                                     : s.tolerance * std::max(1.0, rep.initial_residual))
+                     ? "initial residual satisfies tolerance"
+                     : "converged";
+
+    const auto post_start = std::chrono::steady_clock::now();
+    result.reaction.assign(static_cast<std::size_t>(n), 0.0);
+    dev.check(lfg_reactions(dev.ctx, result.reaction.data()));
+    std::vector<double> centre(7 * static_cast<std::size_t>(n_elems)), gp(36 * static_cast<std::size_t>(n_elems)),
+        nodal(4 * static_cast<std::size_t>(n_nodes));
+    dev.check(lfg_postprocess(dev.ctx, centre.data(), gp.data(), nodal.data()));
+    result.element_results.resize(static_cast<std::size_t>(n_elems));
+    for (int64_t e = 0; e < n_elems; ++e) {
+        ElementResult& er = result.element_results[static_cast<std::size_t>(e)];
+        er.element_id = model.elements[static_cast<std::size_t>(e)].id;
+        const double* c = centre.data() + 7 * e;
+        er.field.strain = {c[0], c[1], c[2]};
+        er.field.stress = {c[3], c[4], c[5]};
+        er.field.von_mises = c[6];
+        for (int q = 0; q < 4; ++q) {
+            const double* g = gp.data() + 36 * e + 9 * q;
+            er.gauss_points[q].xi = g[0];
+            er.gauss_points[q].eta = g[1];
+            er.gauss_points[q].field.strain = {g[2], g[3], g[4]};
+            er.gauss_points[q].field.stress = {g[5], g[6], g[7]};
+            er.gauss_points[q].field.von_mises = g[8];
+        }
+    }
+    result.node_results.resize(static_cast<std::size_t>(n_nodes));
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        NodeResult& nr = result.node_results[static_cast<std::size_t>(i)];
+        nr.node_id = model.nodes[static_cast<std::size_t>(i)].id;
+        nr.stress = {nodal[4 * i], nodal[4 * i + 1], nodal[4 * i + 2]};
+        nr.von_mises = nodal[4 * i + 3];
+    }
+    result.performance.postprocess_ms = ms_since(post_start);
+    result.performance.total_ms = ms_since(solve_start);
+    return result;
+}
+
+}  // namespace lightfem

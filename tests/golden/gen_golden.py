@@ -189,6 +189,32 @@ def save_digest(name: str, arrays: dict, record: dict, keep: tuple = ()) -> None
     record[name] = entry
 
 
+CLI = os.path.join(ROOT, "oracle", "_ref", "lightfem_ref")
+CLI_MODELS = {"one_element": ONE_ELEMENT, "scrambled": SCRAMBLED, "extended": EXTENDED}
+
+
+def cli_fixtures() -> dict:
+    """Reference CLI outputs for small models: check stdout and the solve CSV tables."""
+    out = {}
+    with tempfile.TemporaryDirectory() as work:
+        for name, text in CLI_MODELS.items():
+            fem = os.path.join(work, f"{name}.fem")
+            with open(fem, "w") as handle:
+                handle.write(text)
+            check = subprocess.run([CLI, "check", fem], capture_output=True, text=True, cwd=work)
+            solve = subprocess.run([CLI, "solve", fem, os.path.join(work, f"{name}.vtk")], capture_output=True,
+                                   text=True, cwd=work)
+            entry = {"fem": text, "check_stdout": check.stdout.replace(fem, "<FEM>"), "check_rc": check.returncode,
+                     "solve_rc": solve.returncode}
+            for table in ("nodes", "gauss_points"):
+                with open(os.path.join(work, f"{name}_{table}.csv")) as handle:
+                    entry[f"{table}_csv"] = handle.read()
+            with open(os.path.join(work, f"{name}.vtk")) as handle:
+                entry["vtk"] = handle.read()
+            out[name] = entry
+    return out
+
+
 def main() -> None:
     if not os.path.exists(LFREF):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -240,6 +266,8 @@ def main() -> None:
             arrays = run_dump(c2_text, ["reordering=rcm", "assembly=coloring", f"preconditioner={pname}", *extra],
                               work, f"c2{precond}")
             save_digest(f"c2_{precond}", arrays, record)
+    # CLI fixtures: `check` stdout and `solve` CSV tables of the reference CLI.
+    record["cli"] = cli_fixtures()
     with open(os.path.join(HERE, "golden.json"), "w") as handle:
         json.dump(record, handle, indent=1, sort_keys=True)
     print("wrote", len(record), "golden cases")
