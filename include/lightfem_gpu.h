@@ -59,7 +59,9 @@ typedef struct {
     int32_t check_every;     /* iterations per captured graph between convergence reads; 0 = default */
     double tolerance;        /* SolverSettings::tolerance */
     int64_t max_iterations;  /* SolverSettings::max_iterations */
-    double replace_drop;     /* mixed: replace the residual in FP64 when it falls by this factor; 0 = default */
+    double replace_drop;     /* mixed: replace the residual in FP64 when it falls by this factor; 0 = default 0.1 */
+    double mixed_switch;     /* mixed: SpMVs use FP64 values until ||r|| <= mixed_switch*||r0||, then FP32
+                                values; 0 = default 1e-4, negative = FP32 values from the first iteration */
     int32_t direct_launch;   /* 1: launch each kernel directly instead of replaying a captured hipGraph */
     int32_t spmv_variant;    /* 0 = autotuned/default */
 } lfg_pcg_options;

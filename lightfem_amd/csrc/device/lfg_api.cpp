@@ -362,7 +362,7 @@ lfg::PcgBuffers pcg_buffers(lfg_ctx* c, int kind) {
 
 void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int variant) {
     const lfg::CsrView a = csr_view(c);
-    lfg::launch_pcg_spmv(c->stream, b, a, mixed ? nullptr : c->Kc.p, mixed ? c->K32.p : nullptr, variant);
+    lfg::launch_pcg_spmv(c->stream, b, a, c->Kc.p, mixed ? c->K32.p : nullptr, variant);
     lfg::launch_pcg_update(c->stream, b, mixed);
     if (mixed) lfg::launch_pcg_replace(c->stream, b, a, c->Kc.p);
     if (b.precond == LFG_PRECOND_IC0) lfg::launch_pcg_ic0_step(c->stream, b, ic0_view(c));
@@ -687,6 +687,7 @@ int lfg_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, double* u_out, lfg_pcg_rep
         const int variant = opt->spmv_variant;
         const int per_graph = opt->check_every > 0 ? opt->check_every : 32;
         const double drop = opt->replace_drop > 0.0 ? opt->replace_drop : 0.1;
+        const double switch_tau = !mixed ? 0.0 : (opt->mixed_switch != 0.0 ? opt->mixed_switch : 1.0e-4);
         // capture `per_graph` iterations once
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
@@ -707,7 +708,7 @@ int lfg_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, double* u_out, lfg_pcg_rep
         } guard{graph, exec};
 
         LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-        lfg::launch_pcg_init(ctx->stream, b, opt->stop_rule, opt->tolerance, opt->max_iterations, drop);
+        lfg::launch_pcg_init(ctx->stream, b, opt->stop_rule, opt->tolerance, opt->max_iterations, drop, switch_tau);
         if (kind == LFG_PRECOND_IC0) {
             lfg::launch_pcg_ic0_step(ctx->stream, b, ic0_view(ctx));
             lfg::launch_pcg_direction(ctx->stream, b);
@@ -759,7 +760,8 @@ int lfg_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, double* u_out, lfg_pcg_rep
         if (report) *report = rep;
         ctx->have_solution = true;
         if (st.status != LFG_OK) throw LfgError(st.status, status_text(st.status));
-        return st.converged ? LFG_OK : LFG_ERR_NOT_CONVERGED;
+        if (!st.converged) throw LfgError(LFG_ERR_NOT_CONVERGED, "maximum iterations reached without convergence");
+        return LFG_OK;
     });
 }
 
@@ -819,7 +821,7 @@ static void run_spmv(lfg_ctx* ctx, int precision, int variant) {
     lfg::PcgBuffers b = pcg_buffers(ctx, LFG_PRECOND_JACOBI);
     b.st = ctx->state.p;
     lfg::launch_pcg_spmv(ctx->stream, b, csr_view(ctx), precision == LFG_PRECISION_MIXED ? nullptr : ctx->Kc.p,
-                         precision == LFG_PRECISION_MIXED ? ctx->K32.p : nullptr, variant);
+                         precision == LFG_PRECISION_MIXED ? ctx->K32.p : nullptr, variant);  // one value array
 }
 
 static void prepare_spmv(lfg_ctx* ctx, int precision) {

@@ -24,6 +24,7 @@ struct PcgState {
     double maxr;       // mixed: residual norm at the last replacement
     double drop;       // mixed: replace when ||r|| < drop * maxr
     double rz_new;     // r.z computed by the update kernel
+    double switch_rr;  // mixed: FP32 values once ||r||^2 <= switch_rr
     long long iter;
     long long max_iter;
     long long replacements;
@@ -31,6 +32,7 @@ struct PcgState {
     int status;        // lfg_status
     int converged;
     int need_replace;  // mixed: the replacement kernels run this iteration
+    int phase32;       // mixed: SpMVs read the FP32 values
     unsigned int ticket[4];
 };
 

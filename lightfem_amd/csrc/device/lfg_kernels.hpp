@@ -67,8 +67,10 @@ struct PcgBuffers {
 };
 
 int spmv_blocks(int n, int variant);
-void launch_pcg_init(hipStream_t s, const PcgBuffers& b, int stop_rule, double tol, long long max_iter, double drop);
-// q = A p with A in FP64 (vals64) or FP32 (vals32); finishes p.q and alpha.
+void launch_pcg_init(hipStream_t s, const PcgBuffers& b, int stop_rule, double tol, long long max_iter, double drop,
+                     double mixed_switch);
+// q = A p; finishes p.q and alpha. Without vals32 the FP64 values are used; with
+// both, the kernel reads the FP32 values once the state is in the FP32 phase.
 void launch_pcg_spmv(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* vals64, const float* vals32,
                      int variant);
 // x += alpha p, r -= alpha q, ||r||^2, r.z; convergence / replacement decision, beta.

@@ -208,9 +208,11 @@ __device__ __forceinline__ double row_dot(const int* __restrict__ rp, const int*
 }
 
 // r0 = b, x0 = 0, z0 = M^-1 r0, p0 = z0; ||r0||^2 and r0.z0; threshold.
+// switch_tau: 0 keeps FP64 values throughout, > 0 switches to the FP32 values once
+// ||r|| <= switch_tau ||r0||, < 0 uses the FP32 values from the first iteration.
 template <int Prec>
 __global__ __launch_bounds__(kThreads) void pcg_init_kernel(PcgBuffers b, int stop_rule, double tol,
-                                                            long long max_iter, double drop) {
+                                                            long long max_iter, double drop, double switch_tau) {
     __shared__ double sh[kThreads / kWave];
     const int n_nodes = b.n >> 1;
     double rr = 0.0, rz = 0.0;
@@ -254,6 +256,8 @@ __global__ __launch_bounds__(kThreads) void pcg_init_kernel(PcgBuffers b, int st
         st->maxr = rnorm0;
         st->drop = drop;
         st->need_replace = 0;
+        st->phase32 = switch_tau < 0.0 ? 1 : 0;
+        st->switch_rr = switch_tau > 0.0 ? (switch_tau * rnorm0) * (switch_tau * rnorm0) : -1.0;
         st->converged = 0;
         st->status = LFG_OK;
         st->done = 0;
@@ -271,16 +275,10 @@ __global__ __launch_bounds__(kThreads) void pcg_init_kernel(PcgBuffers b, int st
 // one contiguous node range (XCD-remapped), so partials stay few and the rows a
 // XCD touches, with the p entries they gather, stay in that XCD's L2.
 template <int W, typename VT>
-__global__ __launch_bounds__(kThreads) void pcg_spmv_kernel(PcgBuffers b, const int* __restrict__ rp,
-                                                            const int* __restrict__ col, const VT* __restrict__ val) {
-    if (b.st->done) return;
-    __shared__ double sh[kThreads / kWave];
+__device__ __forceinline__ double spmv_range(const PcgBuffers& b, const int* __restrict__ rp,
+                                             const int* __restrict__ col, const VT* __restrict__ val, int start,
+                                             int stop) {
     constexpr int kNodesPerStep = kThreads / (2 * W);
-    const int n_nodes = b.n >> 1;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int per_block = (n_nodes + gridDim.x - 1) / gridDim.x;
-    const int start = lb * per_block;
-    const int stop = min(start + per_block, n_nodes);
     const int group = threadIdx.x / (2 * W);
     const int half = (threadIdx.x / W) & 1;
     const int lane = threadIdx.x & (W - 1);
@@ -296,6 +294,28 @@ __global__ __launch_bounds__(kThreads) void pcg_spmv_kernel(PcgBuffers b, const 
             }
         }
     }
+    return pq;
+}
+
+// q = A p (node groups of 2W lanes), fused p.q, alpha = rho / p.q. Each block owns
+// one contiguous node range (XCD-remapped), so partials stay few and the rows a
+// XCD touches, with the p entries they gather, stay in that XCD's L2. With FP32
+// values present, the state's phase selects which value array is streamed.
+template <int W>
+__global__ __launch_bounds__(kThreads) void pcg_spmv_kernel(PcgBuffers b, const int* __restrict__ rp,
+                                                            const int* __restrict__ col,
+                                                            const double* __restrict__ val64,
+                                                            const float* __restrict__ val32) {
+    if (b.st->done) return;
+    __shared__ double sh[kThreads / kWave];
+    const bool use32 = val32 != nullptr && (val64 == nullptr || b.st->phase32 != 0);
+    const int n_nodes = b.n >> 1;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = (n_nodes + gridDim.x - 1) / gridDim.x;
+    const int start = lb * per_block;
+    const int stop = min(start + per_block, n_nodes);
+    const double pq = use32 ? spmv_range<W, float>(b, rp, col, val32, start, stop)
+                            : spmv_range<W, double>(b, rp, col, val64, start, stop);
     const double spq = block_sum<kThreads>(pq, sh);
     if (threadIdx.x == 0) b.partials[blockIdx.x] = spq;
     if (!ticket_last(&b.st->ticket[1], gridDim.x)) return;
@@ -367,9 +387,9 @@ __global__ __launch_bounds__(kThreads) void pcg_update_kernel(PcgBuffers b) {
         st->rr = trr;
         st->rz_new = trz;
         const double rnorm = sqrt(max0(trr));
-        if (Mixed) {
-            // Replace the recurrence residual by the FP64 residual when it has
-            // dropped far enough or claims convergence.
+        if (Mixed && st->phase32) {
+            // FP32 phase: replace the recurrence residual by the FP64 residual when
+            // it has dropped far enough or claims convergence.
             if (rnorm <= st->target || rnorm < st->drop * st->maxr) {
                 st->need_replace = 1;
                 return;
@@ -378,6 +398,10 @@ __global__ __launch_bounds__(kThreads) void pcg_update_kernel(PcgBuffers b) {
             st->converged = 1;
             st->done = 1;
             return;
+        }
+        if (Mixed && !st->phase32 && trr <= st->switch_rr) {
+            st->phase32 = 1;  // the next SpMVs stream the FP32 values
+            st->maxr = rnorm;
         }
         if (Prec == LFG_PRECOND_IC0) {
             if (st->iter >= st->max_iter) st->done = 1;  // z and beta come from the IC(0) step
@@ -636,17 +660,18 @@ int spmv_blocks(int n, int variant) {
     return static_cast<int>(round8(need < kSpmvBlocks ? need : kSpmvBlocks));
 }
 
-void launch_pcg_init(hipStream_t s, const PcgBuffers& b, int stop_rule, double tol, long long max_iter, double drop) {
+void launch_pcg_init(hipStream_t s, const PcgBuffers& b, int stop_rule, double tol, long long max_iter, double drop,
+                     double switch_tau) {
     const unsigned int grid = update_blocks(b.n);
     switch (b.precond) {
         case LFG_PRECOND_JACOBI:
-            pcg_init_kernel<LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop);
+            pcg_init_kernel<LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop, switch_tau);
             break;
         case LFG_PRECOND_BLOCK2:
-            pcg_init_kernel<LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop);
+            pcg_init_kernel<LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop, switch_tau);
             break;
         default:
-            pcg_init_kernel<LFG_PRECOND_IC0><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop);
+            pcg_init_kernel<LFG_PRECOND_IC0><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop, switch_tau);
             break;
     }
 }
@@ -654,8 +679,7 @@ void launch_pcg_init(hipStream_t s, const PcgBuffers& b, int stop_rule, double t
 template <int W>
 static void spmv_w(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* v64, const float* v32) {
     const unsigned int grid = static_cast<unsigned int>(spmv_blocks(b.n, W == 4 ? 1 : (W == 16 ? 3 : 2)));
-    if (v32) pcg_spmv_kernel<W, float><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, v32);
-    else pcg_spmv_kernel<W, double><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, v64);
+    pcg_spmv_kernel<W><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, v64, v32);
 }
 
 void launch_pcg_spmv(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* vals64, const float* vals32,

@@ -41,7 +41,8 @@ class LightFemError(RuntimeError):
 class PcgOptions(ctypes.Structure):
     _fields_ = [("preconditioner", ctypes.c_int32), ("precision", ctypes.c_int32), ("stop_rule", ctypes.c_int32),
                 ("check_every", ctypes.c_int32), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int64),
-                ("replace_drop", ctypes.c_double), ("direct_launch", ctypes.c_int32),
+                ("replace_drop", ctypes.c_double), ("mixed_switch", ctypes.c_double),
+                ("direct_launch", ctypes.c_int32),
                 ("spmv_variant", ctypes.c_int32)]
 
 
@@ -280,9 +281,9 @@ class Device:
     def pcg(self, preconditioner: str = "jacobi", precision: str = "fp64", stop_rule: str = "reference",
             tolerance: float = 1e-10, max_iterations: int = 100000, check_every: int = 0,
             replace_drop: float = 0.0, direct_launch: bool = False, spmv_variant: int = 0,
-            raise_on_error: bool = True):
+            raise_on_error: bool = True, mixed_switch: float = 0.0):
         opt = PcgOptions(PRECOND[preconditioner], PRECISION[precision], STOP[stop_rule], check_every, tolerance,
-                         max_iterations, replace_drop, int(direct_launch), spmv_variant)
+                         max_iterations, replace_drop, mixed_switch, int(direct_launch), spmv_variant)
         rep = PcgReport()
         u = np.empty(self.n)
         rc = self._lib.lfg_pcg(self._ctx, ctypes.byref(opt), _ptr(u), ctypes.byref(rep))
