@@ -76,6 +76,8 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--check-every", type=int, default=0)
+    ap.add_argument("--profile", type=int, default=0,
+                    help="profiling run: one FP64 and one mixed solve capped at this many iterations, no CPU leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the path is a single-device solve and --gpus {args.gpus} is not "
@@ -103,33 +105,51 @@ def main() -> int:
     spmv = {}
     for precision, vbytes in (("fp64", 8), ("mixed", 4)):
         best = None
-        for variant in (1, 2, 3):
+        for variant in (1, 2, 3, 4):
             ms = dev.bench_spmv(precision, variant, 50)
             if best is None or ms < best[1]:
                 best = (variant, ms)
         b = spmv_bytes(n, nnz, vbytes)
         spmv[precision] = {"variant": best[0], "ms": best[1], "bytes": b, "GBs": b / (best[1] * 1e-3) / 1e9}
     print(f"bench: spmv {json.dumps(spmv)}", file=sys.stderr, flush=True)
+    if args.profile:
+        for form in ("fp64", "mixed"):
+            _, rep = dev.pcg("jacobi", form, "relative", args.tol, args.profile, args.check_every,
+                             spmv_variant=args.variant or spmv[form]["variant"], raise_on_error=False,
+                             mixed_switch=-1.0 if form == "mixed" else 0.0)
+            print(f"bench: profile {form} {rep['iterations']} iterations {rep['solve_ms']:.1f} ms", file=sys.stderr,
+                  flush=True)
+        print(json.dumps({"profile_iterations": args.profile, "spmv": spmv}))
+        dev.close()
+        return 0
     forms = {}
+    cap = 10 ** 6
     for form in ("fp64", "mixed"):
         variant = args.variant or spmv[form]["variant"]
         for _ in range(args.warmup):
-            dev.pcg("jacobi", form, "relative", args.tol, 10 ** 6, args.check_every, spmv_variant=variant)
+            dev.pcg("jacobi", form, "relative", args.tol, cap, args.check_every, spmv_variant=variant,
+                    raise_on_error=False)
+            print(f"bench: {form} warm-up solve done", file=sys.stderr, flush=True)
         _sync()
         t = time.perf_counter()
         reps = []
         for _ in range(args.steps):
-            u, rep = dev.pcg("jacobi", form, "relative", args.tol, 10 ** 6, args.check_every, spmv_variant=variant)
+            u, rep = dev.pcg("jacobi", form, "relative", args.tol, cap, args.check_every, spmv_variant=variant,
+                             raise_on_error=False)
             reps.append(rep)
+            print(f"bench: {form} solve {rep['iterations']} iterations, converged={rep['converged']}", file=sys.stderr,
+                  flush=True)
         _sync()
+        if form == "fp64":
+            cap = 2 * reps[-1]["iterations"]  # the mixed forms get at most twice the FP64 count
         wall = (time.perf_counter() - t) * 1e3 / args.steps
         forms[form] = {"ms_per_solve": wall, "device_solve_ms": float(np.median([r["solve_ms"] for r in reps])),
-                       "iterations": reps[-1]["iterations"], "replacements": reps[-1]["residual_replacements"],
+                       "iterations": reps[-1]["iterations"], "converged": bool(reps[-1]["converged"]),
+                       "replacements": reps[-1]["residual_replacements"],
                        "true_relative_residual": reps[-1]["true_residual"] / reps[-1]["initial_residual"],
                        "spmv_variant": variant, "u": u}
         print(f"bench: {form} {wall:.1f} ms/solve, {reps[-1]['iterations']} iterations", file=sys.stderr, flush=True)
-    # FP32 values from the first iteration (residual replacement only), capped at 3x the FP64 count
-    cap = 3 * forms["fp64"]["iterations"]
+    # FP32 values from the first iteration (residual replacement only), capped at 2x the FP64 count
     t = time.perf_counter()
     u32, rep32 = dev.pcg("jacobi", "mixed", "relative", args.tol, cap, args.check_every,
                          spmv_variant=spmv["mixed"]["variant"], mixed_switch=-1.0, raise_on_error=False)

@@ -98,6 +98,7 @@ struct lfg_ctx {
     // solver
     DevArray<double> x, r, p, q, z, minv, factor, diag, partials, scratch;
     DevArray<lfg::PcgState> state;
+    DevArray<lfg::TicketSet> tickets;  // 0 init, 1 SpMV, 2 update, 3 replacement, 4 residual
     DevArray<int> flags;
     lfg::PcgState* h_state = nullptr;  // pinned, 2 slots
     bool have_solution = false;
@@ -309,6 +310,8 @@ void ensure_vectors(lfg_ctx* c) {
     if (!c->state.p) {
         c->state.alloc(1);
         LFG_HIP(hipMemsetAsync(c->state.p, 0, sizeof(lfg::PcgState), c->stream));
+        c->tickets.alloc(5);
+        LFG_HIP(hipMemsetAsync(c->tickets.p, 0, 5 * sizeof(lfg::TicketSet), c->stream));
         c->flags.alloc(4);
         LFG_HIP(hipMemsetAsync(c->scratch.p, 0, c->scratch.n * sizeof(double), c->stream));
     }
@@ -356,6 +359,7 @@ lfg::PcgBuffers pcg_buffers(lfg_ctx* c, int kind) {
     b.m = kind == LFG_PRECOND_JACOBI ? c->minv.p : (kind == LFG_PRECOND_BLOCK2 ? c->factor.p : nullptr);
     b.precond = kind;
     b.st = c->state.p;
+    b.tickets = c->tickets.p;
     b.partials = c->partials.p;
     return b;
 }
@@ -745,8 +749,8 @@ int lfg_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, double* u_out, lfg_pcg_rep
         // FP64 true residual ||b - K u|| for the report
         double r2 = 0.0;
         lfg::launch_residual_norm(ctx->stream, csr_view(ctx), ctx->Kc.p, ctx->x.p, ctx->fc.p, ctx->scratch.p,
-                                  ctx->scratch.p + 4096 + 4);  // partials [0, 2048), ticket at 4096
-        LFG_HIP(hipMemcpyAsync(&r2, ctx->scratch.p + 4096 + 4, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+                                  ctx->tickets.p + 4, ctx->scratch.p + 4096);
+        LFG_HIP(hipMemcpyAsync(&r2, ctx->scratch.p + 4096, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         if (u_out) ctx->x.download(u_out, static_cast<size_t>(ctx->n), ctx->stream);
         sync(ctx);
         rep.converged = st.converged;

@@ -11,8 +11,8 @@ namespace lfg {
 constexpr int kWave = 64;
 
 // Device-resident Krylov state. One instance per solve lives in device memory;
-// the last block of each reducing kernel (ticket pattern) updates it, and the
-// host reads it back once per captured graph launch.
+// the final block of each reducing kernel updates it, and the host reads it
+// back once per captured graph launch.
 struct PcgState {
     double rho;        // r.z of the current iterate
     double rr;         // ||r||^2 of the current (recurrence or replaced) residual
@@ -25,15 +25,23 @@ struct PcgState {
     double drop;       // mixed: replace when ||r|| < drop * maxr
     double rz_new;     // r.z computed by the update kernel
     double switch_rr;  // mixed: FP32 values once ||r||^2 <= switch_rr
+    double true_rr;    // mixed: ||r||^2 at the last replacement
     long long iter;
     long long max_iter;
     long long replacements;
     int done;          // 1 once converged, failed or out of iterations
     int status;        // lfg_status
     int converged;
-    int need_replace;  // mixed: the replacement kernels run this iteration
+    int need_replace;  // mixed: the replacement kernel runs this iteration
     int phase32;       // mixed: SpMVs read the FP32 values
-    unsigned int ticket[4];
+    int pad;
+};
+
+// Completion counters of one reducing kernel: one per XCD and one on top, each on
+// its own 128-byte line, so the blocks of an XCD contend only with each other.
+struct TicketSet {
+    unsigned int xcd[8][32];
+    unsigned int top[32];
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -59,35 +67,72 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     return t;
 }
 
-// Ticket pattern: thread 0 of every block publishes its partial, then takes a
-// ticket; the block that takes the last ticket sees every partial (release /
-// acquire at agent scope) and finishes the reduction. Returns true in every
-// thread of that last block.
-__device__ __forceinline__ bool ticket_last(unsigned int* ticket, unsigned int nblocks) {
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        const unsigned int t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == nblocks - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    const bool is_last = last != 0;
-    if (is_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return is_last;
+// Partials cross blocks (and XCDs) through relaxed agent-scope atomic stores and
+// loads, which the hardware keeps coherent across the XCDs' L2s by themselves; a
+// workgroup-scope release (a wait for the store to complete) orders a partial
+// before its ticket. Agent-scope acquire/release would instead write back and
+// invalidate the whole L2 of the XCD in every block. Everything else a kernel
+// writes reaches the next kernel through the kernel boundary.
+__device__ __forceinline__ void publish_partial(double* slot, double v) {
+    __hip_atomic_store(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Deterministic final reduction of `count` partials (fixed assignment + fixed tree).
-template <int NT>
-__device__ __forceinline__ double reduce_partials(const double* partials, int count, int stride, double* sh) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int i = threadIdx.x;
-    for (; i + 3 * NT < count; i += 4 * NT) {  // four independent loads in flight
-        s0 += partials[i * stride];
-        s1 += partials[(i + NT) * stride];
-        s2 += partials[(i + 2 * NT) * stride];
-        s3 += partials[(i + 3 * NT) * stride];
+__device__ __forceinline__ double read_partial(const double* slot) {
+    return __hip_atomic_load(const_cast<double*>(slot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Two-level in-kernel reduction of K values per block (grid a multiple of 8).
+// Thread 0 holds the block's values in `vals`. Blocks are grouped as the
+// hardware dispatches them (block b on XCD b % 8): the last block of each group
+// folds its group's partials in a fixed order and publishes one group partial,
+// and the last of the eight group finishers folds those. Returns true in every
+// thread of that final block, with the totals in `out` of thread 0. The fold
+// order is fixed, so results are bit-reproducible run to run. `partials` holds
+// (gridDim.x + 8) * K doubles.
+template <int NT, int K>
+__device__ __forceinline__ bool finish_reduction(TicketSet* t, double* partials, const double* vals, double* out,
+                                                 double* sh) {
+    __shared__ int flag;
+    const int nb = gridDim.x;
+    const int x = blockIdx.x & 7;
+    const int per = nb >> 3;
+    double* group = partials + static_cast<long long>(nb) * K;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) publish_partial(partials + blockIdx.x * K + k, vals[k]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        const unsigned int a = __hip_atomic_fetch_add(&t->xcd[x][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        flag = (a == static_cast<unsigned int>(per - 1)) ? 1 : 0;
     }
-    for (; i < count; i += NT) s0 += partials[i * stride];
-    return block_sum<NT>((s0 + s1) + (s2 + s3), sh);
+    __syncthreads();
+    if (!flag) return false;
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double acc = 0.0;
+        for (int i = threadIdx.x; i < per; i += NT) acc += read_partial(partials + (x + 8 * i) * K + k);
+        s[k] = block_sum<NT>(acc, sh);
+    }
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&t->xcd[x][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int k = 0; k < K; ++k) publish_partial(group + x * K + k, s[k]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        const unsigned int a = __hip_atomic_fetch_add(&t->top[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        flag = (a == 7u) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!flag) return false;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double total = 0.0;
+            for (int g = 0; g < 8; ++g) total += read_partial(group + g * K + k);
+            out[k] = total;
+        }
+        __hip_atomic_store(&t->top[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8).

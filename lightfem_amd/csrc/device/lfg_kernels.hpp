@@ -63,7 +63,8 @@ struct PcgBuffers {
     const double* m;    // Jacobi inverse diagonal (n) or 2x2 factors (3 per node)
     int precond;        // LFG_PRECOND_*
     PcgState* st;
-    double* partials;   // >= 2 * max blocks
+    TicketSet* tickets; // 0 init, 1 SpMV, 2 update, 3 replacement
+    double* partials;   // >= 2 * (max blocks + 8)
 };
 
 int spmv_blocks(int n, int variant);
@@ -100,6 +101,6 @@ void launch_ic0_apply(hipStream_t s, const Ic0View& f, const double* in, double*
 void launch_pcg_ic0_step(hipStream_t s, const PcgBuffers& b, const Ic0View& f);
 // ||b - K x|| in FP64 (reporting).
 void launch_residual_norm(hipStream_t s, const CsrView& a, const double* vals64, const double* x, const double* bvec,
-                          double* work, double* out2);
+                          double* partials, TicketSet* tickets, double* out2);
 
 }  // namespace lfg

@@ -230,18 +230,12 @@ __global__ __launch_bounds__(kThreads) void pcg_init_kernel(PcgBuffers b, int st
             rz += r.x * z0 + r.y * z1;
         }
     }
-    const double srr = block_sum<kThreads>(rr, sh);
-    const double srz = block_sum<kThreads>(rz, sh);
+    const double vals[2] = {block_sum<kThreads>(rr, sh), block_sum<kThreads>(rz, sh)};
+    double tot[2];
+    if (!finish_reduction<kThreads, 2>(b.tickets + 0, b.partials, vals, tot, sh)) return;
     if (threadIdx.x == 0) {
-        b.partials[2 * blockIdx.x] = srr;
-        b.partials[2 * blockIdx.x + 1] = srz;
-    }
-    if (!ticket_last(&b.st->ticket[0], gridDim.x)) return;
-    const double trr = reduce_partials<kThreads>(b.partials, gridDim.x, 2, sh);
-    const double trz = reduce_partials<kThreads>(b.partials + 1, gridDim.x, 2, sh);
-    if (threadIdx.x == 0) {
+        const double trr = tot[0], trz = tot[1];
         PcgState* st = b.st;
-        st->ticket[0] = 0;
         const double rnorm0 = sqrt(max0(trr));
         const double base = stop_rule == LFG_STOP_REFERENCE ? ((1.0 < rnorm0) ? rnorm0 : 1.0) : rnorm0;
         st->rnorm0 = rnorm0;
@@ -257,6 +251,7 @@ __global__ __launch_bounds__(kThreads) void pcg_init_kernel(PcgBuffers b, int st
         st->drop = drop;
         st->need_replace = 0;
         st->phase32 = switch_tau < 0.0 ? 1 : 0;
+        st->true_rr = 0.0;
         st->switch_rr = switch_tau > 0.0 ? (switch_tau * rnorm0) * (switch_tau * rnorm0) : -1.0;
         st->converged = 0;
         st->status = LFG_OK;
@@ -316,13 +311,120 @@ __global__ __launch_bounds__(kThreads) void pcg_spmv_kernel(PcgBuffers b, const 
     const int stop = min(start + per_block, n_nodes);
     const double pq = use32 ? spmv_range<W, float>(b, rp, col, val32, start, stop)
                             : spmv_range<W, double>(b, rp, col, val64, start, stop);
-    const double spq = block_sum<kThreads>(pq, sh);
-    if (threadIdx.x == 0) b.partials[blockIdx.x] = spq;
-    if (!ticket_last(&b.st->ticket[1], gridDim.x)) return;
-    const double tpq = reduce_partials<kThreads>(b.partials, gridDim.x, 1, sh);
+    const double vals[1] = {block_sum<kThreads>(pq, sh)};
+    double tot[1];
+    if (!finish_reduction<kThreads, 1>(b.tickets + 1, b.partials, vals, tot, sh)) return;
     if (threadIdx.x == 0) {
+        const double tpq = tot[0];
         PcgState* st = b.st;
-        st->ticket[1] = 0;
+        st->pq = tpq;
+        if (!(tpq > 0.0) || !isfinite(tpq)) {  // solver.cpp:591-593
+            st->status = LFG_ERR_NOT_SPD;
+            st->done = 1;
+        } else {
+            st->alpha = st->rho / tpq;
+        }
+    }
+}
+
+// Row-segment streaming SpMV (variant 4). A wave owns 64 consecutive rows, whose
+// nonzeros form one contiguous segment: lane l loads entries seg + l, seg + 64 + l,
+// ... (every load instruction moves 64 consecutive values / columns), all value
+// and column loads are issued before the gathers of p, the products are staged in
+// LDS, and lane l then sums row l of the wave. Rows longer than the staging room
+// (kSlots per lane on average) fall back to 8 lanes per row.
+constexpr int kStreamRows = 64;
+constexpr int kSlots = 18;
+
+template <typename VT>
+__device__ __forceinline__ double stream_rows(const PcgBuffers& b, const int* __restrict__ rp,
+                                              const int* __restrict__ col, const VT* __restrict__ val, int row0,
+                                              int row_end, double* __restrict__ stage) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int seg = rp[row0];
+    const int len = rp[row_end] - seg;
+    double pq = 0.0;
+    if (len > kWave * kSlots) {  // wave-uniform fallback for long rows
+        for (int r = row0 + (lane >> 3); r < row_end + 7; r += 8) {
+            const int row = min(r, row_end - 1);
+            const double s = row_dot<8, VT>(rp, col, val, b.p, row, lane & 7);
+            if ((lane & 7) == 0 && r < row_end) {
+                b.q[row] = s;
+                pq += b.p[row] * s;
+            }
+        }
+        return pq;
+    }
+    VT v[kSlots];
+    int c[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+        const int k = j * kWave + lane;
+        if (k < len) {
+            v[j] = val[seg + k];
+            c[j] = col[seg + k];
+        }
+    }
+    double x[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+        if (j * kWave + lane < len) x[j] = b.p[c[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+        const int k = j * kWave + lane;
+        if (k < len) stage[k] = static_cast<double>(v[j]) * x[j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int row = row0 + lane;
+    if (row < row_end) {
+        const int a = rp[row] - seg, e = rp[row + 1] - seg;
+        double sum = 0.0;
+        for (int k = a; k < e; ++k) sum += stage[k];
+        b.q[row] = sum;
+        pq = b.p[row] * sum;
+    }
+    __builtin_amdgcn_wave_barrier();  // the stage is reused by the wave's next rows
+    return pq;
+}
+
+template <typename VT>
+__device__ __forceinline__ double stream_range(const PcgBuffers& b, const int* __restrict__ rp,
+                                               const int* __restrict__ col, const VT* __restrict__ val,
+                                               int row_start, int row_stop, double* stage) {
+    const int wave = threadIdx.x / kWave;
+    constexpr int kWaves = kThreads / kWave;
+    double pq = 0.0;
+    for (int r0 = row_start + wave * kStreamRows; r0 < row_stop; r0 += kWaves * kStreamRows)
+        pq += stream_rows<VT>(b, rp, col, val, r0, min(r0 + kStreamRows, row_stop), stage);
+    return pq;
+}
+
+__global__ __launch_bounds__(kThreads) void pcg_spmv_stream_kernel(PcgBuffers b, const int* __restrict__ rp,
+                                                                   const int* __restrict__ col,
+                                                                   const double* __restrict__ val64,
+                                                                   const float* __restrict__ val32) {
+    if (b.st->done) return;
+    __shared__ double sh[kThreads / kWave];
+    __shared__ double stage_all[kThreads / kWave][kWave * kSlots];
+    double* stage = stage_all[threadIdx.x / kWave];
+    const bool use32 = val32 != nullptr && (val64 == nullptr || b.st->phase32 != 0);
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    // contiguous row range per block, in whole waves' worth of rows (rows stay node-aligned)
+    const int chunks = (b.n + kStreamRows - 1) / kStreamRows;
+    const int per_block = (chunks + gridDim.x - 1) / gridDim.x;
+    const int start = min(lb * per_block * kStreamRows, b.n);
+    const int stop = min(start + per_block * kStreamRows, b.n);
+    const double pq = use32 ? stream_range<float>(b, rp, col, val32, start, stop, stage)
+                            : stream_range<double>(b, rp, col, val64, start, stop, stage);
+    const double vals[1] = {block_sum<kThreads>(pq, sh)};
+    double tot[1];
+    if (!finish_reduction<kThreads, 1>(b.tickets + 1, b.partials, vals, tot, sh)) return;
+    if (threadIdx.x == 0) {
+        const double tpq = tot[0];
+        PcgState* st = b.st;
         st->pq = tpq;
         if (!(tpq > 0.0) || !isfinite(tpq)) {  // solver.cpp:591-593
             st->status = LFG_ERR_NOT_SPD;
@@ -371,32 +473,25 @@ __global__ __launch_bounds__(kThreads) void pcg_update_kernel(PcgBuffers b) {
             rz += r.x * z0 + r.y * z1;
         }
     }
-    const double srr = block_sum<kThreads>(rr, sh);
-    const double srz = block_sum<kThreads>(rz, sh);
+    const double vals[2] = {block_sum<kThreads>(rr, sh), block_sum<kThreads>(rz, sh)};
+    double tot[2];
+    if (!finish_reduction<kThreads, 2>(b.tickets + 2, b.partials, vals, tot, sh)) return;
     if (threadIdx.x == 0) {
-        b.partials[2 * blockIdx.x] = srr;
-        b.partials[2 * blockIdx.x + 1] = srz;
-    }
-    if (!ticket_last(&b.st->ticket[2], gridDim.x)) return;
-    const double trr = reduce_partials<kThreads>(b.partials, gridDim.x, 2, sh);
-    const double trz = reduce_partials<kThreads>(b.partials + 1, gridDim.x, 2, sh);
-    if (threadIdx.x == 0) {
+        const double trr = tot[0], trz = tot[1];
         PcgState* st = b.st;
-        st->ticket[2] = 0;
         st->iter += 1;
         st->rr = trr;
         st->rz_new = trz;
         const double rnorm = sqrt(max0(trr));
-        if (Mixed && st->phase32) {
-            // FP32 phase: replace the recurrence residual by the FP64 residual when
-            // it has dropped far enough or claims convergence.
-            if (rnorm <= st->target || rnorm < st->drop * st->maxr) {
-                st->need_replace = 1;
-                return;
-            }
-        } else if (rnorm <= st->target) {  // solver.cpp:606-611
+        if (rnorm <= st->target) {  // solver.cpp:606-611, on the recurrence residual in every form
             st->converged = 1;
             st->done = 1;
+            return;
+        }
+        if (Mixed && st->phase32 && rnorm < st->drop * st->maxr) {
+            // FP32 phase: replace the recurrence residual by the FP64 residual once
+            // it has dropped by `drop` since the last replacement.
+            st->need_replace = 1;
             return;
         }
         if (Mixed && !st->phase32 && trr <= st->switch_rr) {
@@ -442,18 +537,12 @@ __global__ __launch_bounds__(kThreads) void pcg_replace_kernel(PcgBuffers b, con
             }
         }
     }
-    const double srr = block_sum<kThreads>(rr, sh);
-    const double srz = block_sum<kThreads>(rz, sh);
+    const double vals[2] = {block_sum<kThreads>(rr, sh), block_sum<kThreads>(rz, sh)};
+    double tot[2];
+    if (!finish_reduction<kThreads, 2>(b.tickets + 3, b.partials, vals, tot, sh)) return;
     if (threadIdx.x == 0) {
-        b.partials[2 * blockIdx.x] = srr;
-        b.partials[2 * blockIdx.x + 1] = srz;
-    }
-    if (!ticket_last(&b.st->ticket[3], gridDim.x)) return;
-    const double trr = reduce_partials<kThreads>(b.partials, gridDim.x, 2, sh);
-    const double trz = reduce_partials<kThreads>(b.partials + 1, gridDim.x, 2, sh);
-    if (threadIdx.x == 0) {
+        const double trr = tot[0], trz = tot[1];
         PcgState* st = b.st;
-        st->ticket[3] = 0;
         st->need_replace = 0;
         st->replacements += 1;
         st->rr = trr;
@@ -464,6 +553,10 @@ __global__ __launch_bounds__(kThreads) void pcg_replace_kernel(PcgBuffers b, con
             st->done = 1;
             return;
         }
+        // The FP64 residual has an attainable floor; once a replacement no longer
+        // halves it, further replacements only restart the recurrence, so stop.
+        if (st->true_rr > 0.0 && trr > 0.25 * st->true_rr) st->drop = 0.0;
+        st->true_rr = trr;
         accept_rz(st, trz);
     }
 }
@@ -569,7 +662,7 @@ __global__ __launch_bounds__(kThreads) void residual_norm_kernel(int n, const in
                                                                  const double* __restrict__ x,
                                                                  const double* __restrict__ bvec,
                                                                  double* __restrict__ partials,
-                                                                 unsigned int* ticket, double* out2) {
+                                                                 TicketSet* tickets, double* out2) {
     __shared__ double sh[kThreads / kWave];
     constexpr int kRows = kThreads / W;
     const int per_block = (n + gridDim.x - 1) / gridDim.x;
@@ -587,17 +680,16 @@ __global__ __launch_bounds__(kThreads) void residual_norm_kernel(int n, const in
             }
         }
     }
-    const double sacc = block_sum<kThreads>(acc, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x] = sacc;
-    if (!ticket_last(ticket, gridDim.x)) return;
-    const double total = reduce_partials<kThreads>(partials, gridDim.x, 1, sh);
-    if (threadIdx.x == 0) {
-        *ticket = 0;
-        *out2 = total;
-    }
+    const double vals[1] = {block_sum<kThreads>(acc, sh)};
+    double tot[1];
+    if (!finish_reduction<kThreads, 1>(tickets, partials, vals, tot, sh)) return;
+    if (threadIdx.x == 0) *out2 = tot[0];
 }
 
-// Node-group width per SpMV variant: 1 -> 4 lanes per row, 2 -> 8, 3 -> 16.
+// SpMV variants: 1, 2, 3 -> node groups with 4, 8, 16 lanes per row; 4 (the
+// default) -> row-segment streaming.
+constexpr int kDefaultVariant = 4;
+
 inline int variant_width(int variant) {
     switch (variant) {
         case 1: return 4;
@@ -654,6 +746,11 @@ void launch_to_f32(hipStream_t s, const double* in, float* out, long long n) {
 constexpr unsigned int kSpmvBlocks = 2048;
 
 int spmv_blocks(int n, int variant) {
+    if (variant == 0) variant = kDefaultVariant;
+    if (variant == 4) {
+        const unsigned int need = blocks_for((n + kStreamRows - 1) / kStreamRows, kThreads / kWave);
+        return static_cast<int>(round8(need < kSpmvBlocks ? need : kSpmvBlocks));
+    }
     const int w = variant_width(variant);
     const int nodes_per_step = kThreads / (2 * w);
     const unsigned int need = blocks_for(n >> 1, nodes_per_step);
@@ -684,6 +781,12 @@ static void spmv_w(hipStream_t s, const PcgBuffers& b, const CsrView& a, const d
 
 void launch_pcg_spmv(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* vals64, const float* vals32,
                      int variant) {
+    if (variant == 0) variant = kDefaultVariant;
+    if (variant == 4) {
+        const unsigned int grid = static_cast<unsigned int>(spmv_blocks(b.n, 4));
+        pcg_spmv_stream_kernel<<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, vals64, vals32);
+        return;
+    }
     switch (variant_width(variant)) {
         case 4: spmv_w<4>(s, b, a, vals64, vals32); break;
         case 16: spmv_w<16>(s, b, a, vals64, vals32); break;
@@ -707,7 +810,7 @@ void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed) {
 
 void launch_pcg_replace(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* vals64) {
     constexpr int W = 8;
-    const unsigned int grid = 2048;
+    const unsigned int grid = 512;
     if (b.precond == LFG_PRECOND_BLOCK2)
         pcg_replace_kernel<W, LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, vals64);
     else
@@ -732,14 +835,11 @@ void launch_pcg_ic0_step(hipStream_t s, const PcgBuffers& b, const Ic0View& f) {
 }
 
 void launch_residual_norm(hipStream_t s, const CsrView& a, const double* vals64, const double* x, const double* bvec,
-                          double* work, double* out2) {
+                          double* partials, TicketSet* tickets, double* out2) {
     constexpr int W = 8;
-    const unsigned int need = blocks_for(a.n, kThreads / W);
+    const unsigned int need = round8(blocks_for(a.n, kThreads / W));
     const unsigned int grid = need < 2048u ? need : 2048u;
-    // work: partials in [0, 2048), the ticket word at double slot 4096 (zeroed once,
-    // reset by the last block).
-    unsigned int* ticket = reinterpret_cast<unsigned int*>(work + 4096);
-    residual_norm_kernel<W><<<grid, kThreads, 0, s>>>(a.n, a.row_ptr, a.col, vals64, x, bvec, work, ticket, out2);
+    residual_norm_kernel<W><<<grid, kThreads, 0, s>>>(a.n, a.row_ptr, a.col, vals64, x, bvec, partials, tickets, out2);
 }
 
 }  // namespace lfg

@@ -169,7 +169,9 @@ def test_mixed_precision_matches_fp64(dev, precond, switch):
     u64, r64 = dev.pcg(precond, "fp64", "relative", 1e-10)
     um, rm = dev.pcg(precond, "mixed", "relative", 1e-10, mixed_switch=switch)
     assert r64["converged"] and rm["converged"] and rm["residual_replacements"] >= 1
-    assert rm["true_residual"] <= 1.0e-10 * rm["initial_residual"]
+    # both forms stop on the recurrence residual; the true residual stays close to it
+    assert rm["true_residual"] <= 1.0e-9 * rm["initial_residual"]
+    assert r64["true_residual"] <= 1.0e-9 * r64["initial_residual"]
     A = scipy.sparse.csr_matrix((kc, sys_.col, sys_.row_ptr))
     exact = scipy.sparse.linalg.spsolve(A.tocsc(), fc)
     for u in (u64, um):
@@ -223,13 +225,14 @@ def test_full_size_assembly_properties(dev):
     scale = np.max(np.abs(k_serial))
     assert np.max(np.abs(k_col - k_serial)) <= 1e-13 * scale
     assert np.max(np.abs(k_at - k_serial)) <= 1e-13 * scale
-    # symmetry of the assembled matrix on a sample of rows
+    # symmetry of the assembled matrix on a sample of rows (Ke is symmetric to rounding:
+    # its (a,b) and (b,a) entries are summed in different orders, element.cpp:150-157)
     rows = np.arange(0, 2004002, 9973)
     for r in rows:
         for k in range(sys_.row_ptr[r], sys_.row_ptr[r + 1]):
             c = sys_.col[k]
             kk = sys_.row_ptr[c] + np.searchsorted(sys_.col[sys_.row_ptr[c]:sys_.row_ptr[c + 1]], r)
-            assert k_serial[k] == k_serial[kk]
+            assert abs(k_serial[k] - k_serial[kk]) <= 1e-14 * scale
     dev.set_stiffness(k_serial)
     dev.set_load(sys_.indexed.load)
     dev.dirichlet(sys_.indexed.cdofs, sys_.indexed.cvals)
