@@ -105,13 +105,17 @@ def main() -> int:
     spmv = {}
     for precision, vbytes in (("fp64", 8), ("mixed", 4)):
         best = None
-        for variant in (1, 2, 3, 4):
+        for variant in (1, 2, 3, 4, 5):
             ms = dev.bench_spmv(precision, variant, 50)
             if best is None or ms < best[1]:
                 best = (variant, ms)
         b = spmv_bytes(n, nnz, vbytes)
         spmv[precision] = {"variant": best[0], "ms": best[1], "bytes": b, "GBs": b / (best[1] * 1e-3) / 1e9}
     print(f"bench: spmv {json.dumps(spmv)}", file=sys.stderr, flush=True)
+    copy_bytes = 1 << 30
+    copy_ms = dev.bench_copy(copy_bytes, 20)
+    copy = {"bytes_per_copy": copy_bytes, "ms": copy_ms, "GBs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9}
+    print(f"bench: device copy {json.dumps(copy)}", file=sys.stderr, flush=True)
     if args.profile:
         for form in ("fp64", "mixed"):
             _, rep = dev.pcg("jacobi", form, "relative", args.tol, args.profile, args.check_every,
@@ -119,35 +123,37 @@ def main() -> int:
                              mixed_switch=-1.0 if form == "mixed" else 0.0)
             print(f"bench: profile {form} {rep['iterations']} iterations {rep['solve_ms']:.1f} ms", file=sys.stderr,
                   flush=True)
-        print(json.dumps({"profile_iterations": args.profile, "spmv": spmv}))
+        print(json.dumps({"profile_iterations": args.profile, "spmv": spmv, "copy": copy}))
         dev.close()
         return 0
     forms = {}
     cap = 10 ** 6
     for form in ("fp64", "mixed"):
         variant = args.variant or spmv[form]["variant"]
-        for _ in range(args.warmup):
+        steps = args.steps if form == "fp64" else 1  # the mixed form is reported, not the headline
+        warmup = args.warmup if form == "fp64" else 0
+        for _ in range(warmup):
             dev.pcg("jacobi", form, "relative", args.tol, cap, args.check_every, spmv_variant=variant,
                     raise_on_error=False)
             print(f"bench: {form} warm-up solve done", file=sys.stderr, flush=True)
         _sync()
         t = time.perf_counter()
         reps = []
-        for _ in range(args.steps):
+        for _ in range(steps):
             u, rep = dev.pcg("jacobi", form, "relative", args.tol, cap, args.check_every, spmv_variant=variant,
                              raise_on_error=False)
             reps.append(rep)
             print(f"bench: {form} solve {rep['iterations']} iterations, converged={rep['converged']}", file=sys.stderr,
                   flush=True)
         _sync()
+        wall = (time.perf_counter() - t) * 1e3 / steps
         if form == "fp64":
             cap = 2 * reps[-1]["iterations"]  # the mixed forms get at most twice the FP64 count
-        wall = (time.perf_counter() - t) * 1e3 / args.steps
         forms[form] = {"ms_per_solve": wall, "device_solve_ms": float(np.median([r["solve_ms"] for r in reps])),
                        "iterations": reps[-1]["iterations"], "converged": bool(reps[-1]["converged"]),
                        "replacements": reps[-1]["residual_replacements"],
                        "true_relative_residual": reps[-1]["true_residual"] / reps[-1]["initial_residual"],
-                       "spmv_variant": variant, "u": u}
+                       "spmv_variant": variant, "steps": steps, "u": u}
         print(f"bench: {form} {wall:.1f} ms/solve, {reps[-1]['iterations']} iterations", file=sys.stderr, flush=True)
     # FP32 values from the first iteration (residual replacement only), capped at 2x the FP64 count
     t = time.perf_counter()
@@ -162,7 +168,6 @@ def main() -> int:
     u64, umx = forms["fp64"].pop("u"), forms["mixed"].pop("u")
     forms["mixed"]["rel_u_diff_vs_fp64"] = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
     per_iter_ms = {k: v["device_solve_ms"] / max(1, v["iterations"]) for k, v in forms.items()}
-    iter_bytes64 = spmv_bytes(n, nnz, 8) + 112 * n  # SpMV + update (56n) + direction (32n) + p.q/q traffic
     cpu = None
     if not args.skip_cpu:
         try:
@@ -170,25 +175,32 @@ def main() -> int:
                                forms["fp64"]["iterations"], args.cpu_iters)
         except Exception as exc:  # reported, not fatal
             print(f"bench: cpu baseline failed: {exc}", file=sys.stderr)
-    value = forms["mixed"]["ms_per_solve"]
-    dominant = spmv["mixed"]
+    # Headline: the FP64 form, the one that reaches 1e-10 here. FP32 values stagnate above
+    # it on this system (reported in "forms"), so the mixed time is not a time-to-1e-10.
+    value = forms["fp64"]["ms_per_solve"]
+    dominant = spmv["fp64"]
+    mixed_ok = forms["mixed"]["converged"]
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32-values+f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"q4-cantilever-{n_mesh}x{n_mesh}-plane-strain-loguniformE-1e-3..1-seed"
                                f"{meshgen.SEED_LOGE}", "dofs": n, "nnz": nnz, "preconditioner": "jacobi",
-                   "stop_rule": f"true relative residual <= {args.tol:g}", "assembly": "coloring"},
+                   "stop_rule": f"recurrence relative residual ||r||/||b|| <= {args.tol:g}", "assembly": "coloring",
+                   "form": "fp64"},
         "fp64_ms": round(forms["fp64"]["ms_per_solve"], 3),
-        "mixed_speedup_vs_fp64": round(forms["fp64"]["ms_per_solve"] / value, 4),
+        "mixed_ms": round(forms["mixed"]["ms_per_solve"], 3) if mixed_ok else None,
+        "mixed_converged": mixed_ok,
+        "mixed_speedup_vs_fp64": round(forms["fp64"]["ms_per_solve"] / forms["mixed"]["ms_per_solve"], 4)
+        if mixed_ok else None,
         "forms": forms, "per_iteration_ms": per_iter_ms,
-        "iteration_bytes_fp64_est": iter_bytes64,
-        "spmv": spmv, "assembly": assembly, "setup_s": round(setup_s, 2),
-        "roofline": {"bound": "hbm", "kernel": "pcg_spmv_kernel<float values>",
+        "spmv": spmv, "device_copy": copy, "assembly": assembly, "setup_s": round(setup_s, 2),
+        "roofline": {"bound": "hbm", "kernel": "pcg_spmv_stream_kernel (FP64 values)",
                      "achieved": round(dominant["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dominant["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "fp64_spmv_GBs": round(spmv["fp64"]["GBs"], 1),
-                     "fp64_frac": round(spmv["fp64"]["GBs"] / HBM_PEAK_GBS, 4)},
+                     "bytes_per_launch": dominant["bytes"], "avg_launch_ms": dominant["ms"],
+                     "fp32_values_spmv_GBs": round(spmv["mixed"]["GBs"], 1),
+                     "copy_ceiling_GBs": round(copy["GBs"], 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -147,6 +147,10 @@ int lfg_spmv(lfg_ctx* ctx, int32_t precision, const double* x, double* y);
 /* Average device time (ms) of `reps` SpMV launches of the given precision and
  * variant on resident data. */
 int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t reps, double* avg_ms);
+/* Average device time (ms) of a device-to-device copy of `bytes` bytes: the
+ * streaming-bandwidth ceiling of this device, and the calibration kernel for
+ * hardware traffic counters. */
+int lfg_bench_copy(lfg_ctx* ctx, int64_t bytes, int32_t reps, double* avg_ms);
 
 #ifdef __cplusplus
 }

@@ -81,6 +81,7 @@ struct lfg_ctx {
 
     // pattern
     int n = 0;
+    int max_row = 0;
     long long nnz = 0;
     std::vector<int> h_row_ptr, h_col;
     DevArray<int> row_ptr, col, emap, cptr, contrib;
@@ -134,7 +135,7 @@ lfg::MeshView mesh_view(const lfg_ctx* c) {
     return lfg::MeshView{c->n_nodes, c->n_elems, c->xy.p, c->enodes.p, c->elem_mat.p, c->materials.p, c->plane_mode};
 }
 
-lfg::CsrView csr_view(const lfg_ctx* c) { return lfg::CsrView{c->n, c->nnz, c->row_ptr.p, c->col.p}; }
+lfg::CsrView csr_view(const lfg_ctx* c) { return lfg::CsrView{c->n, c->nnz, c->row_ptr.p, c->col.p, c->max_row}; }
 
 void sync(lfg_ctx* c) { LFG_HIP(hipStreamSynchronize(c->stream)); }
 
@@ -477,6 +478,11 @@ int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* ro
         ctx->n = static_cast<int>(n_rows);
         ctx->nnz = nnz;
         ctx->h_row_ptr.assign(row_ptr, row_ptr + n_rows + 1);
+        ctx->max_row = 0;
+        for (int64_t r = 0; r < n_rows; ++r) {
+            require(row_ptr[r + 1] >= row_ptr[r], LFG_ERR_ARG, "row offsets must be non-decreasing");
+            ctx->max_row = std::max(ctx->max_row, row_ptr[r + 1] - row_ptr[r]);
+        }
         ctx->h_col.assign(col, col + nnz);
         ctx->row_ptr.upload(row_ptr, static_cast<size_t>(n_rows) + 1, ctx->stream);
         ctx->col.upload(col, static_cast<size_t>(nnz), ctx->stream);
@@ -866,6 +872,33 @@ int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t rep
         sync(ctx);
         *avg_ms = elapsed(ctx->ev_start, ctx->ev_stop) / reps;
         ctx->have_solution = false;
+        return LFG_OK;
+    });
+}
+
+int lfg_bench_copy(lfg_ctx* ctx, int64_t bytes, int32_t reps, double* avg_ms) {
+    return guarded(ctx, [&] {
+        require(bytes > 0 && reps > 0 && avg_ms, LFG_ERR_ARG, "copy size and repetitions must be positive");
+        LFG_HIP(hipSetDevice(ctx->device));
+        void* src = nullptr;
+        void* dst = nullptr;
+        LFG_HIP(hipMalloc(&src, static_cast<size_t>(bytes)));
+        struct Free {
+            void* p;
+            ~Free() {
+                if (p) (void)hipFree(p);
+            }
+        } free_src{src};
+        LFG_HIP(hipMalloc(&dst, static_cast<size_t>(bytes)));
+        Free free_dst{dst};
+        LFG_HIP(hipMemsetAsync(src, 1, static_cast<size_t>(bytes), ctx->stream));
+        LFG_HIP(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToDevice, ctx->stream));
+        LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+        for (int i = 0; i < reps; ++i)
+            LFG_HIP(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToDevice, ctx->stream));
+        LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+        sync(ctx);
+        *avg_ms = elapsed(ctx->ev_start, ctx->ev_stop) / reps;
         return LFG_OK;
     });
 }

@@ -24,6 +24,7 @@ struct CsrView {
     long long nnz;
     const int* row_ptr;
     const int* col;
+    int max_row;  // longest row (entries)
 };
 
 // ---- element kernels (lfg_element.hip)

@@ -333,13 +333,16 @@ __global__ __launch_bounds__(kThreads) void pcg_spmv_kernel(PcgBuffers b, const 
 // and column loads are issued before the gathers of p, the products are staged in
 // LDS, and lane l then sums row l of the wave. Rows longer than the staging room
 // (kSlots per lane on average) fall back to 8 lanes per row.
-constexpr int kStreamRows = 64;
-constexpr int kSlots = 18;
+// Rows per wave chunk: 64 in variant 4, 32 in variant 5.
+constexpr int slots_for(int rows) { return (rows * 18 + kWave - 1) / kWave; }
 
-template <typename VT>
+// kMaxRow > 0: every row has at most kMaxRow entries (checked on the host), so
+// the row sums are unrolled with clamped, unconditional LDS reads; 0: generic loop.
+template <typename VT, int kMaxRow, int kRows>
 __device__ __forceinline__ double stream_rows(const PcgBuffers& b, const int* __restrict__ rp,
                                               const int* __restrict__ col, const VT* __restrict__ val, int row0,
                                               int row_end, double* __restrict__ stage) {
+    constexpr int kSlots = slots_for(kRows);
     const int lane = threadIdx.x & (kWave - 1);
     const int seg = rp[row0];
     const int len = rp[row_end] - seg;
@@ -355,21 +358,22 @@ __device__ __forceinline__ double stream_rows(const PcgBuffers& b, const int* __
         }
         return pq;
     }
+    const int row = row0 + lane;
+    const int my_row = min(row, row_end - 1);
+    const int a = rp[my_row] - seg, e = rp[my_row + 1] - seg;
+    // Loads are unconditional (indices clamped into the segment) so the compiler
+    // issues all of them back to back; only the LDS stores are predicated.
     VT v[kSlots];
     int c[kSlots];
 #pragma unroll
     for (int j = 0; j < kSlots; ++j) {
-        const int k = j * kWave + lane;
-        if (k < len) {
-            v[j] = val[seg + k];
-            c[j] = col[seg + k];
-        }
+        const int k = min(j * kWave + lane, len - 1);
+        v[j] = __builtin_nontemporal_load(val + seg + k);
+        c[j] = __builtin_nontemporal_load(col + seg + k);
     }
     double x[kSlots];
 #pragma unroll
-    for (int j = 0; j < kSlots; ++j) {
-        if (j * kWave + lane < len) x[j] = b.p[c[j]];
-    }
+    for (int j = 0; j < kSlots; ++j) x[j] = b.p[c[j]];
 #pragma unroll
     for (int j = 0; j < kSlots; ++j) {
         const int k = j * kWave + lane;
@@ -378,11 +382,17 @@ __device__ __forceinline__ double stream_rows(const PcgBuffers& b, const int* __
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int row = row0 + lane;
-    if (row < row_end) {
-        const int a = rp[row] - seg, e = rp[row + 1] - seg;
-        double sum = 0.0;
+    double sum = 0.0;
+    if (kMaxRow > 0) {
+#pragma unroll
+        for (int t = 0; t < kMaxRow; ++t) {
+            const double term = stage[min(a + t, e - 1)];
+            sum += (a + t < e) ? term : 0.0;
+        }
+    } else {
         for (int k = a; k < e; ++k) sum += stage[k];
+    }
+    if (lane < kRows && row < row_end) {
         b.q[row] = sum;
         pq = b.p[row] * sum;
     }
@@ -390,35 +400,36 @@ __device__ __forceinline__ double stream_rows(const PcgBuffers& b, const int* __
     return pq;
 }
 
-template <typename VT>
+template <typename VT, int kMaxRow, int kRows>
 __device__ __forceinline__ double stream_range(const PcgBuffers& b, const int* __restrict__ rp,
                                                const int* __restrict__ col, const VT* __restrict__ val,
                                                int row_start, int row_stop, double* stage) {
     const int wave = threadIdx.x / kWave;
     constexpr int kWaves = kThreads / kWave;
     double pq = 0.0;
-    for (int r0 = row_start + wave * kStreamRows; r0 < row_stop; r0 += kWaves * kStreamRows)
-        pq += stream_rows<VT>(b, rp, col, val, r0, min(r0 + kStreamRows, row_stop), stage);
+    for (int r0 = row_start + wave * kRows; r0 < row_stop; r0 += kWaves * kRows)
+        pq += stream_rows<VT, kMaxRow, kRows>(b, rp, col, val, r0, min(r0 + kRows, row_stop), stage);
     return pq;
 }
 
+template <int kMaxRow, int kRows>
 __global__ __launch_bounds__(kThreads) void pcg_spmv_stream_kernel(PcgBuffers b, const int* __restrict__ rp,
                                                                    const int* __restrict__ col,
                                                                    const double* __restrict__ val64,
                                                                    const float* __restrict__ val32) {
     if (b.st->done) return;
     __shared__ double sh[kThreads / kWave];
-    __shared__ double stage_all[kThreads / kWave][kWave * kSlots];
+    __shared__ double stage_all[kThreads / kWave][kWave * slots_for(kRows)];
     double* stage = stage_all[threadIdx.x / kWave];
     const bool use32 = val32 != nullptr && (val64 == nullptr || b.st->phase32 != 0);
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     // contiguous row range per block, in whole waves' worth of rows (rows stay node-aligned)
-    const int chunks = (b.n + kStreamRows - 1) / kStreamRows;
+    const int chunks = (b.n + kRows - 1) / kRows;
     const int per_block = (chunks + gridDim.x - 1) / gridDim.x;
-    const int start = min(lb * per_block * kStreamRows, b.n);
-    const int stop = min(start + per_block * kStreamRows, b.n);
-    const double pq = use32 ? stream_range<float>(b, rp, col, val32, start, stop, stage)
-                            : stream_range<double>(b, rp, col, val64, start, stop, stage);
+    const int start = min(lb * per_block * kRows, b.n);
+    const int stop = min(start + per_block * kRows, b.n);
+    const double pq = use32 ? stream_range<float, kMaxRow, kRows>(b, rp, col, val32, start, stop, stage)
+                            : stream_range<double, kMaxRow, kRows>(b, rp, col, val64, start, stop, stage);
     const double vals[1] = {block_sum<kThreads>(pq, sh)};
     double tot[1];
     if (!finish_reduction<kThreads, 1>(b.tickets + 1, b.partials, vals, tot, sh)) return;
@@ -687,7 +698,7 @@ __global__ __launch_bounds__(kThreads) void residual_norm_kernel(int n, const in
 }
 
 // SpMV variants: 1, 2, 3 -> node groups with 4, 8, 16 lanes per row; 4 (the
-// default) -> row-segment streaming.
+// default) and 5 -> row-segment streaming with 64 / 32 rows per wave chunk.
 constexpr int kDefaultVariant = 4;
 
 inline int variant_width(int variant) {
@@ -747,8 +758,9 @@ constexpr unsigned int kSpmvBlocks = 2048;
 
 int spmv_blocks(int n, int variant) {
     if (variant == 0) variant = kDefaultVariant;
-    if (variant == 4) {
-        const unsigned int need = blocks_for((n + kStreamRows - 1) / kStreamRows, kThreads / kWave);
+    if (variant == 4 || variant == 5) {
+        const int rows = variant == 4 ? 64 : 32;
+        const unsigned int need = blocks_for((n + rows - 1) / rows, kThreads / kWave);
         return static_cast<int>(round8(need < kSpmvBlocks ? need : kSpmvBlocks));
     }
     const int w = variant_width(variant);
@@ -782,9 +794,19 @@ static void spmv_w(hipStream_t s, const PcgBuffers& b, const CsrView& a, const d
 void launch_pcg_spmv(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* vals64, const float* vals32,
                      int variant) {
     if (variant == 0) variant = kDefaultVariant;
-    if (variant == 4) {
-        const unsigned int grid = static_cast<unsigned int>(spmv_blocks(b.n, 4));
-        pcg_spmv_stream_kernel<<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, vals64, vals32);
+    if (variant == 4 || variant == 5) {
+        const unsigned int grid = static_cast<unsigned int>(spmv_blocks(b.n, variant));
+        if (variant == 4) {
+            if (a.max_row <= 24)
+                pcg_spmv_stream_kernel<24, 64><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, vals64, vals32);
+            else
+                pcg_spmv_stream_kernel<0, 64><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, vals64, vals32);
+        } else {
+            if (a.max_row <= 24)
+                pcg_spmv_stream_kernel<24, 32><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, vals64, vals32);
+            else
+                pcg_spmv_stream_kernel<0, 32><<<grid, kThreads, 0, s>>>(b, a.row_ptr, a.col, vals64, vals32);
+        }
         return;
     }
     switch (variant_width(variant)) {

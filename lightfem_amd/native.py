@@ -26,7 +26,7 @@ STOP = {"reference": 0, "relative": 1}
 GPU_SYMBOLS = ["lfg_create", "lfg_destroy", "lfg_last_error", "lfg_status_string", "lfg_set_mesh", "lfg_set_pattern",
                "lfg_set_colors", "lfg_element_stiffness", "lfg_assemble", "lfg_get_stiffness", "lfg_set_stiffness",
                "lfg_set_load", "lfg_dirichlet", "lfg_get_system", "lfg_pcg", "lfg_reactions", "lfg_postprocess",
-               "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv"]
+               "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv", "lfg_bench_copy"]
 HOST_SYMBOLS = ["lfh_csr_pattern_nnz", "lfh_csr_pattern", "lfh_rcm", "lfh_element_colors", "lfh_csr_profile"]
 
 
@@ -104,6 +104,7 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lfg_spmv.argtypes = [_P, ctypes.c_int32, _P, _P]
         lib.lfg_bench_spmv.argtypes = [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.POINTER(ctypes.c_double)]
+        lib.lfg_bench_copy.argtypes = [_P, _I64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
         _gpu = lib
     return _gpu
 
@@ -314,6 +315,11 @@ class Device:
         y = np.empty_like(x)
         self._check(self._lib.lfg_spmv(self._ctx, PRECISION[precision], _ptr(x), _ptr(y)))
         return y
+
+    def bench_copy(self, nbytes: int, reps: int = 20) -> float:
+        ms = ctypes.c_double(0.0)
+        self._check(self._lib.lfg_bench_copy(self._ctx, nbytes, reps, ctypes.byref(ms)))
+        return ms.value
 
     def bench_spmv(self, precision: str = "fp64", variant: int = 0, reps: int = 20) -> float:
         ms = ctypes.c_double(0.0)
