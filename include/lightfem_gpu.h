@@ -63,7 +63,10 @@ typedef struct {
     double mixed_switch;     /* mixed: SpMVs use FP64 values until ||r|| <= mixed_switch*||r0||, then FP32
                                 values; 0 = default 1e-4, negative = FP32 values from the first iteration */
     int32_t direct_launch;   /* 1: launch each kernel directly instead of replaying a captured hipGraph */
-    int32_t spmv_variant;    /* 0 = autotuned/default */
+    int32_t spmv_variant;    /* 0 = default (6 when the pattern is node-blocked, else 4); 1-3: node
+                                groups of 4/8/16 lanes per row; 4, 5: row-segment streaming with
+                                64 / 32 rows per wavefront; 6: node-slice layout (one column per
+                                2x2 block, 64 nodes per wavefront) */
 } lfg_pcg_options;
 
 typedef struct {
@@ -143,7 +146,8 @@ int lfg_postprocess(lfg_ctx* ctx, double* centre, double* gauss_points, double* 
 
 /* Checks and measurement hooks on the constrained system. */
 int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, double* out);
-int lfg_spmv(lfg_ctx* ctx, int32_t precision, const double* x, double* y);
+/* y = Kc x with the PCG SpMV kernel of `variant` (values of `precision`). */
+int lfg_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x, double* y);
 /* Average device time (ms) of `reps` SpMV launches of the given precision and
  * variant on resident data. */
 int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t reps, double* avg_ms);

@@ -86,6 +86,13 @@ struct lfg_ctx {
     std::vector<int> h_row_ptr, h_col;
     DevArray<int> row_ptr, col, emap, cptr, contrib;
     bool have_pattern = false;
+    // node-slice copy of the constrained values (SpMV variant 6), when the pattern is node-blocked
+    bool have_slices = false, slices64_valid = false, slices32_valid = false;
+    int n_slices = 0, slice_max_width = 0;
+    long long slice_blocks = 0;
+    DevArray<int> slice_ptr, slice_col;
+    DevArray<double> slices64;
+    DevArray<float> slices32;
     std::vector<int> h_color_ptr;
     DevArray<int> color_elems;
 
@@ -136,6 +143,75 @@ lfg::MeshView mesh_view(const lfg_ctx* c) {
 }
 
 lfg::CsrView csr_view(const lfg_ctx* c) { return lfg::CsrView{c->n, c->nnz, c->row_ptr.p, c->col.p, c->max_row}; }
+
+lfg::SliceView slice_view(const lfg_ctx* c) {
+    return lfg::SliceView{c->n_slices, c->slice_blocks, c->slice_ptr.p, c->slice_col.p, c->slice_max_width};
+}
+
+constexpr int kSliceVariant = 6;
+
+// 0 selects the node-slice kernel where the pattern allows it, else row-segment streaming.
+int resolve_variant(const lfg_ctx* c, int variant) {
+    if (variant == 0) return c->have_slices ? kSliceVariant : 4;
+    require(variant >= 1 && variant <= kSliceVariant, LFG_ERR_ARG, "SpMV variant must be 0-6");
+    require(variant != kSliceVariant || c->have_slices, LFG_ERR_UNSUPPORTED,
+            "the node-slice SpMV needs rows 2i, 2i+1 sharing one column list of dof pairs");
+    return variant;
+}
+
+// Node-blocked patterns (every two-dof-per-node pattern): builds the slice offsets
+// on the host and the node columns on the device.
+void build_slices(lfg_ctx* c) {
+    c->have_slices = false;
+    c->slices64_valid = c->slices32_valid = false;
+    const int n_nodes = c->n / 2;
+    const std::vector<int>& rp = c->h_row_ptr;
+    const std::vector<int>& col = c->h_col;
+    std::vector<int> deg(static_cast<size_t>(n_nodes));
+    for (int i = 0; i < n_nodes; ++i) {
+        const int r0 = rp[2 * i], r1 = rp[2 * i + 1], r2 = rp[2 * i + 2];
+        const int len = r1 - r0;
+        if (len != r2 - r1 || (len & 1)) return;
+        for (int k = 0; k < len; k += 2) {
+            const int c0 = col[r0 + k];
+            if ((c0 & 1) || col[r0 + k + 1] != c0 + 1 || col[r1 + k] != c0 || col[r1 + k + 1] != c0 + 1) return;
+        }
+        deg[i] = len / 2;
+    }
+    const int n_slices = (n_nodes + 63) / 64;
+    std::vector<int> ptr(static_cast<size_t>(n_slices) + 1, 0);
+    long long total = 0;
+    int max_width = 0;
+    for (int s = 0; s < n_slices; ++s) {
+        int w = 0;
+        for (int i = s * 64; i < std::min(n_nodes, s * 64 + 64); ++i) w = std::max(w, deg[i]);
+        max_width = std::max(max_width, w);
+        total += 64LL * w;
+        if (total >= (1LL << 31)) return;  // offsets are int32
+        ptr[s + 1] = static_cast<int>(total);
+    }
+    c->n_slices = n_slices;
+    c->slice_blocks = total;
+    c->slice_max_width = max_width;
+    c->slice_ptr.upload(ptr.data(), ptr.size(), c->stream);
+    c->slice_col.alloc(static_cast<size_t>(total));
+    lfg::launch_pack_slices(c->stream, csr_view(c), slice_view(c), nullptr, nullptr, nullptr, c->slice_col.p);
+    c->have_slices = true;
+}
+
+// Packs the constrained values into the slice planes that the next SpMVs read.
+void ensure_slices(lfg_ctx* c, bool need64, bool need32) {
+    if (need64 && !c->slices64_valid) {
+        c->slices64.alloc(4 * static_cast<size_t>(c->slice_blocks));
+        lfg::launch_pack_slices(c->stream, csr_view(c), slice_view(c), c->Kc.p, c->slices64.p, nullptr, nullptr);
+        c->slices64_valid = true;
+    }
+    if (need32 && !c->slices32_valid) {
+        c->slices32.alloc(4 * static_cast<size_t>(c->slice_blocks));
+        lfg::launch_pack_slices(c->stream, csr_view(c), slice_view(c), c->Kc.p, nullptr, c->slices32.p, nullptr);
+        c->slices32_valid = true;
+    }
+}
 
 void sync(lfg_ctx* c) { LFG_HIP(hipStreamSynchronize(c->stream)); }
 
@@ -367,7 +443,10 @@ lfg::PcgBuffers pcg_buffers(lfg_ctx* c, int kind) {
 
 void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int variant) {
     const lfg::CsrView a = csr_view(c);
-    lfg::launch_pcg_spmv(c->stream, b, a, c->Kc.p, mixed ? c->K32.p : nullptr, variant);
+    if (variant == kSliceVariant)
+        lfg::launch_pcg_spmv_slices(c->stream, b, slice_view(c), c->slices64.p, mixed ? c->slices32.p : nullptr);
+    else
+        lfg::launch_pcg_spmv(c->stream, b, a, c->Kc.p, mixed ? c->K32.p : nullptr, variant);
     lfg::launch_pcg_update(c->stream, b, mixed);
     if (mixed) lfg::launch_pcg_replace(c->stream, b, a, c->Kc.p);
     if (b.precond == LFG_PRECOND_IC0) lfg::launch_pcg_ic0_step(c->stream, b, ic0_view(c));
@@ -488,6 +567,7 @@ int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* ro
         ctx->col.upload(col, static_cast<size_t>(nnz), ctx->stream);
         ctx->emap.upload(emap, 64 * static_cast<size_t>(ctx->n_elems), ctx->stream);
         build_gather_map(ctx, emap);
+        build_slices(ctx);
         ctx->K.alloc(static_cast<size_t>(nnz));
         ctx->have_pattern = true;
         ctx->have_K = ctx->have_system = ctx->have_solution = false;
@@ -643,6 +723,7 @@ int lfg_dirichlet(lfg_ctx* ctx, int64_t n_constraints, const int32_t* dofs, cons
         ctx->have_system = true;
         ctx->have_solution = false;
         ctx->k32_valid = false;
+        ctx->slices64_valid = ctx->slices32_valid = false;
         ctx->precond_ready = -1;
         return LFG_OK;
     });
@@ -679,7 +760,10 @@ int lfg_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, double* u_out, lfg_pcg_rep
         const int previous = ctx->precond_ready;
         setup_preconditioner(ctx, kind, &degenerate);
         if (previous == kind && kind == LFG_PRECOND_BLOCK2) degenerate = -1;  // factor reused; count unknown here
-        if (mixed && !ctx->k32_valid) {
+        const int variant = resolve_variant(ctx, opt->spmv_variant);
+        if (variant == kSliceVariant) {
+            ensure_slices(ctx, true, mixed != 0);
+        } else if (mixed && !ctx->k32_valid) {
             ctx->K32.alloc(static_cast<size_t>(ctx->nnz));
             lfg::launch_to_f32(ctx->stream, ctx->Kc.p, ctx->K32.p, ctx->nnz);
             ctx->k32_valid = true;
@@ -694,7 +778,6 @@ int lfg_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, double* u_out, lfg_pcg_rep
         }
 
         const lfg::PcgBuffers b = pcg_buffers(ctx, kind);
-        const int variant = opt->spmv_variant;
         const int per_graph = opt->check_every > 0 ? opt->check_every : 32;
         const double drop = opt->replace_drop > 0.0 ? opt->replace_drop : 0.1;
         const double switch_tau = !mixed ? 0.0 : (opt->mixed_switch != 0.0 ? opt->mixed_switch : 1.0e-4);
@@ -723,7 +806,8 @@ int lfg_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, double* u_out, lfg_pcg_rep
             lfg::launch_pcg_ic0_step(ctx->stream, b, ic0_view(ctx));
             lfg::launch_pcg_direction(ctx->stream, b);
         }
-        const long long launch_cap = opt->max_iterations / per_graph + 2;
+        // every iteration the cap allows fits in this many graph launches
+        const long long launch_cap = std::max<long long>(1, (opt->max_iterations + per_graph - 1) / per_graph);
         long long launched = 0;
         auto launch_one = [&](int slot) {
             if (use_graph) {
@@ -830,28 +914,36 @@ int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, do
 static void run_spmv(lfg_ctx* ctx, int precision, int variant) {
     lfg::PcgBuffers b = pcg_buffers(ctx, LFG_PRECOND_JACOBI);
     b.st = ctx->state.p;
-    lfg::launch_pcg_spmv(ctx->stream, b, csr_view(ctx), precision == LFG_PRECISION_MIXED ? nullptr : ctx->Kc.p,
-                         precision == LFG_PRECISION_MIXED ? ctx->K32.p : nullptr, variant);  // one value array
+    const bool f32 = precision == LFG_PRECISION_MIXED;  // one value array per launch
+    if (variant == kSliceVariant)
+        lfg::launch_pcg_spmv_slices(ctx->stream, b, slice_view(ctx), f32 ? nullptr : ctx->slices64.p,
+                                    f32 ? ctx->slices32.p : nullptr);
+    else
+        lfg::launch_pcg_spmv(ctx->stream, b, csr_view(ctx), f32 ? nullptr : ctx->Kc.p, f32 ? ctx->K32.p : nullptr,
+                             variant);
 }
 
-static void prepare_spmv(lfg_ctx* ctx, int precision) {
+static void prepare_spmv(lfg_ctx* ctx, int precision, int variant) {
     ensure_vectors(ctx);
     lfg::PcgState zero{};
     LFG_HIP(hipMemcpyAsync(ctx->state.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
-    if (precision == LFG_PRECISION_MIXED && !ctx->k32_valid) {
+    if (variant == kSliceVariant) {
+        ensure_slices(ctx, precision != LFG_PRECISION_MIXED, precision == LFG_PRECISION_MIXED);
+    } else if (precision == LFG_PRECISION_MIXED && !ctx->k32_valid) {
         ctx->K32.alloc(static_cast<size_t>(ctx->nnz));
         lfg::launch_to_f32(ctx->stream, ctx->Kc.p, ctx->K32.p, ctx->nnz);
         ctx->k32_valid = true;
     }
 }
 
-int lfg_spmv(lfg_ctx* ctx, int32_t precision, const double* x, double* y) {
+int lfg_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x, double* y) {
     return guarded(ctx, [&] {
         require(ctx->have_system && x && y, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_spmv");
         LFG_HIP(hipSetDevice(ctx->device));
-        prepare_spmv(ctx, precision);
+        const int v = resolve_variant(ctx, variant);
+        prepare_spmv(ctx, precision, v);
         LFG_HIP(hipMemcpyAsync(ctx->p.p, x, ctx->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        run_spmv(ctx, precision, 0);
+        run_spmv(ctx, precision, v);
         ctx->q.download(y, static_cast<size_t>(ctx->n), ctx->stream);
         sync(ctx);
         ctx->have_solution = false;
@@ -863,11 +955,12 @@ int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t rep
     return guarded(ctx, [&] {
         require(ctx->have_system && reps > 0 && avg_ms, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_bench_spmv");
         LFG_HIP(hipSetDevice(ctx->device));
-        prepare_spmv(ctx, precision);
+        const int v = resolve_variant(ctx, variant);
+        prepare_spmv(ctx, precision, v);
         lfg::launch_fill(ctx->stream, ctx->p.p, ctx->n, 1.0);
-        run_spmv(ctx, precision, variant);  // warm-up
+        run_spmv(ctx, precision, v);  // warm-up
         LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-        for (int i = 0; i < reps; ++i) run_spmv(ctx, precision, variant);
+        for (int i = 0; i < reps; ++i) run_spmv(ctx, precision, v);
         LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
         sync(ctx);
         *avg_ms = elapsed(ctx->ev_start, ctx->ev_stop) / reps;

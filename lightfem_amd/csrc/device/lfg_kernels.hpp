@@ -27,6 +27,23 @@ struct CsrView {
     int max_row;  // longest row (entries)
 };
 
+// Node-slice layout of a node-blocked CSR matrix (rows 2i and 2i+1 share one
+// column list made of dof pairs 2j, 2j+1, as every two-dof-per-node pattern is):
+// nodes are cut into slices of 64 (one wavefront), and slice s stores width_s x 64
+// 2x2 blocks column-major, entry k of node 64s + l at block ptr[s] + 64k + l, in
+// the row's column order. Each block holds a node column and two value pairs,
+// upper = (K[2i][2j], K[2i][2j+1]) in the first plane and lower =
+// (K[2i+1][2j], K[2i+1][2j+1]) in the second; padding blocks are zero and point
+// at their own node. One int32 column per four values, and every load of a
+// wavefront is 64 consecutive blocks.
+struct SliceView {
+    int n_slices;
+    long long blocks;  // padded blocks, ptr[n_slices]
+    const int* ptr;    // n_slices + 1 block offsets (multiples of 64)
+    const int* col;    // node column per block
+    int max_width;
+};
+
 // ---- element kernels (lfg_element.hip)
 enum KeMode { KE_STORE = 0, KE_ATOMIC = 1, KE_COLORED = 2 };
 // Ke for elements [0, count) of `elems` (or all when elems == nullptr). STORE writes
@@ -75,6 +92,15 @@ void launch_pcg_init(hipStream_t s, const PcgBuffers& b, int stop_rule, double t
 // both, the kernel reads the FP32 values once the state is in the FP32 phase.
 void launch_pcg_spmv(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* vals64, const float* vals32,
                      int variant);
+// Packs the CSR `values` into the slice planes (out64: 4 * blocks doubles, out32:
+// 4 * blocks floats, either may be null) and, with col_out, the node columns.
+void launch_pack_slices(hipStream_t s, const CsrView& a, const SliceView& v, const double* values, double* out64,
+                        float* out32, int* col_out);
+// q = A p on the slice layout; finishes p.q and alpha like launch_pcg_spmv. Each
+// row is summed in column order, as the CSR row loop does.
+void launch_pcg_spmv_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
+                            const float* planes32);
+int slice_spmv_blocks(int n_slices);
 // x += alpha p, r -= alpha q, ||r||^2, r.z; convergence / replacement decision, beta.
 void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed);
 // mixed only: r = b - K x in FP64 when a replacement is due; then r.z and beta.

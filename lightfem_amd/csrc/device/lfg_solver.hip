@@ -446,6 +446,166 @@ __global__ __launch_bounds__(kThreads) void pcg_spmv_stream_kernel(PcgBuffers b,
     }
 }
 
+// ------------------------------------------------------------ node-slice SpMV
+
+typedef double dpair __attribute__((ext_vector_type(2)));
+typedef float fpair __attribute__((ext_vector_type(2)));
+
+template <typename VT>
+struct PairOf;
+template <>
+struct PairOf<double> {
+    using type = dpair;
+};
+template <>
+struct PairOf<float> {
+    using type = fpair;
+};
+
+// One thread per (slice, lane): copies node 64s + l's blocks from its two CSR rows
+// (entries 2k, 2k+1 of each row form block k) into the planes; padding is zero.
+__global__ __launch_bounds__(256) void pack_slices_kernel(int n_nodes, int n_slices, long long blocks,
+                                                          const int* __restrict__ rp, const int* __restrict__ col,
+                                                          const int* __restrict__ ptr,
+                                                          const double* __restrict__ values, double* __restrict__ out64,
+                                                          float* __restrict__ out32, int* __restrict__ col_out) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int s = static_cast<int>(t >> 6);
+    const int lane = static_cast<int>(t & (kWave - 1));
+    if (s >= n_slices) return;
+    const int node = s * kWave + lane;
+    const int begin = ptr[s];
+    const int width = (ptr[s + 1] - begin) / kWave;
+    int r0 = 0, r1 = 0, deg = 0;
+    if (node < n_nodes) {
+        r0 = rp[2 * node];
+        r1 = rp[2 * node + 1];
+        deg = (r1 - r0) >> 1;
+    }
+    const int self = node < n_nodes ? node : n_nodes - 1;
+    for (int k = 0; k < width; ++k) {
+        const long long at = begin + static_cast<long long>(k) * kWave + lane;
+        double u0 = 0.0, u1 = 0.0, l0 = 0.0, l1 = 0.0;
+        int c = self;
+        if (k < deg) {
+            c = col[r0 + 2 * k] >> 1;
+            if (values) {
+                u0 = values[r0 + 2 * k];
+                u1 = values[r0 + 2 * k + 1];
+                l0 = values[r1 + 2 * k];
+                l1 = values[r1 + 2 * k + 1];
+            }
+        }
+        if (out64) {
+            out64[2 * at] = u0;
+            out64[2 * at + 1] = u1;
+            out64[2 * (blocks + at)] = l0;
+            out64[2 * (blocks + at) + 1] = l1;
+        }
+        if (out32) {
+            out32[2 * at] = static_cast<float>(u0);
+            out32[2 * at + 1] = static_cast<float>(u1);
+            out32[2 * (blocks + at)] = static_cast<float>(l0);
+            out32[2 * (blocks + at) + 1] = static_cast<float>(l1);
+        }
+        if (col_out) col_out[at] = c;
+    }
+}
+
+// Rows 2i, 2i+1 of node i = 64s + lane: every load instruction of the wave moves 64
+// consecutive blocks of a plane (or columns), p is gathered as one pair per block,
+// and each row accumulates its products in column order from 0.0, exactly as the
+// serial CSR row loop (sparse.cpp:104-120). kW > 0: slices up to kW wide take the
+// unrolled path, whose loads are unconditional (indices clamped to the last
+// block) and whose accumulation skips the clamped entries.
+template <typename VT, int kW>
+__device__ __forceinline__ double slice_rows(const PcgBuffers& b, const SliceView& v, const VT* __restrict__ planes,
+                                             int s) {
+    using P = typename PairOf<VT>::type;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int node = s * kWave + lane;
+    const int begin = v.ptr[s];
+    const int width = (v.ptr[s + 1] - begin) / kWave;
+    const P* __restrict__ upper = reinterpret_cast<const P*>(planes);
+    const P* __restrict__ lower = upper + v.blocks;
+    const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(b.p);
+    double a0 = 0.0, a1 = 0.0;
+    if (kW > 0 && width > 0 && width <= kW) {
+        P u[kW > 0 ? kW : 1], l[kW > 0 ? kW : 1];
+        int c[kW > 0 ? kW : 1];
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            const int at = begin + min(k, width - 1) * kWave + lane;
+            u[k] = __builtin_nontemporal_load(upper + at);
+            l[k] = __builtin_nontemporal_load(lower + at);
+            c[k] = __builtin_nontemporal_load(v.col + at);
+        }
+        dpair x[kW > 0 ? kW : 1];
+#pragma unroll
+        for (int k = 0; k < kW; ++k) x[k] = p2[c[k]];
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            if (k < width) {
+                a0 += static_cast<double>(u[k].x) * x[k].x;
+                a0 += static_cast<double>(u[k].y) * x[k].y;
+                a1 += static_cast<double>(l[k].x) * x[k].x;
+                a1 += static_cast<double>(l[k].y) * x[k].y;
+            }
+        }
+    } else {
+        for (int k = 0; k < width; ++k) {
+            const int at = begin + k * kWave + lane;
+            const P uk = __builtin_nontemporal_load(upper + at);
+            const P lk = __builtin_nontemporal_load(lower + at);
+            const dpair xk = p2[__builtin_nontemporal_load(v.col + at)];
+            a0 += static_cast<double>(uk.x) * xk.x;
+            a0 += static_cast<double>(uk.y) * xk.y;
+            a1 += static_cast<double>(lk.x) * xk.x;
+            a1 += static_cast<double>(lk.y) * xk.y;
+        }
+    }
+    double pq = 0.0;
+    if (node < (b.n >> 1)) {
+        const dpair qv = {a0, a1};
+        reinterpret_cast<dpair*>(b.q)[node] = qv;
+        const dpair pv = p2[node];
+        pq = pv.x * a0 + pv.y * a1;
+    }
+    return pq;
+}
+
+// q = A p over the node-slice layout, fused p.q and alpha = rho / p.q. Each block
+// owns a contiguous slice range (XCD-remapped) and its waves take one slice each.
+template <int kW>
+__global__ __launch_bounds__(kThreads) void pcg_spmv_slices_kernel(PcgBuffers b, SliceView v,
+                                                                   const double* __restrict__ planes64,
+                                                                   const float* __restrict__ planes32) {
+    if (b.st->done) return;
+    __shared__ double sh[kThreads / kWave];
+    const bool use32 = planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = (v.n_slices + gridDim.x - 1) / gridDim.x;
+    const int start = lb * per_block;
+    const int stop = min(start + per_block, v.n_slices);
+    double pq = 0.0;
+    for (int s = start + static_cast<int>(threadIdx.x) / kWave; s < stop; s += kThreads / kWave)
+        pq += use32 ? slice_rows<float, kW>(b, v, planes32, s) : slice_rows<double, kW>(b, v, planes64, s);
+    const double vals[1] = {block_sum<kThreads>(pq, sh)};
+    double tot[1];
+    if (!finish_reduction<kThreads, 1>(b.tickets + 1, b.partials, vals, tot, sh)) return;
+    if (threadIdx.x == 0) {
+        const double tpq = tot[0];
+        PcgState* st = b.st;
+        st->pq = tpq;
+        if (!(tpq > 0.0) || !isfinite(tpq)) {  // solver.cpp:591-593
+            st->status = LFG_ERR_NOT_SPD;
+            st->done = 1;
+        } else {
+            st->alpha = st->rho / tpq;
+        }
+    }
+}
+
 // Finishes an iteration after r.z is known (solver.cpp:614-621 checks).
 __device__ __forceinline__ void accept_rz(PcgState* st, double rz) {
     if (!(rz > 0.0) || !isfinite(rz)) {
@@ -814,6 +974,29 @@ void launch_pcg_spmv(hipStream_t s, const PcgBuffers& b, const CsrView& a, const
         case 16: spmv_w<16>(s, b, a, vals64, vals32); break;
         default: spmv_w<8>(s, b, a, vals64, vals32); break;
     }
+}
+
+void launch_pack_slices(hipStream_t s, const CsrView& a, const SliceView& v, const double* values, double* out64,
+                        float* out32, int* col_out) {
+    if (v.n_slices <= 0) return;
+    pack_slices_kernel<<<blocks_for(static_cast<long long>(v.n_slices) * kWave, 256), 256, 0, s>>>(
+        a.n >> 1, v.n_slices, v.blocks, a.row_ptr, a.col, v.ptr, values, out64, out32, col_out);
+}
+
+int slice_spmv_blocks(int n_slices) {
+    const unsigned int need = blocks_for(n_slices, kThreads / kWave);
+    return static_cast<int>(round8(need < 4096u ? need : 4096u));
+}
+
+void launch_pcg_spmv_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
+                            const float* planes32) {
+    const unsigned int grid = static_cast<unsigned int>(slice_spmv_blocks(v.n_slices));
+    if (v.max_width <= 9)
+        pcg_spmv_slices_kernel<9><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32);
+    else if (v.max_width <= 16)
+        pcg_spmv_slices_kernel<16><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32);
+    else
+        pcg_spmv_slices_kernel<0><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32);
 }
 
 void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed) {

@@ -101,7 +101,7 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lfg_reactions.argtypes = [_P, _P]
         lib.lfg_postprocess.argtypes = [_P, _P, _P, _P]
         lib.lfg_precond_apply.argtypes = [_P, ctypes.c_int32, _P, _P]
-        lib.lfg_spmv.argtypes = [_P, ctypes.c_int32, _P, _P]
+        lib.lfg_spmv.argtypes = [_P, ctypes.c_int32, ctypes.c_int32, _P, _P]
         lib.lfg_bench_spmv.argtypes = [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.POINTER(ctypes.c_double)]
         lib.lfg_bench_copy.argtypes = [_P, _I64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
@@ -310,10 +310,10 @@ class Device:
         self._check(self._lib.lfg_precond_apply(self._ctx, PRECOND[preconditioner], _ptr(vec), _ptr(out)))
         return out
 
-    def spmv(self, x, precision: str = "fp64") -> np.ndarray:
+    def spmv(self, x, precision: str = "fp64", variant: int = 0) -> np.ndarray:
         x = _f64(x)
         y = np.empty_like(x)
-        self._check(self._lib.lfg_spmv(self._ctx, PRECISION[precision], _ptr(x), _ptr(y)))
+        self._check(self._lib.lfg_spmv(self._ctx, PRECISION[precision], variant, _ptr(x), _ptr(y)))
         return y
 
     def bench_copy(self, nbytes: int, reps: int = 20) -> float:

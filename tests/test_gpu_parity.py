@@ -94,6 +94,13 @@ def test_stages_match_reference(dev, golden_index, name):
     assert np.array_equal(bits(dev.precond_apply("jacobi", g["probe"])), bits(g["jacobi_invdiag"] * g["probe"]))
     assert np.array_equal(bits(dev.precond_apply("block", g["probe"])), bits(g["block1_apply_probe"]))
     assert np.array_equal(bits(dev.precond_apply("ic0", g["probe"])), bits(g["ic0_apply_probe"]))
+    # PCG SpMV kernels (row-segment streaming 4, 5 and node-slice 6) sum each row in
+    # column order from 0.0, as the reference's CSR loop (sparse.cpp:104-120)
+    y_ref = lfo.spmv(g["row_ptr"], g["col"], g["Kc"], g["probe"])
+    y32_ref = lfo.spmv(g["row_ptr"], g["col"], g["Kc"].astype(np.float32).astype(np.float64), g["probe"])
+    for variant in (4, 5, 6):
+        assert np.array_equal(bits(dev.spmv(g["probe"], "fp64", variant)), bits(y_ref)), variant
+        assert np.array_equal(bits(dev.spmv(g["probe"], "mixed", variant)), bits(y32_ref)), variant
     # PCG with the case's preconditioner and the reference stop rule
     precond = PRECOND_NAME[case_precond(meta)]
     u, rep = dev.pcg(precond, "fp64", "reference", meta["tolerance"])
@@ -186,7 +193,7 @@ def test_deterministic_and_graph_equals_direct(dev):
     u3, r3 = dev.pcg("jacobi", direct_launch=True, check_every=7)
     assert np.array_equal(bits(u1), bits(u2)) and np.array_equal(bits(u1), bits(u3))
     assert r1["iterations"] == r2["iterations"] == r3["iterations"]
-    for variant in (1, 2, 3):
+    for variant in (1, 2, 3, 4, 5, 6):
         u4, r4 = dev.pcg("jacobi", spmv_variant=variant)
         assert r4["converged"] and np.linalg.norm(u4 - u1) <= 1e-12 * np.linalg.norm(u1)
 
@@ -246,3 +253,6 @@ def test_full_size_assembly_properties(dev):
     A = scipy.sparse.csr_matrix((kc, sys_.col, sys_.row_ptr), shape=(2004002, 2004002))
     ref = A @ x
     assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # the node-slice and row-segment kernels both sum rows in column order
+    assert np.array_equal(bits(dev.spmv(x, "fp64", 6)), bits(dev.spmv(x, "fp64", 4)))
+    assert np.array_equal(bits(dev.spmv(x, "mixed", 6)), bits(dev.spmv(x, "mixed", 4)))
