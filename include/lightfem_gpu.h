@@ -103,6 +103,12 @@ const char* lfg_status_string(int32_t status);
 int lfg_set_mesh(lfg_ctx* ctx, int64_t n_nodes, const double* xy, int64_t n_elems, const int32_t* enodes,
                  const int32_t* elem_material, int64_t n_materials, const double* materials, int32_t plane_mode);
 
+/* Replaces the material table (E, nu, t per material; same count as lfg_set_mesh)
+ * without touching mesh or pattern; assembly must follow. Serves repeated solves
+ * with changing moduli (the material rows parsed at input_parser.cpp:162-342),
+ * e.g. the forward solves of a density-based gradient loop. */
+int lfg_set_materials(lfg_ctx* ctx, int64_t n_materials, const double* materials);
+
 /* CSR pattern and element->CSR-value map of build_csr_assembly_pattern
  * (sparse.cpp:15-58): row_ptr n_rows+1, col nnz, emap 64*n_elems (row-major local r*8+c). */
 int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* row_ptr, const int32_t* col,
@@ -143,6 +149,21 @@ int lfg_reactions(lfg_ctx* ctx, double* reaction_out);
  * points x 9 doubles [xi, eta, strain3, stress3, vm] per element) and nodal
  * smoothing (solver.cpp:807-871: 4 doubles per node). Any output may be NULL. */
 int lfg_postprocess(lfg_ctx* ctx, double* centre, double* gauss_points, double* nodal);
+
+/* Adjoint solve for a functional J(u) of the last lfg_pcg solution: K lambda = dJ/du
+ * on the constrained system (the entries of dJdu at constrained dofs are ignored and
+ * lambda is zero there), by the same PCG (solver.cpp:532-627) with the options'
+ * preconditioner, which is reused when it matches the forward solve's. The reference
+ * has no gradient code; this is the adjoint of its forward path. lambda_out may be NULL
+ * (lambda stays on the device for lfg_sensitivity). */
+int lfg_adjoint_solve(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* dJdu, double* lambda_out,
+                      lfg_pcg_report* report);
+
+/* Element sensitivities grad_e = -lambda_e^T (dKe/dE_e) u_e * dE_drho[e] (dKe/dE = Ke
+ * of q4_stiffness, element.cpp:130-161, at E = 1; dE_drho NULL means 1, giving dJ/dE_e),
+ * with u the last lfg_pcg solution and lambda the given array (n dofs) or, when NULL,
+ * the last lfg_adjoint_solve result. For compliance f^T u, lambda = u. */
+int lfg_sensitivity(lfg_ctx* ctx, const double* lambda, const double* dE_drho, double* grad_out);
 
 /* Checks and measurement hooks on the constrained system. */
 int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, double* out);

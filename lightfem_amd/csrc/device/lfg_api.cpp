@@ -105,6 +105,8 @@ struct lfg_ctx {
 
     // solver
     DevArray<double> x, r, p, q, z, minv, factor, diag, partials, scratch;
+    DevArray<double> lam, adj_rhs, dE, grad;  // adjoint solution and right-hand side, sensitivities
+    bool have_adjoint = false;
     DevArray<lfg::PcgState> state;
     DevArray<lfg::TicketSet> tickets;  // 0 init, 1 SpMV, 2 update, 3 replacement, 4 residual
     DevArray<int> flags;
@@ -424,15 +426,15 @@ void setup_preconditioner(lfg_ctx* c, int kind, long long* degenerate) {
     c->precond_ready = kind;
 }
 
-lfg::PcgBuffers pcg_buffers(lfg_ctx* c, int kind) {
+lfg::PcgBuffers pcg_buffers(lfg_ctx* c, int kind, const double* rhs, double* xs) {
     lfg::PcgBuffers b{};
     b.n = c->n;
-    b.x = c->x.p;
+    b.x = xs;
     b.r = c->r.p;
     b.p = c->p.p;
     b.q = c->q.p;
     b.z = c->z.p;
-    b.b = c->fc.p;
+    b.b = rhs;
     b.m = kind == LFG_PRECOND_JACOBI ? c->minv.p : (kind == LFG_PRECOND_BLOCK2 ? c->factor.p : nullptr);
     b.precond = kind;
     b.st = c->state.p;
@@ -539,6 +541,20 @@ int lfg_set_mesh(lfg_ctx* ctx, int64_t n_nodes, const double* xy, int64_t n_elem
         sync(ctx);
         ctx->have_mesh = true;
         ctx->have_K = ctx->have_system = ctx->have_solution = false;
+        ctx->precond_ready = -1;
+        return LFG_OK;
+    });
+}
+
+int lfg_set_materials(lfg_ctx* ctx, int64_t n_materials, const double* materials) {
+    return guarded(ctx, [&] {
+        require(ctx->have_mesh, LFG_ERR_STATE, "lfg_set_mesh must precede lfg_set_materials");
+        require(n_materials == ctx->n_mats && materials, LFG_ERR_ARG,
+                "the material table must keep the size given to lfg_set_mesh");
+        LFG_HIP(hipSetDevice(ctx->device));
+        ctx->materials.upload(materials, 3 * static_cast<size_t>(n_materials), ctx->stream);
+        sync(ctx);
+        ctx->have_K = ctx->have_system = ctx->have_solution = ctx->have_adjoint = false;
         ctx->precond_ready = -1;
         return LFG_OK;
     });
@@ -739,122 +755,189 @@ int lfg_get_system(lfg_ctx* ctx, double* values_out, double* rhs_out) {
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Preconditioned CG on the constrained system for right-hand side `rhs` (device),
+// solution into `xs` (device) and, when `x_out` is given, the host. Shared by the
+// forward solve and the adjoint solve, which reuses the forward's preconditioner.
+void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double* xs, double* x_out,
+         lfg_pcg_report* report) {
+    require(ctx->have_system, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_pcg");
+    require(opt != nullptr, LFG_ERR_ARG, "missing PCG options");
+    const int kind = opt->preconditioner;
+    require(kind == LFG_PRECOND_JACOBI || kind == LFG_PRECOND_BLOCK2 || kind == LFG_PRECOND_IC0, LFG_ERR_ARG,
+            "preconditioner must be jacobi, block, or ic0");
+    const int mixed = opt->precision == LFG_PRECISION_MIXED;
+    require(!(mixed && kind == LFG_PRECOND_IC0), LFG_ERR_UNSUPPORTED,
+            "mixed precision is implemented for the jacobi and 2x2 block preconditioners");
+    require(opt->tolerance > 0.0 && opt->tolerance < 1.0, LFG_ERR_ARG, "solver tolerance must satisfy 0 < tolerance < 1");
+    require(opt->max_iterations > 0, LFG_ERR_ARG, "max_iterations must be greater than zero");
+    LFG_HIP(hipSetDevice(ctx->device));
+    ensure_vectors(ctx);
+    lfg_pcg_report rep{};
+    // set-up: preconditioner factors and FP32 values
+    LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    long long degenerate = 0;
+    const int previous = ctx->precond_ready;
+    setup_preconditioner(ctx, kind, &degenerate);
+    if (previous == kind && kind == LFG_PRECOND_BLOCK2) degenerate = -1;  // factor reused; count unknown here
+    const int variant = resolve_variant(ctx, opt->spmv_variant);
+    if (variant == kSliceVariant) {
+        ensure_slices(ctx, true, mixed != 0);
+    } else if (mixed && !ctx->k32_valid) {
+        ctx->K32.alloc(static_cast<size_t>(ctx->nnz));
+        lfg::launch_to_f32(ctx->stream, ctx->Kc.p, ctx->K32.p, ctx->nnz);
+        ctx->k32_valid = true;
+    }
+    LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+    sync(ctx);
+    rep.setup_ms = elapsed(ctx->ev_start, ctx->ev_stop);
+    rep.degenerate_blocks = degenerate;
+    if (kind == LFG_PRECOND_IC0) {
+        rep.breakdowns = ctx->ic0.breakdowns;
+        rep.levels = ctx->ic0.fwd_levels + ctx->ic0.bwd_levels;
+    }
+
+    const lfg::PcgBuffers b = pcg_buffers(ctx, kind, rhs, xs);
+    const int per_graph = opt->check_every > 0 ? opt->check_every : 32;
+    const double drop = opt->replace_drop > 0.0 ? opt->replace_drop : 0.1;
+    const double switch_tau = !mixed ? 0.0 : (opt->mixed_switch != 0.0 ? opt->mixed_switch : 1.0e-4);
+    // capture `per_graph` iterations once
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const bool use_graph = opt->direct_launch == 0;
+    if (use_graph) {
+        LFG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant);
+        LFG_HIP(hipStreamEndCapture(ctx->stream, &graph));
+        LFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    struct GraphGuard {
+        hipGraph_t g;
+        hipGraphExec_t e;
+        ~GraphGuard() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } guard{graph, exec};
+
+    LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    lfg::launch_pcg_init(ctx->stream, b, opt->stop_rule, opt->tolerance, opt->max_iterations, drop, switch_tau);
+    if (kind == LFG_PRECOND_IC0) {
+        lfg::launch_pcg_ic0_step(ctx->stream, b, ic0_view(ctx));
+        lfg::launch_pcg_direction(ctx->stream, b);
+    }
+    // every iteration the cap allows fits in this many graph launches
+    const long long launch_cap = std::max<long long>(1, (opt->max_iterations + per_graph - 1) / per_graph);
+    long long launched = 0;
+    auto launch_one = [&](int slot) {
+        if (use_graph) {
+            LFG_HIP(hipGraphLaunch(exec, ctx->stream));
+        } else {
+            for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant);
+        }
+        LFG_HIP(hipMemcpyAsync(ctx->h_state + slot, ctx->state.p, sizeof(lfg::PcgState), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        LFG_HIP(hipEventRecord(ctx->ev_ring[slot], ctx->stream));
+        ++launched;
+    };
+    launch_one(0);
+    long long k = 0;
+    float solve_ms = 0.0f;
+    while (true) {
+        const int slot = static_cast<int>(k & 1);
+        if (launched < launch_cap) launch_one(1 - slot);
+        LFG_HIP(hipEventSynchronize(ctx->ev_ring[slot]));
+        if (ctx->h_state[slot].done || launched >= launch_cap) {
+            solve_ms = elapsed(ctx->ev_start, ctx->ev_ring[slot]);
+            break;
+        }
+        ++k;
+    }
+    sync(ctx);
+    lfg::PcgState st{};
+    LFG_HIP(hipMemcpy(&st, ctx->state.p, sizeof(st), hipMemcpyDeviceToHost));
+    // FP64 true residual ||b - K u|| for the report
+    double r2 = 0.0;
+    lfg::launch_residual_norm(ctx->stream, csr_view(ctx), ctx->Kc.p, xs, rhs, ctx->scratch.p, ctx->tickets.p + 4,
+                              ctx->scratch.p + 4096);
+    LFG_HIP(hipMemcpyAsync(&r2, ctx->scratch.p + 4096, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (x_out) LFG_HIP(hipMemcpyAsync(x_out, xs, ctx->n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    rep.converged = st.converged;
+    rep.iterations = st.iter;
+    rep.residual_replacements = st.replacements;
+    rep.initial_residual = st.rnorm0;
+    rep.final_residual = std::sqrt(std::max(0.0, st.rr));
+    rep.true_residual = std::sqrt(std::max(0.0, r2));
+    rep.solve_ms = solve_ms;
+    rep.status = st.status != LFG_OK ? st.status : (st.converged ? LFG_OK : LFG_ERR_NOT_CONVERGED);
+    if (report) *report = rep;
+    if (st.status != LFG_OK) throw LfgError(st.status, status_text(st.status));
+    if (!st.converged) throw LfgError(LFG_ERR_NOT_CONVERGED, "maximum iterations reached without convergence");
+}
+
+}  // namespace
+
+extern "C" {
+
+
 int lfg_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, double* u_out, lfg_pcg_report* report) {
     return guarded(ctx, [&] {
         require(ctx->have_system, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_pcg");
-        require(opt != nullptr, LFG_ERR_ARG, "missing PCG options");
-        const int kind = opt->preconditioner;
-        require(kind == LFG_PRECOND_JACOBI || kind == LFG_PRECOND_BLOCK2 || kind == LFG_PRECOND_IC0, LFG_ERR_ARG,
-                "preconditioner must be jacobi, block, or ic0");
-        const int mixed = opt->precision == LFG_PRECISION_MIXED;
-        require(!(mixed && kind == LFG_PRECOND_IC0), LFG_ERR_UNSUPPORTED,
-                "mixed precision is implemented for the jacobi and 2x2 block preconditioners");
-        require(opt->tolerance > 0.0 && opt->tolerance < 1.0, LFG_ERR_ARG, "solver tolerance must satisfy 0 < tolerance < 1");
-        require(opt->max_iterations > 0, LFG_ERR_ARG, "max_iterations must be greater than zero");
-        LFG_HIP(hipSetDevice(ctx->device));
+        ctx->have_solution = false;
+        ctx->have_adjoint = false;
         ensure_vectors(ctx);
-        lfg_pcg_report rep{};
-        // set-up: preconditioner factors and FP32 values
-        LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-        long long degenerate = 0;
-        const int previous = ctx->precond_ready;
-        setup_preconditioner(ctx, kind, &degenerate);
-        if (previous == kind && kind == LFG_PRECOND_BLOCK2) degenerate = -1;  // factor reused; count unknown here
-        const int variant = resolve_variant(ctx, opt->spmv_variant);
-        if (variant == kSliceVariant) {
-            ensure_slices(ctx, true, mixed != 0);
-        } else if (mixed && !ctx->k32_valid) {
-            ctx->K32.alloc(static_cast<size_t>(ctx->nnz));
-            lfg::launch_to_f32(ctx->stream, ctx->Kc.p, ctx->K32.p, ctx->nnz);
-            ctx->k32_valid = true;
+        try {
+            run_pcg(ctx, opt, ctx->fc.p, ctx->x.p, u_out, report);
+        } catch (const LfgError& e) {
+            // a failed or unconverged solve still leaves its last iterate for reactions/fields
+            ctx->have_solution = e.code == LFG_ERR_NOT_CONVERGED;
+            throw;
         }
-        LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
-        sync(ctx);
-        rep.setup_ms = elapsed(ctx->ev_start, ctx->ev_stop);
-        rep.degenerate_blocks = degenerate;
-        if (kind == LFG_PRECOND_IC0) {
-            rep.breakdowns = ctx->ic0.breakdowns;
-            rep.levels = ctx->ic0.fwd_levels + ctx->ic0.bwd_levels;
-        }
-
-        const lfg::PcgBuffers b = pcg_buffers(ctx, kind);
-        const int per_graph = opt->check_every > 0 ? opt->check_every : 32;
-        const double drop = opt->replace_drop > 0.0 ? opt->replace_drop : 0.1;
-        const double switch_tau = !mixed ? 0.0 : (opt->mixed_switch != 0.0 ? opt->mixed_switch : 1.0e-4);
-        // capture `per_graph` iterations once
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        const bool use_graph = opt->direct_launch == 0;
-        if (use_graph) {
-            LFG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant);
-            LFG_HIP(hipStreamEndCapture(ctx->stream, &graph));
-            LFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        }
-        struct GraphGuard {
-            hipGraph_t g;
-            hipGraphExec_t e;
-            ~GraphGuard() {
-                if (e) (void)hipGraphExecDestroy(e);
-                if (g) (void)hipGraphDestroy(g);
-            }
-        } guard{graph, exec};
-
-        LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-        lfg::launch_pcg_init(ctx->stream, b, opt->stop_rule, opt->tolerance, opt->max_iterations, drop, switch_tau);
-        if (kind == LFG_PRECOND_IC0) {
-            lfg::launch_pcg_ic0_step(ctx->stream, b, ic0_view(ctx));
-            lfg::launch_pcg_direction(ctx->stream, b);
-        }
-        // every iteration the cap allows fits in this many graph launches
-        const long long launch_cap = std::max<long long>(1, (opt->max_iterations + per_graph - 1) / per_graph);
-        long long launched = 0;
-        auto launch_one = [&](int slot) {
-            if (use_graph) {
-                LFG_HIP(hipGraphLaunch(exec, ctx->stream));
-            } else {
-                for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant);
-            }
-            LFG_HIP(hipMemcpyAsync(ctx->h_state + slot, ctx->state.p, sizeof(lfg::PcgState), hipMemcpyDeviceToHost,
-                                   ctx->stream));
-            LFG_HIP(hipEventRecord(ctx->ev_ring[slot], ctx->stream));
-            ++launched;
-        };
-        launch_one(0);
-        long long k = 0;
-        float solve_ms = 0.0f;
-        while (true) {
-            const int slot = static_cast<int>(k & 1);
-            if (launched < launch_cap) launch_one(1 - slot);
-            LFG_HIP(hipEventSynchronize(ctx->ev_ring[slot]));
-            if (ctx->h_state[slot].done || launched >= launch_cap) {
-                solve_ms = elapsed(ctx->ev_start, ctx->ev_ring[slot]);
-                break;
-            }
-            ++k;
-        }
-        sync(ctx);
-        lfg::PcgState st{};
-        LFG_HIP(hipMemcpy(&st, ctx->state.p, sizeof(st), hipMemcpyDeviceToHost));
-        // FP64 true residual ||b - K u|| for the report
-        double r2 = 0.0;
-        lfg::launch_residual_norm(ctx->stream, csr_view(ctx), ctx->Kc.p, ctx->x.p, ctx->fc.p, ctx->scratch.p,
-                                  ctx->tickets.p + 4, ctx->scratch.p + 4096);
-        LFG_HIP(hipMemcpyAsync(&r2, ctx->scratch.p + 4096, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        if (u_out) ctx->x.download(u_out, static_cast<size_t>(ctx->n), ctx->stream);
-        sync(ctx);
-        rep.converged = st.converged;
-        rep.iterations = st.iter;
-        rep.residual_replacements = st.replacements;
-        rep.initial_residual = st.rnorm0;
-        rep.final_residual = std::sqrt(std::max(0.0, st.rr));
-        rep.true_residual = std::sqrt(std::max(0.0, r2));
-        rep.solve_ms = solve_ms;
-        rep.status = st.status != LFG_OK ? st.status : (st.converged ? LFG_OK : LFG_ERR_NOT_CONVERGED);
-        if (report) *report = rep;
         ctx->have_solution = true;
-        if (st.status != LFG_OK) throw LfgError(st.status, status_text(st.status));
-        if (!st.converged) throw LfgError(LFG_ERR_NOT_CONVERGED, "maximum iterations reached without convergence");
+        return LFG_OK;
+    });
+}
+
+int lfg_adjoint_solve(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* dJdu, double* lambda_out,
+                      lfg_pcg_report* report) {
+    return guarded(ctx, [&] {
+        require(ctx->have_solution, LFG_ERR_STATE, "lfg_pcg must precede lfg_adjoint_solve");
+        require(dJdu != nullptr, LFG_ERR_ARG, "missing adjoint right-hand side");
+        LFG_HIP(hipSetDevice(ctx->device));
+        ctx->have_adjoint = false;
+        ctx->lam.alloc(static_cast<size_t>(ctx->n));
+        ctx->adj_rhs.upload(dJdu, static_cast<size_t>(ctx->n), ctx->stream);
+        lfg::launch_mask_constrained(ctx->stream, ctx->n, ctx->constrained.p, ctx->adj_rhs.p, ctx->adj_rhs.p);
+        run_pcg(ctx, opt, ctx->adj_rhs.p, ctx->lam.p, lambda_out, report);
+        ctx->have_adjoint = true;
+        return LFG_OK;
+    });
+}
+
+int lfg_sensitivity(lfg_ctx* ctx, const double* lambda, const double* dE_drho, double* grad_out) {
+    return guarded(ctx, [&] {
+        require(ctx->have_solution, LFG_ERR_STATE, "lfg_pcg must precede lfg_sensitivity");
+        require(lambda != nullptr || ctx->have_adjoint, LFG_ERR_STATE,
+                "lfg_adjoint_solve or an explicit lambda must precede lfg_sensitivity");
+        require(grad_out != nullptr, LFG_ERR_ARG, "missing gradient output");
+        LFG_HIP(hipSetDevice(ctx->device));
+        if (lambda) {
+            ctx->lam.upload(lambda, static_cast<size_t>(ctx->n), ctx->stream);
+            ctx->have_adjoint = true;
+        }
+        const double* dE = nullptr;
+        if (dE_drho) {
+            ctx->dE.upload(dE_drho, static_cast<size_t>(ctx->n_elems), ctx->stream);
+            dE = ctx->dE.p;
+        }
+        ctx->grad.alloc(static_cast<size_t>(ctx->n_elems));
+        lfg::launch_sensitivity(ctx->stream, mesh_view(ctx), ctx->x.p, ctx->lam.p, dE, ctx->grad.p);
+        ctx->grad.download(grad_out, static_cast<size_t>(ctx->n_elems), ctx->stream);
+        sync(ctx);
         return LFG_OK;
     });
 }
@@ -912,7 +995,7 @@ int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, do
 }
 
 static void run_spmv(lfg_ctx* ctx, int precision, int variant) {
-    lfg::PcgBuffers b = pcg_buffers(ctx, LFG_PRECOND_JACOBI);
+    lfg::PcgBuffers b = pcg_buffers(ctx, LFG_PRECOND_JACOBI, ctx->fc.p, ctx->x.p);
     b.st = ctx->state.p;
     const bool f32 = precision == LFG_PRECISION_MIXED;  // one value array per launch
     if (variant == kSliceVariant)

@@ -275,6 +275,42 @@ __global__ __launch_bounds__(256) void smooth_kernel(int n_nodes, const int* __r
     out[3] = sqrt(max0(sx * sx - sx * sy + sy * sy + 3.0 * txy * txy));
 }
 
+// Element sensitivities of J(u(E)) with K(E) u = f: dJ/dE_e = -lambda_e^T (dKe/dE) u_e,
+// where K lambda = dJ/du (lambda zero on constrained dofs). Ke is linear in E
+// (E enters only through D, element.cpp:16-32), so dKe/dE = Ke evaluated with
+// E = 1. Lane r of an element's 8-lane group forms row r of Ke(1) (ke_row, the
+// reference's operation order), dots it with u_e in column order and weights it
+// by lambda_e[r]; the group then sums its 8 terms pairwise (lanes r^1, r^2, r^4),
+// grad_e = -sum * dE_e (dE_e = dE/drho_e, or 1 without dE).
+__global__ __launch_bounds__(256) void sensitivity_kernel(MeshView m, const double* __restrict__ u,
+                                                          const double* __restrict__ lam,
+                                                          const double* __restrict__ dE, double* __restrict__ grad) {
+    const long long tid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int e = static_cast<int>(tid >> 3);
+    const int row = static_cast<int>(tid & 7);
+    if (e >= m.n_elems) return;  // whole 8-lane groups leave together
+    Coords c;
+    double young, nu, thick;
+    load_element(m, e, c, young, nu, thick);
+    double ke[8];
+    ke_row(c, 1.0, nu, thick, m.plane_mode, row, ke);
+    const int4 nodes = reinterpret_cast<const int4*>(m.enodes)[e];
+    const int ids[4] = {nodes.x, nodes.y, nodes.z, nodes.w};
+    double ku = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double2 v = reinterpret_cast<const double2*>(u)[ids[i]];
+        ku += ke[2 * i] * v.x;
+        ku += ke[2 * i + 1] * v.y;
+    }
+    const int my_node = ids[row >> 1];
+    double t = lam[2 * my_node + (row & 1)] * ku;
+    t = t + __shfl_xor(t, 1, 8);
+    t = t + __shfl_xor(t, 2, 8);
+    t = t + __shfl_xor(t, 4, 8);
+    if (row == 0) grad[e] = -t * (dE ? dE[e] : 1.0);
+}
+
 inline unsigned int blocks_for(long long work, int per_block) {
     return static_cast<unsigned int>((work + per_block - 1) / per_block);
 }
@@ -312,6 +348,12 @@ void launch_fill(hipStream_t s, double* p, long long n, double v) {
 void launch_fields(hipStream_t s, const MeshView& m, const double* u, double* centre, double* gp) {
     if (m.n_elems <= 0) return;
     fields_kernel<<<blocks_for(m.n_elems, 128), 128, 0, s>>>(m, u, centre, gp);
+}
+
+void launch_sensitivity(hipStream_t s, const MeshView& m, const double* u, const double* lam, const double* dE,
+                        double* grad) {
+    if (m.n_elems <= 0) return;
+    sensitivity_kernel<<<blocks_for(8LL * m.n_elems, 256), 256, 0, s>>>(m, u, lam, dE, grad);
 }
 
 void launch_smooth(hipStream_t s, int n_nodes, const int* ptr, const int* list, const double* gp, double* nodal) {

@@ -57,6 +57,9 @@ void launch_fill(hipStream_t s, double* p, long long n, double v);
 void launch_fields(hipStream_t s, const MeshView& m, const double* u, double* centre, double* gp);
 void launch_smooth(hipStream_t s, int n_nodes, const int* node_elem_ptr, const int* node_elem, const double* gp,
                    double* nodal);
+// grad[e] = -lam_e^T Ke(E = 1) u_e * (dE ? dE[e] : 1).
+void launch_sensitivity(hipStream_t s, const MeshView& m, const double* u, const double* lam, const double* dE,
+                        double* grad);
 
 // ---- solver kernels (lfg_solver.hip)
 void launch_dirichlet(hipStream_t s, const CsrView& a, const double* values, const double* f, const int* constrained,
@@ -68,6 +71,8 @@ void launch_block2_factor(hipStream_t s, const CsrView& a, const double* values,
 void launch_jacobi_inv(hipStream_t s, int n, const double* diag, double* inv, int* status);
 void launch_precond_apply(hipStream_t s, int kind, int n, const double* m, const double* in, double* out);
 void launch_to_f32(hipStream_t s, const double* in, float* out, long long n);
+// out[i] = constrained[i] ? 0 : in[i] (adjoint right-hand side).
+void launch_mask_constrained(hipStream_t s, int n, const int* constrained, const double* in, double* out);
 
 // PCG building blocks; every kernel returns at once when st->done is set.
 struct PcgBuffers {

@@ -857,6 +857,12 @@ __global__ __launch_bounds__(kThreads) void residual_norm_kernel(int n, const in
     if (threadIdx.x == 0) *out2 = tot[0];
 }
 
+__global__ void mask_constrained_kernel(int n, const int* __restrict__ constrained, const double* __restrict__ in,
+                                        double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = constrained[i] ? 0.0 : in[i];
+}
+
 // SpMV variants: 1, 2, 3 -> node groups with 4, 8, 16 lanes per row; 4 (the
 // default) and 5 -> row-segment streaming with 64 / 32 rows per wave chunk.
 constexpr int kDefaultVariant = 4;
@@ -911,6 +917,11 @@ void launch_precond_apply(hipStream_t s, int kind, int n, const double* m, const
 void launch_to_f32(hipStream_t s, const double* in, float* out, long long n) {
     if (n <= 0) return;
     to_f32_kernel<<<blocks_for(n, 256), 256, 0, s>>>(in, out, n);
+}
+
+void launch_mask_constrained(hipStream_t s, int n, const int* constrained, const double* in, double* out) {
+    if (n <= 0) return;
+    mask_constrained_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, constrained, in, out);
 }
 
 // Fixed SpMV grid: at most kSpmvBlocks blocks, each looping over its node range.

@@ -26,7 +26,8 @@ STOP = {"reference": 0, "relative": 1}
 GPU_SYMBOLS = ["lfg_create", "lfg_destroy", "lfg_last_error", "lfg_status_string", "lfg_set_mesh", "lfg_set_pattern",
                "lfg_set_colors", "lfg_element_stiffness", "lfg_assemble", "lfg_get_stiffness", "lfg_set_stiffness",
                "lfg_set_load", "lfg_dirichlet", "lfg_get_system", "lfg_pcg", "lfg_reactions", "lfg_postprocess",
-               "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv", "lfg_bench_copy"]
+               "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv", "lfg_bench_copy", "lfg_set_materials",
+               "lfg_adjoint_solve", "lfg_sensitivity"]
 HOST_SYMBOLS = ["lfh_csr_pattern_nnz", "lfh_csr_pattern", "lfh_rcm", "lfh_element_colors", "lfh_csr_profile"]
 
 
@@ -105,6 +106,9 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lfg_bench_spmv.argtypes = [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.POINTER(ctypes.c_double)]
         lib.lfg_bench_copy.argtypes = [_P, _I64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        lib.lfg_set_materials.argtypes = [_P, _I64, _P]
+        lib.lfg_adjoint_solve.argtypes = [_P, ctypes.POINTER(PcgOptions), _P, _P, ctypes.POINTER(PcgReport)]
+        lib.lfg_sensitivity.argtypes = [_P, _P, _P, _P]
         _gpu = lib
     return _gpu
 
@@ -201,6 +205,7 @@ class Device:
         self.nnz = 0
         self.n_nodes = 0
         self.n_elems = 0
+        self.lightfem_owner = None  # (problem, solve token) of the state held on the device
 
     def close(self) -> None:
         if getattr(self, "_ctx", None):
@@ -234,6 +239,10 @@ class Device:
                                            _ptr(elem_material), materials.shape[0], _ptr(materials), plane_mode))
         self.n_nodes = xy.shape[0]
         self.n_elems = enodes.shape[0]
+
+    def set_materials(self, materials) -> None:
+        materials = _f64(materials).reshape(-1, 3)
+        self._check(self._lib.lfg_set_materials(self._ctx, materials.shape[0], _ptr(materials)))
 
     def set_pattern(self, row_ptr, col, emap) -> None:
         row_ptr, col, emap = _i32(row_ptr), _i32(col), _i32(emap)
@@ -291,6 +300,27 @@ class Device:
         if rc != OK and raise_on_error:
             self._check(rc)
         return u, rep.as_dict()
+
+    def adjoint_solve(self, dJdu, preconditioner: str = "jacobi", precision: str = "fp64",
+                      stop_rule: str = "reference", tolerance: float = 1e-10, max_iterations: int = 100000,
+                      check_every: int = 0, spmv_variant: int = 0):
+        """K lambda = dJ/du on the constrained system of the last forward solve."""
+        dJdu = _f64(dJdu)
+        opt = PcgOptions(PRECOND[preconditioner], PRECISION[precision], STOP[stop_rule], check_every, tolerance,
+                         max_iterations, 0.0, 0.0, 0, spmv_variant)
+        rep = PcgReport()
+        lam = np.empty(self.n)
+        self._check(self._lib.lfg_adjoint_solve(self._ctx, ctypes.byref(opt), _ptr(dJdu), _ptr(lam),
+                                                ctypes.byref(rep)))
+        return lam, rep.as_dict()
+
+    def sensitivity(self, lam=None, dE_drho=None) -> np.ndarray:
+        """-lambda_e^T (dKe/dE) u_e (times dE_drho[e]) per element; lam None = last adjoint."""
+        lam = None if lam is None else _f64(lam)
+        dE = None if dE_drho is None else _f64(dE_drho)
+        grad = np.empty(self.n_elems)
+        self._check(self._lib.lfg_sensitivity(self._ctx, _ptr(lam), _ptr(dE), _ptr(grad)))
+        return grad
 
     def reactions(self) -> np.ndarray:
         out = np.empty(self.n)

@@ -86,6 +86,43 @@ int lfo_q4_stiffness(const double xy[8], double young, double nu, double thickne
     return 0;
 }
 
+/* Adjoint element sensitivity. The reference has no gradient code: this is the
+ * derivative of its forward path. Ke is linear in E (E enters only through D,
+ * element.cpp:16-32), so dKe/dE is q4_stiffness (element.cpp:130-161) at E = 1.
+ * grad_e = -(lambda_e^T Ke(1) u_e) * dE_e: per row r, (Ke(1) u_e)_r sums columns in
+ * order, t_r = lambda_e[r] * that, and the eight t_r are summed pairwise,
+ * ((t0+t1)+(t2+t3)) + ((t4+t5)+(t6+t7)). props: (E, nu, t) per element (E unused);
+ * dE NULL means 1. Returns the number of elements with a non-positive Jacobian. */
+int64_t lfo_sensitivity(int64_t nelem, const double* xy_nodes, const int64_t* enode_idx, const double* props,
+                        int plane_mode, const double* u, const double* lam, const double* dE, double* grad) {
+    int64_t failures = 0;
+    for (int64_t e = 0; e < nelem; ++e) {
+        double xy[8], ke[64], ue[8], le[8], t[8];
+        for (int i = 0; i < 4; ++i) {
+            const int64_t node = enode_idx[4 * e + i];
+            xy[2 * i] = xy_nodes[2 * node];
+            xy[2 * i + 1] = xy_nodes[2 * node + 1];
+            ue[2 * i] = u[2 * node];
+            ue[2 * i + 1] = u[2 * node + 1];
+            le[2 * i] = lam[2 * node];
+            le[2 * i + 1] = lam[2 * node + 1];
+        }
+        if (lfo_q4_stiffness(xy, 1.0, props[3 * e + 1], props[3 * e + 2], plane_mode, ke)) {
+            ++failures;
+            grad[e] = 0.0;
+            continue;
+        }
+        for (int r = 0; r < 8; ++r) {
+            double ku = 0.0;
+            for (int c = 0; c < 8; ++c) ku += ke[8 * r + c] * ue[c];
+            t[r] = le[r] * ku;
+        }
+        const double s = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+        grad[e] = -s * (dE ? dE[e] : 1.0);
+    }
+    return failures;
+}
+
 /* element.cpp:75-95 and 212-248 */
 static void field_at(const double b[24], const double d[9], const double u[8], double out[7]) {
     double strain[3] = {0.0, 0.0, 0.0}, stress[3] = {0.0, 0.0, 0.0};

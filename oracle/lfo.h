@@ -80,6 +80,11 @@ int lfo_pcg(int64_t n, const int64_t* row_ptr, const int64_t* col, const double*
             int precond, int stop_rule, double tolerance, int64_t max_iterations, double* x,
             lfo_pcg_report* report);
 
+/* Adjoint element sensitivities -lambda_e^T Ke(E=1) u_e * dE_e (no reference
+ * counterpart; the derivative of q4_stiffness in E). props (E, nu, t) per element. */
+int64_t lfo_sensitivity(int64_t nelem, const double* xy_nodes, const int64_t* enode_idx, const double* props,
+                        int plane_mode, const double* u, const double* lam, const double* dE, double* grad);
+
 /* Nodal smoothing (solver.cpp:807-871); node index = dof_x / 2. */
 void lfo_smooth_nodal(int64_t nnodes, int64_t nelem, const int64_t* enode_idx, const double* gp,
                       double* nodal);

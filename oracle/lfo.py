@@ -39,6 +39,7 @@ def lib() -> ctypes.CDLL:
         _lib.lfo_ic0_factor_size.restype = ctypes.c_int64
         _lib.lfo_ic0_factor.restype = ctypes.c_int64
         _lib.lfo_block1_factor.restype = ctypes.c_int64
+        _lib.lfo_sensitivity.restype = ctypes.c_int64
     return _lib
 
 
@@ -237,3 +238,47 @@ def solve_model(xy_nodes, enode_idx, props, plane_mode, f, cdofs, cvals, precond
     nodal = smooth_nodal(xy_nodes.shape[0], enode_idx, gp.reshape(-1, 36))
     return {"row_ptr": row_ptr, "col": col, "emap": emap, "edofs": edofs, "ke": ke, "K": K, "Kc": Kc, "fc": fc,
             "u": u, "R": R, "centre": centre, "gp": gp, "nodal": nodal, "report": rep}
+
+
+def sensitivity(xy_nodes, enode_idx, props, plane_mode, u, lam, dE=None) -> np.ndarray:
+    """-lambda_e^T Ke(E=1) u_e (* dE[e]) per element (lfo_sensitivity)."""
+    xy_nodes, enode_idx, props = _f64(xy_nodes), _i64(enode_idx), _f64(props)
+    u, lam = _f64(u), _f64(lam)
+    dE = None if dE is None else _f64(dE)
+    grad = np.empty(enode_idx.shape[0])
+    bad = lib().lfo_sensitivity(_I(enode_idx.shape[0]), _p(xy_nodes), _p(enode_idx), _p(props),
+                                ctypes.c_int(plane_mode), _p(u), _p(lam), None if dE is None else _p(dE), _p(grad))
+    if bad:
+        raise RuntimeError("Q4 stiffness evaluation received a non-positive Jacobian")
+    return grad
+
+
+class DirectSystem:
+    """FP64 direct solves of the constrained system (scipy SuperLU on the oracle's
+    serial K and Dirichlet elimination), with iterative refinement. The CPU
+    reference for the adjoint gradient; the reference itself has none."""
+
+    def __init__(self, xy_nodes, enode_idx, props, plane_mode, f, cdofs, cvals):
+        import scipy.sparse
+        import scipy.sparse.linalg
+        enode_idx = _i64(enode_idx)
+        self.n = 2 * xy_nodes.shape[0]
+        edofs = np.stack([2 * enode_idx, 2 * enode_idx + 1], axis=2).reshape(-1, 8)
+        self.row_ptr, self.col, self.emap = pattern(self.n, edofs)
+        self.ke = element_stiffness(xy_nodes, enode_idx, props, plane_mode)
+        self.K = assemble_serial(self.ke, self.emap, self.col.shape[0])
+        self.Kc, self.fc = dirichlet(self.row_ptr, self.col, self.K, f, cdofs, cvals)
+        self.cdofs = _i64(cdofs)
+        self.A = scipy.sparse.csr_matrix((self.Kc, self.col, self.row_ptr), shape=(self.n, self.n))
+        self.lu = scipy.sparse.linalg.splu(self.A.tocsc())
+
+    def solve(self, rhs, refine: int = 3) -> np.ndarray:
+        x = self.lu.solve(rhs)
+        for _ in range(refine):
+            x = x + self.lu.solve(rhs - self.A @ x)
+        return x
+
+    def adjoint_rhs(self, dJdu) -> np.ndarray:
+        g = np.array(dJdu, dtype=np.float64, copy=True)
+        g[self.cdofs] = 0.0
+        return g
