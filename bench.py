@@ -50,6 +50,20 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int) -> int:
     return nnz * (value_bytes + 4) + 4 * (n + 1) + 16 * n
 
 
+def measured_traffic(kernel: str, n_mesh: int):
+    """HBM bytes per launch of `kernel` at this mesh size from the committed FETCH_SIZE /
+    WRITE_SIZE counter summary (rocprofv3 --pmc, calibrated on a 1 GiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"pcg_spmv_slices_{n_mesh}x{n_mesh}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as handle:
+        for line in handle:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel.replace(" ", ""):
+                return int(parts[2])
+    return None
+
+
 def cpu_baseline(n_mesh: int, young: np.ndarray, iterations_full: int, sample_iters: int) -> dict:
     ref = os.path.join(ROOT, "oracle", "_ref", "lfref")
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -126,6 +140,9 @@ def main() -> int:
                          raise_on_error=False)
         prof["pcg_iterations"] = rep["iterations"]
         prof["pcg_ms_per_iteration"] = rep["solve_ms"] / max(1, rep["iterations"])
+        copy_bytes = 1 << 30  # calibration dispatches for the traffic counters: 1 GiB read + 1 GiB written
+        prof["copy_bytes"] = copy_bytes
+        prof["copy_ms"] = dev.bench_copy(copy_bytes, 2)
         print(json.dumps(prof))
         dev.close()
         return 0
@@ -216,7 +233,8 @@ def main() -> int:
         "spmv": spmv, "device_copy": copy, "assembly": assembly, "setup_s": round(setup_s, 2),
         "roofline": {"bound": "hbm", "kernel": f"{dominant['kernel']} (FP64 values)",
                      "achieved": round(dominant["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dominant["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(dominant["GBs"] / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(dominant["kernel"], n_mesh),
                      "bytes_per_launch": dominant["bytes"], "avg_launch_ms": dominant["ms"],
                      "fp32_values_spmv_GBs": round(spmv["mixed"]["GBs"], 1),
                      "copy_ceiling_GBs": round(copy["GBs"], 1)},
