@@ -109,6 +109,11 @@ int lfg_set_mesh(lfg_ctx* ctx, int64_t n_nodes, const double* xy, int64_t n_elem
  * e.g. the forward solves of a density-based gradient loop. */
 int lfg_set_materials(lfg_ctx* ctx, int64_t n_materials, const double* materials);
 
+/* Plane mode per material row (LFG_PLANE_*), overriding lfg_set_mesh's single mode, for
+ * batches whose problems mix plane stress and plane strain (each problem has one mode in
+ * the reference, model.hpp settings.plane_mode). NULL restores the single mode. */
+int lfg_set_material_modes(lfg_ctx* ctx, const int32_t* modes);
+
 /* CSR pattern and element->CSR-value map of build_csr_assembly_pattern
  * (sparse.cpp:15-58): row_ptr n_rows+1, col nnz, emap 64*n_elems (row-major local r*8+c). */
 int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* row_ptr, const int32_t* col,
@@ -164,6 +169,61 @@ int lfg_adjoint_solve(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* dJ
  * with u the last lfg_pcg solution and lambda the given array (n dofs) or, when NULL,
  * the last lfg_adjoint_solve result. For compliance f^T u, lambda = u. */
 int lfg_sensitivity(lfg_ctx* ctx, const double* lambda, const double* dE_drho, double* grad_out);
+
+/* ---- Batches of independent problems (many meshes and load cases in one solve).
+ *
+ * The mesh given to lfg_set_mesh / lfg_set_pattern is the disjoint union of the
+ * problems: problem p owns nodes [node_ptr[p], node_ptr[p+1]), every node_ptr entry a
+ * multiple of 64 (problems are padded with nodes that no element uses), and no element
+ * spans two problems. Each problem is a LightFEM model of its own (per-problem plane
+ * modes through lfg_set_material_modes); its systems are solved by Jacobi-PCG
+ * (solver.cpp:532-627) with per-system scalars and convergence. */
+int lfg_batch_set_problems(lfg_ctx* ctx, int32_t n_problems, const int32_t* node_ptr);
+
+/* After assembly: n_cases (1-4) load cases, loads[c * n_dofs + i], and the Dirichlet
+ * supports shared by all cases (apply_dirichlet, solver.cpp:355-413, per case). */
+int lfg_batch_set_loads(lfg_ctx* ctx, int32_t n_cases, const double* loads, int64_t n_constraints,
+                        const int32_t* dofs, const double* values);
+
+typedef struct {
+    int32_t precision;        /* LFG_PRECISION_FP64, or MIXED: FP32 values with FP64 refinement passes */
+    int32_t stop_rule;        /* LFG_STOP_* on each system's own ||b|| */
+    int32_t check_every;      /* iterations per captured graph; 0 = 32 */
+    int32_t max_passes;       /* mixed: refinement passes including the first; 0 = 4 */
+    double tolerance;
+    int64_t max_iterations;   /* per system and pass */
+    double pass_reduction;    /* mixed: residual reduction asked of each later pass; 0 = 1e-6 */
+    int32_t tune;             /* 1: choose the launch shape by timing candidates (cached per context) */
+    int32_t block_threads;    /* launch shape when not tuning: 64-512 (0 = 256) */
+    int32_t slices_per_wave;  /* slices each wavefront walks per launch (0 = 1) */
+    int32_t pad;
+} lfg_batch_options;
+
+typedef struct {
+    int32_t converged;        /* fp64: recurrence residual reached the target; mixed: FP64 residual did */
+    int32_t status;           /* lfg_status of the system */
+    int64_t iterations;       /* over all passes */
+    double initial_residual;  /* ||b|| */
+    double final_residual;    /* last recurrence residual */
+    double true_residual;     /* ||b - K u|| in FP64 */
+} lfg_batch_system_report;
+
+typedef struct {
+    double solve_ms;          /* first init to the final residual, device time */
+    double tune_ms;
+    int64_t graph_launches;
+    int32_t passes;
+    int32_t block_threads;
+    int32_t slices_per_wave;
+    int32_t systems_converged;
+    int64_t max_iterations;   /* over systems */
+} lfg_batch_summary;
+
+/* Solves every (problem, case) system; u_out[c * n_dofs + i] (NULL to skip), reports
+ * [p * n_cases + c] (NULL to skip). Returns LFG_ERR_NOT_CONVERGED when a system did not
+ * converge (u_out and reports are still written). */
+int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg_batch_system_report* reports,
+                  lfg_batch_summary* summary);
 
 /* Checks and measurement hooks on the constrained system. */
 int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, double* out);

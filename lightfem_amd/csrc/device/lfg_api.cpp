@@ -76,6 +76,7 @@ struct lfg_ctx {
     // mesh
     int n_nodes = 0, n_elems = 0, n_mats = 0, plane_mode = 0;
     DevArray<double> xy, materials;
+    DevArray<int> mat_mode;  // per-material plane modes (lfg_set_material_modes), else empty
     DevArray<int> enodes, elem_mat, node_elem_ptr, node_elem;
     bool have_mesh = false;
 
@@ -117,6 +118,19 @@ struct lfg_ctx {
 
     // fields
     DevArray<double> centre, gp, nodal;
+
+    // batch of independent problems
+    int batch_problems = 0, batch_cases = 0;
+    bool have_batch = false, have_batch_loads = false, batch_tuned = false;
+    lfg::BatchShape batch_shape{256, 1};
+    double batch_tune_ms = 0.0;
+    DevArray<int> batch_slice_problem, batch_pslice_ptr, batch_problem_done;
+    DevArray<unsigned int> batch_tickets;
+    DevArray<double> batch_b, batch_rhs, batch_x, batch_r, batch_p, batch_q, batch_xacc, batch_res, batch_minv,
+        batch_partials, batch_tmp;
+    DevArray<lfg::BatchSystem> batch_sys;
+    DevArray<lfg::BatchGlobal> batch_global;
+    lfg::BatchGlobal* h_batch_global = nullptr;  // pinned, 2 slots
 };
 
 namespace {
@@ -141,7 +155,9 @@ void require(bool cond, int code, const char* msg) {
 }
 
 lfg::MeshView mesh_view(const lfg_ctx* c) {
-    return lfg::MeshView{c->n_nodes, c->n_elems, c->xy.p, c->enodes.p, c->elem_mat.p, c->materials.p, c->plane_mode};
+    return lfg::MeshView{c->n_nodes,     c->n_elems,     c->xy.p,         c->enodes.p,
+                         c->elem_mat.p,  c->materials.p, c->plane_mode,
+                         c->mat_mode.p};
 }
 
 lfg::CsrView csr_view(const lfg_ctx* c) { return lfg::CsrView{c->n, c->nnz, c->row_ptr.p, c->col.p, c->max_row}; }
@@ -162,7 +178,8 @@ int resolve_variant(const lfg_ctx* c, int variant) {
 }
 
 // Node-blocked patterns (every two-dof-per-node pattern): builds the slice offsets
-// on the host and the node columns on the device.
+// on the host and the node columns on the device. A node in no element has only its
+// two diagonal entries; it becomes one diagonal block.
 void build_slices(lfg_ctx* c) {
     c->have_slices = false;
     c->slices64_valid = c->slices32_valid = false;
@@ -173,6 +190,10 @@ void build_slices(lfg_ctx* c) {
     for (int i = 0; i < n_nodes; ++i) {
         const int r0 = rp[2 * i], r1 = rp[2 * i + 1], r2 = rp[2 * i + 2];
         const int len = r1 - r0;
+        if (len == 1 && r2 - r1 == 1 && col[r0] == 2 * i && col[r1] == 2 * i + 1) {  // node in no element
+            deg[i] = 1;
+            continue;
+        }
         if (len != r2 - r1 || (len & 1)) return;
         for (int k = 0; k < len; k += 2) {
             const int c0 = col[r0 + k];
@@ -497,6 +518,7 @@ void lfg_destroy(lfg_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->h_state) (void)hipHostFree(ctx->h_state);
+    if (ctx->h_batch_global) (void)hipHostFree(ctx->h_batch_global);
     for (hipEvent_t e : {ctx->ev_start, ctx->ev_stop, ctx->ev_ring[0], ctx->ev_ring[1]})
         if (e) (void)hipEventDestroy(e);
     hipStream_t s = ctx->stream;
@@ -528,6 +550,7 @@ int lfg_set_mesh(lfg_ctx* ctx, int64_t n_nodes, const double* xy, int64_t n_elem
         ctx->enodes.upload(enodes, 4 * static_cast<size_t>(n_elems), ctx->stream);
         ctx->elem_mat.upload(elem_material, static_cast<size_t>(n_elems), ctx->stream);
         ctx->materials.upload(materials, 3 * static_cast<size_t>(n_materials), ctx->stream);
+        ctx->mat_mode.release();
         // node -> (element << 2 | local node) in ascending element order, for smoothing
         std::vector<int> ptr(static_cast<size_t>(n_nodes) + 1, 0);
         for (int64_t k = 0; k < 4 * n_elems; ++k) ptr[static_cast<size_t>(enodes[k]) + 1]++;
@@ -554,6 +577,24 @@ int lfg_set_materials(lfg_ctx* ctx, int64_t n_materials, const double* materials
         LFG_HIP(hipSetDevice(ctx->device));
         ctx->materials.upload(materials, 3 * static_cast<size_t>(n_materials), ctx->stream);
         sync(ctx);
+        ctx->have_K = ctx->have_system = ctx->have_solution = ctx->have_adjoint = false;
+        ctx->precond_ready = -1;
+        return LFG_OK;
+    });
+}
+
+int lfg_set_material_modes(lfg_ctx* ctx, const int32_t* modes) {
+    return guarded(ctx, [&] {
+        require(ctx->have_mesh, LFG_ERR_STATE, "lfg_set_mesh must precede lfg_set_material_modes");
+        LFG_HIP(hipSetDevice(ctx->device));
+        if (!modes) {
+            ctx->mat_mode.release();
+        } else {
+            for (int k = 0; k < ctx->n_mats; ++k)
+                require(modes[k] == LFG_PLANE_STRESS || modes[k] == LFG_PLANE_STRAIN, LFG_ERR_ARG, "unknown plane mode");
+            ctx->mat_mode.upload(modes, static_cast<size_t>(ctx->n_mats), ctx->stream);
+            sync(ctx);
+        }
         ctx->have_K = ctx->have_system = ctx->have_solution = ctx->have_adjoint = false;
         ctx->precond_ready = -1;
         return LFG_OK;
@@ -1048,6 +1089,314 @@ int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t rep
         sync(ctx);
         *avg_ms = elapsed(ctx->ev_start, ctx->ev_stop) / reps;
         ctx->have_solution = false;
+        return LFG_OK;
+    });
+}
+
+int lfg_batch_set_problems(lfg_ctx* ctx, int32_t n_problems, const int32_t* node_ptr) {
+    return guarded(ctx, [&] {
+        require(ctx->have_pattern, LFG_ERR_STATE, "lfg_set_pattern must precede lfg_batch_set_problems");
+        require(ctx->have_slices, LFG_ERR_UNSUPPORTED, "batches need a node-blocked pattern");
+        require(n_problems > 0 && node_ptr && node_ptr[0] == 0 && node_ptr[n_problems] == ctx->n_nodes, LFG_ERR_ARG,
+                "problem node ranges must start at 0 and cover every node");
+        const int n_slices = ctx->n_slices;
+        std::vector<int> slice_problem(static_cast<size_t>(n_slices));
+        std::vector<int> pslice(static_cast<size_t>(n_problems) + 1);
+        for (int p = 0; p < n_problems; ++p) {
+            require(node_ptr[p + 1] > node_ptr[p] && node_ptr[p] % 64 == 0 && node_ptr[p + 1] % 64 == 0, LFG_ERR_ARG,
+                    "problem node ranges must be non-empty multiples of 64");
+            pslice[p] = node_ptr[p] / 64;
+            for (int sl = node_ptr[p] / 64; sl < node_ptr[p + 1] / 64; ++sl) slice_problem[sl] = p;
+            // block diagonal: rows of problem p only reference problem p's nodes
+            const long long lo = 2LL * node_ptr[p], hi = 2LL * node_ptr[p + 1];
+            for (long long r = lo; r < hi; ++r)
+                for (int k = ctx->h_row_ptr[r]; k < ctx->h_row_ptr[r + 1]; ++k)
+                    require(ctx->h_col[k] >= lo && ctx->h_col[k] < hi, LFG_ERR_ARG, "an element spans two problems");
+        }
+        pslice[n_problems] = n_slices;
+        LFG_HIP(hipSetDevice(ctx->device));
+        ctx->batch_problems = n_problems;
+        ctx->batch_slice_problem.upload(slice_problem.data(), slice_problem.size(), ctx->stream);
+        ctx->batch_pslice_ptr.upload(pslice.data(), pslice.size(), ctx->stream);
+        ctx->batch_problem_done.alloc(static_cast<size_t>(n_problems));
+        ctx->batch_tickets.alloc(32 * static_cast<size_t>(n_problems));
+        LFG_HIP(hipMemsetAsync(ctx->batch_tickets.p, 0, ctx->batch_tickets.n * sizeof(unsigned int), ctx->stream));
+        ctx->batch_partials.alloc(8 * static_cast<size_t>(n_slices));
+        ctx->batch_global.alloc(1);
+        if (!ctx->h_batch_global)
+            LFG_HIP(hipHostMalloc(&ctx->h_batch_global, 2 * sizeof(lfg::BatchGlobal), hipHostMallocDefault));
+        sync(ctx);
+        ctx->have_batch = true;
+        ctx->have_batch_loads = false;
+        ctx->batch_tuned = false;
+        return LFG_OK;
+    });
+}
+
+int lfg_batch_set_loads(lfg_ctx* ctx, int32_t n_cases, const double* loads, int64_t n_constraints,
+                        const int32_t* dofs, const double* values) {
+    return guarded(ctx, [&] {
+        require(ctx->have_batch && ctx->have_K, LFG_ERR_STATE,
+                "lfg_batch_set_problems and assembly must precede lfg_batch_set_loads");
+        require(n_cases >= 1 && n_cases <= 4 && loads, LFG_ERR_ARG, "a batch takes 1 to 4 load cases");
+        require(n_constraints >= 0 && (n_constraints == 0 || (dofs && values)), LFG_ERR_ARG, "invalid constraints");
+        const size_t n = static_cast<size_t>(ctx->n);
+        std::vector<int> flag(n, 0);
+        std::vector<double> presc(n, 0.0);
+        for (int64_t k = 0; k < n_constraints; ++k) {
+            require(dofs[k] >= 0 && dofs[k] < ctx->n, LFG_ERR_ARG, "validated boundary references an unknown node");
+            flag[dofs[k]] = 1;
+            presc[dofs[k]] = values[k];
+        }
+        LFG_HIP(hipSetDevice(ctx->device));
+        ctx->constrained.upload(flag.data(), n, ctx->stream);
+        ctx->prescribed.upload(presc.data(), n, ctx->stream);
+        ctx->Kc.alloc(static_cast<size_t>(ctx->nnz));
+        ctx->fc.alloc(n);
+        const size_t vec = n * static_cast<size_t>(n_cases);
+        ctx->batch_b.alloc(vec);
+        for (int c = 0; c < n_cases; ++c) {
+            ctx->f.upload(loads + static_cast<size_t>(c) * n, n, ctx->stream);
+            lfg::launch_dirichlet(ctx->stream, csr_view(ctx), ctx->K.p, ctx->f.p, ctx->constrained.p, ctx->prescribed.p,
+                                  ctx->Kc.p, ctx->fc.p);
+            lfg::launch_batch_scatter_case(ctx->stream, ctx->n, n_cases, c, ctx->fc.p, ctx->batch_b.p);
+        }
+        ctx->batch_minv.alloc(n);
+        ensure_vectors(ctx);
+        LFG_HIP(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), ctx->stream));
+        lfg::launch_batch_jacobi(ctx->stream, csr_view(ctx), ctx->Kc.p, ctx->batch_minv.p, ctx->flags.p);
+        for (DevArray<double>* v : {&ctx->batch_rhs, &ctx->batch_x, &ctx->batch_r, &ctx->batch_p, &ctx->batch_q,
+                                    &ctx->batch_xacc, &ctx->batch_res})
+            v->alloc(vec);
+        ctx->batch_tmp.alloc(n);
+        int bad = 0;
+        LFG_HIP(hipMemcpyAsync(&bad, ctx->flags.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        sync(ctx);
+        require(bad == 0, LFG_ERR_NONFINITE, "Jacobi preconditioner encountered a non-finite diagonal");
+        ctx->batch_cases = n_cases;
+        ctx->batch_sys.alloc(static_cast<size_t>(ctx->batch_problems) * n_cases);
+        ctx->have_system = true;
+        ctx->have_solution = ctx->have_adjoint = false;
+        ctx->k32_valid = false;
+        ctx->slices64_valid = ctx->slices32_valid = false;
+        ctx->precond_ready = -1;
+        ctx->have_batch_loads = true;
+        return LFG_OK;
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+lfg::BatchView batch_view(lfg_ctx* c, const lfg_batch_options* opt) {
+    lfg::BatchView bv{};
+    bv.n_problems = c->batch_problems;
+    bv.n_cases = c->batch_cases;
+    bv.n_slices = c->n_slices;
+    bv.slice_problem = c->batch_slice_problem.p;
+    bv.pslice_ptr = c->batch_pslice_ptr.p;
+    bv.x = c->batch_x.p;
+    bv.r = c->batch_r.p;
+    bv.p = c->batch_p.p;
+    bv.q = c->batch_q.p;
+    bv.b = c->batch_rhs.p;
+    bv.minv = c->batch_minv.p;
+    bv.sys = c->batch_sys.p;
+    bv.problem_done = c->batch_problem_done.p;
+    bv.tickets = c->batch_tickets.p;
+    bv.partials = c->batch_partials.p;
+    bv.global = c->batch_global.p;
+    bv.tol = opt->tolerance;
+    bv.stop_rule = opt->stop_rule;
+    bv.max_iter = opt->max_iterations;
+    return bv;
+}
+
+// Launch-shape autotuner: times the batch multiply (the dominant kernel) for each
+// block size and slices-per-wavefront candidate on this batch and keeps the fastest.
+void batch_autotune(lfg_ctx* c, const lfg::BatchView& bv, bool mixed) {
+    const int threads[] = {64, 128, 256, 512};
+    const int per_wave[] = {1, 2, 4};
+    lfg::launch_fill(c->stream, c->batch_p.p, static_cast<long long>(c->batch_p.n), 1.0);
+    const lfg::SliceView v = slice_view(c);
+    const double* p64 = mixed ? nullptr : c->slices64.p;
+    const float* p32 = mixed ? c->slices32.p : nullptr;
+    float best = 0.0f;
+    lfg::BatchShape chosen{256, 1};
+    LFG_HIP(hipEventRecord(c->ev_ring[0], c->stream));
+    for (int t : threads)
+        for (int w : per_wave) {
+            const lfg::BatchShape shape{t, w};
+            lfg::launch_batch_spmv(c->stream, bv, shape, v, p64, p32, 1);  // warm-up
+            LFG_HIP(hipEventRecord(c->ev_start, c->stream));
+            for (int i = 0; i < 5; ++i) lfg::launch_batch_spmv(c->stream, bv, shape, v, p64, p32, 1);
+            LFG_HIP(hipEventRecord(c->ev_stop, c->stream));
+            sync(c);
+            const float ms = elapsed(c->ev_start, c->ev_stop);
+            if (best == 0.0f || ms < best) {
+                best = ms;
+                chosen = shape;
+            }
+        }
+    LFG_HIP(hipEventRecord(c->ev_ring[1], c->stream));
+    sync(c);
+    c->batch_tune_ms = elapsed(c->ev_ring[0], c->ev_ring[1]);
+    c->batch_shape = chosen;
+    c->batch_tuned = true;
+}
+
+// One PCG pass over all systems with right-hand side batch_rhs, solution in batch_x.
+long long batch_pass(lfg_ctx* c, const lfg_batch_options* opt, const lfg::BatchView& bv, bool mixed,
+                     int use_pass_target) {
+    const lfg::SliceView v = slice_view(c);
+    const lfg::BatchShape shape = c->batch_shape;
+    const int per_graph = opt->check_every > 0 ? opt->check_every : 32;
+    LFG_HIP(hipMemsetAsync(c->batch_global.p, 0, sizeof(lfg::BatchGlobal), c->stream));
+    lfg::launch_batch_init(c->stream, bv, shape, use_pass_target);
+    auto iteration = [&] {
+        lfg::launch_batch_spmv(c->stream, bv, shape, v, mixed ? nullptr : c->slices64.p, mixed ? c->slices32.p : nullptr,
+                               0);
+        lfg::launch_batch_update(c->stream, bv, shape);
+        lfg::launch_batch_direction(c->stream, bv, shape);
+    };
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    LFG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < per_graph; ++i) iteration();
+    LFG_HIP(hipStreamEndCapture(c->stream, &graph));
+    struct GraphGuard {
+        hipGraph_t g;
+        hipGraphExec_t e;
+        ~GraphGuard() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } guard{graph, nullptr};
+    LFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    guard.e = exec;
+    const long long launch_cap = std::max<long long>(1, (opt->max_iterations + per_graph - 1) / per_graph);
+    long long launched = 0;
+    auto launch_one = [&](int slot) {
+        LFG_HIP(hipGraphLaunch(exec, c->stream));
+        LFG_HIP(hipMemcpyAsync(c->h_batch_global + slot, c->batch_global.p, sizeof(lfg::BatchGlobal),
+                               hipMemcpyDeviceToHost, c->stream));
+        LFG_HIP(hipEventRecord(c->ev_ring[slot], c->stream));
+        ++launched;
+    };
+    launch_one(0);
+    for (long long k = 0;; ++k) {
+        const int slot = static_cast<int>(k & 1);
+        if (launched < launch_cap) launch_one(1 - slot);
+        LFG_HIP(hipEventSynchronize(c->ev_ring[slot]));
+        if (c->h_batch_global[slot].done_all || launched >= launch_cap) break;
+    }
+    sync(c);
+    return launched;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg_batch_system_report* reports,
+                  lfg_batch_summary* summary) {
+    return guarded(ctx, [&] {
+        require(ctx->have_batch_loads, LFG_ERR_STATE, "lfg_batch_set_loads must precede lfg_batch_pcg");
+        require(opt != nullptr, LFG_ERR_ARG, "missing batch options");
+        require(opt->tolerance > 0.0 && opt->tolerance < 1.0, LFG_ERR_ARG, "solver tolerance must satisfy 0 < tolerance < 1");
+        require(opt->max_iterations > 0, LFG_ERR_ARG, "max_iterations must be greater than zero");
+        const bool mixed = opt->precision == LFG_PRECISION_MIXED;
+        LFG_HIP(hipSetDevice(ctx->device));
+        ensure_slices(ctx, true, mixed);
+        lfg::BatchView bv = batch_view(ctx, opt);
+        if (opt->tune) {
+            if (!ctx->batch_tuned) batch_autotune(ctx, bv, mixed);
+        } else {
+            const int t = opt->block_threads > 0 ? opt->block_threads : 256;
+            const int w = opt->slices_per_wave > 0 ? opt->slices_per_wave : 1;
+            require(t % 64 == 0 && t >= 64 && t <= 512 && w >= 1, LFG_ERR_ARG,
+                    "block_threads must be 64-512 in steps of 64, slices_per_wave positive");
+            ctx->batch_shape = lfg::BatchShape{t, w};
+        }
+        const long long vec = static_cast<long long>(ctx->batch_b.n);
+        const int max_passes = mixed ? (opt->max_passes > 0 ? opt->max_passes : 4) : 1;
+        const double reduction = opt->pass_reduction > 0.0 ? opt->pass_reduction : 1.0e-6;
+        lfg::BatchView orig = bv;  // residuals against the original right-hand sides
+        orig.b = ctx->batch_b.p;
+        const lfg::SliceView v = slice_view(ctx);
+        const size_t n_sys = static_cast<size_t>(ctx->batch_problems) * ctx->batch_cases;
+        std::vector<lfg::BatchSystem> sys(n_sys);
+
+        LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+        LFG_HIP(hipMemcpyAsync(ctx->batch_rhs.p, ctx->batch_b.p, vec * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        LFG_HIP(hipMemsetAsync(ctx->batch_xacc.p, 0, vec * sizeof(double), ctx->stream));
+        long long launches = 0;
+        int passes = 0;
+        for (; passes < max_passes;) {
+            launches += batch_pass(ctx, opt, bv, mixed, passes > 0 ? 1 : 0);
+            lfg::launch_batch_accumulate(ctx->stream, vec, ctx->batch_x.p, ctx->batch_xacc.p);
+            ++passes;
+            // FP64 residual of the accumulated solution; in mixed mode it also decides the next pass
+            lfg::launch_batch_residual(ctx->stream, orig, ctx->batch_shape, v, ctx->slices64.p, ctx->batch_xacc.p,
+                                       ctx->batch_res.p, mixed ? 1 : 0, reduction);
+            if (!mixed) break;
+            LFG_HIP(hipMemcpyAsync(sys.data(), ctx->batch_sys.p, n_sys * sizeof(lfg::BatchSystem),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            sync(ctx);
+            bool any = false;
+            for (const lfg::BatchSystem& st : sys) any = any || (st.active && st.status == LFG_OK);
+            if (!any || passes == max_passes) break;
+            lfg::launch_batch_select_rhs(ctx->stream, bv, ctx->batch_shape, ctx->batch_res.p, ctx->batch_rhs.p);
+        }
+        LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+        LFG_HIP(hipMemcpyAsync(sys.data(), ctx->batch_sys.p, n_sys * sizeof(lfg::BatchSystem), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        sync(ctx);
+        const double solve_ms = elapsed(ctx->ev_start, ctx->ev_stop);
+        if (u_out) {
+            for (int c = 0; c < ctx->batch_cases; ++c) {
+                lfg::launch_batch_gather_case(ctx->stream, ctx->n, ctx->batch_cases, c, ctx->batch_xacc.p,
+                                              ctx->batch_tmp.p);
+                ctx->batch_tmp.download(u_out + static_cast<size_t>(c) * ctx->n, static_cast<size_t>(ctx->n),
+                                        ctx->stream);
+            }
+            sync(ctx);
+        }
+        int converged = 0;
+        long long max_iter = 0;
+        int first_status = LFG_OK;
+        for (size_t k = 0; k < n_sys; ++k) {
+            const lfg::BatchSystem& st = sys[k];
+            const double true_res = std::sqrt(std::max(0.0, st.true_rr));
+            const int ok = mixed ? (true_res <= st.final_target ? 1 : 0) : st.converged;
+            converged += ok;
+            max_iter = std::max(max_iter, st.iterations_total);
+            if (first_status == LFG_OK && st.status != LFG_OK) first_status = st.status;
+            if (reports) {
+                lfg_batch_system_report& r = reports[k];
+                r.converged = ok;
+                r.status = st.status != LFG_OK ? st.status : (ok ? LFG_OK : LFG_ERR_NOT_CONVERGED);
+                r.iterations = st.iterations_total;
+                r.initial_residual = st.bnorm;
+                r.final_residual = std::sqrt(std::max(0.0, st.rr));
+                r.true_residual = true_res;
+            }
+        }
+        if (summary) {
+            summary->solve_ms = solve_ms;
+            summary->tune_ms = ctx->batch_tune_ms;
+            summary->graph_launches = launches;
+            summary->passes = passes;
+            summary->block_threads = ctx->batch_shape.block_threads;
+            summary->slices_per_wave = ctx->batch_shape.slices_per_wave;
+            summary->systems_converged = converged;
+            summary->max_iterations = max_iter;
+        }
+        if (first_status != LFG_OK) throw LfgError(first_status, status_text(first_status));
+        if (converged != static_cast<int>(n_sys))
+            throw LfgError(LFG_ERR_NOT_CONVERGED, "maximum iterations reached without convergence");
         return LFG_OK;
     });
 }

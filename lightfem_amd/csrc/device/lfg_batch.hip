@@ -1,0 +1,562 @@
+// Batched Jacobi-PCG for many independent problems, each with NC load cases, on gfx950.
+//
+// The problems are one disjoint mesh whose nodes are grouped per problem in whole
+// 64-node slices (the node-slice layout of the single-problem SpMV), so a wavefront
+// always works inside one problem. Vectors hold the NC cases of a node together
+// (x[node][case][2]): one load of a 2x2 block of K serves all NC right-hand sides, and
+// the p entries the block multiplies arrive as NC consecutive pairs. Every system
+// (problem, case) has its own CG scalars and convergence flag; per-problem sums
+// (p.Ap, r.r, r.z) are finished in-kernel by the problem's last wavefront (a ticket
+// per problem, partials folded in slice order, so results are run-to-run identical).
+// A problem whose cases have all stopped is skipped by every later kernel, and every
+// kernel returns at once when all problems have stopped, so a captured graph of many
+// iterations costs almost nothing after convergence.
+#include "lfg_kernels.hpp"
+
+#include "lightfem_gpu.h"
+
+namespace lfg {
+namespace {
+
+typedef double dpair __attribute__((ext_vector_type(2)));
+typedef float fpair __attribute__((ext_vector_type(2)));
+
+template <typename VT>
+struct PairType;
+template <>
+struct PairType<double> {
+    using type = dpair;
+};
+template <>
+struct PairType<float> {
+    using type = fpair;
+};
+
+__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x) & (kWave - 1); }
+
+// Contiguous slice range of this block (XCD-remapped), waves taking every wpb-th slice.
+struct SliceRange {
+    int first, stop, step;
+};
+
+__device__ __forceinline__ SliceRange block_slices(int n_slices) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = (n_slices + gridDim.x - 1) / gridDim.x;
+    const int start = lb * per_block;
+    const int stop = min(start + per_block, n_slices);
+    return SliceRange{start + static_cast<int>(threadIdx.x) / kWave, stop, static_cast<int>(blockDim.x) / kWave};
+}
+
+// Per-problem reduction of K values, finished by the problem's last wavefront. Lane 0
+// of every wave holds the slice's values in vals; returns true (in every lane of the
+// wave) in the last wave, with the totals in out (all lanes).
+template <int K>
+__device__ __forceinline__ bool finish_problem(const BatchView& bv, int p, int s, const double* vals, double* out) {
+    const int first = bv.pslice_ptr[p];
+    const int count = bv.pslice_ptr[p + 1] - first;
+    double* part = bv.partials + static_cast<long long>(first) * K;
+    unsigned int* ticket = bv.tickets + 32 * p;
+    int last = 0;
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) publish_partial(part + (s - first) * K + k, vals[k]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        const unsigned int a = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = a == static_cast<unsigned int>(count - 1);
+    }
+    last = __shfl(last, 0, kWave);
+    if (!last) return false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double acc = 0.0;
+        for (int i = lane_id(); i < count; i += kWave) acc += read_partial(part + i * K + k);
+        out[k] = wave_sum(acc);
+    }
+    if (lane_id() == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// Marks problem p stopped once all its cases are; the last problem to stop ends the batch.
+__device__ __forceinline__ void close_problem_if_done(const BatchView& bv, int p) {
+    for (int c = 0; c < bv.n_cases; ++c)
+        if (!bv.sys[p * bv.n_cases + c].done) return;
+    bv.problem_done[p] = 1;
+    const unsigned int before =
+        __hip_atomic_fetch_add(&bv.global->problems_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (before + 1 == static_cast<unsigned int>(bv.n_problems))
+        __hip_atomic_store(&bv.global->done_all, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------------ init
+
+// x = 0, r = b, p = z = M^-1 r; per system ||r0||, target, rho (solver.cpp:560-583 per case).
+// With pass_targets (refinement passes) a system stops at max(its final target,
+// the pass target the residual kernel set).
+template <int NC>
+__global__ __launch_bounds__(512) void batch_init_kernel(BatchView bv, int use_pass_target) {
+    const SliceRange range = block_slices(bv.n_slices);
+    for (int s = range.first; s < range.stop; s += range.step) {
+        const int p = bv.slice_problem[s];
+        const int node = s * kWave + lane_id();
+        const double m0 = bv.minv[2 * node], m1 = bv.minv[2 * node + 1];
+        double vals[2 * NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const long long at = (static_cast<long long>(node) * NC + c) * 2;
+            const double r0 = bv.b[at], r1 = bv.b[at + 1];
+            const double z0 = m0 * r0, z1 = m1 * r1;
+            bv.x[at] = 0.0;
+            bv.x[at + 1] = 0.0;
+            bv.r[at] = r0;
+            bv.r[at + 1] = r1;
+            bv.p[at] = z0;
+            bv.p[at + 1] = z1;
+            vals[2 * c] = wave_sum(r0 * r0 + r1 * r1);
+            vals[2 * c + 1] = wave_sum(r0 * z0 + r1 * z1);
+        }
+        double tot[2 * NC];
+        if (!finish_problem<2 * NC>(bv, p, s, vals, tot)) continue;
+        if (lane_id() == 0) {
+            for (int c = 0; c < NC; ++c) {
+                BatchSystem* st = bv.sys + p * NC + c;
+                const double rr = tot[2 * c], rz = tot[2 * c + 1];
+                const double rnorm0 = sqrt(max0(rr));
+                double target;
+                if (use_pass_target) {
+                    target = st->pass_target;
+                } else {
+                    const double base = bv.stop_rule == LFG_STOP_REFERENCE ? ((1.0 < rnorm0) ? rnorm0 : 1.0) : rnorm0;
+                    target = bv.tol * base;
+                    st->final_target = target;
+                    st->bnorm = rnorm0;
+                    st->iterations_total = 0;
+                }
+                st->rnorm0 = rnorm0;
+                st->target = target;
+                st->rr = rr;
+                st->rho = rz;
+                st->alpha = 0.0;
+                st->beta = 0.0;
+                st->iter = 0;
+                st->status = LFG_OK;
+                st->converged = 0;
+                st->done = 0;
+                if (rnorm0 <= target) {
+                    st->converged = 1;
+                    st->done = 1;
+                } else if (!(rz > 0.0) || !isfinite(rz)) {
+                    st->status = LFG_ERR_NONFINITE;
+                    st->done = 1;
+                }
+            }
+            bv.problem_done[p] = 0;
+            close_problem_if_done(bv, p);
+        }
+    }
+}
+
+// ------------------------------------------------------------ SpMV
+
+// q = K p for the NC cases of the slice's nodes; rows summed in column order from
+// 0.0 as the CSR row loop (sparse.cpp:104-120). Returns the per-case p.q of the lane.
+template <int NC, typename VT, int kW>
+__device__ __forceinline__ void slice_multiply(const BatchView& bv, const SliceView& v, const VT* __restrict__ planes,
+                                               int s, double* pq) {
+    using P = typename PairType<VT>::type;
+    const int node = s * kWave + lane_id();
+    const int begin = v.ptr[s];
+    const int width = (v.ptr[s + 1] - begin) / kWave;
+    const P* __restrict__ upper = reinterpret_cast<const P*>(planes);
+    const P* __restrict__ lower = upper + v.blocks;
+    const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(bv.p);
+    double a0[NC], a1[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a0[c] = a1[c] = 0.0;
+    if (kW > 0 && width > 0 && width <= kW) {
+        P u[kW > 0 ? kW : 1], l[kW > 0 ? kW : 1];
+        int col[kW > 0 ? kW : 1];
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            const int at = begin + min(k, width - 1) * kWave + lane_id();
+            u[k] = __builtin_nontemporal_load(upper + at);
+            l[k] = __builtin_nontemporal_load(lower + at);
+            col[k] = __builtin_nontemporal_load(v.col + at);
+        }
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            if (k < width) {
+                const dpair* pc = p2 + static_cast<long long>(col[k]) * NC;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const dpair x = pc[c];
+                    a0[c] += static_cast<double>(u[k].x) * x.x;
+                    a0[c] += static_cast<double>(u[k].y) * x.y;
+                    a1[c] += static_cast<double>(l[k].x) * x.x;
+                    a1[c] += static_cast<double>(l[k].y) * x.y;
+                }
+            }
+        }
+    } else {
+        for (int k = 0; k < width; ++k) {
+            const int at = begin + k * kWave + lane_id();
+            const P uk = __builtin_nontemporal_load(upper + at);
+            const P lk = __builtin_nontemporal_load(lower + at);
+            const dpair* pc = p2 + static_cast<long long>(__builtin_nontemporal_load(v.col + at)) * NC;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const dpair x = pc[c];
+                a0[c] += static_cast<double>(uk.x) * x.x;
+                a0[c] += static_cast<double>(uk.y) * x.y;
+                a1[c] += static_cast<double>(lk.x) * x.x;
+                a1[c] += static_cast<double>(lk.y) * x.y;
+            }
+        }
+    }
+    dpair* q2 = reinterpret_cast<dpair*>(bv.q) + static_cast<long long>(node) * NC;
+    const dpair* pn = p2 + static_cast<long long>(node) * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const dpair qv = {a0[c], a1[c]};
+        q2[c] = qv;
+        const dpair pv = pn[c];
+        pq[c] = pv.x * a0[c] + pv.y * a1[c];
+    }
+}
+
+// q = K p, then per system alpha = rho / p.q (solver.cpp:585-595).
+template <int NC, typename VT, int kW>
+__global__ __launch_bounds__(512) void batch_spmv_kernel(BatchView bv, SliceView v, const VT* __restrict__ planes) {
+    if (bv.global->done_all) return;
+    const SliceRange range = block_slices(bv.n_slices);
+    for (int s = range.first; s < range.stop; s += range.step) {
+        const int p = bv.slice_problem[s];
+        if (bv.problem_done[p]) continue;
+        double pq[NC];
+        slice_multiply<NC, VT, kW>(bv, v, planes, s, pq);
+        double vals[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) vals[c] = wave_sum(pq[c]);
+        double tot[NC];
+        if (!finish_problem<NC>(bv, p, s, vals, tot)) continue;
+        if (lane_id() == 0) {
+            for (int c = 0; c < NC; ++c) {
+                BatchSystem* st = bv.sys + p * NC + c;
+                if (st->done) continue;
+                const double tpq = tot[c];
+                if (!(tpq > 0.0) || !isfinite(tpq)) {
+                    st->status = LFG_ERR_NOT_SPD;
+                    st->done = 1;
+                } else {
+                    st->alpha = st->rho / tpq;
+                }
+            }
+            close_problem_if_done(bv, p);
+        }
+    }
+}
+
+// Plain q = K p over all slices (no reductions, no state): timing the launch shapes.
+template <int NC, typename VT, int kW>
+__global__ __launch_bounds__(512) void batch_multiply_kernel(BatchView bv, SliceView v, const VT* __restrict__ planes) {
+    const SliceRange range = block_slices(bv.n_slices);
+    for (int s = range.first; s < range.stop; s += range.step) {
+        double pq[NC];
+        slice_multiply<NC, VT, kW>(bv, v, planes, s, pq);
+    }
+}
+
+// ------------------------------------------------------------ update and direction
+
+// x += alpha p, r -= alpha q, z = M^-1 r; per system ||r||^2 and r.z, then the
+// stop test on the recurrence residual and beta (solver.cpp:597-621).
+template <int NC>
+__global__ __launch_bounds__(512) void batch_update_kernel(BatchView bv) {
+    if (bv.global->done_all) return;
+    const SliceRange range = block_slices(bv.n_slices);
+    for (int s = range.first; s < range.stop; s += range.step) {
+        const int p = bv.slice_problem[s];
+        if (bv.problem_done[p]) continue;
+        const int node = s * kWave + lane_id();
+        const double m0 = bv.minv[2 * node], m1 = bv.minv[2 * node + 1];
+        double vals[2 * NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const BatchSystem* st = bv.sys + p * NC + c;
+            double rr = 0.0, rz = 0.0;
+            if (!st->done) {
+                const double alpha = st->alpha;
+                const long long at = (static_cast<long long>(node) * NC + c) * 2;
+                const double p0 = bv.p[at], p1 = bv.p[at + 1];
+                const double q0 = bv.q[at], q1 = bv.q[at + 1];
+                bv.x[at] += alpha * p0;
+                bv.x[at + 1] += alpha * p1;
+                const double r0 = bv.r[at] - alpha * q0;
+                const double r1 = bv.r[at + 1] - alpha * q1;
+                bv.r[at] = r0;
+                bv.r[at + 1] = r1;
+                rr = r0 * r0 + r1 * r1;
+                rz = r0 * (m0 * r0) + r1 * (m1 * r1);
+            }
+            vals[2 * c] = wave_sum(rr);
+            vals[2 * c + 1] = wave_sum(rz);
+        }
+        double tot[2 * NC];
+        if (!finish_problem<2 * NC>(bv, p, s, vals, tot)) continue;
+        if (lane_id() == 0) {
+            for (int c = 0; c < NC; ++c) {
+                BatchSystem* st = bv.sys + p * NC + c;
+                if (st->done) continue;
+                st->iter += 1;
+                st->iterations_total += 1;
+                st->rr = tot[2 * c];
+                const double rz = tot[2 * c + 1];
+                if (sqrt(max0(st->rr)) <= st->target) {
+                    st->converged = 1;
+                    st->done = 1;
+                } else if (!(rz > 0.0) || !isfinite(rz)) {
+                    st->status = LFG_ERR_NONFINITE;
+                    st->done = 1;
+                } else {
+                    st->beta = rz / st->rho;
+                    st->rho = rz;
+                    if (st->iter >= bv.max_iter) st->done = 1;
+                }
+            }
+            close_problem_if_done(bv, p);
+        }
+    }
+}
+
+// p = z + beta p for the systems still running.
+template <int NC>
+__global__ __launch_bounds__(512) void batch_direction_kernel(BatchView bv) {
+    if (bv.global->done_all) return;
+    const SliceRange range = block_slices(bv.n_slices);
+    for (int s = range.first; s < range.stop; s += range.step) {
+        const int p = bv.slice_problem[s];
+        if (bv.problem_done[p]) continue;
+        const int node = s * kWave + lane_id();
+        const double m0 = bv.minv[2 * node], m1 = bv.minv[2 * node + 1];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const BatchSystem* st = bv.sys + p * NC + c;
+            if (st->done) continue;
+            const double beta = st->beta;
+            const long long at = (static_cast<long long>(node) * NC + c) * 2;
+            bv.p[at] = m0 * bv.r[at] + beta * bv.p[at];
+            bv.p[at + 1] = m1 * bv.r[at + 1] + beta * bv.p[at + 1];
+        }
+    }
+}
+
+// ------------------------------------------------------------ FP64 residual, refinement
+
+// res = b - K x with FP64 values (x = the accumulated solution), per system ||res||.
+// With set_pass (refinement) each system's next right-hand side becomes res when it
+// is still above its final target (else zero) and its pass target max(final target,
+// reduction * ||res||).
+template <int NC, int kW>
+__global__ __launch_bounds__(512) void batch_residual_kernel(BatchView bv, SliceView v, const double* __restrict__ planes,
+                                                            const double* __restrict__ xacc, double* __restrict__ res,
+                                                            int set_pass, double reduction) {
+    const SliceRange range = block_slices(bv.n_slices);
+    for (int s = range.first; s < range.stop; s += range.step) {
+        const int p = bv.slice_problem[s];
+        const int node = s * kWave + lane_id();
+        // K xacc through the same multiply as the solver, with xacc as the multiplied vector
+        BatchView tmp = bv;
+        tmp.p = const_cast<double*>(xacc);
+        tmp.q = res;
+        double unused[NC];
+        slice_multiply<NC, double, kW>(tmp, v, planes, s, unused);
+        double vals[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const long long at = (static_cast<long long>(node) * NC + c) * 2;
+            const double r0 = bv.b[at] - res[at];
+            const double r1 = bv.b[at + 1] - res[at + 1];
+            res[at] = r0;
+            res[at + 1] = r1;
+            vals[c] = wave_sum(r0 * r0 + r1 * r1);
+        }
+        double tot[NC];
+        if (!finish_problem<NC>(bv, p, s, vals, tot)) continue;
+        if (lane_id() == 0) {
+            for (int c = 0; c < NC; ++c) {
+                BatchSystem* st = bv.sys + p * NC + c;
+                st->true_rr = tot[c];
+                if (set_pass) {
+                    const double rn = sqrt(max0(tot[c]));
+                    st->active = rn > st->final_target ? 1 : 0;
+                    const double t = reduction * rn;
+                    st->pass_target = st->final_target > t ? st->final_target : t;
+                }
+            }
+        }
+    }
+}
+
+// Next refinement right-hand side: rhs = res for systems still above target, else 0.
+template <int NC>
+__global__ __launch_bounds__(512) void batch_select_rhs_kernel(BatchView bv, const double* __restrict__ res,
+                                                              double* __restrict__ rhs) {
+    const SliceRange range = block_slices(bv.n_slices);
+    for (int s = range.first; s < range.stop; s += range.step) {
+        const int p = bv.slice_problem[s];
+        const int node = s * kWave + lane_id();
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const bool on = bv.sys[p * NC + c].active != 0;
+            const long long at = (static_cast<long long>(node) * NC + c) * 2;
+            rhs[at] = on ? res[at] : 0.0;
+            rhs[at + 1] = on ? res[at + 1] : 0.0;
+        }
+    }
+}
+
+__global__ void batch_accumulate_kernel(long long n, const double* __restrict__ x, double* __restrict__ acc) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) acc[i] += x[i];
+}
+
+// Jacobi inverse of the constrained matrix; 0 on the rows of padding nodes (in no
+// element: the row is its diagonal alone, and it is zero).
+__global__ void batch_jacobi_kernel(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                    const double* __restrict__ values, double* __restrict__ minv, int* __restrict__ bad) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    const int begin = rp[row], end = rp[row + 1];
+    if (begin == end || (end - begin == 1 && col[begin] == row && values[begin] == 0.0)) {
+        minv[row] = 0.0;
+        return;
+    }
+    double d = 0.0;
+    bool found = false;
+    for (int k = begin; k < end; ++k)
+        if (col[k] == row) {
+            d = values[k];
+            found = true;
+        }
+    if (!found || !(d != 0.0) || !isfinite(d)) {  // solver.cpp:656-669
+        atomicAdd(bad, 1);
+        minv[row] = 0.0;
+        return;
+    }
+    minv[row] = 1.0 / d;
+}
+
+// dst[node][case][2] = src (one case, dof order).
+__global__ void batch_scatter_case_kernel(int n, int n_cases, int c, const double* __restrict__ src,
+                                          double* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[(static_cast<long long>(i >> 1) * n_cases + c) * 2 + (i & 1)] = src[i];
+}
+
+__global__ void batch_gather_case_kernel(int n, int n_cases, int c, const double* __restrict__ src,
+                                         double* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[i] = src[(static_cast<long long>(i >> 1) * n_cases + c) * 2 + (i & 1)];
+}
+
+inline unsigned int blocks_for(long long work, int per_block) {
+    return static_cast<unsigned int>((work + per_block - 1) / per_block);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+
+unsigned int batch_grid(const BatchShape& shape, int n_slices) {
+    const int waves = shape.block_threads / kWave;
+    const long long need = (static_cast<long long>(n_slices) + waves * shape.slices_per_wave - 1) /
+                           (static_cast<long long>(waves) * shape.slices_per_wave);
+    const unsigned int g = static_cast<unsigned int>(need < 1 ? 1 : need);
+    return (g + 7u) & ~7u;
+}
+
+#define LFG_BATCH_NC(nc, body)                                 \
+    switch (nc) {                                              \
+        case 1: { constexpr int NC = 1; body; } break;         \
+        case 2: { constexpr int NC = 2; body; } break;         \
+        case 3: { constexpr int NC = 3; body; } break;         \
+        default: { constexpr int NC = 4; body; } break;        \
+    }
+
+void launch_batch_init(hipStream_t s, const BatchView& bv, const BatchShape& shape, int use_pass_target) {
+    const unsigned int grid = batch_grid(shape, bv.n_slices);
+    LFG_BATCH_NC(bv.n_cases, (batch_init_kernel<NC><<<grid, shape.block_threads, 0, s>>>(bv, use_pass_target)));
+}
+
+template <int NC, typename VT>
+static void spmv_nc(hipStream_t s, const BatchView& bv, const SliceView& v, const VT* planes, unsigned int grid,
+                    int threads, bool plain) {
+    if (v.max_width <= 9) {
+        if (plain) batch_multiply_kernel<NC, VT, 9><<<grid, threads, 0, s>>>(bv, v, planes);
+        else batch_spmv_kernel<NC, VT, 9><<<grid, threads, 0, s>>>(bv, v, planes);
+    } else {
+        if (plain) batch_multiply_kernel<NC, VT, 0><<<grid, threads, 0, s>>>(bv, v, planes);
+        else batch_spmv_kernel<NC, VT, 0><<<grid, threads, 0, s>>>(bv, v, planes);
+    }
+}
+
+void launch_batch_spmv(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
+                       const double* planes64, const float* planes32, int plain) {
+    const unsigned int grid = batch_grid(shape, bv.n_slices);
+    if (planes32) {
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, float>(s, bv, v, planes32, grid, shape.block_threads, plain != 0)));
+    } else {
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, double>(s, bv, v, planes64, grid, shape.block_threads, plain != 0)));
+    }
+}
+
+void launch_batch_update(hipStream_t s, const BatchView& bv, const BatchShape& shape) {
+    const unsigned int grid = batch_grid(shape, bv.n_slices);
+    LFG_BATCH_NC(bv.n_cases, (batch_update_kernel<NC><<<grid, shape.block_threads, 0, s>>>(bv)));
+}
+
+void launch_batch_direction(hipStream_t s, const BatchView& bv, const BatchShape& shape) {
+    const unsigned int grid = batch_grid(shape, bv.n_slices);
+    LFG_BATCH_NC(bv.n_cases, (batch_direction_kernel<NC><<<grid, shape.block_threads, 0, s>>>(bv)));
+}
+
+void launch_batch_residual(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
+                           const double* planes64, const double* xacc, double* res, int set_pass, double reduction) {
+    const unsigned int grid = batch_grid(shape, bv.n_slices);
+    if (v.max_width <= 9) {
+        LFG_BATCH_NC(bv.n_cases, (batch_residual_kernel<NC, 9><<<grid, shape.block_threads, 0, s>>>(
+                                     bv, v, planes64, xacc, res, set_pass, reduction)));
+    } else {
+        LFG_BATCH_NC(bv.n_cases, (batch_residual_kernel<NC, 0><<<grid, shape.block_threads, 0, s>>>(
+                                     bv, v, planes64, xacc, res, set_pass, reduction)));
+    }
+}
+
+void launch_batch_select_rhs(hipStream_t s, const BatchView& bv, const BatchShape& shape, const double* res,
+                             double* rhs) {
+    const unsigned int grid = batch_grid(shape, bv.n_slices);
+    LFG_BATCH_NC(bv.n_cases, (batch_select_rhs_kernel<NC><<<grid, shape.block_threads, 0, s>>>(bv, res, rhs)));
+}
+
+void launch_batch_accumulate(hipStream_t s, long long n, const double* x, double* acc) {
+    if (n <= 0) return;
+    batch_accumulate_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, x, acc);
+}
+
+void launch_batch_jacobi(hipStream_t s, const CsrView& a, const double* values, double* minv, int* bad) {
+    if (a.n <= 0) return;
+    batch_jacobi_kernel<<<blocks_for(a.n, 256), 256, 0, s>>>(a.n, a.row_ptr, a.col, values, minv, bad);
+}
+
+void launch_batch_scatter_case(hipStream_t s, int n, int n_cases, int c, const double* src, double* dst) {
+    if (n <= 0) return;
+    batch_scatter_case_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, n_cases, c, src, dst);
+}
+
+void launch_batch_gather_case(hipStream_t s, int n, int n_cases, int c, const double* src, double* dst) {
+    if (n <= 0) return;
+    batch_gather_case_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, n_cases, c, src, dst);
+}
+
+}  // namespace lfg

@@ -68,7 +68,7 @@ __device__ __forceinline__ bool kinematics(const Coords& c, double xi, double et
 }
 
 __device__ __forceinline__ void load_element(const MeshView& m, int e, Coords& c, double& young, double& nu,
-                                             double& thick) {
+                                             double& thick, int& mode) {
     const int4 nodes = reinterpret_cast<const int4*>(m.enodes)[e];
     const int ids[4] = {nodes.x, nodes.y, nodes.z, nodes.w};
 #pragma unroll
@@ -81,6 +81,7 @@ __device__ __forceinline__ void load_element(const MeshView& m, int e, Coords& c
     young = m.materials[3 * mat];
     nu = m.materials[3 * mat + 1];
     thick = m.materials[3 * mat + 2];
+    mode = m.mat_mode ? m.mat_mode[mat] : m.plane_mode;
 }
 
 // Row `row` of Ke (element.cpp:130-161): ke[c] = sum_gp (sum_s b[s][row] db[s][c]) * detJ * t.
@@ -131,9 +132,10 @@ __global__ __launch_bounds__(256) void ke_kernel(MeshView m, const int* __restri
     const int e = elems ? elems[slot] : slot;
     Coords c;
     double young, nu, thick;
-    load_element(m, e, c, young, nu, thick);
+    int mode;
+    load_element(m, e, c, young, nu, thick, mode);
     double ke[8];
-    const bool ok = ke_row(c, young, nu, thick, m.plane_mode, row, ke);
+    const bool ok = ke_row(c, young, nu, thick, mode, row, ke);
     if (!ok) {
         if (row == 0) atomicAdd(failures, 1);
 #pragma unroll
@@ -205,7 +207,8 @@ __global__ __launch_bounds__(128) void fields_kernel(MeshView m, const double* _
     if (e >= m.n_elems) return;
     Coords c;
     double young, nu, thick;
-    load_element(m, e, c, young, nu, thick);
+    int mode;
+    load_element(m, e, c, young, nu, thick, mode);
     const int4 nodes = reinterpret_cast<const int4*>(m.enodes)[e];
     const int ids[4] = {nodes.x, nodes.y, nodes.z, nodes.w};
     double ue[8];
@@ -216,7 +219,7 @@ __global__ __launch_bounds__(128) void fields_kernel(MeshView m, const double* _
         ue[2 * i + 1] = v.y;
     }
     double d[9], b[24], det;
-    constitutive(young, nu, m.plane_mode, d);
+    constitutive(young, nu, mode, d);
     double out[7];
     kinematics(c, 0.0, 0.0, b, det);
     field_at(b, d, ue, out);
@@ -291,9 +294,10 @@ __global__ __launch_bounds__(256) void sensitivity_kernel(MeshView m, const doub
     if (e >= m.n_elems) return;  // whole 8-lane groups leave together
     Coords c;
     double young, nu, thick;
-    load_element(m, e, c, young, nu, thick);
+    int mode;
+    load_element(m, e, c, young, nu, thick, mode);
     double ke[8];
-    ke_row(c, 1.0, nu, thick, m.plane_mode, row, ke);
+    ke_row(c, 1.0, nu, thick, mode, row, ke);
     const int4 nodes = reinterpret_cast<const int4*>(m.enodes)[e];
     const int ids[4] = {nodes.x, nodes.y, nodes.z, nodes.w};
     double ku = 0.0;

@@ -17,6 +17,7 @@ struct MeshView {
     const int* elem_mat;     // material index per element
     const double* materials; // E, nu, t per material
     int plane_mode;
+    const int* mat_mode;     // plane mode per material (batches mixing modes), or null: plane_mode
 };
 
 struct CsrView {
@@ -131,6 +132,63 @@ struct Ic0View {
 };
 void launch_ic0_apply(hipStream_t s, const Ic0View& f, const double* in, double* out);
 void launch_pcg_ic0_step(hipStream_t s, const PcgBuffers& b, const Ic0View& f);
+// ---- batched PCG (lfg_batch.hip): independent problems in whole 64-node slices,
+// NC load cases per problem stored per node ([node][case][2]).
+struct BatchSystem {      // one (problem, case)
+    double rho, alpha, beta, rr, target;
+    double rnorm0;        // ||r0|| of the current pass
+    double final_target;  // stop threshold of the whole solve (first pass: tol on ||b||)
+    double pass_target;   // refinement: this pass's threshold
+    double bnorm;         // ||b|| of the original right-hand side
+    double true_rr;       // ||b - K x||^2 in FP64 from the residual kernel
+    long long iter;       // iterations of the current pass
+    long long iterations_total;
+    int status, converged, done, active;
+};
+struct BatchGlobal {
+    unsigned int problems_done;
+    int done_all;
+    int pad[14];
+};
+struct BatchView {
+    int n_problems, n_cases, n_slices;
+    const int* slice_problem;  // problem of each slice
+    const int* pslice_ptr;     // slices of problem p: [pslice_ptr[p], pslice_ptr[p+1])
+    double* x;
+    double* r;
+    double* p;
+    double* q;
+    const double* b;
+    const double* minv;        // Jacobi inverse per dof (0 on padding rows)
+    BatchSystem* sys;
+    int* problem_done;
+    unsigned int* tickets;     // one per problem, 32 apart
+    double* partials;          // slices x 2 * n_cases
+    BatchGlobal* global;
+    double tol;
+    int stop_rule;
+    long long max_iter;
+};
+struct BatchShape {
+    int block_threads;    // 64 .. 512
+    int slices_per_wave;  // slices each wavefront walks per launch
+};
+unsigned int batch_grid(const BatchShape& shape, int n_slices);
+void launch_batch_init(hipStream_t s, const BatchView& bv, const BatchShape& shape, int use_pass_target);
+// plain = 1: the multiply alone (autotuning), no reductions or state.
+void launch_batch_spmv(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
+                       const double* planes64, const float* planes32, int plain);
+void launch_batch_update(hipStream_t s, const BatchView& bv, const BatchShape& shape);
+void launch_batch_direction(hipStream_t s, const BatchView& bv, const BatchShape& shape);
+void launch_batch_residual(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
+                           const double* planes64, const double* xacc, double* res, int set_pass, double reduction);
+void launch_batch_select_rhs(hipStream_t s, const BatchView& bv, const BatchShape& shape, const double* res,
+                             double* rhs);
+void launch_batch_accumulate(hipStream_t s, long long n, const double* x, double* acc);
+void launch_batch_jacobi(hipStream_t s, const CsrView& a, const double* values, double* minv, int* bad);
+void launch_batch_scatter_case(hipStream_t s, int n, int n_cases, int c, const double* src, double* dst);
+void launch_batch_gather_case(hipStream_t s, int n, int n_cases, int c, const double* src, double* dst);
+
 // ||b - K x|| in FP64 (reporting).
 void launch_residual_norm(hipStream_t s, const CsrView& a, const double* vals64, const double* x, const double* bvec,
                           double* partials, TicketSet* tickets, double* out2);

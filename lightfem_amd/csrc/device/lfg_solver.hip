@@ -477,17 +477,24 @@ __global__ __launch_bounds__(256) void pack_slices_kernel(int n_nodes, int n_sli
     const int begin = ptr[s];
     const int width = (ptr[s + 1] - begin) / kWave;
     int r0 = 0, r1 = 0, deg = 0;
+    bool lone = false;  // a node in no element: rows hold only their diagonals
     if (node < n_nodes) {
         r0 = rp[2 * node];
         r1 = rp[2 * node + 1];
-        deg = (r1 - r0) >> 1;
+        lone = r1 - r0 == 1;
+        deg = lone ? 1 : (r1 - r0) >> 1;
     }
     const int self = node < n_nodes ? node : n_nodes - 1;
     for (int k = 0; k < width; ++k) {
         const long long at = begin + static_cast<long long>(k) * kWave + lane;
         double u0 = 0.0, u1 = 0.0, l0 = 0.0, l1 = 0.0;
         int c = self;
-        if (k < deg) {
+        if (lone && k == 0) {
+            if (values) {
+                u0 = values[r0];
+                l1 = values[r1];
+            }
+        } else if (k < deg) {
             c = col[r0 + 2 * k] >> 1;
             if (values) {
                 u0 = values[r0 + 2 * k];

@@ -178,3 +178,38 @@ def c4_density(n: int = 400, seed: int = SEED_DENSITY) -> np.ndarray:
 
 def c4_model(n: int = 400, seed: int = SEED_DENSITY) -> Q4Model:
     return structured_cantilever(n, n, young=simp_young(c4_density(n, seed)))
+
+
+def c5_problems(n_problems: int = 256, seed: int = SEED_BATCH, n_min: int = 32, n_max: int = 256,
+                n_cases: int = 4) -> list:
+    """Uneven batch: nx, ny ~ U{n_min..n_max} on an nx x ny-unit rectangle, nu ~ U[0.2, 0.45],
+    plane stress or strain 50/50, per-element E log-uniform in [1e-2, 1], left edge fixed,
+    n_cases right-edge tractions of unit total force at angles theta ~ U[0, 2 pi).
+
+    Draws: 8 per problem from splitmix64(seed) (nx, ny, nu, mode, 4 angles), and the
+    element moduli from splitmix64(seed + 1 + p)."""
+    span = n_max - n_min + 1
+    draws = uniform01(seed, 8 * n_problems).reshape(n_problems, 8)
+    out = []
+    for p in range(n_problems):
+        d = draws[p]
+        nx = n_min + min(int(d[0] * span), span - 1)
+        ny = n_min + min(int(d[1] * span), span - 1)
+        out.append({"nx": nx, "ny": ny, "nu": 0.2 + 0.25 * d[2],
+                    "plane_mode": "plane_strain" if d[3] < 0.5 else "plane_stress",
+                    "theta": 2.0 * np.pi * d[4:4 + n_cases],
+                    "young": loguniform_young(nx * ny, seed + 1 + p, decades=2.0)})
+    return out
+
+
+def c5_model(spec: dict) -> Q4Model:
+    """One batch problem (first load case as the model's load)."""
+    return structured_cantilever(spec["nx"], spec["ny"], lx=float(spec["nx"]), ly=float(spec["ny"]),
+                                 young=spec["young"], nu=spec["nu"], plane_mode=spec["plane_mode"],
+                                 total_load=(float(np.cos(spec["theta"][0])), float(np.sin(spec["theta"][0]))))
+
+
+def c5_case_loads(model: Q4Model, spec: dict) -> np.ndarray:
+    """(cases, loaded nodes, 2) tractions: the model's right-edge shares times (cos, sin) theta."""
+    share = np.hypot(model.load_values[:, 0], model.load_values[:, 1])
+    return np.stack([np.stack([share * np.cos(t), share * np.sin(t)], axis=1) for t in spec["theta"]])

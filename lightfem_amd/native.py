@@ -27,7 +27,8 @@ GPU_SYMBOLS = ["lfg_create", "lfg_destroy", "lfg_last_error", "lfg_status_string
                "lfg_set_colors", "lfg_element_stiffness", "lfg_assemble", "lfg_get_stiffness", "lfg_set_stiffness",
                "lfg_set_load", "lfg_dirichlet", "lfg_get_system", "lfg_pcg", "lfg_reactions", "lfg_postprocess",
                "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv", "lfg_bench_copy", "lfg_set_materials",
-               "lfg_adjoint_solve", "lfg_sensitivity"]
+               "lfg_adjoint_solve", "lfg_sensitivity", "lfg_set_material_modes", "lfg_batch_set_problems",
+               "lfg_batch_set_loads", "lfg_batch_pcg"]
 HOST_SYMBOLS = ["lfh_csr_pattern_nnz", "lfh_csr_pattern", "lfh_rcm", "lfh_element_colors", "lfh_csr_profile"]
 
 
@@ -61,6 +62,28 @@ class PcgReport(ctypes.Structure):
 class AssemblyReport(ctypes.Structure):
     _fields_ = [("element_stiffness_ms", ctypes.c_double), ("assembly_ms", ctypes.c_double),
                 ("coloring_count", ctypes.c_int32), ("jacobian_failures", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class BatchOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("stop_rule", ctypes.c_int32), ("check_every", ctypes.c_int32),
+                ("max_passes", ctypes.c_int32), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int64),
+                ("pass_reduction", ctypes.c_double), ("tune", ctypes.c_int32), ("block_threads", ctypes.c_int32),
+                ("slices_per_wave", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class BatchSystemReport(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int32), ("status", ctypes.c_int32), ("iterations", ctypes.c_int64),
+                ("initial_residual", ctypes.c_double), ("final_residual", ctypes.c_double),
+                ("true_residual", ctypes.c_double)]
+
+
+class BatchSummary(ctypes.Structure):
+    _fields_ = [("solve_ms", ctypes.c_double), ("tune_ms", ctypes.c_double), ("graph_launches", ctypes.c_int64),
+                ("passes", ctypes.c_int32), ("block_threads", ctypes.c_int32), ("slices_per_wave", ctypes.c_int32),
+                ("systems_converged", ctypes.c_int32), ("max_iterations", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -109,6 +132,10 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lfg_set_materials.argtypes = [_P, _I64, _P]
         lib.lfg_adjoint_solve.argtypes = [_P, ctypes.POINTER(PcgOptions), _P, _P, ctypes.POINTER(PcgReport)]
         lib.lfg_sensitivity.argtypes = [_P, _P, _P, _P]
+        lib.lfg_set_material_modes.argtypes = [_P, _P]
+        lib.lfg_batch_set_problems.argtypes = [_P, ctypes.c_int32, _P]
+        lib.lfg_batch_set_loads.argtypes = [_P, ctypes.c_int32, _P, _I64, _P, _P]
+        lib.lfg_batch_pcg.argtypes = [_P, ctypes.POINTER(BatchOptions), _P, _P, ctypes.POINTER(BatchSummary)]
         _gpu = lib
     return _gpu
 
@@ -243,6 +270,41 @@ class Device:
     def set_materials(self, materials) -> None:
         materials = _f64(materials).reshape(-1, 3)
         self._check(self._lib.lfg_set_materials(self._ctx, materials.shape[0], _ptr(materials)))
+
+    def set_material_modes(self, modes) -> None:
+        modes = None if modes is None else _i32(modes)
+        self._check(self._lib.lfg_set_material_modes(self._ctx, _ptr(modes)))
+
+    def batch_set_problems(self, node_ptr) -> None:
+        node_ptr = _i32(node_ptr)
+        self._check(self._lib.lfg_batch_set_problems(self._ctx, node_ptr.shape[0] - 1, _ptr(node_ptr)))
+        self.batch_problems = node_ptr.shape[0] - 1
+
+    def batch_set_loads(self, loads, dofs, values) -> None:
+        loads = _f64(loads).reshape(-1, self.n)
+        dofs, values = _i32(dofs), _f64(values)
+        self._check(self._lib.lfg_batch_set_loads(self._ctx, loads.shape[0], _ptr(loads), dofs.shape[0], _ptr(dofs),
+                                                  _ptr(values)))
+        self.batch_cases = loads.shape[0]
+
+    def batch_pcg(self, precision: str = "fp64", stop_rule: str = "relative", tolerance: float = 1e-10,
+                  max_iterations: int = 1_000_000, check_every: int = 0, max_passes: int = 0,
+                  pass_reduction: float = 0.0, tune: bool = True, block_threads: int = 0, slices_per_wave: int = 0,
+                  raise_on_error: bool = True):
+        """All (problem, case) systems; returns u (cases x dofs), per-system reports, summary."""
+        opt = BatchOptions(PRECISION[precision], STOP[stop_rule], check_every, max_passes, tolerance, max_iterations,
+                           pass_reduction, int(tune), block_threads, slices_per_wave, 0)
+        n_sys = self.batch_problems * self.batch_cases
+        reports = (BatchSystemReport * n_sys)()
+        summary = BatchSummary()
+        u = np.empty((self.batch_cases, self.n))
+        rc = self._lib.lfg_batch_pcg(self._ctx, ctypes.byref(opt), _ptr(u), ctypes.cast(reports, _P),
+                                     ctypes.byref(summary))
+        if rc != OK and raise_on_error:
+            self._check(rc)
+        rep = {name: np.array([getattr(r, name) for r in reports]).reshape(self.batch_problems, self.batch_cases)
+               for name, _ in BatchSystemReport._fields_}
+        return u, rep, summary.as_dict()
 
     def set_pattern(self, row_ptr, col, emap) -> None:
         row_ptr, col, emap = _i32(row_ptr), _i32(col), _i32(emap)

@@ -33,6 +33,14 @@ class IndexedModel:
     cdofs: np.ndarray       # constrained dofs, ascending
     cvals: np.ndarray
     rcm_order: Optional[np.ndarray] = None
+    load_pos: Optional[np.ndarray] = None  # node index of each [loads] row
+
+    def load_vector(self, values: np.ndarray) -> np.ndarray:
+        """Nodal load vector for per-row (fx, fy) values of the model's [loads] rows."""
+        load = np.zeros(2 * self.n_nodes)
+        np.add.at(load, 2 * self.load_pos, values[:, 0])
+        np.add.at(load, 2 * self.load_pos + 1, values[:, 1])
+        return load
 
     @property
     def n_nodes(self) -> int:
@@ -90,7 +98,8 @@ def index_model(model: Q4Model, reordering: str = "none") -> IndexedModel:
     plane = native.PLANE_STRAIN if model.settings.get("plane_mode") == "plane_strain" else native.PLANE_STRESS
     return IndexedModel(node_ids=order_ids.astype(np.int64), xy=xy, enodes=enodes.astype(np.int64),
                         elem_ids=model.elem_ids[elem_sort], elem_mat=elem_mat, materials=model.mat_props[mat_sort],
-                        plane_mode=plane, load=load, cdofs=dofs_a[order], cvals=vals_a[order], rcm_order=rcm_order)
+                        plane_mode=plane, load=load, cdofs=dofs_a[order], cvals=vals_a[order], rcm_order=rcm_order,
+                        load_pos=lpos)
 
 
 @dataclasses.dataclass
@@ -141,3 +150,73 @@ def solve(model: Q4Model, assembly: str = "serial", reordering: str = "none", pr
     if fields:
         out["centre"], out["gauss_points"], out["nodal"] = dev.postprocess()
     return out
+
+
+@dataclasses.dataclass
+class BatchSystem:
+    """Independent problems merged into one disjoint mesh on one device."""
+
+    device: native.Device
+    indexed: list            # IndexedModel per problem
+    node_ptr: np.ndarray     # (P+1,) node ranges, multiples of 64 (padding nodes unused)
+    loads: np.ndarray        # (cases, dofs) merged load vectors
+    cdofs: np.ndarray
+    cvals: np.ndarray
+    row_ptr: np.ndarray
+    col: np.ndarray
+    host_ms: dict
+
+    def split(self, u: np.ndarray) -> list:
+        """Per-problem (cases, problem dofs) views of a (cases, dofs) batch solution."""
+        return [u[:, 2 * self.node_ptr[p]:2 * (self.node_ptr[p] + ix.n_nodes)] for p, ix in enumerate(self.indexed)]
+
+
+def batch_setup(models: list, case_loads: list, device: Optional[native.Device] = None,
+                assembly: str = "coloring") -> BatchSystem:
+    """Numbers every model (model.cpp:24-92), pads each to whole 64-node slices, merges them
+    into one disjoint mesh with one material table (plane mode per material), builds the
+    pattern and colours of the union on the host, assembles on the device and applies each
+    load case's Dirichlet elimination. case_loads[p] is (cases, rows of the model's [loads], 2)."""
+    host_ms = {}
+    t0 = time.perf_counter()
+    ixs = [index_model(m) for m in models]
+    n_cases = int(np.asarray(case_loads[0]).shape[0])
+    sizes = np.array([ix.n_nodes for ix in ixs], dtype=np.int64)
+    padded = (sizes + 63) // 64 * 64
+    node_ptr = np.concatenate([[0], np.cumsum(padded)]).astype(np.int64)
+    n_nodes = int(node_ptr[-1])
+    xy = np.zeros((n_nodes, 2))
+    enodes, elem_mat, mats, modes, cdofs, cvals = [], [], [], [], [], []
+    loads = np.zeros((n_cases, 2 * n_nodes))
+    mat_off = 0
+    for p, ix in enumerate(ixs):
+        off = int(node_ptr[p])
+        xy[off:off + ix.n_nodes] = ix.xy
+        enodes.append(ix.enodes + off)
+        elem_mat.append(ix.elem_mat + mat_off)
+        mats.append(ix.materials)
+        modes.append(np.full(ix.materials.shape[0], ix.plane_mode, dtype=np.int32))
+        mat_off += ix.materials.shape[0]
+        cdofs.append(ix.cdofs + 2 * off)
+        cvals.append(ix.cvals)
+        for c in range(n_cases):
+            loads[c, 2 * off:2 * (off + ix.n_nodes)] = ix.load_vector(np.asarray(case_loads[p][c]))
+    enodes = np.concatenate(enodes)
+    elem_mat = np.concatenate(elem_mat)
+    host_ms["index_ms"] = 1e3 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    row_ptr, col, emap = native.csr_pattern(n_nodes, enodes)
+    host_ms["pattern_ms"] = 1e3 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    _, color_ptr, color_elems = native.element_colors(n_nodes, enodes)
+    host_ms["coloring_ms"] = 1e3 * (time.perf_counter() - t0)
+    dev = device or native.Device(0)
+    dev.set_mesh(xy, enodes, elem_mat, np.concatenate(mats), native.PLANE_STRESS)
+    dev.set_material_modes(np.concatenate(modes))
+    dev.set_pattern(row_ptr, col, emap)
+    dev.set_colors(color_ptr, color_elems)
+    dev.assemble(assembly)
+    dev.batch_set_problems(node_ptr)
+    cd, cv = np.concatenate(cdofs), np.concatenate(cvals)
+    dev.batch_set_loads(loads, cd, cv)
+    return BatchSystem(dev, ixs, node_ptr, loads, cd, cv, row_ptr, col, host_ms)
