@@ -180,7 +180,7 @@ int lfg_sensitivity(lfg_ctx* ctx, const double* lambda, const double* dE_drho, d
  * (solver.cpp:532-627) with per-system scalars and convergence. */
 int lfg_batch_set_problems(lfg_ctx* ctx, int32_t n_problems, const int32_t* node_ptr);
 
-/* After assembly: n_cases (1-4) load cases, loads[c * n_dofs + i], and the Dirichlet
+/* After assembly: n_cases (1, 2 or 4) load cases, loads[c * n_dofs + i], and the Dirichlet
  * supports shared by all cases (apply_dirichlet, solver.cpp:355-413, per case). */
 int lfg_batch_set_loads(lfg_ctx* ctx, int32_t n_cases, const double* loads, int64_t n_constraints,
                         const int32_t* dofs, const double* values);

@@ -121,7 +121,8 @@ struct lfg_ctx {
 
     // batch of independent problems
     int batch_problems = 0, batch_cases = 0;
-    bool have_batch = false, have_batch_loads = false, batch_tuned = false;
+    bool have_batch = false, have_batch_loads = false;
+    int batch_tuned_for = -1;  // precision the cached launch shape was timed with
     lfg::BatchShape batch_shape{256, 1};
     double batch_tune_ms = 0.0;
     DevArray<int> batch_slice_problem, batch_pslice_ptr, batch_problem_done;
@@ -1128,7 +1129,7 @@ int lfg_batch_set_problems(lfg_ctx* ctx, int32_t n_problems, const int32_t* node
         sync(ctx);
         ctx->have_batch = true;
         ctx->have_batch_loads = false;
-        ctx->batch_tuned = false;
+        ctx->batch_tuned_for = -1;
         return LFG_OK;
     });
 }
@@ -1138,7 +1139,8 @@ int lfg_batch_set_loads(lfg_ctx* ctx, int32_t n_cases, const double* loads, int6
     return guarded(ctx, [&] {
         require(ctx->have_batch && ctx->have_K, LFG_ERR_STATE,
                 "lfg_batch_set_problems and assembly must precede lfg_batch_set_loads");
-        require(n_cases >= 1 && n_cases <= 4 && loads, LFG_ERR_ARG, "a batch takes 1 to 4 load cases");
+        require((n_cases == 1 || n_cases == 2 || n_cases == 4) && loads, LFG_ERR_ARG,
+                "a batch takes 1, 2 or 4 load cases");
         require(n_constraints >= 0 && (n_constraints == 0 || (dofs && values)), LFG_ERR_ARG, "invalid constraints");
         const size_t n = static_cast<size_t>(ctx->n);
         std::vector<int> flag(n, 0);
@@ -1243,7 +1245,7 @@ void batch_autotune(lfg_ctx* c, const lfg::BatchView& bv, bool mixed) {
     sync(c);
     c->batch_tune_ms = elapsed(c->ev_ring[0], c->ev_ring[1]);
     c->batch_shape = chosen;
-    c->batch_tuned = true;
+    c->batch_tuned_for = mixed ? LFG_PRECISION_MIXED : LFG_PRECISION_FP64;
 }
 
 // One PCG pass over all systems with right-hand side batch_rhs, solution in batch_x.
@@ -1311,7 +1313,7 @@ int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg
         ensure_slices(ctx, true, mixed);
         lfg::BatchView bv = batch_view(ctx, opt);
         if (opt->tune) {
-            if (!ctx->batch_tuned) batch_autotune(ctx, bv, mixed);
+            if (ctx->batch_tuned_for != opt->precision) batch_autotune(ctx, bv, mixed);
         } else {
             const int t = opt->block_threads > 0 ? opt->block_threads : 256;
             const int w = opt->slices_per_wave > 0 ? opt->slices_per_wave : 1;

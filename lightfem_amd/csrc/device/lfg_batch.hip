@@ -34,6 +34,17 @@ struct PairType<float> {
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x) & (kWave - 1); }
 
+// Vector kernels map lane l of sub-step j to the (node, case) pair t = 64 j + l of the
+// slice, so each load moves 64 consecutive pairs; the case of a lane is l % NC in every
+// sub-step. Sum over the lanes of one case (NC a power of two); every lane gets its
+// case's total.
+template <int NC>
+__device__ __forceinline__ double case_sum(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o >= NC; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
 // Contiguous slice range of this block (XCD-remapped), waves taking every wpb-th slice.
 struct SliceRange {
     int first, stop, step;
@@ -95,24 +106,32 @@ __device__ __forceinline__ void close_problem_if_done(const BatchView& bv, int p
 template <int NC>
 __global__ __launch_bounds__(512) void batch_init_kernel(BatchView bv, int use_pass_target) {
     const SliceRange range = block_slices(bv.n_slices);
+    const dpair* __restrict__ b2 = reinterpret_cast<const dpair*>(bv.b);
+    const dpair* __restrict__ m2 = reinterpret_cast<const dpair*>(bv.minv);
     for (int s = range.first; s < range.stop; s += range.step) {
         const int p = bv.slice_problem[s];
-        const int node = s * kWave + lane_id();
-        const double m0 = bv.minv[2 * node], m1 = bv.minv[2 * node + 1];
+        double rr = 0.0, rz = 0.0;
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            const int t = j * kWave + lane_id();
+            const long long pair = static_cast<long long>(s) * kWave * NC + t;
+            const dpair m = m2[s * kWave + t / NC];
+            const dpair r = b2[pair];
+            const dpair z = {m.x * r.x, m.y * r.y};
+            const dpair zero = {0.0, 0.0};
+            reinterpret_cast<dpair*>(bv.x)[pair] = zero;
+            reinterpret_cast<dpair*>(bv.r)[pair] = r;
+            reinterpret_cast<dpair*>(bv.p)[pair] = z;
+            rr += r.x * r.x + r.y * r.y;
+            rz += r.x * z.x + r.y * z.y;
+        }
+        rr = case_sum<NC>(rr);
+        rz = case_sum<NC>(rz);
         double vals[2 * NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const long long at = (static_cast<long long>(node) * NC + c) * 2;
-            const double r0 = bv.b[at], r1 = bv.b[at + 1];
-            const double z0 = m0 * r0, z1 = m1 * r1;
-            bv.x[at] = 0.0;
-            bv.x[at + 1] = 0.0;
-            bv.r[at] = r0;
-            bv.r[at + 1] = r1;
-            bv.p[at] = z0;
-            bv.p[at + 1] = z1;
-            vals[2 * c] = wave_sum(r0 * r0 + r1 * r1);
-            vals[2 * c + 1] = wave_sum(r0 * z0 + r1 * z1);
+        for (int k = 0; k < NC; ++k) {
+            vals[2 * k] = __shfl(rr, k, kWave);
+            vals[2 * k + 1] = __shfl(rz, k, kWave);
         }
         double tot[2 * NC];
         if (!finish_problem<2 * NC>(bv, p, s, vals, tot)) continue;
@@ -273,32 +292,44 @@ template <int NC>
 __global__ __launch_bounds__(512) void batch_update_kernel(BatchView bv) {
     if (bv.global->done_all) return;
     const SliceRange range = block_slices(bv.n_slices);
+    const int my_case = lane_id() % NC;
+    const dpair* __restrict__ m2 = reinterpret_cast<const dpair*>(bv.minv);
+    dpair* __restrict__ x2 = reinterpret_cast<dpair*>(bv.x);
+    dpair* __restrict__ r2 = reinterpret_cast<dpair*>(bv.r);
+    const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(bv.p);
+    const dpair* __restrict__ q2 = reinterpret_cast<const dpair*>(bv.q);
     for (int s = range.first; s < range.stop; s += range.step) {
         const int p = bv.slice_problem[s];
         if (bv.problem_done[p]) continue;
-        const int node = s * kWave + lane_id();
-        const double m0 = bv.minv[2 * node], m1 = bv.minv[2 * node + 1];
+        const BatchSystem* st = bv.sys + p * NC + my_case;
+        const bool running = !st->done;
+        const double alpha = st->alpha;
+        double rr = 0.0, rz = 0.0;
+        if (running) {
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                const int t = j * kWave + lane_id();
+                const long long pair = static_cast<long long>(s) * kWave * NC + t;
+                const dpair m = m2[s * kWave + t / NC];
+                const dpair pv = p2[pair], qv = q2[pair];
+                dpair xv = x2[pair], rv = r2[pair];
+                xv.x += alpha * pv.x;
+                xv.y += alpha * pv.y;
+                rv.x = rv.x - alpha * qv.x;
+                rv.y = rv.y - alpha * qv.y;
+                x2[pair] = xv;
+                r2[pair] = rv;
+                rr += rv.x * rv.x + rv.y * rv.y;
+                rz += rv.x * (m.x * rv.x) + rv.y * (m.y * rv.y);
+            }
+        }
+        rr = case_sum<NC>(rr);
+        rz = case_sum<NC>(rz);
         double vals[2 * NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const BatchSystem* st = bv.sys + p * NC + c;
-            double rr = 0.0, rz = 0.0;
-            if (!st->done) {
-                const double alpha = st->alpha;
-                const long long at = (static_cast<long long>(node) * NC + c) * 2;
-                const double p0 = bv.p[at], p1 = bv.p[at + 1];
-                const double q0 = bv.q[at], q1 = bv.q[at + 1];
-                bv.x[at] += alpha * p0;
-                bv.x[at + 1] += alpha * p1;
-                const double r0 = bv.r[at] - alpha * q0;
-                const double r1 = bv.r[at + 1] - alpha * q1;
-                bv.r[at] = r0;
-                bv.r[at + 1] = r1;
-                rr = r0 * r0 + r1 * r1;
-                rz = r0 * (m0 * r0) + r1 * (m1 * r1);
-            }
-            vals[2 * c] = wave_sum(rr);
-            vals[2 * c + 1] = wave_sum(rz);
+        for (int k = 0; k < NC; ++k) {
+            vals[2 * k] = __shfl(rr, k, kWave);
+            vals[2 * k + 1] = __shfl(rz, k, kWave);
         }
         double tot[2 * NC];
         if (!finish_problem<2 * NC>(bv, p, s, vals, tot)) continue;
@@ -332,19 +363,26 @@ template <int NC>
 __global__ __launch_bounds__(512) void batch_direction_kernel(BatchView bv) {
     if (bv.global->done_all) return;
     const SliceRange range = block_slices(bv.n_slices);
+    const int my_case = lane_id() % NC;
+    const dpair* __restrict__ m2 = reinterpret_cast<const dpair*>(bv.minv);
+    const dpair* __restrict__ r2 = reinterpret_cast<const dpair*>(bv.r);
+    dpair* __restrict__ p2 = reinterpret_cast<dpair*>(bv.p);
     for (int s = range.first; s < range.stop; s += range.step) {
         const int p = bv.slice_problem[s];
         if (bv.problem_done[p]) continue;
-        const int node = s * kWave + lane_id();
-        const double m0 = bv.minv[2 * node], m1 = bv.minv[2 * node + 1];
+        const BatchSystem* st = bv.sys + p * NC + my_case;
+        if (st->done) continue;
+        const double beta = st->beta;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const BatchSystem* st = bv.sys + p * NC + c;
-            if (st->done) continue;
-            const double beta = st->beta;
-            const long long at = (static_cast<long long>(node) * NC + c) * 2;
-            bv.p[at] = m0 * bv.r[at] + beta * bv.p[at];
-            bv.p[at + 1] = m1 * bv.r[at + 1] + beta * bv.p[at + 1];
+        for (int j = 0; j < NC; ++j) {
+            const int t = j * kWave + lane_id();
+            const long long pair = static_cast<long long>(s) * kWave * NC + t;
+            const dpair m = m2[s * kWave + t / NC];
+            const dpair rv = r2[pair];
+            dpair pv = p2[pair];
+            pv.x = m.x * rv.x + beta * pv.x;
+            pv.y = m.y * rv.y + beta * pv.y;
+            p2[pair] = pv;
         }
     }
 }
@@ -480,7 +518,6 @@ unsigned int batch_grid(const BatchShape& shape, int n_slices) {
     switch (nc) {                                              \
         case 1: { constexpr int NC = 1; body; } break;         \
         case 2: { constexpr int NC = 2; body; } break;         \
-        case 3: { constexpr int NC = 3; body; } break;         \
         default: { constexpr int NC = 4; body; } break;        \
     }
 
