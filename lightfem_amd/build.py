@@ -100,7 +100,7 @@ def build_host() -> None:
 
 def build_oracle() -> None:
     make = shutil.which("make") or "make"
-    _run([make, "-C", os.path.join(ROOT, "oracle"), "port"])
+    _run([make, "-C", os.path.join(ROOT, "oracle"), "port", "probe"])
     if os.path.isdir("/root/reference/src"):
         _run([make, "-C", os.path.join(ROOT, "oracle"), "-j8", "ref"])
 

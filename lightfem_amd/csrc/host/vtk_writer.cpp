@@ -1,12 +1,16 @@
 // Result export with the reference's file layout (src/vtk_writer.cpp:110-337):
 // legacy ASCII VTK unstructured grid (deformed points, nodal and cell fields,
 // per-Gauss-point stresses) and the two CSV tables. Numbers are written with
-// std::to_chars, the shortest text that round-trips, through a buffered sink.
+// std::to_chars, the shortest text that round-trips; large tables are formatted in
+// parallel chunks (OpenMP) and written in order, so the bytes do not depend on the
+// thread count.
+#include <algorithm>
 #include <charconv>
 #include <cmath>
 #include <fstream>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "lightfem/lightfem.hpp"
 
@@ -14,32 +18,80 @@ namespace lightfem {
 
 namespace {
 
-class Sink {
+// Appends numbers to a text buffer with std::to_chars.
+struct Text {
+    std::string s;
+    Text& operator<<(const char* text) {
+        s += text;
+        return *this;
+    }
+    Text& operator<<(char c) {
+        s += c;
+        return *this;
+    }
+    Text& integer(long long v) {
+        char tmp[24];
+        const auto r = std::to_chars(tmp, tmp + sizeof tmp, v);
+        s.append(tmp, r.ptr);
+        return *this;
+    }
+    Text& real(double v) {
+        char tmp[40];
+        const auto r = std::to_chars(tmp, tmp + sizeof tmp, v);
+        if (r.ec != std::errc{}) throw std::runtime_error("failed to format a number");
+        s.append(tmp, r.ptr);
+        return *this;
+    }
+};
+
+class Sink : public Text {
 public:
     explicit Sink(const std::string& path) : path_(path), out_(path, std::ios::binary) {
         if (!out_) throw std::runtime_error("cannot open output file: " + path);
-        buf_.reserve(kChunk + 256);
+        s.reserve(kChunk + 256);
     }
     Sink& operator<<(const char* text) {
-        buf_ += text;
+        Text::operator<<(text);
         return spill();
     }
     Sink& operator<<(char c) {
-        buf_ += c;
+        Text::operator<<(c);
         return spill();
     }
     Sink& integer(long long v) {
-        char tmp[24];
-        const auto r = std::to_chars(tmp, tmp + sizeof tmp, v);
-        buf_.append(tmp, r.ptr);
+        Text::integer(v);
         return spill();
     }
     Sink& real(double v) {
-        char tmp[40];
-        const auto r = std::to_chars(tmp, tmp + sizeof tmp, v);
-        if (r.ec != std::errc{}) throw std::runtime_error("failed to format a number for " + path_);
-        buf_.append(tmp, r.ptr);
+        Text::real(v);
         return spill();
+    }
+    // Rows [0, n) formatted by row(i, Text&) in parallel chunks, written in order.
+    template <typename Row>
+    Sink& rows(std::size_t n, Row&& row) {
+        constexpr std::size_t kRows = 8192;
+        const std::size_t chunks = (n + kRows - 1) / kRows;
+        const std::size_t wave = 256;  // chunks formatted per round (bounded memory)
+        std::vector<Text> parts;
+        for (std::size_t c0 = 0; c0 < chunks; c0 += wave) {
+            const std::size_t c1 = std::min(chunks, c0 + wave);
+            parts.assign(c1 - c0, Text{});
+            bool failed = false;
+#pragma omp parallel for schedule(dynamic, 1)
+            for (std::size_t c = c0; c < c1; ++c) {
+                Text& t = parts[c - c0];
+                try {
+                    for (std::size_t i = c * kRows; i < std::min(n, (c + 1) * kRows); ++i) row(i, t);
+                } catch (...) {
+#pragma omp atomic write
+                    failed = true;
+                }
+            }
+            if (failed) throw std::runtime_error("failed to format a number for " + path_);
+            flush();
+            for (const Text& t : parts) write(t.s);
+        }
+        return *this;
     }
     void close() {
         flush();
@@ -50,17 +102,19 @@ public:
 private:
     static constexpr std::size_t kChunk = std::size_t{1} << 20;
     Sink& spill() {
-        if (buf_.size() >= kChunk) flush();
+        if (s.size() >= kChunk) flush();
         return *this;
     }
-    void flush() {
-        out_.write(buf_.data(), static_cast<std::streamsize>(buf_.size()));
+    void write(const std::string& text) {
+        out_.write(text.data(), static_cast<std::streamsize>(text.size()));
         if (!out_) throw std::runtime_error("failed while writing " + path_);
-        buf_.clear();
+    }
+    void flush() {
+        write(s);
+        s.clear();
     }
     std::string path_;
     std::ofstream out_;
-    std::string buf_;
 };
 
 void require_dofs(const Model& model, const AnalysisResult& result, const char* format) {
@@ -74,7 +128,7 @@ void require_dofs(const Model& model, const AnalysisResult& result, const char* 
 void scalar_block(Sink& out, const char* name, const std::vector<ElementResult>& cells,
                   double (*get)(const ElementResult&)) {
     out << "SCALARS " << name << " double 1\nLOOKUP_TABLE default\n";
-    for (const ElementResult& cell : cells) out.real(get(cell)) << '\n';
+    out.rows(cells.size(), [&](std::size_t i, Text& t) { t.real(get(cells[i])) << '\n'; });
 }
 
 }  // namespace
@@ -92,23 +146,25 @@ void write_vtk(const Model& model, const AnalysisResult& result, const std::stri
     Sink out(path);
     out << "# vtk DataFile Version 3.0\nLightFEM linear static Q4 result\nASCII\nDATASET UNSTRUCTURED_GRID\nPOINTS ";
     out.integer(static_cast<long long>(model.nodes.size())) << " double\n";
-    for (const Node& node : model.nodes) {
-        out.real(node.position.x + scale * u[node.dof_x]) << ' ';
-        out.real(node.position.y + scale * u[node.dof_y]) << " 0\n";
-    }
+    out.rows(model.nodes.size(), [&](std::size_t i, Text& t) {
+        const Node& node = model.nodes[i];
+        t.real(node.position.x + scale * u[node.dof_x]) << ' ';
+        t.real(node.position.y + scale * u[node.dof_y]) << " 0\n";
+    });
     out << "CELLS ";
     out.integer(static_cast<long long>(cells)) << ' ';
     out.integer(static_cast<long long>(5 * model.elements.size() + 3 * model.trusses.size())) << '\n';
-    for (const Q4Element& e : model.elements) {
-        out << '4';
-        for (const int id : e.node_ids) {
-            const Node* node = model.find_node(id);
-            if (!node) throw std::logic_error("validated element references an unknown node");
-            out << ' ';
-            out.integer(static_cast<long long>(node->dof_x / 2));
+    for (const Q4Element& e : model.elements)
+        for (const int id : e.node_ids)
+            if (!model.find_node(id)) throw std::logic_error("validated element references an unknown node");
+    out.rows(model.elements.size(), [&](std::size_t i, Text& t) {
+        t << '4';
+        for (const int id : model.elements[i].node_ids) {
+            t << ' ';
+            t.integer(static_cast<long long>(model.find_node(id)->dof_x / 2));
         }
-        out << '\n';
-    }
+        t << '\n';
+    });
     for (const TrussElement& t : model.trusses) {
         out << '2';
         for (const int id : t.node_ids) {
@@ -121,31 +177,38 @@ void write_vtk(const Model& model, const AnalysisResult& result, const std::stri
     }
     out << "CELL_TYPES ";
     out.integer(static_cast<long long>(cells)) << '\n';
-    for (std::size_t i = 0; i < model.elements.size(); ++i) out << "9\n";
+    out.rows(model.elements.size(), [](std::size_t, Text& t) { t << "9\n"; });
     for (std::size_t i = 0; i < model.trusses.size(); ++i) out << "3\n";
     out << "POINT_DATA ";
     out.integer(static_cast<long long>(model.nodes.size())) << "\nVECTORS displacement double\n";
-    for (const Node& node : model.nodes) {
-        out.real(u[node.dof_x]) << ' ';
-        out.real(u[node.dof_y]) << " 0\n";
-    }
+    out.rows(model.nodes.size(), [&](std::size_t i, Text& t) {
+        const Node& node = model.nodes[i];
+        t.real(u[node.dof_x]) << ' ';
+        t.real(u[node.dof_y]) << " 0\n";
+    });
     const char* nodal_names[3] = {"nodal_stress_xx", "nodal_stress_yy", "nodal_stress_xy"};
     for (int c = 0; c < 3; ++c) {
         out << "SCALARS " << nodal_names[c] << " double 1\nLOOKUP_TABLE default\n";
-        for (const NodeResult& nr : result.node_results) out.real(nr.stress[c]) << '\n';
+        out.rows(result.node_results.size(),
+                 [&](std::size_t i, Text& t) { t.real(result.node_results[i].stress[c]) << '\n'; });
     }
     out << "SCALARS nodal_von_mises_in_plane double 1\nLOOKUP_TABLE default\n";
-    for (const NodeResult& nr : result.node_results) out.real(nr.von_mises) << '\n';
+    out.rows(result.node_results.size(), [&](std::size_t i, Text& t) { t.real(result.node_results[i].von_mises) << '\n'; });
     out << "VECTORS reaction double\n";
-    for (const Node& node : model.nodes) {
-        out.real(result.reaction[node.dof_x]) << ' ';
-        out.real(result.reaction[node.dof_y]) << " 0\n";
-    }
+    out.rows(model.nodes.size(), [&](std::size_t i, Text& t) {
+        const Node& node = model.nodes[i];
+        t.real(result.reaction[node.dof_x]) << ' ';
+        t.real(result.reaction[node.dof_y]) << " 0\n";
+    });
     out << "SCALARS displacement_magnitude double 1\nLOOKUP_TABLE default\n";
-    for (const Node& node : model.nodes) out.real(std::hypot(u[node.dof_x], u[node.dof_y])) << '\n';
+    out.rows(model.nodes.size(), [&](std::size_t i, Text& t) {
+        const Node& node = model.nodes[i];
+        t.real(std::hypot(u[node.dof_x], u[node.dof_y])) << '\n';
+    });
     out << "CELL_DATA ";
     out.integer(static_cast<long long>(cells)) << "\nSCALARS element_id int 1\nLOOKUP_TABLE default\n";
-    for (const ElementResult& er : result.element_results) out.integer(er.element_id) << '\n';
+    out.rows(result.element_results.size(),
+             [&](std::size_t i, Text& t) { t.integer(result.element_results[i].element_id) << '\n'; });
     const auto& cellsv = result.element_results;
     scalar_block(out, "strain_xx", cellsv, [](const ElementResult& e) { return e.field.strain[0]; });
     scalar_block(out, "strain_yy", cellsv, [](const ElementResult& e) { return e.field.strain[1]; });
@@ -159,11 +222,11 @@ void write_vtk(const Model& model, const AnalysisResult& result, const std::stri
         for (int c = 0; c < 3; ++c) {
             out << "SCALARS " << stress_names[c] << "_gp";
             out.integer(g + 1) << " double 1\nLOOKUP_TABLE default\n";
-            for (const ElementResult& er : cellsv) out.real(er.gauss_points[g].field.stress[c]) << '\n';
+            out.rows(cellsv.size(), [&](std::size_t i, Text& t) { t.real(cellsv[i].gauss_points[g].field.stress[c]) << '\n'; });
         }
         out << "SCALARS von_mises_gp";
         out.integer(g + 1) << " double 1\nLOOKUP_TABLE default\n";
-        for (const ElementResult& er : cellsv) out.real(er.gauss_points[g].field.von_mises) << '\n';
+        out.rows(cellsv.size(), [&](std::size_t i, Text& t) { t.real(cellsv[i].gauss_points[g].field.von_mises) << '\n'; });
     }
     out.close();
 }
@@ -180,47 +243,48 @@ CsvExportPaths write_csv_results(const Model& model, const AnalysisResult& resul
     const auto& u = result.displacement;
     Sink nodes(paths.nodes);
     nodes << "node_id,x,y,deformed_x,deformed_y,ux,uy,reaction_x,reaction_y,stress_xx,stress_yy,stress_xy,von_mises\n";
-    for (std::size_t i = 0; i < model.nodes.size(); ++i) {
+    nodes.rows(model.nodes.size(), [&](std::size_t i, Text& t) {
         const Node& node = model.nodes[i];
         const NodeResult& nr = result.node_results[i];
-        nodes.integer(node.id) << ',';
-        nodes.real(node.position.x) << ',';
-        nodes.real(node.position.y) << ',';
-        nodes.real(node.position.x + scale * u[node.dof_x]) << ',';
-        nodes.real(node.position.y + scale * u[node.dof_y]) << ',';
-        nodes.real(u[node.dof_x]) << ',';
-        nodes.real(u[node.dof_y]) << ',';
-        nodes.real(result.reaction[node.dof_x]) << ',';
-        nodes.real(result.reaction[node.dof_y]);
+        t.integer(node.id) << ',';
+        t.real(node.position.x) << ',';
+        t.real(node.position.y) << ',';
+        t.real(node.position.x + scale * u[node.dof_x]) << ',';
+        t.real(node.position.y + scale * u[node.dof_y]) << ',';
+        t.real(u[node.dof_x]) << ',';
+        t.real(u[node.dof_y]) << ',';
+        t.real(result.reaction[node.dof_x]) << ',';
+        t.real(result.reaction[node.dof_y]);
         for (int c = 0; c < 3; ++c) {
-            nodes << ',';
-            nodes.real(nr.stress[c]);
+            t << ',';
+            t.real(nr.stress[c]);
         }
-        nodes << ',';
-        nodes.real(nr.von_mises) << '\n';
-    }
+        t << ',';
+        t.real(nr.von_mises) << '\n';
+    });
     nodes.close();
     Sink gps(paths.gauss_points);
     gps << "element_id,gauss_point,xi,eta,strain_xx,strain_yy,strain_xy,stress_xx,stress_yy,stress_xy,von_mises\n";
-    for (const ElementResult& er : result.element_results) {
+    gps.rows(result.element_results.size(), [&](std::size_t i, Text& t) {
+        const ElementResult& er = result.element_results[i];
         for (int g = 0; g < 4; ++g) {
             const GaussPointField& p = er.gauss_points[g];
-            gps.integer(er.element_id) << ',';
-            gps.integer(g + 1) << ',';
-            gps.real(p.xi) << ',';
-            gps.real(p.eta);
+            t.integer(er.element_id) << ',';
+            t.integer(g + 1) << ',';
+            t.real(p.xi) << ',';
+            t.real(p.eta);
             for (int c = 0; c < 3; ++c) {
-                gps << ',';
-                gps.real(p.field.strain[c]);
+                t << ',';
+                t.real(p.field.strain[c]);
             }
             for (int c = 0; c < 3; ++c) {
-                gps << ',';
-                gps.real(p.field.stress[c]);
+                t << ',';
+                t.real(p.field.stress[c]);
             }
-            gps << ',';
-            gps.real(p.field.von_mises) << '\n';
+            t << ',';
+            t.real(p.field.von_mises) << '\n';
         }
-    }
+    });
     gps.close();
     return paths;
 }
