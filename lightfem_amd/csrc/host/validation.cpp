@@ -3,7 +3,9 @@
 // sweep), materials, Q4 connectivity and Jacobian sign at the centre and the
 // four Gauss points, trusses, unused entities, single connected mesh, duplicate
 // constraints, rigid-body modes (rank of the constraint rows), loads and the
-// setting values, including the backend's precision and stop_rule.
+// setting values, including the backend's precision and stop_rule. Ids resolve
+// through direct tables, the per-element geometry runs in parallel, and every
+// diagnostic is emitted in the reference's order.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -20,18 +22,108 @@ namespace lightfem {
 
 namespace {
 
+// id -> position of its first record: a direct table for positive ids of moderate
+// range (the usual numbering), a hash map otherwise.
+class IdIndex {
+public:
+    template <typename T>
+    explicit IdIndex(const std::vector<T>& items) {
+        long long max_id = 0;
+        bool positive = true;
+        for (const T& item : items) {
+            positive = positive && item.id > 0;
+            max_id = std::max<long long>(max_id, item.id);
+        }
+        dense_ = positive && max_id <= 4LL * static_cast<long long>(items.size()) + 1024;
+        if (dense_) {
+            table_.assign(static_cast<std::size_t>(max_id) + 1, -1);
+            for (std::size_t i = 0; i < items.size(); ++i)
+                if (table_[items[i].id] < 0) table_[items[i].id] = static_cast<long long>(i);
+        } else {
+            map_.reserve(items.size());
+            for (std::size_t i = 0; i < items.size(); ++i) map_.emplace(items[i].id, static_cast<long long>(i));
+        }
+    }
+    long long find(int id) const {
+        if (dense_) return (id > 0 && static_cast<std::size_t>(id) < table_.size()) ? table_[id] : -1;
+        const auto it = map_.find(id);
+        return it == map_.end() ? -1 : it->second;
+    }
+
+private:
+    bool dense_ = false;
+    std::vector<long long> table_;
+    std::unordered_map<int, long long> map_;
+};
+
+// A record is a duplicate when an earlier record has its id (validation.cpp:111-130).
 template <typename T>
-void check_ids(const std::vector<T>& items, const char* what, const char* duplicate_code, ValidationReport& r) {
-    std::unordered_set<int> seen;
-    for (const T& item : items) {
+void check_ids(const std::vector<T>& items, const IdIndex& index, const char* what, const char* duplicate_code,
+               ValidationReport& r) {
+    for (std::size_t i = 0; i < items.size(); ++i) {
+        const T& item = items[i];
         if (item.id <= 0)
             r.error("E_NONPOSITIVE_ID", std::string(what) + " ID must be positive: " + std::to_string(item.id),
                     item.source_line);
-        if (!seen.insert(item.id).second)
+        if (index.find(item.id) != static_cast<long long>(i))
             r.error(duplicate_code, std::string("duplicate ") + what + " ID: " + std::to_string(item.id),
                     item.source_line);
     }
 }
+
+// Whether any two nodes lie within 1e-14 in both x and y. In x order, such a pair is
+// in one run of nodes whose consecutive x differ by at most 1e-14, and then the
+// pair's y values are at most 1e-14 apart, so some two y-neighbours of that run are
+// too. False means the reference's pairwise sweep would report nothing; true sends
+// the model through that sweep, which reports the pairs in the reference's order.
+bool may_have_coincident_nodes(const std::vector<Node>& nodes) {
+    const std::size_t n = nodes.size();
+    std::vector<std::pair<double, double>> xy(n);
+    for (std::size_t i = 0; i < n; ++i) xy[i] = {nodes[i].position.x, nodes[i].position.y};
+    for (const auto& p : xy)
+        if (!std::isfinite(p.first) || !std::isfinite(p.second)) return true;  // the reference's sweep decides
+    std::sort(xy.begin(), xy.end());
+    std::vector<std::size_t> run_start;
+    for (std::size_t i = 0; i < n; ++i)
+        if (i == 0 || !(xy[i].first - xy[i - 1].first <= 1.0e-14)) run_start.push_back(i);
+    run_start.push_back(n);
+    int found = 0;
+    const std::size_t runs = run_start.size() - 1;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : found)
+    for (std::size_t k = 0; k < runs; ++k) {
+        const std::size_t a = run_start[k], b = run_start[k + 1];
+        if (b - a < 2) continue;
+        std::vector<double> ys(b - a);
+        for (std::size_t i = a; i < b; ++i) ys[i - a] = xy[i].second;
+        std::sort(ys.begin(), ys.end());
+        for (std::size_t i = 1; i < ys.size(); ++i)
+            if (ys[i] - ys[i - 1] <= 1.0e-14) {
+                found |= 1;
+                break;
+            }
+    }
+    return found != 0;
+}
+
+// Connected components over node positions.
+struct UnionFind {
+    std::vector<long long> parent;
+    explicit UnionFind(std::size_t n) : parent(n) {
+        for (std::size_t i = 0; i < n; ++i) parent[i] = static_cast<long long>(i);
+    }
+    long long root(long long a) {
+        while (parent[a] != a) {
+            parent[a] = parent[parent[a]];
+            a = parent[a];
+        }
+        return a;
+    }
+    void join(long long a, long long b) {
+        a = root(a);
+        b = root(b);
+        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+    }
+};
 
 // det J of the bilinear map at (xi, eta); the 1/4 factors are applied per term.
 double jacobian_det(const std::array<const Node*, 4>& n, double xi, double eta) {
@@ -105,19 +197,17 @@ ValidationReport ModelValidator::validate(const Model& model) const {
     if (model.materials.empty()) r.error("E_NO_MATERIAL", "at least one material is required");
     if (model.elements.empty() && model.trusses.empty())
         r.error("E_NO_ELEMENT", "at least one Q4 element or truss member is required");
-    check_ids(model.nodes, "node", "E_DUPLICATE_NODE_ID", r);
-    check_ids(model.materials, "material", "E_DUPLICATE_MATERIAL_ID", r);
-    check_ids(model.elements, "element", "E_DUPLICATE_ELEMENT_ID", r);
-    check_ids(model.trusses, "truss", "E_DUPLICATE_TRUSS_ID", r);
+    const IdIndex node_index(model.nodes), material_index(model.materials), element_index(model.elements),
+        truss_index(model.trusses);
+    check_ids(model.nodes, node_index, "node", "E_DUPLICATE_NODE_ID", r);
+    check_ids(model.materials, material_index, "material", "E_DUPLICATE_MATERIAL_ID", r);
+    check_ids(model.elements, element_index, "element", "E_DUPLICATE_ELEMENT_ID", r);
+    check_ids(model.trusses, truss_index, "truss", "E_DUPLICATE_TRUSS_ID", r);
 
-    std::unordered_map<int, const Node*> node_of;
-    node_of.reserve(model.nodes.size());
-    for (const Node& node : model.nodes) {
-        node_of.emplace(node.id, &node);
+    for (const Node& node : model.nodes)
         if (!std::isfinite(node.position.x) || !std::isfinite(node.position.y))
             r.error("E_INVALID_COORDINATE", "node coordinates must be finite", node.source_line);
-    }
-    {  // coincident nodes: sweep in x order, compare only within the x tolerance
+    if (may_have_coincident_nodes(model.nodes)) {  // coincident nodes: sweep in x order within the x tolerance
         std::vector<std::size_t> order(model.nodes.size());
         for (std::size_t i = 0; i < order.size(); ++i) order[i] = i;
         std::sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) {
@@ -135,9 +225,7 @@ ValidationReport ModelValidator::validate(const Model& model) const {
             }
         }
     }
-    std::unordered_set<int> material_ids;
     for (const Material& m : model.materials) {
-        material_ids.insert(m.id);
         if (!std::isfinite(m.young_modulus) || m.young_modulus <= 0.0)
             r.error("E_INVALID_YOUNG_MODULUS", "Young's modulus must be finite and positive", m.source_line);
         if (!std::isfinite(m.poisson_ratio) || m.poisson_ratio <= -1.0 || m.poisson_ratio >= 0.5)
@@ -146,14 +234,41 @@ ValidationReport ModelValidator::validate(const Model& model) const {
             r.error("E_INVALID_THICKNESS", "material thickness must be finite and positive", m.source_line);
     }
 
-    std::unordered_set<int> used_nodes, used_materials, element_ids;
-    std::unordered_map<int, std::vector<int>> edges;
+    // used flags per first record of an id; components by union-find over those records
+    std::vector<char> node_used(model.nodes.size(), 0), material_used(model.materials.size(), 0);
+    UnionFind components_of(model.nodes.size());
     const double g = 0.57735026918962576451;
     const double points[5][2] = {{0.0, 0.0}, {-g, -g}, {g, -g}, {g, g}, {-g, g}};
-    for (const Q4Element& e : model.elements) {
-        element_ids.insert(e.id);
-        const std::set<int> distinct(e.node_ids.begin(), e.node_ids.end());
-        if (distinct.size() != 4) {
+    // per-element facts in parallel, diagnostics and bookkeeping in element order
+    const std::size_t n_elems = model.elements.size();
+    std::vector<std::array<long long, 4>> corner_of(n_elems);
+    std::vector<unsigned char> facts(n_elems);  // bit 0: repeated node, bit 1: bad Jacobian
+#pragma omp parallel for schedule(static)
+    for (std::size_t k = 0; k < n_elems; ++k) {
+        const Q4Element& e = model.elements[k];
+        const auto& id = e.node_ids;
+        const bool repeated = id[0] == id[1] || id[0] == id[2] || id[0] == id[3] || id[1] == id[2] ||
+                              id[1] == id[3] || id[2] == id[3];
+        bool known = true;
+        std::array<const Node*, 4> corner{{nullptr, nullptr, nullptr, nullptr}};
+        for (int i = 0; i < 4; ++i) {
+            corner_of[k][i] = node_index.find(id[i]);
+            if (corner_of[k][i] < 0) known = false;
+            else corner[i] = &model.nodes[static_cast<std::size_t>(corner_of[k][i])];
+        }
+        bool bad_jacobian = false;
+        if (known && !repeated)
+            for (const auto& p : points)
+                if (!(jacobian_det(corner, p[0], p[1]) > 1.0e-14)) {
+                    bad_jacobian = true;
+                    break;
+                }
+        facts[k] = static_cast<unsigned char>((repeated ? 1 : 0) | (bad_jacobian ? 2 : 0));
+    }
+    for (std::size_t k = 0; k < n_elems; ++k) {
+        const Q4Element& e = model.elements[k];
+        const bool repeated = facts[k] & 1;
+        if (repeated) {
             std::ostringstream nodes;
             nodes << '[' << e.node_ids[0] << ", " << e.node_ids[1] << ", " << e.node_ids[2] << ", " << e.node_ids[3]
                   << ']';
@@ -161,43 +276,33 @@ ValidationReport ModelValidator::validate(const Model& model) const {
                     "Q4 element " + std::to_string(e.id) + " must reference four distinct nodes: " + nodes.str(),
                     e.source_line);
         }
-        std::array<const Node*, 4> corner{{nullptr, nullptr, nullptr, nullptr}};
         bool known = true;
         for (int i = 0; i < 4; ++i) {
-            const auto it = node_of.find(e.node_ids[i]);
-            if (it == node_of.end()) {
+            if (corner_of[k][i] < 0) {
                 r.error("E_UNKNOWN_ELEMENT_NODE", "element " + std::to_string(e.id) + " references unknown node " +
                                                       std::to_string(e.node_ids[i]),
                         e.source_line);
                 known = false;
             } else {
-                corner[i] = it->second;
-                used_nodes.insert(e.node_ids[i]);
+                node_used[static_cast<std::size_t>(corner_of[k][i])] = 1;
             }
         }
-        if (!material_ids.count(e.material_id)) {
+        const long long mat = material_index.find(e.material_id);
+        if (mat < 0) {
             r.error("E_UNKNOWN_MATERIAL", "element " + std::to_string(e.id) + " references unknown material " +
                                               std::to_string(e.material_id),
                     e.source_line);
         } else {
-            used_materials.insert(e.material_id);
+            material_used[static_cast<std::size_t>(mat)] = 1;
         }
-        if (known && distinct.size() == 4) {
-            for (const auto& p : points) {
-                if (!(jacobian_det(corner, p[0], p[1]) > 1.0e-14)) {
-                    r.error("E_NONPOSITIVE_JACOBIAN",
-                            "element " + std::to_string(e.id) +
-                                " has non-positive or near-zero Jacobian; use counter-clockwise, "
-                                "non-self-intersecting Q4 connectivity",
-                            e.source_line);
-                    break;
-                }
-            }
-            for (int i = 0; i < 4; ++i) {
-                const int a = e.node_ids[i], b = e.node_ids[(i + 1) % 4];
-                edges[a].push_back(b);
-                edges[b].push_back(a);
-            }
+        if (known && !repeated) {
+            if (facts[k] & 2)
+                r.error("E_NONPOSITIVE_JACOBIAN",
+                        "element " + std::to_string(e.id) +
+                            " has non-positive or near-zero Jacobian; use counter-clockwise, "
+                            "non-self-intersecting Q4 connectivity",
+                        e.source_line);
+            for (int i = 0; i < 4; ++i) components_of.join(corner_of[k][i], corner_of[k][(i + 1) % 4]);
         }
     }
     for (const TrussElement& t : model.trusses) {
@@ -206,64 +311,53 @@ ValidationReport ModelValidator::validate(const Model& model) const {
                                                  std::to_string(t.node_ids[0]) + ", " + std::to_string(t.node_ids[1]) +
                                                  "]",
                     t.source_line);
-        std::array<const Node*, 2> ends{{nullptr, nullptr}};
+        std::array<long long, 2> ends{{-1, -1}};
         bool known = true;
         for (int i = 0; i < 2; ++i) {
-            const auto it = node_of.find(t.node_ids[i]);
-            if (it == node_of.end()) {
+            ends[i] = node_index.find(t.node_ids[i]);
+            if (ends[i] < 0) {
                 r.error("E_UNKNOWN_TRUSS_NODE", "truss " + std::to_string(t.id) + " references unknown node " +
                                                     std::to_string(t.node_ids[i]),
                         t.source_line);
                 known = false;
             } else {
-                ends[i] = it->second;
-                used_nodes.insert(t.node_ids[i]);
+                node_used[static_cast<std::size_t>(ends[i])] = 1;
             }
         }
-        if (!material_ids.count(t.material_id)) {
+        const long long mat = material_index.find(t.material_id);
+        if (mat < 0) {
             r.error("E_UNKNOWN_MATERIAL", "truss " + std::to_string(t.id) + " references unknown material " +
                                               std::to_string(t.material_id),
                     t.source_line);
         } else {
-            used_materials.insert(t.material_id);
+            material_used[static_cast<std::size_t>(mat)] = 1;
         }
         if (!std::isfinite(t.area) || t.area <= 0.0)
             r.error("E_INVALID_TRUSS_AREA", "truss cross-section area must be finite and positive", t.source_line);
         if (known && t.node_ids[0] != t.node_ids[1]) {
-            const double dx = ends[1]->position.x - ends[0]->position.x;
-            const double dy = ends[1]->position.y - ends[0]->position.y;
+            const Node& a = model.nodes[static_cast<std::size_t>(ends[0])];
+            const Node& b = model.nodes[static_cast<std::size_t>(ends[1])];
+            const double dx = b.position.x - a.position.x;
+            const double dy = b.position.y - a.position.y;
             if (!(std::sqrt(dx * dx + dy * dy) > 1.0e-12)) {
                 r.error("E_ZERO_LENGTH_TRUSS", "truss " + std::to_string(t.id) + " has zero length", t.source_line);
             } else {
-                edges[t.node_ids[0]].push_back(t.node_ids[1]);
-                edges[t.node_ids[1]].push_back(t.node_ids[0]);
+                components_of.join(ends[0], ends[1]);
             }
         }
     }
     for (const Node& node : model.nodes)
-        if (!used_nodes.count(node.id))
+        if (!node_used[static_cast<std::size_t>(node_index.find(node.id))])
             r.warning("W_UNUSED_NODE", "node " + std::to_string(node.id) + " is not used by any element",
                       node.source_line);
     for (const Material& m : model.materials)
-        if (!used_materials.count(m.id))
+        if (!material_used[static_cast<std::size_t>(material_index.find(m.id))])
             r.warning("W_UNUSED_MATERIAL", "material " + std::to_string(m.id) + " is not used by any element",
                       m.source_line);
-    if (!used_nodes.empty()) {  // connected components over element edges
-        std::unordered_set<int> visited;
+    {  // connected components over element edges, among the used nodes
         std::size_t components = 0;
-        for (const int start : used_nodes) {
-            if (visited.count(start)) continue;
-            ++components;
-            std::queue<int> pending;
-            pending.push(start);
-            visited.insert(start);
-            while (!pending.empty()) {
-                const int current = pending.front();
-                pending.pop();
-                for (const int next : edges[current])
-                    if (visited.insert(next).second) pending.push(next);
-            }
-        }
+        for (std::size_t i = 0; i < model.nodes.size(); ++i)
+            if (node_used[i] && components_of.root(static_cast<long long>(i)) == static_cast<long long>(i)) ++components;
         if (components > 1)
             r.error("E_DISCONNECTED_MESH",
                     "the current solver scope requires one connected mesh; found " + std::to_string(components) +
@@ -274,14 +368,14 @@ ValidationReport ModelValidator::validate(const Model& model) const {
     std::vector<std::array<double, 3>> rigid;
     bool prescribed_motion = false;
     for (const DirichletBC& bc : model.boundaries) {
-        const auto it = node_of.find(bc.node_id);
-        if (it == node_of.end()) {
+        const long long at = node_index.find(bc.node_id);
+        if (at < 0) {
             r.error("E_UNKNOWN_BOUNDARY_NODE", "boundary condition references unknown node " + std::to_string(bc.node_id),
                     bc.source_line);
             continue;
         }
         if (!bc.ux && !bc.uy) r.warning("W_EMPTY_BOUNDARY", "boundary record constrains neither ux nor uy", bc.source_line);
-        const Vec2 p = it->second->position;
+        const Vec2 p = model.nodes[static_cast<std::size_t>(at)].position;
         if (bc.ux) {
             if (!constrained.emplace(bc.node_id, 'x').second)
                 r.error("E_DUPLICATE_CONSTRAINT", "ux is constrained more than once at node " + std::to_string(bc.node_id),
@@ -303,7 +397,7 @@ ValidationReport ModelValidator::validate(const Model& model) const {
     std::unordered_map<int, std::size_t> loads_at;
     bool excited = false;
     for (const NodalLoad& load : model.loads) {
-        if (!node_of.count(load.node_id)) {
+        if (node_index.find(load.node_id) < 0) {
             r.error("E_UNKNOWN_LOAD_NODE", "load references unknown node " + std::to_string(load.node_id),
                     load.source_line);
             continue;
@@ -321,7 +415,7 @@ ValidationReport ModelValidator::validate(const Model& model) const {
                       "nonzero fy is applied to constrained uy at node " + std::to_string(load.node_id), load.source_line);
     }
     for (const BodyForce& f : model.body_forces) {
-        if (!element_ids.count(f.element_id)) {
+        if (element_index.find(f.element_id) < 0) {
             r.error("E_UNKNOWN_BODY_FORCE_ELEMENT", "body force references unknown element " + std::to_string(f.element_id),
                     f.source_line);
             continue;
@@ -329,7 +423,7 @@ ValidationReport ModelValidator::validate(const Model& model) const {
         excited |= std::abs(f.bx) > 0.0 || std::abs(f.by) > 0.0;
     }
     for (const EdgeLoad& load : model.edge_loads) {
-        if (!element_ids.count(load.element_id)) {
+        if (element_index.find(load.element_id) < 0) {
             r.error("E_UNKNOWN_EDGE_LOAD_ELEMENT", "edge load references unknown element " +
                                                        std::to_string(load.element_id),
                     load.source_line);
