@@ -36,17 +36,21 @@ def _sync():
 
 
 KERNELS = {1: "pcg_spmv_kernel<4>", 2: "pcg_spmv_kernel<8>", 3: "pcg_spmv_kernel<16>",
-           4: "pcg_spmv_stream_kernel<24, 64>", 5: "pcg_spmv_stream_kernel<24, 32>", 6: "pcg_spmv_slices_kernel<9>"}
+           4: "pcg_spmv_stream_kernel<24, 64>", 5: "pcg_spmv_stream_kernel<24, 32>",
+           6: "pcg_spmv_slices_kernel<9>", 7: "pcg_spmv_slices_dir_kernel<9, 0>"}
 
 
 def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int) -> int:
-    """Algorithmic bytes of one SpMV: the values, the column indices of the layout,
-    its row (or slice) offsets, x read once and y written once. CSR variants carry
-    one int32 column per value; the node-slice layout one per 2x2 block of four
-    values and one offset per 64 nodes (padding not counted)."""
-    if variant == 6:
+    """Algorithmic bytes of one SpMV launch: the values, the column indices of the
+    layout, its row (or slice) offsets, x read once and y written once. CSR variants
+    carry one int32 column per value; the node-slice layout one per 2x2 block of four
+    values and one offset per 64 nodes (padding not counted). Variant 7 forms the
+    search direction p = M^-1 r + beta p_old on the fly: it reads r, the Jacobi
+    inverse and p_old once and writes q and the new p."""
+    if variant in (6, 7):
         n_slices = (n // 2 + 63) // 64
-        return nnz * value_bytes + 4 * (nnz // 4) + 4 * (n_slices + 1) + 16 * n
+        matrix = nnz * value_bytes + 4 * (nnz // 4) + 4 * (n_slices + 1)
+        return matrix + (40 * n if variant == 7 else 16 * n)
     return nnz * (value_bytes + 4) + 4 * (n + 1) + 16 * n
 
 
@@ -174,7 +178,7 @@ def main() -> int:
     if args.profile:
         # Profiling run: the FP64 SpMV of the headline variant alone (args.profile launches),
         # then FP64 Jacobi-PCG iterations (a whole number of captured graphs), no other kernels.
-        variant = args.variant or 6
+        variant = args.variant or 6  # the multiply the default PCG iteration runs
         ms = dev.bench_spmv("fp64", variant, args.profile)
         b = spmv_bytes(n, nnz, 8, variant)
         prof = {"kernel": KERNELS[variant], "launches": args.profile, "avg_launch_ms": ms, "bytes": b,
@@ -192,17 +196,19 @@ def main() -> int:
         print(json.dumps(prof))
         dev.close()
         return 0
-    # SpMV kernel: average launch duration over 50 launches (hipEvents on the library stream)
-    spmv = {}
+    # SpMV kernels: average launch duration over 50 launches (hipEvents on the library
+    # stream). The solves use the library's default, variant 6 (node slices), unless
+    # --variant says otherwise.
+    solve_variant = args.variant or 6
+    spmv, sweep = {}, {}
     for precision, vbytes in (("fp64", 8), ("mixed", 4)):
-        best = None
-        for variant in (1, 2, 3, 4, 5, 6):
+        sweep[precision] = {}
+        for variant in (1, 2, 3, 4, 5, 6, 7):
             ms = dev.bench_spmv(precision, variant, 50)
-            if best is None or ms < best[1]:
-                best = (variant, ms)
-        b = spmv_bytes(n, nnz, vbytes, best[0])
-        spmv[precision] = {"variant": best[0], "kernel": KERNELS[best[0]], "ms": best[1], "bytes": b,
-                           "GBs": b / (best[1] * 1e-3) / 1e9}
+            b = spmv_bytes(n, nnz, vbytes, variant)
+            sweep[precision][variant] = {"kernel": KERNELS[variant], "ms": ms, "bytes": b,
+                                         "GBs": b / (ms * 1e-3) / 1e9}
+        spmv[precision] = dict(sweep[precision][solve_variant], variant=solve_variant)
     print(f"bench: spmv {json.dumps(spmv)}", file=sys.stderr, flush=True)
     copy_bytes = 1 << 30
     copy_ms = dev.bench_copy(copy_bytes, 20)
@@ -211,7 +217,7 @@ def main() -> int:
     forms = {}
     cap = 10 ** 6
     for form in ("fp64", "mixed"):
-        variant = args.variant or spmv[form]["variant"]
+        variant = solve_variant
         steps = args.steps if form == "fp64" else 1  # the mixed form is reported, not the headline
         warmup = args.warmup if form == "fp64" else 0
         for _ in range(warmup):
@@ -240,7 +246,7 @@ def main() -> int:
     # FP32 values from the first iteration (residual replacement only), capped at 2x the FP64 count
     t = time.perf_counter()
     u32, rep32 = dev.pcg("jacobi", "mixed", "relative", args.tol, cap, args.check_every,
-                         spmv_variant=spmv["mixed"]["variant"], mixed_switch=-1.0, raise_on_error=False)
+                         spmv_variant=solve_variant, mixed_switch=-1.0, raise_on_error=False)
     forms["fp32_values_throughout"] = {
         "ms_per_solve": (time.perf_counter() - t) * 1e3, "device_solve_ms": rep32["solve_ms"],
         "iterations": rep32["iterations"], "converged": bool(rep32["converged"]),
@@ -279,7 +285,8 @@ def main() -> int:
         "mixed_speedup_vs_fp64": round(forms["fp64"]["ms_per_solve"] / forms["mixed"]["ms_per_solve"], 4)
         if mixed_ok else None,
         "forms": forms, "per_iteration_ms": per_iter_ms,
-        "spmv": spmv, "device_copy": copy, "assembly": assembly, "setup_s": round(setup_s, 2),
+        "spmv": spmv, "spmv_variants": {p: {str(k): round(v["GBs"], 1) for k, v in d.items()} for p, d in sweep.items()},
+        "device_copy": copy, "assembly": assembly, "setup_s": round(setup_s, 2),
         "roofline": {"bound": "hbm", "kernel": f"{dominant['kernel']} (FP64 values)",
                      "achieved": round(dominant["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dominant["GBs"] / HBM_PEAK_GBS, 4),

@@ -66,7 +66,10 @@ typedef struct {
     int32_t spmv_variant;    /* 0 = default (6 when the pattern is node-blocked, else 4); 1-3: node
                                 groups of 4/8/16 lanes per row; 4, 5: row-segment streaming with
                                 64 / 32 rows per wavefront; 6: node-slice layout (one column per
-                                2x2 block, 64 nodes per wavefront) */
+                                2x2 block, 64 nodes per wavefront); 7: node-slice layout with the
+                                Jacobi / block search direction formed inside the multiply (two
+                                launches per iteration, bit-identical to 6; slower on gfx950: the
+                                extra gathers cost more than the launch they save) */
 } lfg_pcg_options;
 
 typedef struct {
@@ -230,7 +233,8 @@ int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, do
 /* y = Kc x with the PCG SpMV kernel of `variant` (values of `precision`). */
 int lfg_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x, double* y);
 /* Average device time (ms) of `reps` SpMV launches of the given precision and
- * variant on resident data. */
+ * variant on resident data (variant 7: the multiply with the Jacobi direction formed
+ * on the fly, as its PCG iteration runs it). */
 int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t reps, double* avg_ms);
 /* Average device time (ms) of a device-to-device copy of `bytes` bytes: the
  * streaming-bandwidth ceiling of this device, and the calibration kernel for

@@ -105,7 +105,7 @@ struct lfg_ctx {
     DevArray<double> prescribed;
 
     // solver
-    DevArray<double> x, r, p, q, z, minv, factor, diag, partials, scratch;
+    DevArray<double> x, r, p, p2, q, z, minv, factor, diag, partials, scratch;
     DevArray<double> lam, adj_rhs, dE, grad;  // adjoint solution and right-hand side, sensitivities
     bool have_adjoint = false;
     DevArray<lfg::PcgState> state;
@@ -167,13 +167,16 @@ lfg::SliceView slice_view(const lfg_ctx* c) {
     return lfg::SliceView{c->n_slices, c->slice_blocks, c->slice_ptr.p, c->slice_col.p, c->slice_max_width};
 }
 
-constexpr int kSliceVariant = 6;
+constexpr int kSliceVariant = 6;  // node-slice SpMV, separate direction kernel
+constexpr int kSliceFused = 7;    // node-slice SpMV forming the Jacobi / block direction on the fly
+
+bool slice_layout(int variant) { return variant == kSliceVariant || variant == kSliceFused; }
 
 // 0 selects the node-slice kernel where the pattern allows it, else row-segment streaming.
 int resolve_variant(const lfg_ctx* c, int variant) {
     if (variant == 0) return c->have_slices ? kSliceVariant : 4;
-    require(variant >= 1 && variant <= kSliceVariant, LFG_ERR_ARG, "SpMV variant must be 0-6");
-    require(variant != kSliceVariant || c->have_slices, LFG_ERR_UNSUPPORTED,
+    require(variant >= 1 && variant <= kSliceFused, LFG_ERR_ARG, "SpMV variant must be 0-7");
+    require(!slice_layout(variant) || c->have_slices, LFG_ERR_UNSUPPORTED,
             "the node-slice SpMV needs rows 2i, 2i+1 sharing one column list of dof pairs");
     return variant;
 }
@@ -465,9 +468,26 @@ lfg::PcgBuffers pcg_buffers(lfg_ctx* c, int kind, const double* rhs, double* xs)
     return b;
 }
 
-void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int variant) {
+// Whether an iteration forms its search direction inside the slice SpMV (two
+// launches per iteration, p alternating between two buffers) instead of in a
+// direction kernel after the update.
+bool fused_direction(int variant, int precond) { return variant == kSliceFused && precond != LFG_PRECOND_IC0; }
+
+// One PCG iteration. With the fused direction, iteration `parity` reads p from
+// buffer parity (b.p or c->p2) and writes the new p into the other one.
+void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int variant, int parity) {
     const lfg::CsrView a = csr_view(c);
-    if (variant == kSliceVariant)
+    if (fused_direction(variant, b.precond)) {
+        double* const pb[2] = {b.p, c->p2.p};
+        lfg::PcgBuffers bi = b;
+        bi.p = pb[parity ^ 1];
+        lfg::launch_pcg_spmv_slices_dir(c->stream, b, slice_view(c), c->slices64.p, mixed ? c->slices32.p : nullptr,
+                                        pb[parity], bi.p);
+        lfg::launch_pcg_update(c->stream, bi, mixed);
+        if (mixed) lfg::launch_pcg_replace(c->stream, bi, a, c->Kc.p);
+        return;
+    }
+    if (slice_layout(variant))
         lfg::launch_pcg_spmv_slices(c->stream, b, slice_view(c), c->slices64.p, mixed ? c->slices32.p : nullptr);
     else
         lfg::launch_pcg_spmv(c->stream, b, a, c->Kc.p, mixed ? c->K32.p : nullptr, variant);
@@ -826,7 +846,7 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     setup_preconditioner(ctx, kind, &degenerate);
     if (previous == kind && kind == LFG_PRECOND_BLOCK2) degenerate = -1;  // factor reused; count unknown here
     const int variant = resolve_variant(ctx, opt->spmv_variant);
-    if (variant == kSliceVariant) {
+    if (slice_layout(variant)) {
         ensure_slices(ctx, true, mixed != 0);
     } else if (mixed && !ctx->k32_valid) {
         ctx->K32.alloc(static_cast<size_t>(ctx->nnz));
@@ -843,7 +863,11 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     }
 
     const lfg::PcgBuffers b = pcg_buffers(ctx, kind, rhs, xs);
-    const int per_graph = opt->check_every > 0 ? opt->check_every : 32;
+    const bool fused = fused_direction(variant, kind);
+    if (fused) ctx->p2.alloc(static_cast<size_t>(ctx->n));
+    // a captured graph starts every launch on buffer 0, so it holds an even count when fused
+    int per_graph = opt->check_every > 0 ? opt->check_every : 32;
+    if (fused && (per_graph & 1)) ++per_graph;
     const double drop = opt->replace_drop > 0.0 ? opt->replace_drop : 0.1;
     const double switch_tau = !mixed ? 0.0 : (opt->mixed_switch != 0.0 ? opt->mixed_switch : 1.0e-4);
     // capture `per_graph` iterations once
@@ -852,7 +876,7 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     const bool use_graph = opt->direct_launch == 0;
     if (use_graph) {
         LFG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant);
+        for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant, i & 1);
         LFG_HIP(hipStreamEndCapture(ctx->stream, &graph));
         LFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     }
@@ -878,7 +902,7 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
         if (use_graph) {
             LFG_HIP(hipGraphLaunch(exec, ctx->stream));
         } else {
-            for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant);
+            for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant, i & 1);
         }
         LFG_HIP(hipMemcpyAsync(ctx->h_state + slot, ctx->state.p, sizeof(lfg::PcgState), hipMemcpyDeviceToHost,
                                ctx->stream));
@@ -1036,11 +1060,16 @@ int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, do
     });
 }
 
-static void run_spmv(lfg_ctx* ctx, int precision, int variant) {
+// One SpMV launch of `variant`; with fused_dir (timing only) variant 7 runs the
+// multiply its PCG iteration uses, forming p = M^-1 r + beta p_old (Jacobi) on the fly.
+static void run_spmv(lfg_ctx* ctx, int precision, int variant, bool fused_dir = false) {
     lfg::PcgBuffers b = pcg_buffers(ctx, LFG_PRECOND_JACOBI, ctx->fc.p, ctx->x.p);
     b.st = ctx->state.p;
     const bool f32 = precision == LFG_PRECISION_MIXED;  // one value array per launch
-    if (variant == kSliceVariant)
+    if (fused_dir && variant == kSliceFused)
+        lfg::launch_pcg_spmv_slices_dir(ctx->stream, b, slice_view(ctx), f32 ? nullptr : ctx->slices64.p,
+                                        f32 ? ctx->slices32.p : nullptr, ctx->p.p, ctx->p2.p);
+    else if (slice_layout(variant))
         lfg::launch_pcg_spmv_slices(ctx->stream, b, slice_view(ctx), f32 ? nullptr : ctx->slices64.p,
                                     f32 ? ctx->slices32.p : nullptr);
     else
@@ -1052,7 +1081,7 @@ static void prepare_spmv(lfg_ctx* ctx, int precision, int variant) {
     ensure_vectors(ctx);
     lfg::PcgState zero{};
     LFG_HIP(hipMemcpyAsync(ctx->state.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
-    if (variant == kSliceVariant) {
+    if (slice_layout(variant)) {
         ensure_slices(ctx, precision != LFG_PRECISION_MIXED, precision == LFG_PRECISION_MIXED);
     } else if (precision == LFG_PRECISION_MIXED && !ctx->k32_valid) {
         ctx->K32.alloc(static_cast<size_t>(ctx->nnz));
@@ -1081,11 +1110,18 @@ int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t rep
         require(ctx->have_system && reps > 0 && avg_ms, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_bench_spmv");
         LFG_HIP(hipSetDevice(ctx->device));
         const int v = resolve_variant(ctx, variant);
+        const bool fused = v == kSliceFused;  // the multiply its Jacobi / block PCG iteration runs
+        if (fused) {
+            long long degenerate = 0;
+            if (ctx->precond_ready != LFG_PRECOND_JACOBI) setup_preconditioner(ctx, LFG_PRECOND_JACOBI, &degenerate);
+            ctx->p2.alloc(static_cast<size_t>(ctx->n));
+        }
         prepare_spmv(ctx, precision, v);
         lfg::launch_fill(ctx->stream, ctx->p.p, ctx->n, 1.0);
-        run_spmv(ctx, precision, v);  // warm-up
+        lfg::launch_fill(ctx->stream, ctx->r.p, ctx->n, 1.0);
+        run_spmv(ctx, precision, v, fused);  // warm-up
         LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-        for (int i = 0; i < reps; ++i) run_spmv(ctx, precision, v);
+        for (int i = 0; i < reps; ++i) run_spmv(ctx, precision, v, fused);
         LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
         sync(ctx);
         *avg_ms = elapsed(ctx->ev_start, ctx->ev_stop) / reps;

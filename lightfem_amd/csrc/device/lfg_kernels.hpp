@@ -107,6 +107,12 @@ void launch_pack_slices(hipStream_t s, const CsrView& a, const SliceView& v, con
 void launch_pcg_spmv_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
                             const float* planes32);
 int slice_spmv_blocks(int n_slices);
+// Jacobi / 2x2 block only: the slice multiply of p = z + beta p_old, z = M^-1 r,
+// formed per gathered column (p_old, r and the preconditioner read-only), p written
+// for the own nodes into p_new; replaces launch_pcg_direction + the SpMV (variant 7:
+// 83 us against 64 + 10 us at 1000x1000, so not the default).
+void launch_pcg_spmv_slices_dir(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
+                                const float* planes32, const double* p_old, double* p_new);
 // x += alpha p, r -= alpha q, ||r||^2, r.z; convergence / replacement decision, beta.
 void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed);
 // mixed only: r = b - K x in FP64 when a replacement is due; then r.z and beta.

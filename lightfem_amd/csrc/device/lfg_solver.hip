@@ -613,6 +613,110 @@ __global__ __launch_bounds__(kThreads) void pcg_spmv_slices_kernel(PcgBuffers b,
     }
 }
 
+// The same multiply with the search direction of this iteration formed on the fly:
+// p = z + beta p_old with z = M^-1 r (solver.cpp:620), computed for every column node a
+// lane gathers (from r, the preconditioner and p_old, all read-only here) and written
+// for the lane's own node into p_new. The values equal those of the separate
+// direction kernel, so the iteration is bit-identical with one launch less.
+template <typename VT, int kW, int Prec>
+__device__ __forceinline__ double slice_rows_dir(const PcgBuffers& b, const SliceView& v, const VT* __restrict__ planes,
+                                                 int s, const double* __restrict__ p_old, double* __restrict__ p_new,
+                                                 double beta) {
+    using P = typename PairOf<VT>::type;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int node = s * kWave + lane;
+    const int begin = v.ptr[s];
+    const int width = (v.ptr[s + 1] - begin) / kWave;
+    const P* __restrict__ upper = reinterpret_cast<const P*>(planes);
+    const P* __restrict__ lower = upper + v.blocks;
+    const dpair* __restrict__ r2 = reinterpret_cast<const dpair*>(b.r);
+    const dpair* __restrict__ po2 = reinterpret_cast<const dpair*>(p_old);
+    auto direction = [&](int c) {
+        const dpair r = r2[c];
+        double z0, z1;
+        precond_node<Prec>(b.m, c, r.x, r.y, z0, z1);
+        const dpair po = po2[c];
+        const dpair pn = {z0 + beta * po.x, z1 + beta * po.y};
+        return pn;
+    };
+    double a0 = 0.0, a1 = 0.0;
+    if (kW > 0 && width > 0 && width <= kW) {
+        P u[kW > 0 ? kW : 1], l[kW > 0 ? kW : 1];
+        int c[kW > 0 ? kW : 1];
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            const int at = begin + min(k, width - 1) * kWave + lane;
+            u[k] = __builtin_nontemporal_load(upper + at);
+            l[k] = __builtin_nontemporal_load(lower + at);
+            c[k] = __builtin_nontemporal_load(v.col + at);
+        }
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            if (k < width) {
+                const dpair x = direction(c[k]);
+                a0 += static_cast<double>(u[k].x) * x.x;
+                a0 += static_cast<double>(u[k].y) * x.y;
+                a1 += static_cast<double>(l[k].x) * x.x;
+                a1 += static_cast<double>(l[k].y) * x.y;
+            }
+        }
+    } else {
+        for (int k = 0; k < width; ++k) {
+            const int at = begin + k * kWave + lane;
+            const P uk = __builtin_nontemporal_load(upper + at);
+            const P lk = __builtin_nontemporal_load(lower + at);
+            const dpair x = direction(__builtin_nontemporal_load(v.col + at));
+            a0 += static_cast<double>(uk.x) * x.x;
+            a0 += static_cast<double>(uk.y) * x.y;
+            a1 += static_cast<double>(lk.x) * x.x;
+            a1 += static_cast<double>(lk.y) * x.y;
+        }
+    }
+    double pq = 0.0;
+    if (node < (b.n >> 1)) {
+        const dpair qv = {a0, a1};
+        reinterpret_cast<dpair*>(b.q)[node] = qv;
+        const dpair pv = direction(node);
+        reinterpret_cast<dpair*>(p_new)[node] = pv;
+        pq = pv.x * a0 + pv.y * a1;
+    }
+    return pq;
+}
+
+template <int kW, int Prec>
+__global__ __launch_bounds__(kThreads) void pcg_spmv_slices_dir_kernel(PcgBuffers b, SliceView v,
+                                                                       const double* __restrict__ planes64,
+                                                                       const float* __restrict__ planes32,
+                                                                       const double* __restrict__ p_old,
+                                                                       double* __restrict__ p_new) {
+    if (b.st->done) return;
+    __shared__ double sh[kThreads / kWave];
+    const bool use32 = planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
+    const double beta = b.st->beta;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = (v.n_slices + gridDim.x - 1) / gridDim.x;
+    const int start = lb * per_block;
+    const int stop = min(start + per_block, v.n_slices);
+    double pq = 0.0;
+    for (int s = start + static_cast<int>(threadIdx.x) / kWave; s < stop; s += kThreads / kWave)
+        pq += use32 ? slice_rows_dir<float, kW, Prec>(b, v, planes32, s, p_old, p_new, beta)
+                    : slice_rows_dir<double, kW, Prec>(b, v, planes64, s, p_old, p_new, beta);
+    const double vals[1] = {block_sum<kThreads>(pq, sh)};
+    double tot[1];
+    if (!finish_reduction<kThreads, 1>(b.tickets + 1, b.partials, vals, tot, sh)) return;
+    if (threadIdx.x == 0) {
+        const double tpq = tot[0];
+        PcgState* st = b.st;
+        st->pq = tpq;
+        if (!(tpq > 0.0) || !isfinite(tpq)) {  // solver.cpp:591-593
+            st->status = LFG_ERR_NOT_SPD;
+            st->done = 1;
+        } else {
+            st->alpha = st->rho / tpq;
+        }
+    }
+}
+
 // Finishes an iteration after r.z is known (solver.cpp:614-621 checks).
 __device__ __forceinline__ void accept_rz(PcgState* st, double rz) {
     if (!(rz > 0.0) || !isfinite(rz)) {
@@ -1015,6 +1119,27 @@ void launch_pcg_spmv_slices(hipStream_t s, const PcgBuffers& b, const SliceView&
         pcg_spmv_slices_kernel<16><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32);
     else
         pcg_spmv_slices_kernel<0><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32);
+}
+
+void launch_pcg_spmv_slices_dir(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
+                                const float* planes32, const double* p_old, double* p_new) {
+    const unsigned int grid = static_cast<unsigned int>(slice_spmv_blocks(v.n_slices));
+    const bool narrow = v.max_width <= 9;
+    if (b.precond == LFG_PRECOND_BLOCK2) {
+        if (narrow)
+            pcg_spmv_slices_dir_kernel<9, LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32, p_old,
+                                                                                      p_new);
+        else
+            pcg_spmv_slices_dir_kernel<0, LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32, p_old,
+                                                                                      p_new);
+    } else {
+        if (narrow)
+            pcg_spmv_slices_dir_kernel<9, LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32, p_old,
+                                                                                      p_new);
+        else
+            pcg_spmv_slices_dir_kernel<0, LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32, p_old,
+                                                                                      p_new);
+    }
 }
 
 void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed) {

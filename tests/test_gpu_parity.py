@@ -193,9 +193,27 @@ def test_deterministic_and_graph_equals_direct(dev):
     u3, r3 = dev.pcg("jacobi", direct_launch=True, check_every=7)
     assert np.array_equal(bits(u1), bits(u2)) and np.array_equal(bits(u1), bits(u3))
     assert r1["iterations"] == r2["iterations"] == r3["iterations"]
-    for variant in (1, 2, 3, 4, 5, 6):
+    for variant in (1, 2, 3, 4, 5, 6, 7):
         u4, r4 = dev.pcg("jacobi", spmv_variant=variant)
         assert r4["converged"] and np.linalg.norm(u4 - u1) <= 1e-12 * np.linalg.norm(u1)
+
+
+def test_fused_direction_is_bit_identical(dev):
+    """Variant 7 forms p = M^-1 r + beta p inside the node-slice multiply (two launches
+    per iteration); variant 6 runs the same multiply and a separate direction kernel.
+    Same values in the same order: identical solutions and iteration counts, with and
+    without graphs, for both fused preconditioners and both precisions."""
+    print("case fused direction 200x200", flush=True)
+    pipeline.setup(meshgen.c2_model(), device=dev, assembly="coloring", reordering="rcm")
+    for precond in ("jacobi", "block"):
+        u6, r6 = dev.pcg(precond, spmv_variant=6)
+        u7, r7 = dev.pcg(precond, spmv_variant=7)
+        assert np.array_equal(bits(u6), bits(u7)) and r6["iterations"] == r7["iterations"], precond
+    u7, r7 = dev.pcg("jacobi", spmv_variant=7, direct_launch=True, check_every=5)
+    assert np.array_equal(bits(u7), bits(dev.pcg("jacobi", spmv_variant=6)[0]))
+    m6, q6 = dev.pcg("jacobi", "mixed", "relative", 1e-8, spmv_variant=6)
+    m7, q7 = dev.pcg("jacobi", "mixed", "relative", 1e-8, spmv_variant=7)
+    assert np.array_equal(bits(m6), bits(m7)) and q6["iterations"] == q7["iterations"]
 
 
 def test_error_paths(dev):
