@@ -37,16 +37,33 @@ def _sync():
 
 KERNELS = {1: "pcg_spmv_kernel<4>", 2: "pcg_spmv_kernel<8>", 3: "pcg_spmv_kernel<16>",
            4: "pcg_spmv_stream_kernel<24, 64>", 5: "pcg_spmv_stream_kernel<24, 32>",
-           6: "pcg_spmv_slices_kernel<9>", 7: "pcg_spmv_slices_dir_kernel<9, 0>"}
+           6: "pcg_spmv_slices_kernel<9>", 7: "pcg_spmv_slices_dir_kernel<9, 0>",
+           8: "pcg_spmv_stencil_kernel<4, true>", 9: "pcg_spmv_stencil_kernel<9, false>",
+           10: "pcg_fused_stencil_kernel<4, 0>"}
 
 
-def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int) -> int:
+def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int, layout: dict) -> int:
     """Algorithmic bytes of one SpMV launch: the values, the column indices of the
     layout, its row (or slice) offsets, x read once and y written once. CSR variants
     carry one int32 column per value; the node-slice layout one per 2x2 block of four
     values and one offset per 64 nodes (padding not counted). Variant 7 forms the
     search direction p = M^-1 r + beta p_old on the fly: it reads r, the Jacobi
-    inverse and p_old once and writes q and the new p."""
+    inverse and p_old once and writes q and the new p. The node-stencil layouts store
+    no columns: one 2x2 block per node and plane (the symmetric form: the diagonal
+    plane and one plane per positive offset, its FP32 copy with an FP64 diagonal)."""
+    n_nodes = n // 2
+    if variant == 10:
+        # the whole Jacobi iteration: the symmetric planes, and per node x, r, q, p and the
+        # Jacobi inverse read (5 x 16 B) and x, r, p, q written (4 x 16 B)
+        planes = layout["stencil_sym_planes"]
+        matrix = n_nodes * (planes * 32 if value_bytes == 8 else (planes - 1) * 16 + 32)
+        return matrix + n_nodes * 144
+    if variant == 8:
+        planes = layout["stencil_sym_planes"]
+        matrix = n_nodes * (planes * 32 if value_bytes == 8 else (planes - 1) * 16 + 32)
+        return matrix + 16 * n
+    if variant == 9:
+        return n_nodes * layout["stencil_exact_planes"] * 32 + 16 * n
     if variant in (6, 7):
         n_slices = (n // 2 + 63) // 64
         matrix = nnz * value_bytes + 4 * (nnz // 4) + 4 * (n_slices + 1)
@@ -55,16 +72,20 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int) -> int:
 
 
 def measured_traffic(kernel: str, n_mesh: int):
-    """HBM bytes per launch of `kernel` at this mesh size from the committed FETCH_SIZE /
+    """HBM bytes per launch of `kernel` at this mesh size from a committed FETCH_SIZE /
     WRITE_SIZE counter summary (rocprofv3 --pmc, calibrated on a 1 GiB copy), or None."""
-    path = os.path.join(ROOT, "profiles", f"pcg_spmv_slices_{n_mesh}x{n_mesh}_traffic.md")
-    if not os.path.exists(path):
+    prof = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(prof):
         return None
-    with open(path) as handle:
-        for line in handle:
-            parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel.replace(" ", ""):
-                return int(parts[2])
+    key = kernel.replace(" ", "")
+    for name in sorted(os.listdir(prof)):
+        if not name.endswith(f"_{n_mesh}x{n_mesh}_traffic.md"):
+            continue
+        with open(os.path.join(prof, name)) as handle:
+            for line in handle:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == key:
+                    return int(parts[2])
     return None
 
 
@@ -175,12 +196,15 @@ def main() -> int:
         assembly[strategy] = {"element_stiffness_ms": rep["element_stiffness_ms"], "assembly_ms": rep["assembly_ms"]}
     dev.set_load(sys_.indexed.load)
     dev.dirichlet(sys_.indexed.cdofs, sys_.indexed.cvals)
+    layout = dev.layout_info()
+    # the library's default Jacobi iteration: fused stencil, stencil multiply, node slices
+    default_variant = 10 if layout["fused_window_pairs"] else (8 if layout["stencil_sym_planes"] else 6)
     if args.profile:
         # Profiling run: the FP64 SpMV of the headline variant alone (args.profile launches),
         # then FP64 Jacobi-PCG iterations (a whole number of captured graphs), no other kernels.
-        variant = args.variant or 6  # the multiply the default PCG iteration runs
+        variant = args.variant or default_variant  # the multiply the default PCG iteration runs
         ms = dev.bench_spmv("fp64", variant, args.profile)
-        b = spmv_bytes(n, nnz, 8, variant)
+        b = spmv_bytes(n, nnz, 8, variant, layout)
         prof = {"kernel": KERNELS[variant], "launches": args.profile, "avg_launch_ms": ms, "bytes": b,
                 "GBs": b / (ms * 1e-3) / 1e9}
         print(f"bench: profile spmv {json.dumps(prof)}", file=sys.stderr, flush=True)
@@ -197,19 +221,28 @@ def main() -> int:
         dev.close()
         return 0
     # SpMV kernels: average launch duration over 50 launches (hipEvents on the library
-    # stream). The solves use the library's default, variant 6 (node slices), unless
-    # --variant says otherwise.
-    solve_variant = args.variant or 6
-    spmv, sweep = {}, {}
+    # stream). The solves use the library's default, variant 8 (node stencil, symmetric
+    # form) where the pattern admits it, unless --variant says otherwise.
+    solve_variant = args.variant or default_variant
+    spmv, sweep, iteration_kernel = {}, {}, {}
     for precision, vbytes in (("fp64", 8), ("mixed", 4)):
         sweep[precision] = {}
-        for variant in (1, 2, 3, 4, 5, 6, 7):
+        for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10):
+            if (variant == 8 and not layout["stencil_sym_planes"]) or (variant == 9 and (
+                    precision == "mixed" or not layout["stencil_exact_planes"])) or (
+                    variant == 10 and not layout["fused_window_pairs"]):
+                continue
             ms = dev.bench_spmv(precision, variant, 50)
-            b = spmv_bytes(n, nnz, vbytes, variant)
+            b = spmv_bytes(n, nnz, vbytes, variant, layout)
             sweep[precision][variant] = {"kernel": KERNELS[variant], "ms": ms, "bytes": b,
                                          "GBs": b / (ms * 1e-3) / 1e9}
-        spmv[precision] = dict(sweep[precision][solve_variant], variant=solve_variant)
+        # the multiply alone (the stand-alone kernel of the solve's layout) and the kernel the
+        # PCG iteration spends its time in (the same, or the fused iteration)
+        multiply = 8 if solve_variant == 10 else solve_variant
+        spmv[precision] = dict(sweep[precision][multiply], variant=multiply)
+        iteration_kernel[precision] = dict(sweep[precision][solve_variant], variant=solve_variant)
     print(f"bench: spmv {json.dumps(spmv)}", file=sys.stderr, flush=True)
+    print(f"bench: iteration kernel {json.dumps(iteration_kernel)}", file=sys.stderr, flush=True)
     copy_bytes = 1 << 30
     copy_ms = dev.bench_copy(copy_bytes, 20)
     copy = {"bytes_per_copy": copy_bytes, "ms": copy_ms, "GBs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9}
@@ -255,6 +288,18 @@ def main() -> int:
     print(f"bench: fp32-throughout {forms['fp32_values_throughout']}", file=sys.stderr, flush=True)
     u64, umx = forms["fp64"].pop("u"), forms["mixed"].pop("u")
     forms["mixed"]["rel_u_diff_vs_fp64"] = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
+    forms["fp32_values_throughout"]["rel_u_diff_vs_fp64"] = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+    if solve_variant != 6 and layout["node_blocked"]:
+        # FP64 with the node-slice multiply, whose rows sum exactly as the reference's CSR loop
+        t = time.perf_counter()
+        u6, rep6 = dev.pcg("jacobi", "fp64", "relative", args.tol, 10 ** 6, args.check_every, spmv_variant=6,
+                           raise_on_error=False)
+        forms["fp64_node_slices"] = {
+            "ms_per_solve": (time.perf_counter() - t) * 1e3, "device_solve_ms": rep6["solve_ms"],
+            "iterations": rep6["iterations"], "converged": bool(rep6["converged"]), "spmv_variant": 6,
+            "true_relative_residual": rep6["true_residual"] / rep6["initial_residual"],
+            "rel_u_diff_vs_fp64": float(np.linalg.norm(u6 - u64) / np.linalg.norm(u64))}
+        print(f"bench: fp64 node slices {forms['fp64_node_slices']}", file=sys.stderr, flush=True)
     per_iter_ms = {k: v["device_solve_ms"] / max(1, v["iterations"]) for k, v in forms.items()}
     cpu = None
     if not args.skip_cpu:
@@ -266,10 +311,11 @@ def main() -> int:
     batch = None
     if not args.skip_batch:
         batch = run_batch(dev, args.batch_problems, args.tol, args.batch_forms.split(","))
-    # Headline: the FP64 form, the one that reaches 1e-10 here. FP32 values stagnate above
-    # it on this system (reported in "forms"), so the mixed time is not a time-to-1e-10.
+    # Headline: the FP64 form. The mixed form (FP32 values with a compensated FP64
+    # diagonal once ||r|| <= 1e-2 ||r0||, FP64 vectors and sums) stops on the same
+    # recurrence rule and is reported beside it with its displacement difference.
     value = forms["fp64"]["ms_per_solve"]
-    dominant = spmv["fp64"]
+    dominant = iteration_kernel["fp64"]
     mixed_ok = forms["mixed"]["converged"]
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
@@ -285,14 +331,18 @@ def main() -> int:
         "mixed_speedup_vs_fp64": round(forms["fp64"]["ms_per_solve"] / forms["mixed"]["ms_per_solve"], 4)
         if mixed_ok else None,
         "forms": forms, "per_iteration_ms": per_iter_ms,
-        "spmv": spmv, "spmv_variants": {p: {str(k): round(v["GBs"], 1) for k, v in d.items()} for p, d in sweep.items()},
+        "spmv": spmv, "iteration_kernel": iteration_kernel,
+        "spmv_variants": {p: {str(k): round(v["GBs"], 1) for k, v in d.items()} for p, d in sweep.items()},
         "device_copy": copy, "assembly": assembly, "setup_s": round(setup_s, 2),
         "roofline": {"bound": "hbm", "kernel": f"{dominant['kernel']} (FP64 values)",
                      "achieved": round(dominant["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dominant["GBs"] / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(dominant["kernel"], n_mesh),
                      "bytes_per_launch": dominant["bytes"], "avg_launch_ms": dominant["ms"],
+                     "spmv_kernel": spmv["fp64"]["kernel"], "spmv_GBs": round(spmv["fp64"]["GBs"], 1),
+                     "spmv_frac": round(spmv["fp64"]["GBs"] / HBM_PEAK_GBS, 4),
                      "fp32_values_spmv_GBs": round(spmv["mixed"]["GBs"], 1),
+                     "fp32_values_iteration_GBs": round(iteration_kernel["mixed"]["GBs"], 1),
                      "copy_ceiling_GBs": round(copy["GBs"], 1)},
         "cpu_baseline": cpu,
         "batch": batch,

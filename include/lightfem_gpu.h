@@ -59,17 +59,29 @@ typedef struct {
     int32_t check_every;     /* iterations per captured graph between convergence reads; 0 = default */
     double tolerance;        /* SolverSettings::tolerance */
     int64_t max_iterations;  /* SolverSettings::max_iterations */
-    double replace_drop;     /* mixed: replace the residual in FP64 when it falls by this factor; 0 = default 0.1 */
+    double replace_drop;     /* mixed: replace the recurrence residual by the FP64 residual b - K x each time
+                                it falls by this factor (kept while replacements halve the FP64 residual);
+                                0 = default: no replacements (variant 10 has none) */
     double mixed_switch;     /* mixed: SpMVs use FP64 values until ||r|| <= mixed_switch*||r0||, then FP32
-                                values; 0 = default 1e-4, negative = FP32 values from the first iteration */
+                                values; 0 = default 1e-2, negative = FP32 values from the first iteration */
     int32_t direct_launch;   /* 1: launch each kernel directly instead of replaying a captured hipGraph */
-    int32_t spmv_variant;    /* 0 = default (6 when the pattern is node-blocked, else 4); 1-3: node
-                                groups of 4/8/16 lanes per row; 4, 5: row-segment streaming with
-                                64 / 32 rows per wavefront; 6: node-slice layout (one column per
-                                2x2 block, 64 nodes per wavefront); 7: node-slice layout with the
-                                Jacobi / block search direction formed inside the multiply (two
-                                launches per iteration, bit-identical to 6; slower on gfx950: the
-                                extra gathers cost more than the launch they save) */
+    int32_t spmv_variant;    /* 0 = default (10 for jacobi / block and 8 for ic0 when the pattern
+                                allows it, else 6 when it is node-blocked, else 4); 1-3: node
+                                groups of 4/8/16 lanes per row;
+                                4, 5: row-segment streaming with 64 / 32 rows per wavefront; 6:
+                                node-slice layout (one column per 2x2 block, 64 nodes per
+                                wavefront); 7: node-slice layout with the Jacobi / block search
+                                direction formed inside the multiply (two launches per iteration,
+                                bit-identical to 6; slower on gfx950: the extra gathers cost more
+                                than the launch they save); 8: node-stencil layout, symmetric form
+                                (structured meshes numbered row by row: no column indices, the
+                                diagonal and positive-offset blocks only, block (i, i-d) read as
+                                the transpose of (i-d, i); FP32 copy with a rounding-compensated
+                                FP64 diagonal); 9: node-stencil layout, every offset (bit-identical
+                                to 4 and 6, FP64 only); 10: the layout of 8 with the whole Jacobi /
+                                block iteration in one launch (update, search direction, multiply
+                                and all sums; beta from the next residual's r.z recurrence, so one
+                                reduction per iteration) */
 } lfg_pcg_options;
 
 typedef struct {
@@ -228,13 +240,29 @@ typedef struct {
 int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg_batch_system_report* reports,
                   lfg_batch_summary* summary);
 
+/* SpMV layouts the pattern set by lfg_set_pattern admits. The node-stencil layouts
+ * (variants 8, 9) need every 2x2 block (i, j) at one of a few node offsets j - i
+ * (a structured mesh numbered row by row: the CSR pattern of sparse.cpp:15-58 then
+ * has the same column offsets in every row). */
+typedef struct {
+    int32_t node_blocked;          /* rows 2i, 2i+1 share one column list of dof pairs (variants 6, 7) */
+    int32_t n_slices;              /* 64-node slices of the node-slice layout */
+    int32_t stencil_exact_planes;  /* variant 9: offsets of the pattern (0 = not available) */
+    int32_t stencil_sym_planes;    /* variant 8: 0 and the positive offsets (0 = not available) */
+    int32_t fused_window_pairs;    /* variant 10: LDS pairs per 256-node tile (0 = not available) */
+    int32_t stencil_offsets[17];   /* the exact form's offsets, ascending */
+} lfg_layout_info;
+int lfg_layout_info_get(lfg_ctx* ctx, lfg_layout_info* info);
+
 /* Checks and measurement hooks on the constrained system. */
 int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, double* out);
-/* y = Kc x with the PCG SpMV kernel of `variant` (values of `precision`). */
+/* y = Kc x with the PCG SpMV kernel of `variant` (values of `precision`; 10 multiplies
+ * as 8). */
 int lfg_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x, double* y);
 /* Average device time (ms) of `reps` SpMV launches of the given precision and
  * variant on resident data (variant 7: the multiply with the Jacobi direction formed
- * on the fly, as its PCG iteration runs it). */
+ * on the fly, as its PCG iteration runs it; variant 10: one whole fused Jacobi PCG
+ * iteration). */
 int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t reps, double* avg_ms);
 /* Average device time (ms) of a device-to-device copy of `bytes` bytes: the
  * streaming-bandwidth ceiling of this device, and the calibration kernel for

@@ -94,6 +94,14 @@ struct lfg_ctx {
     DevArray<int> slice_ptr, slice_col;
     DevArray<double> slices64;
     DevArray<float> slices32;
+    // node-stencil copies of the constrained values (SpMV variants 8 and 9), when every
+    // block column lies at one of a few fixed node offsets (structured meshes numbered
+    // row by row)
+    bool have_stencil_sym = false, have_stencil_exact = false;
+    bool stencil64_valid = false, stencil32_valid = false, stencil_exact_valid = false;
+    lfg::StencilView stencil_sym{}, stencil_exact{};
+    DevArray<double> stencil64, stencil_exact64, stencil_diag;
+    DevArray<float> stencil32;
     std::vector<int> h_color_ptr;
     DevArray<int> color_elems;
 
@@ -106,6 +114,7 @@ struct lfg_ctx {
 
     // solver
     DevArray<double> x, r, p, p2, q, z, minv, factor, diag, partials, scratch;
+    DevArray<double> r_alt, q_alt;  // fused stencil iteration: second r and q buffers (p uses p2)
     DevArray<double> lam, adj_rhs, dE, grad;  // adjoint solution and right-hand side, sensitivities
     bool have_adjoint = false;
     DevArray<lfg::PcgState> state;
@@ -169,15 +178,36 @@ lfg::SliceView slice_view(const lfg_ctx* c) {
 
 constexpr int kSliceVariant = 6;  // node-slice SpMV, separate direction kernel
 constexpr int kSliceFused = 7;    // node-slice SpMV forming the Jacobi / block direction on the fly
+constexpr int kStencilSym = 8;    // node-stencil planes, symmetric form (diagonal + positive offsets)
+constexpr int kStencilExact = 9;  // node-stencil planes, every offset (CSR column order)
+constexpr int kStencilFused = 10; // symmetric node stencil, whole Jacobi / block PCG iteration in one launch
 
 bool slice_layout(int variant) { return variant == kSliceVariant || variant == kSliceFused; }
+bool stencil_layout(int variant) {
+    return variant == kStencilSym || variant == kStencilExact || variant == kStencilFused;
+}
+bool fused_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stencil_sym.win_total > 0; }
 
-// 0 selects the node-slice kernel where the pattern allows it, else row-segment streaming.
-int resolve_variant(const lfg_ctx* c, int variant) {
-    if (variant == 0) return c->have_slices ? kSliceVariant : 4;
-    require(variant >= 1 && variant <= kSliceFused, LFG_ERR_ARG, "SpMV variant must be 0-7");
+// 0 selects, in order: the fused stencil iteration (Jacobi and block preconditioners),
+// the symmetric node-stencil multiply, the node-slice multiply, row-segment streaming.
+int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JACOBI) {
+    if (variant == 0) {
+        if (fused_stencil_ok(c) && precond != LFG_PRECOND_IC0) return kStencilFused;
+        return c->have_stencil_sym ? kStencilSym : (c->have_slices ? kSliceVariant : 4);
+    }
+    require(variant >= 1 && variant <= kStencilFused, LFG_ERR_ARG, "SpMV variant must be 0-10");
+    require(variant != kStencilFused || fused_stencil_ok(c), LFG_ERR_UNSUPPORTED,
+            "the fused stencil iteration needs the symmetric node-stencil layout with offset windows that fit "
+            "in LDS");
+    require(variant != kStencilFused || precond != LFG_PRECOND_IC0, LFG_ERR_UNSUPPORTED,
+            "the fused stencil iteration applies the jacobi and 2x2 block preconditioners, not ic0");
     require(!slice_layout(variant) || c->have_slices, LFG_ERR_UNSUPPORTED,
             "the node-slice SpMV needs rows 2i, 2i+1 sharing one column list of dof pairs");
+    require(variant != kStencilSym || c->have_stencil_sym, LFG_ERR_UNSUPPORTED,
+            "the symmetric node-stencil SpMV needs a symmetric node-blocked pattern with at most 8 positive node "
+            "offsets");
+    require(variant != kStencilExact || c->have_stencil_exact, LFG_ERR_UNSUPPORTED,
+            "the node-stencil SpMV needs a node-blocked pattern with at most 17 node offsets");
     return variant;
 }
 
@@ -224,6 +254,146 @@ void build_slices(lfg_ctx* c) {
     c->slice_col.alloc(static_cast<size_t>(total));
     lfg::launch_pack_slices(c->stream, csr_view(c), slice_view(c), nullptr, nullptr, nullptr, c->slice_col.p);
     c->have_slices = true;
+}
+
+// LDS windows of the fused stencil iteration: the offsets -off[k], 0, off[k] in
+// clusters (a gap above 64 starts a new one), one window of 256 + (hi - lo) nodes per
+// cluster. Leaves win_total = 0 when the windows exceed the LDS budget.
+void plan_fused_windows(lfg::StencilView& v) {
+    std::vector<int> all;
+    for (int k = v.n_off - 1; k >= 1; --k) all.push_back(-v.off[k]);
+    for (int k = 0; k < v.n_off; ++k) all.push_back(v.off[k]);
+    v.n_clusters = 0;
+    v.win_total = 0;
+    int base = 0;
+    for (size_t a = 0; a < all.size();) {
+        size_t e = a;
+        while (e + 1 < all.size() && all[e + 1] - all[e] <= 64) ++e;
+        const int c = v.n_clusters++;
+        v.win_lo[c] = all[a];
+        v.win_hi[c] = all[e];
+        v.win_base[c] = base;
+        base += lfg::kFusedTile + all[e] - all[a];
+        a = e + 1;
+    }
+    auto slot = [&](int d) {
+        for (int c = 0; c < v.n_clusters; ++c)
+            if (d >= v.win_lo[c] && d <= v.win_hi[c]) return v.win_base[c] + d - v.win_lo[c];
+        return -1;
+    };
+    v.slot_zero = slot(0);
+    for (int k = 0; k < v.n_off; ++k) {
+        v.slot_up[k] = slot(v.off[k]);
+        v.slot_lo[k] = slot(-v.off[k]);
+    }
+    if (base <= lfg::kFusedMaxWindow) v.win_total = base;
+}
+
+// After build_slices (node-blocked patterns): the node offsets j - i of all blocks
+// (i, j). Up to kMaxStencil offsets give the exact stencil form; a symmetric pattern
+// with at most 8 positive offsets also gives the symmetric form.
+void build_stencil(lfg_ctx* c) {
+    c->have_stencil_sym = c->have_stencil_exact = false;
+    c->stencil64_valid = c->stencil32_valid = c->stencil_exact_valid = false;
+    if (!c->have_slices) return;
+    const int n_nodes = c->n / 2;
+    const std::vector<int>& rp = c->h_row_ptr;
+    const std::vector<int>& col = c->h_col;
+    std::vector<int> offs;
+    auto add = [&](int d) {
+        if (std::find(offs.begin(), offs.end(), d) != offs.end()) return true;
+        if (static_cast<int>(offs.size()) >= lfg::kMaxStencil) return false;
+        offs.push_back(d);
+        return true;
+    };
+    if (!add(0)) return;
+    for (int i = 0; i < n_nodes; ++i) {
+        const int r0 = rp[2 * i], r1 = rp[2 * i + 1];
+        if (r1 - r0 == 1) continue;  // node in no element: diagonal only
+        for (int k = r0; k < r1; k += 2)
+            if (!add((col[k] >> 1) - i)) return;
+    }
+    std::sort(offs.begin(), offs.end());
+    lfg::StencilView ex{};
+    ex.n_nodes = n_nodes;
+    ex.n_off = static_cast<int>(offs.size());
+    ex.sym = 0;
+    for (int k = 0; k < ex.n_off; ++k) ex.off[k] = offs[k];
+    c->stencil_exact = ex;
+    c->have_stencil_exact = true;
+    // symmetric form: the pattern must hold block (j, i) for every block (i, j)
+    std::vector<int> pos;
+    for (int d : offs)
+        if (d > 0) pos.push_back(d);
+    if (pos.size() > 8) return;
+    for (int d : pos)
+        if (std::find(offs.begin(), offs.end(), -d) == offs.end()) return;
+    auto has_block = [&](int i, int j) {
+        const int r0 = rp[2 * i], r1 = rp[2 * i + 1];
+        if (r1 - r0 == 1) return j == i;
+        int lo = 0, hi = (r1 - r0) / 2;
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if ((col[r0 + 2 * mid] >> 1) < j) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo < (r1 - r0) / 2 && (col[r0 + 2 * lo] >> 1) == j;
+    };
+    for (int i = 0; i < n_nodes; ++i) {
+        const int r0 = rp[2 * i], r1 = rp[2 * i + 1];
+        if (r1 - r0 == 1) continue;
+        for (int k = r0; k < r1; k += 2) {
+            const int j = col[k] >> 1;
+            if (j > i && !has_block(j, i)) return;
+        }
+    }
+    lfg::StencilView sy{};
+    sy.n_nodes = n_nodes;
+    sy.n_off = 1 + static_cast<int>(pos.size());
+    sy.sym = 1;
+    sy.off[0] = 0;
+    for (size_t k = 0; k < pos.size(); ++k) sy.off[k + 1] = pos[k];
+    plan_fused_windows(sy);
+    c->stencil_sym = sy;
+    c->have_stencil_sym = true;
+}
+
+// Packs the constrained values into the stencil planes that the next SpMVs read.
+void ensure_stencil(lfg_ctx* c, int variant, bool need64, bool need32) {
+    if (variant == kStencilFused) variant = kStencilSym;
+    const size_t nn = 64 * static_cast<size_t>((c->n / 2 + 63) / 64);  // whole 64-node chunks
+    if (variant == kStencilExact) {
+        if (!c->stencil_exact_valid) {
+            c->stencil_exact64.alloc(4 * nn * c->stencil_exact.n_off);
+            lfg::launch_pack_stencil(c->stream, csr_view(c), c->stencil_exact, c->Kc.p, c->stencil_exact64.p, nullptr,
+                                     nullptr);
+            c->stencil_exact_valid = true;
+        }
+        return;
+    }
+    if (need64 && !c->stencil64_valid) {
+        c->stencil64.alloc(4 * nn * c->stencil_sym.n_off);
+        lfg::launch_pack_stencil(c->stream, csr_view(c), c->stencil_sym, c->Kc.p, c->stencil64.p, nullptr, nullptr);
+        c->stencil64_valid = true;
+    }
+    if (need32 && !c->stencil32_valid) {
+        c->stencil32.alloc(4 * nn * std::max(1, c->stencil_sym.n_off - 1));
+        c->stencil_diag.alloc(4 * nn);
+        lfg::launch_pack_stencil(c->stream, csr_view(c), c->stencil_sym, c->Kc.p, nullptr, c->stencil32.p,
+                                 c->stencil_diag.p);
+        c->stencil32_valid = true;
+    }
+}
+
+// The multiply of one PCG iteration for a stencil variant (q = A p, p.q, alpha). With
+// mixed, the symmetric form switches to its FP32 copy in the FP32 phase.
+void launch_stencil_spmv(lfg_ctx* c, const lfg::PcgBuffers& b, int variant, bool fp64, bool fp32) {
+    if (variant == kStencilFused) variant = kStencilSym;  // the plain multiply of the same planes
+    if (variant == kStencilExact)
+        lfg::launch_pcg_spmv_stencil(c->stream, b, c->stencil_exact, c->stencil_exact64.p, nullptr, nullptr);
+    else
+        lfg::launch_pcg_spmv_stencil(c->stream, b, c->stencil_sym, fp64 ? c->stencil64.p : nullptr,
+                                     fp32 ? c->stencil32.p : nullptr, fp32 ? c->stencil_diag.p : nullptr);
 }
 
 // Packs the constrained values into the slice planes that the next SpMVs read.
@@ -409,7 +579,7 @@ void ensure_vectors(lfg_ctx* c) {
     c->p.alloc(n);
     c->q.alloc(n);
     c->z.alloc(n);
-    c->partials.alloc(16384);
+    c->partials.alloc(65536);
     c->scratch.alloc(4096 + 8);
     if (!c->state.p) {
         c->state.alloc(1);
@@ -475,8 +645,17 @@ bool fused_direction(int variant, int precond) { return variant == kSliceFused &
 
 // One PCG iteration. With the fused direction, iteration `parity` reads p from
 // buffer parity (b.p or c->p2) and writes the new p into the other one.
-void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int variant, int parity) {
+void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int replace, int variant, int parity) {
     const lfg::CsrView a = csr_view(c);
+    if (variant == kStencilFused) {
+        double* const rb[2] = {c->r.p, c->r_alt.p};
+        double* const pb[2] = {c->p.p, c->p2.p};
+        double* const qb[2] = {c->q.p, c->q_alt.p};
+        lfg::launch_pcg_fused_stencil(c->stream, b, c->stencil_sym, c->stencil64.p, mixed ? c->stencil32.p : nullptr,
+                                      mixed ? c->stencil_diag.p : nullptr, rb[parity], rb[parity ^ 1], pb[parity],
+                                      pb[parity ^ 1], qb[parity], qb[parity ^ 1]);
+        return;
+    }
     if (fused_direction(variant, b.precond)) {
         double* const pb[2] = {b.p, c->p2.p};
         lfg::PcgBuffers bi = b;
@@ -484,15 +663,17 @@ void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int vari
         lfg::launch_pcg_spmv_slices_dir(c->stream, b, slice_view(c), c->slices64.p, mixed ? c->slices32.p : nullptr,
                                         pb[parity], bi.p);
         lfg::launch_pcg_update(c->stream, bi, mixed);
-        if (mixed) lfg::launch_pcg_replace(c->stream, bi, a, c->Kc.p);
+        if (replace) lfg::launch_pcg_replace(c->stream, bi, a, c->Kc.p);
         return;
     }
-    if (slice_layout(variant))
+    if (stencil_layout(variant))
+        launch_stencil_spmv(c, b, variant, true, mixed != 0);
+    else if (slice_layout(variant))
         lfg::launch_pcg_spmv_slices(c->stream, b, slice_view(c), c->slices64.p, mixed ? c->slices32.p : nullptr);
     else
         lfg::launch_pcg_spmv(c->stream, b, a, c->Kc.p, mixed ? c->K32.p : nullptr, variant);
     lfg::launch_pcg_update(c->stream, b, mixed);
-    if (mixed) lfg::launch_pcg_replace(c->stream, b, a, c->Kc.p);
+    if (replace) lfg::launch_pcg_replace(c->stream, b, a, c->Kc.p);
     if (b.precond == LFG_PRECOND_IC0) lfg::launch_pcg_ic0_step(c->stream, b, ic0_view(c));
     lfg::launch_pcg_direction(c->stream, b);
 }
@@ -646,6 +827,7 @@ int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* ro
         ctx->emap.upload(emap, 64 * static_cast<size_t>(ctx->n_elems), ctx->stream);
         build_gather_map(ctx, emap);
         build_slices(ctx);
+        build_stencil(ctx);
         ctx->K.alloc(static_cast<size_t>(nnz));
         ctx->have_pattern = true;
         ctx->have_K = ctx->have_system = ctx->have_solution = false;
@@ -802,6 +984,7 @@ int lfg_dirichlet(lfg_ctx* ctx, int64_t n_constraints, const int32_t* dofs, cons
         ctx->have_solution = false;
         ctx->k32_valid = false;
         ctx->slices64_valid = ctx->slices32_valid = false;
+        ctx->stencil64_valid = ctx->stencil32_valid = ctx->stencil_exact_valid = false;
         ctx->precond_ready = -1;
         return LFG_OK;
     });
@@ -845,8 +1028,12 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     const int previous = ctx->precond_ready;
     setup_preconditioner(ctx, kind, &degenerate);
     if (previous == kind && kind == LFG_PRECOND_BLOCK2) degenerate = -1;  // factor reused; count unknown here
-    const int variant = resolve_variant(ctx, opt->spmv_variant);
-    if (slice_layout(variant)) {
+    const int variant = resolve_variant(ctx, opt->spmv_variant, kind);
+    require(!(mixed && variant == kStencilExact), LFG_ERR_UNSUPPORTED,
+            "mixed precision uses the symmetric stencil form (variant 8), not the exact one");
+    if (stencil_layout(variant)) {
+        ensure_stencil(ctx, variant, true, mixed != 0);
+    } else if (slice_layout(variant)) {
         ensure_slices(ctx, true, mixed != 0);
     } else if (mixed && !ctx->k32_valid) {
         ctx->K32.alloc(static_cast<size_t>(ctx->nnz));
@@ -863,20 +1050,28 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     }
 
     const lfg::PcgBuffers b = pcg_buffers(ctx, kind, rhs, xs);
-    const bool fused = fused_direction(variant, kind);
+    const bool stencil_fused = variant == kStencilFused;
+    const bool fused = fused_direction(variant, kind) || stencil_fused;
     if (fused) ctx->p2.alloc(static_cast<size_t>(ctx->n));
+    if (stencil_fused) {
+        ctx->r_alt.alloc(static_cast<size_t>(ctx->n));
+        ctx->q_alt.alloc(static_cast<size_t>(ctx->n));
+    }
     // a captured graph starts every launch on buffer 0, so it holds an even count when fused
     int per_graph = opt->check_every > 0 ? opt->check_every : 32;
     if (fused && (per_graph & 1)) ++per_graph;
-    const double drop = opt->replace_drop > 0.0 ? opt->replace_drop : 0.1;
-    const double switch_tau = !mixed ? 0.0 : (opt->mixed_switch != 0.0 ? opt->mixed_switch : 1.0e-4);
+    // mixed: FP32 values once ||r|| <= 1e-2 ||r0|| by default; FP64 residual replacements
+    // only on request (the fused stencil iteration has none)
+    const double drop = opt->replace_drop > 0.0 && !stencil_fused ? opt->replace_drop : 0.0;
+    const double switch_tau = !mixed ? 0.0 : (opt->mixed_switch != 0.0 ? opt->mixed_switch : 1.0e-2);
+    const int replace = mixed && drop > 0.0;
     // capture `per_graph` iterations once
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const bool use_graph = opt->direct_launch == 0;
     if (use_graph) {
         LFG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant, i & 1);
+        for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, replace, variant, i & 1);
         LFG_HIP(hipStreamEndCapture(ctx->stream, &graph));
         LFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     }
@@ -890,19 +1085,22 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     } guard{graph, exec};
 
     LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-    lfg::launch_pcg_init(ctx->stream, b, opt->stop_rule, opt->tolerance, opt->max_iterations, drop, switch_tau);
+    lfg::launch_pcg_init(ctx->stream, b, opt->stop_rule, opt->tolerance, opt->max_iterations, drop, switch_tau,
+                         stencil_fused ? 1 : 0);
     if (kind == LFG_PRECOND_IC0) {
         lfg::launch_pcg_ic0_step(ctx->stream, b, ic0_view(ctx));
         lfg::launch_pcg_direction(ctx->stream, b);
     }
     // every iteration the cap allows fits in this many graph launches
-    const long long launch_cap = std::max<long long>(1, (opt->max_iterations + per_graph - 1) / per_graph);
+    // (the fused stencil iteration spends its first launch on p0 and A p0)
+    const long long launches_needed = opt->max_iterations + (stencil_fused ? 1 : 0);
+    const long long launch_cap = std::max<long long>(1, (launches_needed + per_graph - 1) / per_graph);
     long long launched = 0;
     auto launch_one = [&](int slot) {
         if (use_graph) {
             LFG_HIP(hipGraphLaunch(exec, ctx->stream));
         } else {
-            for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, variant, i & 1);
+            for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, replace, variant, i & 1);
         }
         LFG_HIP(hipMemcpyAsync(ctx->h_state + slot, ctx->state.p, sizeof(lfg::PcgState), hipMemcpyDeviceToHost,
                                ctx->stream));
@@ -1069,6 +1267,8 @@ static void run_spmv(lfg_ctx* ctx, int precision, int variant, bool fused_dir = 
     if (fused_dir && variant == kSliceFused)
         lfg::launch_pcg_spmv_slices_dir(ctx->stream, b, slice_view(ctx), f32 ? nullptr : ctx->slices64.p,
                                         f32 ? ctx->slices32.p : nullptr, ctx->p.p, ctx->p2.p);
+    else if (stencil_layout(variant))
+        launch_stencil_spmv(ctx, b, variant, !f32, f32);
     else if (slice_layout(variant))
         lfg::launch_pcg_spmv_slices(ctx->stream, b, slice_view(ctx), f32 ? nullptr : ctx->slices64.p,
                                     f32 ? ctx->slices32.p : nullptr);
@@ -1081,13 +1281,32 @@ static void prepare_spmv(lfg_ctx* ctx, int precision, int variant) {
     ensure_vectors(ctx);
     lfg::PcgState zero{};
     LFG_HIP(hipMemcpyAsync(ctx->state.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
-    if (slice_layout(variant)) {
+    if (stencil_layout(variant)) {
+        require(!(variant == kStencilExact && precision == LFG_PRECISION_MIXED), LFG_ERR_UNSUPPORTED,
+                "the exact stencil form has no FP32 copy");
+        ensure_stencil(ctx, variant, precision != LFG_PRECISION_MIXED, precision == LFG_PRECISION_MIXED);
+    } else if (slice_layout(variant)) {
         ensure_slices(ctx, precision != LFG_PRECISION_MIXED, precision == LFG_PRECISION_MIXED);
     } else if (precision == LFG_PRECISION_MIXED && !ctx->k32_valid) {
         ctx->K32.alloc(static_cast<size_t>(ctx->nnz));
         lfg::launch_to_f32(ctx->stream, ctx->Kc.p, ctx->K32.p, ctx->nnz);
         ctx->k32_valid = true;
     }
+}
+
+int lfg_layout_info_get(lfg_ctx* ctx, lfg_layout_info* info) {
+    return guarded(ctx, [&] {
+        require(ctx->have_pattern && info, LFG_ERR_STATE, "lfg_set_pattern must precede lfg_layout_info_get");
+        lfg_layout_info out{};
+        out.node_blocked = ctx->have_slices ? 1 : 0;
+        out.n_slices = ctx->have_slices ? ctx->n_slices : 0;
+        out.stencil_exact_planes = ctx->have_stencil_exact ? ctx->stencil_exact.n_off : 0;
+        out.stencil_sym_planes = ctx->have_stencil_sym ? ctx->stencil_sym.n_off : 0;
+        out.fused_window_pairs = ctx->have_stencil_sym ? ctx->stencil_sym.win_total : 0;
+        for (int k = 0; k < out.stencil_exact_planes; ++k) out.stencil_offsets[k] = ctx->stencil_exact.off[k];
+        *info = out;
+        return LFG_OK;
+    });
 }
 
 int lfg_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x, double* y) {
@@ -1110,6 +1329,29 @@ int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t rep
         require(ctx->have_system && reps > 0 && avg_ms, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_bench_spmv");
         LFG_HIP(hipSetDevice(ctx->device));
         const int v = resolve_variant(ctx, variant);
+        if (v == kStencilFused) {
+            // whole Jacobi PCG iterations (the fused kernel does the multiply and the vector
+            // work), from x = 0 with a stop threshold no residual reaches
+            long long degenerate = 0;
+            ensure_vectors(ctx);
+            if (ctx->precond_ready != LFG_PRECOND_JACOBI) setup_preconditioner(ctx, LFG_PRECOND_JACOBI, &degenerate);
+            const bool f32 = precision == LFG_PRECISION_MIXED;
+            ensure_stencil(ctx, v, !f32, f32);
+            ctx->p2.alloc(static_cast<size_t>(ctx->n));
+            ctx->r_alt.alloc(static_cast<size_t>(ctx->n));
+            ctx->q_alt.alloc(static_cast<size_t>(ctx->n));
+            const lfg::PcgBuffers b = pcg_buffers(ctx, LFG_PRECOND_JACOBI, ctx->fc.p, ctx->x.p);
+            lfg::launch_pcg_init(ctx->stream, b, LFG_STOP_RELATIVE, 1e-300, 1LL << 60, 0.0, f32 ? -1.0 : 0.0, 1);
+            enqueue_iteration(ctx, b, f32, 0, v, 0);  // warm-up: p0, A p0
+            enqueue_iteration(ctx, b, f32, 0, v, 1);
+            LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+            for (int i = 0; i < reps; ++i) enqueue_iteration(ctx, b, f32, 0, v, i & 1);
+            LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+            sync(ctx);
+            *avg_ms = elapsed(ctx->ev_start, ctx->ev_stop) / reps;
+            ctx->have_solution = false;
+            return LFG_OK;
+        }
         const bool fused = v == kSliceFused;  // the multiply its Jacobi / block PCG iteration runs
         if (fused) {
             long long degenerate = 0;
@@ -1217,6 +1459,7 @@ int lfg_batch_set_loads(lfg_ctx* ctx, int32_t n_cases, const double* loads, int6
         ctx->have_solution = ctx->have_adjoint = false;
         ctx->k32_valid = false;
         ctx->slices64_valid = ctx->slices32_valid = false;
+        ctx->stencil64_valid = ctx->stencil32_valid = ctx->stencil_exact_valid = false;
         ctx->precond_ready = -1;
         ctx->have_batch_loads = true;
         return LFG_OK;

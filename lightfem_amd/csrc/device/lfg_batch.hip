@@ -18,20 +18,6 @@
 namespace lfg {
 namespace {
 
-typedef double dpair __attribute__((ext_vector_type(2)));
-typedef float fpair __attribute__((ext_vector_type(2)));
-
-template <typename VT>
-struct PairType;
-template <>
-struct PairType<double> {
-    using type = dpair;
-};
-template <>
-struct PairType<float> {
-    using type = fpair;
-};
-
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x) & (kWave - 1); }
 
 // Vector kernels map lane l of sub-step j to the (node, case) pair t = 64 j + l of the
@@ -181,7 +167,7 @@ __global__ __launch_bounds__(512) void batch_init_kernel(BatchView bv, int use_p
 template <int NC, typename VT, int kW>
 __device__ __forceinline__ void slice_multiply(const BatchView& bv, const SliceView& v, const VT* __restrict__ planes,
                                                int s, double* pq) {
-    using P = typename PairType<VT>::type;
+    using P = typename PairOf<VT>::type;
     const int node = s * kWave + lane_id();
     const int begin = v.ptr[s];
     const int width = (v.ptr[s + 1] - begin) / kWave;

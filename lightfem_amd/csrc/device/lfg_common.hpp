@@ -34,7 +34,7 @@ struct PcgState {
     int converged;
     int need_replace;  // mixed: the replacement kernel runs this iteration
     int phase32;       // mixed: SpMVs read the FP32 values
-    int pad;
+    int replaced;      // mixed: this iteration's residual is the FP64 one in z (accepted replacement)
 };
 
 // Completion counters of one reducing kernel: one per XCD and one on top, each on
@@ -144,5 +144,20 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
 }
 
 __device__ __forceinline__ double max0(double v) { return (0.0 < v) ? v : 0.0; }
+
+// Value pairs of one 2x2 block row (16-byte FP64 or 8-byte FP32 loads).
+typedef double dpair __attribute__((ext_vector_type(2)));
+typedef float fpair __attribute__((ext_vector_type(2)));
+
+template <typename VT>
+struct PairOf;
+template <>
+struct PairOf<double> {
+    using type = dpair;
+};
+template <>
+struct PairOf<float> {
+    using type = fpair;
+};
 
 }  // namespace lfg

@@ -45,6 +45,43 @@ struct SliceView {
     int max_width;
 };
 
+// Node-stencil layout of a node-blocked matrix whose blocks lie at a few fixed node
+// offsets, as on a structured mesh numbered row by row (node i couples with nodes
+// i + d, d in a short list). Plane k holds for every node i the 2x2 block
+// (i, i + off[k]) as an upper pair (K[2i][2j], K[2i][2j+1]) and a lower pair
+// (K[2i+1][2j], K[2i+1][2j+1]); a block the pattern lacks, or whose column falls
+// outside the mesh, is zero. Storage is per 64-node chunk (one wavefront): chunk c
+// holds, plane after plane, the 64 upper pairs then the 64 lower pairs of its
+// nodes, so a wavefront streams one contiguous run per chunk. No column index is
+// stored, and every load of a wavefront moves 64 consecutive pairs.
+//  - exact form: every offset of the pattern, ascending, so a row accumulates in the
+//    CSR column order (bit-identical to the CSR row loop);
+//  - symmetric form: off[0] = 0 and the positive offsets; block (i, i - d) is read as
+//    the transpose of block (i - d, i), which halves the values streamed. Its FP32
+//    copy keeps the diagonal plane in FP64 and gives it the rounding error of the
+//    node's off-diagonal FP32 blocks with the opposite sign, so every row keeps its
+//    FP64 sum and the FP32 operator acts exactly on nodal translations.
+constexpr int kMaxStencil = 17;
+struct StencilView {
+    int n_nodes;
+    int n_off;              // planes (exact: all offsets; symmetric: 0 and the positive ones)
+    int sym;
+    int off[kMaxStencil];   // ascending
+    // Symmetric form, fused PCG iteration: the offsets -off[k], 0, off[k] grouped into
+    // clusters of nearby offsets; a 256-node tile needs the search direction on one
+    // window per cluster, [i0 + lo, i0 + 255 + hi], kept in LDS at win_base.
+    int n_clusters;
+    int win_lo[kMaxStencil];
+    int win_hi[kMaxStencil];
+    int win_base[kMaxStencil];
+    int win_total;          // LDS pairs of all windows (0: no fused iteration)
+    int slot_zero;          // LDS offset (plus the node's tile position t) of node i = i0 + t
+    int slot_up[kMaxStencil];  // ... of i + off[k]
+    int slot_lo[kMaxStencil];  // ... of i - off[k]
+};
+constexpr int kFusedTile = 256;         // nodes per tile (one per thread)
+constexpr int kFusedMaxWindow = 2048;   // LDS pairs of all windows of a tile
+
 // ---- element kernels (lfg_element.hip)
 enum KeMode { KE_STORE = 0, KE_ATOMIC = 1, KE_COLORED = 2 };
 // Ke for elements [0, count) of `elems` (or all when elems == nullptr). STORE writes
@@ -92,8 +129,10 @@ struct PcgBuffers {
 };
 
 int spmv_blocks(int n, int variant);
+// fused = 1 prepares the fused stencil iteration: p and q zero, alpha = beta = 0 and
+// the iteration count at -1, so the first fused launch forms p0 = M^-1 r0 and A p0.
 void launch_pcg_init(hipStream_t s, const PcgBuffers& b, int stop_rule, double tol, long long max_iter, double drop,
-                     double mixed_switch);
+                     double mixed_switch, int fused = 0);
 // q = A p; finishes p.q and alpha. Without vals32 the FP64 values are used; with
 // both, the kernel reads the FP32 values once the state is in the FP32 phase.
 void launch_pcg_spmv(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* vals64, const float* vals32,
@@ -113,6 +152,28 @@ int slice_spmv_blocks(int n_slices);
 // 83 us against 64 + 10 us at 1000x1000, so not the default).
 void launch_pcg_spmv_slices_dir(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
                                 const float* planes32, const double* p_old, double* p_new);
+// Packs the constrained CSR `values` into the stencil planes (C = ceil(n_nodes / 64)
+// chunks): out64 (all planes, 256 * n_off * C doubles); out32 (symmetric form: the
+// off-diagonal planes, 256 * (n_off - 1) * C floats) with diag_c (256 * C doubles: the
+// diagonal blocks carrying the FP32 rounding compensation). Any output may be null.
+void launch_pack_stencil(hipStream_t s, const CsrView& a, const StencilView& v, const double* values, double* out64,
+                         float* out32, double* diag_c);
+// q = A p on the stencil layout; finishes p.q and alpha like launch_pcg_spmv. With
+// planes32 (and diag_c), the FP32 copy is read once the state is in the FP32 phase
+// (always, without planes64).
+void launch_pcg_spmv_stencil(hipStream_t s, const PcgBuffers& b, const StencilView& v, const double* planes64,
+                             const float* planes32, const double* diag_c);
+// One whole Jacobi / 2x2 block-Jacobi PCG iteration on the symmetric stencil form
+// (requires v.win_total > 0). With alpha, beta of the previous launch: x += alpha p,
+// r' = r - alpha q, z = M^-1 r', p' = z + beta p (own nodes written, the windows the
+// multiply needs formed in LDS), q' = A p'; sums ||r'||^2, r'.z, p'.q', q'.z, q'.M^-1 q'.
+// The last block applies the stop test to r', then alpha' = r'.z / p'.q' and
+// beta' = rho'' / r'.z with rho'' = r'.z - 2 alpha' q'.z + alpha'^2 q'.M^-1 q', the r.z
+// of the next residual, so no second reduction is needed. r, p, q alternate between
+// the *_in and *_out buffers from one launch to the next; x is updated in place.
+void launch_pcg_fused_stencil(hipStream_t s, const PcgBuffers& b, const StencilView& v, const double* planes64,
+                              const float* planes32, const double* diag_c, const double* r_in, double* r_out,
+                              const double* p_in, double* p_out, const double* q_in, double* q_out);
 // x += alpha p, r -= alpha q, ||r||^2, r.z; convergence / replacement decision, beta.
 void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed);
 // mixed only: r = b - K x in FP64 when a replacement is due; then r.z and beta.

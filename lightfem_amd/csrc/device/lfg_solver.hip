@@ -11,6 +11,7 @@
 // + r.z, direction) that a hipGraph replays without host round trips; each
 // kernel returns immediately once the device state says the solve is done.
 #include "lfg_kernels.hpp"
+#include "lfg_precond.hpp"
 
 #include "lightfem_gpu.h"
 
@@ -147,33 +148,6 @@ __global__ void block2_factor_kernel(int n_nodes, const int* __restrict__ rp, co
     t[2] = 1.0 / dense[3];
 }
 
-// precond.cpp:266-287 for a size-2 block: forward then backward substitution with
-// the packed factor (reciprocal pivots).
-__device__ __forceinline__ void block2_solve(const double* f, double r0, double r1, double& z0, double& z1) {
-    const double f0 = f[0], f1 = f[1], f2 = f[2];
-    double s0 = r0 * f0;
-    double s1 = r1;
-    s1 -= f1 * s0;
-    s1 = s1 * f2;
-    s1 = s1 * f2;
-    double t = s0;
-    t -= f1 * s1;
-    z0 = t * f0;
-    z1 = s1;
-}
-
-// z = M^-1 r for node i (dofs 2i, 2i+1).
-template <int Prec>
-__device__ __forceinline__ void precond_node(const double* m, int node, double r0, double r1, double& z0, double& z1) {
-    if (Prec == LFG_PRECOND_JACOBI) {
-        const double2 d = reinterpret_cast<const double2*>(m)[node];
-        z0 = d.x * r0;  // solver.cpp:502-505
-        z1 = d.y * r1;
-    } else {
-        block2_solve(m + 3LL * node, r0, r1, z0, z1);
-    }
-}
-
 __global__ void precond_apply_kernel(int kind, int n_nodes, const double* __restrict__ m, const double* __restrict__ in,
                                      double* __restrict__ out) {
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
@@ -212,7 +186,8 @@ __device__ __forceinline__ double row_dot(const int* __restrict__ rp, const int*
 // ||r|| <= switch_tau ||r0||, < 0 uses the FP32 values from the first iteration.
 template <int Prec>
 __global__ __launch_bounds__(kThreads) void pcg_init_kernel(PcgBuffers b, int stop_rule, double tol,
-                                                            long long max_iter, double drop, double switch_tau) {
+                                                            long long max_iter, double drop, double switch_tau,
+                                                            int fused) {
     __shared__ double sh[kThreads / kWave];
     const int n_nodes = b.n >> 1;
     double rr = 0.0, rz = 0.0;
@@ -226,7 +201,8 @@ __global__ __launch_bounds__(kThreads) void pcg_init_kernel(PcgBuffers b, int st
         } else {
             double z0, z1;
             precond_node<Prec>(b.m, node, r.x, r.y, z0, z1);
-            reinterpret_cast<double2*>(b.p)[node] = make_double2(z0, z1);
+            reinterpret_cast<double2*>(b.p)[node] = fused ? make_double2(0.0, 0.0) : make_double2(z0, z1);
+            if (fused) reinterpret_cast<double2*>(b.q)[node] = make_double2(0.0, 0.0);
             rz += r.x * z0 + r.y * z1;
         }
     }
@@ -244,12 +220,13 @@ __global__ __launch_bounds__(kThreads) void pcg_init_kernel(PcgBuffers b, int st
         st->rho = trz;
         st->beta = 0.0;
         st->alpha = 0.0;
-        st->iter = 0;
+        st->iter = fused ? -1 : 0;  // the first fused launch only forms p0 and A p0
         st->max_iter = max_iter;
         st->replacements = 0;
         st->maxr = rnorm0;
         st->drop = drop;
         st->need_replace = 0;
+        st->replaced = 0;
         st->phase32 = switch_tau < 0.0 ? 1 : 0;
         st->true_rr = 0.0;
         st->switch_rr = switch_tau > 0.0 ? (switch_tau * rnorm0) * (switch_tau * rnorm0) : -1.0;
@@ -448,20 +425,6 @@ __global__ __launch_bounds__(kThreads) void pcg_spmv_stream_kernel(PcgBuffers b,
 
 // ------------------------------------------------------------ node-slice SpMV
 
-typedef double dpair __attribute__((ext_vector_type(2)));
-typedef float fpair __attribute__((ext_vector_type(2)));
-
-template <typename VT>
-struct PairOf;
-template <>
-struct PairOf<double> {
-    using type = dpair;
-};
-template <>
-struct PairOf<float> {
-    using type = fpair;
-};
-
 // One thread per (slice, lane): copies node 64s + l's blocks from its two CSR rows
 // (entries 2k, 2k+1 of each row form block k) into the planes; padding is zero.
 __global__ __launch_bounds__(256) void pack_slices_kernel(int n_nodes, int n_slices, long long blocks,
@@ -629,7 +592,7 @@ __device__ __forceinline__ double slice_rows_dir(const PcgBuffers& b, const Slic
     const int width = (v.ptr[s + 1] - begin) / kWave;
     const P* __restrict__ upper = reinterpret_cast<const P*>(planes);
     const P* __restrict__ lower = upper + v.blocks;
-    const dpair* __restrict__ r2 = reinterpret_cast<const dpair*>(b.r);
+    const dpair* __restrict__ r2 = reinterpret_cast<const dpair*>(b.st->replaced ? b.z : b.r);
     const dpair* __restrict__ po2 = reinterpret_cast<const dpair*>(p_old);
     auto direction = [&](int c) {
         const dpair r = r2[c];
@@ -735,13 +698,15 @@ __global__ __launch_bounds__(kThreads) void pcg_update_kernel(PcgBuffers b) {
     if (b.st->done) return;
     __shared__ double sh[kThreads / kWave];
     const double alpha = b.st->alpha;
+    // after an accepted replacement the current residual is the FP64 one in z
+    const double* __restrict__ rsrc = (Mixed && b.st->replaced) ? b.z : b.r;
     const int n_nodes = b.n >> 1;
     double rr = 0.0, rz = 0.0;
     for (int node = blockIdx.x * kThreads + threadIdx.x; node < n_nodes; node += gridDim.x * kThreads) {
         const double2 p = reinterpret_cast<const double2*>(b.p)[node];
         const double2 q = reinterpret_cast<const double2*>(b.q)[node];
         double2 x = reinterpret_cast<double2*>(b.x)[node];
-        double2 r = reinterpret_cast<double2*>(b.r)[node];
+        double2 r = reinterpret_cast<const double2*>(rsrc)[node];
         x.x += alpha * p.x;
         x.y += alpha * p.y;
         r.x = r.x - alpha * q.x;
@@ -761,6 +726,7 @@ __global__ __launch_bounds__(kThreads) void pcg_update_kernel(PcgBuffers b) {
     if (threadIdx.x == 0) {
         const double trr = tot[0], trz = tot[1];
         PcgState* st = b.st;
+        st->replaced = 0;  // every block has read it; r now holds the updated residual
         st->iter += 1;
         st->rr = trr;
         st->rz_new = trz;
@@ -788,8 +754,13 @@ __global__ __launch_bounds__(kThreads) void pcg_update_kernel(PcgBuffers b) {
     }
 }
 
-// Mixed precision: r = b - K x in FP64 when the update kernel asked for it, then
-// ||r||^2, r.z, the convergence test on the replaced residual, and beta.
+// Mixed precision, when the update kernel asked for it: the FP64 residual b - K x
+// into z (tentatively), with its ||.||^2 and r.z. It is accepted as the residual of
+// this iteration (state `replaced`: the direction and the next update read it from
+// z) while replacements keep halving the FP64 residual; the first one that does not
+// has reached the residual floor of FP64 arithmetic on this system, where a replaced
+// residual is rounding noise that would restart the recurrence, so it is dropped,
+// replacements stop and the recurrence continues as the FP64 form's does.
 template <int W, int Prec>
 __global__ __launch_bounds__(kThreads) void pcg_replace_kernel(PcgBuffers b, const int* __restrict__ rp,
                                                                const int* __restrict__ col,
@@ -810,7 +781,7 @@ __global__ __launch_bounds__(kThreads) void pcg_replace_kernel(PcgBuffers b, con
         const double ri = b.b[row] - s;
         const double r_other = __shfl_xor(ri, W, 2 * W);
         if (active && lane == 0) {
-            b.r[row] = ri;
+            b.z[row] = ri;
             if (half == 0) {
                 double z0, z1;
                 precond_node<Prec>(b.m, node, ri, r_other, z0, z1);
@@ -826,8 +797,15 @@ __global__ __launch_bounds__(kThreads) void pcg_replace_kernel(PcgBuffers b, con
         const double trr = tot[0], trz = tot[1];
         PcgState* st = b.st;
         st->need_replace = 0;
+        if (st->true_rr > 0.0 && !(trr <= 0.25 * st->true_rr)) {
+            st->drop = 0.0;  // keep the recurrence residual and its r.z
+            accept_rz(st, st->rz_new);
+            return;
+        }
         st->replacements += 1;
+        st->replaced = 1;
         st->rr = trr;
+        st->true_rr = trr;
         const double rnorm = sqrt(max0(trr));
         st->maxr = rnorm;
         if (rnorm <= st->target) {
@@ -835,10 +813,6 @@ __global__ __launch_bounds__(kThreads) void pcg_replace_kernel(PcgBuffers b, con
             st->done = 1;
             return;
         }
-        // The FP64 residual has an attainable floor; once a replacement no longer
-        // halves it, further replacements only restart the recurrence, so stop.
-        if (st->true_rr > 0.0 && trr > 0.25 * st->true_rr) st->drop = 0.0;
-        st->true_rr = trr;
         accept_rz(st, trz);
     }
 }
@@ -848,6 +822,7 @@ template <int Prec>
 __global__ __launch_bounds__(kThreads) void pcg_direction_kernel(PcgBuffers b) {
     if (b.st->done) return;
     const double beta = b.st->beta;
+    const double* __restrict__ rsrc = b.st->replaced ? b.z : b.r;  // mixed: accepted FP64 residual
     const int n_nodes = b.n >> 1;
     for (int node = blockIdx.x * kThreads + threadIdx.x; node < n_nodes; node += gridDim.x * kThreads) {
         double z0, z1;
@@ -856,7 +831,7 @@ __global__ __launch_bounds__(kThreads) void pcg_direction_kernel(PcgBuffers b) {
             z0 = z.x;
             z1 = z.y;
         } else {
-            const double2 r = reinterpret_cast<const double2*>(b.r)[node];
+            const double2 r = reinterpret_cast<const double2*>(rsrc)[node];
             precond_node<Prec>(b.m, node, r.x, r.y, z0, z1);
         }
         double2 p = reinterpret_cast<double2*>(b.p)[node];
@@ -1052,17 +1027,20 @@ int spmv_blocks(int n, int variant) {
 }
 
 void launch_pcg_init(hipStream_t s, const PcgBuffers& b, int stop_rule, double tol, long long max_iter, double drop,
-                     double switch_tau) {
+                     double switch_tau, int fused) {
     const unsigned int grid = update_blocks(b.n);
     switch (b.precond) {
         case LFG_PRECOND_JACOBI:
-            pcg_init_kernel<LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop, switch_tau);
+            pcg_init_kernel<LFG_PRECOND_JACOBI>
+                <<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop, switch_tau, fused);
             break;
         case LFG_PRECOND_BLOCK2:
-            pcg_init_kernel<LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop, switch_tau);
+            pcg_init_kernel<LFG_PRECOND_BLOCK2>
+                <<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop, switch_tau, fused);
             break;
         default:
-            pcg_init_kernel<LFG_PRECOND_IC0><<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop, switch_tau);
+            pcg_init_kernel<LFG_PRECOND_IC0>
+                <<<grid, kThreads, 0, s>>>(b, stop_rule, tol, max_iter, drop, switch_tau, 0);
             break;
     }
 }

@@ -1,0 +1,589 @@
+// Node-stencil SpMV of the PCG iteration on gfx950 (see StencilView).
+//
+// On a structured mesh numbered row by row, node i couples only with nodes i + d for
+// a handful of offsets d, so the multiply needs no column indices: plane k of the
+// layout holds block (i, i + off[k]) for every node i, a wavefront's 64 lanes own 64
+// consecutive nodes, and every value load and every p load of the wavefront moves
+// 64 consecutive pairs. The symmetric form stores only the diagonal and the positive
+// offsets and reads block (i, i - d) as the transpose of block (i - d, i): the
+// second read of a plane trails the first by d nodes and is served from the XCD's
+// L2 (each XCD walks one contiguous node range), so HBM sees each value once.
+#include "lfg_kernels.hpp"
+#include "lfg_precond.hpp"
+
+#include "lightfem_gpu.h"
+
+namespace lfg {
+namespace {
+
+constexpr int kThreads = 256;
+
+inline unsigned int blocks_for(long long work, int per_block) {
+    return static_cast<unsigned int>((work + per_block - 1) / per_block);
+}
+
+inline unsigned int round8(unsigned int b) { return (b + 7u) & ~7u; }
+
+// Block (i, j) of the CSR matrix: rows 2i and 2i+1 share one column list of dof pairs
+// (sorted), so the pair of column 2j sits at the same offset in both rows. A node in
+// no element has only its two diagonal entries. Returns false when the pattern has
+// no such block.
+struct Block {
+    double a, b, c, d;  // (K[2i][2j], K[2i][2j+1]; K[2i+1][2j], K[2i+1][2j+1])
+};
+
+__device__ bool load_block(const int* __restrict__ rp, const int* __restrict__ col, const double* __restrict__ vals,
+                           int i, int j, Block& out) {
+    const int r0 = rp[2 * i], r1 = rp[2 * i + 1];
+    if (r1 - r0 == 1) {
+        if (j != i) return false;
+        out = Block{vals[r0], 0.0, 0.0, vals[r1]};
+        return true;
+    }
+    int lo = 0, hi = (r1 - r0) >> 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((col[r0 + 2 * mid] >> 1) < j) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo >= ((r1 - r0) >> 1) || (col[r0 + 2 * lo] >> 1) != j) return false;
+    const int t = 2 * lo;
+    out = Block{vals[r0 + t], vals[r0 + t + 1], vals[r1 + t], vals[r1 + t + 1]};
+    return true;
+}
+
+__device__ __forceinline__ float f32(double v) { return static_cast<float>(v); }
+
+// Pair index of half h (0 upper, 1 lower) of plane k for node i in an array of np
+// planes, stored per 64-node chunk: chunk c holds plane 0 upper, plane 0 lower,
+// plane 1 upper, ... as 64 consecutive pairs each, so a wavefront's loads for one
+// chunk walk one contiguous 128 * np-pair run.
+__device__ __forceinline__ long long plane_at(int i, int k, int h, int np) {
+    return ((static_cast<long long>(i >> 6) * np + k) * 2 + h) * kWave + (i & (kWave - 1));
+}
+
+// One thread per node: the node's block in every plane; with diag_c, the diagonal
+// block minus the rounding error (FP32 value - FP64 value) of each off-diagonal
+// block the FP32 multiply uses for this node's rows, in the row's column order:
+// blocks to the right are the node's own rounded values, blocks to the left are the
+// rounded transposes of the neighbours' blocks.
+__global__ __launch_bounds__(256) void pack_stencil_kernel(int n_nodes, const int* __restrict__ rp,
+                                                           const int* __restrict__ col,
+                                                           const double* __restrict__ vals, StencilView v,
+                                                           double* __restrict__ out64, float* __restrict__ out32,
+                                                           double* __restrict__ diag_c) {
+    const int i = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_nodes) return;
+    const int np64 = v.n_off, np32 = v.n_off - 1;
+    for (int k = 0; k < v.n_off; ++k) {
+        const int j = i + v.off[k];
+        Block blk{0.0, 0.0, 0.0, 0.0};
+        if (j >= 0 && j < n_nodes) load_block(rp, col, vals, i, j, blk);
+        if (out64) {
+            dpair* o = reinterpret_cast<dpair*>(out64);
+            o[plane_at(i, k, 0, np64)] = dpair{blk.a, blk.b};
+            o[plane_at(i, k, 1, np64)] = dpair{blk.c, blk.d};
+        }
+        if (out32 && k >= 1) {
+            fpair* o = reinterpret_cast<fpair*>(out32);
+            o[plane_at(i, k - 1, 0, np32)] = fpair{f32(blk.a), f32(blk.b)};
+            o[plane_at(i, k - 1, 1, np32)] = fpair{f32(blk.c), f32(blk.d)};
+        }
+    }
+    if (!diag_c) return;
+    Block d{0.0, 0.0, 0.0, 0.0};
+    load_block(rp, col, vals, i, i, d);
+    const int r0 = rp[2 * i], r1 = rp[2 * i + 1];
+    if (r1 - r0 > 1) {
+        for (int t = 0; t < r1 - r0; t += 2) {
+            const int j = col[r0 + t] >> 1;
+            if (j == i) continue;
+            const double ea = vals[r0 + t], eb = vals[r0 + t + 1], ec = vals[r1 + t], ed = vals[r1 + t + 1];
+            double ua, ub, uc, ud;  // the block the FP32 multiply applies at (i, j)
+            if (j > i) {
+                ua = f32(ea);
+                ub = f32(eb);
+                uc = f32(ec);
+                ud = f32(ed);
+            } else {
+                Block tr{0.0, 0.0, 0.0, 0.0};
+                load_block(rp, col, vals, j, i, tr);
+                ua = f32(tr.a);
+                ub = f32(tr.c);
+                uc = f32(tr.b);
+                ud = f32(tr.d);
+            }
+            d.a -= ua - ea;
+            d.b -= ub - eb;
+            d.c -= uc - ec;
+            d.d -= ud - ed;
+        }
+    }
+    dpair* o = reinterpret_cast<dpair*>(diag_c);
+    o[plane_at(i, 0, 0, 1)] = dpair{d.a, d.b};
+    o[plane_at(i, 0, 1, 1)] = dpair{d.c, d.d};
+}
+
+// Block (i, i - d) = transpose of block (i - d, i), whose upper / lower pairs are lu, ll.
+template <typename P>
+__device__ __forceinline__ void add_lower(P lu, P ll, dpair x, double& a0, double& a1) {
+    a0 += static_cast<double>(lu.x) * x.x;
+    a0 += static_cast<double>(ll.x) * x.y;
+    a1 += static_cast<double>(lu.y) * x.x;
+    a1 += static_cast<double>(ll.y) * x.y;
+}
+
+template <typename P>
+__device__ __forceinline__ void add_upper(P uu, P ul, dpair x, double& a0, double& a1) {
+    a0 += static_cast<double>(uu.x) * x.x;
+    a0 += static_cast<double>(uu.y) * x.y;
+    a1 += static_cast<double>(ul.x) * x.x;
+    a1 += static_cast<double>(ul.y) * x.y;
+}
+
+// Symmetric form, node i: rows 2i and 2i+1 accumulate the transposed blocks to the
+// left (farthest first), the diagonal block, then the blocks to the right, i.e. in
+// column order. The diagonal block is plane 0 of `diag` (np_d planes); the block of
+// offset off[k + 1] is plane kb + k of `planes` (np planes). kM > 0: exactly kM
+// positive offsets, every load issued before the arithmetic; kM = 0: any count.
+// kLds: the multiplied vector comes from a tile's LDS windows (node i = i0 + t):
+// node i + d at win[slot + t] (see StencilView); otherwise from p2[i + d].
+template <typename VT, int kM, bool kLds>
+__device__ __forceinline__ void stencil_sym_node(const StencilView& v, const dpair* __restrict__ diag, int np_d,
+                                                 const VT* __restrict__ planes, int np, int kb,
+                                                 const dpair* __restrict__ p2, int i, double& a0, double& a1,
+                                                 const dpair* win, int t) {
+    using P = typename PairOf<VT>::type;
+    const P* __restrict__ pl = reinterpret_cast<const P*>(planes);
+    const dpair du = __builtin_nontemporal_load(diag + plane_at(i, 0, 0, np_d));
+    const dpair dl = __builtin_nontemporal_load(diag + plane_at(i, 0, 1, np_d));
+    const dpair xi = kLds ? win[v.slot_zero + t] : p2[i];
+    a0 = 0.0;
+    a1 = 0.0;
+    if (kM > 0) {
+        constexpr int M = kM > 0 ? kM : 1;
+        P lu[M], ll[M], uu[M], ul[M];
+        dpair xl[M], xu[M];
+        bool okl[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int d = v.off[k + 1];
+            const int jl = i - d;
+            okl[k] = jl >= 0;
+            const int jlc = okl[k] ? jl : i;
+            const int ju = min(i + d, v.n_nodes - 1);
+            uu[k] = pl[plane_at(i, kb + k, 0, np)];
+            ul[k] = pl[plane_at(i, kb + k, 1, np)];
+            lu[k] = __builtin_nontemporal_load(pl + plane_at(jlc, kb + k, 0, np));
+            ll[k] = __builtin_nontemporal_load(pl + plane_at(jlc, kb + k, 1, np));
+            xl[k] = kLds ? win[v.slot_lo[k + 1] + t] : p2[jlc];
+            xu[k] = kLds ? win[v.slot_up[k + 1] + t] : p2[ju];
+        }
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k)
+            if (okl[k]) add_lower(lu[k], ll[k], xl[k], a0, a1);
+        add_upper(du, dl, xi, a0, a1);
+#pragma unroll
+        for (int k = 0; k < M; ++k) add_upper(uu[k], ul[k], xu[k], a0, a1);
+    } else {
+        const int m = v.n_off - 1;
+        for (int k = m - 1; k >= 0; --k) {
+            const int jl = i - v.off[k + 1];
+            if (jl < 0) continue;
+            const dpair x = kLds ? win[v.slot_lo[k + 1] + t] : p2[jl];
+            add_lower(pl[plane_at(jl, kb + k, 0, np)], pl[plane_at(jl, kb + k, 1, np)], x, a0, a1);
+        }
+        add_upper(du, dl, xi, a0, a1);
+        for (int k = 0; k < m; ++k) {
+            const int ju = min(i + v.off[k + 1], v.n_nodes - 1);
+            const dpair x = kLds ? win[v.slot_up[k + 1] + t] : p2[ju];
+            add_upper(pl[plane_at(i, kb + k, 0, np)], pl[plane_at(i, kb + k, 1, np)], x, a0, a1);
+        }
+    }
+}
+
+// Exact form, node i: every plane in ascending offset order (the CSR column order);
+// the zero blocks of absent couplings add nothing. kN > 0: exactly kN planes, loads
+// first; kN = 0: any count.
+template <int kN>
+__device__ __forceinline__ void stencil_exact_node(const StencilView& v, const double* __restrict__ planes,
+                                                   const dpair* __restrict__ p2, int i, double& a0, double& a1) {
+    const dpair* __restrict__ pl = reinterpret_cast<const dpair*>(planes);
+    auto term = [&](dpair u, dpair l, dpair x) {
+        a0 += u.x * x.x;
+        a0 += u.y * x.y;
+        a1 += l.x * x.x;
+        a1 += l.y * x.y;
+    };
+    a0 = 0.0;
+    a1 = 0.0;
+    if (kN > 0) {
+        constexpr int N = kN > 0 ? kN : 1;
+        dpair u[N], l[N], x[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int j = min(max(i + v.off[k], 0), v.n_nodes - 1);
+            u[k] = __builtin_nontemporal_load(pl + plane_at(i, k, 0, N));
+            l[k] = __builtin_nontemporal_load(pl + plane_at(i, k, 1, N));
+            x[k] = p2[j];
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) term(u[k], l[k], x[k]);
+    } else {
+        for (int k = 0; k < v.n_off; ++k) {
+            const int j = min(max(i + v.off[k], 0), v.n_nodes - 1);
+            term(pl[plane_at(i, k, 0, v.n_off)], pl[plane_at(i, k, 1, v.n_off)], p2[j]);
+        }
+    }
+}
+
+// q = A p over the stencil planes, fused p.q and alpha = rho / p.q (solver.cpp:589-597).
+// Each block owns a contiguous node range (XCD-remapped), one node per thread and step.
+// kSym: symmetric form with kM positive offsets; otherwise the exact form with kM planes.
+template <int kM, bool kSym>
+__global__ __launch_bounds__(kThreads) void pcg_spmv_stencil_kernel(PcgBuffers b, StencilView v,
+                                                                    const double* __restrict__ planes64,
+                                                                    const float* __restrict__ planes32,
+                                                                    const double* __restrict__ diag_c) {
+    if (b.st->done) return;
+    __shared__ double sh[kThreads / kWave];
+    const bool use32 = kSym && planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = ((v.n_nodes + gridDim.x - 1) / gridDim.x + kWave - 1) / kWave * kWave;
+    const int start = lb * per_block;
+    const int stop = min(start + per_block, v.n_nodes);
+    const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(b.p);
+    dpair* __restrict__ q2 = reinterpret_cast<dpair*>(b.q);
+    double pq = 0.0;
+    for (int i = start + static_cast<int>(threadIdx.x); i < stop; i += kThreads) {
+        double a0, a1;
+        if (kSym) {
+            if (use32)
+                stencil_sym_node<float, kM, false>(v, reinterpret_cast<const dpair*>(diag_c), 1, planes32,
+                                                   v.n_off - 1, 0, p2, i, a0, a1, nullptr, 0);
+            else
+                stencil_sym_node<double, kM, false>(v, reinterpret_cast<const dpair*>(planes64), v.n_off, planes64,
+                                                    v.n_off, 1, p2, i, a0, a1, nullptr, 0);
+        } else {
+            stencil_exact_node<kM>(v, planes64, p2, i, a0, a1);
+        }
+        q2[i] = dpair{a0, a1};
+        const dpair pv = p2[i];
+        pq += pv.x * a0 + pv.y * a1;
+    }
+    const double vals[1] = {block_sum<kThreads>(pq, sh)};
+    double tot[1];
+    if (!finish_reduction<kThreads, 1>(b.tickets + 1, b.partials, vals, tot, sh)) return;
+    if (threadIdx.x == 0) {
+        const double tpq = tot[0];
+        PcgState* st = b.st;
+        st->pq = tpq;
+        if (!(tpq > 0.0) || !isfinite(tpq)) {  // solver.cpp:591-593
+            st->status = LFG_ERR_NOT_SPD;
+            st->done = 1;
+        } else {
+            st->alpha = st->rho / tpq;
+        }
+    }
+}
+
+// K sums over a block of NT threads in one LDS round; totals in v of thread 0.
+template <int NT, int K>
+__device__ __forceinline__ void block_sums(double (&v)[K], double* sh) {
+    constexpr int W = NT / kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) sh[k * W + threadIdx.x / kWave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double acc = 0.0;
+            for (int w = 0; w < W; ++w) acc += sh[k * W + w];
+            v[k] = acc;
+        }
+    }
+    __syncthreads();
+}
+
+// p' = M^-1 (r - alpha q) + beta p for node j (solver.cpp:599-604, 612-620).
+template <int Prec>
+__device__ __forceinline__ dpair next_direction(const double* __restrict__ m, const dpair* __restrict__ r2,
+                                                const dpair* __restrict__ q2, const dpair* __restrict__ p2, int j,
+                                                double alpha, double beta) {
+    const dpair ro = r2[j], qo = q2[j], po = p2[j];
+    double z0, z1;
+    precond_node<Prec>(m, j, ro.x - alpha * qo.x, ro.y - alpha * qo.y, z0, z1);
+    return dpair{z0 + beta * po.x, z1 + beta * po.y};
+}
+
+// The matrix values one node's rows need in the symmetric form (kM positive offsets).
+// They are loaded after the tile's vector phase: loading them first keeps ~70 more
+// registers live through it, and the lower occupancy costs more (83 against 65 us per
+// iteration at 1000x1000) than the overlap gains.
+template <typename VT, int kM>
+struct SymRows {
+    using P = typename PairOf<VT>::type;
+    P lu[kM], ll[kM], uu[kM], ul[kM];
+    bool okl[kM];
+    dpair du, dl;
+};
+
+template <typename VT, int kM>
+__device__ __forceinline__ void load_sym_rows(const StencilView& v, const dpair* __restrict__ diag, int np_d,
+                                              const VT* __restrict__ planes, int np, int kb, int i,
+                                              SymRows<VT, kM>& a) {
+    using P = typename PairOf<VT>::type;
+    const P* __restrict__ pl = reinterpret_cast<const P*>(planes);
+#pragma unroll
+    for (int k = 0; k < kM; ++k) {
+        const int jl = i - v.off[k + 1];
+        a.okl[k] = jl >= 0;
+        const int jlc = a.okl[k] ? jl : i;
+        a.uu[k] = pl[plane_at(i, kb + k, 0, np)];
+        a.ul[k] = pl[plane_at(i, kb + k, 1, np)];
+        a.lu[k] = __builtin_nontemporal_load(pl + plane_at(jlc, kb + k, 0, np));
+        a.ll[k] = __builtin_nontemporal_load(pl + plane_at(jlc, kb + k, 1, np));
+    }
+    a.du = __builtin_nontemporal_load(diag + plane_at(i, 0, 0, np_d));
+    a.dl = __builtin_nontemporal_load(diag + plane_at(i, 0, 1, np_d));
+}
+
+// Rows 2i, 2i+1 times the tile's LDS windows, in column order (as stencil_sym_node).
+template <typename VT, int kM>
+__device__ __forceinline__ void apply_sym_rows(const StencilView& v, const SymRows<VT, kM>& a,
+                                               const dpair* __restrict__ win, int t, double& a0, double& a1) {
+    a0 = 0.0;
+    a1 = 0.0;
+#pragma unroll
+    for (int k = kM - 1; k >= 0; --k)
+        if (a.okl[k]) add_lower(a.lu[k], a.ll[k], win[v.slot_lo[k + 1] + t], a0, a1);
+    add_upper(a.du, a.dl, win[v.slot_zero + t], a0, a1);
+#pragma unroll
+    for (int k = 0; k < kM; ++k) add_upper(a.uu[k], a.ul[k], win[v.slot_up[k + 1] + t], a0, a1);
+}
+
+constexpr int kHalo = 3;  // halo window positions per thread and round
+
+// The tiles of one block for values of type VT (see pcg_fused_stencil_kernel).
+template <typename VT, int kM, int Prec>
+__device__ __forceinline__ void fused_tiles(const PcgBuffers& b, const StencilView& v, const dpair* __restrict__ diag,
+                                            int np_d, const VT* __restrict__ planes, int np, int kb,
+                                            const double* __restrict__ r_in, double* __restrict__ r_out,
+                                            const double* __restrict__ p_in, double* __restrict__ p_out,
+                                            const double* __restrict__ q_in, double* __restrict__ q_out,
+                                            double alpha, double beta, int tile_begin, int tile_end, dpair* win,
+                                            double (&acc)[5]) {
+    const int n = v.n_nodes;
+    const int t = threadIdx.x;
+    const dpair* __restrict__ r2 = reinterpret_cast<const dpair*>(r_in);
+    const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(p_in);
+    const dpair* __restrict__ q2 = reinterpret_cast<const dpair*>(q_in);
+    dpair* __restrict__ x2 = reinterpret_cast<dpair*>(b.x);
+    for (int tile = tile_begin; tile < tile_end; ++tile) {
+        const int i0 = tile * kFusedTile;
+        const int i = i0 + t;
+        const bool own = i < n;
+        const int ic = own ? i : n - 1;
+        constexpr int M = kM > 0 ? kM : 1;
+        SymRows<VT, M> rows;
+        dpair pn = {0.0, 0.0}, zn = {0.0, 0.0};
+        if (own) {
+            const dpair ro = r2[i], qo = q2[i], po = p2[i];
+            dpair x = x2[i];
+            x.x += alpha * po.x;  // solver.cpp:599-601
+            x.y += alpha * po.y;
+            const dpair rn = {ro.x - alpha * qo.x, ro.y - alpha * qo.y};
+            double z0, z1;
+            precond_node<Prec>(b.m, i, rn.x, rn.y, z0, z1);
+            zn = dpair{z0, z1};
+            pn = dpair{z0 + beta * po.x, z1 + beta * po.y};
+            x2[i] = x;
+            reinterpret_cast<dpair*>(r_out)[i] = rn;
+            reinterpret_cast<dpair*>(p_out)[i] = pn;
+            acc[0] += rn.x * rn.x + rn.y * rn.y;
+            acc[1] += rn.x * z0 + rn.y * z1;
+        }
+        win[v.slot_zero + t] = pn;
+        // halo positions h of the windows (every slot but the own ones), kHalo per thread
+        // and round with all their loads issued before any is used
+        const int halo = v.win_total - kFusedTile;
+        for (int h0 = 0; h0 < halo; h0 += kHalo * kFusedTile) {
+            int hw[kHalo], hj[kHalo];
+            dpair hr[kHalo], hq[kHalo], hp[kHalo];
+            NodePrecond<Prec> hm[kHalo];
+#pragma unroll
+            for (int u = 0; u < kHalo; ++u) {
+                const int h = h0 + u * kFusedTile + t;
+                const int w = h < v.slot_zero ? h : h + kFusedTile;
+                int c = 0;
+                while (c + 1 < v.n_clusters && w >= v.win_base[c + 1]) ++c;
+                const int j = i0 + v.win_lo[c] + (w - v.win_base[c]);
+                hw[u] = h < halo ? w : -1;
+                hj[u] = (h < halo && j >= 0 && j < n) ? j : -1;
+                const int jc = hj[u] >= 0 ? hj[u] : 0;
+                hr[u] = r2[jc];
+                hq[u] = q2[jc];
+                hp[u] = p2[jc];
+                hm[u].load(b.m, jc);
+            }
+#pragma unroll
+            for (int u = 0; u < kHalo; ++u) {
+                if (hw[u] < 0) continue;
+                dpair pj = {0.0, 0.0};
+                if (hj[u] >= 0) {
+                    double z0, z1;
+                    hm[u].apply(hr[u].x - alpha * hq[u].x, hr[u].y - alpha * hq[u].y, z0, z1);
+                    pj = dpair{z0 + beta * hp[u].x, z1 + beta * hp[u].y};
+                }
+                win[hw[u]] = pj;
+            }
+        }
+        __syncthreads();
+        if (own) {
+            double a0, a1;
+            if (kM > 0) {
+                load_sym_rows<VT, M>(v, diag, np_d, planes, np, kb, ic, rows);
+                apply_sym_rows<VT, M>(v, rows, win, t, a0, a1);
+            }
+            else
+                stencil_sym_node<VT, 0, true>(v, diag, np_d, planes, np, kb, nullptr, i, a0, a1, win, t);
+            reinterpret_cast<dpair*>(q_out)[i] = dpair{a0, a1};
+            double m0, m1;
+            precond_node<Prec>(b.m, i, a0, a1, m0, m1);
+            acc[2] += pn.x * a0 + pn.y * a1;
+            acc[3] += a0 * zn.x + a1 * zn.y;
+            acc[4] += a0 * m0 + a1 * m1;
+        }
+        __syncthreads();  // the next tile overwrites the windows
+    }
+}
+
+// One PCG iteration per launch on the symmetric stencil form (launch_pcg_fused_stencil).
+// Each block walks a contiguous range of 256-node tiles (XCD-remapped). Per tile:
+//   1. every thread updates its own node: x += alpha p, r' = r - alpha q, z = M^-1 r',
+//      p' = z + beta p, written to x, r_out, p_out and to the tile's own LDS window slot;
+//   2. the remaining window positions (the halo the multiply reads, up to the largest
+//      offset on either side) get p' formed the same way from the neighbours' r, q, p;
+//   3. q' = A p' for the own node from the rows and the LDS windows, written to q_out.
+// Inputs and outputs of r, p, q are different buffers, so a neighbour's old values are
+// never overwritten while another block still reads them.
+template <int kM, int Prec>
+__global__ __launch_bounds__(kFusedTile) void pcg_fused_stencil_kernel(
+    PcgBuffers b, StencilView v, const double* __restrict__ planes64, const float* __restrict__ planes32,
+    const double* __restrict__ diag_c, const double* __restrict__ r_in, double* __restrict__ r_out,
+    const double* __restrict__ p_in, double* __restrict__ p_out, const double* __restrict__ q_in,
+    double* __restrict__ q_out) {
+    if (b.st->done) return;
+    extern __shared__ double win_lds[];
+    dpair* const win = reinterpret_cast<dpair*>(win_lds);
+    __shared__ double sh[5 * (kFusedTile / kWave)];
+    const double alpha = b.st->alpha, beta = b.st->beta;
+    const bool use32 = planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
+    const int n_tiles = (v.n_nodes + kFusedTile - 1) / kFusedTile;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = (n_tiles + gridDim.x - 1) / gridDim.x;
+    const int tile_begin = lb * per_block;
+    const int tile_end = min(tile_begin + per_block, n_tiles);
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // r'.r', r'.z, p'.q', q'.z, q'.M^-1 q'
+    if (use32)
+        fused_tiles<float, kM, Prec>(b, v, reinterpret_cast<const dpair*>(diag_c), 1, planes32, v.n_off - 1, 0, r_in,
+                                     r_out, p_in, p_out, q_in, q_out, alpha, beta, tile_begin, tile_end, win, acc);
+    else
+        fused_tiles<double, kM, Prec>(b, v, reinterpret_cast<const dpair*>(planes64), v.n_off, planes64, v.n_off, 1,
+                                      r_in, r_out, p_in, p_out, q_in, q_out, alpha, beta, tile_begin, tile_end, win,
+                                      acc);
+    block_sums<kFusedTile, 5>(acc, sh);
+    double tot[5];
+    if (!finish_reduction<kFusedTile, 5>(b.tickets + 1, b.partials, acc, tot, sh)) return;
+    if (threadIdx.x == 0) {
+        PcgState* st = b.st;
+        const double rr = tot[0], rho = tot[1], pq = tot[2], qz = tot[3], qmq = tot[4];
+        st->iter += 1;  // -1 -> 0 on the first launch, which only forms p0 and A p0
+        st->rr = rr;
+        if (st->iter > 0 && sqrt(max0(rr)) <= st->target) {  // solver.cpp:606-611
+            st->converged = 1;
+            st->done = 1;
+            return;
+        }
+        if (!(rho > 0.0) || !isfinite(rho)) {  // solver.cpp:579-581, 614-618
+            st->status = LFG_ERR_NONFINITE;
+            st->done = 1;
+            return;
+        }
+        if (st->iter >= st->max_iter) {
+            st->done = 1;
+            return;
+        }
+        if (!(pq > 0.0) || !isfinite(pq)) {  // solver.cpp:591-593 of the next iteration
+            st->status = LFG_ERR_NOT_SPD;
+            st->done = 1;
+            return;
+        }
+        const double a = rho / pq;
+        // r''.M^-1 r'' for r'' = r' - a q' (M^-1 symmetric); a cancellation below zero
+        // means r'' is negligible next to r', where beta = 0 is the exact limit
+        const double rho_next = rho - 2.0 * a * qz + a * a * qmq;
+        st->pq = pq;
+        st->rho = rho;
+        st->alpha = a;
+        st->beta = max0(rho_next) / rho;
+        if (!st->phase32 && rr <= st->switch_rr) st->phase32 = 1;  // mixed: FP32 values from here on
+    }
+}
+
+}  // namespace
+
+void launch_pack_stencil(hipStream_t s, const CsrView& a, const StencilView& v, const double* values, double* out64,
+                         float* out32, double* diag_c) {
+    if (v.n_nodes <= 0) return;
+    pack_stencil_kernel<<<blocks_for(v.n_nodes, 256), 256, 0, s>>>(v.n_nodes, a.row_ptr, a.col, values, v, out64,
+                                                                   out32, diag_c);
+}
+
+void launch_pcg_spmv_stencil(hipStream_t s, const PcgBuffers& b, const StencilView& v, const double* planes64,
+                             const float* planes32, const double* diag_c) {
+    const unsigned int need = round8(blocks_for(v.n_nodes, kThreads));
+    const unsigned int grid = need < 4096u ? need : 4096u;
+    if (v.sym) {
+        // Q4 meshes numbered row by row have 4 positive offsets; other counts take the
+        // general loop
+        if (v.n_off == 5)
+            pcg_spmv_stencil_kernel<4, true><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32, diag_c);
+        else
+            pcg_spmv_stencil_kernel<0, true><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32, diag_c);
+    } else {
+        if (v.n_off == 9)
+            pcg_spmv_stencil_kernel<9, false><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32, diag_c);
+        else
+            pcg_spmv_stencil_kernel<0, false><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32, diag_c);
+    }
+}
+
+void launch_pcg_fused_stencil(hipStream_t s, const PcgBuffers& b, const StencilView& v, const double* planes64,
+                              const float* planes32, const double* diag_c, const double* r_in, double* r_out,
+                              const double* p_in, double* p_out, const double* q_in, double* q_out) {
+    const unsigned int tiles = blocks_for(v.n_nodes, kFusedTile);
+    const unsigned int grid = round8(tiles < 8192u ? tiles : 8192u);
+    const size_t lds = static_cast<size_t>(v.win_total) * sizeof(dpair);
+    if (v.n_off == 5) {
+        if (b.precond == LFG_PRECOND_BLOCK2)
+            pcg_fused_stencil_kernel<4, LFG_PRECOND_BLOCK2><<<grid, kFusedTile, lds, s>>>(
+                b, v, planes64, planes32, diag_c, r_in, r_out, p_in, p_out, q_in, q_out);
+        else
+            pcg_fused_stencil_kernel<4, LFG_PRECOND_JACOBI><<<grid, kFusedTile, lds, s>>>(
+                b, v, planes64, planes32, diag_c, r_in, r_out, p_in, p_out, q_in, q_out);
+    } else {
+        if (b.precond == LFG_PRECOND_BLOCK2)
+            pcg_fused_stencil_kernel<0, LFG_PRECOND_BLOCK2><<<grid, kFusedTile, lds, s>>>(
+                b, v, planes64, planes32, diag_c, r_in, r_out, p_in, p_out, q_in, q_out);
+        else
+            pcg_fused_stencil_kernel<0, LFG_PRECOND_JACOBI><<<grid, kFusedTile, lds, s>>>(
+                b, v, planes64, planes32, diag_c, r_in, r_out, p_in, p_out, q_in, q_out);
+    }
+}
+
+}  // namespace lfg

@@ -28,7 +28,7 @@ GPU_SYMBOLS = ["lfg_create", "lfg_destroy", "lfg_last_error", "lfg_status_string
                "lfg_set_load", "lfg_dirichlet", "lfg_get_system", "lfg_pcg", "lfg_reactions", "lfg_postprocess",
                "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv", "lfg_bench_copy", "lfg_set_materials",
                "lfg_adjoint_solve", "lfg_sensitivity", "lfg_set_material_modes", "lfg_batch_set_problems",
-               "lfg_batch_set_loads", "lfg_batch_pcg"]
+               "lfg_batch_set_loads", "lfg_batch_pcg", "lfg_layout_info_get"]
 HOST_SYMBOLS = ["lfh_csr_pattern_nnz", "lfh_csr_pattern", "lfh_rcm", "lfh_element_colors", "lfh_csr_profile"]
 
 
@@ -65,6 +65,17 @@ class AssemblyReport(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class LayoutInfo(ctypes.Structure):
+    _fields_ = [("node_blocked", ctypes.c_int32), ("n_slices", ctypes.c_int32),
+                ("stencil_exact_planes", ctypes.c_int32), ("stencil_sym_planes", ctypes.c_int32),
+                ("fused_window_pairs", ctypes.c_int32), ("stencil_offsets", ctypes.c_int32 * 17)]
+
+    def as_dict(self) -> dict:
+        out = {name: getattr(self, name) for name, _ in self._fields_[:5]}
+        out["stencil_offsets"] = list(self.stencil_offsets[:self.stencil_exact_planes])
+        return out
 
 
 class BatchOptions(ctypes.Structure):
@@ -136,6 +147,7 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lfg_batch_set_problems.argtypes = [_P, ctypes.c_int32, _P]
         lib.lfg_batch_set_loads.argtypes = [_P, ctypes.c_int32, _P, _I64, _P, _P]
         lib.lfg_batch_pcg.argtypes = [_P, ctypes.POINTER(BatchOptions), _P, _P, ctypes.POINTER(BatchSummary)]
+        lib.lfg_layout_info_get.argtypes = [_P, ctypes.POINTER(LayoutInfo)]
         _gpu = lib
     return _gpu
 
@@ -401,6 +413,12 @@ class Device:
         out = np.empty_like(vec)
         self._check(self._lib.lfg_precond_apply(self._ctx, PRECOND[preconditioner], _ptr(vec), _ptr(out)))
         return out
+
+    def layout_info(self) -> dict:
+        """SpMV layouts the current pattern admits (node slices, node-stencil forms)."""
+        info = LayoutInfo()
+        self._check(self._lib.lfg_layout_info_get(self._ctx, ctypes.byref(info)))
+        return info.as_dict()
 
     def spmv(self, x, precision: str = "fp64", variant: int = 0) -> np.ndarray:
         x = _f64(x)

@@ -101,6 +101,7 @@ def test_stages_match_reference(dev, golden_index, name):
     for variant in (4, 5, 6):
         assert np.array_equal(bits(dev.spmv(g["probe"], "fp64", variant)), bits(y_ref)), variant
         assert np.array_equal(bits(dev.spmv(g["probe"], "mixed", variant)), bits(y32_ref)), variant
+    check_stencil_spmv(dev, g["row_ptr"], g["col"], g["Kc"], g["probe"])
     # PCG with the case's preconditioner and the reference stop rule
     precond = PRECOND_NAME[case_precond(meta)]
     u, rep = dev.pcg(precond, "fp64", "reference", meta["tolerance"])
@@ -119,6 +120,30 @@ def test_stages_match_reference(dev, golden_index, name):
     scale = np.max(np.abs(g["R"]))
     assert np.max(np.abs(R - g["R"])) <= 1e-9 * scale
     assert np.max(np.abs(centre - g["elem_centre"])) <= 1e-9 * np.max(np.abs(g["elem_centre"]))
+
+
+def check_stencil_spmv(dev, row_ptr, col, kc, x):
+    """Node-stencil multiplies where the pattern admits them. Exact form: every offset
+    in column order, bit-identical to the CSR row loop. Symmetric form: block (i, i-d)
+    is the transpose of block (i-d, i), equal to the stored one up to the rounding of
+    Ke's (a,b)/(b,a) entries, so each row is within 1e-14 of |K||x|. FP32 copy: FP32
+    rounding of each off-diagonal value (2^-24 relative), once in the product and once
+    in the diagonal's compensation, and the nodal translations act exactly."""
+    info = dev.layout_info()
+    y_ref = lfo.spmv(row_ptr, col, kc, x)
+    scale = lfo.spmv(row_ptr, col, np.abs(kc), np.abs(x))
+    row_abs = lfo.spmv(row_ptr, col, np.abs(kc), np.ones_like(x))
+    if info["stencil_exact_planes"]:
+        assert np.array_equal(bits(dev.spmv(x, "fp64", 9)), bits(y_ref))
+    if info["stencil_sym_planes"]:
+        assert np.all(np.abs(dev.spmv(x, "fp64", 8) - y_ref) <= 1e-14 * scale)
+        bound = 2.0 ** -23 * (scale + row_abs * np.abs(x)) + 1e-14 * scale
+        assert np.all(np.abs(dev.spmv(x, "mixed", 8) - y_ref) <= bound)
+        for t in (np.tile([1.0, 0.0], x.shape[0] // 2), np.tile([0.0, 1.0], x.shape[0] // 2)):
+            kt = lfo.spmv(row_ptr, col, kc, t)
+            t_scale = lfo.spmv(row_ptr, col, np.abs(kc), t)
+            assert np.all(np.abs(dev.spmv(t, "mixed", 8) - kt) <= 1e-13 * t_scale)
+    return info
 
 
 def _c1(dev, assembly="serial"):
@@ -162,22 +187,31 @@ def test_c2_preconditioners(dev, golden_index, precond, iters):
         assert rep["breakdowns"] == 0 and rep["levels"] > 0
 
 
-@pytest.mark.parametrize("precond,switch", [("jacobi", 0.0), ("block", 0.0), ("jacobi", -1.0)])
-def test_mixed_precision_matches_fp64(dev, precond, switch):
+@pytest.mark.parametrize("precond,switch,variant,drop", [
+    ("jacobi", 0.0, 0, 0.0), ("block", 0.0, 0, 0.0), ("jacobi", 0.0, 8, 0.0), ("jacobi", 0.0, 8, 0.1),
+    ("jacobi", -1.0, 8, 0.1), ("jacobi", 0.0, 6, 0.1), ("jacobi", -1.0, 6, 0.1)])
+def test_mixed_precision_matches_fp64(dev, precond, switch, variant, drop):
     """FP32 values (after an FP64 phase, or from the start), FP64 vectors and dots,
-    FP64 residual replacement: the same answer as FP64 and a direct solve to 1e-9."""
+    with or without FP64 residual replacements: the same answer as FP64 and a direct
+    solve to 1e-9."""
     import scipy.sparse
     import scipy.sparse.linalg
     n = 64
-    print(f"case mixed {n}x{n} log-uniform E plane strain {precond} switch={switch}", flush=True)
+    print(f"case mixed {n}x{n} log-uniform E plane strain {precond} switch={switch} variant={variant} drop={drop}",
+          flush=True)
     m = meshgen.c3_model(n)
     sys_ = pipeline.setup(m, device=dev, assembly="coloring")
     kc, fc = dev.system()
     u64, r64 = dev.pcg(precond, "fp64", "relative", 1e-10)
-    um, rm = dev.pcg(precond, "mixed", "relative", 1e-10, mixed_switch=switch)
-    assert r64["converged"] and rm["converged"] and rm["residual_replacements"] >= 1
-    # both forms stop on the recurrence residual; the true residual stays close to it
-    assert rm["true_residual"] <= 1.0e-9 * rm["initial_residual"]
+    um, rm = dev.pcg(precond, "mixed", "relative", 1e-10, mixed_switch=switch, spmv_variant=variant,
+                     replace_drop=drop)
+    assert r64["converged"] and rm["converged"]
+    assert (rm["residual_replacements"] >= 1) == (drop > 0.0)
+    # both forms stop on the recurrence residual. With replacements the FP64 residual
+    # follows it; without, the FP32 products after the switch leave b - K u at the
+    # rounding level of the FP32 operator (a few 1e-9 here), which moves u far less
+    # than the 1e-9 bar below.
+    assert rm["true_residual"] <= (1.0e-9 if drop > 0.0 else 1.0e-8) * rm["initial_residual"]
     assert r64["true_residual"] <= 1.0e-9 * r64["initial_residual"]
     A = scipy.sparse.csr_matrix((kc, sys_.col, sys_.row_ptr))
     exact = scipy.sparse.linalg.spsolve(A.tocsc(), fc)
@@ -193,9 +227,50 @@ def test_deterministic_and_graph_equals_direct(dev):
     u3, r3 = dev.pcg("jacobi", direct_launch=True, check_every=7)
     assert np.array_equal(bits(u1), bits(u2)) and np.array_equal(bits(u1), bits(u3))
     assert r1["iterations"] == r2["iterations"] == r3["iterations"]
-    for variant in (1, 2, 3, 4, 5, 6, 7):
+    for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10):
         u4, r4 = dev.pcg("jacobi", spmv_variant=variant)
-        assert r4["converged"] and np.linalg.norm(u4 - u1) <= 1e-12 * np.linalg.norm(u1)
+        assert r4["converged"] and np.linalg.norm(u4 - u1) <= 1e-12 * np.linalg.norm(u1), variant
+    u10, r10 = dev.pcg("jacobi", spmv_variant=10, direct_launch=True, check_every=7)
+    assert np.array_equal(bits(u10), bits(dev.pcg("jacobi", spmv_variant=10)[0]))
+
+
+def test_stencil_layouts(dev):
+    """The node-stencil SpMV (variants 8, 9) on structured meshes numbered row by row:
+    offsets found from the pattern, multiplies against the CSR row loop, PCG against
+    the node-slice PCG (both preconditioners it fuses), run-to-run identical."""
+    print("case stencil 50x50 and 64x64 log-E", flush=True)
+    sys_ = pipeline.setup(meshgen.c1_model(), device=dev)
+    info = dev.layout_info()
+    assert info["stencil_offsets"] == [-52, -51, -50, -1, 0, 1, 50, 51, 52]
+    assert info["stencil_sym_planes"] == 5 and info["stencil_exact_planes"] == 9
+    kc, _ = dev.system()
+    x = np.random.default_rng(7).standard_normal(dev.n)
+    check_stencil_spmv(dev, sys_.row_ptr, sys_.col, kc, x)
+    assert np.array_equal(bits(dev.spmv(x, "fp64", 9)), bits(dev.spmv(x, "fp64", 6)))
+    pipeline.setup(meshgen.c3_model(64), device=dev, assembly="coloring")
+    for precond in ("jacobi", "block"):
+        u6, r6 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=6)
+        u9, r9 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=9)
+        assert np.array_equal(bits(u6), bits(u9)) and r6["iterations"] == r9["iterations"], precond
+        for variant in (8, 10):  # 10: the whole iteration fused into one launch
+            u8, r8 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=variant)
+            u8b, r8b = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=variant)
+            assert np.array_equal(bits(u8), bits(u8b)) and r8["iterations"] == r8b["iterations"]
+            assert abs(r8["iterations"] - r6["iterations"]) <= 2 + 0.01 * r6["iterations"], (precond, variant)
+            assert np.linalg.norm(u8 - u6) <= 1e-9 * np.linalg.norm(u6), (precond, variant)
+    # the fused iteration honours the iteration cap and reports the reference's count
+    with pytest.raises(native.LightFemError) as err:
+        dev.pcg("jacobi", "fp64", "relative", 1e-10, max_iterations=5, spmv_variant=10)
+    assert err.value.code == native.ERR_NOT_CONVERGED
+    _, capped = dev.pcg("jacobi", "fp64", "relative", 1e-10, max_iterations=5, spmv_variant=10, raise_on_error=False)
+    assert capped["iterations"] == 5 and not capped["converged"]
+    # an RCM-numbered mesh has no fixed offsets: the default falls back to node slices
+    pipeline.setup(meshgen.c2_model(), device=dev, assembly="coloring", reordering="rcm")
+    info = dev.layout_info()
+    assert info["node_blocked"] == 1 and info["stencil_sym_planes"] == 0 and info["fused_window_pairs"] == 0
+    with pytest.raises(native.LightFemError) as err:
+        dev.spmv(np.ones(dev.n), "fp64", 8)
+    assert err.value.code == native.ERR_UNSUPPORTED
 
 
 def test_fused_direction_is_bit_identical(dev):
@@ -271,6 +346,9 @@ def test_full_size_assembly_properties(dev):
     A = scipy.sparse.csr_matrix((kc, sys_.col, sys_.row_ptr), shape=(2004002, 2004002))
     ref = A @ x
     assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref))
-    # the node-slice and row-segment kernels both sum rows in column order
+    # the node-slice, node-stencil (exact form) and row-segment kernels all sum rows in column order
     assert np.array_equal(bits(dev.spmv(x, "fp64", 6)), bits(dev.spmv(x, "fp64", 4)))
+    assert np.array_equal(bits(dev.spmv(x, "fp64", 9)), bits(dev.spmv(x, "fp64", 4)))
     assert np.array_equal(bits(dev.spmv(x, "mixed", 6)), bits(dev.spmv(x, "mixed", 4)))
+    info = check_stencil_spmv(dev, sys_.row_ptr, sys_.col, kc, x)
+    assert info["stencil_sym_planes"] == 5
