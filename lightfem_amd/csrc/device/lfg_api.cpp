@@ -57,9 +57,24 @@ struct DevArray {
     }
 };
 
+struct Ic0SweepDevice {
+    DevArray<int> ord, ptr, key, level_ptr;
+    DevArray<double> val, scale;
+    int levels = 0, reach = 0, widest = 0;
+};
+
+struct Ic0NodeSweepDevice {
+    DevArray<int> ord, ptr, key, level_ptr;
+    DevArray<double> v_first, v_second, cross, scale_first, scale_second;
+    int levels = 0, reach = 0, widest = 0;
+    bool ready = false;
+};
+
 struct Ic0Device {
     DevArray<int> lrow, lcol, urow, ucol, fptr, frows, bptr, brows;
     DevArray<double> lval, uval, inv_diag;
+    Ic0SweepDevice fwd, bwd;  // level-ordered sweeps
+    Ic0NodeSweepDevice nfwd, nbwd;  // by nodes
     int fwd_levels = 0, bwd_levels = 0;
     long long breakdowns = 0;
     bool ready = false;
@@ -436,6 +451,118 @@ void build_gather_map(lfg_ctx* c, const int32_t* emap) {
 
 // IC(0) of the lower triangle (precond.cpp:298-384), level sets for the device
 // sweeps, and the transposed strict lower part with rows in descending order.
+// Level parts of at most 64 positions (rows of one level are independent, so any split
+// keeps the order valid).
+std::vector<int> split_levels(const std::vector<int>& level_ptr) {
+    std::vector<int> parts(1, 0);
+    for (size_t l = 0; l + 1 < level_ptr.size(); ++l) {
+        const int b = level_ptr[l], e = level_ptr[l + 1];
+        const int pieces = (e - b + 63) / 64;
+        for (int k = 1; k <= pieces; ++k)
+            parts.push_back(b + static_cast<int>((static_cast<long long>(e - b) * k) / pieces));
+    }
+    return parts;
+}
+
+// The node form of the two sweeps (Ic0NodeSweep), when rows 2i, 2i+1 of the factor
+// have the same dependency nodes: row 2i's strict lower part is the pairs (2j, 2j+1)
+// of its nodes j in ascending order and row 2i+1's is the same followed by 2i (the
+// transposed columns alike: column 2i+1 holds the pairs (2j+1, 2j) in descending j,
+// column 2i the same followed by row 2i+1). Otherwise the row form serves.
+void build_ic0_nodes(lfg_ctx* c, const std::vector<int>& lrow, const std::vector<int>& lcol,
+                     const std::vector<double>& lval, const std::vector<int>& urow, const std::vector<int>& ucol,
+                     const std::vector<double>& uval, const std::vector<double>& inv_diag) {
+    Ic0Device& d = c->ic0;
+    d.nfwd.ready = d.nbwd.ready = false;
+    const int nn = c->n / 2;
+    // rows (forward) or columns (backward) f = first, s = second of node i: entries
+    // [b_f, e_f) must be pairs over nodes, [b_s, e_s) the same pairs plus one cross entry
+    auto node_form = [&](bool backward, const std::vector<int>& ptr, const std::vector<int>& col,
+                         const std::vector<double>& val, Ic0NodeSweepDevice& out) {
+        std::vector<int> deps_ptr(static_cast<size_t>(nn) + 1, 0), deps;
+        std::vector<double> vf, vs, cross(static_cast<size_t>(nn)), sf(static_cast<size_t>(nn)),
+            ss(static_cast<size_t>(nn));
+        for (int i = 0; i < nn; ++i) {
+            const int first = backward ? 2 * i + 1 : 2 * i, second = backward ? 2 * i : 2 * i + 1;
+            const int bf = ptr[first], ef = backward ? ptr[first + 1] : ptr[first + 1] - 1;
+            const int bs = ptr[second], es = backward ? ptr[second + 1] : ptr[second + 1] - 1;
+            if ((ef - bf) % 2 != 0 || es - bs != ef - bf + 1 || col[es - 1] != first) return false;
+            for (int k = 0; k < ef - bf; k += 2) {
+                const int c0 = col[bf + k], c1 = col[bf + k + 1];
+                // forward: (2j, 2j+1) ascending; backward: (2j+1, 2j) descending
+                const int j = backward ? c1 / 2 : c0 / 2;
+                const bool pair_ok = backward ? (c0 == 2 * j + 1 && c1 == 2 * j) : (c0 == 2 * j && c1 == 2 * j + 1);
+                if (!pair_ok || col[bs + k] != c0 || col[bs + k + 1] != c1) return false;
+                deps.push_back(j);
+                vf.push_back(val[bf + k]);
+                vf.push_back(val[bf + k + 1]);
+                vs.push_back(val[bs + k]);
+                vs.push_back(val[bs + k + 1]);
+            }
+            deps_ptr[i + 1] = static_cast<int>(deps.size());
+            cross[i] = val[es - 1];
+            sf[i] = inv_diag[first];
+            ss[i] = inv_diag[second];
+        }
+        // node levels (dependencies come before in the sweep's order), positions by level
+        std::vector<int> level(static_cast<size_t>(nn), 0);
+        int n_levels = 0;
+        for (int t = 0; t < nn; ++t) {
+            const int i = backward ? nn - 1 - t : t;
+            int lv = 0;
+            for (int k = deps_ptr[i]; k < deps_ptr[i + 1]; ++k) lv = std::max(lv, level[deps[k]] + 1);
+            level[i] = lv;
+            n_levels = std::max(n_levels, lv + 1);
+        }
+        std::vector<int> lptr(static_cast<size_t>(n_levels) + 1, 0), ord(static_cast<size_t>(nn));
+        for (int i = 0; i < nn; ++i) lptr[level[i] + 1]++;
+        for (int l = 0; l < n_levels; ++l) lptr[l + 1] += lptr[l];
+        {
+            std::vector<int> cur(lptr.begin(), lptr.end() - 1);
+            for (int i = 0; i < nn; ++i) ord[cur[level[i]]++] = i;
+        }
+        std::vector<int> pos(static_cast<size_t>(nn));
+        for (int q = 0; q < nn; ++q) pos[ord[q]] = q;
+        std::vector<int> qptr(static_cast<size_t>(nn) + 1, 0), key;
+        std::vector<double> qvf, qvs, qcross(static_cast<size_t>(nn)), qsf(static_cast<size_t>(nn)),
+            qss(static_cast<size_t>(nn));
+        int reach = 0;
+        for (int q = 0; q < nn; ++q) {
+            const int i = ord[q];
+            for (int k = deps_ptr[i]; k < deps_ptr[i + 1]; ++k) {
+                key.push_back(pos[deps[k]]);
+                reach = std::max(reach, q - pos[deps[k]]);
+                qvf.push_back(vf[2 * static_cast<size_t>(k)]);
+                qvf.push_back(vf[2 * static_cast<size_t>(k) + 1]);
+                qvs.push_back(vs[2 * static_cast<size_t>(k)]);
+                qvs.push_back(vs[2 * static_cast<size_t>(k) + 1]);
+            }
+            qptr[q + 1] = static_cast<int>(key.size());
+            qcross[q] = cross[i];
+            qsf[q] = sf[i];
+            qss[q] = ss[i];
+        }
+        const std::vector<int> parts = split_levels(lptr);
+        out.ord.upload(ord.data(), ord.size(), c->stream);
+        out.ptr.upload(qptr.data(), qptr.size(), c->stream);
+        out.key.upload(key.data(), key.size(), c->stream);
+        out.v_first.upload(qvf.data(), qvf.size(), c->stream);
+        out.v_second.upload(qvs.data(), qvs.size(), c->stream);
+        out.cross.upload(qcross.data(), qcross.size(), c->stream);
+        out.scale_first.upload(qsf.data(), qsf.size(), c->stream);
+        out.scale_second.upload(qss.data(), qss.size(), c->stream);
+        out.level_ptr.upload(parts.data(), parts.size(), c->stream);
+        out.levels = static_cast<int>(parts.size()) - 1;
+        out.reach = reach;
+        out.widest = 0;
+        for (int l = 0; l < out.levels; ++l) out.widest = std::max(out.widest, parts[l + 1] - parts[l]);
+        out.ready = true;
+        return true;
+    };
+    if (nn == 0 || !node_form(false, lrow, lcol, lval, d.nfwd) || !node_form(true, urow, ucol, uval, d.nbwd))
+        d.nfwd.ready = d.nbwd.ready = false;
+}
+
 void build_ic0(lfg_ctx* c) {
     const int n = c->n;
     std::vector<double> kc(static_cast<size_t>(c->nnz));
@@ -560,6 +687,44 @@ void build_ic0(lfg_ctx* c) {
     d.brows.upload(brows.data(), brows.size(), c->stream);
     d.fwd_levels = fwd_levels;
     d.bwd_levels = bwd_levels;
+    // level-ordered sweeps: positions in level order (rows of a level ascending, i.e. the
+    // level lists above), dependencies rewritten as positions, in the reference's order
+    auto level_order = [&](const std::vector<int>& rows_by_level, const std::vector<int>& level_ptr,
+                           const std::vector<int>& dptr, const std::vector<int>& dcol,
+                           const std::vector<double>& dval, bool strict_lower, Ic0SweepDevice& out) {
+        std::vector<int> pos(static_cast<size_t>(n));
+        for (int q = 0; q < n; ++q) pos[rows_by_level[q]] = q;
+        std::vector<int> ptr(static_cast<size_t>(n) + 1, 0), key;
+        std::vector<double> val, scale(static_cast<size_t>(n));
+        key.reserve(dcol.size());
+        val.reserve(dcol.size());
+        int reach = 0;
+        for (int q = 0; q < n; ++q) {
+            const int r = rows_by_level[q];
+            const int end = strict_lower ? dptr[r + 1] - 1 : dptr[r + 1];
+            for (int k = dptr[r]; k < end; ++k) {
+                key.push_back(pos[dcol[k]]);
+                val.push_back(dval[k]);
+                reach = std::max(reach, q - pos[dcol[k]]);
+            }
+            ptr[q + 1] = static_cast<int>(key.size());
+            scale[q] = inv_diag[r];
+        }
+        out.ord.upload(rows_by_level.data(), rows_by_level.size(), c->stream);
+        out.ptr.upload(ptr.data(), ptr.size(), c->stream);
+        out.key.upload(key.data(), key.size(), c->stream);
+        out.val.upload(val.data(), val.size(), c->stream);
+        out.scale.upload(scale.data(), scale.size(), c->stream);
+        const std::vector<int> parts = split_levels(level_ptr);
+        out.level_ptr.upload(parts.data(), parts.size(), c->stream);
+        out.levels = static_cast<int>(parts.size()) - 1;
+        out.reach = reach;
+        out.widest = 0;
+        for (int l = 0; l < out.levels; ++l) out.widest = std::max(out.widest, parts[l + 1] - parts[l]);
+    };
+    level_order(frows, fptr, lrow, lcol, lval, true, d.fwd);
+    level_order(brows, bptr, urow, ucol, uval, false, d.bwd);
+    build_ic0_nodes(c, lrow, lcol, lval, urow, ucol, uval, inv_diag);
     d.breakdowns = breakdowns;
     d.ready = true;
     sync(c);
@@ -567,9 +732,19 @@ void build_ic0(lfg_ctx* c) {
 
 lfg::Ic0View ic0_view(const lfg_ctx* c) {
     const Ic0Device& d = c->ic0;
+    const lfg::Ic0Sweep fwd{d.fwd.ord.p,       d.fwd.ptr.p,  d.fwd.key.p,   d.fwd.val.p,   d.fwd.scale.p,
+                            d.fwd.level_ptr.p, d.fwd.levels, d.fwd.reach, d.fwd.widest};
+    const lfg::Ic0Sweep bwd{d.bwd.ord.p,       d.bwd.ptr.p,  d.bwd.key.p,   d.bwd.val.p,   d.bwd.scale.p,
+                            d.bwd.level_ptr.p, d.bwd.levels, d.bwd.reach, d.bwd.widest};
+    auto node_view = [](const Ic0NodeSweepDevice& v) {
+        return lfg::Ic0NodeSweep{v.ready ? v.ord.p : nullptr, v.ptr.p, v.key.p, v.v_first.p, v.v_second.p,
+                                 v.cross.p, v.scale_first.p, v.scale_second.p, v.level_ptr.p, v.levels,
+                                 v.reach, v.widest};
+    };
+    const lfg::Ic0NodeSweep nfwd = node_view(d.nfwd), nbwd = node_view(d.nbwd);
     return lfg::Ic0View{c->n,        d.lrow.p,  d.lcol.p,       d.lval.p,  d.inv_diag.p, d.urow.p,
                         d.ucol.p,    d.uval.p,  d.fptr.p,       d.frows.p, d.fwd_levels, d.bptr.p,
-                        d.brows.p,   d.bwd_levels};
+                        d.brows.p,   d.bwd_levels, fwd, bwd, nfwd, nbwd, c->flags.p + 2};
 }
 
 void ensure_vectors(lfg_ctx* c) {
@@ -677,6 +852,10 @@ void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int repl
     if (b.precond == LFG_PRECOND_IC0) lfg::launch_pcg_ic0_step(c->stream, b, ic0_view(c));
     lfg::launch_pcg_direction(c->stream, b);
 }
+
+// The IC(0) sweeps bound every dependency wait; a factor whose rows never become
+// ready (no valid factor does) ends the sweep with this error instead of a hang.
+constexpr const char* kIc0Fault = "IC(0) triangular sweep exceeded its dependency wait bound";
 
 const char* status_text(int s) {
     switch (s) {
@@ -1139,6 +1318,7 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     rep.solve_ms = solve_ms;
     rep.status = st.status != LFG_OK ? st.status : (st.converged ? LFG_OK : LFG_ERR_NOT_CONVERGED);
     if (report) *report = rep;
+    if (st.status == LFG_ERR_HIP && kind == LFG_PRECOND_IC0) throw LfgError(st.status, kIc0Fault);
     if (st.status != LFG_OK) throw LfgError(st.status, status_text(st.status));
     if (!st.converged) throw LfgError(LFG_ERR_NOT_CONVERGED, "maximum iterations reached without convergence");
 }
@@ -1245,7 +1425,12 @@ int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, do
         setup_preconditioner(ctx, preconditioner, &degenerate);
         LFG_HIP(hipMemcpyAsync(ctx->r.p, in, ctx->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         if (preconditioner == LFG_PRECOND_IC0) {
+            LFG_HIP(hipMemsetAsync(ctx->flags.p + 2, 0, sizeof(int), ctx->stream));
             lfg::launch_ic0_apply(ctx->stream, ic0_view(ctx), ctx->r.p, ctx->z.p);
+            int fault = 0;
+            LFG_HIP(hipMemcpyAsync(&fault, ctx->flags.p + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            sync(ctx);
+            require(fault == 0, LFG_ERR_HIP, kIc0Fault);
         } else {
             lfg::launch_precond_apply(ctx->stream, preconditioner, ctx->n,
                                       preconditioner == LFG_PRECOND_JACOBI ? ctx->minv.p : ctx->factor.p, ctx->r.p,

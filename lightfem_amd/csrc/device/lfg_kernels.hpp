@@ -180,7 +180,52 @@ void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed);
 void launch_pcg_replace(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* vals64);
 // p = z + beta p.
 void launch_pcg_direction(hipStream_t s, const PcgBuffers& b);
-// IC(0): z = (L L^T)^-1 r by level-scheduled sweeps, then r.z and beta (one block).
+// One triangular sweep of IC(0) in level order: position p (rows sorted by level,
+// stable) holds row ord[p], its dependencies as positions key[ptr[p] .. ptr[p+1]) with
+// factor values val[...] in the reference's summation order, and the reciprocal pivot
+// scale[p]; level l covers positions [level_ptr[l], level_ptr[l+1]). reach = max over
+// rows of (position - smallest dependency position); widest = largest level.
+struct Ic0Sweep {
+    const int* ord;
+    const int* ptr;
+    const int* key;
+    const double* val;
+    const double* scale;
+    const int* level_ptr;
+    int levels;
+    int reach;
+    int widest;
+};
+
+// The same sweep by nodes, for factors whose rows 2i, 2i+1 share their dependency
+// nodes (every two-dof-per-node pattern): position q holds node ord[q], whose two rows
+// are computed by one lane, "first" then "second" (forward: 2i then 2i+1; backward:
+// 2i+1 then 2i), so the dependency between them never crosses lanes and the levels
+// are those of the node graph (half as many). Dependency k of position q is the node at
+// position key[k]; v_first[2k], v_first[2k+1] and v_second[2k], v_second[2k+1] are the
+// factor values its two dofs meet in the two rows, in the reference's order (forward:
+// dof 2j then 2j+1; backward: 2j+1 then 2j); cross[q] couples the second row to the
+// first and comes last; scale_first / scale_second are the reciprocal pivots.
+struct Ic0NodeSweep {
+    const int* ord;
+    const int* ptr;
+    const int* key;
+    const double* v_first;
+    const double* v_second;
+    const double* cross;
+    const double* scale_first;
+    const double* scale_second;
+    const int* level_ptr;
+    int levels;
+    int reach;
+    int widest;
+};
+
+// IC(0): z = (L L^T)^-1 r, then r.z and beta (one block). Factors whose level-ordered
+// dependencies reach back at most 4096 - 128 positions take the pipelined sweeps (the
+// values of the last positions stay in an LDS window, levels run back to back with
+// their data prefetched), by nodes where the factor allows it; others take the plain
+// level-scheduled sweeps.
 struct Ic0View {
     int n;
     const int* lrow;     // lower factor rows (CSR, diagonal last), strict part used
@@ -196,6 +241,9 @@ struct Ic0View {
     const int* bwd_level_ptr;
     const int* bwd_rows;
     int bwd_levels;
+    Ic0Sweep fwd, bwd;   // level-ordered forms of the two sweeps
+    Ic0NodeSweep nfwd, nbwd;  // by nodes (ord == nullptr when the factor has no node form)
+    int* fault;          // set when a dependency wait exceeds its bound (never for a valid factor)
 };
 void launch_ic0_apply(hipStream_t s, const Ic0View& f, const double* in, double* out);
 void launch_pcg_ic0_step(hipStream_t s, const PcgBuffers& b, const Ic0View& f);

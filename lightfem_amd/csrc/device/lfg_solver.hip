@@ -841,75 +841,6 @@ __global__ __launch_bounds__(kThreads) void pcg_direction_kernel(PcgBuffers b) {
     }
 }
 
-// ------------------------------------------------------------ IC(0)
-
-constexpr int kIcThreads = 1024;
-
-// Level-scheduled sweeps (precond.cpp:386-412) in one workgroup. Forward: rows of
-// a level gather over their strict lower row. Backward: the reference scatters
-// column updates row by row in descending order; here column c gathers the same
-// terms from the transposed factor, in descending row order, and scales by its
-// reciprocal pivot. Per-row arithmetic order equals the reference, so the result
-// is bit-identical.
-__device__ void ic0_sweeps(const Ic0View& f, const double* __restrict__ in, double* __restrict__ z) {
-    for (int i = threadIdx.x; i < f.n; i += kIcThreads) z[i] = in[i];
-    __syncthreads();
-    for (int level = 0; level < f.fwd_levels; ++level) {
-        const int begin = f.fwd_level_ptr[level], end = f.fwd_level_ptr[level + 1];
-        for (int idx = begin + threadIdx.x; idx < end; idx += kIcThreads) {
-            const int row = f.fwd_rows[idx];
-            double sum = z[row];
-            const int last = f.lrow[row + 1] - 1;
-            for (int k = f.lrow[row]; k < last; ++k) sum -= f.lval[k] * z[f.lcol[k]];
-            z[row] = sum * f.inv_diag[row];
-        }
-        __syncthreads();
-    }
-    for (int level = 0; level < f.bwd_levels; ++level) {
-        const int begin = f.bwd_level_ptr[level], end = f.bwd_level_ptr[level + 1];
-        for (int idx = begin + threadIdx.x; idx < end; idx += kIcThreads) {
-            const int c = f.bwd_rows[idx];
-            double sum = z[c];
-            for (int k = f.urow[c]; k < f.urow[c + 1]; ++k) sum -= f.uval[k] * z[f.ucol[k]];
-            z[c] = sum * f.inv_diag[c];
-        }
-        __syncthreads();
-    }
-}
-
-__global__ __launch_bounds__(kIcThreads) void ic0_apply_kernel(Ic0View f, const double* __restrict__ in,
-                                                               double* __restrict__ out) {
-    ic0_sweeps(f, in, out);
-}
-
-// z = IC0(r), r.z, then rho/beta (first call: rho = r.z, beta = 0).
-__global__ __launch_bounds__(kIcThreads) void pcg_ic0_step_kernel(PcgBuffers b, Ic0View f) {
-    if (b.st->done) return;
-    __shared__ double sh[kIcThreads / kWave];
-    ic0_sweeps(f, b.r, b.z);
-    double rz = 0.0;
-    for (int i = threadIdx.x; i < f.n; i += kIcThreads) rz += b.r[i] * b.z[i];
-    const double trz = block_sum<kIcThreads>(rz, sh);
-    if (threadIdx.x == 0) {
-        PcgState* st = b.st;
-        if (st->iter == 0) {
-            if (!(trz > 0.0) || !isfinite(trz)) {  // solver.cpp:579-581
-                st->status = LFG_ERR_NONFINITE;
-                st->done = 1;
-            } else {
-                st->rho = trz;
-                st->beta = 0.0;
-            }
-        } else if (!(trz > 0.0) || !isfinite(trz)) {  // :616-618
-            st->status = LFG_ERR_NONFINITE;
-            st->done = 1;
-        } else {
-            st->beta = trz / st->rho;
-            st->rho = trz;
-        }
-    }
-}
-
 // ||b - A x||^2 into out2; blocks own contiguous row ranges, the last block folds
 // the partials.
 template <int W>
@@ -1150,14 +1081,6 @@ void launch_pcg_direction(hipStream_t s, const PcgBuffers& b) {
         case LFG_PRECOND_BLOCK2: pcg_direction_kernel<LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(b); break;
         default: pcg_direction_kernel<LFG_PRECOND_IC0><<<grid, kThreads, 0, s>>>(b); break;
     }
-}
-
-void launch_ic0_apply(hipStream_t s, const Ic0View& f, const double* in, double* out) {
-    ic0_apply_kernel<<<1, kIcThreads, 0, s>>>(f, in, out);
-}
-
-void launch_pcg_ic0_step(hipStream_t s, const PcgBuffers& b, const Ic0View& f) {
-    pcg_ic0_step_kernel<<<1, kIcThreads, 0, s>>>(b, f);
 }
 
 void launch_residual_norm(hipStream_t s, const CsrView& a, const double* vals64, const double* x, const double* bvec,
