@@ -116,8 +116,8 @@ def cpu_baseline(n_mesh: int, young: np.ndarray, iterations_full: int, sample_it
 
 def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict:
     """The uneven batch: n_problems Q4 cantilevers (nx, ny ~ U{32..256}, 4 load cases each)
-    solved together by the batched Jacobi-PCG to ||r|| <= tol ||b|| per system, FP64 and
-    FP32 values with FP64 refinement."""
+    solved together by the batched Jacobi-PCG to ||r|| <= tol ||b|| per system, with FP64
+    values and mixed (each problem moves to compensated FP32 values at 1e-2 ||b||)."""
     from lightfem_amd import meshgen, pipeline
     t0 = time.perf_counter()
     specs = meshgen.c5_problems(n_problems)
@@ -128,7 +128,7 @@ def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict
     out = {"workload": f"c5-{n_problems}-problems-x4-cases-seed{meshgen.SEED_BATCH}", "problems": n_problems,
            "systems": n_problems * 4, "dofs": int(sum(2 * ix.n_nodes for ix in bs.indexed)),
            "dofs_padded": int(dev.n), "nnz": int(dev.nnz), "setup_s": round(setup_s, 2),
-           "stop_rule": f"recurrence ||r||/||b|| <= {tol:g} per system (mixed: FP64 ||b - K u|| <= {tol:g} ||b||)"}
+           "stop_rule": f"recurrence ||r||/||b|| <= {tol:g} per system"}
     print(f"bench: batch {n_problems} problems, {dev.n} dofs, {dev.nnz} nnz, setup {setup_s:.1f} s", file=sys.stderr,
           flush=True)
     for form in forms:
@@ -142,7 +142,8 @@ def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict
                      "iterations_max": int(rep["iterations"].max()),
                      "iterations_median": float(np.median(rep["iterations"])),
                      "iterations_total": int(rep["iterations"].sum()),
-                     "true_relative_residual_max": float(true_rel.max()), "passes": summary["passes"],
+                     "true_relative_residual_max": float(true_rel.max()),
+                     "problems_switched_to_fp32": summary["problems_switched"],
                      "graph_launches": summary["graph_launches"],
                      "launch_shape": {"block_threads": summary["block_threads"],
                                       "slices_per_wave": summary["slices_per_wave"],

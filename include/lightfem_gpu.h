@@ -201,33 +201,35 @@ int lfg_batch_set_loads(lfg_ctx* ctx, int32_t n_cases, const double* loads, int6
                         const int32_t* dofs, const double* values);
 
 typedef struct {
-    int32_t precision;        /* LFG_PRECISION_FP64, or MIXED: FP32 values with FP64 refinement passes */
+    int32_t precision;        /* LFG_PRECISION_FP64, or MIXED (see mixed_switch) */
     int32_t stop_rule;        /* LFG_STOP_* on each system's own ||b|| */
     int32_t check_every;      /* iterations per captured graph; 0 = 32 */
-    int32_t max_passes;       /* mixed: refinement passes including the first; 0 = 4 */
-    double tolerance;
-    int64_t max_iterations;   /* per system and pass */
-    double pass_reduction;    /* mixed: residual reduction asked of each later pass; 0 = 1e-6 */
     int32_t tune;             /* 1: choose the launch shape by timing candidates (cached per context) */
+    double tolerance;
+    int64_t max_iterations;   /* per system */
+    double mixed_switch;      /* mixed: a problem multiplies with FP64 values until every running case's
+                                 residual is at most mixed_switch * ||b||, then with FP32 off-diagonal
+                                 values and an FP64 diagonal carrying their rounding (vectors and scalars
+                                 stay FP64); 0 = 1e-2, negative = FP32 from the first iteration */
     int32_t block_threads;    /* launch shape when not tuning: 64-512 (0 = 256) */
     int32_t slices_per_wave;  /* slices each wavefront walks per launch (0 = 1) */
-    int32_t pad;
 } lfg_batch_options;
 
 typedef struct {
-    int32_t converged;        /* fp64: recurrence residual reached the target; mixed: FP64 residual did */
+    int32_t converged;        /* the recurrence residual reached the target */
     int32_t status;           /* lfg_status of the system */
-    int64_t iterations;       /* over all passes */
+    int64_t iterations;
+    int64_t switch_iteration; /* mixed: iteration after which FP32 values were used (-1: never) */
     double initial_residual;  /* ||b|| */
     double final_residual;    /* last recurrence residual */
     double true_residual;     /* ||b - K u|| in FP64 */
 } lfg_batch_system_report;
 
 typedef struct {
-    double solve_ms;          /* first init to the final residual, device time */
+    double solve_ms;          /* init to the final FP64 residual, device time */
     double tune_ms;
     int64_t graph_launches;
-    int32_t passes;
+    int32_t problems_switched; /* mixed: problems that moved to FP32 values */
     int32_t block_threads;
     int32_t slices_per_wave;
     int32_t systems_converged;

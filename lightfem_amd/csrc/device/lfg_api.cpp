@@ -107,7 +107,7 @@ struct lfg_ctx {
     int n_slices = 0, slice_max_width = 0;
     long long slice_blocks = 0;
     DevArray<int> slice_ptr, slice_col;
-    DevArray<double> slices64;
+    DevArray<double> slices64, slice_diag;  // slice_diag: compensated FP64 diagonal of slices32
     DevArray<float> slices32;
     // node-stencil copies of the constrained values (SpMV variants 8 and 9), when every
     // block column lies at one of a few fixed node offsets (structured meshes numbered
@@ -149,10 +149,9 @@ struct lfg_ctx {
     int batch_tuned_for = -1;  // precision the cached launch shape was timed with
     lfg::BatchShape batch_shape{256, 1};
     double batch_tune_ms = 0.0;
-    DevArray<int> batch_slice_problem, batch_pslice_ptr, batch_problem_done;
+    DevArray<int> batch_slice_problem, batch_pslice_ptr, batch_problem_done, batch_phase32;
     DevArray<unsigned int> batch_tickets;
-    DevArray<double> batch_b, batch_rhs, batch_x, batch_r, batch_p, batch_q, batch_xacc, batch_res, batch_minv,
-        batch_partials, batch_tmp;
+    DevArray<double> batch_b, batch_x, batch_r, batch_p, batch_q, batch_res, batch_minv, batch_partials, batch_tmp;
     DevArray<lfg::BatchSystem> batch_sys;
     DevArray<lfg::BatchGlobal> batch_global;
     lfg::BatchGlobal* h_batch_global = nullptr;  // pinned, 2 slots
@@ -188,7 +187,8 @@ lfg::MeshView mesh_view(const lfg_ctx* c) {
 lfg::CsrView csr_view(const lfg_ctx* c) { return lfg::CsrView{c->n, c->nnz, c->row_ptr.p, c->col.p, c->max_row}; }
 
 lfg::SliceView slice_view(const lfg_ctx* c) {
-    return lfg::SliceView{c->n_slices, c->slice_blocks, c->slice_ptr.p, c->slice_col.p, c->slice_max_width};
+    return lfg::SliceView{c->n_slices,  c->slice_blocks,    c->slice_ptr.p,
+                          c->slice_col.p, c->slice_max_width, c->slice_diag.p};
 }
 
 constexpr int kSliceVariant = 6;  // node-slice SpMV, separate direction kernel
@@ -420,7 +420,9 @@ void ensure_slices(lfg_ctx* c, bool need64, bool need32) {
     }
     if (need32 && !c->slices32_valid) {
         c->slices32.alloc(4 * static_cast<size_t>(c->slice_blocks));
-        lfg::launch_pack_slices(c->stream, csr_view(c), slice_view(c), c->Kc.p, nullptr, c->slices32.p, nullptr);
+        c->slice_diag.alloc(2 * static_cast<size_t>(c->n));
+        lfg::launch_pack_slices(c->stream, csr_view(c), slice_view(c), c->Kc.p, nullptr, c->slices32.p, nullptr,
+                                c->slice_diag.p);
         c->slices32_valid = true;
     }
 }
@@ -1583,6 +1585,7 @@ int lfg_batch_set_problems(lfg_ctx* ctx, int32_t n_problems, const int32_t* node
         ctx->batch_slice_problem.upload(slice_problem.data(), slice_problem.size(), ctx->stream);
         ctx->batch_pslice_ptr.upload(pslice.data(), pslice.size(), ctx->stream);
         ctx->batch_problem_done.alloc(static_cast<size_t>(n_problems));
+        ctx->batch_phase32.alloc(static_cast<size_t>(n_problems));
         ctx->batch_tickets.alloc(32 * static_cast<size_t>(n_problems));
         LFG_HIP(hipMemsetAsync(ctx->batch_tickets.p, 0, ctx->batch_tickets.n * sizeof(unsigned int), ctx->stream));
         ctx->batch_partials.alloc(8 * static_cast<size_t>(n_slices));
@@ -1630,8 +1633,7 @@ int lfg_batch_set_loads(lfg_ctx* ctx, int32_t n_cases, const double* loads, int6
         ensure_vectors(ctx);
         LFG_HIP(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), ctx->stream));
         lfg::launch_batch_jacobi(ctx->stream, csr_view(ctx), ctx->Kc.p, ctx->batch_minv.p, ctx->flags.p);
-        for (DevArray<double>* v : {&ctx->batch_rhs, &ctx->batch_x, &ctx->batch_r, &ctx->batch_p, &ctx->batch_q,
-                                    &ctx->batch_xacc, &ctx->batch_res})
+        for (DevArray<double>* v : {&ctx->batch_x, &ctx->batch_r, &ctx->batch_p, &ctx->batch_q, &ctx->batch_res})
             v->alloc(vec);
         ctx->batch_tmp.alloc(n);
         int bad = 0;
@@ -1666,13 +1668,16 @@ lfg::BatchView batch_view(lfg_ctx* c, const lfg_batch_options* opt) {
     bv.r = c->batch_r.p;
     bv.p = c->batch_p.p;
     bv.q = c->batch_q.p;
-    bv.b = c->batch_rhs.p;
+    bv.b = c->batch_b.p;
     bv.minv = c->batch_minv.p;
     bv.sys = c->batch_sys.p;
     bv.problem_done = c->batch_problem_done.p;
     bv.tickets = c->batch_tickets.p;
     bv.partials = c->batch_partials.p;
     bv.global = c->batch_global.p;
+    const bool mixed = opt->precision == LFG_PRECISION_MIXED;
+    bv.problem_phase32 = mixed ? c->batch_phase32.p : nullptr;
+    bv.switch_tau = opt->mixed_switch != 0.0 ? opt->mixed_switch : 1.0e-2;
     bv.tol = opt->tolerance;
     bv.stop_rule = opt->stop_rule;
     bv.max_iter = opt->max_iterations;
@@ -1712,17 +1717,16 @@ void batch_autotune(lfg_ctx* c, const lfg::BatchView& bv, bool mixed) {
     c->batch_tuned_for = mixed ? LFG_PRECISION_MIXED : LFG_PRECISION_FP64;
 }
 
-// One PCG pass over all systems with right-hand side batch_rhs, solution in batch_x.
-long long batch_pass(lfg_ctx* c, const lfg_batch_options* opt, const lfg::BatchView& bv, bool mixed,
-                     int use_pass_target) {
+// PCG over all systems (right-hand sides batch_b, solutions in batch_x); returns the
+// number of graph launches.
+long long batch_solve(lfg_ctx* c, const lfg_batch_options* opt, const lfg::BatchView& bv, bool mixed) {
     const lfg::SliceView v = slice_view(c);
     const lfg::BatchShape shape = c->batch_shape;
     const int per_graph = opt->check_every > 0 ? opt->check_every : 32;
     LFG_HIP(hipMemsetAsync(c->batch_global.p, 0, sizeof(lfg::BatchGlobal), c->stream));
-    lfg::launch_batch_init(c->stream, bv, shape, use_pass_target);
+    lfg::launch_batch_init(c->stream, bv, shape);
     auto iteration = [&] {
-        lfg::launch_batch_spmv(c->stream, bv, shape, v, mixed ? nullptr : c->slices64.p, mixed ? c->slices32.p : nullptr,
-                               0);
+        lfg::launch_batch_spmv(c->stream, bv, shape, v, c->slices64.p, mixed ? c->slices32.p : nullptr, 0);
         lfg::launch_batch_update(c->stream, bv, shape);
         lfg::launch_batch_direction(c->stream, bv, shape);
     };
@@ -1772,6 +1776,8 @@ int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg
         require(opt != nullptr, LFG_ERR_ARG, "missing batch options");
         require(opt->tolerance > 0.0 && opt->tolerance < 1.0, LFG_ERR_ARG, "solver tolerance must satisfy 0 < tolerance < 1");
         require(opt->max_iterations > 0, LFG_ERR_ARG, "max_iterations must be greater than zero");
+        require(opt->mixed_switch < 1.0, LFG_ERR_ARG,
+                "mixed_switch must be below 1 (negative: FP32 values from the first iteration)");
         const bool mixed = opt->precision == LFG_PRECISION_MIXED;
         LFG_HIP(hipSetDevice(ctx->device));
         ensure_slices(ctx, true, mixed);
@@ -1785,37 +1791,14 @@ int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg
                     "block_threads must be 64-512 in steps of 64, slices_per_wave positive");
             ctx->batch_shape = lfg::BatchShape{t, w};
         }
-        const long long vec = static_cast<long long>(ctx->batch_b.n);
-        const int max_passes = mixed ? (opt->max_passes > 0 ? opt->max_passes : 4) : 1;
-        const double reduction = opt->pass_reduction > 0.0 ? opt->pass_reduction : 1.0e-6;
-        lfg::BatchView orig = bv;  // residuals against the original right-hand sides
-        orig.b = ctx->batch_b.p;
         const lfg::SliceView v = slice_view(ctx);
         const size_t n_sys = static_cast<size_t>(ctx->batch_problems) * ctx->batch_cases;
         std::vector<lfg::BatchSystem> sys(n_sys);
 
         LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-        LFG_HIP(hipMemcpyAsync(ctx->batch_rhs.p, ctx->batch_b.p, vec * sizeof(double), hipMemcpyDeviceToDevice,
-                               ctx->stream));
-        LFG_HIP(hipMemsetAsync(ctx->batch_xacc.p, 0, vec * sizeof(double), ctx->stream));
-        long long launches = 0;
-        int passes = 0;
-        for (; passes < max_passes;) {
-            launches += batch_pass(ctx, opt, bv, mixed, passes > 0 ? 1 : 0);
-            lfg::launch_batch_accumulate(ctx->stream, vec, ctx->batch_x.p, ctx->batch_xacc.p);
-            ++passes;
-            // FP64 residual of the accumulated solution; in mixed mode it also decides the next pass
-            lfg::launch_batch_residual(ctx->stream, orig, ctx->batch_shape, v, ctx->slices64.p, ctx->batch_xacc.p,
-                                       ctx->batch_res.p, mixed ? 1 : 0, reduction);
-            if (!mixed) break;
-            LFG_HIP(hipMemcpyAsync(sys.data(), ctx->batch_sys.p, n_sys * sizeof(lfg::BatchSystem),
-                                   hipMemcpyDeviceToHost, ctx->stream));
-            sync(ctx);
-            bool any = false;
-            for (const lfg::BatchSystem& st : sys) any = any || (st.active && st.status == LFG_OK);
-            if (!any || passes == max_passes) break;
-            lfg::launch_batch_select_rhs(ctx->stream, bv, ctx->batch_shape, ctx->batch_res.p, ctx->batch_rhs.p);
-        }
+        const long long launches = batch_solve(ctx, opt, bv, mixed);
+        // FP64 residual of every system's solution
+        lfg::launch_batch_residual(ctx->stream, bv, ctx->batch_shape, v, ctx->slices64.p, ctx->batch_res.p);
         LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
         LFG_HIP(hipMemcpyAsync(sys.data(), ctx->batch_sys.p, n_sys * sizeof(lfg::BatchSystem), hipMemcpyDeviceToHost,
                                ctx->stream));
@@ -1823,38 +1806,39 @@ int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg
         const double solve_ms = elapsed(ctx->ev_start, ctx->ev_stop);
         if (u_out) {
             for (int c = 0; c < ctx->batch_cases; ++c) {
-                lfg::launch_batch_gather_case(ctx->stream, ctx->n, ctx->batch_cases, c, ctx->batch_xacc.p,
+                lfg::launch_batch_gather_case(ctx->stream, ctx->n, ctx->batch_cases, c, ctx->batch_x.p,
                                               ctx->batch_tmp.p);
                 ctx->batch_tmp.download(u_out + static_cast<size_t>(c) * ctx->n, static_cast<size_t>(ctx->n),
                                         ctx->stream);
             }
             sync(ctx);
         }
-        int converged = 0;
+        int converged = 0, switched = 0;
         long long max_iter = 0;
         int first_status = LFG_OK;
         for (size_t k = 0; k < n_sys; ++k) {
             const lfg::BatchSystem& st = sys[k];
-            const double true_res = std::sqrt(std::max(0.0, st.true_rr));
-            const int ok = mixed ? (true_res <= st.final_target ? 1 : 0) : st.converged;
+            const int ok = st.converged;
             converged += ok;
-            max_iter = std::max(max_iter, st.iterations_total);
+            if (k % static_cast<size_t>(ctx->batch_cases) == 0 && st.switch_iter >= 0) ++switched;
+            max_iter = std::max(max_iter, st.iter);
             if (first_status == LFG_OK && st.status != LFG_OK) first_status = st.status;
             if (reports) {
                 lfg_batch_system_report& r = reports[k];
                 r.converged = ok;
                 r.status = st.status != LFG_OK ? st.status : (ok ? LFG_OK : LFG_ERR_NOT_CONVERGED);
-                r.iterations = st.iterations_total;
-                r.initial_residual = st.bnorm;
+                r.iterations = st.iter;
+                r.switch_iteration = st.switch_iter;
+                r.initial_residual = st.rnorm0;
                 r.final_residual = std::sqrt(std::max(0.0, st.rr));
-                r.true_residual = true_res;
+                r.true_residual = std::sqrt(std::max(0.0, st.true_rr));
             }
         }
         if (summary) {
             summary->solve_ms = solve_ms;
             summary->tune_ms = ctx->batch_tune_ms;
             summary->graph_launches = launches;
-            summary->passes = passes;
+            summary->problems_switched = switched;
             summary->block_threads = ctx->batch_shape.block_threads;
             summary->slices_per_wave = ctx->batch_shape.slices_per_wave;
             summary->systems_converged = converged;

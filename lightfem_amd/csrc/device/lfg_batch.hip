@@ -11,6 +11,12 @@
 // A problem whose cases have all stopped is skipped by every later kernel, and every
 // kernel returns at once when all problems have stopped, so a captured graph of many
 // iterations costs almost nothing after convergence.
+//
+// Mixed precision switches per problem: a problem multiplies with FP64 values until
+// every running case's recurrence residual has fallen to switch_tau of its initial
+// value, then with the FP32 off-diagonal blocks plus the FP64 compensated diagonal
+// block (the rounding of each row's FP32 entries subtracted from its diagonal, so
+// rigid translations still multiply to zero). Vectors and scalars stay FP64.
 #include "lfg_kernels.hpp"
 
 #include "lightfem_gpu.h"
@@ -87,10 +93,8 @@ __device__ __forceinline__ void close_problem_if_done(const BatchView& bv, int p
 // ------------------------------------------------------------ init
 
 // x = 0, r = b, p = z = M^-1 r; per system ||r0||, target, rho (solver.cpp:560-583 per case).
-// With pass_targets (refinement passes) a system stops at max(its final target,
-// the pass target the residual kernel set).
 template <int NC>
-__global__ __launch_bounds__(512) void batch_init_kernel(BatchView bv, int use_pass_target) {
+__global__ __launch_bounds__(512) void batch_init_kernel(BatchView bv) {
     const SliceRange range = block_slices(bv.n_slices);
     const dpair* __restrict__ b2 = reinterpret_cast<const dpair*>(bv.b);
     const dpair* __restrict__ m2 = reinterpret_cast<const dpair*>(bv.minv);
@@ -126,18 +130,11 @@ __global__ __launch_bounds__(512) void batch_init_kernel(BatchView bv, int use_p
                 BatchSystem* st = bv.sys + p * NC + c;
                 const double rr = tot[2 * c], rz = tot[2 * c + 1];
                 const double rnorm0 = sqrt(max0(rr));
-                double target;
-                if (use_pass_target) {
-                    target = st->pass_target;
-                } else {
-                    const double base = bv.stop_rule == LFG_STOP_REFERENCE ? ((1.0 < rnorm0) ? rnorm0 : 1.0) : rnorm0;
-                    target = bv.tol * base;
-                    st->final_target = target;
-                    st->bnorm = rnorm0;
-                    st->iterations_total = 0;
-                }
+                const double base = bv.stop_rule == LFG_STOP_REFERENCE ? ((1.0 < rnorm0) ? rnorm0 : 1.0) : rnorm0;
+                const double target = bv.tol * base;
                 st->rnorm0 = rnorm0;
                 st->target = target;
+                st->switch_iter = (bv.problem_phase32 && bv.switch_tau < 0.0) ? 0 : -1;
                 st->rr = rr;
                 st->rho = rz;
                 st->alpha = 0.0;
@@ -155,6 +152,7 @@ __global__ __launch_bounds__(512) void batch_init_kernel(BatchView bv, int use_p
                 }
             }
             bv.problem_done[p] = 0;
+            if (bv.problem_phase32) bv.problem_phase32[p] = bv.switch_tau < 0.0 ? 1 : 0;
             close_problem_if_done(bv, p);
         }
     }
@@ -219,6 +217,19 @@ __device__ __forceinline__ void slice_multiply(const BatchView& bv, const SliceV
     }
     dpair* q2 = reinterpret_cast<dpair*>(bv.q) + static_cast<long long>(node) * NC;
     const dpair* pn = p2 + static_cast<long long>(node) * NC;
+    if (sizeof(VT) == sizeof(float)) {
+        // FP32 planes hold a zero diagonal block: add the FP64 compensated one
+        const dpair du = reinterpret_cast<const dpair*>(v.diag32)[2 * node];
+        const dpair dl = reinterpret_cast<const dpair*>(v.diag32)[2 * node + 1];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const dpair x = pn[c];
+            a0[c] += du.x * x.x;
+            a0[c] += du.y * x.y;
+            a1[c] += dl.x * x.x;
+            a1[c] += dl.y * x.y;
+        }
+    }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
         const dpair qv = {a0[c], a1[c]};
@@ -228,16 +239,19 @@ __device__ __forceinline__ void slice_multiply(const BatchView& bv, const SliceV
     }
 }
 
-// q = K p, then per system alpha = rho / p.q (solver.cpp:585-595).
-template <int NC, typename VT, int kW>
-__global__ __launch_bounds__(512) void batch_spmv_kernel(BatchView bv, SliceView v, const VT* __restrict__ planes) {
+// q = K p, then per system alpha = rho / p.q (solver.cpp:585-595). kMixed: each
+// problem's phase picks its values (FP64 planes, or FP32 planes + compensated diagonal).
+template <int NC, int kW, bool kMixed>
+__global__ __launch_bounds__(512) void batch_spmv_kernel(BatchView bv, SliceView v, const double* __restrict__ planes64,
+                                                         const float* __restrict__ planes32) {
     if (bv.global->done_all) return;
     const SliceRange range = block_slices(bv.n_slices);
     for (int s = range.first; s < range.stop; s += range.step) {
         const int p = bv.slice_problem[s];
         if (bv.problem_done[p]) continue;
         double pq[NC];
-        slice_multiply<NC, VT, kW>(bv, v, planes, s, pq);
+        if (kMixed && bv.problem_phase32[p]) slice_multiply<NC, float, kW>(bv, v, planes32, s, pq);
+        else slice_multiply<NC, double, kW>(bv, v, planes64, s, pq);
         double vals[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) vals[c] = wave_sum(pq[c]);
@@ -324,7 +338,6 @@ __global__ __launch_bounds__(512) void batch_update_kernel(BatchView bv) {
                 BatchSystem* st = bv.sys + p * NC + c;
                 if (st->done) continue;
                 st->iter += 1;
-                st->iterations_total += 1;
                 st->rr = tot[2 * c];
                 const double rz = tot[2 * c + 1];
                 if (sqrt(max0(st->rr)) <= st->target) {
@@ -337,6 +350,17 @@ __global__ __launch_bounds__(512) void batch_update_kernel(BatchView bv) {
                     st->beta = rz / st->rho;
                     st->rho = rz;
                     if (st->iter >= bv.max_iter) st->done = 1;
+                }
+            }
+            if (bv.problem_phase32 && !bv.problem_phase32[p]) {
+                bool reached = true;
+                for (int c = 0; c < NC; ++c) {
+                    const BatchSystem* st = bv.sys + p * NC + c;
+                    reached = reached && (st->done || sqrt(max0(st->rr)) <= bv.switch_tau * st->rnorm0);
+                }
+                if (reached) {
+                    bv.problem_phase32[p] = 1;
+                    for (int c = 0; c < NC; ++c) bv.sys[p * NC + c].switch_iter = bv.sys[p * NC + c].iter;
                 }
             }
             close_problem_if_done(bv, p);
@@ -373,23 +397,19 @@ __global__ __launch_bounds__(512) void batch_direction_kernel(BatchView bv) {
     }
 }
 
-// ------------------------------------------------------------ FP64 residual, refinement
+// ------------------------------------------------------------ FP64 residual
 
-// res = b - K x with FP64 values (x = the accumulated solution), per system ||res||.
-// With set_pass (refinement) each system's next right-hand side becomes res when it
-// is still above its final target (else zero) and its pass target max(final target,
-// reduction * ||res||).
+// res = b - K x with FP64 values, per system ||res||^2 into true_rr.
 template <int NC, int kW>
 __global__ __launch_bounds__(512) void batch_residual_kernel(BatchView bv, SliceView v, const double* __restrict__ planes,
-                                                            const double* __restrict__ xacc, double* __restrict__ res,
-                                                            int set_pass, double reduction) {
+                                                            double* __restrict__ res) {
     const SliceRange range = block_slices(bv.n_slices);
     for (int s = range.first; s < range.stop; s += range.step) {
         const int p = bv.slice_problem[s];
         const int node = s * kWave + lane_id();
-        // K xacc through the same multiply as the solver, with xacc as the multiplied vector
+        // K x through the same multiply as the solver, with x as the multiplied vector
         BatchView tmp = bv;
-        tmp.p = const_cast<double*>(xacc);
+        tmp.p = bv.x;
         tmp.q = res;
         double unused[NC];
         slice_multiply<NC, double, kW>(tmp, v, planes, s, unused);
@@ -405,42 +425,9 @@ __global__ __launch_bounds__(512) void batch_residual_kernel(BatchView bv, Slice
         }
         double tot[NC];
         if (!finish_problem<NC>(bv, p, s, vals, tot)) continue;
-        if (lane_id() == 0) {
-            for (int c = 0; c < NC; ++c) {
-                BatchSystem* st = bv.sys + p * NC + c;
-                st->true_rr = tot[c];
-                if (set_pass) {
-                    const double rn = sqrt(max0(tot[c]));
-                    st->active = rn > st->final_target ? 1 : 0;
-                    const double t = reduction * rn;
-                    st->pass_target = st->final_target > t ? st->final_target : t;
-                }
-            }
-        }
+        if (lane_id() == 0)
+            for (int c = 0; c < NC; ++c) bv.sys[p * NC + c].true_rr = tot[c];
     }
-}
-
-// Next refinement right-hand side: rhs = res for systems still above target, else 0.
-template <int NC>
-__global__ __launch_bounds__(512) void batch_select_rhs_kernel(BatchView bv, const double* __restrict__ res,
-                                                              double* __restrict__ rhs) {
-    const SliceRange range = block_slices(bv.n_slices);
-    for (int s = range.first; s < range.stop; s += range.step) {
-        const int p = bv.slice_problem[s];
-        const int node = s * kWave + lane_id();
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const bool on = bv.sys[p * NC + c].active != 0;
-            const long long at = (static_cast<long long>(node) * NC + c) * 2;
-            rhs[at] = on ? res[at] : 0.0;
-            rhs[at + 1] = on ? res[at + 1] : 0.0;
-        }
-    }
-}
-
-__global__ void batch_accumulate_kernel(long long n, const double* __restrict__ x, double* __restrict__ acc) {
-    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) acc[i] += x[i];
 }
 
 // Jacobi inverse of the constrained matrix; 0 on the rows of padding nodes (in no
@@ -507,30 +494,32 @@ unsigned int batch_grid(const BatchShape& shape, int n_slices) {
         default: { constexpr int NC = 4; body; } break;        \
     }
 
-void launch_batch_init(hipStream_t s, const BatchView& bv, const BatchShape& shape, int use_pass_target) {
+void launch_batch_init(hipStream_t s, const BatchView& bv, const BatchShape& shape) {
     const unsigned int grid = batch_grid(shape, bv.n_slices);
-    LFG_BATCH_NC(bv.n_cases, (batch_init_kernel<NC><<<grid, shape.block_threads, 0, s>>>(bv, use_pass_target)));
+    LFG_BATCH_NC(bv.n_cases, (batch_init_kernel<NC><<<grid, shape.block_threads, 0, s>>>(bv)));
 }
 
-template <int NC, typename VT>
-static void spmv_nc(hipStream_t s, const BatchView& bv, const SliceView& v, const VT* planes, unsigned int grid,
-                    int threads, bool plain) {
-    if (v.max_width <= 9) {
-        if (plain) batch_multiply_kernel<NC, VT, 9><<<grid, threads, 0, s>>>(bv, v, planes);
-        else batch_spmv_kernel<NC, VT, 9><<<grid, threads, 0, s>>>(bv, v, planes);
+template <int NC, int kW>
+static void spmv_nc(hipStream_t s, const BatchView& bv, const SliceView& v, const double* planes64,
+                    const float* planes32, unsigned int grid, int threads, bool plain) {
+    if (plain) {
+        if (planes64) batch_multiply_kernel<NC, double, kW><<<grid, threads, 0, s>>>(bv, v, planes64);
+        else batch_multiply_kernel<NC, float, kW><<<grid, threads, 0, s>>>(bv, v, planes32);
+    } else if (planes32) {
+        batch_spmv_kernel<NC, kW, true><<<grid, threads, 0, s>>>(bv, v, planes64, planes32);
     } else {
-        if (plain) batch_multiply_kernel<NC, VT, 0><<<grid, threads, 0, s>>>(bv, v, planes);
-        else batch_spmv_kernel<NC, VT, 0><<<grid, threads, 0, s>>>(bv, v, planes);
+        batch_spmv_kernel<NC, kW, false><<<grid, threads, 0, s>>>(bv, v, planes64, nullptr);
     }
 }
 
 void launch_batch_spmv(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
                        const double* planes64, const float* planes32, int plain) {
     const unsigned int grid = batch_grid(shape, bv.n_slices);
-    if (planes32) {
-        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, float>(s, bv, v, planes32, grid, shape.block_threads, plain != 0)));
+    const int t = shape.block_threads;
+    if (v.max_width <= 9) {
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 9>(s, bv, v, planes64, planes32, grid, t, plain != 0)));
     } else {
-        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, double>(s, bv, v, planes64, grid, shape.block_threads, plain != 0)));
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 0>(s, bv, v, planes64, planes32, grid, t, plain != 0)));
     }
 }
 
@@ -545,26 +534,15 @@ void launch_batch_direction(hipStream_t s, const BatchView& bv, const BatchShape
 }
 
 void launch_batch_residual(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
-                           const double* planes64, const double* xacc, double* res, int set_pass, double reduction) {
+                           const double* planes64, double* res) {
     const unsigned int grid = batch_grid(shape, bv.n_slices);
     if (v.max_width <= 9) {
-        LFG_BATCH_NC(bv.n_cases, (batch_residual_kernel<NC, 9><<<grid, shape.block_threads, 0, s>>>(
-                                     bv, v, planes64, xacc, res, set_pass, reduction)));
+        LFG_BATCH_NC(bv.n_cases,
+                     (batch_residual_kernel<NC, 9><<<grid, shape.block_threads, 0, s>>>(bv, v, planes64, res)));
     } else {
-        LFG_BATCH_NC(bv.n_cases, (batch_residual_kernel<NC, 0><<<grid, shape.block_threads, 0, s>>>(
-                                     bv, v, planes64, xacc, res, set_pass, reduction)));
+        LFG_BATCH_NC(bv.n_cases,
+                     (batch_residual_kernel<NC, 0><<<grid, shape.block_threads, 0, s>>>(bv, v, planes64, res)));
     }
-}
-
-void launch_batch_select_rhs(hipStream_t s, const BatchView& bv, const BatchShape& shape, const double* res,
-                             double* rhs) {
-    const unsigned int grid = batch_grid(shape, bv.n_slices);
-    LFG_BATCH_NC(bv.n_cases, (batch_select_rhs_kernel<NC><<<grid, shape.block_threads, 0, s>>>(bv, res, rhs)));
-}
-
-void launch_batch_accumulate(hipStream_t s, long long n, const double* x, double* acc) {
-    if (n <= 0) return;
-    batch_accumulate_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, x, acc);
 }
 
 void launch_batch_jacobi(hipStream_t s, const CsrView& a, const double* values, double* minv, int* bad) {

@@ -37,12 +37,17 @@ struct CsrView {
 // (K[2i+1][2j], K[2i+1][2j+1]) in the second; padding blocks are zero and point
 // at their own node. One int32 column per four values, and every load of a
 // wavefront is 64 consecutive blocks.
+// The FP32 copy of the planes holds the off-diagonal blocks only (the diagonal block
+// is zero there) and diag32 the diagonal blocks in FP64 (upper pair at 2i, lower at
+// 2i + 1), each minus the rounding error of its node's FP32 off-diagonal blocks, so
+// every row keeps its FP64 sum (see StencilView).
 struct SliceView {
     int n_slices;
     long long blocks;  // padded blocks, ptr[n_slices]
     const int* ptr;    // n_slices + 1 block offsets (multiples of 64)
     const int* col;    // node column per block
     int max_width;
+    const double* diag32;  // compensated diagonal of the FP32 copy
 };
 
 // Node-stencil layout of a node-blocked matrix whose blocks lie at a few fixed node
@@ -137,10 +142,11 @@ void launch_pcg_init(hipStream_t s, const PcgBuffers& b, int stop_rule, double t
 // both, the kernel reads the FP32 values once the state is in the FP32 phase.
 void launch_pcg_spmv(hipStream_t s, const PcgBuffers& b, const CsrView& a, const double* vals64, const float* vals32,
                      int variant);
-// Packs the CSR `values` into the slice planes (out64: 4 * blocks doubles, out32:
-// 4 * blocks floats, either may be null) and, with col_out, the node columns.
+// Packs the CSR `values` into the slice planes (out64: 4 * blocks doubles; out32:
+// 4 * blocks floats, off-diagonal blocks only, with diag_out: 4 doubles per node, the
+// compensated diagonal; any may be null) and, with col_out, the node columns.
 void launch_pack_slices(hipStream_t s, const CsrView& a, const SliceView& v, const double* values, double* out64,
-                        float* out32, int* col_out);
+                        float* out32, int* col_out, double* diag_out = nullptr);
 // q = A p on the slice layout; finishes p.q and alpha like launch_pcg_spmv. Each
 // row is summed in column order, as the CSR row loop does.
 void launch_pcg_spmv_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
@@ -251,14 +257,11 @@ void launch_pcg_ic0_step(hipStream_t s, const PcgBuffers& b, const Ic0View& f);
 // NC load cases per problem stored per node ([node][case][2]).
 struct BatchSystem {      // one (problem, case)
     double rho, alpha, beta, rr, target;
-    double rnorm0;        // ||r0|| of the current pass
-    double final_target;  // stop threshold of the whole solve (first pass: tol on ||b||)
-    double pass_target;   // refinement: this pass's threshold
-    double bnorm;         // ||b|| of the original right-hand side
+    double rnorm0;        // ||r0|| = ||b|| (x starts at zero)
     double true_rr;       // ||b - K x||^2 in FP64 from the residual kernel
-    long long iter;       // iterations of the current pass
-    long long iterations_total;
-    int status, converged, done, active;
+    long long iter;
+    long long switch_iter;  // mixed: iteration at which the problem moved to FP32 values (-1: never)
+    int status, converged, done, pad;
 };
 struct BatchGlobal {
     unsigned int problems_done;
@@ -280,6 +283,8 @@ struct BatchView {
     unsigned int* tickets;     // one per problem, 32 apart
     double* partials;          // slices x 2 * n_cases
     BatchGlobal* global;
+    int* problem_phase32;      // mixed: 1 once problem p multiplies with FP32 values; null for FP64
+    double switch_tau;         // mixed: relative residual that moves a problem to FP32 (< 0: from the start)
     double tol;
     int stop_rule;
     long long max_iter;
@@ -289,17 +294,17 @@ struct BatchShape {
     int slices_per_wave;  // slices each wavefront walks per launch
 };
 unsigned int batch_grid(const BatchShape& shape, int n_slices);
-void launch_batch_init(hipStream_t s, const BatchView& bv, const BatchShape& shape, int use_pass_target);
-// plain = 1: the multiply alone (autotuning), no reductions or state.
+void launch_batch_init(hipStream_t s, const BatchView& bv, const BatchShape& shape);
+// plain = 1: the multiply alone (autotuning) with one of planes64 / planes32, no
+// reductions or state. Otherwise planes32 non-null selects the mixed kernel (both
+// plane sets, per-problem phase).
 void launch_batch_spmv(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
                        const double* planes64, const float* planes32, int plain);
 void launch_batch_update(hipStream_t s, const BatchView& bv, const BatchShape& shape);
 void launch_batch_direction(hipStream_t s, const BatchView& bv, const BatchShape& shape);
+// res = b - K x (FP64 values), ||res||^2 per system into true_rr.
 void launch_batch_residual(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
-                           const double* planes64, const double* xacc, double* res, int set_pass, double reduction);
-void launch_batch_select_rhs(hipStream_t s, const BatchView& bv, const BatchShape& shape, const double* res,
-                             double* rhs);
-void launch_batch_accumulate(hipStream_t s, long long n, const double* x, double* acc);
+                           const double* planes64, double* res);
 void launch_batch_jacobi(hipStream_t s, const CsrView& a, const double* values, double* minv, int* bad);
 void launch_batch_scatter_case(hipStream_t s, int n, int n_cases, int c, const double* src, double* dst);
 void launch_batch_gather_case(hipStream_t s, int n, int n_cases, int c, const double* src, double* dst);

@@ -431,7 +431,8 @@ __global__ __launch_bounds__(256) void pack_slices_kernel(int n_nodes, int n_sli
                                                           const int* __restrict__ rp, const int* __restrict__ col,
                                                           const int* __restrict__ ptr,
                                                           const double* __restrict__ values, double* __restrict__ out64,
-                                                          float* __restrict__ out32, int* __restrict__ col_out) {
+                                                          float* __restrict__ out32, int* __restrict__ col_out,
+                                                          double* __restrict__ diag_out) {
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int s = static_cast<int>(t >> 6);
     const int lane = static_cast<int>(t & (kWave - 1));
@@ -448,6 +449,7 @@ __global__ __launch_bounds__(256) void pack_slices_kernel(int n_nodes, int n_sli
         deg = lone ? 1 : (r1 - r0) >> 1;
     }
     const int self = node < n_nodes ? node : n_nodes - 1;
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;  // diagonal block, then its compensation
     for (int k = 0; k < width; ++k) {
         const long long at = begin + static_cast<long long>(k) * kWave + lane;
         double u0 = 0.0, u1 = 0.0, l0 = 0.0, l1 = 0.0;
@@ -473,13 +475,45 @@ __global__ __launch_bounds__(256) void pack_slices_kernel(int n_nodes, int n_sli
             out64[2 * (blocks + at) + 1] = l1;
         }
         if (out32) {
-            out32[2 * at] = static_cast<float>(u0);
-            out32[2 * at + 1] = static_cast<float>(u1);
-            out32[2 * (blocks + at)] = static_cast<float>(l0);
-            out32[2 * (blocks + at) + 1] = static_cast<float>(l1);
+            const bool diag_block = diag_out != nullptr && c == self && k < deg && node < n_nodes;
+            const float f0 = diag_block ? 0.0f : static_cast<float>(u0);
+            const float f1 = diag_block ? 0.0f : static_cast<float>(u1);
+            const float f2 = diag_block ? 0.0f : static_cast<float>(l0);
+            const float f3 = diag_block ? 0.0f : static_cast<float>(l1);
+            out32[2 * at] = f0;
+            out32[2 * at + 1] = f1;
+            out32[2 * (blocks + at)] = f2;
+            out32[2 * (blocks + at) + 1] = f3;
+            if (diag_block) {
+                d0 += u0;
+                d1 += u1;
+                d2 += l0;
+                d3 += l1;
+            } else if (k < deg) {
+                // the row keeps its FP64 sum: the diagonal absorbs each rounding error
+                d0 -= static_cast<double>(f0) - u0;
+                d1 -= static_cast<double>(f1) - u1;
+                d2 -= static_cast<double>(f2) - l0;
+                d3 -= static_cast<double>(f3) - l1;
+            }
         }
         if (col_out) col_out[at] = c;
     }
+    if (diag_out && node < n_nodes) {
+        reinterpret_cast<dpair*>(diag_out)[2 * node] = dpair{d0, d1};
+        reinterpret_cast<dpair*>(diag_out)[2 * node + 1] = dpair{d2, d3};
+    }
+}
+
+// FP32 copy: the diagonal block (FP64, compensated) times the node's own pair, added
+// after the off-diagonal blocks.
+__device__ __forceinline__ void add_diag32(const SliceView& v, int node, dpair x, double& a0, double& a1) {
+    const dpair du = reinterpret_cast<const dpair*>(v.diag32)[2 * node];
+    const dpair dl = reinterpret_cast<const dpair*>(v.diag32)[2 * node + 1];
+    a0 += du.x * x.x;
+    a0 += du.y * x.y;
+    a1 += dl.x * x.x;
+    a1 += dl.y * x.y;
 }
 
 // Rows 2i, 2i+1 of node i = 64s + lane: every load instruction of the wave moves 64
@@ -536,9 +570,10 @@ __device__ __forceinline__ double slice_rows(const PcgBuffers& b, const SliceVie
     }
     double pq = 0.0;
     if (node < (b.n >> 1)) {
+        const dpair pv = p2[node];
+        if (sizeof(VT) == sizeof(float)) add_diag32(v, node, pv, a0, a1);
         const dpair qv = {a0, a1};
         reinterpret_cast<dpair*>(b.q)[node] = qv;
-        const dpair pv = p2[node];
         pq = pv.x * a0 + pv.y * a1;
     }
     return pq;
@@ -637,9 +672,10 @@ __device__ __forceinline__ double slice_rows_dir(const PcgBuffers& b, const Slic
     }
     double pq = 0.0;
     if (node < (b.n >> 1)) {
+        const dpair pv = direction(node);
+        if (sizeof(VT) == sizeof(float)) add_diag32(v, node, pv, a0, a1);
         const dpair qv = {a0, a1};
         reinterpret_cast<dpair*>(b.q)[node] = qv;
-        const dpair pv = direction(node);
         reinterpret_cast<dpair*>(p_new)[node] = pv;
         pq = pv.x * a0 + pv.y * a1;
     }
@@ -1008,10 +1044,10 @@ void launch_pcg_spmv(hipStream_t s, const PcgBuffers& b, const CsrView& a, const
 }
 
 void launch_pack_slices(hipStream_t s, const CsrView& a, const SliceView& v, const double* values, double* out64,
-                        float* out32, int* col_out) {
+                        float* out32, int* col_out, double* diag_out) {
     if (v.n_slices <= 0) return;
     pack_slices_kernel<<<blocks_for(static_cast<long long>(v.n_slices) * kWave, 256), 256, 0, s>>>(
-        a.n >> 1, v.n_slices, v.blocks, a.row_ptr, a.col, v.ptr, values, out64, out32, col_out);
+        a.n >> 1, v.n_slices, v.blocks, a.row_ptr, a.col, v.ptr, values, out64, out32, col_out, diag_out);
 }
 
 int slice_spmv_blocks(int n_slices) {

@@ -80,20 +80,22 @@ class LayoutInfo(ctypes.Structure):
 
 class BatchOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("stop_rule", ctypes.c_int32), ("check_every", ctypes.c_int32),
-                ("max_passes", ctypes.c_int32), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int64),
-                ("pass_reduction", ctypes.c_double), ("tune", ctypes.c_int32), ("block_threads", ctypes.c_int32),
-                ("slices_per_wave", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("tune", ctypes.c_int32), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int64),
+                ("mixed_switch", ctypes.c_double), ("block_threads", ctypes.c_int32),
+                ("slices_per_wave", ctypes.c_int32)]
 
 
 class BatchSystemReport(ctypes.Structure):
     _fields_ = [("converged", ctypes.c_int32), ("status", ctypes.c_int32), ("iterations", ctypes.c_int64),
-                ("initial_residual", ctypes.c_double), ("final_residual", ctypes.c_double),
+                ("switch_iteration", ctypes.c_int64), ("initial_residual", ctypes.c_double),
+                ("final_residual", ctypes.c_double),
                 ("true_residual", ctypes.c_double)]
 
 
 class BatchSummary(ctypes.Structure):
     _fields_ = [("solve_ms", ctypes.c_double), ("tune_ms", ctypes.c_double), ("graph_launches", ctypes.c_int64),
-                ("passes", ctypes.c_int32), ("block_threads", ctypes.c_int32), ("slices_per_wave", ctypes.c_int32),
+                ("problems_switched", ctypes.c_int32), ("block_threads", ctypes.c_int32),
+                ("slices_per_wave", ctypes.c_int32),
                 ("systems_converged", ctypes.c_int32), ("max_iterations", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
@@ -300,12 +302,13 @@ class Device:
         self.batch_cases = loads.shape[0]
 
     def batch_pcg(self, precision: str = "fp64", stop_rule: str = "relative", tolerance: float = 1e-10,
-                  max_iterations: int = 1_000_000, check_every: int = 0, max_passes: int = 0,
-                  pass_reduction: float = 0.0, tune: bool = True, block_threads: int = 0, slices_per_wave: int = 0,
-                  raise_on_error: bool = True):
-        """All (problem, case) systems; returns u (cases x dofs), per-system reports, summary."""
-        opt = BatchOptions(PRECISION[precision], STOP[stop_rule], check_every, max_passes, tolerance, max_iterations,
-                           pass_reduction, int(tune), block_threads, slices_per_wave, 0)
+                  max_iterations: int = 1_000_000, check_every: int = 0, mixed_switch: float = 0.0,
+                  tune: bool = True, block_threads: int = 0, slices_per_wave: int = 0, raise_on_error: bool = True):
+        """All (problem, case) systems; returns u (cases x dofs), per-system reports, summary.
+        mixed_switch (mixed only): relative residual at which a problem moves to FP32
+        values; 0 = 1e-2, negative = FP32 from the first iteration."""
+        opt = BatchOptions(PRECISION[precision], STOP[stop_rule], check_every, int(tune), tolerance, max_iterations,
+                           mixed_switch, block_threads, slices_per_wave)
         n_sys = self.batch_problems * self.batch_cases
         reports = (BatchSystemReport * n_sys)()
         summary = BatchSummary()

@@ -95,11 +95,13 @@ def test_stages_match_reference(dev, golden_index, name):
     assert np.array_equal(bits(dev.precond_apply("block", g["probe"])), bits(g["block1_apply_probe"]))
     assert np.array_equal(bits(dev.precond_apply("ic0", g["probe"])), bits(g["ic0_apply_probe"]))
     # PCG SpMV kernels (row-segment streaming 4, 5 and node-slice 6) sum each row in
-    # column order from 0.0, as the reference's CSR loop (sparse.cpp:104-120)
+    # column order from 0.0, as the reference's CSR loop (sparse.cpp:104-120); the
+    # row-segment kernels' FP32 copy is the plain rounding of every value
     y_ref = lfo.spmv(g["row_ptr"], g["col"], g["Kc"], g["probe"])
     y32_ref = lfo.spmv(g["row_ptr"], g["col"], g["Kc"].astype(np.float32).astype(np.float64), g["probe"])
     for variant in (4, 5, 6):
         assert np.array_equal(bits(dev.spmv(g["probe"], "fp64", variant)), bits(y_ref)), variant
+    for variant in (4, 5):
         assert np.array_equal(bits(dev.spmv(g["probe"], "mixed", variant)), bits(y32_ref)), variant
     check_stencil_spmv(dev, g["row_ptr"], g["col"], g["Kc"], g["probe"])
     # PCG with the case's preconditioner and the reference stop rule
@@ -122,27 +124,43 @@ def test_stages_match_reference(dev, golden_index, name):
     assert np.max(np.abs(centre - g["elem_centre"])) <= 1e-9 * np.max(np.abs(g["elem_centre"]))
 
 
+def check_compensated(y, y_ref, scale, row_abs, x):
+    """FP32 off-diagonal values with the compensated FP64 diagonal: FP32 rounding of each
+    off-diagonal value (2^-24 relative), once in the product and once in the diagonal's
+    compensation, plus FP64 summation."""
+    bound = 2.0 ** -23 * (scale + row_abs * np.abs(x)) + 1e-14 * scale
+    return bool(np.all(np.abs(y - y_ref) <= bound))
+
+
+def check_translations(dev, row_ptr, col, kc, variant):
+    """The compensated FP32 copy multiplies the two nodal translations as K does."""
+    n = row_ptr.shape[0] - 1
+    for t in (np.tile([1.0, 0.0], n // 2), np.tile([0.0, 1.0], n // 2)):
+        kt = lfo.spmv(row_ptr, col, kc, t)
+        t_scale = lfo.spmv(row_ptr, col, np.abs(kc), t)
+        assert np.all(np.abs(dev.spmv(t, "mixed", variant) - kt) <= 1e-13 * t_scale), variant
+
+
 def check_stencil_spmv(dev, row_ptr, col, kc, x):
-    """Node-stencil multiplies where the pattern admits them. Exact form: every offset
-    in column order, bit-identical to the CSR row loop. Symmetric form: block (i, i-d)
-    is the transpose of block (i-d, i), equal to the stored one up to the rounding of
-    Ke's (a,b)/(b,a) entries, so each row is within 1e-14 of |K||x|. FP32 copy: FP32
-    rounding of each off-diagonal value (2^-24 relative), once in the product and once
-    in the diagonal's compensation, and the nodal translations act exactly."""
+    """Node-slice and node-stencil multiplies where the pattern admits them. Exact
+    stencil form: every offset in column order, bit-identical to the CSR row loop.
+    Symmetric form: block (i, i-d) is the transpose of block (i-d, i), equal to the
+    stored one up to the rounding of Ke's (a,b)/(b,a) entries, so each row is within
+    1e-14 of |K||x|. FP32 copies of both layouts: compensated (check_compensated), and
+    the nodal translations act exactly."""
     info = dev.layout_info()
     y_ref = lfo.spmv(row_ptr, col, kc, x)
     scale = lfo.spmv(row_ptr, col, np.abs(kc), np.abs(x))
     row_abs = lfo.spmv(row_ptr, col, np.abs(kc), np.ones_like(x))
+    if info["node_blocked"]:
+        assert check_compensated(dev.spmv(x, "mixed", 6), y_ref, scale, row_abs, x)
+        check_translations(dev, row_ptr, col, kc, 6)
     if info["stencil_exact_planes"]:
         assert np.array_equal(bits(dev.spmv(x, "fp64", 9)), bits(y_ref))
     if info["stencil_sym_planes"]:
         assert np.all(np.abs(dev.spmv(x, "fp64", 8) - y_ref) <= 1e-14 * scale)
-        bound = 2.0 ** -23 * (scale + row_abs * np.abs(x)) + 1e-14 * scale
-        assert np.all(np.abs(dev.spmv(x, "mixed", 8) - y_ref) <= bound)
-        for t in (np.tile([1.0, 0.0], x.shape[0] // 2), np.tile([0.0, 1.0], x.shape[0] // 2)):
-            kt = lfo.spmv(row_ptr, col, kc, t)
-            t_scale = lfo.spmv(row_ptr, col, np.abs(kc), t)
-            assert np.all(np.abs(dev.spmv(t, "mixed", 8) - kt) <= 1e-13 * t_scale)
+        assert check_compensated(dev.spmv(x, "mixed", 8), y_ref, scale, row_abs, x)
+        check_translations(dev, row_ptr, col, kc, 8)
     return info
 
 
@@ -349,6 +367,5 @@ def test_full_size_assembly_properties(dev):
     # the node-slice, node-stencil (exact form) and row-segment kernels all sum rows in column order
     assert np.array_equal(bits(dev.spmv(x, "fp64", 6)), bits(dev.spmv(x, "fp64", 4)))
     assert np.array_equal(bits(dev.spmv(x, "fp64", 9)), bits(dev.spmv(x, "fp64", 4)))
-    assert np.array_equal(bits(dev.spmv(x, "mixed", 6)), bits(dev.spmv(x, "mixed", 4)))
     info = check_stencil_spmv(dev, sys_.row_ptr, sys_.col, kc, x)
     assert info["stencil_sym_planes"] == 5
