@@ -46,8 +46,15 @@ typedef struct lfg_ctx lfg_ctx;
 enum { LFG_PLANE_STRESS = 0, LFG_PLANE_STRAIN = 1 };
 /* `assembly` setting (model.hpp:99, solver.cpp:275-299). */
 enum { LFG_ASSEMBLY_SERIAL = 0, LFG_ASSEMBLY_ATOMIC = 1, LFG_ASSEMBLY_COLORING = 2, LFG_ASSEMBLY_THREAD_LOCAL = 3 };
-/* `preconditioner` setting (model.hpp:105); BLOCK2 = `block` with subdomain_nodes 1. */
-enum { LFG_PRECOND_JACOBI = 0, LFG_PRECOND_BLOCK2 = 1, LFG_PRECOND_IC0 = 2 };
+/* `preconditioner` setting (model.hpp:105); BLOCK2 = `block` with subdomain_nodes 1
+ * (2x2 nodal blocks, fused into the iteration); SUBDOMAIN = `block` on the subdomains
+ * given to lfg_set_subdomains (any size, SubdomainBlockJacobi, precond.cpp:121-281). */
+enum { LFG_PRECOND_JACOBI = 0, LFG_PRECOND_BLOCK2 = 1, LFG_PRECOND_IC0 = 2, LFG_PRECOND_SUBDOMAIN = 3 };
+/* `storage` setting (model.hpp:103): which summation order the products follow, the CSR
+ * row loop (sparse.cpp:104-120) or the 2x2 block storage product (sparse.cpp:128-198). */
+enum { LFG_STORAGE_CSR = 0, LFG_STORAGE_BSR = 1 };
+/* OR'ed into the variant of lfg_spmv / lfg_bench_spmv: the block storage order. */
+enum { LFG_SPMV_BLOCK_ORDER = 256 };
 enum { LFG_PRECISION_FP64 = 0, LFG_PRECISION_MIXED = 1 };
 /* reference: ||r|| <= tol*max(1,||r0||) (solver.cpp:572); relative: ||r|| <= tol*||b||. */
 enum { LFG_STOP_REFERENCE = 0, LFG_STOP_RELATIVE = 1 };
@@ -82,6 +89,10 @@ typedef struct {
                                 block iteration in one launch (update, search direction, multiply
                                 and all sums; beta from the next residual's r.z recurrence, so one
                                 reduction per iteration) */
+    int32_t storage;         /* LFG_STORAGE_*: BSR sums each 2x2 block's products first (variants 6, 7
+                                and 9 follow it bit for bit; 8 and 10 read the transposed blocks as
+                                always; 1-5 are CSR only). Needs a node-blocked pattern. */
+    int32_t pad;
 } lfg_pcg_options;
 
 typedef struct {
@@ -256,10 +267,17 @@ typedef struct {
 } lfg_layout_info;
 int lfg_layout_info_get(lfg_ctx* ctx, lfg_layout_info* info);
 
+/* Subdomains of the `block` preconditioner: node_subdomain[i] in [0, n_subdomains) for
+ * every node index i (both dofs of a node share its subdomain, build_dof_subdomain,
+ * solver.cpp:631-654; lfh_subdomain_split computes the reference's split). The dense
+ * Cholesky factors (precond.cpp:178-232) are formed at the next solve with
+ * LFG_PRECOND_SUBDOMAIN; subdomains up to 4096 dofs. */
+int lfg_set_subdomains(lfg_ctx* ctx, int64_t n_subdomains, const int32_t* node_subdomain);
+
 /* Checks and measurement hooks on the constrained system. */
 int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, double* out);
 /* y = Kc x with the PCG SpMV kernel of `variant` (values of `precision`; 10 multiplies
- * as 8). */
+ * as 8; variant | LFG_SPMV_BLOCK_ORDER for 6, 7, 9: the block storage order). */
 int lfg_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x, double* y);
 /* Average device time (ms) of `reps` SpMV launches of the given precision and
  * variant on resident data (variant 7: the multiply with the Jacobi direction formed

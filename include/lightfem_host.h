@@ -35,6 +35,14 @@ int lfh_rcm(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int
 int lfh_element_colors(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int32_t* color_out,
                        int32_t* n_colors_out, int32_t* color_ptr, int32_t* color_elems);
 
+/* Subdomains of the `block` preconditioner: build_dof_subdomain's node split
+ * (solver.cpp:631-654) into node_subdomain (n_nodes), with split 0 = "rcb" (recursive
+ * coordinate bisection of the node positions xy[2i], xy[2i+1], precond.cpp:30-105)
+ * or 1 = "chunk" (node index / nodes_per_subdomain, precond.cpp:19-28). The count
+ * goes to n_subdomains_out. */
+int lfh_subdomain_split(int64_t n_nodes, const double* xy, int32_t split, int64_t nodes_per_subdomain,
+                        int32_t* node_subdomain, int64_t* n_subdomains_out);
+
 /* csr_profile (sparse.cpp:227-242): out = {nonzeros, bandwidth, profile}. */
 int lfh_csr_profile(int64_t n_rows, const int32_t* row_ptr, const int32_t* col, int64_t out[3]);
 

@@ -8,6 +8,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lfg_kernels.hpp"
@@ -80,6 +81,14 @@ struct Ic0Device {
     bool ready = false;
 };
 
+struct SubdomainDevice {
+    DevArray<int> dof_begin, dofs;
+    DevArray<long long> factor_begin;
+    DevArray<double> factor;
+    int n_sub = 0, max_size = 0;
+    long long degenerate = 0;
+};
+
 }  // namespace
 
 struct lfg_ctx {
@@ -133,11 +142,16 @@ struct lfg_ctx {
     DevArray<double> lam, adj_rhs, dE, grad;  // adjoint solution and right-hand side, sensitivities
     bool have_adjoint = false;
     DevArray<lfg::PcgState> state;
-    DevArray<lfg::TicketSet> tickets;  // 0 init, 1 SpMV, 2 update, 3 replacement, 4 residual
+    DevArray<lfg::TicketSet> tickets;  // 0 init, 1 SpMV, 2 update, 3 replacement, 4 residual, 5 subdomains
     DevArray<int> flags;
     lfg::PcgState* h_state = nullptr;  // pinned, 2 slots
     bool have_solution = false;
     Ic0Device ic0;
+    // subdomain block-Jacobi: node -> subdomain map (host) and the device factors
+    std::vector<int> sub_node;
+    int n_sub = 0;
+    SubdomainDevice sub;
+    int pair_sum = 0;  // products in the block storage order (LFG_STORAGE_BSR) while set
     int precond_ready = -1;  // which preconditioner the device factors belong to
 
     // fields
@@ -187,9 +201,16 @@ lfg::MeshView mesh_view(const lfg_ctx* c) {
 lfg::CsrView csr_view(const lfg_ctx* c) { return lfg::CsrView{c->n, c->nnz, c->row_ptr.p, c->col.p, c->max_row}; }
 
 lfg::SliceView slice_view(const lfg_ctx* c) {
-    return lfg::SliceView{c->n_slices,  c->slice_blocks,    c->slice_ptr.p,
-                          c->slice_col.p, c->slice_max_width, c->slice_diag.p};
+    return lfg::SliceView{c->n_slices,       c->slice_blocks,  c->slice_ptr.p, c->slice_col.p,
+                          c->slice_max_width, c->slice_diag.p, c->pair_sum};
 }
+
+// Sets the block storage summation order for the products enqueued in its scope.
+struct PairScope {
+    lfg_ctx* c;
+    PairScope(lfg_ctx* ctx, bool on) : c(ctx) { c->pair_sum = on ? 1 : 0; }
+    ~PairScope() { c->pair_sum = 0; }
+};
 
 constexpr int kSliceVariant = 6;  // node-slice SpMV, separate direction kernel
 constexpr int kSliceFused = 7;    // node-slice SpMV forming the Jacobi / block direction on the fly
@@ -205,17 +226,30 @@ bool fused_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stenc
 
 // 0 selects, in order: the fused stencil iteration (Jacobi and block preconditioners),
 // the symmetric node-stencil multiply, the node-slice multiply, row-segment streaming.
-int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JACOBI) {
+// Preconditioners applied by a separate sweep kernel into z (not per node in the
+// update / direction / fused kernels).
+bool sweep_precond(int precond) { return precond == LFG_PRECOND_IC0 || precond == LFG_PRECOND_SUBDOMAIN; }
+
+int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JACOBI, bool block_order = false) {
+    if (block_order) {
+        const int v = variant == 0 ? (fused_stencil_ok(c) && !sweep_precond(precond)
+                                          ? kStencilFused
+                                          : (c->have_stencil_sym ? kStencilSym : kSliceVariant))
+                                   : variant;
+        require(v >= kSliceVariant && v <= kStencilFused, LFG_ERR_UNSUPPORTED,
+                "block storage runs on the node-slice (6, 7) and node-stencil (8-10) SpMVs");
+        return resolve_variant(c, v, precond, false);
+    }
     if (variant == 0) {
-        if (fused_stencil_ok(c) && precond != LFG_PRECOND_IC0) return kStencilFused;
+        if (fused_stencil_ok(c) && !sweep_precond(precond)) return kStencilFused;
         return c->have_stencil_sym ? kStencilSym : (c->have_slices ? kSliceVariant : 4);
     }
     require(variant >= 1 && variant <= kStencilFused, LFG_ERR_ARG, "SpMV variant must be 0-10");
     require(variant != kStencilFused || fused_stencil_ok(c), LFG_ERR_UNSUPPORTED,
             "the fused stencil iteration needs the symmetric node-stencil layout with offset windows that fit "
             "in LDS");
-    require(variant != kStencilFused || precond != LFG_PRECOND_IC0, LFG_ERR_UNSUPPORTED,
-            "the fused stencil iteration applies the jacobi and 2x2 block preconditioners, not ic0");
+    require(variant != kStencilFused || !sweep_precond(precond), LFG_ERR_UNSUPPORTED,
+            "the fused stencil iteration applies the jacobi and 2x2 block preconditioners, not ic0 or subdomains");
     require(!slice_layout(variant) || c->have_slices, LFG_ERR_UNSUPPORTED,
             "the node-slice SpMV needs rows 2i, 2i+1 sharing one column list of dof pairs");
     require(variant != kStencilSym || c->have_stencil_sym, LFG_ERR_UNSUPPORTED,
@@ -404,8 +438,11 @@ void ensure_stencil(lfg_ctx* c, int variant, bool need64, bool need32) {
 // mixed, the symmetric form switches to its FP32 copy in the FP32 phase.
 void launch_stencil_spmv(lfg_ctx* c, const lfg::PcgBuffers& b, int variant, bool fp64, bool fp32) {
     if (variant == kStencilFused) variant = kStencilSym;  // the plain multiply of the same planes
-    if (variant == kStencilExact)
-        lfg::launch_pcg_spmv_stencil(c->stream, b, c->stencil_exact, c->stencil_exact64.p, nullptr, nullptr);
+    if (variant == kStencilExact) {
+        lfg::StencilView ex = c->stencil_exact;
+        ex.pair_sum = c->pair_sum;
+        lfg::launch_pcg_spmv_stencil(c->stream, b, ex, c->stencil_exact64.p, nullptr, nullptr);
+    }
     else
         lfg::launch_pcg_spmv_stencil(c->stream, b, c->stencil_sym, fp64 ? c->stencil64.p : nullptr,
                                      fp32 ? c->stencil32.p : nullptr, fp32 ? c->stencil_diag.p : nullptr);
@@ -732,6 +769,109 @@ void build_ic0(lfg_ctx* c) {
     sync(c);
 }
 
+// SubdomainBlockJacobi set-up (precond.cpp:121-232) on the host, in the reference's
+// order: each subdomain's dofs in matrix order, the dense block of its constrained
+// values, an in-place row-by-row Cholesky (a failed block becomes the identity) and
+// the packed lower factor with reciprocal pivots. Subdomains are independent, so
+// host threads take them in interleaved order.
+void build_subdomains(lfg_ctx* c) {
+    require(c->n_sub > 0 && static_cast<int>(c->sub_node.size()) == c->n / 2, LFG_ERR_STATE,
+            "lfg_set_subdomains must precede a subdomain block-Jacobi solve");
+    const int n = c->n, n_sub = c->n_sub;
+    std::vector<double> vals(static_cast<size_t>(c->nnz));
+    c->Kc.download(vals.data(), vals.size(), c->stream);
+    sync(c);
+    const std::vector<int>& rp = c->h_row_ptr;
+    const std::vector<int>& col = c->h_col;
+    std::vector<int> owner(static_cast<size_t>(n));
+    for (int d = 0; d < n; ++d) owner[d] = c->sub_node[d / 2];
+    std::vector<int> dof_begin(static_cast<size_t>(n_sub) + 1, 0), dofs(static_cast<size_t>(n)),
+        local(static_cast<size_t>(n));
+    for (int d = 0; d < n; ++d) dof_begin[owner[d] + 1]++;
+    for (int s = 0; s < n_sub; ++s) dof_begin[s + 1] += dof_begin[s];
+    {
+        std::vector<int> cur(dof_begin.begin(), dof_begin.end() - 1);
+        for (int d = 0; d < n; ++d) {
+            local[d] = cur[owner[d]] - dof_begin[owner[d]];
+            dofs[cur[owner[d]]++] = d;
+        }
+    }
+    std::vector<long long> fbeg(static_cast<size_t>(n_sub) + 1, 0);
+    int max_size = 0;
+    for (int s = 0; s < n_sub; ++s) {
+        const long long size = dof_begin[s + 1] - dof_begin[s];
+        fbeg[s + 1] = fbeg[s] + size * (size + 1) / 2;
+        max_size = std::max(max_size, static_cast<int>(size));
+    }
+    require(max_size <= lfg::kMaxSubdomainDofs, LFG_ERR_UNSUPPORTED,
+            "subdomains above 512 dofs (256 nodes) are outside this backend's block-Jacobi");
+    std::vector<double> factor(static_cast<size_t>(fbeg[n_sub]) + 1, 0.0);
+    std::vector<char> bad(static_cast<size_t>(n_sub), 0);
+    auto factor_range = [&](int first, int step) {
+        std::vector<double> dense(static_cast<size_t>(max_size) * max_size);
+        for (int s = first; s < n_sub; s += step) {
+            const int begin = dof_begin[s], size = dof_begin[s + 1] - begin;
+            if (size == 0) continue;
+            std::fill(dense.begin(), dense.begin() + static_cast<long long>(size) * size, 0.0);
+            for (int i = 0; i < size; ++i) {
+                const int dof = dofs[begin + i];
+                for (int k = rp[dof]; k < rp[dof + 1]; ++k)
+                    if (owner[col[k]] == s) dense[static_cast<size_t>(i) * size + local[col[k]]] = vals[k];
+            }
+            bool ok = true;
+            for (int i = 0; i < size && ok; ++i) {
+                for (int j = 0; j <= i; ++j) {
+                    double sum = dense[static_cast<size_t>(i) * size + j];
+                    for (int k = 0; k < j; ++k)
+                        sum -= dense[static_cast<size_t>(i) * size + k] * dense[static_cast<size_t>(j) * size + k];
+                    if (i == j) {
+                        if (!(sum > 0.0) || !std::isfinite(sum)) {
+                            ok = false;
+                            break;
+                        }
+                        dense[static_cast<size_t>(i) * size + i] = std::sqrt(sum);
+                    } else {
+                        dense[static_cast<size_t>(i) * size + j] = sum / dense[static_cast<size_t>(j) * size + j];
+                    }
+                }
+            }
+            if (!ok) {
+                bad[s] = 1;
+                for (int i = 0; i < size; ++i)
+                    for (int j = 0; j <= i; ++j) dense[static_cast<size_t>(i) * size + j] = i == j ? 1.0 : 0.0;
+            }
+            double* t = factor.data() + fbeg[s];
+            for (int i = 0; i < size; ++i) {
+                for (int j = 0; j <= i; ++j) t[i * (i + 1) / 2 + j] = dense[static_cast<size_t>(i) * size + j];
+                t[i * (i + 1) / 2 + i] = 1.0 / t[i * (i + 1) / 2 + i];
+            }
+        }
+    };
+    const int workers = std::max(1, std::min<int>(16, static_cast<int>(std::thread::hardware_concurrency())));
+    if (workers == 1 || n_sub < 64) {
+        factor_range(0, 1);
+    } else {
+        std::vector<std::thread> pool;
+        for (int w = 0; w < workers; ++w) pool.emplace_back(factor_range, w, workers);
+        for (std::thread& t : pool) t.join();
+    }
+    SubdomainDevice& d = c->sub;
+    d.n_sub = n_sub;
+    d.max_size = max_size;
+    d.degenerate = 0;
+    for (char b : bad) d.degenerate += b;
+    d.dof_begin.upload(dof_begin.data(), dof_begin.size(), c->stream);
+    d.dofs.upload(dofs.data(), dofs.size(), c->stream);
+    d.factor_begin.upload(fbeg.data(), fbeg.size(), c->stream);
+    d.factor.upload(factor.data(), factor.size(), c->stream);
+    sync(c);
+}
+
+lfg::SubdomainView subdomain_view(const lfg_ctx* c) {
+    return lfg::SubdomainView{c->sub.n_sub, c->sub.max_size, c->sub.dof_begin.p, c->sub.dofs.p,
+                              c->sub.factor_begin.p, c->sub.factor.p};
+}
+
 lfg::Ic0View ic0_view(const lfg_ctx* c) {
     const Ic0Device& d = c->ic0;
     const lfg::Ic0Sweep fwd{d.fwd.ord.p,       d.fwd.ptr.p,  d.fwd.key.p,   d.fwd.val.p,   d.fwd.scale.p,
@@ -761,8 +901,8 @@ void ensure_vectors(lfg_ctx* c) {
     if (!c->state.p) {
         c->state.alloc(1);
         LFG_HIP(hipMemsetAsync(c->state.p, 0, sizeof(lfg::PcgState), c->stream));
-        c->tickets.alloc(5);
-        LFG_HIP(hipMemsetAsync(c->tickets.p, 0, 5 * sizeof(lfg::TicketSet), c->stream));
+        c->tickets.alloc(6);
+        LFG_HIP(hipMemsetAsync(c->tickets.p, 0, 6 * sizeof(lfg::TicketSet), c->stream));
         c->flags.alloc(4);
         LFG_HIP(hipMemsetAsync(c->scratch.p, 0, c->scratch.n * sizeof(double), c->stream));
     }
@@ -792,6 +932,9 @@ void setup_preconditioner(lfg_ctx* c, int kind, long long* degenerate) {
         LFG_HIP(hipMemcpyAsync(&deg, c->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         sync(c);
         *degenerate = deg;
+    } else if (kind == LFG_PRECOND_SUBDOMAIN) {
+        build_subdomains(c);
+        *degenerate = c->sub.degenerate;
     } else {
         build_ic0(c);
     }
@@ -818,7 +961,7 @@ lfg::PcgBuffers pcg_buffers(lfg_ctx* c, int kind, const double* rhs, double* xs)
 // Whether an iteration forms its search direction inside the slice SpMV (two
 // launches per iteration, p alternating between two buffers) instead of in a
 // direction kernel after the update.
-bool fused_direction(int variant, int precond) { return variant == kSliceFused && precond != LFG_PRECOND_IC0; }
+bool fused_direction(int variant, int precond) { return variant == kSliceFused && !sweep_precond(precond); }
 
 // One PCG iteration. With the fused direction, iteration `parity` reads p from
 // buffer parity (b.p or c->p2) and writes the new p into the other one.
@@ -852,6 +995,7 @@ void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int repl
     lfg::launch_pcg_update(c->stream, b, mixed);
     if (replace) lfg::launch_pcg_replace(c->stream, b, a, c->Kc.p);
     if (b.precond == LFG_PRECOND_IC0) lfg::launch_pcg_ic0_step(c->stream, b, ic0_view(c));
+    if (b.precond == LFG_PRECOND_SUBDOMAIN) lfg::launch_pcg_subdomain_step(c->stream, b, subdomain_view(c));
     lfg::launch_pcg_direction(c->stream, b);
 }
 
@@ -1013,6 +1157,8 @@ int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* ro
         ctx->have_pattern = true;
         ctx->have_K = ctx->have_system = ctx->have_solution = false;
         ctx->precond_ready = -1;
+        ctx->sub_node.clear();  // subdomains belong to the node numbering of a pattern
+        ctx->n_sub = 0;
         return LFG_OK;
     });
 }
@@ -1193,10 +1339,13 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     require(ctx->have_system, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_pcg");
     require(opt != nullptr, LFG_ERR_ARG, "missing PCG options");
     const int kind = opt->preconditioner;
-    require(kind == LFG_PRECOND_JACOBI || kind == LFG_PRECOND_BLOCK2 || kind == LFG_PRECOND_IC0, LFG_ERR_ARG,
-            "preconditioner must be jacobi, block, or ic0");
+    require(kind == LFG_PRECOND_JACOBI || kind == LFG_PRECOND_BLOCK2 || kind == LFG_PRECOND_IC0 ||
+                kind == LFG_PRECOND_SUBDOMAIN,
+            LFG_ERR_ARG, "preconditioner must be jacobi, block, or ic0");
+    require(opt->storage == LFG_STORAGE_CSR || opt->storage == LFG_STORAGE_BSR, LFG_ERR_ARG,
+            "storage must be csr or bsr");
     const int mixed = opt->precision == LFG_PRECISION_MIXED;
-    require(!(mixed && kind == LFG_PRECOND_IC0), LFG_ERR_UNSUPPORTED,
+    require(!(mixed && sweep_precond(kind)), LFG_ERR_UNSUPPORTED,
             "mixed precision is implemented for the jacobi and 2x2 block preconditioners");
     require(opt->tolerance > 0.0 && opt->tolerance < 1.0, LFG_ERR_ARG, "solver tolerance must satisfy 0 < tolerance < 1");
     require(opt->max_iterations > 0, LFG_ERR_ARG, "max_iterations must be greater than zero");
@@ -1209,7 +1358,10 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     const int previous = ctx->precond_ready;
     setup_preconditioner(ctx, kind, &degenerate);
     if (previous == kind && kind == LFG_PRECOND_BLOCK2) degenerate = -1;  // factor reused; count unknown here
-    const int variant = resolve_variant(ctx, opt->spmv_variant, kind);
+    if (kind == LFG_PRECOND_SUBDOMAIN) degenerate = ctx->sub.degenerate;
+    const bool block_order = opt->storage == LFG_STORAGE_BSR;
+    const int variant = resolve_variant(ctx, opt->spmv_variant, kind, block_order);
+    const PairScope pair_scope(ctx, block_order);
     require(!(mixed && variant == kStencilExact), LFG_ERR_UNSUPPORTED,
             "mixed precision uses the symmetric stencil form (variant 8), not the exact one");
     if (stencil_layout(variant)) {
@@ -1270,6 +1422,9 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
                          stencil_fused ? 1 : 0);
     if (kind == LFG_PRECOND_IC0) {
         lfg::launch_pcg_ic0_step(ctx->stream, b, ic0_view(ctx));
+        lfg::launch_pcg_direction(ctx->stream, b);
+    } else if (kind == LFG_PRECOND_SUBDOMAIN) {
+        lfg::launch_pcg_subdomain_step(ctx->stream, b, subdomain_view(ctx));
         lfg::launch_pcg_direction(ctx->stream, b);
     }
     // every iteration the cap allows fits in this many graph launches
@@ -1418,6 +1573,24 @@ int lfg_postprocess(lfg_ctx* ctx, double* centre, double* gauss_points, double* 
     });
 }
 
+int lfg_set_subdomains(lfg_ctx* ctx, int64_t n_subdomains, const int32_t* node_subdomain) {
+    return guarded(ctx, [&] {
+        require(ctx->have_pattern, LFG_ERR_STATE, "lfg_set_pattern must precede lfg_set_subdomains");
+        require(n_subdomains > 0 && n_subdomains <= (1LL << 30) && node_subdomain, LFG_ERR_ARG,
+                "subdomain count must be positive");
+        const int n_nodes = ctx->n / 2;
+        std::vector<int> map(static_cast<size_t>(n_nodes));
+        for (int i = 0; i < n_nodes; ++i) {
+            require(node_subdomain[i] >= 0 && node_subdomain[i] < n_subdomains, LFG_ERR_ARG, "subdomain id out of range");
+            map[i] = node_subdomain[i];
+        }
+        ctx->sub_node.swap(map);
+        ctx->n_sub = static_cast<int>(n_subdomains);
+        if (ctx->precond_ready == LFG_PRECOND_SUBDOMAIN) ctx->precond_ready = -1;
+        return LFG_OK;
+    });
+}
+
 int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, double* out) {
     return guarded(ctx, [&] {
         require(ctx->have_system && in && out, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_precond_apply");
@@ -1433,6 +1606,8 @@ int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, do
             LFG_HIP(hipMemcpyAsync(&fault, ctx->flags.p + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
             sync(ctx);
             require(fault == 0, LFG_ERR_HIP, kIc0Fault);
+        } else if (preconditioner == LFG_PRECOND_SUBDOMAIN) {
+            lfg::launch_subdomain_apply(ctx->stream, subdomain_view(ctx), ctx->r.p, ctx->z.p);
         } else {
             lfg::launch_precond_apply(ctx->stream, preconditioner, ctx->n,
                                       preconditioner == LFG_PRECOND_JACOBI ? ctx->minv.p : ctx->factor.p, ctx->r.p,
@@ -1500,7 +1675,9 @@ int lfg_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x, 
     return guarded(ctx, [&] {
         require(ctx->have_system && x && y, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_spmv");
         LFG_HIP(hipSetDevice(ctx->device));
-        const int v = resolve_variant(ctx, variant);
+        const bool block_order = (variant & LFG_SPMV_BLOCK_ORDER) != 0;
+        const int v = resolve_variant(ctx, variant & ~LFG_SPMV_BLOCK_ORDER, LFG_PRECOND_JACOBI, block_order);
+        const PairScope pair_scope(ctx, block_order);
         prepare_spmv(ctx, precision, v);
         LFG_HIP(hipMemcpyAsync(ctx->p.p, x, ctx->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         run_spmv(ctx, precision, v);
@@ -1515,7 +1692,9 @@ int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t rep
     return guarded(ctx, [&] {
         require(ctx->have_system && reps > 0 && avg_ms, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_bench_spmv");
         LFG_HIP(hipSetDevice(ctx->device));
-        const int v = resolve_variant(ctx, variant);
+        const bool block_order = (variant & LFG_SPMV_BLOCK_ORDER) != 0;
+        const int v = resolve_variant(ctx, variant & ~LFG_SPMV_BLOCK_ORDER, LFG_PRECOND_JACOBI, block_order);
+        const PairScope pair_scope(ctx, block_order);
         if (v == kStencilFused) {
             // whole Jacobi PCG iterations (the fused kernel does the multiply and the vector
             // work), from x = 0 with a stop threshold no residual reaches

@@ -147,6 +147,22 @@ __device__ __forceinline__ double max0(double v) { return (0.0 < v) ? v : 0.0; }
 
 // Value pairs of one 2x2 block row (16-byte FP64 or 8-byte FP32 loads).
 typedef double dpair __attribute__((ext_vector_type(2)));
+
+// A 2x2 block (rows u, l) times the pair x, added to the row sums a0, a1: one product
+// at a time, as the CSR row loop (sparse.cpp:113-117), or with pair = true the block's
+// two products of a row summed first, as the block storage product (sparse.cpp:188-193).
+__device__ __forceinline__ void add_block(double& a0, double& a1, double ux, double uy, double lx, double ly, dpair x,
+                                          bool pair) {
+    if (pair) {
+        a0 += ux * x.x + uy * x.y;
+        a1 += lx * x.x + ly * x.y;
+    } else {
+        a0 += ux * x.x;
+        a0 += uy * x.y;
+        a1 += lx * x.x;
+        a1 += ly * x.y;
+    }
+}
 typedef float fpair __attribute__((ext_vector_type(2)));
 
 template <typename VT>

@@ -48,6 +48,7 @@ struct SliceView {
     const int* col;    // node column per block
     int max_width;
     const double* diag32;  // compensated diagonal of the FP32 copy
+    int pair_sum;          // 1: block storage summation order (add_block)
 };
 
 // Node-stencil layout of a node-blocked matrix whose blocks lie at a few fixed node
@@ -71,6 +72,7 @@ struct StencilView {
     int n_nodes;
     int n_off;              // planes (exact: all offsets; symmetric: 0 and the positive ones)
     int sym;
+    int pair_sum;           // exact form: 1 = block storage summation order (add_block)
     int off[kMaxStencil];   // ascending
     // Symmetric form, fused PCG iteration: the offsets -off[k], 0, off[k] grouped into
     // clusters of nearby offsets; a 256-node tile needs the search direction on one
@@ -253,6 +255,23 @@ struct Ic0View {
 };
 void launch_ic0_apply(hipStream_t s, const Ic0View& f, const double* in, double* out);
 void launch_pcg_ic0_step(hipStream_t s, const PcgBuffers& b, const Ic0View& f);
+
+// ---- subdomain block-Jacobi (lfg_subdomain.hip)
+// Subdomain s owns dofs[dof_begin[s] .. dof_begin[s+1]) in matrix order and the packed
+// lower Cholesky factor at factor + factor_begin[s] (row i at i(i+1)/2, reciprocal
+// pivots on the diagonal), precond.cpp:121-232. max_size <= kMaxSubdomainDofs.
+constexpr int kMaxSubdomainDofs = 512;
+struct SubdomainView {
+    int n_sub;
+    int max_size;
+    const int* dof_begin;
+    const int* dofs;
+    const long long* factor_begin;
+    const double* factor;
+};
+void launch_subdomain_apply(hipStream_t s, const SubdomainView& v, const double* in, double* out);
+// z = M^-1 r with rho = r.z and beta into the PCG state (uses ticket set 5).
+void launch_pcg_subdomain_step(hipStream_t s, const PcgBuffers& b, const SubdomainView& v);
 // ---- batched PCG (lfg_batch.hip): independent problems in whole 64-node slices,
 // NC load cases per problem stored per node ([node][case][2]).
 struct BatchSystem {      // one (problem, case)

@@ -519,7 +519,8 @@ __device__ __forceinline__ void add_diag32(const SliceView& v, int node, dpair x
 // Rows 2i, 2i+1 of node i = 64s + lane: every load instruction of the wave moves 64
 // consecutive blocks of a plane (or columns), p is gathered as one pair per block,
 // and each row accumulates its products in column order from 0.0, exactly as the
-// serial CSR row loop (sparse.cpp:104-120). kW > 0: slices up to kW wide take the
+// serial CSR row loop (sparse.cpp:104-120), or block by block as the block storage
+// product (sparse.cpp:176-198) when v.pair_sum is set. kW > 0: slices up to kW wide take the
 // unrolled path, whose loads are unconditional (indices clamped to the last
 // block) and whose accumulation skips the clamped entries.
 template <typename VT, int kW>
@@ -550,10 +551,8 @@ __device__ __forceinline__ double slice_rows(const PcgBuffers& b, const SliceVie
 #pragma unroll
         for (int k = 0; k < kW; ++k) {
             if (k < width) {
-                a0 += static_cast<double>(u[k].x) * x[k].x;
-                a0 += static_cast<double>(u[k].y) * x[k].y;
-                a1 += static_cast<double>(l[k].x) * x[k].x;
-                a1 += static_cast<double>(l[k].y) * x[k].y;
+                add_block(a0, a1, static_cast<double>(u[k].x), static_cast<double>(u[k].y),
+                          static_cast<double>(l[k].x), static_cast<double>(l[k].y), x[k], v.pair_sum != 0);
             }
         }
     } else {
@@ -562,10 +561,8 @@ __device__ __forceinline__ double slice_rows(const PcgBuffers& b, const SliceVie
             const P uk = __builtin_nontemporal_load(upper + at);
             const P lk = __builtin_nontemporal_load(lower + at);
             const dpair xk = p2[__builtin_nontemporal_load(v.col + at)];
-            a0 += static_cast<double>(uk.x) * xk.x;
-            a0 += static_cast<double>(uk.y) * xk.y;
-            a1 += static_cast<double>(lk.x) * xk.x;
-            a1 += static_cast<double>(lk.y) * xk.y;
+            add_block(a0, a1, static_cast<double>(uk.x), static_cast<double>(uk.y), static_cast<double>(lk.x),
+                      static_cast<double>(lk.y), xk, v.pair_sum != 0);
         }
     }
     double pq = 0.0;
@@ -652,10 +649,8 @@ __device__ __forceinline__ double slice_rows_dir(const PcgBuffers& b, const Slic
         for (int k = 0; k < kW; ++k) {
             if (k < width) {
                 const dpair x = direction(c[k]);
-                a0 += static_cast<double>(u[k].x) * x.x;
-                a0 += static_cast<double>(u[k].y) * x.y;
-                a1 += static_cast<double>(l[k].x) * x.x;
-                a1 += static_cast<double>(l[k].y) * x.y;
+                add_block(a0, a1, static_cast<double>(u[k].x), static_cast<double>(u[k].y),
+                          static_cast<double>(l[k].x), static_cast<double>(l[k].y), x, v.pair_sum != 0);
             }
         }
     } else {
@@ -664,10 +659,8 @@ __device__ __forceinline__ double slice_rows_dir(const PcgBuffers& b, const Slic
             const P uk = __builtin_nontemporal_load(upper + at);
             const P lk = __builtin_nontemporal_load(lower + at);
             const dpair x = direction(__builtin_nontemporal_load(v.col + at));
-            a0 += static_cast<double>(uk.x) * x.x;
-            a0 += static_cast<double>(uk.y) * x.y;
-            a1 += static_cast<double>(lk.x) * x.x;
-            a1 += static_cast<double>(lk.y) * x.y;
+            add_block(a0, a1, static_cast<double>(uk.x), static_cast<double>(uk.y), static_cast<double>(lk.x),
+                      static_cast<double>(lk.y), x, v.pair_sum != 0);
         }
     }
     double pq = 0.0;

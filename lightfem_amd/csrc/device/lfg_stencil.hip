@@ -202,19 +202,16 @@ __device__ __forceinline__ void stencil_sym_node(const StencilView& v, const dpa
     }
 }
 
-// Exact form, node i: every plane in ascending offset order (the CSR column order);
-// the zero blocks of absent couplings add nothing. kN > 0: exactly kN planes, loads
+// Exact form, node i: every plane in ascending offset order (the CSR column order,
+// or the block storage order with v.pair_sum); the zero blocks of absent couplings
+// add nothing. kN > 0: exactly kN planes, loads
 // first; kN = 0: any count.
 template <int kN>
 __device__ __forceinline__ void stencil_exact_node(const StencilView& v, const double* __restrict__ planes,
                                                    const dpair* __restrict__ p2, int i, double& a0, double& a1) {
     const dpair* __restrict__ pl = reinterpret_cast<const dpair*>(planes);
-    auto term = [&](dpair u, dpair l, dpair x) {
-        a0 += u.x * x.x;
-        a0 += u.y * x.y;
-        a1 += l.x * x.x;
-        a1 += l.y * x.y;
-    };
+    const bool pair = v.pair_sum != 0;
+    auto term = [&](dpair u, dpair l, dpair x) { add_block(a0, a1, u.x, u.y, l.x, l.y, x, pair); };
     a0 = 0.0;
     a1 = 0.0;
     if (kN > 0) {

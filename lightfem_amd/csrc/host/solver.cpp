@@ -5,7 +5,9 @@
 // device is an error.
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -182,10 +184,6 @@ AnalysisResult LinearElasticSolver::solve(Model& model) const {
     const SolverSettings& s = model.settings;
     if (!model.trusses.empty())
         throw std::runtime_error("truss members are outside the gfx950 backend's scope (Q4 models only)");
-    if (s.storage != "csr") throw std::runtime_error("storage bsr is not implemented by the gfx950 backend; use csr");
-    if (s.preconditioner == "block" && s.subdomain_nodes != 1)
-        throw std::runtime_error("the gfx950 backend implements preconditioner block with subdomain_nodes 1 "
-                                 "(2x2 nodal blocks)");
     const auto solve_start = std::chrono::steady_clock::now();
     AnalysisResult result;
     result.performance.reordering = s.reordering;
@@ -261,10 +259,24 @@ AnalysisResult LinearElasticSolver::solve(Model& model) const {
     dev.check(lfg_dirichlet(dev.ctx, static_cast<int64_t>(cdofs.size()), cdofs.data(), cvals.data()));
     result.performance.boundary_ms = ms_since(boundary_start);
 
+    // block: subdomains of build_dof_subdomain (solver.cpp:631-654); one node per
+    // subdomain is the 2x2 nodal form fused into the iteration
+    const bool subdomains = s.preconditioner == "block" && s.subdomain_nodes != 1;
+    std::vector<int32_t> node_sub;
+    int64_t n_sub = 0;
+    if (subdomains) {
+        node_sub.resize(static_cast<std::size_t>(n_nodes));
+        if (lfh_subdomain_split(n_nodes, h.xy.data(), s.subdomain_split == "chunk" ? 1 : 0,
+                                static_cast<int64_t>(s.subdomain_nodes), node_sub.data(), &n_sub) != 0)
+            throw std::invalid_argument("invalid subdomain split");
+        dev.check(lfg_set_subdomains(dev.ctx, n_sub, node_sub.data()));
+    }
     lfg_pcg_options opt{};
-    opt.preconditioner = s.preconditioner == "ic0" ? LFG_PRECOND_IC0
+    opt.preconditioner = s.preconditioner == "ic0"     ? LFG_PRECOND_IC0
+                         : subdomains                  ? LFG_PRECOND_SUBDOMAIN
                          : s.preconditioner == "block" ? LFG_PRECOND_BLOCK2
                                                        : LFG_PRECOND_JACOBI;
+    opt.storage = s.storage == "bsr" ? LFG_STORAGE_BSR : LFG_STORAGE_CSR;
     opt.precision = s.precision == "mixed" ? LFG_PRECISION_MIXED : LFG_PRECISION_FP64;
     opt.stop_rule = s.stop_rule == "relative" ? LFG_STOP_RELATIVE : LFG_STOP_REFERENCE;
     opt.tolerance = s.tolerance;
@@ -286,14 +298,35 @@ AnalysisResult LinearElasticSolver::solve(Model& model) const {
     sr.stop_rule = s.stop_rule;
     sr.true_residual = rep.true_residual;
     sr.residual_replacements = static_cast<std::size_t>(rep.residual_replacements);
-    sr.matrix_bytes_per_matvec = static_cast<std::size_t>(nnz) * ((s.precision == "mixed" ? 4 : 8) + 4);
+    if (s.storage == "bsr") {
+        // 2x2 blocks of the block storage (BsrMatrix::from_csr, sparse.cpp:128-174): the
+        // union of the block columns of rows 2I and 2I+1, 4 doubles and one uint32 each
+        std::size_t blocks = 0;
+        for (int64_t br = 0; br < n / 2; ++br) {
+            int32_t p0 = pat.row_ptr[2 * br], p1 = pat.row_ptr[2 * br + 1];
+            const int32_t e0 = pat.row_ptr[2 * br + 1], e1 = pat.row_ptr[2 * br + 2];
+            while (p0 < e0 || p1 < e1) {
+                const int32_t c0 = p0 < e0 ? pat.col[p0] / 2 : INT32_MAX, c1 = p1 < e1 ? pat.col[p1] / 2 : INT32_MAX;
+                const int32_t bc = std::min(c0, c1);
+                while (p0 < e0 && pat.col[p0] / 2 == bc) ++p0;
+                while (p1 < e1 && pat.col[p1] / 2 == bc) ++p1;
+                ++blocks;
+            }
+        }
+        sr.matrix_bytes_per_matvec = blocks * (4 * sizeof(double) + sizeof(uint32_t));
+    } else {
+        sr.matrix_bytes_per_matvec = static_cast<std::size_t>(nnz) * ((s.precision == "mixed" ? 4 : 8) + 4);
+    }
     if (s.preconditioner == "block") {
-        sr.subdomains = static_cast<std::size_t>(n_nodes);
-        // one node per subdomain: the share of nonzeros inside the 2x2 nodal blocks
+        // subdomain_capture_ratio (precond.cpp:107-119): the share of nonzeros whose row
+        // and column lie in one subdomain
+        sr.subdomains = subdomains ? static_cast<std::size_t>(n_sub) : static_cast<std::size_t>(n_nodes);
         std::size_t inside = 0;
         for (int64_t row = 0; row < n; ++row)
-            for (int32_t k = pat.row_ptr[row]; k < pat.row_ptr[row + 1]; ++k)
-                if (pat.col[k] / 2 == row / 2) ++inside;
+            for (int32_t k = pat.row_ptr[row]; k < pat.row_ptr[row + 1]; ++k) {
+                const int64_t a = row / 2, b = pat.col[k] / 2;
+                if (subdomains ? node_sub[a] == node_sub[b] : a == b) ++inside;
+            }
         sr.capture_ratio = nnz ? static_cast<double>(inside) / static_cast<double>(nnz) : 0.0;
     }
     if (rc == LFG_ERR_NOT_SPD) throw std::runtime_error("PCG encountered non-positive curvature; system is not SPD");

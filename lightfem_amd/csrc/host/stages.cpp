@@ -222,6 +222,70 @@ int lfh_element_colors(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, 
     return 0;
 }
 
+int lfh_subdomain_split(int64_t n_nodes, const double* xy, int32_t split, int64_t nodes_per_subdomain,
+                        int32_t* node_subdomain, int64_t* n_subdomains_out) {
+    if (n_nodes < 0 || (n_nodes > 0 && (!xy || !node_subdomain)) || !n_subdomains_out || (split != 0 && split != 1))
+        return 1;
+    const int64_t per = nodes_per_subdomain > 1 ? nodes_per_subdomain : 1;
+    if (split == 1) {  // chunk: consecutive node indices
+        for (int64_t i = 0; i < n_nodes; ++i) node_subdomain[i] = static_cast<int32_t>(i / per);
+        *n_subdomains_out = n_nodes == 0 ? 0 : (n_nodes + per - 1) / per;
+        return 0;
+    }
+    *n_subdomains_out = 0;
+    if (n_nodes == 0) return 0;
+    // Recursive coordinate bisection to exactly ceil(n / per) parts: the largest
+    // remaining range of the node permutation is cut at its median along the longer
+    // side of its bounding box. Which of several nodes with equal coordinates lands
+    // on each side, and the final numbering (ranges in heap storage order), follow
+    // from std::nth_element and the std heap operations applied in this order, so
+    // the split is the reference's node for node.
+    struct Span {
+        std::size_t lo, hi;
+    };
+    auto smaller = [](const Span& a, const Span& b) { return (a.hi - a.lo) < (b.hi - b.lo); };
+    std::vector<std::size_t> perm(static_cast<std::size_t>(n_nodes));
+    for (std::size_t i = 0; i < perm.size(); ++i) perm[i] = i;
+    const std::size_t parts = static_cast<std::size_t>((n_nodes + per - 1) / per);
+    std::vector<Span> spans{Span{0, perm.size()}};
+    for (std::size_t made = 1; made < parts; ++made) {
+        std::pop_heap(spans.begin(), spans.end(), smaller);
+        const Span cut = spans.back();
+        spans.pop_back();
+        const std::size_t len = cut.hi - cut.lo;
+        if (len < 2) {  // only single nodes are left
+            spans.push_back(cut);
+            std::push_heap(spans.begin(), spans.end(), smaller);
+            break;
+        }
+        double x0 = xy[2 * perm[cut.lo]], x1 = x0, y0 = xy[2 * perm[cut.lo] + 1], y1 = y0;
+        for (std::size_t k = cut.lo + 1; k < cut.hi; ++k) {
+            const double x = xy[2 * perm[k]], y = xy[2 * perm[k] + 1];
+            x0 = std::min(x0, x);
+            x1 = std::max(x1, x);
+            y0 = std::min(y0, y);
+            y1 = std::max(y1, y);
+        }
+        const int axis = (x1 - x0) >= (y1 - y0) ? 0 : 1;
+        const std::size_t mid = cut.lo + len / 2;
+        std::nth_element(perm.begin() + static_cast<std::ptrdiff_t>(cut.lo),
+                         perm.begin() + static_cast<std::ptrdiff_t>(mid),
+                         perm.begin() + static_cast<std::ptrdiff_t>(cut.hi),
+                         [&](std::size_t a, std::size_t b) { return xy[2 * a + axis] < xy[2 * b + axis]; });
+        spans.push_back(Span{cut.lo, mid});
+        std::push_heap(spans.begin(), spans.end(), smaller);
+        spans.push_back(Span{mid, cut.hi});
+        std::push_heap(spans.begin(), spans.end(), smaller);
+    }
+    int32_t id = 0;
+    for (const Span& sp : spans) {
+        for (std::size_t k = sp.lo; k < sp.hi; ++k) node_subdomain[perm[k]] = id;
+        ++id;
+    }
+    *n_subdomains_out = id;
+    return 0;
+}
+
 int lfh_csr_profile(int64_t n_rows, const int32_t* row_ptr, const int32_t* col, int64_t out[3]) {
     if (n_rows < 0 || !row_ptr || !col || !out) return 1;
     int64_t bandwidth = 0, profile = 0;

@@ -19,7 +19,10 @@ HOST_LIB_PATH = os.path.join(LIB_DIR, "liblightfem.so")
 OK, ERR_ARG, ERR_JACOBIAN, ERR_NOT_SPD, ERR_NONFINITE, ERR_NOT_CONVERGED, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED = range(9)
 PLANE_STRESS, PLANE_STRAIN = 0, 1
 ASSEMBLY = {"serial": 0, "atomic": 1, "coloring": 2, "thread_local": 3}
-PRECOND = {"jacobi": 0, "block": 1, "block2": 1, "ic0": 2}
+PRECOND = {"jacobi": 0, "block": 1, "block2": 1, "ic0": 2, "subdomain": 3}
+STORAGE = {"csr": 0, "bsr": 1}
+SPMV_BLOCK_ORDER = 256
+SPLIT = {"rcb": 0, "chunk": 1}
 PRECISION = {"fp64": 0, "mixed": 1}
 STOP = {"reference": 0, "relative": 1}
 
@@ -28,8 +31,9 @@ GPU_SYMBOLS = ["lfg_create", "lfg_destroy", "lfg_last_error", "lfg_status_string
                "lfg_set_load", "lfg_dirichlet", "lfg_get_system", "lfg_pcg", "lfg_reactions", "lfg_postprocess",
                "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv", "lfg_bench_copy", "lfg_set_materials",
                "lfg_adjoint_solve", "lfg_sensitivity", "lfg_set_material_modes", "lfg_batch_set_problems",
-               "lfg_batch_set_loads", "lfg_batch_pcg", "lfg_layout_info_get"]
-HOST_SYMBOLS = ["lfh_csr_pattern_nnz", "lfh_csr_pattern", "lfh_rcm", "lfh_element_colors", "lfh_csr_profile"]
+               "lfg_batch_set_loads", "lfg_batch_pcg", "lfg_layout_info_get", "lfg_set_subdomains"]
+HOST_SYMBOLS = ["lfh_csr_pattern_nnz", "lfh_csr_pattern", "lfh_rcm", "lfh_element_colors", "lfh_csr_profile",
+                "lfh_subdomain_split"]
 
 
 class LightFemError(RuntimeError):
@@ -45,7 +49,7 @@ class PcgOptions(ctypes.Structure):
                 ("check_every", ctypes.c_int32), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int64),
                 ("replace_drop", ctypes.c_double), ("mixed_switch", ctypes.c_double),
                 ("direct_launch", ctypes.c_int32),
-                ("spmv_variant", ctypes.c_int32)]
+                ("spmv_variant", ctypes.c_int32), ("storage", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class PcgReport(ctypes.Structure):
@@ -150,6 +154,7 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lfg_batch_set_loads.argtypes = [_P, ctypes.c_int32, _P, _I64, _P, _P]
         lib.lfg_batch_pcg.argtypes = [_P, ctypes.POINTER(BatchOptions), _P, _P, ctypes.POINTER(BatchSummary)]
         lib.lfg_layout_info_get.argtypes = [_P, ctypes.POINTER(LayoutInfo)]
+        lib.lfg_set_subdomains.argtypes = [_P, _I64, _P]
         _gpu = lib
     return _gpu
 
@@ -165,6 +170,7 @@ def host_lib() -> ctypes.CDLL:
         lib.lfh_rcm.argtypes = [_I64, _P, _I64, _P, _P]
         lib.lfh_element_colors.argtypes = [_I64, _I64, _P, _P, _P, _P, _P]
         lib.lfh_csr_profile.argtypes = [_I64, _P, _P, _P]
+        lib.lfh_subdomain_split.argtypes = [_I64, _P, ctypes.c_int32, _I64, _P, _P]
         _host = lib
     return _host
 
@@ -227,6 +233,18 @@ def csr_profile(row_ptr: np.ndarray, col: np.ndarray) -> dict:
     out = np.empty(3, dtype=np.int64)
     host_lib().lfh_csr_profile(row_ptr.shape[0] - 1, _ptr(row_ptr), _ptr(col), _ptr(out))
     return {"nonzeros": int(out[0]), "bandwidth": int(out[1]), "profile": int(out[2])}
+
+
+def subdomain_split(xy: np.ndarray, split: str = "rcb", nodes_per_subdomain: int = 64):
+    """build_dof_subdomain's node split (solver.cpp:631-654): (node -> subdomain, count)
+    for node positions xy (n_nodes x 2, node-index order)."""
+    xy = _f64(xy).reshape(-1, 2)
+    out = np.empty(xy.shape[0], dtype=np.int32)
+    count = ctypes.c_int64(0)
+    if host_lib().lfh_subdomain_split(xy.shape[0], _ptr(xy), SPLIT[split], nodes_per_subdomain, _ptr(out),
+                                      ctypes.byref(count)):
+        raise LightFemError(ERR_ARG, "invalid subdomain split")
+    return out, int(count.value)
 
 
 # ------------------------------------------------------------- device stages
@@ -368,9 +386,10 @@ class Device:
     def pcg(self, preconditioner: str = "jacobi", precision: str = "fp64", stop_rule: str = "reference",
             tolerance: float = 1e-10, max_iterations: int = 100000, check_every: int = 0,
             replace_drop: float = 0.0, direct_launch: bool = False, spmv_variant: int = 0,
-            raise_on_error: bool = True, mixed_switch: float = 0.0):
+            raise_on_error: bool = True, mixed_switch: float = 0.0, storage: str = "csr"):
         opt = PcgOptions(PRECOND[preconditioner], PRECISION[precision], STOP[stop_rule], check_every, tolerance,
-                         max_iterations, replace_drop, mixed_switch, int(direct_launch), spmv_variant)
+                         max_iterations, replace_drop, mixed_switch, int(direct_launch), spmv_variant,
+                         STORAGE[storage], 0)
         rep = PcgReport()
         u = np.empty(self.n)
         rc = self._lib.lfg_pcg(self._ctx, ctypes.byref(opt), _ptr(u), ctypes.byref(rep))
@@ -380,11 +399,11 @@ class Device:
 
     def adjoint_solve(self, dJdu, preconditioner: str = "jacobi", precision: str = "fp64",
                       stop_rule: str = "reference", tolerance: float = 1e-10, max_iterations: int = 100000,
-                      check_every: int = 0, spmv_variant: int = 0):
+                      check_every: int = 0, spmv_variant: int = 0, storage: str = "csr"):
         """K lambda = dJ/du on the constrained system of the last forward solve."""
         dJdu = _f64(dJdu)
         opt = PcgOptions(PRECOND[preconditioner], PRECISION[precision], STOP[stop_rule], check_every, tolerance,
-                         max_iterations, 0.0, 0.0, 0, spmv_variant)
+                         max_iterations, 0.0, 0.0, 0, spmv_variant, STORAGE[storage], 0)
         rep = PcgReport()
         lam = np.empty(self.n)
         self._check(self._lib.lfg_adjoint_solve(self._ctx, ctypes.byref(opt), _ptr(dJdu), _ptr(lam),
@@ -423,11 +442,18 @@ class Device:
         self._check(self._lib.lfg_layout_info_get(self._ctx, ctypes.byref(info)))
         return info.as_dict()
 
-    def spmv(self, x, precision: str = "fp64", variant: int = 0) -> np.ndarray:
+    def spmv(self, x, precision: str = "fp64", variant: int = 0, block_order: bool = False) -> np.ndarray:
+        """y = Kc x with the given kernel; block_order: the block storage summation order."""
         x = _f64(x)
         y = np.empty_like(x)
-        self._check(self._lib.lfg_spmv(self._ctx, PRECISION[precision], variant, _ptr(x), _ptr(y)))
+        v = variant | (SPMV_BLOCK_ORDER if block_order else 0)
+        self._check(self._lib.lfg_spmv(self._ctx, PRECISION[precision], v, _ptr(x), _ptr(y)))
         return y
+
+    def set_subdomains(self, node_subdomain, n_subdomains: int) -> None:
+        """Subdomains of the block preconditioner (LFG_PRECOND_SUBDOMAIN), per node index."""
+        ns = _i32(node_subdomain)
+        self._check(self._lib.lfg_set_subdomains(self._ctx, n_subdomains, _ptr(ns)))
 
     def bench_copy(self, nbytes: int, reps: int = 20) -> float:
         ms = ctypes.c_double(0.0)

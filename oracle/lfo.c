@@ -467,6 +467,50 @@ void lfo_spmv(int64_t n, const int64_t* row_ptr, const int64_t* col, const doubl
     }
 }
 
+/* sparse.cpp:128-198: the block storage product. Block row I merges the sorted
+ * columns of rows 2I and 2I+1 into 2x2 blocks (zero where a row lacks the column)
+ * and sums each block's two products before adding them to the row total. */
+void lfo_bsr_spmv(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values, const double* x,
+                  double* y) {
+    for (int64_t br = 0; br < n / 2; ++br) {
+        const int64_t r0 = 2 * br, r1 = r0 + 1;
+        int64_t p0 = row_ptr[r0], p1 = row_ptr[r1];
+        const int64_t e0 = row_ptr[r0 + 1], e1 = row_ptr[r1 + 1];
+        double sum0 = 0.0, sum1 = 0.0;
+        while (p0 < e0 || p1 < e1) {
+            const int64_t c0 = p0 < e0 ? col[p0] : INT64_MAX;
+            const int64_t c1 = p1 < e1 ? col[p1] : INT64_MAX;
+            const int64_t bc = (c0 < c1 ? c0 : c1) / 2;
+            double blk[4] = {0.0, 0.0, 0.0, 0.0};
+            for (; p0 < e0 && col[p0] / 2 == bc; ++p0) blk[col[p0] - 2 * bc] = values[p0];
+            for (; p1 < e1 && col[p1] / 2 == bc; ++p1) blk[2 + col[p1] - 2 * bc] = values[p1];
+            const double x0 = x[2 * bc], x1 = x[2 * bc + 1];
+            sum0 += blk[0] * x0 + blk[1] * x1;
+            sum1 += blk[2] * x0 + blk[3] * x1;
+        }
+        y[r0] = sum0;
+        y[r1] = sum1;
+    }
+}
+
+/* Block count of lfo_bsr_spmv's storage (the reference's block_nonzeros()). */
+int64_t lfo_bsr_blocks(int64_t n, const int64_t* row_ptr, const int64_t* col) {
+    int64_t blocks = 0;
+    for (int64_t br = 0; br < n / 2; ++br) {
+        int64_t p0 = row_ptr[2 * br], p1 = row_ptr[2 * br + 1];
+        const int64_t e0 = row_ptr[2 * br + 1], e1 = row_ptr[2 * br + 2];
+        while (p0 < e0 || p1 < e1) {
+            const int64_t c0 = p0 < e0 ? col[p0] : INT64_MAX;
+            const int64_t c1 = p1 < e1 ? col[p1] : INT64_MAX;
+            const int64_t bc = (c0 < c1 ? c0 : c1) / 2;
+            while (p0 < e0 && col[p0] / 2 == bc) ++p0;
+            while (p1 < e1 && col[p1] / 2 == bc) ++p1;
+            ++blocks;
+        }
+    }
+    return blocks;
+}
+
 /* -------------------------------------------------------- preconditioners */
 
 int64_t lfo_ic0_factor_size(int64_t n, const int64_t* row_ptr, const int64_t* col) {
@@ -616,6 +660,127 @@ void lfo_block1_apply(int64_t n, const double* factor, const double* in, double*
     }
 }
 
+/* precond.cpp:19-28: node i belongs to subdomain i / size. Returns the count. */
+int64_t lfo_split_chunks(int64_t n_nodes, int64_t nodes_per_subdomain, int64_t* node_subdomain) {
+    const int64_t size = nodes_per_subdomain > 1 ? nodes_per_subdomain : 1;
+    for (int64_t i = 0; i < n_nodes; ++i) node_subdomain[i] = i / size;
+    return n_nodes == 0 ? 0 : (n_nodes + size - 1) / size;
+}
+
+/* SubdomainBlockJacobi (precond.cpp:121-232): dofs of each subdomain in matrix
+ * order, one dense Cholesky per subdomain (row by row, pivots stored as their
+ * reciprocals in a packed lower triangle; a failed block becomes the identity). */
+struct lfo_block {
+    int64_t n, n_sub, largest;
+    int64_t* dof_begin;    /* n_sub + 1 */
+    int64_t* dofs;         /* n */
+    int64_t* factor_begin; /* n_sub + 1 */
+    double* factor;
+    int64_t degenerate;
+    double capture_ratio;
+};
+
+lfo_block* lfo_block_create(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values,
+                            const int64_t* dof_subdomain, int64_t n_sub) {
+    lfo_block* b = calloc(1, sizeof(lfo_block));
+    b->n = n;
+    b->n_sub = n_sub;
+    b->dof_begin = calloc((size_t)n_sub + 1, sizeof(int64_t));
+    b->dofs = malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    b->factor_begin = calloc((size_t)n_sub + 1, sizeof(int64_t));
+    int64_t interior = 0;
+    for (int64_t row = 0; row < n; ++row)
+        for (int64_t k = row_ptr[row]; k < row_ptr[row + 1]; ++k)
+            if (dof_subdomain[col[k]] == dof_subdomain[row]) ++interior;
+    b->capture_ratio = row_ptr[n] ? (double)interior / (double)row_ptr[n] : 0.0; /* precond.cpp:107-119 */
+    for (int64_t d = 0; d < n; ++d) b->dof_begin[dof_subdomain[d] + 1]++;
+    for (int64_t s = 0; s < n_sub; ++s) b->dof_begin[s + 1] += b->dof_begin[s];
+    int64_t* cursor = malloc((size_t)(n_sub > 0 ? n_sub : 1) * sizeof(int64_t));
+    for (int64_t s = 0; s < n_sub; ++s) cursor[s] = b->dof_begin[s];
+    for (int64_t d = 0; d < n; ++d) b->dofs[cursor[dof_subdomain[d]]++] = d;
+    free(cursor);
+    int64_t* local = malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    for (int64_t s = 0; s < n_sub; ++s) {
+        const int64_t size = b->dof_begin[s + 1] - b->dof_begin[s];
+        for (int64_t i = 0; i < size; ++i) local[b->dofs[b->dof_begin[s] + i]] = i;
+        b->factor_begin[s + 1] = b->factor_begin[s] + size * (size + 1) / 2;
+        if (size > b->largest) b->largest = size;
+    }
+    b->factor = calloc((size_t)b->factor_begin[n_sub] + 1, sizeof(double));
+    double* dense = malloc((size_t)(b->largest * b->largest + 1) * sizeof(double));
+    for (int64_t s = 0; s < n_sub; ++s) {
+        const int64_t begin = b->dof_begin[s], size = b->dof_begin[s + 1] - begin;
+        if (size == 0) continue;
+        memset(dense, 0, (size_t)(size * size) * sizeof(double));
+        for (int64_t i = 0; i < size; ++i) {
+            const int64_t dof = b->dofs[begin + i];
+            for (int64_t k = row_ptr[dof]; k < row_ptr[dof + 1]; ++k)
+                if (dof_subdomain[col[k]] == s) dense[i * size + local[col[k]]] = values[k];
+        }
+        int ok = 1;
+        for (int64_t i = 0; i < size && ok; ++i)
+            for (int64_t j = 0; j <= i; ++j) {
+                double sum = dense[i * size + j];
+                for (int64_t k = 0; k < j; ++k) sum -= dense[i * size + k] * dense[j * size + k];
+                if (i == j) {
+                    if (!(sum > 0.0) || !isfinite(sum)) { ok = 0; break; }
+                    dense[i * size + i] = sqrt(sum);
+                } else {
+                    dense[i * size + j] = sum / dense[j * size + j];
+                }
+            }
+        if (!ok) {
+            ++b->degenerate;
+            for (int64_t i = 0; i < size; ++i)
+                for (int64_t j = 0; j <= i; ++j) dense[i * size + j] = i == j ? 1.0 : 0.0;
+        }
+        double* t = b->factor + b->factor_begin[s];
+        for (int64_t i = 0; i < size; ++i) {
+            for (int64_t j = 0; j <= i; ++j) t[i * (i + 1) / 2 + j] = dense[i * size + j];
+            t[i * (i + 1) / 2 + i] = 1.0 / t[i * (i + 1) / 2 + i];
+        }
+    }
+    free(dense);
+    free(local);
+    return b;
+}
+
+/* precond.cpp:234-281: gather, forward substitution with L, back substitution with
+ * L^T (each sum in the reference's order), scatter. */
+void lfo_block_apply(const lfo_block* b, const double* in, double* out) {
+    double* w = malloc((size_t)(b->largest + 1) * sizeof(double));
+    for (int64_t s = 0; s < b->n_sub; ++s) {
+        const int64_t begin = b->dof_begin[s], size = b->dof_begin[s + 1] - begin;
+        if (size == 0) continue;
+        const double* f = b->factor + b->factor_begin[s];
+        for (int64_t i = 0; i < size; ++i) w[i] = in[b->dofs[begin + i]];
+        for (int64_t i = 0, base = 0; i < size; base += i + 1, ++i) {
+            double sum = w[i];
+            for (int64_t k = 0; k < i; ++k) sum -= f[base + k] * w[k];
+            w[i] = sum * f[base + i];
+        }
+        for (int64_t i = size; i-- > 0;) {
+            double sum = w[i];
+            for (int64_t k = size - 1; k > i; --k) sum -= f[k * (k + 1) / 2 + i] * w[k];
+            w[i] = sum * f[i * (i + 1) / 2 + i];
+        }
+        for (int64_t i = 0; i < size; ++i) out[b->dofs[begin + i]] = w[i];
+    }
+    free(w);
+}
+
+void lfo_block_stats(const lfo_block* b, double out[3]) {
+    out[0] = (double)b->degenerate;
+    out[1] = (double)b->factor_begin[b->n_sub];
+    out[2] = b->capture_ratio;
+}
+
+void lfo_block_free(lfo_block* b) {
+    if (!b) return;
+    free(b->dof_begin); free(b->dofs); free(b->factor_begin); free(b->factor);
+    free(b);
+}
+
 /* ------------------------------------------------------------------- PCG */
 
 typedef struct {
@@ -624,6 +789,7 @@ typedef struct {
     const double* inv_diag; /* jacobi */
     double* factor;         /* block1 */
     int64_t* lrow; int64_t* lcol; double* lval; double* ic_inv; /* ic0 */
+    lfo_block* block;       /* subdomain block-Jacobi */
 } precond_t;
 
 static void precond_apply(const precond_t* m, const double* in, double* out) {
@@ -631,15 +797,25 @@ static void precond_apply(const precond_t* m, const double* in, double* out) {
         for (int64_t i = 0; i < m->n; ++i) out[i] = m->inv_diag[i] * in[i]; /* solver.cpp:502-505 */
     } else if (m->kind == LFO_PRECOND_BLOCK1) {
         lfo_block1_apply(m->n, m->factor, in, out);
+    } else if (m->kind == LFO_PRECOND_BLOCK) {
+        lfo_block_apply(m->block, in, out);
     } else {
         lfo_ic0_apply(m->n, m->lrow, m->lcol, m->lval, m->ic_inv, in, out);
     }
 }
 
-/* solver.cpp:532-627 with the preconditioners of solve_iterative (:675-747). */
 int lfo_pcg(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values, const double* rhs,
             int precond, int stop_rule, double tolerance, int64_t max_iterations, double* x,
             lfo_pcg_report* report) {
+    return lfo_pcg_ex(n, row_ptr, col, values, rhs, precond, LFO_STORAGE_CSR, NULL, 0, stop_rule, tolerance,
+                      max_iterations, x, report);
+}
+
+/* solver.cpp:532-627 with the preconditioners and storage of solve_iterative
+ * (:675-747); dof_subdomain / n_sub for LFO_PRECOND_BLOCK. */
+int lfo_pcg_ex(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values, const double* rhs,
+               int precond, int storage, const int64_t* dof_subdomain, int64_t n_sub, int stop_rule,
+               double tolerance, int64_t max_iterations, double* x, lfo_pcg_report* report) {
     memset(report, 0, sizeof(*report));
     precond_t m;
     memset(&m, 0, sizeof(m));
@@ -656,6 +832,8 @@ int lfo_pcg(int64_t n, const int64_t* row_ptr, const int64_t* col, const double*
     } else if (precond == LFO_PRECOND_BLOCK1) {
         m.factor = malloc((size_t)(3 * (n / 2) + 1) * sizeof(double));
         lfo_block1_factor(n, row_ptr, col, values, m.factor);
+    } else if (precond == LFO_PRECOND_BLOCK) {
+        m.block = lfo_block_create(n, row_ptr, col, values, dof_subdomain, n_sub);
     } else {
         const int64_t size = lfo_ic0_factor_size(n, row_ptr, col);
         m.lrow = malloc(((size_t)n + 1) * sizeof(int64_t));
@@ -690,7 +868,8 @@ int lfo_pcg(int64_t n, const int64_t* row_ptr, const int64_t* col, const double*
     }
     if (!(rho > 0.0) || !isfinite(rho)) { status = 1; goto done; }
     for (int64_t it = 1; it <= max_iterations; ++it) {
-        lfo_spmv(n, row_ptr, col, values, p, q);
+        if (storage == LFO_STORAGE_BSR) lfo_bsr_spmv(n, row_ptr, col, values, p, q);
+        else lfo_spmv(n, row_ptr, col, values, p, q);
         double curvature = 0.0;
         for (int64_t i = 0; i < n; ++i) curvature += p[i] * q[i];
         if (!(curvature > 0.0) || !isfinite(curvature)) { status = 2; goto done; }
@@ -718,6 +897,7 @@ done:
     report->status = status;
     free(r); free(z); free(p); free(q); free(inv_diag);
     free(m.factor); free(m.lrow); free(m.lcol); free(m.lval); free(m.ic_inv);
+    lfo_block_free(m.block);
     return status;
 }
 

@@ -19,6 +19,11 @@
  *   lfo_dirichlet          golden Kc/fc (bit-exact), dense brute-force restatement
  *   lfo_pcg                golden u and iteration counts (bit-exact), scipy direct solve
  *   lfo_ic0_* / block      golden preconditioner applications (bit-exact)
+ *   lfo_bsr_spmv           golden block-storage products (bit-exact) and golden
+ *                          BSR-storage PCG solutions and iteration counts
+ *   lfo_split_chunks       golden chunk subdomain maps (exact)
+ *   lfo_block_*            golden subdomain block-Jacobi applications (bit-exact),
+ *                          golden block-preconditioned solutions and iteration counts
  *   lfo_smooth_nodal       golden nodal fields (bit-exact)
  */
 #ifndef LFO_H
@@ -33,6 +38,8 @@ extern "C" {
 enum { LFO_PLANE_STRESS = 0, LFO_PLANE_STRAIN = 1 };
 enum { LFO_PRECOND_JACOBI = 0, LFO_PRECOND_BLOCK1 = 1, LFO_PRECOND_IC0 = 2 };
 enum { LFO_STOP_REFERENCE = 0, LFO_STOP_RELATIVE = 1 };
+enum { LFO_PRECOND_BLOCK = 3 };  /* SubdomainBlockJacobi on a given dof -> subdomain map */
+enum { LFO_STORAGE_CSR = 0, LFO_STORAGE_BSR = 1 };
 
 /* 0 on success, 1 on a non-positive Jacobian. */
 int lfo_q4_stiffness(const double xy[8], double young, double nu, double thickness, int plane_mode,
@@ -68,6 +75,21 @@ int64_t lfo_block1_factor(int64_t n, const int64_t* row_ptr, const int64_t* col,
                           double* factor);
 void lfo_block1_apply(int64_t n, const double* factor, const double* in, double* out);
 
+/* Block storage product (BsrMatrix::from_csr + multiply_into, sparse.cpp:128-198). */
+void lfo_bsr_spmv(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values, const double* x,
+                  double* y);
+int64_t lfo_bsr_blocks(int64_t n, const int64_t* row_ptr, const int64_t* col);
+
+/* Subdomains: chunk split (precond.cpp:19-28); the coordinate bisection is the host
+ * library's (pinned by the golden maps). SubdomainBlockJacobi on any dof map. */
+int64_t lfo_split_chunks(int64_t n_nodes, int64_t nodes_per_subdomain, int64_t* node_subdomain);
+typedef struct lfo_block lfo_block;
+lfo_block* lfo_block_create(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values,
+                            const int64_t* dof_subdomain, int64_t n_sub);
+void lfo_block_apply(const lfo_block* b, const double* in, double* out);
+void lfo_block_stats(const lfo_block* b, double out[3]); /* degenerate, factor entries, capture ratio */
+void lfo_block_free(lfo_block* b);
+
 typedef struct {
     int64_t iterations;
     int32_t converged;
@@ -79,6 +101,9 @@ typedef struct {
 int lfo_pcg(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values, const double* rhs,
             int precond, int stop_rule, double tolerance, int64_t max_iterations, double* x,
             lfo_pcg_report* report);
+int lfo_pcg_ex(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values, const double* rhs,
+               int precond, int storage, const int64_t* dof_subdomain, int64_t n_sub, int stop_rule,
+               double tolerance, int64_t max_iterations, double* x, lfo_pcg_report* report);
 
 /* Adjoint element sensitivities -lambda_e^T Ke(E=1) u_e * dE_e (no reference
  * counterpart; the derivative of q4_stiffness in E). props (E, nu, t) per element. */

@@ -16,8 +16,9 @@ _LIB_PATH = os.path.join(_HERE, "build", "liblfo.so")
 _lib = None
 
 PLANE_STRESS, PLANE_STRAIN = 0, 1
-JACOBI, BLOCK1, IC0 = 0, 1, 2
+JACOBI, BLOCK1, IC0, BLOCK = 0, 1, 2, 3
 STOP_REFERENCE, STOP_RELATIVE = 0, 1
+STORAGE_CSR, STORAGE_BSR = 0, 1
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int64
@@ -40,6 +41,12 @@ def lib() -> ctypes.CDLL:
         _lib.lfo_ic0_factor.restype = ctypes.c_int64
         _lib.lfo_block1_factor.restype = ctypes.c_int64
         _lib.lfo_sensitivity.restype = ctypes.c_int64
+        _lib.lfo_bsr_blocks.restype = ctypes.c_int64
+        _lib.lfo_split_chunks.restype = ctypes.c_int64
+        _lib.lfo_block_create.restype = ctypes.c_void_p
+        _lib.lfo_block_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _lib.lfo_block_stats.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _lib.lfo_block_free.argtypes = [ctypes.c_void_p]
     return _lib
 
 
@@ -187,13 +194,62 @@ def block1_apply(factor, vec) -> np.ndarray:
     return out
 
 
+def bsr_spmv(row_ptr, col, values, x) -> np.ndarray:
+    """Block storage product (sparse.cpp:128-198)."""
+    row_ptr, col, values, x = _i64(row_ptr), _i64(col), _f64(values), _f64(x)
+    y = np.empty(row_ptr.shape[0] - 1)
+    lib().lfo_bsr_spmv(_I(y.shape[0]), _p(row_ptr), _p(col), _p(values), _p(x), _p(y))
+    return y
+
+
+def bsr_blocks(row_ptr, col) -> int:
+    row_ptr, col = _i64(row_ptr), _i64(col)
+    return int(lib().lfo_bsr_blocks(_I(row_ptr.shape[0] - 1), _p(row_ptr), _p(col)))
+
+
+def split_chunks(n_nodes: int, nodes_per_subdomain: int):
+    """Chunk subdomains (precond.cpp:19-28): (node -> subdomain, count)."""
+    out = np.empty(n_nodes, dtype=np.int64)
+    count = lib().lfo_split_chunks(_I(n_nodes), _I(nodes_per_subdomain), _p(out))
+    return out, int(count)
+
+
+class BlockJacobi:
+    """SubdomainBlockJacobi (precond.cpp:121-281) on a dof -> subdomain map."""
+
+    def __init__(self, row_ptr, col, values, dof_subdomain, n_sub: int):
+        self._keep = (_i64(row_ptr), _i64(col), _f64(values), _i64(dof_subdomain))
+        rp, cl, vals, dsub = self._keep
+        self.n = rp.shape[0] - 1
+        self._h = lib().lfo_block_create(_I(self.n), _p(rp), _p(cl), _p(vals), _p(dsub), _I(n_sub))
+        st = np.empty(3)
+        lib().lfo_block_stats(self._h, _p(st))
+        self.degenerate, self.factor_entries, self.capture_ratio = int(st[0]), int(st[1]), float(st[2])
+
+    def apply(self, vec) -> np.ndarray:
+        vec = _f64(vec)
+        out = np.empty_like(vec)
+        lib().lfo_block_apply(self._h, _p(vec), _p(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lfo_block_free(self._h)
+            self._h = None
+
+
 def pcg(row_ptr, col, values, rhs, precond: int = JACOBI, stop_rule: int = STOP_REFERENCE,
-        tolerance: float = 1e-10, max_iterations: int = 100000):
+        tolerance: float = 1e-10, max_iterations: int = 100000, storage: int = 0, dof_subdomain=None,
+        n_sub: int = 0):
+    """pcg_solve (solver.cpp:532-627); precond BLOCK takes dof_subdomain / n_sub and
+    storage BSR multiplies with the block storage product."""
     row_ptr, col, values, rhs = _i64(row_ptr), _i64(col), _f64(values), _f64(rhs)
+    dsub = _i64(dof_subdomain if dof_subdomain is not None else np.zeros(1, dtype=np.int64))
     x = np.empty(rhs.shape[0])
     rep = PcgReport()
-    lib().lfo_pcg(_I(rhs.shape[0]), _p(row_ptr), _p(col), _p(values), _p(rhs), ctypes.c_int(precond),
-                  ctypes.c_int(stop_rule), ctypes.c_double(tolerance), _I(max_iterations), _p(x), ctypes.byref(rep))
+    lib().lfo_pcg_ex(_I(rhs.shape[0]), _p(row_ptr), _p(col), _p(values), _p(rhs), ctypes.c_int(precond),
+                     ctypes.c_int(storage), _p(dsub), _I(n_sub), ctypes.c_int(stop_rule),
+                     ctypes.c_double(tolerance), _I(max_iterations), _p(x), ctypes.byref(rep))
     return x, {"iterations": rep.iterations, "converged": bool(rep.converged), "status": rep.status,
                "initial_residual": rep.initial_residual, "final_residual": rep.final_residual}
 

@@ -207,10 +207,29 @@ int dump(const std::string& path, const std::string& dir, const std::vector<std:
         SubdomainBlockJacobi block(kc, dsub, count);
         block.apply(probe, z);
         npy_f8(dir, "block1_apply_probe", z);
+        // the block storage product (sparse.cpp:128-198) on the same probe
+        const BsrMatrix bsr = BsrMatrix::from_csr(kc);
+        npy_f8(dir, "bsr_probe", bsr.multiply(probe));
         std::ostringstream o;
         o << "{\"breakdowns\": " << ic0.stats().breakdowns << ", \"factor_entries\": "
           << ic0.stats().factor_entries << ", \"block1_capture_ratio\": " << block.stats().capture_ratio
-          << ", \"block1_degenerate\": " << block.stats().degenerate_subdomains << "}";
+          << ", \"block1_degenerate\": " << block.stats().degenerate_subdomains
+          << ", \"bsr_block_nonzeros\": " << bsr.block_nonzeros();
+        // the model's own subdomain block-Jacobi (solver.cpp:631-654, precond.cpp:19-291)
+        if (model.settings.preconditioner == "block") {
+            std::size_t n_sub = 0;
+            const std::vector<std::uint32_t> own =
+                build_dof_subdomain(model, model.settings.subdomain_split, model.settings.subdomain_nodes, n_sub);
+            std::vector<long long> own64(own.begin(), own.end());
+            npy_i8(dir, "dof_subdomain", own64);
+            SubdomainBlockJacobi sb(kc, own, n_sub);
+            sb.apply(probe, z);
+            npy_f8(dir, "block_apply_probe", z);
+            o << ", \"block_subdomains\": " << sb.stats().subdomains << ", \"block_capture_ratio\": "
+              << std::setprecision(17) << sb.stats().capture_ratio << ", \"block_degenerate\": "
+              << sb.stats().degenerate_subdomains << ", \"block_factor_nonzeros\": " << sb.stats().factor_nonzeros;
+        }
+        o << "}";
         std::ofstream(dir + "/precond.json") << o.str() << "\n";
     }
 
@@ -257,6 +276,9 @@ int dump(const std::string& path, const std::string& dir, const std::vector<std:
          << ", \"before\": " << json_profile(result.performance.before)
          << ", \"after\": " << json_profile(result.performance.after)
          << ", \"subdomains\": " << result.solver.subdomains
+         << ", \"matrix_bytes_per_matvec\": " << result.solver.matrix_bytes_per_matvec
+         << ", \"subdomain_split\": \"" << model.settings.subdomain_split << "\""
+         << ", \"subdomain_nodes\": " << model.settings.subdomain_nodes
          << ", \"capture_ratio\": " << result.solver.capture_ratio << "}\n";
     return 0;
 }

@@ -29,10 +29,19 @@ def load_golden(name: str) -> dict:
 
 FULL_CASES = ["one_element", "scrambled_none", "scrambled_rcm", "extended_loads", "grid6_jacobi", "grid6_ic0",
               "grid6_block1", "grid16_stress", "grid16_rcm_coloring", "grid16_rcm_ic0", "grid16_rcm_block1",
-              "grid16x12_strain_loge"]
+              "grid16x12_strain_loge", "grid6_bsr_jacobi", "grid6_bsr_ic0", "grid6_bsr_block_rcb2",
+              "grid6_bsr_block_chunk4", "grid16_rcm_block64", "grid16_block_chunk8"]
 
 PRECOND_OF = {"jacobi": 0, "block": 1, "ic0": 2}
 
 
 def case_precond(meta: dict) -> int:
+    """Oracle preconditioner code; block cases with more than one node per subdomain
+    use the general subdomain form (code 3) on the case's dof -> subdomain map."""
+    if meta["preconditioner"] == "block" and meta.get("subdomain_nodes", 1) != 1:
+        return 3
     return PRECOND_OF[meta["preconditioner"]]
+
+
+def case_storage(meta: dict) -> int:
+    return 1 if meta.get("storage") == "bsr" else 0

@@ -164,7 +164,8 @@ def digest(array: np.ndarray) -> str:
 
 FULL_KEYS = ["node_ids", "xy", "rcm_order", "row_ptr", "col", "edofs", "emap", "colors", "ke", "elem_material",
              "K", "f", "cdofs", "cvals", "Kc", "fc", "probe", "jacobi_invdiag", "ic0_apply_probe",
-             "block1_apply_probe", "u", "R", "elem_centre", "elem_gp", "node_fields"]
+             "block1_apply_probe", "bsr_probe", "dof_subdomain", "block_apply_probe", "u", "R", "elem_centre",
+             "elem_gp", "node_fields"]
 
 
 def save_full(name: str, arrays: dict, record: dict) -> None:
@@ -190,7 +191,19 @@ def save_digest(name: str, arrays: dict, record: dict, keep: tuple = ()) -> None
 
 
 CLI = os.path.join(ROOT, "oracle", "_ref", "lightfem_ref")
-CLI_MODELS = {"one_element": ONE_ELEMENT, "scrambled": SCRAMBLED, "extended": EXTENDED}
+
+
+def with_settings(text: str, lines: list) -> str:
+    """The model text with extra [settings] lines."""
+    return text.replace("[settings]\n", "[settings]\n" + "".join(f"{line}\n" for line in lines), 1)
+
+
+CLI_MODELS = {"one_element": ONE_ELEMENT, "scrambled": SCRAMBLED, "extended": EXTENDED,
+              # the storage / block settings of tests/test_solver.cpp:398-448 through the CLI
+              "grid6_bsr_block_rcb2": with_settings(grid_model_text(6, 6), [
+                  "storage bsr", "preconditioner block", "subdomain_split rcb", "subdomain_nodes 2"]),
+              "grid6_block_chunk4": with_settings(grid_model_text(6, 6), [
+                  "preconditioner block", "subdomain_split chunk", "subdomain_nodes 4"])}
 
 
 def cli_fixtures() -> dict:
@@ -233,6 +246,14 @@ def main() -> None:
             save_full(f"grid6_{precond}", run_dump(grid6, [f"preconditioner={precond}"], work, f"g6{precond}"),
                       record)
         save_full("grid6_block1", run_dump(grid6, ["preconditioner=block", "subdomain_nodes=1"], work, "g6b"), record)
+        # the storage / preconditioner matrix of tests/test_solver.cpp:398-448
+        for name, extra in (("grid6_bsr_jacobi", ["preconditioner=jacobi"]),
+                            ("grid6_bsr_ic0", ["preconditioner=ic0"]),
+                            ("grid6_bsr_block_rcb2", ["preconditioner=block", "subdomain_split=rcb",
+                                                      "subdomain_nodes=2"]),
+                            ("grid6_bsr_block_chunk4", ["preconditioner=block", "subdomain_split=chunk",
+                                                        "subdomain_nodes=4"])):
+            save_full(name, run_dump(grid6, ["storage=bsr", *extra], work, name), record)
 
         # 16x16 generator cases: plane stress E=1; RCM + coloring; plane strain log-uniform E.
         m = meshgen.structured_cantilever(16, 16)
@@ -244,6 +265,12 @@ def main() -> None:
         save_full("grid16_rcm_block1",
                   run_dump(text, ["reordering=rcm", "preconditioner=block", "subdomain_nodes=1"], work, "g16b"),
                   record)
+        # default block preconditioner (rcb, 64 nodes) and a chunk split, CSR storage
+        save_full("grid16_rcm_block64", run_dump(text, ["reordering=rcm", "preconditioner=block"], work, "g16b64"),
+                  record)
+        save_full("grid16_block_chunk8",
+                  run_dump(text, ["preconditioner=block", "subdomain_split=chunk", "subdomain_nodes=8"], work,
+                           "g16c8"), record)
         m = meshgen.structured_cantilever(16, 12, lx=16.0, ly=12.0,
                                           young=meshgen.loguniform_young(16 * 12, 7), plane_mode="plane_strain")
         m.to_fem(os.path.join(work, "g16e.fem"))
@@ -266,6 +293,11 @@ def main() -> None:
             arrays = run_dump(c2_text, ["reordering=rcm", "assembly=coloring", f"preconditioner={pname}", *extra],
                               work, f"c2{precond}")
             save_digest(f"c2_{precond}", arrays, record)
+        # 200x200 with the default block preconditioner (rcb, 64 nodes) and with block storage
+        arrays = run_dump(c2_text, ["reordering=rcm", "assembly=coloring", "preconditioner=block"], work, "c2b64")
+        save_digest("c2_block64", arrays, record)
+        arrays = run_dump(c2_text, ["reordering=rcm", "assembly=coloring", "storage=bsr"], work, "c2bsr")
+        save_digest("c2_bsr_jacobi", arrays, record)
     # CLI fixtures: `check` stdout and `solve` CSV tables of the reference CLI.
     record["cli"] = cli_fixtures()
     with open(os.path.join(HERE, "golden.json"), "w") as handle:

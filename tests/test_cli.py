@@ -17,7 +17,10 @@ def run(args, cwd):
     return subprocess.run([CLI, *args], capture_output=True, text=True, cwd=cwd, timeout=300)
 
 
-@pytest.mark.parametrize("name", ["one_element", "scrambled", "extended"])
+CLI_CASES = ["one_element", "scrambled", "extended", "grid6_bsr_block_rcb2", "grid6_block_chunk4"]
+
+
+@pytest.mark.parametrize("name", CLI_CASES)
 def test_check_is_identical(golden_index, tmp_path, name):
     ref = golden_index["cli"][name]
     fem = tmp_path / f"{name}.fem"
@@ -52,7 +55,7 @@ def _table(text):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["one_element", "scrambled", "extended"])
+@pytest.mark.parametrize("name", CLI_CASES)
 def test_solve_outputs_match_reference(golden_index, tmp_path, name):
     ref = golden_index["cli"][name]
     fem = tmp_path / f"{name}.fem"

@@ -12,13 +12,13 @@ import hashlib
 import numpy as np
 import pytest
 
-from conftest import FULL_CASES, case_precond, load_golden
+from conftest import FULL_CASES, case_precond, case_storage, load_golden
 from lightfem_amd import meshgen, native, pipeline
 from oracle import lfo
 
 pytestmark = pytest.mark.gpu
 
-PRECOND_NAME = {0: "jacobi", 1: "block", 2: "ic0"}
+PRECOND_NAME = {0: "jacobi", 1: "block", 2: "ic0", 3: "subdomain"}
 
 
 def bits(a):
@@ -104,9 +104,22 @@ def test_stages_match_reference(dev, golden_index, name):
     for variant in (4, 5):
         assert np.array_equal(bits(dev.spmv(g["probe"], "mixed", variant)), bits(y32_ref)), variant
     check_stencil_spmv(dev, g["row_ptr"], g["col"], g["Kc"], g["probe"])
-    # PCG with the case's preconditioner and the reference stop rule
+    # the block storage product (sparse.cpp:128-198): node-slice and exact node-stencil
+    # kernels summing each 2x2 block's products first
+    info = dev.layout_info()
+    if info["node_blocked"]:
+        for variant in (6, 7):
+            assert np.array_equal(bits(dev.spmv(g["probe"], "fp64", variant, block_order=True)), bits(g["bsr_probe"]))
+    if info["stencil_exact_planes"]:
+        assert np.array_equal(bits(dev.spmv(g["probe"], "fp64", 9, block_order=True)), bits(g["bsr_probe"]))
+    # the case's own subdomain block-Jacobi (any subdomain size)
+    if "dof_subdomain" in g:
+        dev.set_subdomains(g["dof_subdomain"][0::2], meta["subdomains"])
+        assert np.array_equal(bits(dev.precond_apply("subdomain", g["probe"])), bits(g["block_apply_probe"]))
+    # PCG with the case's preconditioner and storage and the reference stop rule
     precond = PRECOND_NAME[case_precond(meta)]
-    u, rep = dev.pcg(precond, "fp64", "reference", meta["tolerance"])
+    storage = "bsr" if case_storage(meta) else "csr"
+    u, rep = dev.pcg(precond, "fp64", "reference", meta["tolerance"], storage=storage)
     assert rep["converged"] == 1
     assert abs(rep["iterations"] - meta["iterations"]) <= 2 + 0.01 * meta["iterations"]
     assert rep["initial_residual"] == pytest.approx(meta["initial_residual"], rel=1e-14)
@@ -203,6 +216,41 @@ def test_c2_preconditioners(dev, golden_index, precond, iters):
     assert np.linalg.norm(u - o_u) <= 1e-9 * np.linalg.norm(o_u)
     if precond == "ic0":
         assert rep["breakdowns"] == 0 and rep["levels"] > 0
+
+
+def test_c2_subdomain_block_and_block_storage(dev, golden_index):
+    """200x200, RCM, coloring with the reference's default block preconditioner (rcb,
+    64 nodes per subdomain: 632 dense 2x2-node-blocked Cholesky factors) and with block
+    storage: the split, the subdomain applications and the block storage products are
+    the reference's bit for bit; the solves take its iteration counts (734, 1240) and
+    agree with the Jacobi solve of the same system."""
+    print("case c2 200x200 rcm coloring block rcb64 / storage bsr", flush=True)
+    sys_ = pipeline.setup(meshgen.c2_model(), device=dev, assembly="coloring", reordering="rcm")
+    ref = golden_index["c2_block64"]
+    nodes, count = native.subdomain_split(sys_.indexed.xy, "rcb", 64)
+    assert count == 632 and digest(np.repeat(nodes.astype(np.int64), 2)) == ref["sha256"]["dof_subdomain"]
+    dev.set_subdomains(nodes, count)
+    kc, fc = dev.system()
+    x = np.random.default_rng(11).standard_normal(fc.shape[0])
+    blk = lfo.BlockJacobi(sys_.row_ptr, sys_.col, kc, np.repeat(nodes, 2), count)
+    assert np.array_equal(bits(dev.precond_apply("subdomain", x)), bits(blk.apply(x)))
+    assert blk.capture_ratio == ref["meta"]["capture_ratio"]
+    u_jac, _ = dev.pcg("jacobi", "fp64", "reference", 1e-10)
+    u, rep = dev.pcg("subdomain", "fp64", "reference", 1e-10)
+    assert rep["converged"] == 1 and abs(rep["iterations"] - ref["meta"]["iterations"]) <= 2 + 0.01 * 734
+    assert rep["degenerate_blocks"] == 0
+    assert np.linalg.norm(u - u_jac) <= 1e-9 * np.linalg.norm(u_jac)
+    # block storage: products bit for bit, the PCG on the node-slice and default layouts
+    y_bsr = lfo.bsr_spmv(sys_.row_ptr, sys_.col, kc, x)
+    assert np.array_equal(bits(dev.spmv(x, "fp64", 6, block_order=True)), bits(y_bsr))
+    ref_bsr = golden_index["c2_bsr_jacobi"]["meta"]
+    for variant in (6, 0):
+        ub, rb = dev.pcg("jacobi", "fp64", "reference", 1e-10, storage="bsr", spmv_variant=variant)
+        assert rb["converged"] == 1 and abs(rb["iterations"] - ref_bsr["iterations"]) <= 2 + 0.01 * 1240
+        assert np.linalg.norm(ub - u_jac) <= 1e-9 * np.linalg.norm(u_jac)
+    with pytest.raises(native.LightFemError) as err:
+        dev.pcg("jacobi", "fp64", "reference", 1e-10, storage="bsr", spmv_variant=4)
+    assert err.value.code == native.ERR_UNSUPPORTED
 
 
 @pytest.mark.parametrize("precond,switch,variant,drop", [

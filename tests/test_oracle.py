@@ -10,7 +10,7 @@ import pytest
 import scipy.sparse
 import scipy.sparse.linalg
 
-from conftest import FULL_CASES, case_precond, load_golden
+from conftest import FULL_CASES, case_precond, case_storage, load_golden
 from oracle import lfo
 from lightfem_amd import meshgen
 
@@ -52,8 +52,24 @@ def test_oracle_matches_reference_bit_for_bit(name, golden_index):
     factor, _ = lfo.block1_factor(row_ptr, col, Kc)
     assert np.array_equal(lfo.block1_apply(factor, g["probe"]).view(np.int64),
                           g["block1_apply_probe"].view(np.int64))
-    # PCG with the configured preconditioner, reference stop rule
-    u, rep = lfo.pcg(row_ptr, col, Kc, fc, case_precond(meta), lfo.STOP_REFERENCE, meta["tolerance"])
+    # block storage product (sparse.cpp:128-198) and the case's subdomain block-Jacobi
+    assert np.array_equal(lfo.bsr_spmv(row_ptr, col, Kc, g["probe"]).view(np.int64), g["bsr_probe"].view(np.int64))
+    if meta["storage"] == "bsr":
+        assert meta["matrix_bytes_per_matvec"] == lfo.bsr_blocks(row_ptr, col) * (32 + 4)
+    n_sub = meta.get("subdomains", 0)
+    if "dof_subdomain" in g:
+        dsub = g["dof_subdomain"]
+        assert np.array_equal(dsub[0::2], dsub[1::2]) and dsub.max() + 1 == n_sub
+        if meta["subdomain_split"] == "chunk":
+            nodes, count = lfo.split_chunks(n // 2, meta["subdomain_nodes"])
+            assert count == n_sub and np.array_equal(np.repeat(nodes, 2), dsub)
+        blk = lfo.BlockJacobi(row_ptr, col, Kc, dsub, n_sub)
+        assert np.array_equal(blk.apply(g["probe"]).view(np.int64), g["block_apply_probe"].view(np.int64))
+        assert blk.capture_ratio == meta["capture_ratio"]
+    # PCG with the configured preconditioner and storage, reference stop rule
+    u, rep = lfo.pcg(row_ptr, col, Kc, fc, case_precond(meta), lfo.STOP_REFERENCE, meta["tolerance"],
+                     storage=case_storage(meta), dof_subdomain=g["dof_subdomain"] if "dof_subdomain" in g else None,
+                     n_sub=n_sub)
     assert rep["converged"] and rep["iterations"] == meta["iterations"]
     assert rep["initial_residual"] == meta["initial_residual"]
     assert np.array_equal(u.view(np.int64), g["u"].view(np.int64))

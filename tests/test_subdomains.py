@@ -1,0 +1,98 @@
+"""Subdomains of the block preconditioner and the block storage product on the host
+and in the oracle (no GPU).
+
+The host split (lfh_subdomain_split) must give the reference's node -> subdomain map
+node for node: recursive coordinate bisection ties (nodes with equal coordinates on a
+structured mesh) are broken by the partial sort the reference uses, so the maps are
+compared with the ones the compiled reference wrote. The oracle's subdomain
+block-Jacobi and block storage product are pinned bit for bit by the golden vectors
+(tests/test_oracle.py); here they are also checked against what the mathematics fixes.
+"""
+import ctypes
+import hashlib
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from conftest import FULL_CASES, load_golden
+from lightfem_amd import meshgen, native, pipeline
+from oracle import lfo
+
+BLOCK_CASES = [c for c in FULL_CASES if "block" in c]
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<i8").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", BLOCK_CASES)
+def test_host_split_matches_reference(name, golden_index):
+    g = load_golden(name)
+    meta = golden_index[name]["meta"]
+    nodes, count = native.subdomain_split(g["xy"], meta["subdomain_split"], meta["subdomain_nodes"])
+    assert count == meta["subdomains"]
+    assert np.array_equal(np.repeat(nodes.astype(np.int64), 2), g["dof_subdomain"])
+
+
+def test_host_split_200x200_rcm_matches_reference(golden_index):
+    """The default split (rcb, 64 nodes) of the 200x200 cantilever in RCM order: 632
+    subdomains, map digest from the compiled reference."""
+    ref = golden_index["c2_block64"]
+    ix = pipeline.index_model(meshgen.c2_model(), "rcm")
+    nodes, count = native.subdomain_split(ix.xy, "rcb", 64)
+    assert count == ref["meta"]["subdomains"] == 632
+    assert digest(np.repeat(nodes.astype(np.int64), 2)) == ref["sha256"]["dof_subdomain"]
+
+
+def test_host_split_invariants():
+    """Exactly ceil(n / size) parts of sizes differing by at most a factor 2 (halving the
+    largest part each time), every node in one part; chunk parts are index runs."""
+    rng = np.random.default_rng(3)
+    xy = rng.random((1000, 2))
+    for size in (1, 7, 64, 333, 1000, 5000):
+        nodes, count = native.subdomain_split(xy, "rcb", size)
+        want = -(-1000 // size)
+        assert count == want and nodes.min() == 0 and nodes.max() == want - 1
+        sizes = np.bincount(nodes, minlength=count)
+        assert sizes.sum() == 1000 and sizes.max() <= 2 * max(sizes.min(), 1) + 1
+        chunk, c2 = native.subdomain_split(xy, "chunk", size)
+        assert c2 == want and np.array_equal(chunk, np.arange(1000) // size)
+    # an unknown split code is an error, not a silent default
+    out = np.empty(1000, dtype=np.int32)
+    count = ctypes.c_int64(0)
+    assert native.host_lib().lfh_subdomain_split(1000, xy.ctypes.data_as(ctypes.c_void_p), 7, 4,
+                                                 out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(count)) != 0
+
+
+def test_oracle_block_jacobi_inverts_each_block():
+    """M^-1 restricted to a subdomain is the inverse of that subdomain's dense block
+    (a dense solve by scipy), and zero coupling between subdomains."""
+    g = load_golden("grid16_rcm_block64")
+    row_ptr, col, kc = g["row_ptr"], g["col"], g["Kc"]
+    dsub = g["dof_subdomain"]
+    n_sub = int(dsub.max()) + 1
+    blk = lfo.BlockJacobi(row_ptr, col, kc, dsub, n_sub)
+    n = row_ptr.shape[0] - 1
+    dense = np.zeros((n, n))
+    for r in range(n):
+        dense[r, col[row_ptr[r]:row_ptr[r + 1]]] = kc[row_ptr[r]:row_ptr[r + 1]]
+    x = np.random.default_rng(5).standard_normal(n)
+    z = blk.apply(x)
+    for s in range(n_sub):
+        idx = np.flatnonzero(dsub == s)
+        a = dense[np.ix_(idx, idx)]
+        ref = scipy.linalg.solve(a, x[idx], assume_a="pos")
+        assert np.allclose(z[idx], ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    assert blk.degenerate == 0
+
+
+def test_oracle_bsr_product_is_the_matrix_product():
+    """The block storage product multiplies by the same matrix (sums in another order)."""
+    g = load_golden("grid16x12_strain_loge")
+    x = np.random.default_rng(9).standard_normal(g["f"].shape[0])
+    y_csr = lfo.spmv(g["row_ptr"], g["col"], g["Kc"], x)
+    y_bsr = lfo.bsr_spmv(g["row_ptr"], g["col"], g["Kc"], x)
+    scale = lfo.spmv(g["row_ptr"], g["col"], np.abs(g["Kc"]), np.abs(x))
+    assert np.all(np.abs(y_bsr - y_csr) <= 1e-15 * scale)
+    assert lfo.bsr_blocks(g["row_ptr"], g["col"]) * 4 == g["col"].shape[0]  # node-blocked: full 2x2 blocks
