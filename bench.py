@@ -152,6 +152,29 @@ def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict
     return out
 
 
+def run_c2(dev, tol: float) -> dict:
+    """The 200x200 cantilever in RCM order with coloring assembly, every preconditioner and
+    storage of the reference: Jacobi, 2x2 nodal block, the default block (rcb, 64 nodes per
+    subdomain), IC(0), and Jacobi with block storage. Device solve time (hipEvents) to the
+    reference stop rule ||r|| <= tol max(1, ||r0||)."""
+    from lightfem_amd import meshgen, native, pipeline
+    sys_ = pipeline.setup(meshgen.c2_model(), device=dev, assembly="coloring", reordering="rcm")
+    nodes, count = native.subdomain_split(sys_.indexed.xy, "rcb", 64)
+    dev.set_subdomains(nodes, count)
+    out = {"workload": "q4-cantilever-200x200-plane-stress-rcm-coloring", "dofs": int(dev.n), "nnz": int(dev.nnz),
+           "stop_rule": f"reference ||r|| <= {tol:g} max(1, ||r0||)"}
+    cases = (("jacobi", "jacobi", "csr"), ("block_1_node", "block", "csr"), ("block_rcb_64", "subdomain", "csr"),
+             ("ic0", "ic0", "csr"), ("jacobi_bsr", "jacobi", "bsr"))
+    for name, precond, storage in cases:
+        dev.pcg(precond, "fp64", "reference", tol, storage=storage, raise_on_error=False)  # set-up and warm-up
+        _, rep = dev.pcg(precond, "fp64", "reference", tol, storage=storage, raise_on_error=False)
+        out[name] = {"device_solve_ms": round(rep["solve_ms"], 3), "iterations": rep["iterations"],
+                     "converged": bool(rep["converged"]), "setup_ms": round(rep["setup_ms"], 3),
+                     "us_per_iteration": round(1e3 * rep["solve_ms"] / max(1, rep["iterations"]), 2)}
+        print(f"bench: c2 {name} {json.dumps(out[name])}", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +189,8 @@ def main() -> int:
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--batch-only", action="store_true", help="only the batch workload (profiling)")
     ap.add_argument("--batch-forms", default="fp64,mixed", help="comma-separated batch precisions to run")
+    ap.add_argument("--skip-c2", action="store_true", help="skip the 200x200 preconditioner comparison")
+    ap.add_argument("--c2-only", action="store_true", help="only the 200x200 preconditioner comparison (profiling)")
     ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--profile", type=int, default=0,
                     help="profiling run: this many FP64 SpMV launches and FP64 PCG iterations, nothing else")
@@ -177,6 +202,11 @@ def main() -> int:
 
     from lightfem_amd import meshgen, native, pipeline
 
+    if args.c2_only:
+        dev = native.Device(0)
+        print(json.dumps({"c2_preconditioners": run_c2(dev, args.tol)}))
+        dev.close()
+        return 0
     if args.batch_only:
         dev = native.Device(0)
         print(json.dumps({"batch": run_batch(dev, args.batch_problems, args.tol, args.batch_forms.split(","))}))
@@ -312,6 +342,7 @@ def main() -> int:
     batch = None
     if not args.skip_batch:
         batch = run_batch(dev, args.batch_problems, args.tol, args.batch_forms.split(","))
+    c2 = None if args.skip_c2 else run_c2(dev, args.tol)
     # Headline: the FP64 form. The mixed form (FP32 values with a compensated FP64
     # diagonal once ||r|| <= 1e-2 ||r0||, FP64 vectors and sums) stops on the same
     # recurrence rule and is reported beside it with its displacement difference.
@@ -347,6 +378,7 @@ def main() -> int:
                      "copy_ceiling_GBs": round(copy["GBs"], 1)},
         "cpu_baseline": cpu,
         "batch": batch,
+        "c2_preconditioners": c2,
     }
     print(json.dumps(line))
     dev.close()
