@@ -160,12 +160,14 @@ static int cmp_i64(const void* a, const void* b) {
     return (x > y) - (x < y);
 }
 
-/* sparse.cpp:15-41: per-row column set (diagonal always inserted), sorted. */
-static int64_t build_rows(int64_t ndofs, int64_t nelem, const int64_t* edofs, int64_t* row_ptr, int64_t* col) {
+/* sparse.cpp:15-41: per-row column set (diagonal always inserted), sorted. Element e
+ * has the dofs edofs[eptr[e] .. eptr[e+1]) (Q4: 8, bar: 4). */
+static int64_t build_rows_var(int64_t ndofs, int64_t nelem, const int64_t* eptr, const int64_t* edofs,
+                              int64_t* row_ptr, int64_t* col) {
     int64_t* count = calloc((size_t)ndofs + 1, sizeof(int64_t));
     for (int64_t r = 0; r < ndofs; ++r) count[r] = 1;
     for (int64_t e = 0; e < nelem; ++e)
-        for (int a = 0; a < 8; ++a) count[edofs[8 * e + a]] += 8;
+        for (int64_t a = eptr[e]; a < eptr[e + 1]; ++a) count[edofs[a]] += eptr[e + 1] - eptr[e];
     int64_t* start = malloc(((size_t)ndofs + 1) * sizeof(int64_t));
     start[0] = 0;
     for (int64_t r = 0; r < ndofs; ++r) start[r + 1] = start[r] + count[r];
@@ -176,9 +178,9 @@ static int64_t build_rows(int64_t ndofs, int64_t nelem, const int64_t* edofs, in
         raw[fill[r]++] = r;
     }
     for (int64_t e = 0; e < nelem; ++e)
-        for (int a = 0; a < 8; ++a) {
-            const int64_t row = edofs[8 * e + a];
-            for (int c = 0; c < 8; ++c) raw[fill[row]++] = edofs[8 * e + c];
+        for (int64_t a = eptr[e]; a < eptr[e + 1]; ++a) {
+            const int64_t row = edofs[a];
+            for (int64_t c = eptr[e]; c < eptr[e + 1]; ++c) raw[fill[row]++] = edofs[c];
         }
     int64_t nnz = 0;
     if (row_ptr) row_ptr[0] = 0;
@@ -195,6 +197,23 @@ static int64_t build_rows(int64_t ndofs, int64_t nelem, const int64_t* edofs, in
     }
     free(count); free(start); free(raw); free(fill);
     return nnz;
+}
+
+static int64_t* q4_eptr(int64_t nelem) {
+    int64_t* eptr = malloc(((size_t)nelem + 1) * sizeof(int64_t));
+    for (int64_t e = 0; e <= nelem; ++e) eptr[e] = 8 * e;
+    return eptr;
+}
+
+static int64_t build_rows(int64_t ndofs, int64_t nelem, const int64_t* edofs, int64_t* row_ptr, int64_t* col) {
+    int64_t* eptr = q4_eptr(nelem);
+    const int64_t nnz = build_rows_var(ndofs, nelem, eptr, edofs, row_ptr, col);
+    free(eptr);
+    return nnz;
+}
+
+int64_t lfo_pattern_nnz_var(int64_t ndofs, int64_t nelem, const int64_t* eptr, const int64_t* edofs) {
+    return build_rows_var(ndofs, nelem, eptr, edofs, NULL, NULL);
 }
 
 int64_t lfo_pattern_nnz(int64_t ndofs, int64_t nelem, const int64_t* edofs) {
@@ -222,6 +241,22 @@ int lfo_pattern_build(int64_t ndofs, int64_t nelem, const int64_t* edofs, int64_
                 const int64_t idx = value_index(row_ptr, col, edofs[8 * e + r], edofs[8 * e + c]);
                 if (idx < 0) return 1;
                 emap[64 * e + 8 * r + c] = idx;
+            }
+    return 0;
+}
+
+/* sparse.cpp:15-58 with element dof lists of any length (Q4 elements, then bars,
+ * solver.cpp:71-83); emap gets n_e^2 entries per element, row-major, in element order. */
+int lfo_pattern_build_var(int64_t ndofs, int64_t nelem, const int64_t* eptr, const int64_t* edofs,
+                          int64_t* row_ptr, int64_t* col, int64_t* emap) {
+    build_rows_var(ndofs, nelem, eptr, edofs, row_ptr, col);
+    int64_t m = 0;
+    for (int64_t e = 0; e < nelem; ++e)
+        for (int64_t r = eptr[e]; r < eptr[e + 1]; ++r)
+            for (int64_t c = eptr[e]; c < eptr[e + 1]; ++c) {
+                const int64_t idx = value_index(row_ptr, col, edofs[r], edofs[c]);
+                if (idx < 0) return 1;
+                emap[m++] = idx;
             }
     return 0;
 }
@@ -374,21 +409,23 @@ int lfo_rcm(int64_t nnodes, const int64_t* node_ids, int64_t nelem, const int64_
 }
 
 /* solver.cpp:100-133 greedy first-fit colouring in element order; elements sharing
- * a node conflict. Returns the number of colours. */
-int64_t lfo_colors(int64_t nelem, const int64_t* enode_ids, int64_t* color_out) {
+ * a node conflict. Element e has the node ids enode_ids[nptr[e] .. nptr[e+1]) (Q4: 4,
+ * bar: 2). Returns the number of colours. */
+int64_t lfo_colors_var(int64_t nelem, const int64_t* nptr, const int64_t* enode_ids, int64_t* color_out) {
     if (nelem == 0) return 0;
+    const int64_t total = nptr[nelem];
     /* node id -> element list (elements appended in ascending order) */
     id_index ix;
-    int64_t* ids = malloc((size_t)(4 * nelem) * sizeof(int64_t));
-    memcpy(ids, enode_ids, (size_t)(4 * nelem) * sizeof(int64_t));
-    qsort(ids, (size_t)(4 * nelem), sizeof(int64_t), cmp_i64);
+    int64_t* ids = malloc((size_t)total * sizeof(int64_t));
+    memcpy(ids, enode_ids, (size_t)total * sizeof(int64_t));
+    qsort(ids, (size_t)total, sizeof(int64_t), cmp_i64);
     int64_t nu = 0;
-    for (int64_t k = 0; k < 4 * nelem; ++k)
+    for (int64_t k = 0; k < total; ++k)
         if (k == 0 || ids[k] != ids[k - 1]) ids[nu++] = ids[k];
     id_index_build(&ix, nu, ids);
     int64_t* cnt = calloc((size_t)nu + 1, sizeof(int64_t));
-    int64_t* en = malloc((size_t)(4 * nelem) * sizeof(int64_t));
-    for (int64_t k = 0; k < 4 * nelem; ++k) { en[k] = id_lookup(&ix, enode_ids[k]); cnt[en[k]]++; }
+    int64_t* en = malloc((size_t)total * sizeof(int64_t));
+    for (int64_t k = 0; k < total; ++k) { en[k] = id_lookup(&ix, enode_ids[k]); cnt[en[k]]++; }
     int64_t* start = malloc(((size_t)nu + 1) * sizeof(int64_t));
     start[0] = 0;
     for (int64_t i = 0; i < nu; ++i) start[i + 1] = start[i] + cnt[i];
@@ -396,16 +433,16 @@ int64_t lfo_colors(int64_t nelem, const int64_t* enode_ids, int64_t* color_out) 
     int64_t* fill = malloc((size_t)nu * sizeof(int64_t));
     for (int64_t i = 0; i < nu; ++i) fill[i] = start[i];
     for (int64_t e = 0; e < nelem; ++e)
-        for (int i = 0; i < 4; ++i) list[fill[en[4 * e + i]]++] = e;
+        for (int64_t k = nptr[e]; k < nptr[e + 1]; ++k) list[fill[en[k]]++] = e;
     int64_t* stamp_of = calloc((size_t)nelem, sizeof(int64_t));
     int64_t stamp = 0, maximum = -1;
     for (int64_t e = 0; e < nelem; ++e) color_out[e] = -1;
     for (int64_t e = 0; e < nelem; ++e) {
         ++stamp;
-        for (int i = 0; i < 4; ++i) {
-            const int64_t node = en[4 * e + i];
-            for (int64_t k = start[node]; k < start[node + 1]; ++k) {
-                const int64_t nbr = list[k];
+        for (int64_t k = nptr[e]; k < nptr[e + 1]; ++k) {
+            const int64_t node = en[k];
+            for (int64_t q = start[node]; q < start[node + 1]; ++q) {
+                const int64_t nbr = list[q];
                 if (color_out[nbr] >= 0) stamp_of[color_out[nbr]] = stamp;
             }
         }
@@ -419,6 +456,14 @@ int64_t lfo_colors(int64_t nelem, const int64_t* enode_ids, int64_t* color_out) 
     return maximum + 1;
 }
 
+int64_t lfo_colors(int64_t nelem, const int64_t* enode_ids, int64_t* color_out) {
+    int64_t* nptr = malloc(((size_t)nelem + 1) * sizeof(int64_t));
+    for (int64_t e = 0; e <= nelem; ++e) nptr[e] = 4 * e;
+    const int64_t count = lfo_colors_var(nelem, nptr, enode_ids, color_out);
+    free(nptr);
+    return count;
+}
+
 /* -------------------------------------------------------------- assembly */
 
 /* solver.cpp:135-142 */
@@ -426,6 +471,75 @@ void lfo_assemble_serial(int64_t nelem, const double* ke, const int64_t* emap, i
     for (int64_t k = 0; k < nnz; ++k) values[k] = 0.0;
     for (int64_t e = 0; e < nelem; ++e)
         for (int k = 0; k < 64; ++k) values[emap[64 * e + k]] += ke[64 * e + k];
+}
+
+/* solver.cpp:135-142 over flat element matrices (Q4 64 entries each, then bars 16). */
+void lfo_assemble_flat(int64_t total, const double* ke, const int64_t* emap, int64_t nnz, double* values) {
+    for (int64_t k = 0; k < nnz; ++k) values[k] = 0.0;
+    for (int64_t k = 0; k < total; ++k) values[emap[k]] += ke[k];
+}
+
+/* element.cpp:266-286: bar stiffness EA/L [c^2 cs -c^2 -cs; ...], row-major 4x4. xy =
+ * (x1, y1, x2, y2). Returns 1 for a zero-length bar. */
+int lfo_truss_stiffness(const double xy[4], double young, double area, double ke[16]) {
+    const double dx = xy[2] - xy[0];
+    const double dy = xy[3] - xy[1];
+    const double length = sqrt(dx * dx + dy * dy);
+    if (!(length > 1.0e-12) || !isfinite(length)) return 1;
+    const double c = dx / length, s = dy / length;
+    const double scale = young * area / length;
+    const double cc = c * c, ss = s * s, cs = c * s;
+    const double v[16] = {scale * cc,  scale * cs,  -scale * cc, -scale * cs, scale * cs,  scale * ss,
+                          -scale * cs, -scale * ss, -scale * cc, -scale * cs, scale * cc,  scale * cs,
+                          -scale * cs, -scale * ss, scale * cs,  scale * ss};
+    memcpy(ke, v, sizeof(v));
+    return 0;
+}
+
+/* element.cpp:288-306 and solver.cpp:753-803 for one bar: out = strain (axial, 0, 0),
+ * stress sigma * (c^2, s^2, cs), von Mises |sigma|. Returns 1 for a zero-length bar. */
+int lfo_truss_fields(const double xy[4], double young, const double u[4], double out[7]) {
+    const double dx = xy[2] - xy[0];
+    const double dy = xy[3] - xy[1];
+    const double length = sqrt(dx * dx + dy * dy);
+    if (!(length > 1.0e-12) || !isfinite(length)) return 1;
+    const double c = dx / length, s = dy / length;
+    const double du = u[2] - u[0], dv = u[3] - u[1];
+    const double strain = (du * c + dv * s) / length;
+    const double sigma = young * strain;
+    out[0] = strain;
+    out[1] = 0.0;
+    out[2] = 0.0;
+    out[3] = sigma * c * c;
+    out[4] = sigma * s * s;
+    out[5] = sigma * c * s;
+    out[6] = fabs(sigma);
+    return 0;
+}
+
+/* solver.cpp:916-940: after the Dirichlet elimination, a dof whose diagonal is at most
+ * 1e-30 in magnitude is pinned (diagonal 1, right-hand side 0) unless it carries a load
+ * above 1e-12 of the largest |rhs| (1 when all are zero): then it returns -1 with the
+ * first such dof in *bad. Otherwise returns the number of pinned dofs. */
+int64_t lfo_pin_zero_pivots(int64_t n, const int64_t* row_ptr, const int64_t* col, double* values, double* rhs,
+                            int64_t* bad) {
+    double scale = 0.0;
+    for (int64_t i = 0; i < n; ++i) scale = scale > fabs(rhs[i]) ? scale : fabs(rhs[i]);
+    if (scale <= 0.0) scale = 1.0;
+    int64_t pinned = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t d = value_index(row_ptr, col, i, i);
+        const double diag = d >= 0 ? values[d] : 0.0;
+        if (fabs(diag) > 1.0e-30) continue;
+        if (fabs(rhs[i]) > 1.0e-12 * scale) {
+            *bad = i;
+            return -1;
+        }
+        values[d] = 1.0;
+        rhs[i] = 0.0;
+        ++pinned;
+    }
+    return pinned;
 }
 
 /* solver.cpp:355-413 (the O(nnz) sweep). */
@@ -905,6 +1019,13 @@ done:
 
 /* solver.cpp:807-871 (Q4 part). gp layout per element: 4 x [xi, eta, e3, s3, vm]. */
 void lfo_smooth_nodal(int64_t nnodes, int64_t nelem, const int64_t* enode_idx, const double* gp, double* nodal) {
+    lfo_smooth_nodal_ex(nnodes, nelem, enode_idx, gp, 0, NULL, NULL, nodal);
+}
+
+/* solver.cpp:807-871 with bars: after the Q4 extrapolations, each bar adds its stress
+ * tensor (3 per bar in tstress) to both of its nodes (tnode_idx, 2 per bar). */
+void lfo_smooth_nodal_ex(int64_t nnodes, int64_t nelem, const int64_t* enode_idx, const double* gp, int64_t ntruss,
+                         const int64_t* tnode_idx, const double* tstress, double* nodal) {
     const double root_three = 1.7320508075688772935;
     const double alpha = 0.5 * (1.0 + root_three);
     const double beta = 0.5 * (1.0 - root_three);
@@ -921,6 +1042,12 @@ void lfo_smooth_nodal(int64_t nnodes, int64_t nelem, const int64_t* enode_idx, c
             const int64_t node = enode_idx[4 * e + l];
             for (int c = 0; c < 3; ++c)
                 for (int g = 0; g < 4; ++g) sums[3 * node + c] += ex[l][g] * gp[36 * e + 9 * g + 5 + c];
+            counts[node]++;
+        }
+    for (int64_t t = 0; t < ntruss; ++t)
+        for (int l = 0; l < 2; ++l) {
+            const int64_t node = tnode_idx[2 * t + l];
+            for (int c = 0; c < 3; ++c) sums[3 * node + c] += tstress[3 * t + c];
             counts[node]++;
         }
     for (int64_t node = 0; node < nnodes; ++node) {

@@ -24,6 +24,9 @@
  *   lfo_split_chunks       golden chunk subdomain maps (exact)
  *   lfo_block_*            golden subdomain block-Jacobi applications (bit-exact),
  *                          golden block-preconditioned solutions and iteration counts
+ *   lfo_*_var, lfo_truss_*, lfo_assemble_flat, lfo_pin_zero_pivots, lfo_smooth_nodal_ex
+ *                          golden bar-model patterns, maps, colours, K, u, R and fields
+ *                          (bit-exact), bar statics (u = FL/EA, N = P/(2 sin 45))
  *   lfo_smooth_nodal       golden nodal fields (bit-exact)
  */
 #ifndef LFO_H
@@ -74,6 +77,21 @@ void lfo_ic0_apply(int64_t n, const int64_t* lrow, const int64_t* lcol, const do
 int64_t lfo_block1_factor(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values,
                           double* factor);
 void lfo_block1_apply(int64_t n, const double* factor, const double* in, double* out);
+
+/* Bars (element.cpp:250-306, solver.cpp:71-83, 100-142, 753-871, 916-940). Element
+ * lists of any length: element e owns edofs[eptr[e] .. eptr[e+1]) (Q4 8, bar 4) and
+ * node ids enode_ids[nptr[e] .. nptr[e+1]) (Q4 4, bar 2), Q4 elements first. */
+int64_t lfo_pattern_nnz_var(int64_t ndofs, int64_t nelem, const int64_t* eptr, const int64_t* edofs);
+int lfo_pattern_build_var(int64_t ndofs, int64_t nelem, const int64_t* eptr, const int64_t* edofs,
+                          int64_t* row_ptr, int64_t* col, int64_t* emap);
+int64_t lfo_colors_var(int64_t nelem, const int64_t* nptr, const int64_t* enode_ids, int64_t* color_out);
+void lfo_assemble_flat(int64_t total, const double* ke, const int64_t* emap, int64_t nnz, double* values);
+int lfo_truss_stiffness(const double xy[4], double young, double area, double ke[16]);
+int lfo_truss_fields(const double xy[4], double young, const double u[4], double out[7]);
+int64_t lfo_pin_zero_pivots(int64_t n, const int64_t* row_ptr, const int64_t* col, double* values, double* rhs,
+                            int64_t* bad);
+void lfo_smooth_nodal_ex(int64_t nnodes, int64_t nelem, const int64_t* enode_idx, const double* gp, int64_t ntruss,
+                         const int64_t* tnode_idx, const double* tstress, double* nodal);
 
 /* Block storage product (BsrMatrix::from_csr + multiply_into, sparse.cpp:128-198). */
 void lfo_bsr_spmv(int64_t n, const int64_t* row_ptr, const int64_t* col, const double* values, const double* x,

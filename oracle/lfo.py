@@ -42,6 +42,9 @@ def lib() -> ctypes.CDLL:
         _lib.lfo_block1_factor.restype = ctypes.c_int64
         _lib.lfo_sensitivity.restype = ctypes.c_int64
         _lib.lfo_bsr_blocks.restype = ctypes.c_int64
+        _lib.lfo_pattern_nnz_var.restype = ctypes.c_int64
+        _lib.lfo_colors_var.restype = ctypes.c_int64
+        _lib.lfo_pin_zero_pivots.restype = ctypes.c_int64
         _lib.lfo_split_chunks.restype = ctypes.c_int64
         _lib.lfo_block_create.restype = ctypes.c_void_p
         _lib.lfo_block_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -252,6 +255,74 @@ def pcg(row_ptr, col, values, rhs, precond: int = JACOBI, stop_rule: int = STOP_
                      ctypes.c_double(tolerance), _I(max_iterations), _p(x), ctypes.byref(rep))
     return x, {"iterations": rep.iterations, "converged": bool(rep.converged), "status": rep.status,
                "initial_residual": rep.initial_residual, "final_residual": rep.final_residual}
+
+
+# ---------------------------------------------------------------- bars
+
+def pattern_var(ndofs: int, element_dofs: list):
+    """(row_ptr, col, emap_flat) for element dof lists of any length (Q4 then bars)."""
+    eptr = np.concatenate([[0], np.cumsum([len(d) for d in element_dofs])]).astype(np.int64)
+    edofs = _i64(np.concatenate([np.asarray(d, dtype=np.int64) for d in element_dofs]))
+    nelem = len(element_dofs)
+    nnz = lib().lfo_pattern_nnz_var(_I(ndofs), _I(nelem), _p(eptr), _p(edofs))
+    row_ptr = np.empty(ndofs + 1, dtype=np.int64)
+    col = np.empty(nnz, dtype=np.int64)
+    emap = np.empty(int(sum(len(d) ** 2 for d in element_dofs)), dtype=np.int64)
+    if lib().lfo_pattern_build_var(_I(ndofs), _I(nelem), _p(eptr), _p(edofs), _p(row_ptr), _p(col), _p(emap)):
+        raise RuntimeError("requested entry is absent from preallocated CSR pattern")
+    return row_ptr, col, emap
+
+
+def colors_var(element_node_ids: list):
+    nptr = np.concatenate([[0], np.cumsum([len(n) for n in element_node_ids])]).astype(np.int64)
+    ids = _i64(np.concatenate([np.asarray(n, dtype=np.int64) for n in element_node_ids]))
+    out = np.empty(len(element_node_ids), dtype=np.int64)
+    count = lib().lfo_colors_var(_I(len(element_node_ids)), _p(nptr), _p(ids), _p(out))
+    return out, int(count)
+
+
+def assemble_flat(ke_flat, emap_flat, nnz: int) -> np.ndarray:
+    ke_flat, emap_flat = _f64(ke_flat), _i64(emap_flat)
+    values = np.empty(nnz)
+    lib().lfo_assemble_flat(_I(ke_flat.shape[0]), _p(ke_flat), _p(emap_flat), _I(nnz), _p(values))
+    return values
+
+
+def truss_stiffness(xy4, young: float, area: float) -> np.ndarray:
+    xy4 = _f64(xy4)
+    ke = np.empty(16)
+    if lib().lfo_truss_stiffness(_p(xy4), ctypes.c_double(young), ctypes.c_double(area), _p(ke)):
+        raise RuntimeError("truss stiffness evaluation received a zero-length member")
+    return ke
+
+
+def truss_fields(xy4, young: float, u4) -> np.ndarray:
+    xy4, u4 = _f64(xy4), _f64(u4)
+    out = np.empty(7)
+    if lib().lfo_truss_fields(_p(xy4), ctypes.c_double(young), _p(u4), _p(out)):
+        raise RuntimeError("truss strain evaluation received a zero-length member")
+    return out
+
+
+def pin_zero_pivots(row_ptr, col, values, rhs):
+    """The zero-pivot sweep after the Dirichlet elimination (solver.cpp:916-940):
+    (values, rhs, pinned count, first loaded zero-pivot dof or -1)."""
+    row_ptr, col = _i64(row_ptr), _i64(col)
+    values = np.array(values, dtype=np.float64, copy=True)
+    rhs = np.array(rhs, dtype=np.float64, copy=True)
+    bad = ctypes.c_int64(-1)
+    pinned = lib().lfo_pin_zero_pivots(_I(row_ptr.shape[0] - 1), _p(row_ptr), _p(col), _p(values), _p(rhs),
+                                       ctypes.byref(bad))
+    return values, rhs, int(max(pinned, 0)), int(bad.value) if pinned < 0 else -1
+
+
+def smooth_nodal_ex(nnodes: int, enode_idx, gp, tnode_idx, tstress) -> np.ndarray:
+    enode_idx, gp = _i64(enode_idx).reshape(-1, 4), _f64(gp)
+    tnode_idx, tstress = _i64(tnode_idx).reshape(-1, 2), _f64(tstress).reshape(-1, 3)
+    out = np.empty((nnodes, 4))
+    lib().lfo_smooth_nodal_ex(_I(nnodes), _I(enode_idx.shape[0]), _p(enode_idx), _p(gp), _I(tnode_idx.shape[0]),
+                              _p(tnode_idx), _p(tstress), _p(out))
+    return out
 
 
 def smooth_nodal(nnodes: int, enode_idx, gp) -> np.ndarray:

@@ -167,6 +167,25 @@ int dump(const std::string& path, const std::string& dir, const std::vector<std:
         npy_f8(dir, "ke", ke, {model.elements.size(), 64});
         npy_f8(dir, "elem_material", mat, {model.elements.size(), 3});
     }
+    if (!model.trusses.empty()) {
+        // bars: dofs, element matrices (element.cpp:266-286) and (E, area)
+        std::vector<long long> td;
+        std::vector<double> kt, tp;
+        for (const TrussElement& truss : model.trusses) {
+            const auto d = truss_dof_indices(model, truss);
+            for (std::size_t v : d) td.push_back(static_cast<long long>(v));
+            const Material* material = model.find_material(truss.material_id);
+            std::array<Vec2, 2> coordinates{};
+            for (std::size_t i = 0; i < 2; ++i) coordinates[i] = model.find_node(truss.node_ids[i])->position;
+            const auto k = truss_stiffness(coordinates, truss.area, material->young_modulus);
+            kt.insert(kt.end(), k.begin(), k.end());
+            tp.push_back(material->young_modulus);
+            tp.push_back(truss.area);
+        }
+        npy_i8(dir, "tdofs", td, {model.trusses.size(), 4});
+        npy_f8(dir, "ke_truss", kt, {model.trusses.size(), 16});
+        npy_f8(dir, "truss_props", tp, {model.trusses.size(), 2});
+    }
 
     AssembledSystem assembled = assemble(model);
     npy_f8(dir, "K", assembled.stiffness.values());

@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 from lightfem_amd import meshgen  # noqa: E402
 
 LFREF = os.path.join(ROOT, "oracle", "_ref", "lfref")
+REF_DATA = "/root/reference/tests/data"
 
 # Fixture models held by the reference's own tests (tests/test_solver.cpp:41-62,
 # 143-174, 233-257); they are inputs, kept as data.
@@ -86,6 +87,30 @@ output ignored.vtk
 [loads]
 3 0 -1
 5 0 -1
+"""
+
+PLATE_BAR_MIXED = """
+[settings]
+solver pcg
+tolerance 1e-13
+max_iterations 500
+[materials]
+1 210000000000 0.3 0.01
+[nodes]
+1 0 0
+2 1 0
+3 1 1
+4 0 1
+5 2 1
+[elements]
+1 1 2 3 4 1
+[trusses]
+7 3 5 1 1.0e-3
+[boundaries]
+1 fixed
+4 fixed
+[loads]
+5 1000 0
 """
 
 EXTENDED = """
@@ -162,7 +187,8 @@ def digest(array: np.ndarray) -> str:
     return hashlib.sha256(data.tobytes()).hexdigest()
 
 
-FULL_KEYS = ["node_ids", "xy", "rcm_order", "row_ptr", "col", "edofs", "emap", "colors", "ke", "elem_material",
+FULL_KEYS = ["node_ids", "xy", "rcm_order", "row_ptr", "col", "edofs", "emap", "edofs_flat", "emap_flat", "tdofs",
+             "ke_truss", "truss_props", "colors", "ke", "elem_material",
              "K", "f", "cdofs", "cvals", "Kc", "fc", "probe", "jacobi_invdiag", "ic0_apply_probe",
              "block1_apply_probe", "bsr_probe", "dof_subdomain", "block_apply_probe", "u", "R", "elem_centre",
              "elem_gp", "node_fields"]
@@ -204,6 +230,12 @@ CLI_MODELS = {"one_element": ONE_ELEMENT, "scrambled": SCRAMBLED, "extended": EX
                   "storage bsr", "preconditioner block", "subdomain_split rcb", "subdomain_nodes 2"]),
               "grid6_block_chunk4": with_settings(grid_model_text(6, 6), [
                   "preconditioner block", "subdomain_split chunk", "subdomain_nodes 4"])}
+# bar models of the reference's own test data
+for _name in ("truss_tension", "truss_triangle", "plate_with_bar"):
+    _path = os.path.join(REF_DATA, f"{_name}.fem")
+    if os.path.exists(_path):
+        with open(_path) as _handle:
+            CLI_MODELS[_name] = _handle.read()
 
 
 def cli_fixtures() -> dict:
@@ -241,6 +273,15 @@ def main() -> None:
             np.savez_compressed(os.path.join(HERE, f"scrambled_{strategy}.npz"), u=arrays["u"], K=arrays["K"])
             record[f"scrambled_{strategy}"] = {"meta": arrays["__meta__"], "file": f"scrambled_{strategy}.npz"}
         save_full("extended_loads", run_dump(EXTENDED, [], work, "ext"), record)
+        # bar models: the reference's own test data (tests/data/*.fem) and the plate + bar
+        # model of tests/test_solver.cpp:336-363
+        for name in ("truss_tension", "truss_triangle", "plate_with_bar"):
+            with open(os.path.join(REF_DATA, f"{name}.fem")) as handle:
+                text = handle.read()
+            save_full(name, run_dump(text, [], work, name), record)
+        save_full("plate_bar_mixed", run_dump(PLATE_BAR_MIXED, [], work, "pbm"), record)
+        save_full("plate_bar_mixed_coloring",
+                  run_dump(PLATE_BAR_MIXED, ["assembly=coloring", "threads=2"], work, "pbmc"), record)
         grid6 = grid_model_text(6, 6)
         for precond in ("jacobi", "ic0"):
             save_full(f"grid6_{precond}", run_dump(grid6, [f"preconditioner={precond}"], work, f"g6{precond}"),

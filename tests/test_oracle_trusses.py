@@ -1,0 +1,121 @@
+"""Pins the oracle's bar (truss) restatement (no GPU): element matrices, patterns and
+maps over mixed Q4 + bar element lists, colours, assembly, the zero-pivot sweep, the
+solve, reactions, bar fields and nodal smoothing, bit for bit against the compiled
+reference's dumps of its own bar test models (tests/data/*.fem, test_solver.cpp:271-363),
+plus the closed-form statics those tests state."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from oracle import lfo
+
+TRUSS_CASES = ["truss_tension", "truss_triangle", "plate_with_bar", "plate_bar_mixed", "plate_bar_mixed_coloring"]
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.int64)
+
+
+def decode(g):
+    """Q4 dofs (E, 8), bar dofs (T, 4), the flat map split into Q4 / bar parts."""
+    n_q4 = g["ke"].shape[0]
+    q4 = g["edofs_flat"][:8 * n_q4].reshape(n_q4, 8)
+    bars = g["tdofs"]
+    assert np.array_equal(g["edofs_flat"][8 * n_q4:], bars.ravel())
+    return q4, bars
+
+
+def colored_sum(ke_list, emaps, colors, nnz):
+    values = np.zeros(nnz)
+    for c in range(int(colors.max()) + 1):
+        for e in np.flatnonzero(colors == c):
+            values[emaps[e]] += ke_list[e]
+    return values
+
+
+@pytest.mark.parametrize("name", TRUSS_CASES)
+def test_oracle_bar_models_bit_for_bit(name, golden_index):
+    g = load_golden(name)
+    meta = golden_index[name]["meta"]
+    q4, bars = decode(g)
+    n = g["f"].shape[0]
+    n_q4, n_bar = q4.shape[0], bars.shape[0]
+    # pattern and map over the element dof lists (Q4 first, then bars)
+    dof_lists = [list(d) for d in q4] + [list(d) for d in bars]
+    row_ptr, col, emap = lfo.pattern_var(n, dof_lists)
+    assert np.array_equal(row_ptr, g["row_ptr"]) and np.array_equal(col, g["col"])
+    assert np.array_equal(emap, g["emap_flat"])
+    # colours over node ids
+    ids = g["node_ids"]
+    node_lists = [list(ids[d[0::2] // 2]) for d in q4] + [list(ids[d[0::2] // 2]) for d in bars]
+    colors, _ = lfo.colors_var(node_lists)
+    assert np.array_equal(colors, g["colors"])
+    # element matrices
+    xy = g["xy"]
+    ke_bar = np.array([lfo.truss_stiffness(xy[d[0::2] // 2].ravel(), *g["truss_props"][t])
+                       for t, d in enumerate(bars)]).reshape(n_bar, 16)
+    assert np.array_equal(bits(ke_bar), bits(g["ke_truss"]))
+    # assembly in element order, or colour by colour
+    ke_flat = np.concatenate([g["ke"].ravel(), ke_bar.ravel()])
+    if meta["assembly"] == "serial":
+        K = lfo.assemble_flat(ke_flat, emap, col.shape[0])
+    else:
+        split = np.cumsum([64] * n_q4 + [16] * n_bar)[:-1]
+        K = colored_sum(np.split(ke_flat, split), np.split(emap, split), colors, col.shape[0])
+    assert np.array_equal(bits(K), bits(g["K"]))
+    # Dirichlet elimination, zero-pivot sweep, Jacobi PCG
+    kc, fc = lfo.dirichlet(row_ptr, col, K, g["f"], g["cdofs"], g["cvals"])
+    assert np.array_equal(bits(kc), bits(g["Kc"])) and np.array_equal(bits(fc), bits(g["fc"]))
+    kp, fp, pinned, bad = lfo.pin_zero_pivots(row_ptr, col, kc, fc)
+    assert bad == -1
+    u, rep = lfo.pcg(row_ptr, col, kp, fp, lfo.JACOBI, lfo.STOP_REFERENCE, meta["tolerance"])
+    assert rep["converged"] and rep["iterations"] == meta["iterations"]
+    assert np.array_equal(bits(u), bits(g["u"]))
+    R = lfo.spmv(row_ptr, col, K, u) - g["f"]
+    assert np.array_equal(bits(R), bits(g["R"]))
+    # fields: Q4 centre / Gauss points, bars (every Gauss point carries the bar state at xi = eta = 0)
+    enode = q4[:, 0::2] // 2
+    centre_q4, gp_q4 = lfo.element_fields(xy, enode, g["elem_material"], lfo.PLANE_STRESS, u) if n_q4 else (
+        np.zeros((0, 7)), np.zeros((0, 4, 9)))
+    bar_fields = np.array([lfo.truss_fields(xy[d[0::2] // 2].ravel(), g["truss_props"][t][0], u[d])
+                           for t, d in enumerate(bars)]).reshape(n_bar, 7)
+    gp_bar = np.concatenate([np.zeros((n_bar, 4, 2)), np.repeat(bar_fields[:, None, :], 4, axis=1)], axis=2)
+    assert np.array_equal(bits(np.concatenate([centre_q4, bar_fields])), bits(g["elem_centre"]))
+    assert np.array_equal(bits(np.concatenate([gp_q4.reshape(-1, 4, 9), gp_bar])), bits(g["elem_gp"]))
+    nodal = lfo.smooth_nodal_ex(xy.shape[0], enode, gp_q4.reshape(-1, 36), bars[:, 0::2] // 2, bar_fields[:, 3:6])
+    assert np.array_equal(bits(nodal), bits(g["node_fields"]))
+    if name == "plate_with_bar":
+        assert pinned == 1  # the free bar end has no transverse stiffness
+
+
+def test_bar_statics():
+    """u = FL/(EA) for the bar in tension and the two-bar drop P L / (2 E A sin^2 45)
+    (tests/test_solver.cpp:271-334)."""
+    g = load_golden("truss_tension")
+    assert abs(g["u"][2] - 1000.0 * 1.0 / (2.1e11 * 1.0e-3)) <= 1e-18
+    g = load_golden("truss_triangle")
+    sin45 = np.sqrt(0.5)
+    assert abs(g["u"][5] + 1000.0 * np.sqrt(2.0) / (2.0 * 2.1e11 * 1.0e-3 * sin45 * sin45)) <= 1e-18
+    # a bar along x has stiffness only along x: EA/L on the axial dofs, nothing transverse
+    ke = lfo.truss_stiffness([0.0, 0.0, 2.0, 0.0], 3.0, 0.5).reshape(4, 4)
+    assert np.array_equal(ke, np.array([[0.75, 0, -0.75, 0], [0, 0, 0, 0], [-0.75, 0, 0.75, 0], [0, 0, 0, 0]]))
+    # rigid translations and the rotation about the bar's midpoint (0.7, 0.35) cost no
+    # energy; an elongation along the bar costs EA/L per unit
+    ke = lfo.truss_stiffness([0.3, -0.2, 1.1, 0.9], 7.0, 0.2).reshape(4, 4)
+    for mode in ([1, 0, 1, 0], [0, 1, 0, 1], [0.55, -0.4, -0.55, 0.4]):
+        assert np.allclose(ke @ np.array(mode), 0.0, atol=1e-12)
+    length = np.hypot(0.8, 1.1)
+    c, s = 0.8 / length, 1.1 / length
+    stretch = np.array([-c, -s, c, s]) / 2  # unit elongation
+    assert np.isclose(stretch @ ke @ stretch, 7.0 * 0.2 / length, rtol=1e-14)
+
+
+def test_zero_pivot_sweep_reports_a_loaded_mechanism():
+    """A dof with no stiffness and a load is reported, not pinned (solver.cpp:922-934)."""
+    row_ptr = np.array([0, 1, 2])
+    col = np.array([0, 1])
+    values = np.array([2.0, 0.0])
+    _, _, _, bad = lfo.pin_zero_pivots(row_ptr, col, values, np.array([1.0, 1.0]))
+    assert bad == 1
+    kp, fp, pinned, bad = lfo.pin_zero_pivots(row_ptr, col, values, np.array([1.0, 1e-13]))
+    assert bad == -1 and pinned == 1 and kp[1] == 1.0 and fp[1] == 0.0
