@@ -38,7 +38,8 @@ typedef enum {
     LFG_ERR_NOT_CONVERGED = 5,  /* solver.cpp:948-950 */
     LFG_ERR_HIP = 6,            /* HIP runtime error (device, memory) */
     LFG_ERR_STATE = 7,          /* call made before the stage it needs */
-    LFG_ERR_UNSUPPORTED = 8     /* option outside this backend's scope */
+    LFG_ERR_UNSUPPORTED = 8,    /* option outside this backend's scope */
+    LFG_ERR_MECHANISM = 9       /* a zero pivot carries a load (solver.cpp:922-934) */
 } lfg_status;
 
 typedef struct lfg_ctx lfg_ctx;
@@ -145,7 +146,18 @@ int lfg_set_material_modes(lfg_ctx* ctx, const int32_t* modes);
 int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* row_ptr, const int32_t* col,
                     const int32_t* emap);
 
-/* Colour groups of element_colors (solver.cpp:100-133): color_ptr n_colors+1, elements ascending per colour. */
+/* Bars (truss members, element.cpp:250-306) after the Q4 elements: bar t is element
+ * n_elems + t. bar_nodes 2 node indices per bar, bar_material the material index (E
+ * from the materials table), area the cross-section, bar_map 16 value indices per bar
+ * (lfh_csr_pattern_bars; the pattern given to lfg_set_pattern holds their couplings).
+ * After lfg_set_pattern (which drops earlier bars) and before lfg_set_colors; serial
+ * assembly adds them after the Q4 contributions of each value, and lfg_postprocess
+ * returns n_elems + n_bars rows of element fields. */
+int lfg_set_bars(lfg_ctx* ctx, int64_t n_bars, const int32_t* bar_nodes, const int32_t* bar_material,
+                 const double* area, const int32_t* bar_map);
+
+/* Colour groups of element_colors (solver.cpp:100-133): color_ptr n_colors+1, elements
+ * ascending per colour (Q4 elements and bars, bar t as element n_elems + t). */
 int lfg_set_colors(lfg_ctx* ctx, int32_t n_colors, const int32_t* color_ptr, const int32_t* color_elems);
 
 /* Element stiffness alone (q4_stiffness, element.cpp:130-161) into ke_out (64*n_elems), for checks. */
@@ -168,6 +180,19 @@ int lfg_set_load(lfg_ctx* ctx, const double* f);
 /* Symmetric Dirichlet elimination (solver.cpp:355-413) into a constrained copy;
  * the unconstrained values and load stay for reactions. dofs ascending. */
 int lfg_dirichlet(lfg_ctx* ctx, int64_t n_constraints, const int32_t* dofs, const double* values);
+
+/* The zero-pivot sweep after the elimination (solver.cpp:916-940): a constrained-system
+ * diagonal of magnitude at most 1e-30 (a bar end no other element stiffens) becomes 1
+ * with a zero right-hand side, unless that right-hand side exceeds 1e-12 of the largest
+ * |rhs|: then LFG_ERR_MECHANISM, with the first such dof, its load and diagonal in the
+ * report (the reference's message names them and the dof's node). */
+typedef struct {
+    int64_t pinned;  /* dofs set to diagonal 1 */
+    int64_t dof;     /* first loaded zero-pivot dof, -1 if none */
+    double load;
+    double diag;
+} lfg_pivot_report;
+int lfg_pin_zero_pivots(lfg_ctx* ctx, lfg_pivot_report* report);
 int lfg_get_system(lfg_ctx* ctx, double* values_out, double* rhs_out);
 
 /* PCG on the constrained system (solve_iterative + pcg_solve, solver.cpp:532-747). */

@@ -23,9 +23,19 @@ int64_t lfh_csr_pattern_nnz(int64_t n_nodes, int64_t n_elems, const int32_t* eno
 int lfh_csr_pattern(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int32_t* row_ptr, int32_t* col,
                     int32_t* emap);
 
+/* The same with bars (2 node indices each in bnodes, after the Q4 elements in the
+ * element order of solver.cpp:59-83): bmap gets 16 entries per bar (local r*4+c). */
+int64_t lfh_csr_pattern_nnz_bars(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int64_t n_bars,
+                                 const int32_t* bnodes);
+int lfh_csr_pattern_bars(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int64_t n_bars,
+                         const int32_t* bnodes, int32_t* row_ptr, int32_t* col, int32_t* emap, int32_t* bmap);
+
 /* reverse_cuthill_mckee (reordering.cpp:11-62): node ids in RCM order. node_ids
  * lists the model's nodes; enode_ids the 4 node ids of each Q4 element. */
 int lfh_rcm(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int32_t* enode_ids, int32_t* order_out);
+/* ... with bars (2 node ids each): their node pairs join the graph (reordering.cpp:25-28). */
+int lfh_rcm_bars(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int32_t* enode_ids,
+                 int64_t n_bars, const int32_t* bnode_ids, int32_t* order_out);
 
 /* element_colors (solver.cpp:100-133): greedy first-fit colour per element in
  * element order (elements sharing a node conflict). Returns the colour count via
@@ -34,6 +44,10 @@ int lfh_rcm(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int
  * room for n_elems+1 entries suffices). */
 int lfh_element_colors(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int32_t* color_out,
                        int32_t* n_colors_out, int32_t* color_ptr, int32_t* color_elems);
+/* ... over the Q4 elements then the bars (element n_elems + t is bar t). */
+int lfh_element_colors_bars(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int64_t n_bars,
+                            const int32_t* bnodes, int32_t* color_out, int32_t* n_colors_out, int32_t* color_ptr,
+                            int32_t* color_elems);
 
 /* Subdomains of the `block` preconditioner: build_dof_subdomain's node split
  * (solver.cpp:631-654) into node_subdomain (n_nodes), with split 0 = "rcb" (recursive

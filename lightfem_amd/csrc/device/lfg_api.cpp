@@ -102,7 +102,16 @@ struct lfg_ctx {
     DevArray<double> xy, materials;
     DevArray<int> mat_mode;  // per-material plane modes (lfg_set_material_modes), else empty
     DevArray<int> enodes, elem_mat, node_elem_ptr, node_elem;
+    std::vector<int> h_enodes;  // Q4 node indices (node -> element lists when bars join)
     bool have_mesh = false;
+    // bars (truss members), elements n_elems .. n_elems + n_bars - 1
+    int n_bars = 0;
+    DevArray<int> bar_nodes, bar_mat, bar_map;
+    DevArray<double> bar_area;
+    std::vector<int> h_bar_nodes;
+    // zero-pivot sweep: max |f| bits and {pinned, first loaded zero-pivot row}
+    DevArray<unsigned long long> pivot_scale;
+    DevArray<int> pivot_counters;
 
     // pattern
     int n = 0;
@@ -126,8 +135,10 @@ struct lfg_ctx {
     lfg::StencilView stencil_sym{}, stencil_exact{};
     DevArray<double> stencil64, stencil_exact64, stencil_diag;
     DevArray<float> stencil32;
-    std::vector<int> h_color_ptr;
+    std::vector<int> h_color_ptr;      // Q4 elements of colour c: color_elems[h_color_ptr[c] ..)
     DevArray<int> color_elems;
+    std::vector<int> h_color_bar_ptr;  // bars of colour c: color_bars[h_color_bar_ptr[c] ..)
+    DevArray<int> color_bars;
 
     // values
     DevArray<double> K, Kc, f, fc, ke_buf;
@@ -196,6 +207,28 @@ lfg::MeshView mesh_view(const lfg_ctx* c) {
     return lfg::MeshView{c->n_nodes,     c->n_elems,     c->xy.p,         c->enodes.p,
                          c->elem_mat.p,  c->materials.p, c->plane_mode,
                          c->mat_mode.p};
+}
+
+lfg::BarView bar_view(const lfg_ctx* c) {
+    return lfg::BarView{c->n_bars, c->n_elems, c->bar_nodes.p, c->bar_mat.p, c->bar_area.p, c->bar_map.p};
+}
+
+// node -> (element << 2 | local node) in ascending element order (Q4, then bars), for
+// smoothing
+void build_node_elements(lfg_ctx* c) {
+    const int n_nodes = c->n_nodes, n_q4 = c->n_elems, n_bars = c->n_bars;
+    std::vector<int> ptr(static_cast<size_t>(n_nodes) + 1, 0);
+    for (int k = 0; k < 4 * n_q4; ++k) ptr[static_cast<size_t>(c->h_enodes[k]) + 1]++;
+    for (int k = 0; k < 2 * n_bars; ++k) ptr[static_cast<size_t>(c->h_bar_nodes[k]) + 1]++;
+    for (int i = 0; i < n_nodes; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int> list(static_cast<size_t>(ptr[n_nodes]));
+    std::vector<int> cur(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < n_q4; ++e)
+        for (int l = 0; l < 4; ++l) list[cur[c->h_enodes[4 * e + l]]++] = (e << 2) | l;
+    for (int t = 0; t < n_bars; ++t)
+        for (int l = 0; l < 2; ++l) list[cur[c->h_bar_nodes[2 * t + l]]++] = ((n_q4 + t) << 2) | l;
+    c->node_elem_ptr.upload(ptr.data(), ptr.size(), c->stream);
+    c->node_elem.upload(list.data(), list.size(), c->stream);
 }
 
 lfg::CsrView csr_view(const lfg_ctx* c) { return lfg::CsrView{c->n, c->nnz, c->row_ptr.p, c->col.p, c->max_row}; }
@@ -474,15 +507,18 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 // Stable counting sort of the element->value map by target: for every CSR value,
 // the (element, local entry) pairs that add into it, in ascending element order.
-void build_gather_map(lfg_ctx* c, const int32_t* emap) {
-    const long long total = 64LL * c->n_elems;
+// Contribution k < 64 n_elems is Q4 entry k; 64 n_elems + 16 t + j is entry j of bar t
+// (the bar matrices follow the Q4 ones in the element-matrix buffer).
+void build_gather_map(lfg_ctx* c, const int32_t* emap, const int32_t* bmap = nullptr, int n_bars = 0) {
+    const long long q4 = 64LL * c->n_elems;
+    const long long total = q4 + 16LL * n_bars;
+    auto target = [&](long long k) { return static_cast<size_t>(k < q4 ? emap[k] : bmap[k - q4]); };
     std::vector<int> counts(static_cast<size_t>(c->nnz) + 1, 0);
-    for (long long k = 0; k < total; ++k) counts[static_cast<size_t>(emap[k]) + 1]++;
+    for (long long k = 0; k < total; ++k) counts[target(k) + 1]++;
     for (long long k = 0; k < c->nnz; ++k) counts[static_cast<size_t>(k) + 1] += counts[static_cast<size_t>(k)];
     std::vector<int> contrib(static_cast<size_t>(total));
     std::vector<int> cursor(counts.begin(), counts.end() - 1);
-    for (long long k = 0; k < total; ++k) contrib[static_cast<size_t>(cursor[static_cast<size_t>(emap[k])]++)] =
-        static_cast<int>(k);
+    for (long long k = 0; k < total; ++k) contrib[static_cast<size_t>(cursor[target(k)]++)] = static_cast<int>(k);
     c->cptr.upload(counts.data(), counts.size(), c->stream);
     c->contrib.upload(contrib.data(), contrib.size(), c->stream);
     sync(c);
@@ -1014,6 +1050,7 @@ const char* status_text(int s) {
         case LFG_ERR_HIP: return "HIP runtime error";
         case LFG_ERR_STATE: return "call order violated";
         case LFG_ERR_UNSUPPORTED: return "unsupported option";
+        case LFG_ERR_MECHANISM: return "the structure is a mechanism";
         default: return "unknown status";
     }
 }
@@ -1058,7 +1095,8 @@ const char* lfg_last_error(const lfg_ctx* ctx) { return ctx ? ctx->last_error.c_
 int lfg_set_mesh(lfg_ctx* ctx, int64_t n_nodes, const double* xy, int64_t n_elems, const int32_t* enodes,
                  const int32_t* elem_material, int64_t n_materials, const double* materials, int32_t plane_mode) {
     return guarded(ctx, [&] {
-        require(n_nodes > 0 && n_elems > 0 && n_materials > 0 && xy && enodes && elem_material && materials,
+        require(n_nodes > 0 && n_elems >= 0 && n_materials > 0 && xy && materials &&
+                    (n_elems == 0 || (enodes && elem_material)),
                 LFG_ERR_ARG, "mesh arrays must be non-empty");
         require(n_nodes < (1LL << 30) && n_elems < (1LL << 24), LFG_ERR_UNSUPPORTED,
                 "mesh exceeds the int32 index range of this backend");
@@ -1078,16 +1116,10 @@ int lfg_set_mesh(lfg_ctx* ctx, int64_t n_nodes, const double* xy, int64_t n_elem
         ctx->elem_mat.upload(elem_material, static_cast<size_t>(n_elems), ctx->stream);
         ctx->materials.upload(materials, 3 * static_cast<size_t>(n_materials), ctx->stream);
         ctx->mat_mode.release();
-        // node -> (element << 2 | local node) in ascending element order, for smoothing
-        std::vector<int> ptr(static_cast<size_t>(n_nodes) + 1, 0);
-        for (int64_t k = 0; k < 4 * n_elems; ++k) ptr[static_cast<size_t>(enodes[k]) + 1]++;
-        for (int64_t i = 0; i < n_nodes; ++i) ptr[i + 1] += ptr[i];
-        std::vector<int> list(static_cast<size_t>(4 * n_elems));
-        std::vector<int> cur(ptr.begin(), ptr.end() - 1);
-        for (int64_t e = 0; e < n_elems; ++e)
-            for (int l = 0; l < 4; ++l) list[cur[enodes[4 * e + l]]++] = static_cast<int>((e << 2) | l);
-        ctx->node_elem_ptr.upload(ptr.data(), ptr.size(), ctx->stream);
-        ctx->node_elem.upload(list.data(), list.size(), ctx->stream);
+        ctx->h_enodes.assign(enodes, enodes + 4 * n_elems);
+        ctx->n_bars = 0;
+        ctx->h_bar_nodes.clear();
+        build_node_elements(ctx);
         sync(ctx);
         ctx->have_mesh = true;
         ctx->have_K = ctx->have_system = ctx->have_solution = false;
@@ -1133,7 +1165,8 @@ int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* ro
     return guarded(ctx, [&] {
         require(ctx->have_mesh, LFG_ERR_STATE, "lfg_set_mesh must precede lfg_set_pattern");
         require(n_rows == 2LL * ctx->n_nodes, LFG_ERR_ARG, "CSR dimension must be two dofs per node");
-        require(nnz > 0 && nnz < (1LL << 31) && row_ptr && col && emap, LFG_ERR_ARG, "invalid CSR pattern");
+        require(nnz > 0 && nnz < (1LL << 31) && row_ptr && col && (emap || ctx->n_elems == 0), LFG_ERR_ARG,
+                "invalid CSR pattern");
         require(row_ptr[0] == 0 && row_ptr[n_rows] == nnz, LFG_ERR_ARG, "row offsets do not span the nonzeros");
         for (int64_t k = 0; k < 64LL * ctx->n_elems; ++k)
             require(emap[k] >= 0 && emap[k] < nnz, LFG_ERR_ARG, "element map entry outside the CSR values");
@@ -1159,6 +1192,88 @@ int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* ro
         ctx->precond_ready = -1;
         ctx->sub_node.clear();  // subdomains belong to the node numbering of a pattern
         ctx->n_sub = 0;
+        if (ctx->n_bars > 0) {  // bar maps index the previous pattern
+            ctx->n_bars = 0;
+            ctx->h_bar_nodes.clear();
+            build_node_elements(ctx);
+            sync(ctx);
+        }
+        ctx->h_color_ptr.clear();
+        ctx->h_color_bar_ptr.clear();
+        return LFG_OK;
+    });
+}
+
+int lfg_set_bars(lfg_ctx* ctx, int64_t n_bars, const int32_t* bar_nodes, const int32_t* bar_material,
+                 const double* area, const int32_t* bar_map) {
+    return guarded(ctx, [&] {
+        require(ctx->have_pattern, LFG_ERR_STATE, "lfg_set_pattern must precede lfg_set_bars");
+        require(n_bars >= 0 && n_bars < (1LL << 24) && (n_bars == 0 || (bar_nodes && bar_material && area && bar_map)),
+                LFG_ERR_ARG, "bar arrays must be given");
+        for (int64_t k = 0; k < 2 * n_bars; ++k)
+            require(bar_nodes[k] >= 0 && bar_nodes[k] < ctx->n_nodes, LFG_ERR_ARG,
+                    "validated truss references an unknown node");
+        for (int64_t t = 0; t < n_bars; ++t)
+            require(bar_material[t] >= 0 && bar_material[t] < ctx->n_mats, LFG_ERR_ARG,
+                    "validated truss references an unknown material");
+        for (int64_t k = 0; k < 16 * n_bars; ++k)
+            require(bar_map[k] >= 0 && bar_map[k] < ctx->nnz, LFG_ERR_ARG, "bar map entry is outside the pattern");
+        LFG_HIP(hipSetDevice(ctx->device));
+        ctx->n_bars = static_cast<int>(n_bars);
+        ctx->h_bar_nodes.assign(bar_nodes, bar_nodes + 2 * n_bars);
+        ctx->bar_nodes.upload(bar_nodes, 2 * static_cast<size_t>(n_bars), ctx->stream);
+        ctx->bar_mat.upload(bar_material, static_cast<size_t>(n_bars), ctx->stream);
+        ctx->bar_area.upload(area, static_cast<size_t>(n_bars), ctx->stream);
+        ctx->bar_map.upload(bar_map, 16 * static_cast<size_t>(n_bars), ctx->stream);
+        // the serial gather and the smoothing lists take the bars after the Q4 elements
+        std::vector<int32_t> emap(64 * static_cast<size_t>(ctx->n_elems));
+        ctx->emap.download(emap.data(), emap.size(), ctx->stream);
+        sync(ctx);
+        build_gather_map(ctx, emap.data(), bar_map, ctx->n_bars);
+        build_node_elements(ctx);
+        ctx->h_color_ptr.clear();  // colour lists name the elements with the bars included
+        ctx->h_color_bar_ptr.clear();
+        sync(ctx);
+        ctx->have_K = ctx->have_system = ctx->have_solution = false;
+        ctx->precond_ready = -1;
+        return LFG_OK;
+    });
+}
+
+int lfg_pin_zero_pivots(lfg_ctx* ctx, lfg_pivot_report* report) {
+    return guarded(ctx, [&] {
+        require(ctx->have_system, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_pin_zero_pivots");
+        LFG_HIP(hipSetDevice(ctx->device));
+        ctx->pivot_scale.alloc(1);
+        ctx->pivot_counters.alloc(2);
+        const int init[2] = {0, INT32_MAX};
+        LFG_HIP(hipMemsetAsync(ctx->pivot_scale.p, 0, sizeof(unsigned long long), ctx->stream));
+        LFG_HIP(hipMemcpyAsync(ctx->pivot_counters.p, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+        lfg::launch_pin_zero_pivots(ctx->stream, csr_view(ctx), ctx->Kc.p, ctx->fc.p, ctx->pivot_scale.p,
+                                    ctx->pivot_counters.p);
+        int out[2] = {0, 0};
+        LFG_HIP(hipMemcpyAsync(out, ctx->pivot_counters.p, sizeof(out), hipMemcpyDeviceToHost, ctx->stream));
+        sync(ctx);
+        lfg_pivot_report rep{};
+        rep.pinned = out[0];
+        rep.dof = out[1] == INT32_MAX ? -1 : out[1];
+        if (rep.dof >= 0) {
+            // its load and diagonal, untouched by the sweep
+            const int row = static_cast<int>(rep.dof);
+            LFG_HIP(hipMemcpy(&rep.load, ctx->fc.p + row, sizeof(double), hipMemcpyDeviceToHost));
+            for (int k = ctx->h_row_ptr[row]; k < ctx->h_row_ptr[row + 1]; ++k)
+                if (ctx->h_col[k] == row) LFG_HIP(hipMemcpy(&rep.diag, ctx->Kc.p + k, sizeof(double), hipMemcpyDeviceToHost));
+        }
+        if (report) *report = rep;
+        if (rep.pinned > 0) {  // the packed copies and factors of Kc are stale
+            ctx->k32_valid = false;
+            ctx->slices64_valid = ctx->slices32_valid = false;
+            ctx->stencil64_valid = ctx->stencil32_valid = ctx->stencil_exact_valid = false;
+            ctx->precond_ready = -1;
+        }
+        if (rep.dof >= 0)
+            throw LfgError(LFG_ERR_MECHANISM, "the structure is a mechanism: a bar end has no transverse stiffness but "
+                                              "carries a transverse load");
         return LFG_OK;
     });
 }
@@ -1166,12 +1281,25 @@ int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* ro
 int lfg_set_colors(lfg_ctx* ctx, int32_t n_colors, const int32_t* color_ptr, const int32_t* color_elems) {
     return guarded(ctx, [&] {
         require(ctx->have_mesh, LFG_ERR_STATE, "lfg_set_mesh must precede lfg_set_colors");
-        require(n_colors > 0 && color_ptr && color_elems && color_ptr[0] == 0 && color_ptr[n_colors] == ctx->n_elems,
+        const int total = ctx->n_elems + ctx->n_bars;
+        require(n_colors > 0 && color_ptr && color_elems && color_ptr[0] == 0 && color_ptr[n_colors] == total,
                 LFG_ERR_ARG, "colour groups must partition the elements");
-        for (int64_t k = 0; k < ctx->n_elems; ++k)
-            require(color_elems[k] >= 0 && color_elems[k] < ctx->n_elems, LFG_ERR_ARG, "colour lists an unknown element");
-        ctx->h_color_ptr.assign(color_ptr, color_ptr + n_colors + 1);
-        ctx->color_elems.upload(color_elems, static_cast<size_t>(ctx->n_elems), ctx->stream);
+        for (int64_t k = 0; k < total; ++k)
+            require(color_elems[k] >= 0 && color_elems[k] < total, LFG_ERR_ARG, "colour lists an unknown element");
+        // each colour as its Q4 elements and its bars (bar t is element n_elems + t)
+        std::vector<int> q4, bars;
+        ctx->h_color_ptr.assign(1, 0);
+        ctx->h_color_bar_ptr.assign(1, 0);
+        for (int c = 0; c < n_colors; ++c) {
+            for (int k = color_ptr[c]; k < color_ptr[c + 1]; ++k) {
+                if (color_elems[k] < ctx->n_elems) q4.push_back(color_elems[k]);
+                else bars.push_back(color_elems[k] - ctx->n_elems);
+            }
+            ctx->h_color_ptr.push_back(static_cast<int>(q4.size()));
+            ctx->h_color_bar_ptr.push_back(static_cast<int>(bars.size()));
+        }
+        ctx->color_elems.upload(q4.data(), q4.size(), ctx->stream);
+        ctx->color_bars.upload(bars.data(), bars.size(), ctx->stream);
         sync(ctx);
         return LFG_OK;
     });
@@ -1201,13 +1329,17 @@ int lfg_assemble(lfg_ctx* ctx, int32_t strategy, lfg_assembly_report* report) {
         LFG_HIP(hipSetDevice(ctx->device));
         ensure_vectors(ctx);
         const lfg::MeshView m = mesh_view(ctx);
-        LFG_HIP(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), ctx->stream));
+        const lfg::BarView bv = bar_view(ctx);
+        LFG_HIP(hipMemsetAsync(ctx->flags.p, 0, 4 * sizeof(int), ctx->stream));
         lfg_assembly_report rep{};
         if (strategy == LFG_ASSEMBLY_SERIAL || strategy == LFG_ASSEMBLY_THREAD_LOCAL) {
-            ctx->ke_buf.alloc(64 * static_cast<size_t>(ctx->n_elems));
+            const size_t q4 = 64 * static_cast<size_t>(ctx->n_elems);
+            ctx->ke_buf.alloc(q4 + 16 * static_cast<size_t>(ctx->n_bars));
             LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
             lfg::launch_ke(ctx->stream, m, lfg::KE_STORE, nullptr, ctx->n_elems, nullptr, nullptr, ctx->ke_buf.p,
                            ctx->flags.p);
+            lfg::launch_bar_ke(ctx->stream, m, bv, lfg::KE_STORE, nullptr, ctx->n_bars, nullptr, ctx->ke_buf.p + q4,
+                               ctx->flags.p + 3);
             LFG_HIP(hipEventRecord(ctx->ev_ring[0], ctx->stream));
             lfg::launch_gather(ctx->stream, ctx->nnz, ctx->cptr.p, ctx->contrib.p, ctx->ke_buf.p, ctx->K.p);
             LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
@@ -1219,6 +1351,8 @@ int lfg_assemble(lfg_ctx* ctx, int32_t strategy, lfg_assembly_report* report) {
             lfg::launch_fill(ctx->stream, ctx->K.p, ctx->nnz, 0.0);
             lfg::launch_ke(ctx->stream, m, lfg::KE_ATOMIC, nullptr, ctx->n_elems, ctx->emap.p, ctx->K.p, nullptr,
                            ctx->flags.p);
+            lfg::launch_bar_ke(ctx->stream, m, bv, lfg::KE_ATOMIC, nullptr, ctx->n_bars, ctx->K.p, nullptr,
+                               ctx->flags.p + 3);
             LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
             sync(ctx);
             rep.assembly_ms = elapsed(ctx->ev_start, ctx->ev_stop);  // Ke fused with the scatter
@@ -1231,6 +1365,9 @@ int lfg_assemble(lfg_ctx* ctx, int32_t strategy, lfg_assembly_report* report) {
                 const int begin = ctx->h_color_ptr[cidx];
                 lfg::launch_ke(ctx->stream, m, lfg::KE_COLORED, ctx->color_elems.p + begin,
                                ctx->h_color_ptr[cidx + 1] - begin, ctx->emap.p, ctx->K.p, nullptr, ctx->flags.p);
+                const int bbegin = ctx->h_color_bar_ptr[cidx];
+                lfg::launch_bar_ke(ctx->stream, m, bv, lfg::KE_COLORED, ctx->color_bars.p + bbegin,
+                                   ctx->h_color_bar_ptr[cidx + 1] - bbegin, ctx->K.p, nullptr, ctx->flags.p + 3);
             }
             LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
             sync(ctx);
@@ -1239,11 +1376,13 @@ int lfg_assemble(lfg_ctx* ctx, int32_t strategy, lfg_assembly_report* report) {
         } else {
             throw LfgError(LFG_ERR_ARG, "validated model has an unknown assembly strategy");
         }
-        int failures = 0;
-        LFG_HIP(hipMemcpy(&failures, ctx->flags.p, sizeof(int), hipMemcpyDeviceToHost));
+        int fl[4] = {0, 0, 0, 0};
+        LFG_HIP(hipMemcpy(fl, ctx->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost));
+        const int failures = fl[0];
         rep.jacobian_failures = failures;
         if (report) *report = rep;
         if (failures) throw LfgError(LFG_ERR_JACOBIAN, "Q4 stiffness evaluation received a non-positive Jacobian");
+        if (fl[3]) throw LfgError(LFG_ERR_JACOBIAN, "truss stiffness evaluation received a zero-length member");
         ctx->have_K = true;
         ctx->have_system = ctx->have_solution = false;
         ctx->precond_ready = -1;
@@ -1558,12 +1697,13 @@ int lfg_postprocess(lfg_ctx* ctx, double* centre, double* gauss_points, double* 
     return guarded(ctx, [&] {
         require(ctx->have_solution, LFG_ERR_STATE, "lfg_pcg must precede lfg_postprocess");
         LFG_HIP(hipSetDevice(ctx->device));
-        const size_t e = static_cast<size_t>(ctx->n_elems);
+        const size_t e = static_cast<size_t>(ctx->n_elems) + static_cast<size_t>(ctx->n_bars);
         ctx->centre.alloc(7 * e);
         ctx->gp.alloc(36 * e);
         ctx->nodal.alloc(4 * static_cast<size_t>(ctx->n_nodes));
         lfg::launch_fields(ctx->stream, mesh_view(ctx), ctx->x.p, ctx->centre.p, ctx->gp.p);
-        lfg::launch_smooth(ctx->stream, ctx->n_nodes, ctx->node_elem_ptr.p, ctx->node_elem.p, ctx->gp.p,
+        lfg::launch_bar_fields(ctx->stream, mesh_view(ctx), bar_view(ctx), ctx->x.p, ctx->centre.p, ctx->gp.p);
+        lfg::launch_smooth(ctx->stream, ctx->n_nodes, ctx->n_elems, ctx->node_elem_ptr.p, ctx->node_elem.p, ctx->gp.p,
                            ctx->nodal.p);
         if (centre) ctx->centre.download(centre, 7 * e, ctx->stream);
         if (gauss_points) ctx->gp.download(gauss_points, 36 * e, ctx->stream);

@@ -161,6 +161,94 @@ __global__ __launch_bounds__(256) void ke_kernel(MeshView m, const int* __restri
     }
 }
 
+// ---- bars (element.cpp:266-306)
+
+// Bar t: end coordinates, E of its material and its area; the axial direction and
+// length as the reference forms them. Returns false for a zero-length bar.
+__device__ __forceinline__ bool bar_geometry(const MeshView& m, const BarView& b, int t, double& c, double& s,
+                                             double& length, double& young, double& area) {
+    const int n0 = b.nodes[2 * t], n1 = b.nodes[2 * t + 1];
+    const double dx = m.xy[2 * n1] - m.xy[2 * n0];
+    const double dy = m.xy[2 * n1 + 1] - m.xy[2 * n0 + 1];
+    length = sqrt(dx * dx + dy * dy);
+    young = m.materials[3 * b.mat[t]];
+    area = b.area[t];
+    if (!(length > 1.0e-12) || !isfinite(length)) return false;
+    c = dx / length;
+    s = dy / length;
+    return true;
+}
+
+// EA/L [c^2 cs -c^2 -cs; cs s^2 -cs -s^2; ...] (element.cpp:266-286), one thread per bar.
+template <int Mode>
+__global__ __launch_bounds__(256) void bar_ke_kernel(MeshView m, BarView b, const int* __restrict__ bars, int count,
+                                                     double* __restrict__ values, double* __restrict__ ke_out,
+                                                     int* __restrict__ failures) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= count) return;
+    const int t = bars ? bars[slot] : slot;
+    double c, sn, length, young, area;
+    double k[16];
+    if (!bar_geometry(m, b, t, c, sn, length, young, area)) {
+        atomicAdd(failures, 1);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) k[i] = 0.0;
+    } else {
+        const double scale = young * area / length;
+        const double cc = c * c, ss = sn * sn, cs = c * sn;
+        const double v[16] = {scale * cc,  scale * cs,  -scale * cc, -scale * cs, scale * cs,  scale * ss,
+                              -scale * cs, -scale * ss, -scale * cc, -scale * cs, scale * cc,  scale * cs,
+                              -scale * cs, -scale * ss, scale * cs,  scale * ss};
+#pragma unroll
+        for (int i = 0; i < 16; ++i) k[i] = v[i];
+    }
+    if (Mode == KE_STORE) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ke_out[16LL * t + i] = k[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int at = b.map[16 * t + i];
+            if (Mode == KE_ATOMIC) unsafeAtomicAdd(values + at, k[i]);
+            else values[at] += k[i];  // conflict-free within one colour
+        }
+    }
+}
+
+// Bar fields (element.cpp:288-306, solver.cpp:753-803): axial strain, the stress tensor
+// sigma (c^2, s^2, cs), von Mises |sigma|; every Gauss point carries the same state at
+// xi = eta = 0. Rows n_q4 + t of the centre / Gauss-point arrays.
+__global__ __launch_bounds__(256) void bar_fields_kernel(MeshView m, BarView b, const double* __restrict__ u,
+                                                         double* __restrict__ centre, double* __restrict__ gp) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.n_bars) return;
+    double c, sn, length, young, area;
+    double f[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (bar_geometry(m, b, t, c, sn, length, young, area)) {
+        const int n0 = b.nodes[2 * t], n1 = b.nodes[2 * t + 1];
+        const double du = u[2 * n1] - u[2 * n0];
+        const double dv = u[2 * n1 + 1] - u[2 * n0 + 1];
+        const double strain = (du * c + dv * sn) / length;
+        const double sigma = young * strain;
+        f[0] = strain;
+        f[3] = sigma * c * c;
+        f[4] = sigma * sn * sn;
+        f[5] = sigma * c * sn;
+        f[6] = fabs(sigma);
+    }
+    const long long e = static_cast<long long>(b.n_q4) + t;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) centre[7 * e + i] = f[i];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double* g = gp + 36 * e + 9 * q;
+        g[0] = 0.0;
+        g[1] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) g[2 + i] = f[i];
+    }
+}
+
 // Serial-order assembly as a gather (solver.cpp:135-142): each CSR value sums its
 // contributions in ascending element order starting from 0.0.
 __global__ __launch_bounds__(256) void gather_kernel(long long nnz, const int* __restrict__ cptr,
@@ -241,7 +329,7 @@ __global__ __launch_bounds__(128) void fields_kernel(MeshView m, const double* _
 // solver.cpp:807-871 as a gather: node i sums extrapolated Gauss-point stresses of
 // its elements in ascending element order (the order the reference's element
 // loop adds them), then averages and forms the nodal von Mises stress.
-__global__ __launch_bounds__(256) void smooth_kernel(int n_nodes, const int* __restrict__ ptr,
+__global__ __launch_bounds__(256) void smooth_kernel(int n_nodes, int n_q4, const int* __restrict__ ptr,
                                                      const int* __restrict__ list, const double* __restrict__ gp,
                                                      double* __restrict__ nodal) {
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
@@ -262,6 +350,11 @@ __global__ __launch_bounds__(256) void smooth_kernel(int n_nodes, const int* __r
         const int e = entry >> 2;
         const int local = entry & 3;
         const double* g = gp + 36LL * e;
+        if (e >= n_q4) {  // a bar adds its stress tensor once (solver.cpp:841-851)
+#pragma unroll
+            for (int comp = 0; comp < 3; ++comp) sums[comp] += g[5 + comp];
+            continue;
+        }
 #pragma unroll
         for (int comp = 0; comp < 3; ++comp)
 #pragma unroll
@@ -360,9 +453,29 @@ void launch_sensitivity(hipStream_t s, const MeshView& m, const double* u, const
     sensitivity_kernel<<<blocks_for(8LL * m.n_elems, 256), 256, 0, s>>>(m, u, lam, dE, grad);
 }
 
-void launch_smooth(hipStream_t s, int n_nodes, const int* ptr, const int* list, const double* gp, double* nodal) {
+void launch_smooth(hipStream_t s, int n_nodes, int n_q4, const int* ptr, const int* list, const double* gp,
+                   double* nodal) {
     if (n_nodes <= 0) return;
-    smooth_kernel<<<blocks_for(n_nodes, 256), 256, 0, s>>>(n_nodes, ptr, list, gp, nodal);
+    smooth_kernel<<<blocks_for(n_nodes, 256), 256, 0, s>>>(n_nodes, n_q4, ptr, list, gp, nodal);
+}
+
+void launch_bar_ke(hipStream_t s, const MeshView& m, const BarView& b, KeMode mode, const int* bars, int count,
+                   double* values, double* ke_out, int* failures) {
+    if (count <= 0) return;
+    const unsigned int grid = blocks_for(count, 256);
+    switch (mode) {
+        case KE_STORE: bar_ke_kernel<KE_STORE><<<grid, 256, 0, s>>>(m, b, bars, count, values, ke_out, failures); break;
+        case KE_ATOMIC: bar_ke_kernel<KE_ATOMIC><<<grid, 256, 0, s>>>(m, b, bars, count, values, ke_out, failures); break;
+        case KE_COLORED:
+            bar_ke_kernel<KE_COLORED><<<grid, 256, 0, s>>>(m, b, bars, count, values, ke_out, failures);
+            break;
+    }
+}
+
+void launch_bar_fields(hipStream_t s, const MeshView& m, const BarView& b, const double* u, double* centre,
+                       double* gp) {
+    if (b.n_bars <= 0) return;
+    bar_fields_kernel<<<blocks_for(b.n_bars, 256), 256, 0, s>>>(m, b, u, centre, gp);
 }
 
 }  // namespace lfg

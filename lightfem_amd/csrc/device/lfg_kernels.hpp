@@ -20,6 +20,16 @@ struct MeshView {
     const int* mat_mode;     // plane mode per material (batches mixing modes), or null: plane_mode
 };
 
+// Bars (truss members) after the Q4 elements: bar t is element n_q4 + t.
+struct BarView {
+    int n_bars;
+    int n_q4;
+    const int* nodes;    // 2 node indices per bar
+    const int* mat;      // material index per bar (E from MeshView::materials)
+    const double* area;  // cross-section area per bar
+    const int* map;      // 16 value indices per bar (row-major over its 4 dofs)
+};
+
 struct CsrView {
     int n;
     long long nnz;
@@ -100,8 +110,16 @@ void launch_gather(hipStream_t s, long long nnz, const int* cptr, const int* con
                    double* values);
 void launch_fill(hipStream_t s, double* p, long long n, double v);
 void launch_fields(hipStream_t s, const MeshView& m, const double* u, double* centre, double* gp);
-void launch_smooth(hipStream_t s, int n_nodes, const int* node_elem_ptr, const int* node_elem, const double* gp,
-                   double* nodal);
+// Nodal smoothing over node -> (element << 2 | local node) lists in element order;
+// elements n_q4 and up are bars (their stress tensor counts once, solver.cpp:841-851).
+void launch_smooth(hipStream_t s, int n_nodes, int n_q4, const int* node_elem_ptr, const int* node_elem,
+                   const double* gp, double* nodal);
+// Bar matrices (bars == nullptr: all bars in order), as KE_STORE into ke_out (16 per
+// bar), KE_ATOMIC or KE_COLORED into values; zero-length bars count in failures.
+void launch_bar_ke(hipStream_t s, const MeshView& m, const BarView& b, KeMode mode, const int* bars, int count,
+                   double* values, double* ke_out, int* failures);
+void launch_bar_fields(hipStream_t s, const MeshView& m, const BarView& b, const double* u, double* centre,
+                       double* gp);
 // grad[e] = -lam_e^T Ke(E = 1) u_e * (dE ? dE[e] : 1).
 void launch_sensitivity(hipStream_t s, const MeshView& m, const double* u, const double* lam, const double* dE,
                         double* grad);
@@ -109,6 +127,11 @@ void launch_sensitivity(hipStream_t s, const MeshView& m, const double* u, const
 // ---- solver kernels (lfg_solver.hip)
 void launch_dirichlet(hipStream_t s, const CsrView& a, const double* values, const double* f, const int* constrained,
                       const double* prescribed, double* kc, double* fc);
+// The zero-pivot sweep after the elimination (solver.cpp:916-940) on (values, f) in
+// place. scale_bits must start at 0, counters at {0, INT_MAX}: counters[0] gets the
+// pinned count, counters[1] the first loaded zero-pivot row (INT_MAX: none).
+void launch_pin_zero_pivots(hipStream_t s, const CsrView& a, double* values, double* f,
+                            unsigned long long* scale_bits, int* counters);
 void launch_spmv_serial_rows(hipStream_t s, const CsrView& a, const double* values, const double* x, const double* sub,
                              double* y);
 void launch_extract_diag(hipStream_t s, const CsrView& a, const double* values, double* diag, int* status);

@@ -26,6 +26,50 @@ inline unsigned int blocks_for(long long work, int per_block) {
 
 inline unsigned int round8(unsigned int b) { return (b + 7u) & ~7u; }
 
+// ------------------------------------------------------------ zero pivots
+
+// max |f| into *out as the bit pattern of a non-negative double (ordered like its value).
+__global__ __launch_bounds__(256) void abs_max_kernel(int n, const double* __restrict__ f,
+                                                      unsigned long long* __restrict__ out) {
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = fmax(m, fabs(f[i]));
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0 && m > 0.0)
+        atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(m)));
+}
+
+// solver.cpp:916-940, one row per thread: a diagonal of magnitude at most 1e-30 becomes
+// 1 with its right-hand side 0, unless that right-hand side exceeds 1e-12 of the
+// largest (1 when all are zero): then the row is reported (the smallest such row in
+// counters[1]). counters[0] counts the pinned rows.
+__global__ __launch_bounds__(256) void pin_pivots_kernel(int n, const int* __restrict__ rp,
+                                                         const int* __restrict__ col, double* __restrict__ vals,
+                                                         double* __restrict__ f,
+                                                         const unsigned long long* __restrict__ scale_bits,
+                                                         int* __restrict__ counters) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = rp[i], hi = rp[i + 1];
+    while (lo < hi) {  // the diagonal's position (columns sorted; the pattern always holds it)
+        const int mid = (lo + hi) >> 1;
+        if (col[mid] < i) lo = mid + 1;
+        else hi = mid;
+    }
+    const bool present = lo < rp[i + 1] && col[lo] == i;
+    const double diag = present ? vals[lo] : 0.0;
+    if (fabs(diag) > 1.0e-30) return;
+    double scale = __longlong_as_double(static_cast<long long>(*scale_bits));
+    if (scale <= 0.0) scale = 1.0;
+    if (fabs(f[i]) > 1.0e-12 * scale || !present) {
+        atomicMin(counters + 1, i);
+        return;
+    }
+    vals[lo] = 1.0;
+    f[i] = 0.0;
+    atomicAdd(counters, 1);
+}
+
 // ------------------------------------------------------------ Dirichlet
 
 // solver.cpp:355-413. One node group of 16 lanes per row pair would scatter the
@@ -935,6 +979,14 @@ void launch_dirichlet(hipStream_t s, const CsrView& a, const double* values, con
     if (a.n <= 0) return;
     dirichlet_kernel<<<blocks_for(8LL * a.n, 256), 256, 0, s>>>(a.n, a.row_ptr, a.col, values, f, constrained,
                                                                  prescribed, kc, fc);
+}
+
+void launch_pin_zero_pivots(hipStream_t s, const CsrView& a, double* values, double* f,
+                            unsigned long long* scale_bits, int* counters) {
+    if (a.n <= 0) return;
+    const unsigned int grid = blocks_for(a.n, 256);
+    abs_max_kernel<<<grid < 1024u ? grid : 1024u, 256, 0, s>>>(a.n, f, scale_bits);
+    pin_pivots_kernel<<<grid, 256, 0, s>>>(a.n, a.row_ptr, a.col, values, f, scale_bits, counters);
 }
 
 void launch_spmv_serial_rows(hipStream_t s, const CsrView& a, const double* values, const double* x,

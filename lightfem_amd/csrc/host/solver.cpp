@@ -52,6 +52,10 @@ struct HostArrays {
     std::vector<int32_t> enode_ids;   // node ids (for RCM)
     std::vector<int32_t> elem_mat;
     std::vector<double> materials;
+    std::vector<int32_t> bar_nodes;     // 2 node indices per bar
+    std::vector<int32_t> bar_node_ids;  // 2 node ids per bar (for RCM)
+    std::vector<int32_t> bar_mat;
+    std::vector<double> bar_area;
 };
 
 HostArrays host_arrays(const Model& model) {
@@ -80,23 +84,38 @@ HostArrays host_arrays(const Model& model) {
         if (!m) throw std::logic_error("validated element references an unknown material");
         h.elem_mat.push_back(static_cast<int32_t>(m - model.materials.data()));
     }
+    for (const TrussElement& t : model.trusses) {
+        for (const int id : t.node_ids) {
+            const Node* node = model.find_node(id);
+            if (!node) throw std::logic_error("validated truss references an unknown node");
+            h.bar_nodes.push_back(static_cast<int32_t>(node->dof_x / 2));
+            h.bar_node_ids.push_back(id);
+        }
+        const Material* m = model.find_material(t.material_id);
+        if (!m) throw std::logic_error("validated truss references an unknown material");
+        h.bar_mat.push_back(static_cast<int32_t>(m - model.materials.data()));
+        h.bar_area.push_back(t.area);
+    }
     return h;
 }
 
 struct Pattern {
-    std::vector<int32_t> row_ptr, col, emap;
+    std::vector<int32_t> row_ptr, col, emap, bmap;
 };
 
 Pattern build_pattern(const Model& model, const HostArrays& h) {
     Pattern p;
     const int64_t n_nodes = static_cast<int64_t>(model.nodes.size());
     const int64_t n_elems = static_cast<int64_t>(model.elements.size());
-    const int64_t nnz = lfh_csr_pattern_nnz(n_nodes, n_elems, h.enodes.data());
+    const int64_t n_bars = static_cast<int64_t>(model.trusses.size());
+    const int64_t nnz = lfh_csr_pattern_nnz_bars(n_nodes, n_elems, h.enodes.data(), n_bars, h.bar_nodes.data());
     if (nnz < 0) throw std::out_of_range("element DOF is outside CSR matrix dimension");
     p.row_ptr.resize(static_cast<std::size_t>(2 * n_nodes + 1));
     p.col.resize(static_cast<std::size_t>(nnz));
     p.emap.resize(static_cast<std::size_t>(64 * n_elems));
-    if (lfh_csr_pattern(n_nodes, n_elems, h.enodes.data(), p.row_ptr.data(), p.col.data(), p.emap.data()) != 0)
+    p.bmap.resize(static_cast<std::size_t>(16 * n_bars));
+    if (lfh_csr_pattern_bars(n_nodes, n_elems, h.enodes.data(), n_bars, h.bar_nodes.data(), p.row_ptr.data(),
+                             p.col.data(), p.emap.data(), p.bmap.data()) != 0)
         throw std::overflow_error("CSR pattern exceeds the int32 index range of the gfx950 backend");
     return p;
 }
@@ -168,13 +187,15 @@ std::vector<int> reverse_cuthill_mckee(const Model& model) {
     if (!model.is_indexed()) throw std::logic_error("model must be indexed before RCM ordering");
     std::vector<int32_t> ids(model.nodes.size());
     for (std::size_t i = 0; i < model.nodes.size(); ++i) ids[i] = model.nodes[i].id;
-    std::vector<int32_t> enode_ids;
+    std::vector<int32_t> enode_ids, bar_ids;
     enode_ids.reserve(4 * model.elements.size());
     for (const Q4Element& e : model.elements)
         for (const int id : e.node_ids) enode_ids.push_back(id);
+    for (const TrussElement& t : model.trusses)
+        for (const int id : t.node_ids) bar_ids.push_back(id);
     std::vector<int32_t> order(ids.size());
-    if (lfh_rcm(static_cast<int64_t>(ids.size()), ids.data(), static_cast<int64_t>(model.elements.size()),
-                enode_ids.data(), order.data()) != 0)
+    if (lfh_rcm_bars(static_cast<int64_t>(ids.size()), ids.data(), static_cast<int64_t>(model.elements.size()),
+                     enode_ids.data(), static_cast<int64_t>(model.trusses.size()), bar_ids.data(), order.data()) != 0)
         throw std::logic_error("validated element references an unknown node");
     return std::vector<int>(order.begin(), order.end());
 }
@@ -182,8 +203,6 @@ std::vector<int> reverse_cuthill_mckee(const Model& model) {
 AnalysisResult LinearElasticSolver::solve(Model& model) const {
     if (!model.is_indexed()) throw std::logic_error("model must pass validation and sort_and_index before solving");
     const SolverSettings& s = model.settings;
-    if (!model.trusses.empty())
-        throw std::runtime_error("truss members are outside the gfx950 backend's scope (Q4 models only)");
     const auto solve_start = std::chrono::steady_clock::now();
     AnalysisResult result;
     result.performance.reordering = s.reordering;
@@ -208,12 +227,17 @@ AnalysisResult LinearElasticSolver::solve(Model& model) const {
                            static_cast<int64_t>(model.materials.size()), h.materials.data(),
                            s.plane_mode == PlaneMode::PlaneStrain ? LFG_PLANE_STRAIN : LFG_PLANE_STRESS));
     dev.check(lfg_set_pattern(dev.ctx, n, nnz, pat.row_ptr.data(), pat.col.data(), pat.emap.data()));
+    const int64_t n_bars = static_cast<int64_t>(model.trusses.size());
+    if (n_bars > 0)
+        dev.check(lfg_set_bars(dev.ctx, n_bars, h.bar_nodes.data(), h.bar_mat.data(), h.bar_area.data(),
+                               pat.bmap.data()));
     const int strategy = assembly_code(s.assembly);
     if (strategy == LFG_ASSEMBLY_COLORING) {
-        std::vector<int32_t> color(static_cast<std::size_t>(n_elems)), ptr(static_cast<std::size_t>(n_elems) + 2),
-            elems(static_cast<std::size_t>(n_elems));
+        const std::size_t total = static_cast<std::size_t>(n_elems + n_bars);
+        std::vector<int32_t> color(total), ptr(total + 2), elems(total);
         int32_t count = 0;
-        lfh_element_colors(n_nodes, n_elems, h.enodes.data(), color.data(), &count, ptr.data(), elems.data());
+        lfh_element_colors_bars(n_nodes, n_elems, h.enodes.data(), n_bars, h.bar_nodes.data(), color.data(), &count,
+                                ptr.data(), elems.data());
         dev.check(lfg_set_colors(dev.ctx, count, ptr.data(), elems.data()));
     }
     lfg_assembly_report asm_rep{};
@@ -257,6 +281,20 @@ AnalysisResult LinearElasticSolver::solve(Model& model) const {
         cvals.push_back(c.second);
     }
     dev.check(lfg_dirichlet(dev.ctx, static_cast<int64_t>(cdofs.size()), cdofs.data(), cvals.data()));
+    // solver.cpp:916-940: a bar end no other element stiffens leaves a zero pivot
+    lfg_pivot_report pivots{};
+    const int pivot_rc = lfg_pin_zero_pivots(dev.ctx, &pivots);
+    if (pivot_rc == LFG_ERR_MECHANISM) {
+        const std::size_t i = static_cast<std::size_t>(pivots.dof);
+        const Node& node = model.nodes[i / 2];
+        throw std::runtime_error(
+            "the structure is a mechanism: a bar end has no transverse stiffness but carries a transverse load; add "
+            "a support there or load the bar along its axis [dof " + std::to_string(i) + " load " +
+            std::to_string(pivots.load) + " diag " + std::to_string(pivots.diag) + " node " +
+            std::to_string(node.id) + " xy " + std::to_string(node.position.x) + "," +
+            std::to_string(node.position.y) + "]");
+    }
+    dev.check(pivot_rc);
     result.performance.boundary_ms = ms_since(boundary_start);
 
     // block: subdomains of build_dof_subdomain (solver.cpp:631-654); one node per
@@ -348,13 +386,15 @@ AnalysisResult LinearElasticSolver::solve(Model& model) const {
     const auto post_start = std::chrono::steady_clock::now();
     result.reaction.assign(static_cast<std::size_t>(n), 0.0);
     dev.check(lfg_reactions(dev.ctx, result.reaction.data()));
-    std::vector<double> centre(7 * static_cast<std::size_t>(n_elems)), gp(36 * static_cast<std::size_t>(n_elems)),
+    const int64_t n_results = n_elems + n_bars;  // Q4 elements, then bars (solver.cpp:753-805)
+    std::vector<double> centre(7 * static_cast<std::size_t>(n_results)), gp(36 * static_cast<std::size_t>(n_results)),
         nodal(4 * static_cast<std::size_t>(n_nodes));
     dev.check(lfg_postprocess(dev.ctx, centre.data(), gp.data(), nodal.data()));
-    result.element_results.resize(static_cast<std::size_t>(n_elems));
-    for (int64_t e = 0; e < n_elems; ++e) {
+    result.element_results.resize(static_cast<std::size_t>(n_results));
+    for (int64_t e = 0; e < n_results; ++e) {
         ElementResult& er = result.element_results[static_cast<std::size_t>(e)];
-        er.element_id = model.elements[static_cast<std::size_t>(e)].id;
+        er.element_id = e < n_elems ? model.elements[static_cast<std::size_t>(e)].id
+                                    : model.trusses[static_cast<std::size_t>(e - n_elems)].id;
         const double* c = centre.data() + 7 * e;
         er.field.strain = {c[0], c[1], c[2]};
         er.field.stress = {c[3], c[4], c[5]};

@@ -17,32 +17,46 @@
 
 namespace {
 
+// Q4 elements (4 node indices each) followed by bars (2 each), the element order of
+// solver.cpp:59-83: element e < n_q4 is a Q4, e >= n_q4 is bar e - n_q4.
+struct Elems {
+    int64_t n_q4 = 0, n_bars = 0;
+    const int32_t* q4 = nullptr;
+    const int32_t* bars = nullptr;
+    int64_t count() const { return n_q4 + n_bars; }
+    int nodes(int64_t e) const { return e < n_q4 ? 4 : 2; }
+    int node(int64_t e, int l) const { return e < n_q4 ? q4[4 * e + l] : bars[2 * (e - n_q4) + l]; }
+};
+
 struct NodeElements {
     std::vector<int> ptr;
     std::vector<int> elems;  // ascending element index per node
 };
 
-NodeElements node_elements(int64_t n_nodes, int64_t n_elems, const int32_t* enodes) {
+NodeElements node_elements(int64_t n_nodes, const Elems& el) {
     NodeElements ne;
     ne.ptr.assign(static_cast<size_t>(n_nodes) + 1, 0);
-    for (int64_t k = 0; k < 4 * n_elems; ++k) ne.ptr[static_cast<size_t>(enodes[k]) + 1]++;
+    for (int64_t e = 0; e < el.count(); ++e)
+        for (int l = 0; l < el.nodes(e); ++l) ne.ptr[static_cast<size_t>(el.node(e, l)) + 1]++;
     for (int64_t i = 0; i < n_nodes; ++i) ne.ptr[i + 1] += ne.ptr[i];
-    ne.elems.resize(static_cast<size_t>(4 * n_elems));
+    ne.elems.resize(static_cast<size_t>(ne.ptr[n_nodes]));
     std::vector<int> cur(ne.ptr.begin(), ne.ptr.end() - 1);
-    for (int64_t e = 0; e < n_elems; ++e)
-        for (int l = 0; l < 4; ++l) ne.elems[cur[enodes[4 * e + l]]++] = static_cast<int>(e);
+    for (int64_t e = 0; e < el.count(); ++e)
+        for (int l = 0; l < el.nodes(e); ++l) ne.elems[cur[el.node(e, l)]++] = static_cast<int>(e);
     return ne;
 }
 
-bool valid_elements(int64_t n_nodes, int64_t n_elems, const int32_t* enodes) {
-    if (n_nodes <= 0 || n_elems < 0 || (n_elems > 0 && !enodes)) return false;
-    for (int64_t k = 0; k < 4 * n_elems; ++k)
-        if (enodes[k] < 0 || enodes[k] >= n_nodes) return false;
+bool valid_elements(int64_t n_nodes, const Elems& el) {
+    if (n_nodes <= 0 || el.n_q4 < 0 || el.n_bars < 0 || (el.n_q4 > 0 && !el.q4) || (el.n_bars > 0 && !el.bars))
+        return false;
+    for (int64_t e = 0; e < el.count(); ++e)
+        for (int l = 0; l < el.nodes(e); ++l)
+            if (el.node(e, l) < 0 || el.node(e, l) >= n_nodes) return false;
     return true;
 }
 
 // Sorted unique neighbour nodes (including the node itself when it has elements).
-void neighbour_lists(int64_t n_nodes, const int32_t* enodes, const NodeElements& ne, std::vector<int>& ptr,
+void neighbour_lists(int64_t n_nodes, const Elems& el, const NodeElements& ne, std::vector<int>& ptr,
                      std::vector<int>& list) {
     ptr.assign(static_cast<size_t>(n_nodes) + 1, 0);
     std::vector<std::vector<int>> tmp(static_cast<size_t>(n_nodes));
@@ -51,7 +65,7 @@ void neighbour_lists(int64_t n_nodes, const int32_t* enodes, const NodeElements&
         std::vector<int>& v = tmp[static_cast<size_t>(i)];
         for (int j = ne.ptr[i]; j < ne.ptr[i + 1]; ++j) {
             const int e = ne.elems[j];
-            for (int l = 0; l < 4; ++l) v.push_back(enodes[4 * e + l]);
+            for (int l = 0; l < el.nodes(e); ++l) v.push_back(el.node(e, l));
         }
         std::sort(v.begin(), v.end());
         v.erase(std::unique(v.begin(), v.end()), v.end());
@@ -68,26 +82,39 @@ int64_t row_length(const std::vector<int>& nptr, int64_t node) {
     return m == 0 ? 1 : 2 * m;  // an unused node keeps only its diagonal
 }
 
-}  // namespace
+// The map of one element's (2n)^2 entries (row-major over its dofs) into the pattern.
+void element_map(const Elems& el, int64_t e, const std::vector<int>& nptr, const std::vector<int>& nlist,
+                 const int32_t* row_ptr, int32_t* out) {
+    const int n = el.nodes(e), d = 2 * n;
+    for (int r = 0; r < d; ++r) {
+        const int a = el.node(e, r / 2);
+        const int* begin = nlist.data() + nptr[a];
+        const int* end = nlist.data() + nptr[a + 1];
+        const int32_t row_start = row_ptr[2 * a + (r & 1)];
+        for (int c = 0; c < d; ++c) {
+            const int b = el.node(e, c / 2);
+            const int pos = static_cast<int>(std::lower_bound(begin, end, b) - begin);
+            out[d * r + c] = row_start + 2 * pos + (c & 1);
+        }
+    }
+}
 
-extern "C" {
-
-int64_t lfh_csr_pattern_nnz(int64_t n_nodes, int64_t n_elems, const int32_t* enodes) {
-    if (!valid_elements(n_nodes, n_elems, enodes)) return -1;
-    const NodeElements ne = node_elements(n_nodes, n_elems, enodes);
+int64_t pattern_nnz(int64_t n_nodes, const Elems& el) {
+    if (!valid_elements(n_nodes, el)) return -1;
+    const NodeElements ne = node_elements(n_nodes, el);
     std::vector<int> nptr, nlist;
-    neighbour_lists(n_nodes, enodes, ne, nptr, nlist);
+    neighbour_lists(n_nodes, el, ne, nptr, nlist);
     int64_t nnz = 0;
     for (int64_t i = 0; i < n_nodes; ++i) nnz += 2 * row_length(nptr, i);
     return nnz;
 }
 
-int lfh_csr_pattern(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int32_t* row_ptr, int32_t* col,
-                    int32_t* emap) {
-    if (!valid_elements(n_nodes, n_elems, enodes) || !row_ptr || !col || (n_elems > 0 && !emap)) return 1;
-    const NodeElements ne = node_elements(n_nodes, n_elems, enodes);
+int pattern(int64_t n_nodes, const Elems& el, int32_t* row_ptr, int32_t* col, int32_t* emap, int32_t* bmap) {
+    if (!valid_elements(n_nodes, el) || !row_ptr || !col || (el.n_q4 > 0 && !emap) || (el.n_bars > 0 && !bmap))
+        return 1;
+    const NodeElements ne = node_elements(n_nodes, el);
     std::vector<int> nptr, nlist;
-    neighbour_lists(n_nodes, enodes, ne, nptr, nlist);
+    neighbour_lists(n_nodes, el, ne, nptr, nlist);
     int64_t total = 0;
     row_ptr[0] = 0;
     for (int64_t i = 0; i < n_nodes; ++i) {
@@ -112,24 +139,15 @@ int lfh_csr_pattern(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int
         }
     }
 #pragma omp parallel for schedule(static)
-    for (int64_t e = 0; e < n_elems; ++e) {
-        for (int r = 0; r < 8; ++r) {
-            const int a = enodes[4 * e + r / 2];
-            const int* begin = nlist.data() + nptr[a];
-            const int* end = nlist.data() + nptr[a + 1];
-            const int32_t row_start = row_ptr[2 * a + (r & 1)];
-            for (int c = 0; c < 8; ++c) {
-                const int b = enodes[4 * e + c / 2];
-                const int pos = static_cast<int>(std::lower_bound(begin, end, b) - begin);
-                emap[64 * e + 8 * r + c] = row_start + 2 * pos + (c & 1);
-            }
-        }
-    }
+    for (int64_t e = 0; e < el.count(); ++e)
+        element_map(el, e, nptr, nlist, row_ptr, e < el.n_q4 ? emap + 64 * e : bmap + 16 * (e - el.n_q4));
     return 0;
 }
 
-int lfh_rcm(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int32_t* enode_ids, int32_t* order_out) {
-    if (n_nodes <= 0 || !node_ids || !order_out || (n_elems > 0 && !enode_ids)) return 1;
+int rcm(int64_t n_nodes, const int32_t* node_ids, const Elems& id_el, int32_t* order_out) {
+    if (n_nodes <= 0 || !node_ids || !order_out || (id_el.n_q4 > 0 && !id_el.q4) ||
+        (id_el.n_bars > 0 && !id_el.bars))
+        return 1;
     // node id -> index in node_ids
     std::vector<int> by_id(static_cast<size_t>(n_nodes));
     for (int64_t i = 0; i < n_nodes; ++i) by_id[static_cast<size_t>(i)] = static_cast<int>(i);
@@ -141,15 +159,17 @@ int lfh_rcm(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int
         if (it == sorted_ids.end() || *it != id) return -1;
         return by_id[static_cast<size_t>(it - sorted_ids.begin())];
     };
-    std::vector<int32_t> en(static_cast<size_t>(4 * n_elems));
-    for (int64_t k = 0; k < 4 * n_elems; ++k) {
-        en[static_cast<size_t>(k)] = index_of(enode_ids[k]);
-        if (en[static_cast<size_t>(k)] < 0) return 2;
-    }
-    // distinct neighbours over the 6 node pairs of each Q4 (reordering.cpp:17-24)
-    const NodeElements ne = node_elements(n_nodes, n_elems, en.data());
+    std::vector<int32_t> q4(static_cast<size_t>(4 * id_el.n_q4)), bars(static_cast<size_t>(2 * id_el.n_bars));
+    for (int64_t k = 0; k < 4 * id_el.n_q4; ++k)
+        if ((q4[static_cast<size_t>(k)] = index_of(id_el.q4[k])) < 0) return 2;
+    for (int64_t k = 0; k < 2 * id_el.n_bars; ++k)
+        if ((bars[static_cast<size_t>(k)] = index_of(id_el.bars[k])) < 0) return 2;
+    const Elems el{id_el.n_q4, id_el.n_bars, q4.data(), bars.data()};
+    // distinct neighbours over the 6 node pairs of each Q4 and the pair of each bar
+    // (reordering.cpp:17-28)
+    const NodeElements ne = node_elements(n_nodes, el);
     std::vector<int> nptr, nlist;
-    neighbour_lists(n_nodes, en.data(), ne, nptr, nlist);
+    neighbour_lists(n_nodes, el, ne, nptr, nlist);
     std::vector<int> degree(static_cast<size_t>(n_nodes));
     std::vector<int> adj_ptr(static_cast<size_t>(n_nodes) + 1, 0), adj;
     adj.reserve(nlist.size());
@@ -159,7 +179,7 @@ int lfh_rcm(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int
         adj_ptr[i + 1] = static_cast<int>(adj.size());
         degree[static_cast<size_t>(i)] = adj_ptr[i + 1] - adj_ptr[i];
     }
-    const auto key_less = [&](int a, int b) {  // (degree, id), reordering.cpp:274-278
+    const auto key_less = [&](int a, int b) {  // (degree, id), reordering.cpp:30-34
         if (degree[a] != degree[b]) return degree[a] < degree[b];
         return node_ids[a] < node_ids[b];
     };
@@ -190,17 +210,18 @@ int lfh_rcm(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int
     return 0;
 }
 
-int lfh_element_colors(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int32_t* color_out,
-                       int32_t* n_colors_out, int32_t* color_ptr, int32_t* color_elems) {
-    if (!valid_elements(n_nodes, n_elems, enodes) || !color_out || !n_colors_out) return 1;
-    const NodeElements ne = node_elements(n_nodes, n_elems, enodes);
+int colors(int64_t n_nodes, const Elems& el, int32_t* color_out, int32_t* n_colors_out, int32_t* color_ptr,
+           int32_t* color_elems) {
+    if (!valid_elements(n_nodes, el) || !color_out || !n_colors_out) return 1;
+    const int64_t n_elems = el.count();
+    const NodeElements ne = node_elements(n_nodes, el);
     std::vector<int64_t> stamp_of(static_cast<size_t>(n_elems) + 1, 0);
     int max_color = -1;
     for (int64_t e = 0; e < n_elems; ++e) color_out[e] = -1;
     for (int64_t e = 0; e < n_elems; ++e) {
         const int64_t stamp = e + 1;
-        for (int l = 0; l < 4; ++l) {
-            const int node = enodes[4 * e + l];
+        for (int l = 0; l < el.nodes(e); ++l) {
+            const int node = el.node(e, l);
             for (int j = ne.ptr[node]; j < ne.ptr[node + 1]; ++j) {
                 const int other = ne.elems[j];
                 if (color_out[other] >= 0) stamp_of[static_cast<size_t>(color_out[other])] = stamp;
@@ -220,6 +241,49 @@ int lfh_element_colors(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, 
         for (int64_t e = 0; e < n_elems; ++e) color_elems[cur[color_out[e]]++] = static_cast<int32_t>(e);
     }
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t lfh_csr_pattern_nnz(int64_t n_nodes, int64_t n_elems, const int32_t* enodes) {
+    return pattern_nnz(n_nodes, Elems{n_elems, 0, enodes, nullptr});
+}
+
+int64_t lfh_csr_pattern_nnz_bars(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int64_t n_bars,
+                                 const int32_t* bnodes) {
+    return pattern_nnz(n_nodes, Elems{n_elems, n_bars, enodes, bnodes});
+}
+
+int lfh_csr_pattern(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int32_t* row_ptr, int32_t* col,
+                    int32_t* emap) {
+    return pattern(n_nodes, Elems{n_elems, 0, enodes, nullptr}, row_ptr, col, emap, nullptr);
+}
+
+int lfh_csr_pattern_bars(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int64_t n_bars,
+                         const int32_t* bnodes, int32_t* row_ptr, int32_t* col, int32_t* emap, int32_t* bmap) {
+    return pattern(n_nodes, Elems{n_elems, n_bars, enodes, bnodes}, row_ptr, col, emap, bmap);
+}
+
+int lfh_rcm(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int32_t* enode_ids, int32_t* order_out) {
+    return rcm(n_nodes, node_ids, Elems{n_elems, 0, enode_ids, nullptr}, order_out);
+}
+
+int lfh_rcm_bars(int64_t n_nodes, const int32_t* node_ids, int64_t n_elems, const int32_t* enode_ids,
+                 int64_t n_bars, const int32_t* bnode_ids, int32_t* order_out) {
+    return rcm(n_nodes, node_ids, Elems{n_elems, n_bars, enode_ids, bnode_ids}, order_out);
+}
+
+int lfh_element_colors(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int32_t* color_out,
+                       int32_t* n_colors_out, int32_t* color_ptr, int32_t* color_elems) {
+    return colors(n_nodes, Elems{n_elems, 0, enodes, nullptr}, color_out, n_colors_out, color_ptr, color_elems);
+}
+
+int lfh_element_colors_bars(int64_t n_nodes, int64_t n_elems, const int32_t* enodes, int64_t n_bars,
+                            const int32_t* bnodes, int32_t* color_out, int32_t* n_colors_out, int32_t* color_ptr,
+                            int32_t* color_elems) {
+    return colors(n_nodes, Elems{n_elems, n_bars, enodes, bnodes}, color_out, n_colors_out, color_ptr, color_elems);
 }
 
 int lfh_subdomain_split(int64_t n_nodes, const double* xy, int32_t split, int64_t nodes_per_subdomain,

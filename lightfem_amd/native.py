@@ -16,7 +16,8 @@ LIB_DIR = os.path.join(_PKG, "lib")
 GPU_LIB_PATH = os.path.join(LIB_DIR, "liblightfem_gpu.so")
 HOST_LIB_PATH = os.path.join(LIB_DIR, "liblightfem.so")
 
-OK, ERR_ARG, ERR_JACOBIAN, ERR_NOT_SPD, ERR_NONFINITE, ERR_NOT_CONVERGED, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED = range(9)
+OK, ERR_ARG, ERR_JACOBIAN, ERR_NOT_SPD, ERR_NONFINITE, ERR_NOT_CONVERGED, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, \
+    ERR_MECHANISM = range(10)
 PLANE_STRESS, PLANE_STRAIN = 0, 1
 ASSEMBLY = {"serial": 0, "atomic": 1, "coloring": 2, "thread_local": 3}
 PRECOND = {"jacobi": 0, "block": 1, "block2": 1, "ic0": 2, "subdomain": 3}
@@ -31,9 +32,11 @@ GPU_SYMBOLS = ["lfg_create", "lfg_destroy", "lfg_last_error", "lfg_status_string
                "lfg_set_load", "lfg_dirichlet", "lfg_get_system", "lfg_pcg", "lfg_reactions", "lfg_postprocess",
                "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv", "lfg_bench_copy", "lfg_set_materials",
                "lfg_adjoint_solve", "lfg_sensitivity", "lfg_set_material_modes", "lfg_batch_set_problems",
-               "lfg_batch_set_loads", "lfg_batch_pcg", "lfg_layout_info_get", "lfg_set_subdomains"]
+               "lfg_batch_set_loads", "lfg_batch_pcg", "lfg_layout_info_get", "lfg_set_subdomains", "lfg_set_bars",
+               "lfg_pin_zero_pivots"]
 HOST_SYMBOLS = ["lfh_csr_pattern_nnz", "lfh_csr_pattern", "lfh_rcm", "lfh_element_colors", "lfh_csr_profile",
-                "lfh_subdomain_split"]
+                "lfh_subdomain_split", "lfh_csr_pattern_nnz_bars", "lfh_csr_pattern_bars", "lfh_rcm_bars",
+                "lfh_element_colors_bars"]
 
 
 class LightFemError(RuntimeError):
@@ -80,6 +83,11 @@ class LayoutInfo(ctypes.Structure):
         out = {name: getattr(self, name) for name, _ in self._fields_[:5]}
         out["stencil_offsets"] = list(self.stencil_offsets[:self.stencil_exact_planes])
         return out
+
+
+class PivotReport(ctypes.Structure):
+    _fields_ = [("pinned", ctypes.c_int64), ("dof", ctypes.c_int64), ("load", ctypes.c_double),
+                ("diag", ctypes.c_double)]
 
 
 class BatchOptions(ctypes.Structure):
@@ -155,6 +163,8 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lfg_batch_pcg.argtypes = [_P, ctypes.POINTER(BatchOptions), _P, _P, ctypes.POINTER(BatchSummary)]
         lib.lfg_layout_info_get.argtypes = [_P, ctypes.POINTER(LayoutInfo)]
         lib.lfg_set_subdomains.argtypes = [_P, _I64, _P]
+        lib.lfg_set_bars.argtypes = [_P, _I64, _P, _P, _P, _P]
+        lib.lfg_pin_zero_pivots.argtypes = [_P, ctypes.POINTER(PivotReport)]
         _gpu = lib
     return _gpu
 
@@ -171,6 +181,11 @@ def host_lib() -> ctypes.CDLL:
         lib.lfh_element_colors.argtypes = [_I64, _I64, _P, _P, _P, _P, _P]
         lib.lfh_csr_profile.argtypes = [_I64, _P, _P, _P]
         lib.lfh_subdomain_split.argtypes = [_I64, _P, ctypes.c_int32, _I64, _P, _P]
+        lib.lfh_csr_pattern_nnz_bars.restype = ctypes.c_int64
+        lib.lfh_csr_pattern_nnz_bars.argtypes = [_I64, _I64, _P, _I64, _P]
+        lib.lfh_csr_pattern_bars.argtypes = [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P]
+        lib.lfh_rcm_bars.argtypes = [_I64, _P, _I64, _P, _I64, _P, _P]
+        lib.lfh_element_colors_bars.argtypes = [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P]
         _host = lib
     return _host
 
@@ -189,41 +204,48 @@ def _f64(a) -> np.ndarray:
 
 # ------------------------------------------------------------- host stages
 
-def csr_pattern(n_nodes: int, enodes: np.ndarray):
-    """(row_ptr, col, emap) of build_csr_assembly_pattern for Q4 node-index lists."""
+def csr_pattern(n_nodes: int, enodes: np.ndarray, bnodes: Optional[np.ndarray] = None):
+    """(row_ptr, col, emap) of build_csr_assembly_pattern for Q4 node-index lists; with
+    bars (node-index pairs) also bmap: (row_ptr, col, emap, bmap)."""
     enodes = _i32(enodes).reshape(-1, 4)
+    bars = _i32(bnodes if bnodes is not None else np.zeros((0, 2))).reshape(-1, 2)
     lib = host_lib()
-    nnz = lib.lfh_csr_pattern_nnz(n_nodes, enodes.shape[0], _ptr(enodes))
+    nnz = lib.lfh_csr_pattern_nnz_bars(n_nodes, enodes.shape[0], _ptr(enodes), bars.shape[0], _ptr(bars))
     if nnz < 0:
         raise LightFemError(ERR_ARG, "element DOF is outside CSR matrix dimension")
     row_ptr = np.empty(2 * n_nodes + 1, dtype=np.int32)
     col = np.empty(nnz, dtype=np.int32)
     emap = np.empty((enodes.shape[0], 64), dtype=np.int32)
-    rc = lib.lfh_csr_pattern(n_nodes, enodes.shape[0], _ptr(enodes), _ptr(row_ptr), _ptr(col), _ptr(emap))
+    bmap = np.empty((bars.shape[0], 16), dtype=np.int32)
+    rc = lib.lfh_csr_pattern_bars(n_nodes, enodes.shape[0], _ptr(enodes), bars.shape[0], _ptr(bars),
+                                  _ptr(row_ptr), _ptr(col), _ptr(emap), _ptr(bmap))
     if rc:
         raise LightFemError(ERR_ARG, "CSR pattern exceeds the int32 index range")
-    return row_ptr, col, emap
+    return (row_ptr, col, emap) if bnodes is None else (row_ptr, col, emap, bmap)
 
 
-def rcm(node_ids: np.ndarray, enode_ids: np.ndarray) -> np.ndarray:
+def rcm(node_ids: np.ndarray, enode_ids: np.ndarray, bnode_ids: Optional[np.ndarray] = None) -> np.ndarray:
     node_ids = _i32(node_ids)
     enode_ids = _i32(enode_ids).reshape(-1, 4)
+    bars = _i32(bnode_ids if bnode_ids is not None else np.zeros((0, 2))).reshape(-1, 2)
     out = np.empty(node_ids.shape[0], dtype=np.int32)
-    if host_lib().lfh_rcm(node_ids.shape[0], _ptr(node_ids), enode_ids.shape[0], _ptr(enode_ids), _ptr(out)):
+    if host_lib().lfh_rcm_bars(node_ids.shape[0], _ptr(node_ids), enode_ids.shape[0], _ptr(enode_ids),
+                               bars.shape[0], _ptr(bars), _ptr(out)):
         raise LightFemError(ERR_ARG, "RCM: element references an unknown node")
     return out
 
 
-def element_colors(n_nodes: int, enodes: np.ndarray):
-    """(color per element, color_ptr, color_elems)."""
+def element_colors(n_nodes: int, enodes: np.ndarray, bnodes: Optional[np.ndarray] = None):
+    """(color per element, color_ptr, color_elems); bars follow the Q4 elements."""
     enodes = _i32(enodes).reshape(-1, 4)
-    ne = enodes.shape[0]
+    bars = _i32(bnodes if bnodes is not None else np.zeros((0, 2))).reshape(-1, 2)
+    ne = enodes.shape[0] + bars.shape[0]
     colors = np.empty(ne, dtype=np.int32)
     count = ctypes.c_int32(0)
     ptr = np.empty(ne + 2, dtype=np.int32)
     elems = np.empty(ne, dtype=np.int32)
-    if host_lib().lfh_element_colors(n_nodes, ne, _ptr(enodes), _ptr(colors), ctypes.byref(count), _ptr(ptr),
-                                     _ptr(elems)):
+    if host_lib().lfh_element_colors_bars(n_nodes, enodes.shape[0], _ptr(enodes), bars.shape[0], _ptr(bars),
+                                          _ptr(colors), ctypes.byref(count), _ptr(ptr), _ptr(elems)):
         raise LightFemError(ERR_ARG, "colouring: element references an unknown node")
     return colors, ptr[:count.value + 1].copy(), elems
 
@@ -298,6 +320,7 @@ class Device:
                                            _ptr(elem_material), materials.shape[0], _ptr(materials), plane_mode))
         self.n_nodes = xy.shape[0]
         self.n_elems = enodes.shape[0]
+        self.n_bars = 0
 
     def set_materials(self, materials) -> None:
         materials = _f64(materials).reshape(-1, 3)
@@ -345,6 +368,23 @@ class Device:
                                               _ptr(col), _ptr(emap)))
         self.n = row_ptr.shape[0] - 1
         self.nnz = col.shape[0]
+        self.n_bars = 0  # the library drops bars with a new pattern
+
+    def set_bars(self, bar_nodes, bar_material, area, bar_map) -> None:
+        """Bars after the Q4 elements (after set_pattern, before set_colors)."""
+        bar_nodes = _i32(bar_nodes).reshape(-1, 2)
+        bar_material, area, bar_map = _i32(bar_material), _f64(area), _i32(bar_map).reshape(-1, 16)
+        self._check(self._lib.lfg_set_bars(self._ctx, bar_nodes.shape[0], _ptr(bar_nodes), _ptr(bar_material),
+                                           _ptr(area), _ptr(bar_map)))
+        self.n_bars = bar_nodes.shape[0]
+
+    def pin_zero_pivots(self, raise_on_error: bool = True) -> dict:
+        """The zero-pivot sweep after dirichlet(): pinned count, first loaded zero pivot."""
+        rep = PivotReport()
+        rc = self._lib.lfg_pin_zero_pivots(self._ctx, ctypes.byref(rep))
+        if rc != OK and raise_on_error:
+            self._check(rc)
+        return {name: getattr(rep, name) for name, _ in PivotReport._fields_}
 
     def set_colors(self, color_ptr, color_elems) -> None:
         color_ptr, color_elems = _i32(color_ptr), _i32(color_elems)
@@ -424,8 +464,10 @@ class Device:
         return out
 
     def postprocess(self):
-        centre = np.empty((self.n_elems, 7))
-        gp = np.empty((self.n_elems, 4, 9))
+        """Element fields (Q4 elements, then bars) and smoothed nodal stresses."""
+        e = self.n_elems + getattr(self, "n_bars", 0)
+        centre = np.empty((e, 7))
+        gp = np.empty((e, 4, 9))
         nodal = np.empty((self.n_nodes, 4))
         self._check(self._lib.lfg_postprocess(self._ctx, _ptr(centre), _ptr(gp), _ptr(nodal)))
         return centre, gp, nodal

@@ -17,7 +17,8 @@ def run(args, cwd):
     return subprocess.run([CLI, *args], capture_output=True, text=True, cwd=cwd, timeout=300)
 
 
-CLI_CASES = ["one_element", "scrambled", "extended", "grid6_bsr_block_rcb2", "grid6_block_chunk4"]
+CLI_CASES = ["one_element", "scrambled", "extended", "grid6_bsr_block_rcb2", "grid6_block_chunk4", "truss_tension",
+             "truss_triangle", "plate_with_bar"]
 
 
 @pytest.mark.parametrize("name", CLI_CASES)

@@ -119,3 +119,18 @@ def test_zero_pivot_sweep_reports_a_loaded_mechanism():
     assert bad == 1
     kp, fp, pinned, bad = lfo.pin_zero_pivots(row_ptr, col, values, np.array([1.0, 1e-13]))
     assert bad == -1 and pinned == 1 and kp[1] == 1.0 and fp[1] == 0.0
+
+
+@pytest.mark.parametrize("name", TRUSS_CASES)
+def test_host_pattern_and_colors_with_bars(name):
+    """The host library's bar-aware pattern, map and colouring (lfh_*_bars) are the
+    reference's, entry for entry."""
+    from lightfem_amd import native
+    g = load_golden(name)
+    q4, bars = decode(g)
+    n_nodes = g["xy"].shape[0]
+    row_ptr, col, emap, bmap = native.csr_pattern(n_nodes, q4[:, 0::2] // 2, bars[:, 0::2] // 2)
+    assert np.array_equal(row_ptr, g["row_ptr"]) and np.array_equal(col, g["col"])
+    assert np.array_equal(np.concatenate([emap.ravel(), bmap.ravel()]), g["emap_flat"])
+    colors, _, _ = native.element_colors(n_nodes, q4[:, 0::2] // 2, bars[:, 0::2] // 2)
+    assert np.array_equal(colors, g["colors"])
