@@ -171,6 +171,8 @@ def run_c2(dev, tol: float) -> dict:
         out[name] = {"device_solve_ms": round(rep["solve_ms"], 3), "iterations": rep["iterations"],
                      "converged": bool(rep["converged"]), "setup_ms": round(rep["setup_ms"], 3),
                      "us_per_iteration": round(1e3 * rep["solve_ms"] / max(1, rep["iterations"]), 2)}
+        if precond == "ic0":
+            out[name]["sweep_levels"] = rep["levels"]  # forward + backward
         print(f"bench: c2 {name} {json.dumps(out[name])}", file=sys.stderr, flush=True)
     return out
 
