@@ -87,16 +87,19 @@ __device__ __forceinline__ void load_element(const MeshView& m, int e, Coords& c
 // Row `row` of Ke (element.cpp:130-161): ke[c] = sum_gp (sum_s b[s][row] db[s][c]) * detJ * t.
 __device__ __forceinline__ bool ke_row(const Coords& c, double young, double nu, double thick, int plane_mode,
                                        int row, double ke[8]) {
-    const double points[4][2] = {{-kGauss, -kGauss}, {kGauss, -kGauss}, {kGauss, kGauss}, {-kGauss, kGauss}};
     double d[9];
     constitutive(young, nu, plane_mode, d);
 #pragma unroll
     for (int k = 0; k < 8; ++k) ke[k] = 0.0;
     bool ok = true;
-#pragma unroll
+    // Gauss points (-g,-g), (g,-g), (g,g), (-g,g) one at a time: unrolled, the four points'
+    // B and DB do not fit the register file and spill
+#pragma unroll 1
     for (int p = 0; p < 4; ++p) {
+        const double xi = (p == 1 || p == 2) ? kGauss : -kGauss;
+        const double eta = p >= 2 ? kGauss : -kGauss;
         double b[24], det;
-        ok = kinematics(c, points[p][0], points[p][1], b, det) && ok;
+        ok = kinematics(c, xi, eta, b, det) && ok;
         double db[24];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
@@ -108,7 +111,15 @@ __device__ __forceinline__ bool ke_row(const Coords& c, double young, double nu,
                 db[8 * r + col] = acc;
             }
         const double weight = det * thick;
-        const double b0 = b[row], b1 = b[8 + row], b2 = b[16 + row];
+        // column `row` of B: selects over constant indices (an index by the lane's row
+        // would move b to scratch memory)
+        double b0 = b[0], b1 = b[8], b2 = b[16];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) {
+            b0 = row == k ? b[k] : b0;
+            b1 = row == k ? b[8 + k] : b1;
+            b2 = row == k ? b[16 + k] : b2;
+        }
 #pragma unroll
         for (int col = 0; col < 8; ++col) {
             double contribution = 0.0;
