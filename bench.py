@@ -53,11 +53,13 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int, layout: dict) -
     plane and one plane per positive offset, its FP32 copy with an FP64 diagonal)."""
     n_nodes = n // 2
     if variant == 10:
-        # the whole Jacobi iteration: the symmetric planes, and per node x, r, q, p and the
-        # Jacobi inverse read (5 x 16 B) and x, r, p, q written (4 x 16 B)
+        # the whole Jacobi iteration: the symmetric planes, and per node r, q, p and the
+        # Jacobi inverse read (4 x 16 B), r, p, q written (3 x 16 B), and x updated every
+        # second launch for two iterations (x and the older p read, x written: 48 B per
+        # two launches); per launch on average 136 B
         planes = layout["stencil_sym_planes"]
         matrix = n_nodes * (planes * 32 if value_bytes == 8 else (planes - 1) * 16 + 32)
-        return matrix + n_nodes * 144
+        return matrix + n_nodes * 136
     if variant == 8:
         planes = layout["stencil_sym_planes"]
         matrix = n_nodes * (planes * 32 if value_bytes == 8 else (planes - 1) * 16 + 32)
@@ -193,6 +195,7 @@ def main() -> int:
     ap.add_argument("--batch-forms", default="fp64,mixed", help="comma-separated batch precisions to run")
     ap.add_argument("--skip-c2", action="store_true", help="skip the 200x200 preconditioner comparison")
     ap.add_argument("--c2-only", action="store_true", help="only the 200x200 preconditioner comparison (profiling)")
+    ap.add_argument("--assembly-only", action="store_true", help="only the assembly strategies (profiling)")
     ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--profile", type=int, default=0,
                     help="profiling run: this many FP64 SpMV launches and FP64 PCG iterations, nothing else")
@@ -204,6 +207,18 @@ def main() -> int:
 
     from lightfem_amd import meshgen, native, pipeline
 
+    if args.assembly_only:
+        # the three assembly strategies of the 1000x1000 cantilever, five times each (profiling)
+        dev = native.Device(0)
+        pipeline.setup(meshgen.c3_model(args.n), device=dev, assembly="coloring")
+        out = {}
+        for strategy in ("coloring", "atomic", "serial"):
+            reps = [dev.assemble(strategy) for _ in range(5)]
+            out[strategy] = {"element_stiffness_ms": float(np.median([r["element_stiffness_ms"] for r in reps])),
+                             "assembly_ms": float(np.median([r["assembly_ms"] for r in reps]))}
+        print(json.dumps({"assembly": out}))
+        dev.close()
+        return 0
     if args.c2_only:
         dev = native.Device(0)
         print(json.dumps({"c2_preconditioners": run_c2(dev, args.tol)}))

@@ -1595,6 +1595,7 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
         }
         ++k;
     }
+    if (stencil_fused) lfg::launch_pcg_fused_flush(ctx->stream, b, ctx->p2.p);  // pb[1] of enqueue_iteration
     sync(ctx);
     lfg::PcgState st{};
     LFG_HIP(hipMemcpy(&st, ctx->state.p, sizeof(st), hipMemcpyDeviceToHost));

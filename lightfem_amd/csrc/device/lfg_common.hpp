@@ -26,6 +26,7 @@ struct PcgState {
     double rz_new;     // r.z computed by the update kernel
     double switch_rr;  // mixed: FP32 values once ||r||^2 <= switch_rr
     double true_rr;    // mixed: ||r||^2 at the last replacement
+    double alpha_prev; // fused stencil iteration: the alpha before `alpha` (deferred x update)
     long long iter;
     long long max_iter;
     long long replacements;

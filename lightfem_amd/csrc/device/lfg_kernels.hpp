@@ -201,10 +201,15 @@ void launch_pcg_spmv_stencil(hipStream_t s, const PcgBuffers& b, const StencilVi
 // The last block applies the stop test to r', then alpha' = r'.z / p'.q' and
 // beta' = rho'' / r'.z with rho'' = r'.z - 2 alpha' q'.z + alpha'^2 q'.M^-1 q', the r.z
 // of the next residual, so no second reduction is needed. r, p, q alternate between
-// the *_in and *_out buffers from one launch to the next; x is updated in place.
+// the *_in and *_out buffers from one launch to the next (launch L, counted from 0
+// after pcg_init, reads buffer L & 1). x is updated in place on even launches only,
+// with both iterations' terms; launch_pcg_fused_flush completes it after the solve.
 void launch_pcg_fused_stencil(hipStream_t s, const PcgBuffers& b, const StencilView& v, const double* planes64,
                               const float* planes32, const double* diag_c, const double* r_in, double* r_out,
                               const double* p_in, double* p_out, const double* q_in, double* q_out);
+// After the last fused launch: the x term of an odd final launch (p_odd = its p_in,
+// the buffer odd launches read); nothing after an even one.
+void launch_pcg_fused_flush(hipStream_t s, const PcgBuffers& b, const double* p_odd);
 // x += alpha p, r -= alpha q, ||r||^2, r.z; convergence / replacement decision, beta.
 void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed);
 // mixed only: r = b - K x in FP64 when a replacement is due; then r.z and beta.

@@ -7,6 +7,10 @@
 
 namespace lfg {
 
+// inverse_diagonal_of (solver.cpp:656-669): 1 / d, or 1 for a diagonal of magnitude
+// at most 1e-30.
+__device__ __forceinline__ double jacobi_inverse(double d) { return fabs(d) > 1.0e-30 ? 1.0 / d : 1.0; }
+
 // precond.cpp:266-287 for a size-2 block: forward then backward substitution with
 // the packed factor (reciprocal pivots).
 __device__ __forceinline__ void block2_solve(const double* f, double r0, double r1, double& z0, double& z1) {

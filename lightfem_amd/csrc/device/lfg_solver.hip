@@ -147,7 +147,7 @@ __global__ void jacobi_inv_kernel(int n, const double* __restrict__ diag, double
     if (i >= n) return;
     const double d = diag[i];
     if (!isfinite(d)) atomicOr(status, 2);
-    inv[i] = fabs(d) > 1.0e-30 ? 1.0 / d : 1.0;
+    inv[i] = jacobi_inverse(d);
 }
 
 // SubdomainBlockJacobi factor for one-node subdomains (precond.cpp:178-232):
@@ -264,6 +264,7 @@ __global__ __launch_bounds__(kThreads) void pcg_init_kernel(PcgBuffers b, int st
         st->rho = trz;
         st->beta = 0.0;
         st->alpha = 0.0;
+        st->alpha_prev = 0.0;
         st->iter = fused ? -1 : 0;  // the first fused launch only forms p0 and A p0
         st->max_iter = max_iter;
         st->replacements = 0;

@@ -366,14 +366,19 @@ __device__ __forceinline__ void apply_sym_rows(const StencilView& v, const SymRo
 constexpr int kHalo = 3;  // halo window positions per thread and round
 
 // The tiles of one block for values of type VT (see pcg_fused_stencil_kernel).
+// x_mode: 0 leaves x alone, 1 adds alpha p_in, 2 adds alpha_prev p_out (read before
+// p_out is overwritten) and then alpha p_in. The Jacobi inverses are read, not formed
+// from the diagonal block: every node's inverse is first needed by another tile's
+// halo, so computing it there saves no HBM bytes and adds two divisions per halo node
+// (measured 1% slower at 1000x1000).
 template <typename VT, int kM, int Prec>
 __device__ __forceinline__ void fused_tiles(const PcgBuffers& b, const StencilView& v, const dpair* __restrict__ diag,
                                             int np_d, const VT* __restrict__ planes, int np, int kb,
                                             const double* __restrict__ r_in, double* __restrict__ r_out,
                                             const double* __restrict__ p_in, double* __restrict__ p_out,
                                             const double* __restrict__ q_in, double* __restrict__ q_out,
-                                            double alpha, double beta, int tile_begin, int tile_end, dpair* win,
-                                            double (&acc)[5]) {
+                                            double alpha, double beta, double alpha_prev, int x_mode,
+                                            int tile_begin, int tile_end, dpair* win, double (&acc)[5]) {
     const int n = v.n_nodes;
     const int t = threadIdx.x;
     const dpair* __restrict__ r2 = reinterpret_cast<const dpair*>(r_in);
@@ -390,15 +395,22 @@ __device__ __forceinline__ void fused_tiles(const PcgBuffers& b, const StencilVi
         dpair pn = {0.0, 0.0}, zn = {0.0, 0.0};
         if (own) {
             const dpair ro = r2[i], qo = q2[i], po = p2[i];
-            dpair x = x2[i];
-            x.x += alpha * po.x;  // solver.cpp:599-601
-            x.y += alpha * po.y;
+            if (x_mode != 0) {  // solver.cpp:599-601, two iterations' terms in their order
+                dpair x = x2[i];
+                if (x_mode == 2) {
+                    const dpair pp = reinterpret_cast<const dpair*>(p_out)[i];
+                    x.x += alpha_prev * pp.x;
+                    x.y += alpha_prev * pp.y;
+                }
+                x.x += alpha * po.x;
+                x.y += alpha * po.y;
+                x2[i] = x;
+            }
             const dpair rn = {ro.x - alpha * qo.x, ro.y - alpha * qo.y};
             double z0, z1;
             precond_node<Prec>(b.m, i, rn.x, rn.y, z0, z1);
             zn = dpair{z0, z1};
             pn = dpair{z0 + beta * po.x, z1 + beta * po.y};
-            x2[i] = x;
             reinterpret_cast<dpair*>(r_out)[i] = rn;
             reinterpret_cast<dpair*>(p_out)[i] = pn;
             acc[0] += rn.x * rn.x + rn.y * rn.y;
@@ -478,7 +490,13 @@ __global__ __launch_bounds__(kFusedTile) void pcg_fused_stencil_kernel(
     extern __shared__ double win_lds[];
     dpair* const win = reinterpret_cast<dpair*>(win_lds);
     __shared__ double sh[5 * (kFusedTile / kWave)];
-    const double alpha = b.st->alpha, beta = b.st->beta;
+    const double alpha = b.st->alpha, beta = b.st->beta, alpha_prev = b.st->alpha_prev;
+    // x += alpha p is deferred every second launch and done for two iterations in the
+    // next (launch L reads p_L-1 from p_in and still finds p_L-2 in p_out), which saves
+    // one read and one write of x per two iterations; launch_pcg_fused_flush adds a term
+    // left over at the end. The launch index counts from 0 after pcg_init (iter = -1).
+    const long long launch = b.st->iter + 1;
+    const int x_mode = (launch & 1) ? 0 : (launch >= 2 ? 2 : 1);
     const bool use32 = planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
     const int n_tiles = (v.n_nodes + kFusedTile - 1) / kFusedTile;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -488,11 +506,12 @@ __global__ __launch_bounds__(kFusedTile) void pcg_fused_stencil_kernel(
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // r'.r', r'.z, p'.q', q'.z, q'.M^-1 q'
     if (use32)
         fused_tiles<float, kM, Prec>(b, v, reinterpret_cast<const dpair*>(diag_c), 1, planes32, v.n_off - 1, 0, r_in,
-                                     r_out, p_in, p_out, q_in, q_out, alpha, beta, tile_begin, tile_end, win, acc);
+                                     r_out, p_in, p_out, q_in, q_out, alpha, beta, alpha_prev, x_mode, tile_begin,
+                                     tile_end, win, acc);
     else
         fused_tiles<double, kM, Prec>(b, v, reinterpret_cast<const dpair*>(planes64), v.n_off, planes64, v.n_off, 1,
-                                      r_in, r_out, p_in, p_out, q_in, q_out, alpha, beta, tile_begin, tile_end, win,
-                                      acc);
+                                      r_in, r_out, p_in, p_out, q_in, q_out, alpha, beta, alpha_prev, x_mode,
+                                      tile_begin, tile_end, win, acc);
     block_sums<kFusedTile, 5>(acc, sh);
     double tot[5];
     if (!finish_reduction<kFusedTile, 5>(b.tickets + 1, b.partials, acc, tot, sh)) return;
@@ -526,13 +545,38 @@ __global__ __launch_bounds__(kFusedTile) void pcg_fused_stencil_kernel(
         const double rho_next = rho - 2.0 * a * qz + a * a * qmq;
         st->pq = pq;
         st->rho = rho;
+        st->alpha_prev = alpha;
         st->alpha = a;
         st->beta = max0(rho_next) / rho;
         if (!st->phase32 && rr <= st->switch_rr) st->phase32 = 1;  // mixed: FP32 values from here on
     }
 }
 
+// The x term an odd final launch deferred: x += alpha p_in of that launch. A solve
+// that stopped there (done) left alpha unchanged; otherwise the launch moved it to
+// alpha_prev.
+__global__ __launch_bounds__(kThreads) void fused_flush_kernel(PcgBuffers b, const double* __restrict__ p_odd) {
+    const PcgState* st = b.st;
+    if (st->iter < 1 || (st->iter & 1) == 0) return;
+    const double a = st->done ? st->alpha : st->alpha_prev;
+    const int n_nodes = b.n >> 1;
+    dpair* __restrict__ x2 = reinterpret_cast<dpair*>(b.x);
+    const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(p_odd);
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n_nodes; i += gridDim.x * kThreads) {
+        dpair x = x2[i];
+        const dpair p = p2[i];
+        x.x += a * p.x;
+        x.y += a * p.y;
+        x2[i] = x;
+    }
+}
+
 }  // namespace
+
+void launch_pcg_fused_flush(hipStream_t s, const PcgBuffers& b, const double* p_odd) {
+    const unsigned int need = blocks_for(b.n >> 1, kThreads);
+    fused_flush_kernel<<<need < 4096u ? (need > 0 ? need : 1u) : 4096u, kThreads, 0, s>>>(b, p_odd);
+}
 
 void launch_pack_stencil(hipStream_t s, const CsrView& a, const StencilView& v, const double* values, double* out64,
                          float* out32, double* diag_c) {

@@ -339,6 +339,25 @@ def test_stencil_layouts(dev):
     assert err.value.code == native.ERR_UNSUPPORTED
 
 
+def test_fused_iteration_iterate_at_every_cap(dev):
+    """The fused stencil iteration adds x += alpha p for two iterations on every second
+    launch and completes an odd last launch after the solve: the iterate returned after
+    an odd or even iteration cap, or at convergence, is the reference PCG iterate after
+    as many iterations (a missing or doubled alpha p term is off by orders of magnitude
+    more than the rounding of the symmetric stencil sum order)."""
+    print("case fused iteration caps 64x64 log-E", flush=True)
+    sys_ = pipeline.setup(meshgen.c3_model(64), device=dev, assembly="coloring")
+    kc, fc = dev.system()
+    for cap in (1, 2, 3, 6, 7, 50, 51, 100000):
+        u, rep = dev.pcg("jacobi", "fp64", "relative", 1e-10, max_iterations=cap, spmv_variant=10,
+                         raise_on_error=False)
+        ref, rref = lfo.pcg(sys_.row_ptr, sys_.col, kc, fc, lfo.JACOBI, lfo.STOP_RELATIVE, 1e-10, cap)
+        assert rep["converged"] == rref["converged"], cap
+        if cap < 100000:
+            assert rep["iterations"] == rref["iterations"] == cap
+        assert np.linalg.norm(u - ref) <= 1e-9 * np.linalg.norm(ref), cap
+
+
 def test_fused_direction_is_bit_identical(dev):
     """Variant 7 forms p = M^-1 r + beta p inside the node-slice multiply (two launches
     per iteration); variant 6 runs the same multiply and a separate direction kernel.
