@@ -74,7 +74,10 @@ __global__ __launch_bounds__(256) void pin_pivots_kernel(int n, const int* __res
 
 // solver.cpp:355-413. One node group of 16 lanes per row pair would scatter the
 // RHS sum; the sum must run in column order to stay bit-exact, so lane 0 of each
-// 8-lane row group does it while all 8 lanes copy/zero the row coalesced.
+// 8-lane row group does it while all 8 lanes copy/zero the row coalesced. The 8
+// lanes flag the row's constrained nonzero entries as they copy it; a row with none
+// (all but the rows next to the supports) keeps f unchanged, so lane 0 runs the
+// serial sum only where it subtracts something.
 __global__ __launch_bounds__(256) void dirichlet_kernel(int n, const int* __restrict__ rp, const int* __restrict__ col,
                                                         const double* __restrict__ values, const double* __restrict__ f,
                                                         const int* __restrict__ constrained,
@@ -89,11 +92,17 @@ __global__ __launch_bounds__(256) void dirichlet_kernel(int n, const int* __rest
         if (lane == 0) fc[row] = prescribed[row];
         return;
     }
+    bool hit = false;
     for (int k = begin + lane; k < end; k += 8) {
         const double v = values[k];
-        kc[k] = (constrained[col[k]] != 0 && v != 0.0) ? 0.0 : v;
+        const bool eliminated = constrained[col[k]] != 0 && v != 0.0;
+        kc[k] = eliminated ? 0.0 : v;
+        hit = hit || eliminated;
     }
-    if (lane == 0) {
+    // the 8-lane group of this row within the wavefront's ballot
+    const unsigned long long group = (__ballot(hit) >> (threadIdx.x & (kWave - 8))) & 0xffull;
+    if (lane == 0 && group == 0) fc[row] = f[row];
+    if (lane == 0 && group != 0) {
         double rhs = f[row];
         for (int k = begin; k < end; ++k) {
             const int c = col[k];

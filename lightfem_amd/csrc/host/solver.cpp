@@ -12,6 +12,8 @@
 #include <stdexcept>
 #include <string>
 
+#include <sys/mman.h>
+
 #include "lightfem/lightfem.hpp"
 #include "lightfem_gpu.h"
 #include "lightfem_host.h"
@@ -99,6 +101,19 @@ HostArrays host_arrays(const Model& model) {
     return h;
 }
 
+// Sizes a fresh vector of n value-initialised elements, advising 2 MB pages for its
+// storage first: the first touch of hundreds of MB in 4 KB pages (the pattern and map,
+// K values, element results at 1000x1000) costs several times the work that fills them.
+template <typename T>
+void size_fresh(std::vector<T>& v, std::size_t n) {
+    constexpr uintptr_t kHuge = uintptr_t{2} << 20;
+    v.reserve(n);
+    const uintptr_t lo = (reinterpret_cast<uintptr_t>(v.data()) + kHuge - 1) & ~(kHuge - 1);
+    const uintptr_t hi = reinterpret_cast<uintptr_t>(v.data() + n) & ~(kHuge - 1);
+    if (hi > lo) (void)madvise(reinterpret_cast<void*>(lo), hi - lo, MADV_HUGEPAGE);  // advice only
+    v.resize(n);
+}
+
 struct Pattern {
     std::vector<int32_t> row_ptr, col, emap, bmap;
 };
@@ -111,8 +126,8 @@ Pattern build_pattern(const Model& model, const HostArrays& h) {
     const int64_t nnz = lfh_csr_pattern_nnz_bars(n_nodes, n_elems, h.enodes.data(), n_bars, h.bar_nodes.data());
     if (nnz < 0) throw std::out_of_range("element DOF is outside CSR matrix dimension");
     p.row_ptr.resize(static_cast<std::size_t>(2 * n_nodes + 1));
-    p.col.resize(static_cast<std::size_t>(nnz));
-    p.emap.resize(static_cast<std::size_t>(64 * n_elems));
+    size_fresh(p.col, static_cast<std::size_t>(nnz));
+    size_fresh(p.emap, static_cast<std::size_t>(64 * n_elems));
     p.bmap.resize(static_cast<std::size_t>(16 * n_bars));
     if (lfh_csr_pattern_bars(n_nodes, n_elems, h.enodes.data(), n_bars, h.bar_nodes.data(), p.row_ptr.data(),
                              p.col.data(), p.emap.data(), p.bmap.data()) != 0)
@@ -171,6 +186,17 @@ std::vector<double> load_vector(const Model& model) {
                                                     : q4_edge_pressure(c, *m, load.edge, load.first));
     }
     return f;
+}
+
+// The int32 row offsets / columns widened to the CsrMatrix index type (parallel: 36 M
+// columns at 1000x1000).
+std::vector<std::size_t> widened(const std::vector<int32_t>& v) {
+    std::vector<std::size_t> out;
+    size_fresh(out, v.size());
+    const int64_t count = static_cast<int64_t>(v.size());
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < count; ++i) out[static_cast<std::size_t>(i)] = static_cast<std::size_t>(v[i]);
+    return out;
 }
 
 int assembly_code(const std::string& name) {
@@ -246,11 +272,10 @@ AnalysisResult LinearElasticSolver::solve(Model& model) const {
     dev.check(lfg_set_load(dev.ctx, load.data()));
 
     const auto retain_start = std::chrono::steady_clock::now();
-    std::vector<double> values(static_cast<std::size_t>(nnz));
+    std::vector<double> values;
+    size_fresh(values, static_cast<std::size_t>(nnz));
     dev.check(lfg_get_stiffness(dev.ctx, values.data()));
-    result.stiffness = CsrMatrix(static_cast<std::size_t>(n),
-                                 std::vector<std::size_t>(pat.row_ptr.begin(), pat.row_ptr.end()),
-                                 std::vector<std::size_t>(pat.col.begin(), pat.col.end()), std::move(values));
+    result.stiffness = CsrMatrix(static_cast<std::size_t>(n), widened(pat.row_ptr), widened(pat.col), std::move(values));
     result.load = load;
     result.performance.retain_matrix_ms = ms_since(retain_start);
     const auto after_start = std::chrono::steady_clock::now();
@@ -384,13 +409,18 @@ AnalysisResult LinearElasticSolver::solve(Model& model) const {
                      : "converged";
 
     const auto post_start = std::chrono::steady_clock::now();
-    result.reaction.assign(static_cast<std::size_t>(n), 0.0);
+    result.reaction.clear();
+    size_fresh(result.reaction, static_cast<std::size_t>(n));
     dev.check(lfg_reactions(dev.ctx, result.reaction.data()));
     const int64_t n_results = n_elems + n_bars;  // Q4 elements, then bars (solver.cpp:753-805)
-    std::vector<double> centre(7 * static_cast<std::size_t>(n_results)), gp(36 * static_cast<std::size_t>(n_results)),
-        nodal(4 * static_cast<std::size_t>(n_nodes));
+    std::vector<double> centre, gp, nodal;
+    size_fresh(centre, 7 * static_cast<std::size_t>(n_results));
+    size_fresh(gp, 36 * static_cast<std::size_t>(n_results));
+    size_fresh(nodal, 4 * static_cast<std::size_t>(n_nodes));
     dev.check(lfg_postprocess(dev.ctx, centre.data(), gp.data(), nodal.data()));
-    result.element_results.resize(static_cast<std::size_t>(n_results));
+    result.element_results.clear();
+    size_fresh(result.element_results, static_cast<std::size_t>(n_results));
+#pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < n_results; ++e) {
         ElementResult& er = result.element_results[static_cast<std::size_t>(e)];
         er.element_id = e < n_elems ? model.elements[static_cast<std::size_t>(e)].id
@@ -408,7 +438,9 @@ AnalysisResult LinearElasticSolver::solve(Model& model) const {
             er.gauss_points[q].field.von_mises = g[8];
         }
     }
-    result.node_results.resize(static_cast<std::size_t>(n_nodes));
+    result.node_results.clear();
+    size_fresh(result.node_results, static_cast<std::size_t>(n_nodes));
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n_nodes; ++i) {
         NodeResult& nr = result.node_results[static_cast<std::size_t>(i)];
         nr.node_id = model.nodes[static_cast<std::size_t>(i)].id;
