@@ -112,8 +112,9 @@ typedef struct {
 } lfg_pcg_report;
 
 typedef struct {
-    double element_stiffness_ms;  /* Ke kernel (PerformanceReport::element_stiffness_ms) */
-    double assembly_ms;           /* scatter / gather (PerformanceReport::assembly_ms) */
+    double element_stiffness_ms;  /* separate Ke kernel (PerformanceReport::element_stiffness_ms); 0 where
+                                     the assembly kernel forms the element matrices itself */
+    double assembly_ms;           /* assembly (PerformanceReport::assembly_ms) */
     int32_t coloring_count;
     int32_t jacobian_failures;
 } lfg_assembly_report;
@@ -163,10 +164,13 @@ int lfg_set_colors(lfg_ctx* ctx, int32_t n_colors, const int32_t* color_ptr, con
 /* Element stiffness alone (q4_stiffness, element.cpp:130-161) into ke_out (64*n_elems), for checks. */
 int lfg_element_stiffness(lfg_ctx* ctx, double* ke_out);
 
-/* Numeric assembly (solver.cpp:243-301): Ke kernel + strategy. serial = deterministic
- * gather in element order (bit-identical to serial_assembly), coloring = conflict-free
- * scatter per colour (bit-identical to coloring_assembly), atomic = FP64 hardware
- * atomics, thread_local = the deterministic gather. */
+/* Numeric assembly (solver.cpp:243-301). serial and thread_local sum every value's
+ * contributions in element order (bit-identical to serial_assembly), coloring in colour
+ * order (bit-identical to coloring_assembly), atomic adds with FP64 hardware atomics.
+ * Q4 meshes with rows of at most 32 values run serial, thread_local and coloring as one
+ * owner-computes pass (each node's rows formed and written by one lane group); meshes
+ * with bars or longer rows store the element matrices and gather them (serial,
+ * thread_local) or scatter them conflict-free colour by colour (coloring). */
 int lfg_assemble(lfg_ctx* ctx, int32_t strategy, lfg_assembly_report* report);
 
 /* Unconstrained stiffness values (AnalysisResult::stiffness, solver.cpp:906-909). */

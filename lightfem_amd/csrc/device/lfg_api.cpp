@@ -119,6 +119,7 @@ struct lfg_ctx {
     long long nnz = 0;
     std::vector<int> h_row_ptr, h_col;
     DevArray<int> row_ptr, col, emap, cptr, contrib;
+    bool gather_valid = false;  // cptr / contrib match the current pattern and bars
     bool have_pattern = false;
     // node-slice copy of the constrained values (SpMV variant 6), when the pattern is node-blocked
     bool have_slices = false, slices64_valid = false, slices32_valid = false;
@@ -137,6 +138,7 @@ struct lfg_ctx {
     DevArray<float> stencil32;
     std::vector<int> h_color_ptr;      // Q4 elements of colour c: color_elems[h_color_ptr[c] ..)
     DevArray<int> color_elems;
+    DevArray<int> color_node_ptr, color_node;  // Q4-only meshes: node -> (element, corner) in colour order
     std::vector<int> h_color_bar_ptr;  // bars of colour c: color_bars[h_color_bar_ptr[c] ..)
     DevArray<int> color_bars;
 
@@ -509,7 +511,7 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 // the (element, local entry) pairs that add into it, in ascending element order.
 // Contribution k < 64 n_elems is Q4 entry k; 64 n_elems + 16 t + j is entry j of bar t
 // (the bar matrices follow the Q4 ones in the element-matrix buffer).
-void build_gather_map(lfg_ctx* c, const int32_t* emap, const int32_t* bmap = nullptr, int n_bars = 0) {
+void build_gather_map(lfg_ctx* c, const int32_t* emap, const int32_t* bmap, int n_bars) {
     const long long q4 = 64LL * c->n_elems;
     const long long total = q4 + 16LL * n_bars;
     auto target = [&](long long k) { return static_cast<size_t>(k < q4 ? emap[k] : bmap[k - q4]); };
@@ -522,6 +524,18 @@ void build_gather_map(lfg_ctx* c, const int32_t* emap, const int32_t* bmap = nul
     c->cptr.upload(counts.data(), counts.size(), c->stream);
     c->contrib.upload(contrib.data(), contrib.size(), c->stream);
     sync(c);
+    c->gather_valid = true;
+}
+
+// The gather map of the serial assembly with bars, built on first use from the maps on
+// the device (meshes of Q4 elements only assemble row by row and never need it).
+void ensure_gather_map(lfg_ctx* c) {
+    if (c->gather_valid) return;
+    std::vector<int32_t> emap(64 * static_cast<size_t>(c->n_elems)), bmap(16 * static_cast<size_t>(c->n_bars));
+    c->emap.download(emap.data(), emap.size(), c->stream);
+    c->bar_map.download(bmap.data(), bmap.size(), c->stream);
+    sync(c);
+    build_gather_map(c, emap.data(), bmap.data(), c->n_bars);
 }
 
 // IC(0) of the lower triangle (precond.cpp:298-384), level sets for the device
@@ -1183,7 +1197,7 @@ int lfg_set_pattern(lfg_ctx* ctx, int64_t n_rows, int64_t nnz, const int32_t* ro
         ctx->row_ptr.upload(row_ptr, static_cast<size_t>(n_rows) + 1, ctx->stream);
         ctx->col.upload(col, static_cast<size_t>(nnz), ctx->stream);
         ctx->emap.upload(emap, 64 * static_cast<size_t>(ctx->n_elems), ctx->stream);
-        build_gather_map(ctx, emap);
+        ctx->gather_valid = false;
         build_slices(ctx);
         build_stencil(ctx);
         ctx->K.alloc(static_cast<size_t>(nnz));
@@ -1226,10 +1240,7 @@ int lfg_set_bars(lfg_ctx* ctx, int64_t n_bars, const int32_t* bar_nodes, const i
         ctx->bar_area.upload(area, static_cast<size_t>(n_bars), ctx->stream);
         ctx->bar_map.upload(bar_map, 16 * static_cast<size_t>(n_bars), ctx->stream);
         // the serial gather and the smoothing lists take the bars after the Q4 elements
-        std::vector<int32_t> emap(64 * static_cast<size_t>(ctx->n_elems));
-        ctx->emap.download(emap.data(), emap.size(), ctx->stream);
-        sync(ctx);
-        build_gather_map(ctx, emap.data(), bar_map, ctx->n_bars);
+        ctx->gather_valid = false;
         build_node_elements(ctx);
         ctx->h_color_ptr.clear();  // colour lists name the elements with the bars included
         ctx->h_color_bar_ptr.clear();
@@ -1300,6 +1311,20 @@ int lfg_set_colors(lfg_ctx* ctx, int32_t n_colors, const int32_t* color_ptr, con
         }
         ctx->color_elems.upload(q4.data(), q4.size(), ctx->stream);
         ctx->color_bars.upload(bars.data(), bars.size(), ctx->stream);
+        if (ctx->n_bars == 0) {
+            // node -> (element << 2 | corner), colour by colour: the order in which the
+            // coloured passes add into each value of the node's rows
+            const int n_nodes = ctx->n_nodes;
+            std::vector<int> ptr(static_cast<size_t>(n_nodes) + 1, 0);
+            for (int k = 0; k < 4 * ctx->n_elems; ++k) ptr[static_cast<size_t>(ctx->h_enodes[k]) + 1]++;
+            for (int i = 0; i < n_nodes; ++i) ptr[i + 1] += ptr[i];
+            std::vector<int> list(static_cast<size_t>(ptr[n_nodes]));
+            std::vector<int> cur(ptr.begin(), ptr.end() - 1);
+            for (const int e : q4)
+                for (int l = 0; l < 4; ++l) list[static_cast<size_t>(cur[ctx->h_enodes[4 * e + l]]++)] = (e << 2) | l;
+            ctx->color_node_ptr.upload(ptr.data(), ptr.size(), ctx->stream);
+            ctx->color_node.upload(list.data(), list.size(), ctx->stream);
+        }
         sync(ctx);
         return LFG_OK;
     });
@@ -1332,7 +1357,26 @@ int lfg_assemble(lfg_ctx* ctx, int32_t strategy, lfg_assembly_report* report) {
         const lfg::BarView bv = bar_view(ctx);
         LFG_HIP(hipMemsetAsync(ctx->flags.p, 0, 4 * sizeof(int), ctx->stream));
         lfg_assembly_report rep{};
-        if (strategy == LFG_ASSEMBLY_SERIAL || strategy == LFG_ASSEMBLY_THREAD_LOCAL) {
+        const bool rows = ctx->n_bars == 0 && strategy != LFG_ASSEMBLY_ATOMIC && ctx->max_row <= lfg::kAssembleRowSlots;
+        if (rows) {
+            // Q4 meshes, serial and coloured sums: one pass, each row owned by one thread
+            // (element order for serial / thread_local, colour order for coloring)
+            const bool colored = strategy == LFG_ASSEMBLY_COLORING;
+            require(strategy == LFG_ASSEMBLY_SERIAL || strategy == LFG_ASSEMBLY_THREAD_LOCAL || colored, LFG_ERR_ARG,
+                    "validated model has an unknown assembly strategy");
+            require(!colored || !ctx->h_color_ptr.empty(), LFG_ERR_STATE,
+                    "lfg_set_colors must precede coloring assembly");
+            LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+            lfg::launch_assemble_rows(ctx->stream, m, ctx->n_nodes, ctx->row_ptr.p,
+                                      colored ? ctx->color_node_ptr.p : ctx->node_elem_ptr.p,
+                                      colored ? ctx->color_node.p : ctx->node_elem.p, ctx->emap.p, ctx->K.p,
+                                      ctx->flags.p);
+            LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+            sync(ctx);
+            rep.assembly_ms = elapsed(ctx->ev_start, ctx->ev_stop);  // element matrices formed in the pass
+            if (colored) rep.coloring_count = static_cast<int>(ctx->h_color_ptr.size()) - 1;
+        } else if (strategy == LFG_ASSEMBLY_SERIAL || strategy == LFG_ASSEMBLY_THREAD_LOCAL) {
+            ensure_gather_map(ctx);
             const size_t q4 = 64 * static_cast<size_t>(ctx->n_elems);
             ctx->ke_buf.alloc(q4 + 16 * static_cast<size_t>(ctx->n_bars));
             LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));

@@ -172,6 +172,92 @@ __global__ __launch_bounds__(256) void ke_kernel(MeshView m, const int* __restri
     }
 }
 
+// Owner-computes assembly of a Q4 mesh. An 8-lane group owns node i's rows 2i, 2i+1;
+// its node list holds (element << 2 | corner a) in the order the values receive their
+// contributions: element order is serial_assembly (solver.cpp:135-142), colour order is
+// coloring_assembly (:163-179; at most one contribution per value and colour). Lane
+// 2 * slot + h forms row 2a + h of the element in list position 4j + slot (ke_row: the
+// bits of ke_kernel). Lane h of the group (slot 0) owns row 2i + h: it takes the four
+// slots' rows with __shfl and adds them, in list order, into its own LDS copy of the
+// row (touched by no other lane until the block barrier), onto 0.0. After the barrier
+// the group writes both rows once, coalesced: no element matrix is stored, K needs no
+// zeroing and no value is read back. Rows of up to kRowSlots values (positions are
+// clamped into the row copy, so a map entry outside its row cannot write elsewhere).
+constexpr int kRowSlots = kAssembleRowSlots;
+constexpr int kAsmNodes = 256 / 8;
+
+__global__ __launch_bounds__(256) void assemble_rows_kernel(MeshView m, int n_nodes, const int* __restrict__ rp,
+                                                            const int* __restrict__ node_ptr,
+                                                            const int* __restrict__ node_list,
+                                                            const int* __restrict__ emap, double* __restrict__ values,
+                                                            int* __restrict__ failures) {
+    __shared__ double acc[kAsmNodes * 2 * kRowSlots];
+    const int g = static_cast<int>(threadIdx.x) >> 3;
+    const int lane = static_cast<int>(threadIdx.x) & 7;
+    const int slot = lane >> 1, h = lane & 1;
+    const int group_lane0 = static_cast<int>(threadIdx.x) & (kWave - 8);  // within the wavefront
+    const int node = static_cast<int>(blockIdx.x) * kAsmNodes + g;
+    const bool active = node < n_nodes;  // no early exit: every thread reaches the barrier
+    double* const mine = acc + (g * 2 + h) * kRowSlots;  // row 2i + h, owned by lane h
+    if (slot == 0) {
+        for (int k = 0; k < kRowSlots; ++k) mine[k] = 0.0;
+    }
+    int begin = 0, end = 0, row_begin = 0;
+    if (active) {
+        begin = node_ptr[node];
+        end = node_ptr[node + 1];
+        row_begin = rp[2 * node + h];
+    }
+    for (int s0 = begin; s0 < end; s0 += 4) {  // the same trip count for the 8 lanes of a group
+        const int s = s0 + slot;
+        double ke[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int pos[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (s < end) {
+            const int code = node_list[s];
+            const int e = code >> 2;
+            const int row = 2 * (code & 3) + h;
+            Coords c;
+            double young, nu, thick;
+            int mode;
+            load_element(m, e, c, young, nu, thick, mode);
+            if (!ke_row(c, young, nu, thick, mode, row, ke)) {
+                if (row == 0) atomicAdd(failures, 1);  // once per element
+#pragma unroll
+                for (int k = 0; k < 8; ++k) ke[k] = 0.0;
+            }
+            const int4* map4 = reinterpret_cast<const int4*>(emap + 64LL * e + 8 * row);
+            const int4 m0 = map4[0], m1 = map4[1];
+            const int idx[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) pos[k] = min(max(idx[k] - row_begin, 0), kRowSlots - 1);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {  // list order
+            const int src = group_lane0 + 2 * q + h;
+            double v[8];
+            int p[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                v[k] = __shfl(ke[k], src);
+                p[k] = __shfl(pos[k], src);
+            }
+            if (slot == 0 && s0 + q < end) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) mine[p[k]] += v[k];
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            const int b = rp[2 * node + hh], len = rp[2 * node + hh + 1] - b;
+            const double* row = acc + (g * 2 + hh) * kRowSlots;
+            for (int k = lane; k < len; k += 8) values[b + k] = row[k];
+        }
+    }
+}
+
 // ---- bars (element.cpp:266-306)
 
 // Bar t: end coordinates, E of its material and its area; the axial direction and
@@ -440,6 +526,13 @@ void launch_ke(hipStream_t s, const MeshView& m, KeMode mode, const int* elems, 
             ke_kernel<KE_COLORED><<<grid, 256, 0, s>>>(m, elems, count, emap, values, ke_out, failures);
             break;
     }
+}
+
+void launch_assemble_rows(hipStream_t s, const MeshView& m, int n_nodes, const int* row_ptr, const int* node_ptr,
+                          const int* node_list, const int* emap, double* values, int* jacobian_failures) {
+    if (n_nodes <= 0) return;
+    assemble_rows_kernel<<<blocks_for(n_nodes, kAsmNodes), 256, 0, s>>>(m, n_nodes, row_ptr, node_ptr, node_list,
+                                                                        emap, values, jacobian_failures);
 }
 
 void launch_gather(hipStream_t s, long long nnz, const int* cptr, const int* contrib, const double* ke,

@@ -105,6 +105,12 @@ enum KeMode { KE_STORE = 0, KE_ATOMIC = 1, KE_COLORED = 2 };
 // ke_out[64e + 8r + c]; ATOMIC/COLORED scatter into values through emap.
 void launch_ke(hipStream_t s, const MeshView& m, KeMode mode, const int* elems, int count, const int* emap,
                double* values, double* ke_out, int* jacobian_failures);
+// Q4 meshes whose rows hold at most kAssembleRowSlots values: writes every value of
+// rows 2i, 2i+1 once, summing its contributions from 0.0 in the order of node i's
+// (element << 2 | corner) list (element order: serial; colour order: coloring).
+constexpr int kAssembleRowSlots = 32;
+void launch_assemble_rows(hipStream_t s, const MeshView& m, int n_nodes, const int* row_ptr, const int* node_ptr,
+                          const int* node_list, const int* emap, double* values, int* jacobian_failures);
 // values[k] = sum of ke[contrib[j]] for j in [cptr[k], cptr[k+1]) in element order.
 void launch_gather(hipStream_t s, long long nnz, const int* cptr, const int* contrib, const double* ke,
                    double* values);

@@ -57,6 +57,44 @@ def colored_sum(ke, emap, ptr, elems, nnz):
     return values
 
 
+def fan_mesh(sectors: int = 8):
+    """`sectors` quads around node 0 (node 0, ring 2i, ring 2i+1, ring 2i+2 of a 2*sectors
+    node ring), plus a quad outside the ring: node 0's rows hold 2 * (2 * sectors + 1)
+    values."""
+    ring = 2 * sectors
+    ang = 2.0 * np.pi * np.arange(ring) / ring
+    xy = np.vstack([[0.0, 0.0], np.column_stack([np.cos(ang), np.sin(ang)]),
+                    [[1.6, -0.2], [1.6, 0.6]]])
+    quads = [[0, 1 + 2 * i, 1 + (2 * i + 1) % ring, 1 + (2 * i + 2) % ring] for i in range(sectors)]
+    quads.append([1, ring + 1, ring + 2, 2])
+    return xy, np.array(quads, dtype=np.int32)
+
+
+def test_assembly_orders_on_long_rows(dev):
+    """Serial and coloured assembly keep the reference's sums where a row holds more values
+    than the row kernel's LDS copy (34 at the fan's centre: the store + gather and coloured
+    scatter kernels), and where it does not (the same fan with 4 sectors: the row kernel)."""
+    for sectors in (8, 4):
+        print(f"case fan mesh, {sectors} sectors", flush=True)
+        xy, enode = fan_mesh(sectors)
+        n_nodes = xy.shape[0]
+        young = 1.0 + 0.1 * np.arange(enode.shape[0])
+        materials = np.column_stack([young, np.full_like(young, 0.3), np.ones_like(young)])
+        dev.set_mesh(xy, enode, np.arange(enode.shape[0]), materials, native.PLANE_STRESS)
+        row_ptr, col, emap = native.csr_pattern(n_nodes, enode)
+        colors, ptr, elems = native.element_colors(n_nodes, enode)
+        dev.set_pattern(row_ptr, col, emap)
+        dev.set_colors(ptr, elems)
+        assert np.max(np.diff(row_ptr)) == 2 * (2 * sectors + 1)
+        ke = dev.element_stiffness()
+        nnz = col.shape[0]
+        emap2 = np.asarray(emap).reshape(-1, 64)
+        dev.assemble("serial")
+        assert np.array_equal(bits(dev.stiffness()), bits(lfo.assemble_serial(ke, emap2, nnz))), sectors
+        dev.assemble("coloring")
+        assert np.array_equal(bits(dev.stiffness()), bits(colored_sum(ke, emap2, ptr, elems, nnz))), sectors
+
+
 @pytest.mark.parametrize("name", FULL_CASES)
 def test_stages_match_reference(dev, golden_index, name):
     g = load_golden(name)
