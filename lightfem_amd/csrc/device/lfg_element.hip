@@ -84,13 +84,18 @@ __device__ __forceinline__ void load_element(const MeshView& m, int e, Coords& c
     mode = m.mat_mode ? m.mat_mode[mat] : m.plane_mode;
 }
 
-// Row `row` of Ke (element.cpp:130-161): ke[c] = sum_gp (sum_s b[s][row] db[s][c]) * detJ * t.
-__device__ __forceinline__ bool ke_row(const Coords& c, double young, double nu, double thick, int plane_mode,
-                                       int row, double ke[8]) {
+// Rows row0 .. row0 + NR - 1 of Ke (element.cpp:130-161):
+// ke[j][c] = sum_gp (sum_s b[s][row0 + j] db[s][c]) * detJ * t; every row with the same
+// operations whatever NR (rows of one Gauss point share its B and DB).
+template <int NR>
+__device__ __forceinline__ bool ke_rows(const Coords& c, double young, double nu, double thick, int plane_mode,
+                                        int row0, double (&ke)[NR][8]) {
     double d[9];
     constitutive(young, nu, plane_mode, d);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) ke[k] = 0.0;
+    for (int j = 0; j < NR; ++j)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ke[j][k] = 0.0;
     bool ok = true;
     // Gauss points (-g,-g), (g,-g), (g,g), (-g,g) one at a time: unrolled, the four points'
     // B and DB do not fit the register file and spill
@@ -111,24 +116,38 @@ __device__ __forceinline__ bool ke_row(const Coords& c, double young, double nu,
                 db[8 * r + col] = acc;
             }
         const double weight = det * thick;
-        // column `row` of B: selects over constant indices (an index by the lane's row
-        // would move b to scratch memory)
-        double b0 = b[0], b1 = b[8], b2 = b[16];
 #pragma unroll
-        for (int k = 1; k < 8; ++k) {
-            b0 = row == k ? b[k] : b0;
-            b1 = row == k ? b[8 + k] : b1;
-            b2 = row == k ? b[16 + k] : b2;
-        }
+        for (int j = 0; j < NR; ++j) {
+            // column `row` of B: selects over constant indices (an index by the lane's row
+            // would move b to scratch memory)
+            const int row = row0 + j;
+            double b0 = b[0], b1 = b[8], b2 = b[16];
 #pragma unroll
-        for (int col = 0; col < 8; ++col) {
-            double contribution = 0.0;
-            contribution += b0 * db[col];
-            contribution += b1 * db[8 + col];
-            contribution += b2 * db[16 + col];
-            ke[col] += contribution * weight;
+            for (int k = 1; k < 8; ++k) {
+                b0 = row == k ? b[k] : b0;
+                b1 = row == k ? b[8 + k] : b1;
+                b2 = row == k ? b[16 + k] : b2;
+            }
+#pragma unroll
+            for (int col = 0; col < 8; ++col) {
+                double contribution = 0.0;
+                contribution += b0 * db[col];
+                contribution += b1 * db[8 + col];
+                contribution += b2 * db[16 + col];
+                ke[j][col] += contribution * weight;
+            }
         }
     }
+    return ok;
+}
+
+// Row `row` of Ke.
+__device__ __forceinline__ bool ke_row(const Coords& c, double young, double nu, double thick, int plane_mode,
+                                       int row, double ke[8]) {
+    double one[1][8];
+    const bool ok = ke_rows<1>(c, young, nu, thick, plane_mode, row, one);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ke[k] = one[0][k];
     return ok;
 }
 
@@ -172,19 +191,20 @@ __global__ __launch_bounds__(256) void ke_kernel(MeshView m, const int* __restri
     }
 }
 
-// Owner-computes assembly of a Q4 mesh. An 8-lane group owns node i's rows 2i, 2i+1;
+// Owner-computes assembly of a Q4 mesh. A 4-lane group owns node i's rows 2i, 2i+1;
 // its node list holds (element << 2 | corner a) in the order the values receive their
 // contributions: element order is serial_assembly (solver.cpp:135-142), colour order is
 // coloring_assembly (:163-179; at most one contribution per value and colour). Lane
-// 2 * slot + h forms row 2a + h of the element in list position 4j + slot (ke_row: the
-// bits of ke_kernel). Lane h of the group (slot 0) owns row 2i + h: it takes the four
-// slots' rows with __shfl and adds them, in list order, into its own LDS copy of the
-// row (touched by no other lane until the block barrier), onto 0.0. After the barrier
-// the group writes both rows once, coalesced: no element matrix is stored, K needs no
-// zeroing and no value is read back. Rows of up to kRowSlots values (positions are
-// clamped into the row copy, so a map entry outside its row cannot write elsewhere).
+// `slot` forms rows 2a, 2a + 1 of the element in list position 4j + slot (ke_rows: the
+// bits of ke_kernel, one B and DB per Gauss point for both rows). Lane h < 2 of the group
+// owns row 2i + h: it takes the four slots' rows with __shfl and adds them, in list
+// order, into its own LDS copy of the row (touched by no other lane until the block
+// barrier), onto 0.0. After the barrier the group writes both rows once, coalesced: no
+// element matrix is stored, K needs no zeroing and no value is read back. Rows of up to
+// kRowSlots values (positions are clamped into the row copy, so a map entry outside its
+// row cannot write elsewhere).
 constexpr int kRowSlots = kAssembleRowSlots;
-constexpr int kAsmNodes = 256 / 8;
+constexpr int kAsmNodes = 256 / 4;
 
 __global__ __launch_bounds__(256) void assemble_rows_kernel(MeshView m, int n_nodes, const int* __restrict__ rp,
                                                             const int* __restrict__ node_ptr,
@@ -192,58 +212,65 @@ __global__ __launch_bounds__(256) void assemble_rows_kernel(MeshView m, int n_no
                                                             const int* __restrict__ emap, double* __restrict__ values,
                                                             int* __restrict__ failures) {
     __shared__ double acc[kAsmNodes * 2 * kRowSlots];
-    const int g = static_cast<int>(threadIdx.x) >> 3;
-    const int lane = static_cast<int>(threadIdx.x) & 7;
-    const int slot = lane >> 1, h = lane & 1;
-    const int group_lane0 = static_cast<int>(threadIdx.x) & (kWave - 8);  // within the wavefront
+    const int g = static_cast<int>(threadIdx.x) >> 2;
+    const int slot = static_cast<int>(threadIdx.x) & 3;
+    const int group_lane0 = static_cast<int>(threadIdx.x) & (kWave - 4);  // within the wavefront
     const int node = static_cast<int>(blockIdx.x) * kAsmNodes + g;
     const bool active = node < n_nodes;  // no early exit: every thread reaches the barrier
-    double* const mine = acc + (g * 2 + h) * kRowSlots;  // row 2i + h, owned by lane h
-    if (slot == 0) {
+    const bool owner = slot < 2;         // lane h owns row 2i + h
+    double* const mine = acc + (g * 2 + (slot & 1)) * kRowSlots;
+    if (owner) {
         for (int k = 0; k < kRowSlots; ++k) mine[k] = 0.0;
     }
-    int begin = 0, end = 0, row_begin = 0;
+    int begin = 0, end = 0, row_begin0 = 0, row_begin1 = 0;
     if (active) {
         begin = node_ptr[node];
         end = node_ptr[node + 1];
-        row_begin = rp[2 * node + h];
+        row_begin0 = rp[2 * node];
+        row_begin1 = rp[2 * node + 1];
     }
-    for (int s0 = begin; s0 < end; s0 += 4) {  // the same trip count for the 8 lanes of a group
+    for (int s0 = begin; s0 < end; s0 += 4) {  // the same trip count for the 4 lanes of a group
         const int s = s0 + slot;
-        double ke[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        int pos[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        double ke[2][8] = {{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
+        int pos0[8] = {0, 0, 0, 0, 0, 0, 0, 0}, pos1[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (s < end) {
             const int code = node_list[s];
             const int e = code >> 2;
-            const int row = 2 * (code & 3) + h;
+            const int a = code & 3;
             Coords c;
             double young, nu, thick;
             int mode;
             load_element(m, e, c, young, nu, thick, mode);
-            if (!ke_row(c, young, nu, thick, mode, row, ke)) {
-                if (row == 0) atomicAdd(failures, 1);  // once per element
+            if (!ke_rows<2>(c, young, nu, thick, mode, 2 * a, ke)) {
+                if (a == 0) atomicAdd(failures, 1);  // once per element
 #pragma unroll
-                for (int k = 0; k < 8; ++k) ke[k] = 0.0;
+                for (int k = 0; k < 8; ++k) ke[0][k] = ke[1][k] = 0.0;
             }
-            const int4* map4 = reinterpret_cast<const int4*>(emap + 64LL * e + 8 * row);
-            const int4 m0 = map4[0], m1 = map4[1];
-            const int idx[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+            const int4* map4 = reinterpret_cast<const int4*>(emap + 64LL * e + 16 * a);  // rows 2a, 2a + 1
+            const int4 m0 = map4[0], m1 = map4[1], m2 = map4[2], m3 = map4[3];
+            const int i0[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+            const int i1[8] = {m2.x, m2.y, m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
 #pragma unroll
-            for (int k = 0; k < 8; ++k) pos[k] = min(max(idx[k] - row_begin, 0), kRowSlots - 1);
+            for (int k = 0; k < 8; ++k) {
+                pos0[k] = min(max(i0[k] - row_begin0, 0), kRowSlots - 1);
+                pos1[k] = min(max(i1[k] - row_begin1, 0), kRowSlots - 1);
+            }
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {  // list order
-            const int src = group_lane0 + 2 * q + h;
-            double v[8];
-            int p[8];
+            const int src = group_lane0 + q;
+            double v0[8], v1[8];
+            int p0[8], p1[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                v[k] = __shfl(ke[k], src);
-                p[k] = __shfl(pos[k], src);
+                v0[k] = __shfl(ke[0][k], src);
+                v1[k] = __shfl(ke[1][k], src);
+                p0[k] = __shfl(pos0[k], src);
+                p1[k] = __shfl(pos1[k], src);
             }
-            if (slot == 0 && s0 + q < end) {
+            if (owner && s0 + q < end) {
 #pragma unroll
-                for (int k = 0; k < 8; ++k) mine[p[k]] += v[k];
+                for (int k = 0; k < 8; ++k) mine[slot ? p1[k] : p0[k]] += slot ? v1[k] : v0[k];
             }
         }
     }
@@ -253,7 +280,7 @@ __global__ __launch_bounds__(256) void assemble_rows_kernel(MeshView m, int n_no
         for (int hh = 0; hh < 2; ++hh) {
             const int b = rp[2 * node + hh], len = rp[2 * node + hh + 1] - b;
             const double* row = acc + (g * 2 + hh) * kRowSlots;
-            for (int k = lane; k < len; k += 8) values[b + k] = row[k];
+            for (int k = slot; k < len; k += 4) values[b + k] = row[k];
         }
     }
 }
