@@ -122,12 +122,12 @@ def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict
     values and mixed (each problem moves to compensated FP32 values at 1e-2 ||b||)."""
     from lightfem_amd import meshgen, pipeline
     t0 = time.perf_counter()
-    specs = meshgen.c5_problems(n_problems)
-    models = [meshgen.c5_model(sp) for sp in specs]
-    loads = [meshgen.c5_case_loads(m, sp) for m, sp in zip(models, specs)]
+    specs = meshgen.batch_problems(n_problems)
+    models = [meshgen.batch_problem_model(sp) for sp in specs]
+    loads = [meshgen.batch_case_loads(m, sp) for m, sp in zip(models, specs)]
     bs = pipeline.batch_setup(models, loads, device=dev)
     setup_s = time.perf_counter() - t0
-    out = {"workload": f"c5-{n_problems}-problems-x4-cases-seed{meshgen.SEED_BATCH}", "problems": n_problems,
+    out = {"workload": f"uneven-batch-{n_problems}-problems-x4-cases-seed{meshgen.SEED_BATCH}", "problems": n_problems,
            "systems": n_problems * 4, "dofs": int(sum(2 * ix.n_nodes for ix in bs.indexed)),
            "dofs_padded": int(dev.n), "nnz": int(dev.nnz), "setup_s": round(setup_s, 2),
            "stop_rule": f"recurrence ||r||/||b|| <= {tol:g} per system"}
@@ -154,13 +154,13 @@ def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict
     return out
 
 
-def run_c2(dev, tol: float) -> dict:
+def run_preconditioners_200(dev, tol: float) -> dict:
     """The 200x200 cantilever in RCM order with coloring assembly, every preconditioner and
     storage of the reference: Jacobi, 2x2 nodal block, the default block (rcb, 64 nodes per
     subdomain), IC(0), and Jacobi with block storage. Device solve time (hipEvents) to the
     reference stop rule ||r|| <= tol max(1, ||r0||)."""
     from lightfem_amd import meshgen, native, pipeline
-    sys_ = pipeline.setup(meshgen.c2_model(), device=dev, assembly="coloring", reordering="rcm")
+    sys_ = pipeline.setup(meshgen.cantilever_200_rcm(), device=dev, assembly="coloring", reordering="rcm")
     nodes, count = native.subdomain_split(sys_.indexed.xy, "rcb", 64)
     dev.set_subdomains(nodes, count)
     out = {"workload": "q4-cantilever-200x200-plane-stress-rcm-coloring", "dofs": int(dev.n), "nnz": int(dev.nnz),
@@ -175,7 +175,7 @@ def run_c2(dev, tol: float) -> dict:
                      "us_per_iteration": round(1e3 * rep["solve_ms"] / max(1, rep["iterations"]), 2)}
         if precond == "ic0":
             out[name]["sweep_levels"] = rep["levels"]  # forward + backward
-        print(f"bench: c2 {name} {json.dumps(out[name])}", file=sys.stderr, flush=True)
+        print(f"bench: preconditioners 200x200 {name} {json.dumps(out[name])}", file=sys.stderr, flush=True)
     return out
 
 
@@ -193,8 +193,8 @@ def main() -> int:
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--batch-only", action="store_true", help="only the batch workload (profiling)")
     ap.add_argument("--batch-forms", default="fp64,mixed", help="comma-separated batch precisions to run")
-    ap.add_argument("--skip-c2", action="store_true", help="skip the 200x200 preconditioner comparison")
-    ap.add_argument("--c2-only", action="store_true", help="only the 200x200 preconditioner comparison (profiling)")
+    ap.add_argument("--skip-precond", action="store_true", help="skip the 200x200 preconditioner comparison")
+    ap.add_argument("--precond-only", action="store_true", help="only the 200x200 preconditioner comparison (profiling)")
     ap.add_argument("--assembly-only", action="store_true", help="only the assembly strategies (profiling)")
     ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--profile", type=int, default=0,
@@ -210,7 +210,7 @@ def main() -> int:
     if args.assembly_only:
         # the three assembly strategies of the 1000x1000 cantilever, five times each (profiling)
         dev = native.Device(0)
-        pipeline.setup(meshgen.c3_model(args.n), device=dev, assembly="coloring")
+        pipeline.setup(meshgen.loguniform_cantilever(args.n), device=dev, assembly="coloring")
         out = {}
         for strategy in ("coloring", "atomic", "serial"):
             reps = [dev.assemble(strategy) for _ in range(5)]
@@ -219,9 +219,9 @@ def main() -> int:
         print(json.dumps({"assembly": out}))
         dev.close()
         return 0
-    if args.c2_only:
+    if args.precond_only:
         dev = native.Device(0)
-        print(json.dumps({"c2_preconditioners": run_c2(dev, args.tol)}))
+        print(json.dumps({"preconditioners_200x200": run_preconditioners_200(dev, args.tol)}))
         dev.close()
         return 0
     if args.batch_only:
@@ -231,7 +231,7 @@ def main() -> int:
         return 0
     n_mesh = args.n
     t0 = time.perf_counter()
-    model = meshgen.c3_model(n_mesh)
+    model = meshgen.loguniform_cantilever(n_mesh)
     dev = native.Device(0)
     print(f"bench: {n_mesh}x{n_mesh} plane-strain log-uniform E, setup ...", file=sys.stderr, flush=True)
     sys_ = pipeline.setup(model, device=dev, assembly="coloring")
@@ -371,7 +371,7 @@ def main() -> int:
     batch = None
     if not args.skip_batch:
         batch = run_batch(dev, args.batch_problems, args.tol, args.batch_forms.split(","))
-    c2 = None if args.skip_c2 else run_c2(dev, args.tol)
+    c2 = None if args.skip_precond else run_preconditioners_200(dev, args.tol)
     # Headline: the FP64 form. The mixed form (FP32 values with a compensated FP64
     # diagonal once ||r|| <= 1e-2 ||r0||, FP64 vectors and sums) stops on the same
     # recurrence rule and is reported beside it with its displacement difference.
@@ -407,7 +407,7 @@ def main() -> int:
                      "copy_ceiling_GBs": round(copy["GBs"], 1)},
         "cpu_baseline": cpu,
         "batch": batch,
-        "c2_preconditioners": c2,
+        "preconditioners_200x200": c2,
     }
     print(json.dumps(line))
     dev.close()

@@ -127,7 +127,9 @@ const char* lfg_status_string(int32_t status);
 
 /* Mesh and materials. Replaces q4_coordinates/find_material inside the Ke loop
  * (solver.cpp:48-56, 243-270). xy: 2*n_nodes; enodes: 4*n_elems node indices;
- * elem_material: n_elems material indices; materials: 3*n_materials (E, nu, t). */
+ * elem_material: n_elems material indices; materials: 3*n_materials (E, nu, t). A new
+ * mesh drops the pattern, bars and colours of the previous one (lfg_set_pattern and
+ * lfg_set_colors follow it). */
 int lfg_set_mesh(lfg_ctx* ctx, int64_t n_nodes, const double* xy, int64_t n_elems, const int32_t* enodes,
                  const int32_t* elem_material, int64_t n_materials, const double* materials, int32_t plane_mode);
 

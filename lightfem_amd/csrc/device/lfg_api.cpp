@@ -1135,6 +1135,11 @@ int lfg_set_mesh(lfg_ctx* ctx, int64_t n_nodes, const double* xy, int64_t n_elem
         ctx->h_bar_nodes.clear();
         build_node_elements(ctx);
         sync(ctx);
+        // the pattern, its element map and the colour groups describe the previous mesh
+        ctx->have_pattern = false;
+        ctx->gather_valid = false;
+        ctx->h_color_ptr.clear();
+        ctx->h_color_bar_ptr.clear();
         ctx->have_mesh = true;
         ctx->have_K = ctx->have_system = ctx->have_solution = false;
         ctx->precond_ready = -1;

@@ -147,12 +147,12 @@ SEED_DENSITY = 20260401  # 400x400 SIMP density
 SEED_BATCH = 20260501    # batch of uneven problems
 
 
-def c1_model() -> Q4Model:
+def cantilever_50() -> Q4Model:
     """50x50 plane-stress cantilever, E=1, nu=0.3, serial, no RCM, Jacobi."""
     return structured_cantilever(50, 50, settings={"assembly": "serial", "reordering": "none"})
 
 
-def c2_model(assembly: str = "coloring", preconditioner: str = "jacobi") -> Q4Model:
+def cantilever_200_rcm(assembly: str = "coloring", preconditioner: str = "jacobi") -> Q4Model:
     """200x200 cantilever, RCM on."""
     return structured_cantilever(200, 200, settings={"assembly": assembly, "reordering": "rcm",
                                                      "preconditioner": preconditioner})
@@ -163,7 +163,7 @@ def loguniform_young(n_elems: int, seed: int, decades: float = 3.0) -> np.ndarra
     return np.power(10.0, -decades * uniform01(seed, n_elems))
 
 
-def c3_model(n: int = 1000, seed: int = SEED_LOGE) -> Q4Model:
+def loguniform_cantilever(n: int = 1000, seed: int = SEED_LOGE) -> Q4Model:
     """n x n plane-strain cantilever with log-uniform E in [1e-3, 1]."""
     return structured_cantilever(n, n, young=loguniform_young(n * n, seed), plane_mode="plane_strain")
 
@@ -172,15 +172,15 @@ def simp_young(density: np.ndarray, emin: float = 1e-9, penal: float = 3.0) -> n
     return emin + density ** penal * (1.0 - emin)
 
 
-def c4_density(n: int = 400, seed: int = SEED_DENSITY) -> np.ndarray:
+def simp_density(n: int = 400, seed: int = SEED_DENSITY) -> np.ndarray:
     return 0.3 + 0.7 * uniform01(seed, n * n)
 
 
-def c4_model(n: int = 400, seed: int = SEED_DENSITY) -> Q4Model:
-    return structured_cantilever(n, n, young=simp_young(c4_density(n, seed)))
+def simp_cantilever(n: int = 400, seed: int = SEED_DENSITY) -> Q4Model:
+    return structured_cantilever(n, n, young=simp_young(simp_density(n, seed)))
 
 
-def c5_problems(n_problems: int = 256, seed: int = SEED_BATCH, n_min: int = 32, n_max: int = 256,
+def batch_problems(n_problems: int = 256, seed: int = SEED_BATCH, n_min: int = 32, n_max: int = 256,
                 n_cases: int = 4) -> list:
     """Uneven batch: nx, ny ~ U{n_min..n_max} on an nx x ny-unit rectangle, nu ~ U[0.2, 0.45],
     plane stress or strain 50/50, per-element E log-uniform in [1e-2, 1], left edge fixed,
@@ -202,14 +202,14 @@ def c5_problems(n_problems: int = 256, seed: int = SEED_BATCH, n_min: int = 32, 
     return out
 
 
-def c5_model(spec: dict) -> Q4Model:
+def batch_problem_model(spec: dict) -> Q4Model:
     """One batch problem (first load case as the model's load)."""
     return structured_cantilever(spec["nx"], spec["ny"], lx=float(spec["nx"]), ly=float(spec["ny"]),
                                  young=spec["young"], nu=spec["nu"], plane_mode=spec["plane_mode"],
                                  total_load=(float(np.cos(spec["theta"][0])), float(np.sin(spec["theta"][0]))))
 
 
-def c5_case_loads(model: Q4Model, spec: dict) -> np.ndarray:
+def batch_case_loads(model: Q4Model, spec: dict) -> np.ndarray:
     """(cases, loaded nodes, 2) tractions: the model's right-edge shares times (cos, sin) theta."""
     share = np.hypot(model.load_values[:, 0], model.load_values[:, 1])
     return np.stack([np.stack([share * np.cos(t), share * np.sin(t)], axis=1) for t in spec["theta"]])

@@ -318,27 +318,27 @@ def main() -> None:
         save_full("grid16x12_strain_loge", run_dump(open(os.path.join(work, "g16e.fem")).read(), [], work, "g16e"),
                   record)
 
-        # 50x50 (C1): digests plus the full displacement.
-        m = meshgen.c1_model()
-        m.to_fem(os.path.join(work, "c1.fem"))
-        c1 = run_dump(open(os.path.join(work, "c1.fem")).read(), [], work, "c1")
-        save_digest("c1", c1, record)
-        np.savez_compressed(os.path.join(HERE, "c1_u.npz"), u=c1["u"], R=c1["R"])
+        # 50x50 (cantilever): digests plus the full displacement.
+        m = meshgen.cantilever_50()
+        m.to_fem(os.path.join(work, "cantilever50.fem"))
+        c50 = run_dump(open(os.path.join(work, "cantilever50.fem")).read(), [], work, "cantilever50")
+        save_digest("cantilever50", c50, record)
+        np.savez_compressed(os.path.join(HERE, "cantilever50_u.npz"), u=c50["u"], R=c50["R"])
 
-        # 200x200 (C2): RCM with coloring, three preconditioners.
-        m = meshgen.c2_model()
-        m.to_fem(os.path.join(work, "c2.fem"))
-        c2_text = open(os.path.join(work, "c2.fem")).read()
+        # 200x200 (cantilever, RCM): RCM with coloring, three preconditioners.
+        m = meshgen.cantilever_200_rcm()
+        m.to_fem(os.path.join(work, "cantilever200.fem"))
+        c200_text = open(os.path.join(work, "cantilever200.fem")).read()
         for precond, extra in (("jacobi", []), ("ic0", []), ("block1", ["subdomain_nodes=1"])):
             pname = "block" if precond == "block1" else precond
-            arrays = run_dump(c2_text, ["reordering=rcm", "assembly=coloring", f"preconditioner={pname}", *extra],
-                              work, f"c2{precond}")
-            save_digest(f"c2_{precond}", arrays, record)
+            arrays = run_dump(c200_text, ["reordering=rcm", "assembly=coloring", f"preconditioner={pname}", *extra],
+                              work, f"cantilever200{precond}")
+            save_digest(f"cantilever200_{precond}", arrays, record)
         # 200x200 with the default block preconditioner (rcb, 64 nodes) and with block storage
-        arrays = run_dump(c2_text, ["reordering=rcm", "assembly=coloring", "preconditioner=block"], work, "c2b64")
-        save_digest("c2_block64", arrays, record)
-        arrays = run_dump(c2_text, ["reordering=rcm", "assembly=coloring", "storage=bsr"], work, "c2bsr")
-        save_digest("c2_bsr_jacobi", arrays, record)
+        arrays = run_dump(c200_text, ["reordering=rcm", "assembly=coloring", "preconditioner=block"], work, "cantilever200b64")
+        save_digest("cantilever200_block64", arrays, record)
+        arrays = run_dump(c200_text, ["reordering=rcm", "assembly=coloring", "storage=bsr"], work, "cantilever200bsr")
+        save_digest("cantilever200_bsr_jacobi", arrays, record)
     # CLI fixtures: `check` stdout and `solve` CSV tables of the reference CLI.
     record["cli"] = cli_fixtures()
     with open(os.path.join(HERE, "golden.json"), "w") as handle:

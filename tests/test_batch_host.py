@@ -5,8 +5,8 @@ from lightfem_amd import meshgen
 
 
 def test_c5_generator_ranges_and_determinism():
-    specs = meshgen.c5_problems(256)
-    again = meshgen.c5_problems(256)
+    specs = meshgen.batch_problems(256)
+    again = meshgen.batch_problems(256)
     nx = np.array([s["nx"] for s in specs])
     ny = np.array([s["ny"] for s in specs])
     assert nx.min() >= 32 and nx.max() <= 256 and ny.min() >= 32 and ny.max() <= 256
@@ -22,9 +22,9 @@ def test_c5_generator_ranges_and_determinism():
 
 
 def test_c5_load_cases_have_unit_total_force():
-    spec = meshgen.c5_problems(3, n_min=5, n_max=9)[1]
-    model = meshgen.c5_model(spec)
-    cases = meshgen.c5_case_loads(model, spec)
+    spec = meshgen.batch_problems(3, n_min=5, n_max=9)[1]
+    model = meshgen.batch_problem_model(spec)
+    cases = meshgen.batch_case_loads(model, spec)
     assert cases.shape == (4, spec["ny"] + 1, 2)
     for c, theta in enumerate(spec["theta"]):
         total = cases[c].sum(axis=0)

@@ -131,10 +131,10 @@ def correction_fd(prob, k_base, base_u, young, rho, e, h, selectors):
 def test_c4_gradients_400(dev):
     """400x400 SIMP cantilever: compliance and tip-displacement gradients."""
     print("case adjoint 400x400 SIMP (compliance, tip uy)", flush=True)
-    rho = meshgen.c4_density(400)
+    rho = meshgen.simp_density(400)
     young = simp(rho)
     dE = dsimp(rho)
-    prob = ElasticProblem(meshgen.c4_model(400), device=dev)
+    prob = ElasticProblem(meshgen.simp_cantilever(400), device=dev)
     u = prob.solve(young)
     k_base = dev.stiffness()
     e_d, d = tip_selector(prob)

@@ -28,9 +28,9 @@ def dev():
 
 
 def small_batch(n_problems=7, seed=4242, n_min=3, n_max=14, zero_case=True):
-    specs = meshgen.c5_problems(n_problems, seed=seed, n_min=n_min, n_max=n_max)
-    models = [meshgen.c5_model(sp) for sp in specs]
-    loads = [meshgen.c5_case_loads(m, sp) for m, sp in zip(models, specs)]
+    specs = meshgen.batch_problems(n_problems, seed=seed, n_min=n_min, n_max=n_max)
+    models = [meshgen.batch_problem_model(sp) for sp in specs]
+    loads = [meshgen.batch_case_loads(m, sp) for m, sp in zip(models, specs)]
     if zero_case:
         loads[2][1] = 0.0  # a case with no load converges at once with u = 0
     return specs, models, loads

@@ -216,13 +216,13 @@ def check_stencil_spmv(dev, row_ptr, col, kc, x):
 
 
 def _c1(dev, assembly="serial"):
-    return pipeline.solve(meshgen.c1_model(), assembly=assembly, device=dev)
+    return pipeline.solve(meshgen.cantilever_50(), assembly=assembly, device=dev)
 
 
 def test_c1_pipeline_matches_reference(dev, golden_index):
-    """50x50 cantilever: structures and K bit-exact, u and R to 1e-9."""
-    ref = golden_index["c1"]
-    print("case c1 50x50 serial jacobi", flush=True)
+    """50x50 plane-stress cantilever: structures and K bit-exact, u and R to 1e-9."""
+    ref = golden_index["cantilever50"]
+    print("case 50x50 serial jacobi", flush=True)
     out = _c1(dev)
     s = out["system"]
     assert digest(s.row_ptr.astype(np.int64)) == ref["sha256"]["row_ptr"]
@@ -230,7 +230,7 @@ def test_c1_pipeline_matches_reference(dev, golden_index):
     assert digest(s.device.stiffness()) == ref["sha256"]["K"]
     kc, fc = s.device.system()
     assert digest(kc) == ref["sha256"]["Kc"] and digest(fc) == ref["sha256"]["fc"]
-    gold = load_golden("c1_u")
+    gold = load_golden("cantilever50_u")
     assert abs(out["pcg"]["iterations"] - 313) <= 3
     assert np.linalg.norm(out["u"] - gold["u"]) <= 1e-9 * np.linalg.norm(gold["u"])
     assert np.max(np.abs(out["reaction"] - gold["R"])) <= 1e-9 * np.max(np.abs(gold["R"]))
@@ -238,11 +238,11 @@ def test_c1_pipeline_matches_reference(dev, golden_index):
 
 
 @pytest.mark.parametrize("precond,iters", [("jacobi", 1240), ("block", 1240), ("ic0", 359)])
-def test_c2_preconditioners(dev, golden_index, precond, iters):
-    """200x200, RCM, coloring: K digest exact, PCG vs the oracle."""
-    print(f"case c2 200x200 rcm coloring {precond}", flush=True)
-    sys_ = pipeline.setup(meshgen.c2_model(), device=dev, assembly="coloring", reordering="rcm")
-    ref = golden_index[f"c2_{'block1' if precond == 'block' else precond}"]
+def test_preconditioners_200x200(dev, golden_index, precond, iters):
+    """200x200 cantilever, RCM, coloring: K digest exact, PCG vs the oracle."""
+    print(f"case 200x200 rcm coloring {precond}", flush=True)
+    sys_ = pipeline.setup(meshgen.cantilever_200_rcm(), device=dev, assembly="coloring", reordering="rcm")
+    ref = golden_index[f"cantilever200_{'block1' if precond == 'block' else precond}"]
     assert digest(dev.stiffness()) == ref["sha256"]["K"]
     assert sys_.assembly_report["coloring_count"] == 4
     kc, fc = dev.system()
@@ -256,15 +256,15 @@ def test_c2_preconditioners(dev, golden_index, precond, iters):
         assert rep["breakdowns"] == 0 and rep["levels"] > 0
 
 
-def test_c2_subdomain_block_and_block_storage(dev, golden_index):
+def test_cantilever200_subdomain_block_and_block_storage(dev, golden_index):
     """200x200, RCM, coloring with the reference's default block preconditioner (rcb,
     64 nodes per subdomain: 632 dense 2x2-node-blocked Cholesky factors) and with block
     storage: the split, the subdomain applications and the block storage products are
     the reference's bit for bit; the solves take its iteration counts (734, 1240) and
     agree with the Jacobi solve of the same system."""
-    print("case c2 200x200 rcm coloring block rcb64 / storage bsr", flush=True)
-    sys_ = pipeline.setup(meshgen.c2_model(), device=dev, assembly="coloring", reordering="rcm")
-    ref = golden_index["c2_block64"]
+    print("case 200x200 rcm coloring block rcb64 / storage bsr", flush=True)
+    sys_ = pipeline.setup(meshgen.cantilever_200_rcm(), device=dev, assembly="coloring", reordering="rcm")
+    ref = golden_index["cantilever200_block64"]
     nodes, count = native.subdomain_split(sys_.indexed.xy, "rcb", 64)
     assert count == 632 and digest(np.repeat(nodes.astype(np.int64), 2)) == ref["sha256"]["dof_subdomain"]
     dev.set_subdomains(nodes, count)
@@ -281,7 +281,7 @@ def test_c2_subdomain_block_and_block_storage(dev, golden_index):
     # block storage: products bit for bit, the PCG on the node-slice and default layouts
     y_bsr = lfo.bsr_spmv(sys_.row_ptr, sys_.col, kc, x)
     assert np.array_equal(bits(dev.spmv(x, "fp64", 6, block_order=True)), bits(y_bsr))
-    ref_bsr = golden_index["c2_bsr_jacobi"]["meta"]
+    ref_bsr = golden_index["cantilever200_bsr_jacobi"]["meta"]
     for variant in (6, 0):
         ub, rb = dev.pcg("jacobi", "fp64", "reference", 1e-10, storage="bsr", spmv_variant=variant)
         assert rb["converged"] == 1 and abs(rb["iterations"] - ref_bsr["iterations"]) <= 2 + 0.01 * 1240
@@ -303,7 +303,7 @@ def test_mixed_precision_matches_fp64(dev, precond, switch, variant, drop):
     n = 64
     print(f"case mixed {n}x{n} log-uniform E plane strain {precond} switch={switch} variant={variant} drop={drop}",
           flush=True)
-    m = meshgen.c3_model(n)
+    m = meshgen.loguniform_cantilever(n)
     sys_ = pipeline.setup(m, device=dev, assembly="coloring")
     kc, fc = dev.system()
     u64, r64 = dev.pcg(precond, "fp64", "relative", 1e-10)
@@ -325,7 +325,7 @@ def test_mixed_precision_matches_fp64(dev, precond, switch, variant, drop):
 
 def test_deterministic_and_graph_equals_direct(dev):
     print("case determinism 50x50", flush=True)
-    pipeline.setup(meshgen.c1_model(), device=dev)
+    pipeline.setup(meshgen.cantilever_50(), device=dev)
     u1, r1 = dev.pcg("jacobi")
     u2, r2 = dev.pcg("jacobi")
     u3, r3 = dev.pcg("jacobi", direct_launch=True, check_every=7)
@@ -343,7 +343,7 @@ def test_stencil_layouts(dev):
     offsets found from the pattern, multiplies against the CSR row loop, PCG against
     the node-slice PCG (both preconditioners it fuses), run-to-run identical."""
     print("case stencil 50x50 and 64x64 log-E", flush=True)
-    sys_ = pipeline.setup(meshgen.c1_model(), device=dev)
+    sys_ = pipeline.setup(meshgen.cantilever_50(), device=dev)
     info = dev.layout_info()
     assert info["stencil_offsets"] == [-52, -51, -50, -1, 0, 1, 50, 51, 52]
     assert info["stencil_sym_planes"] == 5 and info["stencil_exact_planes"] == 9
@@ -351,7 +351,7 @@ def test_stencil_layouts(dev):
     x = np.random.default_rng(7).standard_normal(dev.n)
     check_stencil_spmv(dev, sys_.row_ptr, sys_.col, kc, x)
     assert np.array_equal(bits(dev.spmv(x, "fp64", 9)), bits(dev.spmv(x, "fp64", 6)))
-    pipeline.setup(meshgen.c3_model(64), device=dev, assembly="coloring")
+    pipeline.setup(meshgen.loguniform_cantilever(64), device=dev, assembly="coloring")
     for precond in ("jacobi", "block"):
         u6, r6 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=6)
         u9, r9 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=9)
@@ -369,7 +369,7 @@ def test_stencil_layouts(dev):
     _, capped = dev.pcg("jacobi", "fp64", "relative", 1e-10, max_iterations=5, spmv_variant=10, raise_on_error=False)
     assert capped["iterations"] == 5 and not capped["converged"]
     # an RCM-numbered mesh has no fixed offsets: the default falls back to node slices
-    pipeline.setup(meshgen.c2_model(), device=dev, assembly="coloring", reordering="rcm")
+    pipeline.setup(meshgen.cantilever_200_rcm(), device=dev, assembly="coloring", reordering="rcm")
     info = dev.layout_info()
     assert info["node_blocked"] == 1 and info["stencil_sym_planes"] == 0 and info["fused_window_pairs"] == 0
     with pytest.raises(native.LightFemError) as err:
@@ -384,7 +384,7 @@ def test_fused_iteration_iterate_at_every_cap(dev):
     as many iterations (a missing or doubled alpha p term is off by orders of magnitude
     more than the rounding of the symmetric stencil sum order)."""
     print("case fused iteration caps 64x64 log-E", flush=True)
-    sys_ = pipeline.setup(meshgen.c3_model(64), device=dev, assembly="coloring")
+    sys_ = pipeline.setup(meshgen.loguniform_cantilever(64), device=dev, assembly="coloring")
     kc, fc = dev.system()
     for cap in (1, 2, 3, 6, 7, 50, 51, 100000):
         u, rep = dev.pcg("jacobi", "fp64", "relative", 1e-10, max_iterations=cap, spmv_variant=10,
@@ -402,7 +402,7 @@ def test_fused_direction_is_bit_identical(dev):
     Same values in the same order: identical solutions and iteration counts, with and
     without graphs, for both fused preconditioners and both precisions."""
     print("case fused direction 200x200", flush=True)
-    pipeline.setup(meshgen.c2_model(), device=dev, assembly="coloring", reordering="rcm")
+    pipeline.setup(meshgen.cantilever_200_rcm(), device=dev, assembly="coloring", reordering="rcm")
     for precond in ("jacobi", "block"):
         u6, r6 = dev.pcg(precond, spmv_variant=6)
         u7, r7 = dev.pcg(precond, spmv_variant=7)
@@ -432,12 +432,21 @@ def test_error_paths(dev):
     with pytest.raises(native.LightFemError) as err:
         dev.pcg("jacobi")
     assert err.value.code in (native.ERR_NOT_SPD, native.ERR_NONFINITE)
+    # a new mesh drops the previous mesh's pattern and colours: assembling before they are
+    # set again is a state error, not a sum over stale lists
+    sys_ = pipeline.setup(m, device=dev)
+    ix = sys_.indexed
+    dev.set_mesh(ix.xy, ix.enodes, ix.elem_mat, ix.materials, ix.plane_mode)
+    for strategy in ("coloring", "serial"):
+        with pytest.raises(native.LightFemError) as err:
+            dev.assemble(strategy)
+        assert err.value.code == native.ERR_STATE, strategy
 
 
 def test_full_size_assembly_properties(dev):
-    """1000x1000 (BASELINE config 3 size): strategies agree, Dirichlet rows are identity."""
+    """1000x1000 log-uniform-E cantilever: strategies agree, Dirichlet rows are identity."""
     print("case 1000x1000 assembly strategies", flush=True)
-    m = meshgen.c3_model(1000)
+    m = meshgen.loguniform_cantilever(1000)
     sys_ = pipeline.setup(m, device=dev, assembly="serial")
     assert sys_.col.shape[0] == 36024004 and sys_.row_ptr.shape[0] - 1 == 2004002
     k_serial = dev.stiffness()

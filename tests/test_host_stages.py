@@ -131,8 +131,8 @@ def test_profile_readme_200():
 def test_c2_integer_structures_match_reference_digests(golden_index):
     import hashlib
     dig = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<i8").tobytes()).hexdigest()  # noqa: E731
-    ref = golden_index["c2_jacobi"]["sha256"]
-    ix = pipeline.index_model(meshgen.c2_model(), "rcm")
+    ref = golden_index["cantilever200_jacobi"]["sha256"]
+    ix = pipeline.index_model(meshgen.cantilever_200_rcm(), "rcm")
     assert dig(ix.node_ids) == ref["node_ids"]
     row_ptr, col, emap = native.csr_pattern(ix.n_nodes, ix.enodes)
     assert dig(row_ptr) == ref["row_ptr"] and dig(col) == ref["col"] and dig(emap) == ref["emap"]

@@ -100,14 +100,14 @@ def test_oracle_assembly_variants_agree():
 
 
 def test_oracle_c1_digests(golden_index):
-    """50x50 cantilever: every stage hashed bit-exactly."""
+    """50x50 plane-stress cantilever: every stage hashed bit-exactly."""
     import hashlib
-    m = meshgen.c1_model()
+    m = meshgen.cantilever_50()
     xy = m.xy
     enode = m.enodes - 1
     n = 2 * xy.shape[0]
     out = lfo.solve_model(xy, enode, m.element_young(), lfo.PLANE_STRESS, _load_vector(m), *_constraints(m))
-    ref = golden_index["c1"]
+    ref = golden_index["cantilever50"]
     dig = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype="<i8" if a.dtype.kind in "iu" else "<f8").tobytes()).hexdigest()  # noqa
     for key, ours in (("row_ptr", out["row_ptr"]), ("col", out["col"]), ("emap", out["emap"]), ("ke", out["ke"]),
                       ("K", out["K"]), ("Kc", out["Kc"]), ("fc", out["fc"]), ("u", out["u"]), ("R", out["R"])):

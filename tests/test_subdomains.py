@@ -38,8 +38,8 @@ def test_host_split_matches_reference(name, golden_index):
 def test_host_split_200x200_rcm_matches_reference(golden_index):
     """The default split (rcb, 64 nodes) of the 200x200 cantilever in RCM order: 632
     subdomains, map digest from the compiled reference."""
-    ref = golden_index["c2_block64"]
-    ix = pipeline.index_model(meshgen.c2_model(), "rcm")
+    ref = golden_index["cantilever200_block64"]
+    ix = pipeline.index_model(meshgen.cantilever_200_rcm(), "rcm")
     nodes, count = native.subdomain_split(ix.xy, "rcb", 64)
     assert count == ref["meta"]["subdomains"] == 632
     assert digest(np.repeat(nodes.astype(np.int64), 2)) == ref["sha256"]["dof_subdomain"]
