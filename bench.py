@@ -371,7 +371,7 @@ def main() -> int:
     batch = None
     if not args.skip_batch:
         batch = run_batch(dev, args.batch_problems, args.tol, args.batch_forms.split(","))
-    c2 = None if args.skip_precond else run_preconditioners_200(dev, args.tol)
+    precond200 = None if args.skip_precond else run_preconditioners_200(dev, args.tol)
     # Headline: the FP64 form. The mixed form (FP32 values with a compensated FP64
     # diagonal once ||r|| <= 1e-2 ||r0||, FP64 vectors and sums) stops on the same
     # recurrence rule and is reported beside it with its displacement difference.
@@ -407,7 +407,7 @@ def main() -> int:
                      "copy_ceiling_GBs": round(copy["GBs"], 1)},
         "cpu_baseline": cpu,
         "batch": batch,
-        "preconditioners_200x200": c2,
+        "preconditioners_200x200": precond200,
     }
     print(json.dumps(line))
     dev.close()
