@@ -63,7 +63,7 @@ __device__ __forceinline__ bool finish_problem(const BatchView& bv, int p, int s
     if (lane_id() == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) publish_partial(part + (s - first) * K + k, vals[k]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        stores_complete();
         const unsigned int a = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         last = a == static_cast<unsigned int>(count - 1);
     }

@@ -69,14 +69,21 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 }
 
 // Partials cross blocks (and XCDs) through relaxed agent-scope atomic stores and
-// loads, which the hardware keeps coherent across the XCDs' L2s by themselves; a
-// workgroup-scope release (a wait for the store to complete) orders a partial
-// before its ticket. Agent-scope acquire/release would instead write back and
-// invalidate the whole L2 of the XCD in every block. Everything else a kernel
-// writes reaches the next kernel through the kernel boundary.
+// loads, which the hardware keeps coherent across the XCDs' L2s by themselves.
+// What orders a partial before the ticket that announces it is an explicit wait
+// for the thread's outstanding vector-memory operations (stores_complete): a store
+// and an atomic to different addresses go to different L2 channels and may
+// complete in either order, and a workgroup-scope release fence emits no wait on
+// gfx950 outside threadgroup-split mode. Agent-scope acquire/release would also
+// order them, but writes back and invalidates the whole L2 of the XCD in every
+// block. Everything else a kernel writes reaches the next kernel through the
+// kernel boundary.
 __device__ __forceinline__ void publish_partial(double* slot, double v) {
     __hip_atomic_store(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Returns once every global store this thread issued has been acknowledged.
+__device__ __forceinline__ void stores_complete() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 __device__ __forceinline__ double read_partial(const double* slot) {
     return __hip_atomic_load(const_cast<double*>(slot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -101,7 +108,7 @@ __device__ __forceinline__ bool finish_reduction(TicketSet* t, double* partials,
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) publish_partial(partials + blockIdx.x * K + k, vals[k]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        stores_complete();
         const unsigned int a = __hip_atomic_fetch_add(&t->xcd[x][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         flag = (a == static_cast<unsigned int>(per - 1)) ? 1 : 0;
     }
@@ -118,7 +125,7 @@ __device__ __forceinline__ bool finish_reduction(TicketSet* t, double* partials,
         __hip_atomic_store(&t->xcd[x][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int k = 0; k < K; ++k) publish_partial(group + x * K + k, s[k]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        stores_complete();
         const unsigned int a = __hip_atomic_fetch_add(&t->top[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         flag = (a == 7u) ? 1 : 0;
     }

@@ -132,6 +132,9 @@ __device__ __forceinline__ double subdomain_solves(const SubdomainView& v, const
     for (int s = blockIdx.x; s < v.n_sub; s += gridDim.x) {
         const int begin = v.dof_begin[s];
         const int size = v.dof_begin[s + 1] - begin;
+        // an empty subdomain (an id no node carries) has no factor: its pivot loads
+        // would read the entry before it; size is the same in every thread of the block
+        if (size == 0) continue;
         const double* __restrict__ fg = v.factor + v.factor_begin[s];
         const double* F = fg;
         if (kLds) {
