@@ -91,9 +91,72 @@ def measured_traffic(kernel: str, n_mesh: int):
     return None
 
 
+def host_threads() -> int:
+    """This host's CPU share: OMP_NUM_THREADS where the job sets it, else every core."""
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def parse_cli_line(stdout: str, tag: str) -> dict:
+    """The numeric key=value fields of one `TAG: k=v, k=v` line of the CLI's stdout."""
+    out = {}
+    for line in stdout.splitlines():
+        if line.startswith(tag + ": "):
+            for kv in line[len(tag) + 2:].split(", "):
+                key, _, value = kv.partition("=")
+                try:
+                    out[key] = float(value)
+                except ValueError:
+                    pass
+            break
+    return out
+
+
+def cpu_protocol(runs: int = 7) -> dict:
+    """The reference's measurement protocol (README.md:115-138: CLI `solve`, the TIMING
+    line's element_stiffness_ms, assembly_ms and pcg_ms, median of `runs` runs in one
+    batch) on C1 (50x50, serial, Jacobi) and C2 (200x200, RCM, coloring, Jacobi): the
+    compiled reference CLI at threads = this host's share and threads = 1, and this
+    repository's CLI (GPU backend) at the host's share."""
+    from lightfem_amd import meshgen
+    ref_cli = os.path.join(ROOT, "oracle", "_ref", "lightfem_ref")
+    our_cli = os.path.join(ROOT, "lightfem_amd", "bin", "lightfem")
+    if not (os.path.exists(ref_cli) and os.path.exists(our_cli)):
+        return None
+    keys = ("element_stiffness_ms", "assembly_ms", "pcg_ms", "matvec_ms", "process_ms")
+    nproc = host_threads()
+    out = {"runs": runs, "statistic": "median", "host_threads": nproc}
+    with tempfile.TemporaryDirectory() as work:
+        for name, make in (("c1_50x50_serial_jacobi", meshgen.cantilever_50),
+                           ("c2_200x200_rcm_coloring_jacobi", meshgen.cantilever_200_rcm)):
+            entry = {}
+            for who, cli, threads in (("reference", ref_cli, nproc), ("reference", ref_cli, 1),
+                                      ("gpu_cli", our_cli, nproc)):
+                model = make()
+                model.settings["threads"] = threads
+                fem = os.path.join(work, f"{name}_{threads}.fem")
+                model.to_fem(fem)
+                samples, iterations = [], None
+                for _ in range(runs):
+                    res = subprocess.run([cli, "solve", fem, os.path.join(work, "out.vtk")], capture_output=True,
+                                         text=True, timeout=600)
+                    if res.returncode != 0:
+                        break
+                    samples.append(parse_cli_line(res.stdout, "TIMING"))
+                    iterations = int(parse_cli_line(res.stdout, "SOLVED")["pcg_iterations"])
+                if not samples:
+                    entry[f"{who}_threads_{threads}"] = {"error": res.stderr.strip()[-200:]}
+                    continue
+                med = {k: float(np.median([smp[k] for smp in samples])) for k in keys}
+                entry[f"{who}_threads_{threads}"] = dict(med, pcg_iterations=iterations)
+                print(f"bench: cpu protocol {name} {who} threads {threads} {json.dumps(entry[f'{who}_threads_{threads}'])}",
+                      file=sys.stderr, flush=True)
+            out[name] = entry
+    return out
+
+
 def cpu_baseline(n_mesh: int, young: np.ndarray, iterations_full: int, sample_iters: int) -> dict:
     ref = os.path.join(ROOT, "oracle", "_ref", "lfref")
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = host_threads()
     if os.path.exists(ref):
         with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as handle:
             young.astype("<f8").tofile(handle)
@@ -110,10 +173,42 @@ def cpu_baseline(n_mesh: int, young: np.ndarray, iterations_full: int, sample_it
                 "kind": "reference",
                 "sample": f"compiled reference pcg_solve, {rec['iterations']} Jacobi-PCG iterations of the same "
                           f"1000x1000 system ({rec['pcg_ms']:.0f} ms, matvec {rec['matvec_ms']:.0f} ms) scaled to "
-                          f"{iterations_full} iterations",
+                          f"{iterations_full} iterations, the reference's own count to ||r|| <= 1e-10 ||b|| on "
+                          f"this system (its complete run is recorded in tests/golden/c3_oracle_1000.npz)",
                 "ms_per_iteration": per_iter, "element_stiffness_ms": rec["element_stiffness_ms"],
                 "assembly_ms": rec["assembly_ms"]}
     return None
+
+
+def load_c3_oracle(n_mesh: int):
+    """The FP64 oracle fixture of the 1000x1000 system (tests/golden/c3_oracle_1000.npz,
+    written by tests/golden/gen_c3_oracle.py): a fixed subsample of u, R and the
+    element-centre von Mises stress of the compiled reference's Jacobi-PCG solution and
+    of a SuperLU direct solve."""
+    path = os.path.join(ROOT, "tests", "golden", f"c3_oracle_{n_mesh}.npz")
+    if not os.path.exists(path):
+        return None
+    with np.load(path) as data:
+        out = {k: data[k] for k in data.files}
+    out["meta"] = json.loads(out["meta"].tobytes().decode())
+    return out
+
+
+def oracle_errors(dev, u: np.ndarray, oracle: dict) -> dict:
+    """Relative (norm-wise) differences of u, the reactions at the constrained dofs and the
+    sampled element-centre von Mises stress of the device's last solve from both FP64
+    oracles, on the fixture's subsample."""
+    reactions = dev.reactions()[oracle["cdofs"]]
+    centre, _, _ = dev.postprocess()
+    vm = centre[oracle["elem_idx"], 6]
+    us = u[oracle["dof_idx"]]
+    out = {}
+    for name in ("ref", "direct"):
+        for key, mine in (("u", us), ("R", reactions), ("vm", vm)):
+            theirs = oracle[f"{key}_{name}"]
+            out[f"{key}_vs_{'reference_pcg' if name == 'ref' else 'direct'}"] = float(
+                np.linalg.norm(mine - theirs) / np.linalg.norm(theirs))
+    return out
 
 
 def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict:
@@ -188,6 +283,9 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--cpu-iters", type=int, default=2000)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-cpu-protocol", action="store_true",
+                    help="skip the C1 / C2 CLI timings (reference protocol, median of --protocol-runs)")
+    ap.add_argument("--protocol-runs", type=int, default=7)
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--skip-batch", action="store_true", help="skip the 256-problem batch workload")
     ap.add_argument("--batch-problems", type=int, default=256)
@@ -309,10 +407,10 @@ def main() -> int:
     print(f"bench: device copy {json.dumps(copy)}", file=sys.stderr, flush=True)
     forms = {}
     cap = 10 ** 6
+    oracle = load_c3_oracle(n_mesh)
     for form in ("fp64", "mixed"):
         variant = solve_variant
-        steps = args.steps if form == "fp64" else 1  # the mixed form is reported, not the headline
-        warmup = args.warmup if form == "fp64" else 0
+        steps, warmup = args.steps, args.warmup  # both forms timed alike
         for _ in range(warmup):
             dev.pcg("jacobi", form, "relative", args.tol, cap, args.check_every, spmv_variant=variant,
                     raise_on_error=False)
@@ -332,9 +430,13 @@ def main() -> int:
             cap = 2 * reps[-1]["iterations"]  # the mixed forms get at most twice the FP64 count
         forms[form] = {"ms_per_solve": wall, "device_solve_ms": float(np.median([r["solve_ms"] for r in reps])),
                        "iterations": reps[-1]["iterations"], "converged": bool(reps[-1]["converged"]),
-                       "replacements": reps[-1]["residual_replacements"],
+                       "refine_passes": reps[-1]["refine_passes"],
                        "true_relative_residual": reps[-1]["true_residual"] / reps[-1]["initial_residual"],
-                       "spmv_variant": variant, "steps": steps, "u": u}
+                       "spmv_variant": variant, "steps": steps, "warmup": warmup, "u": u}
+        if oracle is not None:
+            forms[form]["error_vs_fp64_oracles"] = oracle_errors(dev, u, oracle)
+            print(f"bench: {form} against the FP64 oracles {json.dumps(forms[form]['error_vs_fp64_oracles'])}",
+                  file=sys.stderr, flush=True)
         print(f"bench: {form} {wall:.1f} ms/solve, {reps[-1]['iterations']} iterations", file=sys.stderr, flush=True)
     # FP32 values from the first iteration (residual replacement only), capped at 2x the FP64 count
     t = time.perf_counter()
@@ -343,8 +445,10 @@ def main() -> int:
     forms["fp32_values_throughout"] = {
         "ms_per_solve": (time.perf_counter() - t) * 1e3, "device_solve_ms": rep32["solve_ms"],
         "iterations": rep32["iterations"], "converged": bool(rep32["converged"]),
-        "replacements": rep32["residual_replacements"],
+        "refine_passes": rep32["refine_passes"],
         "true_relative_residual": rep32["true_residual"] / rep32["initial_residual"], "iteration_cap": cap}
+    if oracle is not None:
+        forms["fp32_values_throughout"]["error_vs_fp64_oracles"] = oracle_errors(dev, u32, oracle)
     print(f"bench: fp32-throughout {forms['fp32_values_throughout']}", file=sys.stderr, flush=True)
     u64, umx = forms["fp64"].pop("u"), forms["mixed"].pop("u")
     forms["mixed"]["rel_u_diff_vs_fp64"] = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
@@ -361,13 +465,26 @@ def main() -> int:
             "rel_u_diff_vs_fp64": float(np.linalg.norm(u6 - u64) / np.linalg.norm(u64))}
         print(f"bench: fp64 node slices {forms['fp64_node_slices']}", file=sys.stderr, flush=True)
     per_iter_ms = {k: v["device_solve_ms"] / max(1, v["iterations"]) for k, v in forms.items()}
-    cpu = None
+    cpu, protocol = None, None
     if not args.skip_cpu:
+        ref_iterations = forms["fp64"]["iterations"]
+        if oracle is not None and oracle["meta"].get("reference_pcg", {}).get("iterations"):
+            ref_iterations = int(oracle["meta"]["reference_pcg"]["iterations"])
         try:
-            cpu = cpu_baseline(n_mesh, sys_.indexed.materials[sys_.indexed.elem_mat][:, 0],
-                               forms["fp64"]["iterations"], args.cpu_iters)
+            cpu = cpu_baseline(n_mesh, sys_.indexed.materials[sys_.indexed.elem_mat][:, 0], ref_iterations,
+                               args.cpu_iters)
+            if oracle is not None:
+                cpu["reference_complete_run_recorded"] = {
+                    k: oracle["meta"]["reference_pcg"][k] for k in ("iterations", "pcg_ms", "matvec_ms",
+                                                                    "threads_used")}
+                cpu["reference_complete_run_recorded"]["where"] = "the build container's cores, not this host"
         except Exception as exc:  # reported, not fatal
             print(f"bench: cpu baseline failed: {exc}", file=sys.stderr)
+        if not args.skip_cpu_protocol:
+            try:
+                protocol = cpu_protocol(args.protocol_runs)
+            except Exception as exc:  # reported, not fatal
+                print(f"bench: cpu protocol failed: {exc}", file=sys.stderr)
     batch = None
     if not args.skip_batch:
         batch = run_batch(dev, args.batch_problems, args.tol, args.batch_forms.split(","))
@@ -406,6 +523,7 @@ def main() -> int:
                      "fp32_values_iteration_GBs": round(iteration_kernel["mixed"]["GBs"], 1),
                      "copy_ceiling_GBs": round(copy["GBs"], 1)},
         "cpu_baseline": cpu,
+        "cpu_protocol_c1_c2": protocol,
         "batch": batch,
         "preconditioners_200x200": precond200,
     }

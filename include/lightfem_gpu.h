@@ -93,14 +93,22 @@ typedef struct {
     int32_t storage;         /* LFG_STORAGE_*: BSR sums each 2x2 block's products first (variants 6, 7
                                 and 9 follow it bit for bit; 8 and 10 read the transposed blocks as
                                 always; 1-5 are CSR only). Needs a node-blocked pattern. */
-    int32_t pad;
+    int32_t refine_passes;   /* FP64 iterative refinement after the solve: up to this many passes of
+                                r = b - K u with FP64 values, a correction solve K d = r (same precision
+                                policy) to ||r_d|| <= max(refine_reduction ||r||, target) and u += d,
+                                until ||b - K u|| meets the target or a pass fails to halve it (the FP64
+                                residual floor; a pass that made it worse is undone). 0 = default: 4
+                                for mixed precision, none for FP64 (the reference's recurrence rule
+                                alone); negative = none. Mixed precision with passes left and the FP64
+                                residual above the target and still falling reports NOT_CONVERGED. */
+    double refine_reduction; /* residual reduction of each correction solve; 0 = default 1e-2 */
 } lfg_pcg_options;
 
 typedef struct {
     int32_t converged;
     int32_t status;               /* lfg_status of the solve */
     int64_t iterations;
-    int64_t residual_replacements;  /* mixed precision: FP64 residual recomputations */
+    int64_t residual_replacements;  /* FP64 residual recomputations: replacements and refinement passes */
     double initial_residual;      /* ||r0|| = ||b|| (x0 = 0) */
     double final_residual;        /* ||r|| at stop: recurrence residual (FP64) or replaced residual (mixed) */
     double true_residual;         /* ||b - K u|| recomputed in FP64 after the solve */
@@ -109,6 +117,8 @@ typedef struct {
     int64_t breakdowns;           /* IC(0) clamped pivots (precond.cpp:373-378) */
     int64_t degenerate_blocks;    /* 2x2 blocks replaced by identity (precond.cpp:218-223) */
     int64_t levels;               /* IC(0): forward + backward level count */
+    int64_t refine_passes;        /* FP64 iterative-refinement correction solves run (their iterations
+                                     are included in `iterations`) */
 } lfg_pcg_report;
 
 typedef struct {

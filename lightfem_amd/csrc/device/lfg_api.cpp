@@ -152,6 +152,7 @@ struct lfg_ctx {
     // solver
     DevArray<double> x, r, p, p2, q, z, minv, factor, diag, partials, scratch;
     DevArray<double> r_alt, q_alt;  // fused stencil iteration: second r and q buffers (p uses p2)
+    DevArray<double> ir_r, ir_d;    // iterative refinement: FP64 residual b - K x and the correction
     DevArray<double> lam, adj_rhs, dE, grad;  // adjoint solution and right-hand side, sensitivities
     bool have_adjoint = false;
     DevArray<lfg::PcgState> state;
@@ -1570,7 +1571,6 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
         rep.levels = ctx->ic0.fwd_levels + ctx->ic0.bwd_levels;
     }
 
-    const lfg::PcgBuffers b = pcg_buffers(ctx, kind, rhs, xs);
     const bool stencil_fused = variant == kStencilFused;
     const bool fused = fused_direction(variant, kind) || stencil_fused;
     if (fused) ctx->p2.alloc(static_cast<size_t>(ctx->n));
@@ -1586,82 +1586,143 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     const double drop = opt->replace_drop > 0.0 && !stencil_fused ? opt->replace_drop : 0.0;
     const double switch_tau = !mixed ? 0.0 : (opt->mixed_switch != 0.0 ? opt->mixed_switch : 1.0e-2);
     const int replace = mixed && drop > 0.0;
-    // capture `per_graph` iterations once
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    const bool use_graph = opt->direct_launch == 0;
-    if (use_graph) {
-        LFG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, replace, variant, i & 1);
-        LFG_HIP(hipStreamEndCapture(ctx->stream, &graph));
-        LFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    }
-    struct GraphGuard {
-        hipGraph_t g;
-        hipGraphExec_t e;
-        ~GraphGuard() {
-            if (e) (void)hipGraphExecDestroy(e);
-            if (g) (void)hipGraphDestroy(g);
-        }
-    } guard{graph, exec};
 
-    LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-    lfg::launch_pcg_init(ctx->stream, b, opt->stop_rule, opt->tolerance, opt->max_iterations, drop, switch_tau,
-                         stencil_fused ? 1 : 0);
-    if (kind == LFG_PRECOND_IC0) {
-        lfg::launch_pcg_ic0_step(ctx->stream, b, ic0_view(ctx));
-        lfg::launch_pcg_direction(ctx->stream, b);
-    } else if (kind == LFG_PRECOND_SUBDOMAIN) {
-        lfg::launch_pcg_subdomain_step(ctx->stream, b, subdomain_view(ctx));
-        lfg::launch_pcg_direction(ctx->stream, b);
-    }
-    // every iteration the cap allows fits in this many graph launches
-    // (the fused stencil iteration spends its first launch on p0 and A p0)
-    const long long launches_needed = opt->max_iterations + (stencil_fused ? 1 : 0);
-    const long long launch_cap = std::max<long long>(1, (launches_needed + per_graph - 1) / per_graph);
-    long long launched = 0;
-    auto launch_one = [&](int slot) {
-        if (use_graph) {
-            LFG_HIP(hipGraphLaunch(exec, ctx->stream));
-        } else {
+    // One PCG solve of K x = b_vec from x = 0 into x_vec (solver.cpp:532-627) with the
+    // stop rule (rule, tol); returns the final device state and the device milliseconds.
+    auto pcg_pass = [&](const double* b_vec, double* x_vec, int rule, double tol, long long max_iter,
+                        float* ms) -> lfg::PcgState {
+        const lfg::PcgBuffers b = pcg_buffers(ctx, kind, b_vec, x_vec);
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        const bool use_graph = opt->direct_launch == 0;
+        if (use_graph) {  // capture `per_graph` iterations once
+            LFG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, replace, variant, i & 1);
+            LFG_HIP(hipStreamEndCapture(ctx->stream, &graph));
+            LFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         }
-        LFG_HIP(hipMemcpyAsync(ctx->h_state + slot, ctx->state.p, sizeof(lfg::PcgState), hipMemcpyDeviceToHost,
-                               ctx->stream));
-        LFG_HIP(hipEventRecord(ctx->ev_ring[slot], ctx->stream));
-        ++launched;
+        struct GraphGuard {
+            hipGraph_t g;
+            hipGraphExec_t e;
+            ~GraphGuard() {
+                if (e) (void)hipGraphExecDestroy(e);
+                if (g) (void)hipGraphDestroy(g);
+            }
+        } guard{graph, exec};
+
+        LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+        lfg::launch_pcg_init(ctx->stream, b, rule, tol, max_iter, drop, switch_tau, stencil_fused ? 1 : 0);
+        if (kind == LFG_PRECOND_IC0) {
+            lfg::launch_pcg_ic0_step(ctx->stream, b, ic0_view(ctx));
+            lfg::launch_pcg_direction(ctx->stream, b);
+        } else if (kind == LFG_PRECOND_SUBDOMAIN) {
+            lfg::launch_pcg_subdomain_step(ctx->stream, b, subdomain_view(ctx));
+            lfg::launch_pcg_direction(ctx->stream, b);
+        }
+        // every iteration the cap allows fits in this many graph launches
+        // (the fused stencil iteration spends its first launch on p0 and A p0)
+        const long long launches_needed = max_iter + (stencil_fused ? 1 : 0);
+        const long long launch_cap = std::max<long long>(1, (launches_needed + per_graph - 1) / per_graph);
+        long long launched = 0;
+        auto launch_one = [&](int slot) {
+            if (use_graph) {
+                LFG_HIP(hipGraphLaunch(exec, ctx->stream));
+            } else {
+                for (int i = 0; i < per_graph; ++i) enqueue_iteration(ctx, b, mixed, replace, variant, i & 1);
+            }
+            LFG_HIP(hipMemcpyAsync(ctx->h_state + slot, ctx->state.p, sizeof(lfg::PcgState), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            LFG_HIP(hipEventRecord(ctx->ev_ring[slot], ctx->stream));
+            ++launched;
+        };
+        launch_one(0);
+        long long k = 0;
+        while (true) {
+            const int slot = static_cast<int>(k & 1);
+            if (launched < launch_cap) launch_one(1 - slot);
+            LFG_HIP(hipEventSynchronize(ctx->ev_ring[slot]));
+            if (ctx->h_state[slot].done || launched >= launch_cap) {
+                *ms = elapsed(ctx->ev_start, ctx->ev_ring[slot]);
+                break;
+            }
+            ++k;
+        }
+        if (stencil_fused) lfg::launch_pcg_fused_flush(ctx->stream, b, ctx->p2.p);  // pb[1] of enqueue_iteration
+        sync(ctx);
+        lfg::PcgState st{};
+        LFG_HIP(hipMemcpy(&st, ctx->state.p, sizeof(st), hipMemcpyDeviceToHost));
+        return st;
     };
-    launch_one(0);
-    long long k = 0;
-    float solve_ms = 0.0f;
-    while (true) {
-        const int slot = static_cast<int>(k & 1);
-        if (launched < launch_cap) launch_one(1 - slot);
-        LFG_HIP(hipEventSynchronize(ctx->ev_ring[slot]));
-        if (ctx->h_state[slot].done || launched >= launch_cap) {
-            solve_ms = elapsed(ctx->ev_start, ctx->ev_ring[slot]);
+    // ||b - K x|| with FP64 values; r_vec, when given, receives b - K x
+    auto true_residual = [&](const double* b_vec, const double* x_vec, double* r_vec) {
+        double r2 = 0.0;
+        lfg::launch_residual_norm(ctx->stream, csr_view(ctx), ctx->Kc.p, x_vec, b_vec, ctx->scratch.p,
+                                  ctx->tickets.p + 4, ctx->scratch.p + 4096, r_vec);
+        LFG_HIP(hipMemcpyAsync(&r2, ctx->scratch.p + 4096, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        sync(ctx);
+        return std::sqrt(std::max(0.0, r2));
+    };
+
+    float pass_ms = 0.0f;
+    lfg::PcgState st = pcg_pass(rhs, xs, opt->stop_rule, opt->tolerance, opt->max_iterations, &pass_ms);
+    double solve_ms = pass_ms;
+    long long iterations = st.iter;
+    const double target = st.target;
+    // FP64 iterative refinement: r = b - K x with FP64 values, a correction solve K d = r
+    // (same precision policy) to ||r_d|| <= max(eta ||r||, target), x += d; until the FP64
+    // residual meets the target, the passes run out, or a pass fails to halve it (the
+    // FP64 residual floor, where the iterate is as accurate as FP64 values make it: a
+    // pass that made it worse is undone).
+    const int passes_allowed = opt->refine_passes > 0 ? opt->refine_passes : (opt->refine_passes == 0 && mixed ? 4 : 0);
+    const double eta = opt->refine_reduction > 0.0 && opt->refine_reduction < 1.0 ? opt->refine_reduction : 1.0e-2;
+    const bool refine = passes_allowed > 0 && st.status == LFG_OK && st.converged;
+    if (refine) {
+        ctx->ir_r.alloc(static_cast<size_t>(ctx->n));
+        ctx->ir_d.alloc(static_cast<size_t>(ctx->n));
+    }
+    double rtrue = true_residual(rhs, xs, refine ? ctx->ir_r.p : nullptr);
+    int passes = 0;
+    bool floor_reached = false;
+    while (refine && rtrue > target && passes < passes_allowed) {
+        const double inner_tol = std::min(0.5, std::max(eta, target / rtrue));
+        const lfg::PcgState sd = pcg_pass(ctx->ir_r.p, ctx->ir_d.p, LFG_STOP_RELATIVE, inner_tol,
+                                          opt->max_iterations, &pass_ms);
+        solve_ms += pass_ms;
+        iterations += sd.iter;
+        ++passes;
+        if (sd.status != LFG_OK || !sd.converged) {  // the correction solve failed: keep x as it is
+            st.status = sd.status != LFG_OK ? sd.status : LFG_ERR_NOT_CONVERGED;
+            st.converged = 0;
             break;
         }
-        ++k;
+        lfg::launch_axpy(ctx->stream, ctx->n, 1.0, ctx->ir_d.p, xs);
+        const double rnew = true_residual(rhs, xs, ctx->ir_r.p);
+        st.rr = sd.rr;
+        if (rnew > 0.5 * rtrue) {
+            floor_reached = true;
+            if (rnew > rtrue) {  // undo a correction that made the FP64 residual worse
+                lfg::launch_axpy(ctx->stream, ctx->n, -1.0, ctx->ir_d.p, xs);
+                sync(ctx);
+                break;
+            }
+            rtrue = rnew;
+            break;
+        }
+        rtrue = rnew;
     }
-    if (stencil_fused) lfg::launch_pcg_fused_flush(ctx->stream, b, ctx->p2.p);  // pb[1] of enqueue_iteration
-    sync(ctx);
-    lfg::PcgState st{};
-    LFG_HIP(hipMemcpy(&st, ctx->state.p, sizeof(st), hipMemcpyDeviceToHost));
-    // FP64 true residual ||b - K u|| for the report
-    double r2 = 0.0;
-    lfg::launch_residual_norm(ctx->stream, csr_view(ctx), ctx->Kc.p, xs, rhs, ctx->scratch.p, ctx->tickets.p + 4,
-                              ctx->scratch.p + 4096);
-    LFG_HIP(hipMemcpyAsync(&r2, ctx->scratch.p + 4096, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (refine && st.status == LFG_OK && rtrue > target && !floor_reached) {
+        st.converged = 0;  // the passes ran out while each still halved the FP64 residual
+    }
     if (x_out) LFG_HIP(hipMemcpyAsync(x_out, xs, ctx->n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
     rep.converged = st.converged;
-    rep.iterations = st.iter;
-    rep.residual_replacements = st.replacements;
+    rep.iterations = iterations;
+    rep.residual_replacements = st.replacements + passes;
     rep.initial_residual = st.rnorm0;
     rep.final_residual = std::sqrt(std::max(0.0, st.rr));
-    rep.true_residual = std::sqrt(std::max(0.0, r2));
+    rep.true_residual = rtrue;
     rep.solve_ms = solve_ms;
+    rep.refine_passes = passes;
     rep.status = st.status != LFG_OK ? st.status : (st.converged ? LFG_OK : LFG_ERR_NOT_CONVERGED);
     if (report) *report = rep;
     if (st.status == LFG_ERR_HIP && kind == LFG_PRECOND_IC0) throw LfgError(st.status, kIc0Fault);

@@ -362,8 +362,11 @@ void launch_batch_jacobi(hipStream_t s, const CsrView& a, const double* values, 
 void launch_batch_scatter_case(hipStream_t s, int n, int n_cases, int c, const double* src, double* dst);
 void launch_batch_gather_case(hipStream_t s, int n, int n_cases, int c, const double* src, double* dst);
 
-// ||b - K x|| in FP64 (reporting).
+// ||b - K x||^2 in FP64 into out2 (reporting, iterative refinement); r_out, when not
+// null, receives r = b - K x.
 void launch_residual_norm(hipStream_t s, const CsrView& a, const double* vals64, const double* x, const double* bvec,
-                          double* partials, TicketSet* tickets, double* out2);
+                          double* partials, TicketSet* tickets, double* out2, double* r_out = nullptr);
+// y += a x
+void launch_axpy(hipStream_t s, long long n, double a, const double* x, double* y);
 
 }  // namespace lfg

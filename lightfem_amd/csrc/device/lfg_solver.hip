@@ -924,8 +924,8 @@ __global__ __launch_bounds__(kThreads) void pcg_direction_kernel(PcgBuffers b) {
     }
 }
 
-// ||b - A x||^2 into out2; blocks own contiguous row ranges, the last block folds
-// the partials.
+// ||b - A x||^2 into out2 and, when r_out is given, r = b - A x into r_out; blocks own
+// contiguous row ranges, the last block folds the partials.
 template <int W>
 __global__ __launch_bounds__(kThreads) void residual_norm_kernel(int n, const int* __restrict__ rp,
                                                                  const int* __restrict__ col,
@@ -933,7 +933,8 @@ __global__ __launch_bounds__(kThreads) void residual_norm_kernel(int n, const in
                                                                  const double* __restrict__ x,
                                                                  const double* __restrict__ bvec,
                                                                  double* __restrict__ partials,
-                                                                 TicketSet* tickets, double* out2) {
+                                                                 TicketSet* tickets, double* out2,
+                                                                 double* __restrict__ r_out) {
     __shared__ double sh[kThreads / kWave];
     constexpr int kRows = kThreads / W;
     const int per_block = (n + gridDim.x - 1) / gridDim.x;
@@ -948,6 +949,7 @@ __global__ __launch_bounds__(kThreads) void residual_norm_kernel(int n, const in
             if (lane == 0) {
                 const double ri = bvec[row] - s;
                 acc += ri * ri;
+                if (r_out) r_out[row] = ri;
             }
         }
     }
@@ -955,6 +957,14 @@ __global__ __launch_bounds__(kThreads) void residual_norm_kernel(int n, const in
     double tot[1];
     if (!finish_reduction<kThreads, 1>(tickets, partials, vals, tot, sh)) return;
     if (threadIdx.x == 0) *out2 = tot[0];
+}
+
+// y += a x (the iterative-refinement correction of the solution).
+__global__ __launch_bounds__(kThreads) void axpy_kernel(long long n, double a, const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+    for (long long i = blockIdx.x * static_cast<long long>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * kThreads)
+        y[i] += a * x[i];
 }
 
 __global__ void mask_constrained_kernel(int n, const int* __restrict__ constrained, const double* __restrict__ in,
@@ -1175,11 +1185,17 @@ void launch_pcg_direction(hipStream_t s, const PcgBuffers& b) {
 }
 
 void launch_residual_norm(hipStream_t s, const CsrView& a, const double* vals64, const double* x, const double* bvec,
-                          double* partials, TicketSet* tickets, double* out2) {
+                          double* partials, TicketSet* tickets, double* out2, double* r_out) {
     constexpr int W = 8;
     const unsigned int need = round8(blocks_for(a.n, kThreads / W));
     const unsigned int grid = need < 2048u ? need : 2048u;
-    residual_norm_kernel<W><<<grid, kThreads, 0, s>>>(a.n, a.row_ptr, a.col, vals64, x, bvec, partials, tickets, out2);
+    residual_norm_kernel<W>
+        <<<grid, kThreads, 0, s>>>(a.n, a.row_ptr, a.col, vals64, x, bvec, partials, tickets, out2, r_out);
+}
+
+void launch_axpy(hipStream_t s, long long n, double a, const double* x, double* y) {
+    const unsigned int need = blocks_for(n, kThreads);
+    axpy_kernel<<<need < 4096u ? (need > 0 ? need : 1u) : 4096u, kThreads, 0, s>>>(n, a, x, y);
 }
 
 }  // namespace lfg

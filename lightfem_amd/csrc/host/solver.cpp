@@ -378,7 +378,9 @@ AnalysisResult LinearElasticSolver::solve(Model& model) const {
         }
         sr.matrix_bytes_per_matvec = blocks * (4 * sizeof(double) + sizeof(uint32_t));
     } else {
-        sr.matrix_bytes_per_matvec = static_cast<std::size_t>(nnz) * ((s.precision == "mixed" ? 4 : 8) + 4);
+        // the report's figure for CSR storage (solver.cpp:683-684, 743-745): a double value
+        // and a size_t column per nonzero, whatever layout the device multiplies with
+        sr.matrix_bytes_per_matvec = static_cast<std::size_t>(nnz) * (sizeof(double) + sizeof(std::size_t));
     }
     if (s.preconditioner == "block") {
         // subdomain_capture_ratio (precond.cpp:107-119): the share of nonzeros whose row

@@ -52,7 +52,8 @@ class PcgOptions(ctypes.Structure):
                 ("check_every", ctypes.c_int32), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int64),
                 ("replace_drop", ctypes.c_double), ("mixed_switch", ctypes.c_double),
                 ("direct_launch", ctypes.c_int32),
-                ("spmv_variant", ctypes.c_int32), ("storage", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("spmv_variant", ctypes.c_int32), ("storage", ctypes.c_int32), ("refine_passes", ctypes.c_int32),
+                ("refine_reduction", ctypes.c_double)]
 
 
 class PcgReport(ctypes.Structure):
@@ -60,7 +61,7 @@ class PcgReport(ctypes.Structure):
                 ("residual_replacements", ctypes.c_int64), ("initial_residual", ctypes.c_double),
                 ("final_residual", ctypes.c_double), ("true_residual", ctypes.c_double),
                 ("solve_ms", ctypes.c_double), ("setup_ms", ctypes.c_double), ("breakdowns", ctypes.c_int64),
-                ("degenerate_blocks", ctypes.c_int64), ("levels", ctypes.c_int64)]
+                ("degenerate_blocks", ctypes.c_int64), ("levels", ctypes.c_int64), ("refine_passes", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -426,10 +427,11 @@ class Device:
     def pcg(self, preconditioner: str = "jacobi", precision: str = "fp64", stop_rule: str = "reference",
             tolerance: float = 1e-10, max_iterations: int = 100000, check_every: int = 0,
             replace_drop: float = 0.0, direct_launch: bool = False, spmv_variant: int = 0,
-            raise_on_error: bool = True, mixed_switch: float = 0.0, storage: str = "csr"):
+            raise_on_error: bool = True, mixed_switch: float = 0.0, storage: str = "csr",
+            refine_passes: int = 0, refine_reduction: float = 0.0):
         opt = PcgOptions(PRECOND[preconditioner], PRECISION[precision], STOP[stop_rule], check_every, tolerance,
                          max_iterations, replace_drop, mixed_switch, int(direct_launch), spmv_variant,
-                         STORAGE[storage], 0)
+                         STORAGE[storage], refine_passes, refine_reduction)
         rep = PcgReport()
         u = np.empty(self.n)
         rc = self._lib.lfg_pcg(self._ctx, ctypes.byref(opt), _ptr(u), ctypes.byref(rep))

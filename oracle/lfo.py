@@ -385,7 +385,7 @@ class DirectSystem:
     serial K and Dirichlet elimination), with iterative refinement. The CPU
     reference for the adjoint gradient; the reference itself has none."""
 
-    def __init__(self, xy_nodes, enode_idx, props, plane_mode, f, cdofs, cvals):
+    def __init__(self, xy_nodes, enode_idx, props, plane_mode, f, cdofs, cvals, factor: bool = True):
         import scipy.sparse
         import scipy.sparse.linalg
         enode_idx = _i64(enode_idx)
@@ -397,7 +397,7 @@ class DirectSystem:
         self.Kc, self.fc = dirichlet(self.row_ptr, self.col, self.K, f, cdofs, cvals)
         self.cdofs = _i64(cdofs)
         self.A = scipy.sparse.csr_matrix((self.Kc, self.col, self.row_ptr), shape=(self.n, self.n))
-        self.lu = scipy.sparse.linalg.splu(self.A.tocsc())
+        self.lu = scipy.sparse.linalg.splu(self.A.tocsc()) if factor else None
 
     def solve(self, rhs, refine: int = 3) -> np.ndarray:
         x = self.lu.solve(rhs)

@@ -10,7 +10,11 @@
 // Modes
 //   dump  <model.fem> <outdir> [key=value ...]   every intermediate of one solve as .npy
 //   bench <nx> <ny> <young.bin> <plane_mode> <nu> <iterations> <threads> <preconditioner>
-//         bounded PCG sample on a structured cantilever, one JSON line on stdout
+//         [<relative_tolerance> <u_out.bin>]
+//         PCG on a structured cantilever, one JSON line on stdout: exactly
+//         <iterations> steps, or with <relative_tolerance> to ||r|| <= tol ||r0|| (the
+//         reference's rule tol' max(1, ||r0||) with tol' = tol ||r0|| / max(1, ||r0||))
+//         capped at <iterations>, the displacements written as raw float64
 #include "src/solver.cpp"  // reference solver TU (anonymous-namespace stages)
 
 #include "lightfem/input_parser.hpp"
@@ -335,10 +339,12 @@ Model structured_model(int nx, int ny, const std::vector<double>& young, PlaneMo
 }
 
 int bench(int argc, char** argv) {
-    if (argc != 10) {
-        std::cerr << "bench <nx> <ny> <young.bin|-> <plane_mode> <nu> <iterations> <threads> <preconditioner>\n";
+    if (argc != 10 && argc != 12) {
+        std::cerr << "bench <nx> <ny> <young.bin|-> <plane_mode> <nu> <iterations> <threads> <preconditioner> "
+                     "[<relative_tolerance> <u_out.bin>]\n";
         return 1;
     }
+    const double rel_tol = argc == 12 ? std::atof(argv[10]) : 0.0;
     const int nx = std::atoi(argv[2]);
     const int ny = std::atoi(argv[3]);
     std::vector<double> young;
@@ -367,8 +373,19 @@ int bench(int argc, char** argv) {
     const auto tb = std::chrono::steady_clock::now();
     apply_dirichlet(kc, fc, collect_constraints(model));
     const double boundary_ms = elapsed_ms(tb);
+    if (rel_tol > 0.0) {
+        double r0 = 0.0;
+        for (double v : fc) r0 += v * v;
+        r0 = std::sqrt(r0);
+        model.settings.tolerance = rel_tol * r0 / std::max(1.0, r0);
+    }
     std::vector<double> u;
     const LinearSolverReport rep = solve_iterative(model, kc, fc, u);
+    if (argc == 12) {
+        std::ofstream out(argv[11], std::ios::binary);
+        out.write(reinterpret_cast<const char*>(u.data()), static_cast<std::streamsize>(u.size() * sizeof(double)));
+        if (!out) throw std::runtime_error("cannot write displacement file");
+    }
     std::cout << std::setprecision(17) << "{\"dofs\": " << kc.rows() << ", \"nnz\": " << kc.nonzeros()
               << ", \"threads_used\": " << assembled.threads_used << ", \"iterations\": " << rep.iterations
               << ", \"pcg_ms\": " << rep.solve_ms << ", \"matvec_ms\": " << rep.matvec_ms
@@ -378,7 +395,8 @@ int bench(int argc, char** argv) {
               << ", \"assembly_ms\": " << assembled.assembly_ms << ", \"pattern_ms\": " << assembled.pattern_ms
               << ", \"setup_ms\": " << setup_ms << ", \"boundary_ms\": " << boundary_ms
               << ", \"initial_residual\": " << rep.initial_residual << ", \"final_residual\": " << rep.final_residual
-              << "}" << std::endl;
+              << ", \"converged\": " << (rep.converged ? "true" : "false") << ", \"tolerance\": "
+              << model.settings.tolerance << "}" << std::endl;
     return 0;
 }
 

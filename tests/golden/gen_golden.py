@@ -9,7 +9,7 @@ and keeps:
 * SHA-256 digests of the exact bytes (int64 / little-endian float64) plus scalar
   results, for the 50x50 and 200x200 configurations (``golden.json``).
 
-Usage: python tests/golden/gen_golden.py
+Usage: python tests/golden/gen_golden.py [--cli-only]
 """
 from __future__ import annotations
 
@@ -250,7 +250,7 @@ def cli_fixtures() -> dict:
             solve = subprocess.run([CLI, "solve", fem, os.path.join(work, f"{name}.vtk")], capture_output=True,
                                    text=True, cwd=work)
             entry = {"fem": text, "check_stdout": check.stdout.replace(fem, "<FEM>"), "check_rc": check.returncode,
-                     "solve_rc": solve.returncode}
+                     "solve_rc": solve.returncode, "solve_stdout": solve.stdout.replace(work, "<DIR>")}
             for table in ("nodes", "gauss_points"):
                 with open(os.path.join(work, f"{name}_{table}.csv")) as handle:
                     entry[f"{table}_csv"] = handle.read()
@@ -260,7 +260,46 @@ def cli_fixtures() -> dict:
     return out
 
 
+# Generated models whose reference `solve` report lines (SOLVED / PROFILE / PARALLEL /
+# ITERATIVE) are kept; their `.fem` text is regenerated by the tests from the generator.
+CLI_REPORT_MODELS = {
+    "cantilever50": (lambda: meshgen.cantilever_50(), []),
+    "cantilever200_rcm_coloring_jacobi": (lambda: meshgen.cantilever_200_rcm(), []),
+    "cantilever200_rcm_atomic_ic0": (lambda: meshgen.cantilever_200_rcm("atomic", "ic0"), []),
+    "cantilever200_rcm_coloring_block64": (lambda: meshgen.cantilever_200_rcm("coloring", "block"), []),
+    "cantilever200_rcm_block1_bsr": (lambda: meshgen.cantilever_200_rcm("coloring", "block"),
+                                     ["subdomain_nodes 1", "storage bsr"]),
+}
+
+
+def cli_report_fixtures() -> dict:
+    """Reference CLI `solve` stdout for the generated 50x50 and 200x200 cantilevers."""
+    out = {}
+    with tempfile.TemporaryDirectory() as work:
+        for name, (make, extra) in CLI_REPORT_MODELS.items():
+            fem = os.path.join(work, f"{name}.fem")
+            make().to_fem(fem)
+            if extra:
+                text = open(fem).read().replace("[settings]\n", "[settings]\n" + "\n".join(extra) + "\n", 1)
+                with open(fem, "w") as handle:
+                    handle.write(text)
+            solve = subprocess.run([CLI, "solve", fem, os.path.join(work, f"{name}.vtk")], capture_output=True,
+                                   text=True, cwd=work, check=True)
+            out[name] = {"extra_settings": extra, "solve_stdout": solve.stdout.replace(work, "<DIR>")}
+    return out
+
+
 def main() -> None:
+    if len(sys.argv) > 1 and sys.argv[1] == "--cli-only":
+        # refresh only the CLI fixtures of an existing golden.json
+        with open(os.path.join(HERE, "golden.json")) as handle:
+            record = json.load(handle)
+        record["cli"] = cli_fixtures()
+        record["cli_reports"] = cli_report_fixtures()
+        with open(os.path.join(HERE, "golden.json"), "w") as handle:
+            json.dump(record, handle, indent=1, sort_keys=True)
+        print("refreshed the CLI fixtures")
+        return
     if not os.path.exists(LFREF):
         raise SystemExit("build the reference first: make -C oracle ref")
     record: dict = {}
@@ -341,6 +380,7 @@ def main() -> None:
         save_digest("cantilever200_bsr_jacobi", arrays, record)
     # CLI fixtures: `check` stdout and `solve` CSV tables of the reference CLI.
     record["cli"] = cli_fixtures()
+    record["cli_reports"] = cli_report_fixtures()
     with open(os.path.join(HERE, "golden.json"), "w") as handle:
         json.dump(record, handle, indent=1, sort_keys=True)
     print("wrote", len(record), "golden cases")
