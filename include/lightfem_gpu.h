@@ -320,6 +320,18 @@ int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, do
 /* y = Kc x with the PCG SpMV kernel of `variant` (values of `precision`; 10 multiplies
  * as 8; variant | LFG_SPMV_BLOCK_ORDER for 6, 7, 9: the block storage order). */
 int lfg_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x, double* y);
+/* Allocation-free form of lfg_spmv, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops): no counterpart in the reference. y_dev =
+ * Kc x_dev for caller-owned device vectors of n doubles, and x.y into the device double
+ * xy_dev when it is not NULL. Allocates nothing: the layout of this precision and variant
+ * must already exist (an lfg_spmv or lfg_pcg with them built it; else LFG_ERR_STATE), and
+ * the in-kernel reduction's state, counters and per-block partials use one 68 KB scratch
+ * array per device that holds partials for up to 8192 blocks (every multiply's grid is
+ * capped at 4096, so no system size exceeds it; a larger grid is refused with
+ * LFG_ERR_UNSUPPORTED). Calls on one device share that array: calls that overlap on one
+ * device (several host threads, or streams of other contexts) race. Synchronous. */
+int lfg_spmv_noalloc(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x_dev, double* y_dev,
+                     double* xy_dev);
 /* Average device time (ms) of `reps` SpMV launches of the given precision and
  * variant on resident data (variant 7: the multiply with the Jacobi direction formed
  * on the fly, as its PCG iteration runs it; variant 10: one whole fused Jacobi PCG

@@ -1085,6 +1085,8 @@ int lfg_create(int32_t device, lfg_ctx** out) {
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return LFG_ERR_HIP;
     if (hipSetDevice(device) != hipSuccess) return LFG_ERR_HIP;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return LFG_ERR_HIP;
+    // this device's address of the allocation-free multiply's scratch array (lfg_spmv_noalloc)
+    if (lfg::noalloc_scratch_init(device) != hipSuccess) return LFG_ERR_HIP;
     if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess ||
         hipEventCreate(&ctx->ev_ring[0]) != hipSuccess || hipEventCreate(&ctx->ev_ring[1]) != hipSuccess)
         return LFG_ERR_HIP;
@@ -1905,6 +1907,68 @@ static void prepare_spmv(lfg_ctx* ctx, int precision, int variant) {
         lfg::launch_to_f32(ctx->stream, ctx->Kc.p, ctx->K32.p, ctx->nnz);
         ctx->k32_valid = true;
     }
+}
+
+// lfg_spmv on caller-owned device vectors without any allocation: the layout must exist
+// (lfg_spmv or lfg_pcg with this precision and variant built it), and the p.q
+// reduction's state, counters and partials live in the device's file-scope scratch
+// array (lfg_noalloc.hip), cleared by a fill kernel at every call.
+int lfg_spmv_noalloc(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x_dev, double* y_dev,
+                     double* xy_dev) {
+    return guarded(ctx, [&] {
+        require(ctx->have_system, LFG_ERR_STATE, "lfg_dirichlet must precede lfg_spmv_noalloc");
+        require(x_dev != nullptr && y_dev != nullptr, LFG_ERR_ARG, "missing device vectors");
+        require(precision == LFG_PRECISION_FP64 || precision == LFG_PRECISION_MIXED, LFG_ERR_ARG,
+                "precision must be fp64 or mixed");
+        LFG_HIP(hipSetDevice(ctx->device));
+        const bool block_order = (variant & LFG_SPMV_BLOCK_ORDER) != 0;
+        const int v = resolve_variant(ctx, variant & ~LFG_SPMV_BLOCK_ORDER, LFG_PRECOND_JACOBI, block_order);
+        const bool f32 = precision == LFG_PRECISION_MIXED;
+        bool ready = false;
+        unsigned int grid = 0;
+        if (stencil_layout(v)) {
+            ready = v == kStencilExact ? (!f32 && ctx->stencil_exact_valid)
+                                       : (f32 ? ctx->stencil32_valid : ctx->stencil64_valid);
+            grid = lfg::stencil_spmv_blocks(ctx->n / 2);
+        } else if (slice_layout(v)) {
+            ready = f32 ? ctx->slices32_valid : ctx->slices64_valid;
+            grid = static_cast<unsigned int>(lfg::slice_spmv_blocks(ctx->n_slices));
+        } else {
+            ready = !f32 || ctx->k32_valid;
+            grid = static_cast<unsigned int>(lfg::spmv_blocks(ctx->n, v));
+        }
+        require(ready, LFG_ERR_STATE,
+                "lfg_spmv_noalloc allocates nothing: run lfg_spmv or lfg_pcg with this precision and variant once "
+                "first, which builds its value layout");
+        require(grid <= static_cast<unsigned int>(lfg::kNoallocMaxGrid), LFG_ERR_UNSUPPORTED,
+                "the multiply's grid exceeds the 8192 block partials of the allocation-free scratch");
+        unsigned char* base = static_cast<unsigned char*>(lfg::noalloc_scratch(ctx->device));
+        require(base != nullptr, LFG_ERR_STATE, "no scratch address for this device");
+        const PairScope pair_scope(ctx, block_order);
+        // solver state and counters start at zero (a fill kernel, no memset node)
+        lfg::launch_fill(ctx->stream, reinterpret_cast<double*>(base),
+                         static_cast<long long>(lfg::kNoallocPartialsAt / sizeof(double)), 0.0);
+        lfg::PcgBuffers b{};
+        b.n = ctx->n;
+        b.p = const_cast<double*>(x_dev);
+        b.q = y_dev;
+        b.precond = LFG_PRECOND_JACOBI;
+        b.st = reinterpret_cast<lfg::PcgState*>(base);
+        b.tickets = reinterpret_cast<lfg::TicketSet*>(base + lfg::kNoallocTicketsAt);
+        b.partials = reinterpret_cast<double*>(base + lfg::kNoallocPartialsAt);
+        if (stencil_layout(v))
+            launch_stencil_spmv(ctx, b, v, !f32, f32);
+        else if (slice_layout(v))
+            lfg::launch_pcg_spmv_slices(ctx->stream, b, slice_view(ctx), f32 ? nullptr : ctx->slices64.p,
+                                        f32 ? ctx->slices32.p : nullptr);
+        else
+            lfg::launch_pcg_spmv(ctx->stream, b, csr_view(ctx), f32 ? nullptr : ctx->Kc.p, f32 ? ctx->K32.p : nullptr,
+                                 v);
+        if (xy_dev)
+            LFG_HIP(hipMemcpyAsync(xy_dev, &b.st->pq, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        sync(ctx);
+        return LFG_OK;
+    });
 }
 
 int lfg_layout_info_get(lfg_ctx* ctx, lfg_layout_info* info) {

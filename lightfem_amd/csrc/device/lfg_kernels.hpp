@@ -362,6 +362,29 @@ void launch_batch_jacobi(hipStream_t s, const CsrView& a, const double* values, 
 void launch_batch_scatter_case(hipStream_t s, int n, int n_cases, int c, const double* src, double* dst);
 void launch_batch_gather_case(hipStream_t s, int n, int n_cases, int c, const double* src, double* dst);
 
+// ---- allocation-free multiply (lfg_noalloc.hip, lfg_spmv_noalloc)
+// One file-scope scratch array per device: the PcgState slot at 0, two TicketSets at
+// kNoallocTicketsAt (the multiplies count on the second), and the per-block partials
+// of the p.q reduction at kNoallocPartialsAt for grids of up to kNoallocMaxGrid blocks.
+constexpr size_t kNoallocTicketsAt = 256;
+constexpr size_t kNoallocPartialsAt = 2560;
+constexpr int kNoallocMaxGrid = 8192;
+constexpr size_t kNoallocScratchBytes = kNoallocPartialsAt + sizeof(double) * (kNoallocMaxGrid + 8);
+static_assert(sizeof(PcgState) <= kNoallocTicketsAt, "solver state slot");
+static_assert(kNoallocTicketsAt + 2 * sizeof(TicketSet) <= kNoallocPartialsAt, "ticket slots");
+// With `device` current: the device address of that device's scratch array (once per
+// device; later calls return at once).
+hipError_t noalloc_scratch_init(int device);
+// The address noalloc_scratch_init recorded for `device`, or null before it ran.
+void* noalloc_scratch(int device);
+// The grid of the multiply launch_stencil_spmv / launch_pcg_spmv_slices / launch_pcg_spmv
+// issue for this layout (the blocks whose partials the reduction folds).
+inline unsigned int stencil_spmv_blocks(int n_nodes) {
+    const unsigned int need = (static_cast<unsigned int>(n_nodes) + 255u) / 256u;
+    const unsigned int r8 = (need + 7u) & ~7u;
+    return r8 < 4096u ? r8 : 4096u;
+}
+
 // ||b - K x||^2 in FP64 into out2 (reporting, iterative refinement); r_out, when not
 // null, receives r = b - K x.
 void launch_residual_norm(hipStream_t s, const CsrView& a, const double* vals64, const double* x, const double* bvec,

@@ -587,8 +587,7 @@ void launch_pack_stencil(hipStream_t s, const CsrView& a, const StencilView& v, 
 
 void launch_pcg_spmv_stencil(hipStream_t s, const PcgBuffers& b, const StencilView& v, const double* planes64,
                              const float* planes32, const double* diag_c) {
-    const unsigned int need = round8(blocks_for(v.n_nodes, kThreads));
-    const unsigned int grid = need < 4096u ? need : 4096u;
+    const unsigned int grid = stencil_spmv_blocks(v.n_nodes);
     if (v.sym) {
         // Q4 meshes numbered row by row have 4 positive offsets; other counts take the
         // general loop

@@ -33,7 +33,7 @@ GPU_SYMBOLS = ["lfg_create", "lfg_destroy", "lfg_last_error", "lfg_status_string
                "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv", "lfg_bench_copy", "lfg_set_materials",
                "lfg_adjoint_solve", "lfg_sensitivity", "lfg_set_material_modes", "lfg_batch_set_problems",
                "lfg_batch_set_loads", "lfg_batch_pcg", "lfg_layout_info_get", "lfg_set_subdomains", "lfg_set_bars",
-               "lfg_pin_zero_pivots"]
+               "lfg_pin_zero_pivots", "lfg_spmv_noalloc"]
 HOST_SYMBOLS = ["lfh_csr_pattern_nnz", "lfh_csr_pattern", "lfh_rcm", "lfh_element_colors", "lfh_csr_profile",
                 "lfh_subdomain_split", "lfh_csr_pattern_nnz_bars", "lfh_csr_pattern_bars", "lfh_rcm_bars",
                 "lfh_element_colors_bars"]
@@ -152,6 +152,7 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lfg_postprocess.argtypes = [_P, _P, _P, _P]
         lib.lfg_precond_apply.argtypes = [_P, ctypes.c_int32, _P, _P]
         lib.lfg_spmv.argtypes = [_P, ctypes.c_int32, ctypes.c_int32, _P, _P]
+        lib.lfg_spmv_noalloc.argtypes = [_P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P]
         lib.lfg_bench_spmv.argtypes = [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.POINTER(ctypes.c_double)]
         lib.lfg_bench_copy.argtypes = [_P, _I64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
@@ -493,6 +494,16 @@ class Device:
         v = variant | (SPMV_BLOCK_ORDER if block_order else 0)
         self._check(self._lib.lfg_spmv(self._ctx, PRECISION[precision], v, _ptr(x), _ptr(y)))
         return y
+
+    def spmv_noalloc(self, x_dev: int, y_dev: int, xy_dev: int = 0, precision: str = "fp64", variant: int = 0,
+                     raise_on_error: bool = True) -> int:
+        """y_dev = Kc x_dev (and x.y into xy_dev) on caller-owned device pointers, without any
+        device allocation (lfg_spmv_noalloc); returns the status."""
+        rc = self._lib.lfg_spmv_noalloc(self._ctx, PRECISION[precision], variant, ctypes.c_void_p(x_dev),
+                                        ctypes.c_void_p(y_dev), ctypes.c_void_p(xy_dev or None))
+        if rc != OK and raise_on_error:
+            self._check(rc)
+        return rc
 
     def set_subdomains(self, node_subdomain, n_subdomains: int) -> None:
         """Subdomains of the block preconditioner (LFG_PRECOND_SUBDOMAIN), per node index."""
