@@ -39,7 +39,7 @@ KERNELS = {1: "pcg_spmv_kernel<4>", 2: "pcg_spmv_kernel<8>", 3: "pcg_spmv_kernel
            4: "pcg_spmv_stream_kernel<24, 64>", 5: "pcg_spmv_stream_kernel<24, 32>",
            6: "pcg_spmv_slices_kernel<9>", 7: "pcg_spmv_slices_dir_kernel<9, 0>",
            8: "pcg_spmv_stencil_kernel<4, true>", 9: "pcg_spmv_stencil_kernel<9, false>",
-           10: "pcg_fused_stencil_kernel<4, 0>"}
+           10: "pcg_fused_stencil_kernel<4, 0>", 11: "pcg_fused_grid_kernel<0>"}
 
 
 def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int, layout: dict) -> int:
@@ -52,7 +52,7 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int, layout: dict) -
     no columns: one 2x2 block per node and plane (the symmetric form: the diagonal
     plane and one plane per positive offset, its FP32 copy with an FP64 diagonal)."""
     n_nodes = n // 2
-    if variant == 10:
+    if variant in (10, 11):
         # the whole Jacobi iteration: the symmetric planes, and per node r, q, p and the
         # Jacobi inverse read (4 x 16 B), r, p, q written (3 x 16 B), and x updated every
         # second launch for two iterations (x and the older p read, x written: 48 B per
@@ -356,7 +356,8 @@ def main() -> int:
     dev.dirichlet(sys_.indexed.cdofs, sys_.indexed.cvals)
     layout = dev.layout_info()
     # the library's default Jacobi iteration: fused stencil, stencil multiply, node slices
-    default_variant = 10 if layout["fused_window_pairs"] else (8 if layout["stencil_sym_planes"] else 6)
+    default_variant = 11 if layout["grid_row_len"] else (
+        10 if layout["fused_window_pairs"] else (8 if layout["stencil_sym_planes"] else 6))
     if args.profile:
         # Profiling run: the FP64 SpMV of the headline variant alone (args.profile launches),
         # then FP64 Jacobi-PCG iterations (a whole number of captured graphs), no other kernels.
@@ -385,10 +386,11 @@ def main() -> int:
     spmv, sweep, iteration_kernel = {}, {}, {}
     for precision, vbytes in (("fp64", 8), ("mixed", 4)):
         sweep[precision] = {}
-        for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10):
+        for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11):
             if (variant == 8 and not layout["stencil_sym_planes"]) or (variant == 9 and (
                     precision == "mixed" or not layout["stencil_exact_planes"])) or (
-                    variant == 10 and not layout["fused_window_pairs"]):
+                    variant == 10 and not layout["fused_window_pairs"]) or (
+                    variant == 11 and not layout["grid_row_len"]):
                 continue
             ms = dev.bench_spmv(precision, variant, 50)
             b = spmv_bytes(n, nnz, vbytes, variant, layout)
@@ -396,7 +398,7 @@ def main() -> int:
                                          "GBs": b / (ms * 1e-3) / 1e9}
         # the multiply alone (the stand-alone kernel of the solve's layout) and the kernel the
         # PCG iteration spends its time in (the same, or the fused iteration)
-        multiply = 8 if solve_variant == 10 else solve_variant
+        multiply = 8 if solve_variant in (10, 11) else solve_variant
         spmv[precision] = dict(sweep[precision][multiply], variant=multiply)
         iteration_kernel[precision] = dict(sweep[precision][solve_variant], variant=solve_variant)
     print(f"bench: spmv {json.dumps(spmv)}", file=sys.stderr, flush=True)

@@ -89,7 +89,11 @@ typedef struct {
                                 to 4 and 6, FP64 only); 10: the layout of 8 with the whole Jacobi /
                                 block iteration in one launch (update, search direction, multiply
                                 and all sums; beta from the next residual's r.z recurrence, so one
-                                reduction per iteration) */
+                                reduction per iteration); 11: the iteration of 10 on 2D tiles of 4 mesh
+                                rows by 64 nodes, for symmetric stencils with positive offsets 1, L-1,
+                                L, L+1 (a Q4 mesh numbered row by row, L >= 64 nodes per row): a
+                                140-position halo per 256-node tile instead of 518, every load issued
+                                before the first use (the default where it applies) */
     int32_t storage;         /* LFG_STORAGE_*: BSR sums each 2x2 block's products first (variants 6, 7
                                 and 9 follow it bit for bit; 8 and 10 read the transposed blocks as
                                 always; 1-5 are CSR only). Needs a node-blocked pattern. */
@@ -305,6 +309,7 @@ typedef struct {
     int32_t stencil_sym_planes;    /* variant 8: 0 and the positive offsets (0 = not available) */
     int32_t fused_window_pairs;    /* variant 10: LDS pairs per 256-node tile (0 = not available) */
     int32_t stencil_offsets[17];   /* the exact form's offsets, ascending */
+    int32_t grid_row_len;          /* variant 11: nodes per mesh row L (0 = not available) */
 } lfg_layout_info;
 int lfg_layout_info_get(lfg_ctx* ctx, lfg_layout_info* info);
 

@@ -253,11 +253,15 @@ constexpr int kSliceFused = 7;    // node-slice SpMV forming the Jacobi / block 
 constexpr int kStencilSym = 8;    // node-stencil planes, symmetric form (diagonal + positive offsets)
 constexpr int kStencilExact = 9;  // node-stencil planes, every offset (CSR column order)
 constexpr int kStencilFused = 10; // symmetric node stencil, whole Jacobi / block PCG iteration in one launch
+constexpr int kStencilGrid = 11;  // the iteration of 10 on 2D tiles (row-structured stencils)
 
 bool slice_layout(int variant) { return variant == kSliceVariant || variant == kSliceFused; }
 bool stencil_layout(int variant) {
-    return variant == kStencilSym || variant == kStencilExact || variant == kStencilFused;
+    return variant == kStencilSym || variant == kStencilExact || variant == kStencilFused || variant == kStencilGrid;
 }
+// The whole Jacobi / block iteration in one launch (1D or 2D tiles).
+bool fused_iteration(int variant) { return variant == kStencilFused || variant == kStencilGrid; }
+bool grid_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stencil_sym.row_len > 0; }
 bool fused_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stencil_sym.win_total > 0; }
 
 // 0 selects, in order: the fused stencil iteration (Jacobi and block preconditioners),
@@ -268,19 +272,27 @@ bool sweep_precond(int precond) { return precond == LFG_PRECOND_IC0 || precond =
 
 int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JACOBI, bool block_order = false) {
     if (block_order) {
-        const int v = variant == 0 ? (fused_stencil_ok(c) && !sweep_precond(precond)
-                                          ? kStencilFused
-                                          : (c->have_stencil_sym ? kStencilSym : kSliceVariant))
+        const int v = variant == 0 ? (!sweep_precond(precond) && grid_stencil_ok(c)
+                                          ? kStencilGrid
+                                          : (fused_stencil_ok(c) && !sweep_precond(precond)
+                                                 ? kStencilFused
+                                                 : (c->have_stencil_sym ? kStencilSym : kSliceVariant)))
                                    : variant;
-        require(v >= kSliceVariant && v <= kStencilFused, LFG_ERR_UNSUPPORTED,
+        require(v >= kSliceVariant && v <= kStencilGrid, LFG_ERR_UNSUPPORTED,
                 "block storage runs on the node-slice (6, 7) and node-stencil (8-10) SpMVs");
         return resolve_variant(c, v, precond, false);
     }
     if (variant == 0) {
+        if (grid_stencil_ok(c) && !sweep_precond(precond)) return kStencilGrid;
         if (fused_stencil_ok(c) && !sweep_precond(precond)) return kStencilFused;
         return c->have_stencil_sym ? kStencilSym : (c->have_slices ? kSliceVariant : 4);
     }
-    require(variant >= 1 && variant <= kStencilFused, LFG_ERR_ARG, "SpMV variant must be 0-10");
+    require(variant >= 1 && variant <= kStencilGrid, LFG_ERR_ARG, "SpMV variant must be 0-11");
+    require(variant != kStencilGrid || grid_stencil_ok(c), LFG_ERR_UNSUPPORTED,
+            "the 2D-tiled fused iteration needs a symmetric node stencil with positive offsets 1, L-1, L, L+1 "
+            "(L >= 64 nodes per mesh row)");
+    require(variant != kStencilGrid || !sweep_precond(precond), LFG_ERR_UNSUPPORTED,
+            "the fused stencil iteration applies the jacobi and 2x2 block preconditioners, not ic0 or subdomains");
     require(variant != kStencilFused || fused_stencil_ok(c), LFG_ERR_UNSUPPORTED,
             "the fused stencil iteration needs the symmetric node-stencil layout with offset windows that fit "
             "in LDS");
@@ -439,13 +451,16 @@ void build_stencil(lfg_ctx* c) {
     sy.off[0] = 0;
     for (size_t k = 0; k < pos.size(); ++k) sy.off[k + 1] = pos[k];
     plan_fused_windows(sy);
+    if (sy.n_off == 5 && sy.off[1] == 1 && sy.off[3] >= 64 && sy.off[2] == sy.off[3] - 1 &&
+        sy.off[4] == sy.off[3] + 1)
+        sy.row_len = sy.off[3];
     c->stencil_sym = sy;
     c->have_stencil_sym = true;
 }
 
 // Packs the constrained values into the stencil planes that the next SpMVs read.
 void ensure_stencil(lfg_ctx* c, int variant, bool need64, bool need32) {
-    if (variant == kStencilFused) variant = kStencilSym;
+    if (fused_iteration(variant)) variant = kStencilSym;
     const size_t nn = 64 * static_cast<size_t>((c->n / 2 + 63) / 64);  // whole 64-node chunks
     if (variant == kStencilExact) {
         if (!c->stencil_exact_valid) {
@@ -473,7 +488,7 @@ void ensure_stencil(lfg_ctx* c, int variant, bool need64, bool need32) {
 // The multiply of one PCG iteration for a stencil variant (q = A p, p.q, alpha). With
 // mixed, the symmetric form switches to its FP32 copy in the FP32 phase.
 void launch_stencil_spmv(lfg_ctx* c, const lfg::PcgBuffers& b, int variant, bool fp64, bool fp32) {
-    if (variant == kStencilFused) variant = kStencilSym;  // the plain multiply of the same planes
+    if (fused_iteration(variant)) variant = kStencilSym;  // the plain multiply of the same planes
     if (variant == kStencilExact) {
         lfg::StencilView ex = c->stencil_exact;
         ex.pair_sum = c->pair_sum;
@@ -1018,13 +1033,14 @@ bool fused_direction(int variant, int precond) { return variant == kSliceFused &
 // buffer parity (b.p or c->p2) and writes the new p into the other one.
 void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int replace, int variant, int parity) {
     const lfg::CsrView a = csr_view(c);
-    if (variant == kStencilFused) {
+    if (fused_iteration(variant)) {
         double* const rb[2] = {c->r.p, c->r_alt.p};
         double* const pb[2] = {c->p.p, c->p2.p};
         double* const qb[2] = {c->q.p, c->q_alt.p};
-        lfg::launch_pcg_fused_stencil(c->stream, b, c->stencil_sym, c->stencil64.p, mixed ? c->stencil32.p : nullptr,
-                                      mixed ? c->stencil_diag.p : nullptr, rb[parity], rb[parity ^ 1], pb[parity],
-                                      pb[parity ^ 1], qb[parity], qb[parity ^ 1]);
+        (variant == kStencilGrid ? lfg::launch_pcg_fused_grid : lfg::launch_pcg_fused_stencil)(
+            c->stream, b, c->stencil_sym, c->stencil64.p, mixed ? c->stencil32.p : nullptr,
+            mixed ? c->stencil_diag.p : nullptr, rb[parity], rb[parity ^ 1], pb[parity], pb[parity ^ 1], qb[parity],
+            qb[parity ^ 1]);
         return;
     }
     if (fused_direction(variant, b.precond)) {
@@ -1573,7 +1589,7 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
         rep.levels = ctx->ic0.fwd_levels + ctx->ic0.bwd_levels;
     }
 
-    const bool stencil_fused = variant == kStencilFused;
+    const bool stencil_fused = fused_iteration(variant);
     const bool fused = fused_direction(variant, kind) || stencil_fused;
     if (fused) ctx->p2.alloc(static_cast<size_t>(ctx->n));
     if (stencil_fused) {
@@ -1981,6 +1997,7 @@ int lfg_layout_info_get(lfg_ctx* ctx, lfg_layout_info* info) {
         out.stencil_sym_planes = ctx->have_stencil_sym ? ctx->stencil_sym.n_off : 0;
         out.fused_window_pairs = ctx->have_stencil_sym ? ctx->stencil_sym.win_total : 0;
         for (int k = 0; k < out.stencil_exact_planes; ++k) out.stencil_offsets[k] = ctx->stencil_exact.off[k];
+        out.grid_row_len = ctx->have_stencil_sym ? ctx->stencil_sym.row_len : 0;
         *info = out;
         return LFG_OK;
     });
@@ -2010,7 +2027,7 @@ int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t rep
         const bool block_order = (variant & LFG_SPMV_BLOCK_ORDER) != 0;
         const int v = resolve_variant(ctx, variant & ~LFG_SPMV_BLOCK_ORDER, LFG_PRECOND_JACOBI, block_order);
         const PairScope pair_scope(ctx, block_order);
-        if (v == kStencilFused) {
+        if (fused_iteration(v)) {
             // whole Jacobi PCG iterations (the fused kernel does the multiply and the vector
             // work), from x = 0 with a stop threshold no residual reaches
             long long degenerate = 0;

@@ -95,8 +95,13 @@ struct StencilView {
     int slot_zero;          // LDS offset (plus the node's tile position t) of node i = i0 + t
     int slot_up[kMaxStencil];  // ... of i + off[k]
     int slot_lo[kMaxStencil];  // ... of i - off[k]
+    // Symmetric form whose positive offsets are exactly 1, L-1, L, L+1 (a Q4 mesh numbered
+    // row by row with L nodes per row, L >= 64): row_len = L, and the fused iteration can
+    // tile the mesh in two dimensions (variant 11); else 0.
+    int row_len;
 };
 constexpr int kFusedTile = 256;         // nodes per tile (one per thread)
+constexpr int kGridRows = 4;            // variant 11: mesh rows per tile (one wavefront each, 64 nodes)
 constexpr int kFusedMaxWindow = 2048;   // LDS pairs of all windows of a tile
 
 // ---- element kernels (lfg_element.hip)
@@ -216,6 +221,11 @@ void launch_pcg_fused_stencil(hipStream_t s, const PcgBuffers& b, const StencilV
 // After the last fused launch: the x term of an odd final launch (p_odd = its p_in,
 // the buffer odd launches read); nothing after an even one.
 void launch_pcg_fused_flush(hipStream_t s, const PcgBuffers& b, const double* p_odd);
+// Variant 11: the iteration of launch_pcg_fused_stencil on 2D tiles of kGridRows mesh rows
+// by 64 nodes (needs v.row_len > 0); same buffers, same state updates.
+void launch_pcg_fused_grid(hipStream_t s, const PcgBuffers& b, const StencilView& v, const double* planes64,
+                           const float* planes32, const double* diag_c, const double* r_in, double* r_out,
+                           const double* p_in, double* p_out, const double* q_in, double* q_out);
 // x += alpha p, r -= alpha q, ||r||^2, r.z; convergence / replacement decision, beta.
 void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed);
 // mixed only: r = b - K x in FP64 when a replacement is due; then r.z and beta.

@@ -62,6 +62,37 @@ __device__ __forceinline__ long long plane_at(int i, int k, int h, int np) {
     return ((static_cast<long long>(i >> 6) * np + k) * 2 + h) * kWave + (i & (kWave - 1));
 }
 
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// FP32 planes hold one 16-byte record per node and plane, the upper pair then the lower
+// pair, stored per 64-node chunk plane after plane: one load brings a node's block and a
+// wavefront's load walks 64 consecutive records. Record index of plane k, node i:
+__device__ __forceinline__ long long plane32_at(int i, int k, int np) {
+    return (static_cast<long long>(i >> 6) * np + k) * kWave + (i & (kWave - 1));
+}
+
+// Upper and lower pairs of plane k at node i (FP64: two 16-byte loads; FP32: one);
+// kNT marks the load as the data's last use.
+template <bool kNT>
+__device__ __forceinline__ void plane_block(const double* planes, int i, int k, int np, dpair& u, dpair& l) {
+    const dpair* pl = reinterpret_cast<const dpair*>(planes);
+    if (kNT) {
+        u = __builtin_nontemporal_load(pl + plane_at(i, k, 0, np));
+        l = __builtin_nontemporal_load(pl + plane_at(i, k, 1, np));
+    } else {
+        u = pl[plane_at(i, k, 0, np)];
+        l = pl[plane_at(i, k, 1, np)];
+    }
+}
+
+template <bool kNT>
+__device__ __forceinline__ void plane_block(const float* planes, int i, int k, int np, fpair& u, fpair& l) {
+    const f4* pl = reinterpret_cast<const f4*>(planes);
+    const f4 v = kNT ? __builtin_nontemporal_load(pl + plane32_at(i, k, np)) : pl[plane32_at(i, k, np)];
+    u = fpair{v.x, v.y};
+    l = fpair{v.z, v.w};
+}
+
 // One thread per node: the node's block in every plane; with diag_c, the diagonal
 // block minus the rounding error (FP32 value - FP64 value) of each off-diagonal
 // block the FP32 multiply uses for this node's rows, in the row's column order:
@@ -84,11 +115,8 @@ __global__ __launch_bounds__(256) void pack_stencil_kernel(int n_nodes, const in
             o[plane_at(i, k, 0, np64)] = dpair{blk.a, blk.b};
             o[plane_at(i, k, 1, np64)] = dpair{blk.c, blk.d};
         }
-        if (out32 && k >= 1) {
-            fpair* o = reinterpret_cast<fpair*>(out32);
-            o[plane_at(i, k - 1, 0, np32)] = fpair{f32(blk.a), f32(blk.b)};
-            o[plane_at(i, k - 1, 1, np32)] = fpair{f32(blk.c), f32(blk.d)};
-        }
+        if (out32 && k >= 1)
+            reinterpret_cast<f4*>(out32)[plane32_at(i, k - 1, np32)] = f4{f32(blk.a), f32(blk.b), f32(blk.c), f32(blk.d)};
     }
     if (!diag_c) return;
     Block d{0.0, 0.0, 0.0, 0.0};
@@ -154,7 +182,6 @@ __device__ __forceinline__ void stencil_sym_node(const StencilView& v, const dpa
                                                  const dpair* __restrict__ p2, int i, double& a0, double& a1,
                                                  const dpair* win, int t) {
     using P = typename PairOf<VT>::type;
-    const P* __restrict__ pl = reinterpret_cast<const P*>(planes);
     const dpair du = __builtin_nontemporal_load(diag + plane_at(i, 0, 0, np_d));
     const dpair dl = __builtin_nontemporal_load(diag + plane_at(i, 0, 1, np_d));
     const dpair xi = kLds ? win[v.slot_zero + t] : p2[i];
@@ -172,10 +199,8 @@ __device__ __forceinline__ void stencil_sym_node(const StencilView& v, const dpa
             okl[k] = jl >= 0;
             const int jlc = okl[k] ? jl : i;
             const int ju = min(i + d, v.n_nodes - 1);
-            uu[k] = pl[plane_at(i, kb + k, 0, np)];
-            ul[k] = pl[plane_at(i, kb + k, 1, np)];
-            lu[k] = __builtin_nontemporal_load(pl + plane_at(jlc, kb + k, 0, np));
-            ll[k] = __builtin_nontemporal_load(pl + plane_at(jlc, kb + k, 1, np));
+            plane_block<false>(planes, i, kb + k, np, uu[k], ul[k]);
+            plane_block<true>(planes, jlc, kb + k, np, lu[k], ll[k]);
             xl[k] = kLds ? win[v.slot_lo[k + 1] + t] : p2[jlc];
             xu[k] = kLds ? win[v.slot_up[k + 1] + t] : p2[ju];
         }
@@ -191,13 +216,17 @@ __device__ __forceinline__ void stencil_sym_node(const StencilView& v, const dpa
             const int jl = i - v.off[k + 1];
             if (jl < 0) continue;
             const dpair x = kLds ? win[v.slot_lo[k + 1] + t] : p2[jl];
-            add_lower(pl[plane_at(jl, kb + k, 0, np)], pl[plane_at(jl, kb + k, 1, np)], x, a0, a1);
+            P bu, bl;
+            plane_block<false>(planes, jl, kb + k, np, bu, bl);
+            add_lower(bu, bl, x, a0, a1);
         }
         add_upper(du, dl, xi, a0, a1);
         for (int k = 0; k < m; ++k) {
             const int ju = min(i + v.off[k + 1], v.n_nodes - 1);
             const dpair x = kLds ? win[v.slot_up[k + 1] + t] : p2[ju];
-            add_upper(pl[plane_at(i, kb + k, 0, np)], pl[plane_at(i, kb + k, 1, np)], x, a0, a1);
+            P bu, bl;
+            plane_block<false>(planes, i, kb + k, np, bu, bl);
+            add_upper(bu, bl, x, a0, a1);
         }
     }
 }
@@ -333,17 +362,13 @@ template <typename VT, int kM>
 __device__ __forceinline__ void load_sym_rows(const StencilView& v, const dpair* __restrict__ diag, int np_d,
                                               const VT* __restrict__ planes, int np, int kb, int i,
                                               SymRows<VT, kM>& a) {
-    using P = typename PairOf<VT>::type;
-    const P* __restrict__ pl = reinterpret_cast<const P*>(planes);
 #pragma unroll
     for (int k = 0; k < kM; ++k) {
         const int jl = i - v.off[k + 1];
         a.okl[k] = jl >= 0;
         const int jlc = a.okl[k] ? jl : i;
-        a.uu[k] = pl[plane_at(i, kb + k, 0, np)];
-        a.ul[k] = pl[plane_at(i, kb + k, 1, np)];
-        a.lu[k] = __builtin_nontemporal_load(pl + plane_at(jlc, kb + k, 0, np));
-        a.ll[k] = __builtin_nontemporal_load(pl + plane_at(jlc, kb + k, 1, np));
+        plane_block<false>(planes, i, kb + k, np, a.uu[k], a.ul[k]);
+        plane_block<true>(planes, jlc, kb + k, np, a.lu[k], a.ll[k]);
     }
     a.du = __builtin_nontemporal_load(diag + plane_at(i, 0, 0, np_d));
     a.dl = __builtin_nontemporal_load(diag + plane_at(i, 0, 1, np_d));
@@ -471,6 +496,45 @@ __device__ __forceinline__ void fused_tiles(const PcgBuffers& b, const StencilVi
     }
 }
 
+// The last block's state update of a fused iteration launch (tot: r'.r', r'.z, p'.q',
+// q'.z, q'.M^-1 q'; alpha: the alpha the launch applied): the reference's stop test on r'
+// (solver.cpp:606-611), its breakdown checks, alpha' = r'.z / p'.q' and beta' from the
+// r.z recurrence of the next residual.
+__device__ __forceinline__ void fused_state_update(PcgState* st, const double (&tot)[5], double alpha) {
+    const double rr = tot[0], rho = tot[1], pq = tot[2], qz = tot[3], qmq = tot[4];
+    st->iter += 1;  // -1 -> 0 on the first launch, which only forms p0 and A p0
+    st->rr = rr;
+    if (st->iter > 0 && sqrt(max0(rr)) <= st->target) {  // solver.cpp:606-611
+        st->converged = 1;
+        st->done = 1;
+        return;
+    }
+    if (!(rho > 0.0) || !isfinite(rho)) {  // solver.cpp:579-581, 614-618
+        st->status = LFG_ERR_NONFINITE;
+        st->done = 1;
+        return;
+    }
+    if (st->iter >= st->max_iter) {
+        st->done = 1;
+        return;
+    }
+    if (!(pq > 0.0) || !isfinite(pq)) {  // solver.cpp:591-593 of the next iteration
+        st->status = LFG_ERR_NOT_SPD;
+        st->done = 1;
+        return;
+    }
+    const double a = rho / pq;
+    // r''.M^-1 r'' for r'' = r' - a q' (M^-1 symmetric); a cancellation below zero
+    // means r'' is negligible next to r', where beta = 0 is the exact limit
+    const double rho_next = rho - 2.0 * a * qz + a * a * qmq;
+    st->pq = pq;
+    st->rho = rho;
+    st->alpha_prev = alpha;
+    st->alpha = a;
+    st->beta = max0(rho_next) / rho;
+    if (!st->phase32 && rr <= st->switch_rr) st->phase32 = 1;  // mixed: FP32 values from here on
+}
+
 // One PCG iteration per launch on the symmetric stencil form (launch_pcg_fused_stencil).
 // Each block walks a contiguous range of 256-node tiles (XCD-remapped). Per tile:
 //   1. every thread updates its own node: x += alpha p, r' = r - alpha q, z = M^-1 r',
@@ -515,41 +579,174 @@ __global__ __launch_bounds__(kFusedTile) void pcg_fused_stencil_kernel(
     block_sums<kFusedTile, 5>(acc, sh);
     double tot[5];
     if (!finish_reduction<kFusedTile, 5>(b.tickets + 1, b.partials, acc, tot, sh)) return;
-    if (threadIdx.x == 0) {
-        PcgState* st = b.st;
-        const double rr = tot[0], rho = tot[1], pq = tot[2], qz = tot[3], qmq = tot[4];
-        st->iter += 1;  // -1 -> 0 on the first launch, which only forms p0 and A p0
-        st->rr = rr;
-        if (st->iter > 0 && sqrt(max0(rr)) <= st->target) {  // solver.cpp:606-611
-            st->converged = 1;
-            st->done = 1;
-            return;
+    if (threadIdx.x == 0) fused_state_update(b.st, tot, alpha);
+}
+
+// ---- the fused iteration on 2D tiles (variant 11)
+//
+// A row-structured symmetric stencil (positive offsets 1, L-1, L, L+1) couples node i with
+// i +- 1 in its mesh row and i +- L + {-1, 0, 1} in the rows beside it. A tile of kGridRows
+// row segments of 64 nodes (node i = i0 + r L + l: wavefront r, lane l) then needs p' on a
+// window of (kGridRows + 2) x 66 positions: its own 256 nodes and a ring of 140 around them,
+// against two whole 258-node windows of the 1D tile's neighbouring rows (518 positions).
+// Every load of a tile is issued before any is used: the own r, q, p, M^-1 (and x, older p),
+// the one halo position of the first 140 threads, and the node's 18 plane pairs (own upper
+// blocks, the transposed lower ones, the diagonal), in that order, so the vector phase waits
+// only for its own loads while the planes stream in.
+constexpr int kGridWinCols = kWave + 2;
+constexpr int kGridWin = (kGridRows + 2) * kGridWinCols;
+constexpr int kGridHalo = kGridWin - kGridRows * kWave;  // 2 * 66 + 2 * kGridRows
+
+// Window slot of (row r, column l), r in [-1, kGridRows], l in [-1, 64]; the offsets
+// 1, L-1, L, L+1 move the slot by 1, 65, 66, 67.
+__device__ __forceinline__ int grid_slot(int r, int l) { return (r + 1) * kGridWinCols + l + 1; }
+
+// One tile of the 2D iteration (values of type VT); adds the tile's sums to acc.
+template <typename VT, int Prec>
+__device__ __forceinline__ void grid_tile(const PcgBuffers& b, const StencilView& v, const VT* __restrict__ planes,
+                                          const dpair* __restrict__ diag, int np_d, int np, int kb,
+                                          const double* __restrict__ r_in, double* __restrict__ r_out,
+                                          const double* __restrict__ p_in, double* __restrict__ p_out,
+                                          const double* __restrict__ q_in, double* __restrict__ q_out, double alpha,
+                                          double beta, double alpha_prev, int x_mode, int tile, dpair* win,
+                                          double (&acc)[5]) {
+    const int n = v.n_nodes, L = v.row_len;
+    const int col_tiles = (L + kWave - 1) / kWave;
+    const int row_tiles = ((n + L - 1) / L + kGridRows - 1) / kGridRows;
+    if (tile < row_tiles * col_tiles) {
+        const int rt = tile / col_tiles;
+        const int c0 = (tile - rt * col_tiles) * kWave;
+        const int r = static_cast<int>(threadIdx.x) / kWave, l = static_cast<int>(threadIdx.x) & (kWave - 1);
+        const int i0 = rt * kGridRows * L + c0;
+        const int i = i0 + r * L + l;
+        const bool own = c0 + l < L && i < n;
+        const int ic = own ? i : 0;
+        const dpair* __restrict__ r2 = reinterpret_cast<const dpair*>(r_in);
+        const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(p_in);
+        const dpair* __restrict__ q2 = reinterpret_cast<const dpair*>(q_in);
+        dpair* __restrict__ x2 = reinterpret_cast<dpair*>(b.x);
+        // own vectors
+        const dpair ro = r2[ic], qo = q2[ic], po = p2[ic];
+        NodePrecond<Prec> mo;
+        mo.load(b.m, ic);
+        dpair xo = {0.0, 0.0}, pp = {0.0, 0.0};
+        if (x_mode != 0) {
+            xo = x2[ic];
+            if (x_mode == 2) pp = reinterpret_cast<const dpair*>(p_out)[ic];
         }
-        if (!(rho > 0.0) || !isfinite(rho)) {  // solver.cpp:579-581, 614-618
-            st->status = LFG_ERR_NONFINITE;
-            st->done = 1;
-            return;
+        // halo position of this thread: row -1, row kGridRows, then columns -1 and 64
+        const int h = static_cast<int>(threadIdx.x);
+        int hr = 0, hc = 0, hg = -1;
+        if (h < kGridHalo) {
+            if (h < kGridWinCols) {
+                hr = -1;
+                hc = h - 1;
+            } else if (h < 2 * kGridWinCols) {
+                hr = kGridRows;
+                hc = h - kGridWinCols - 1;
+            } else {
+                hr = (h - 2 * kGridWinCols) >> 1;
+                hc = ((h - 2 * kGridWinCols) & 1) ? kWave : -1;
+            }
+            const int g = i0 + hr * L + hc;
+            hg = (g >= 0 && g < n) ? g : -1;
         }
-        if (st->iter >= st->max_iter) {
-            st->done = 1;
-            return;
+        const int hgc = hg >= 0 ? hg : 0;
+        dpair hrv = {0.0, 0.0}, hqv = {0.0, 0.0}, hpv = {0.0, 0.0};
+        NodePrecond<Prec> hm;
+        if (h < kGridHalo) {
+            hrv = r2[hgc];
+            hqv = q2[hgc];
+            hpv = p2[hgc];
+            hm.load(b.m, hgc);
         }
-        if (!(pq > 0.0) || !isfinite(pq)) {  // solver.cpp:591-593 of the next iteration
-            st->status = LFG_ERR_NOT_SPD;
-            st->done = 1;
-            return;
+        // the node's planes
+        SymRows<VT, 4> rows;
+        load_sym_rows<VT, 4>(v, diag, np_d, planes, np, kb, ic, rows);
+        // 1. own node: x += alpha p (deferred), r' = r - alpha q, z = M^-1 r', p' = z + beta p
+        dpair pn = {0.0, 0.0}, zn = {0.0, 0.0};
+        if (own) {
+            if (x_mode != 0) {  // solver.cpp:599-601, two iterations' terms in their order
+                if (x_mode == 2) {
+                    xo.x += alpha_prev * pp.x;
+                    xo.y += alpha_prev * pp.y;
+                }
+                xo.x += alpha * po.x;
+                xo.y += alpha * po.y;
+                x2[i] = xo;
+            }
+            const dpair rn = {ro.x - alpha * qo.x, ro.y - alpha * qo.y};
+            double z0, z1;
+            mo.apply(rn.x, rn.y, z0, z1);
+            zn = dpair{z0, z1};
+            pn = dpair{z0 + beta * po.x, z1 + beta * po.y};
+            reinterpret_cast<dpair*>(r_out)[i] = rn;
+            reinterpret_cast<dpair*>(p_out)[i] = pn;
+            acc[0] += rn.x * rn.x + rn.y * rn.y;
+            acc[1] += rn.x * z0 + rn.y * z1;
         }
-        const double a = rho / pq;
-        // r''.M^-1 r'' for r'' = r' - a q' (M^-1 symmetric); a cancellation below zero
-        // means r'' is negligible next to r', where beta = 0 is the exact limit
-        const double rho_next = rho - 2.0 * a * qz + a * a * qmq;
-        st->pq = pq;
-        st->rho = rho;
-        st->alpha_prev = alpha;
-        st->alpha = a;
-        st->beta = max0(rho_next) / rho;
-        if (!st->phase32 && rr <= st->switch_rr) st->phase32 = 1;  // mixed: FP32 values from here on
+        win[grid_slot(r, l)] = pn;
+        // 2. the halo ring
+        if (h < kGridHalo) {
+            dpair pj = {0.0, 0.0};
+            if (hg >= 0) {
+                double z0, z1;
+                hm.apply(hrv.x - alpha * hqv.x, hrv.y - alpha * hqv.y, z0, z1);
+                pj = dpair{z0 + beta * hpv.x, z1 + beta * hpv.y};
+            }
+            win[grid_slot(hr, hc)] = pj;
+        }
+        __syncthreads();
+        // 3. q' = A p' in column order (as stencil_sym_node): transposed lower blocks from
+        // the farthest, the diagonal block, the upper blocks
+        if (own) {
+            const int c = grid_slot(r, l);
+            const int dw[4] = {1, kGridWinCols - 1, kGridWinCols, kGridWinCols + 1};
+            double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+            for (int k = 3; k >= 0; --k)
+                if (rows.okl[k]) add_lower(rows.lu[k], rows.ll[k], win[c - dw[k]], a0, a1);
+            add_upper(rows.du, rows.dl, win[c], a0, a1);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) add_upper(rows.uu[k], rows.ul[k], win[c + dw[k]], a0, a1);
+            reinterpret_cast<dpair*>(q_out)[i] = dpair{a0, a1};
+            double m0, m1;
+            mo.apply(a0, a1, m0, m1);
+            acc[2] += pn.x * a0 + pn.y * a1;
+            acc[3] += a0 * zn.x + a1 * zn.y;
+            acc[4] += a0 * m0 + a1 * m1;
+        }
     }
+}
+
+// One fused PCG iteration per launch on 2D tiles (launch_pcg_fused_grid): the FP32 or FP64
+// planes by the state's phase, then the iteration's sums and state update as in
+// pcg_fused_stencil_kernel.
+template <int Prec>
+__global__ __launch_bounds__(kFusedTile) void pcg_fused_grid_kernel(
+    PcgBuffers b, StencilView v, const double* __restrict__ planes64, const float* __restrict__ planes32,
+    const double* __restrict__ diag_c, const double* __restrict__ r_in, double* __restrict__ r_out,
+    const double* __restrict__ p_in, double* __restrict__ p_out, const double* __restrict__ q_in,
+    double* __restrict__ q_out) {
+    if (b.st->done) return;
+    __shared__ dpair win[kGridWin];
+    __shared__ double sh[5 * (kFusedTile / kWave)];
+    const double alpha = b.st->alpha, beta = b.st->beta, alpha_prev = b.st->alpha_prev;
+    const long long launch = b.st->iter + 1;  // x update deferred as in pcg_fused_stencil_kernel
+    const int x_mode = (launch & 1) ? 0 : (launch >= 2 ? 2 : 1);
+    const bool use32 = planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
+    const int tile = xcd_block(blockIdx.x, gridDim.x);  // row bands stay on one XCD
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // r'.r', r'.z, p'.q', q'.z, q'.M^-1 q'
+    if (use32)
+        grid_tile<float, Prec>(b, v, planes32, reinterpret_cast<const dpair*>(diag_c), 1, v.n_off - 1, 0, r_in, r_out,
+                               p_in, p_out, q_in, q_out, alpha, beta, alpha_prev, x_mode, tile, win, acc);
+    else
+        grid_tile<double, Prec>(b, v, planes64, reinterpret_cast<const dpair*>(planes64), v.n_off, v.n_off, 1, r_in,
+                                r_out, p_in, p_out, q_in, q_out, alpha, beta, alpha_prev, x_mode, tile, win, acc);
+    block_sums<kFusedTile, 5>(acc, sh);
+    double tot[5];
+    if (!finish_reduction<kFusedTile, 5>(b.tickets + 1, b.partials, acc, tot, sh)) return;
+    if (threadIdx.x == 0) fused_state_update(b.st, tot, alpha);
 }
 
 // The x term an odd final launch deferred: x += alpha p_in of that launch. A solve
@@ -576,6 +773,21 @@ __global__ __launch_bounds__(kThreads) void fused_flush_kernel(PcgBuffers b, con
 void launch_pcg_fused_flush(hipStream_t s, const PcgBuffers& b, const double* p_odd) {
     const unsigned int need = blocks_for(b.n >> 1, kThreads);
     fused_flush_kernel<<<need < 4096u ? (need > 0 ? need : 1u) : 4096u, kThreads, 0, s>>>(b, p_odd);
+}
+
+void launch_pcg_fused_grid(hipStream_t s, const PcgBuffers& b, const StencilView& v, const double* planes64,
+                           const float* planes32, const double* diag_c, const double* r_in, double* r_out,
+                           const double* p_in, double* p_out, const double* q_in, double* q_out) {
+    const int L = v.row_len;
+    const unsigned int tiles = static_cast<unsigned int>(((v.n_nodes + L - 1) / L + kGridRows - 1) / kGridRows) *
+                               static_cast<unsigned int>((L + kWave - 1) / kWave);
+    const unsigned int grid = round8(tiles);
+    if (b.precond == LFG_PRECOND_BLOCK2)
+        pcg_fused_grid_kernel<LFG_PRECOND_BLOCK2><<<grid, kFusedTile, 0, s>>>(b, v, planes64, planes32, diag_c, r_in,
+                                                                              r_out, p_in, p_out, q_in, q_out);
+    else
+        pcg_fused_grid_kernel<LFG_PRECOND_JACOBI><<<grid, kFusedTile, 0, s>>>(b, v, planes64, planes32, diag_c, r_in,
+                                                                              r_out, p_in, p_out, q_in, q_out);
 }
 
 void launch_pack_stencil(hipStream_t s, const CsrView& a, const StencilView& v, const double* values, double* out64,

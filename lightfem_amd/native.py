@@ -78,11 +78,13 @@ class AssemblyReport(ctypes.Structure):
 class LayoutInfo(ctypes.Structure):
     _fields_ = [("node_blocked", ctypes.c_int32), ("n_slices", ctypes.c_int32),
                 ("stencil_exact_planes", ctypes.c_int32), ("stencil_sym_planes", ctypes.c_int32),
-                ("fused_window_pairs", ctypes.c_int32), ("stencil_offsets", ctypes.c_int32 * 17)]
+                ("fused_window_pairs", ctypes.c_int32), ("stencil_offsets", ctypes.c_int32 * 17),
+                ("grid_row_len", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         out = {name: getattr(self, name) for name, _ in self._fields_[:5]}
         out["stencil_offsets"] = list(self.stencil_offsets[:self.stencil_exact_planes])
+        out["grid_row_len"] = self.grid_row_len
         return out
 
 

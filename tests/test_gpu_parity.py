@@ -355,18 +355,27 @@ def test_stencil_layouts(dev):
         u6, r6 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=6)
         u9, r9 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=9)
         assert np.array_equal(bits(u6), bits(u9)) and r6["iterations"] == r9["iterations"], precond
-        for variant in (8, 10):  # 10: the whole iteration fused into one launch
+        for variant in (8, 10, 11):  # 10, 11: the whole iteration fused into one launch (1D, 2D tiles)
             u8, r8 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=variant)
             u8b, r8b = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=variant)
             assert np.array_equal(bits(u8), bits(u8b)) and r8["iterations"] == r8b["iterations"]
             assert abs(r8["iterations"] - r6["iterations"]) <= 2 + 0.01 * r6["iterations"], (precond, variant)
             assert np.linalg.norm(u8 - u6) <= 1e-9 * np.linalg.norm(u6), (precond, variant)
-    # the fused iteration honours the iteration cap and reports the reference's count
+    assert dev.layout_info()["grid_row_len"] == 65
+    # the fused iterations honour the iteration cap and report the reference's count
+    for variant in (10, 11):
+        with pytest.raises(native.LightFemError) as err:
+            dev.pcg("jacobi", "fp64", "relative", 1e-10, max_iterations=5, spmv_variant=variant)
+        assert err.value.code == native.ERR_NOT_CONVERGED
+        _, capped = dev.pcg("jacobi", "fp64", "relative", 1e-10, max_iterations=5, spmv_variant=variant,
+                            raise_on_error=False)
+        assert capped["iterations"] == 5 and not capped["converged"]
+    # 2D tiles need rows of at least 64 nodes: the 50x50 mesh (51 per row) has none
+    pipeline.setup(meshgen.cantilever_50(), device=dev)
+    assert dev.layout_info()["grid_row_len"] == 0
     with pytest.raises(native.LightFemError) as err:
-        dev.pcg("jacobi", "fp64", "relative", 1e-10, max_iterations=5, spmv_variant=10)
-    assert err.value.code == native.ERR_NOT_CONVERGED
-    _, capped = dev.pcg("jacobi", "fp64", "relative", 1e-10, max_iterations=5, spmv_variant=10, raise_on_error=False)
-    assert capped["iterations"] == 5 and not capped["converged"]
+        dev.pcg("jacobi", spmv_variant=11)
+    assert err.value.code == native.ERR_UNSUPPORTED
     # an RCM-numbered mesh has no fixed offsets: the default falls back to node slices
     pipeline.setup(meshgen.cantilever_200_rcm(), device=dev, assembly="coloring", reordering="rcm")
     info = dev.layout_info()
@@ -376,7 +385,35 @@ def test_stencil_layouts(dev):
     assert err.value.code == native.ERR_UNSUPPORTED
 
 
-def test_fused_iteration_iterate_at_every_cap(dev):
+@pytest.mark.parametrize("nx,ny", [(63, 10), (100, 70), (129, 101), (64, 64)])
+def test_grid_tiles_on_uneven_meshes(dev, nx, ny):
+    """The 2D-tiled iteration (variant 11) on meshes whose rows end inside a 64-node column
+    tile and whose last row tile is partial: the node-slice FP64 PCG's displacements (rows
+    summed as the reference) within 1e-9 for both fused preconditioners, in FP64 and with
+    FP32 values (mixed, with its FP64 refinement); the iteration count of the 1D-tiled
+    fused iteration (variant 10: the same arithmetic, partial sums grouped by other tiles)
+    within 1% + 2 in FP64; two runs bit-identical."""
+    print(f"case grid tiles {nx}x{ny}", flush=True)
+    m = meshgen.structured_cantilever(nx, ny, lx=float(nx), ly=float(ny),
+                                      young=meshgen.loguniform_young(nx * ny, nx + ny), plane_mode="plane_strain")
+    pipeline.setup(m, device=dev, assembly="coloring")
+    assert dev.layout_info()["grid_row_len"] == nx + 1
+    for precond in ("jacobi", "block"):
+        u6, r6 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=6)
+        _, r10 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=10)
+        for precision in ("fp64", "mixed"):
+            u11, r11 = dev.pcg(precond, precision, "relative", 1e-10, spmv_variant=11)
+            u11b, _ = dev.pcg(precond, precision, "relative", 1e-10, spmv_variant=11)
+            print(f"  {precond} {precision}: iterations 6: {r6['iterations']}, 10: {r10['iterations']}, "
+                  f"11: {r11['iterations']}, u vs 6 {np.linalg.norm(u11 - u6) / np.linalg.norm(u6):.2e}", flush=True)
+            assert np.array_equal(bits(u11), bits(u11b)), (precond, precision)
+            assert np.linalg.norm(u11 - u6) <= 1e-9 * np.linalg.norm(u6), (precond, precision)
+            if precision == "fp64":
+                assert abs(r11["iterations"] - r10["iterations"]) <= 2 + 0.01 * r10["iterations"], precond
+
+
+@pytest.mark.parametrize("variant", [10, 11])
+def test_fused_iteration_iterate_at_every_cap(dev, variant):
     """The fused stencil iteration adds x += alpha p for two iterations on every second
     launch and completes an odd last launch after the solve: the iterate returned after
     an odd or even iteration cap, or at convergence, is the reference PCG iterate after
@@ -386,7 +423,7 @@ def test_fused_iteration_iterate_at_every_cap(dev):
     sys_ = pipeline.setup(meshgen.loguniform_cantilever(64), device=dev, assembly="coloring")
     kc, fc = dev.system()
     for cap in (1, 2, 3, 6, 7, 50, 51, 100000):
-        u, rep = dev.pcg("jacobi", "fp64", "relative", 1e-10, max_iterations=cap, spmv_variant=10,
+        u, rep = dev.pcg("jacobi", "fp64", "relative", 1e-10, max_iterations=cap, spmv_variant=variant,
                          raise_on_error=False)
         ref, rref = lfo.pcg(sys_.row_ptr, sys_.col, kc, fc, lfo.JACOBI, lfo.STOP_RELATIVE, 1e-10, cap)
         assert rep["converged"] == rref["converged"], cap
