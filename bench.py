@@ -39,7 +39,7 @@ KERNELS = {1: "pcg_spmv_kernel<4>", 2: "pcg_spmv_kernel<8>", 3: "pcg_spmv_kernel
            4: "pcg_spmv_stream_kernel<24, 64>", 5: "pcg_spmv_stream_kernel<24, 32>",
            6: "pcg_spmv_slices_kernel<9>", 7: "pcg_spmv_slices_dir_kernel<9, 0>",
            8: "pcg_spmv_stencil_kernel<4, true>", 9: "pcg_spmv_stencil_kernel<9, false>",
-           10: "pcg_fused_stencil_kernel<4, 0>", 11: "pcg_fused_grid_kernel<0>"}
+           10: "pcg_fused_stencil_kernel<4, 0>", 11: "pcg_fused_grid_kernel<0, 4>"}
 
 
 def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int, layout: dict) -> int:

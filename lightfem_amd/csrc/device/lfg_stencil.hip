@@ -374,6 +374,29 @@ __device__ __forceinline__ void load_sym_rows(const StencilView& v, const dpair*
     a.dl = __builtin_nontemporal_load(diag + plane_at(i, 0, 1, np_d));
 }
 
+// The two halves of load_sym_rows: the node's own blocks (upper and diagonal, first read
+// from HBM) and the transposed lower blocks (the neighbours' planes, mostly in L2).
+template <typename VT, int kM>
+__device__ __forceinline__ void load_sym_upper(const dpair* __restrict__ diag, int np_d, const VT* __restrict__ planes,
+                                               int np, int kb, int i, SymRows<VT, kM>& a) {
+#pragma unroll
+    for (int k = 0; k < kM; ++k) plane_block<false>(planes, i, kb + k, np, a.uu[k], a.ul[k]);
+    a.du = __builtin_nontemporal_load(diag + plane_at(i, 0, 0, np_d));
+    a.dl = __builtin_nontemporal_load(diag + plane_at(i, 0, 1, np_d));
+}
+
+template <typename VT, int kM>
+__device__ __forceinline__ void load_sym_lower(const StencilView& v, const VT* __restrict__ planes, int np, int kb,
+                                               int i, SymRows<VT, kM>& a) {
+#pragma unroll
+    for (int k = 0; k < kM; ++k) {
+        const int jl = i - v.off[k + 1];
+        a.okl[k] = jl >= 0;
+        const int jlc = a.okl[k] ? jl : i;
+        plane_block<true>(planes, jlc, kb + k, np, a.lu[k], a.ll[k]);
+    }
+}
+
 // Rows 2i, 2i+1 times the tile's LDS windows, in column order (as stencil_sym_node).
 template <typename VT, int kM>
 __device__ __forceinline__ void apply_sym_rows(const StencilView& v, const SymRows<VT, kM>& a,
@@ -589,20 +612,25 @@ __global__ __launch_bounds__(kFusedTile) void pcg_fused_stencil_kernel(
 // row segments of 64 nodes (node i = i0 + r L + l: wavefront r, lane l) then needs p' on a
 // window of (kGridRows + 2) x 66 positions: its own 256 nodes and a ring of 140 around them,
 // against two whole 258-node windows of the 1D tile's neighbouring rows (518 positions).
-// Every load of a tile is issued before any is used: the own r, q, p, M^-1 (and x, older p),
-// the one halo position of the first 140 threads, and the node's 18 plane pairs (own upper
-// blocks, the transposed lower ones, the diagonal), in that order, so the vector phase waits
-// only for its own loads while the planes stream in.
+// A tile issues its loads before it uses any: the own r, q, p, M^-1 (and x, older p), one halo
+// position in each of the first 35 lanes of every wavefront, and the node's own plane pairs
+// (upper and diagonal blocks, 10 x 16 B), in that order, so the vector phase waits only for
+// its own loads while the planes stream in from HBM. The transposed lower blocks (8 x 16 B,
+// mostly L2 hits) are issued once the vector phase has freed its registers: 117 VGPRs, four
+// wavefronts per SIMD. At 1000 x 1000: 52.4 us per FP64 launch (5.66 TB/s of algorithmic
+// bytes), 42.5 us with FP32 values.
 constexpr int kGridWinCols = kWave + 2;
-constexpr int kGridWin = (kGridRows + 2) * kGridWinCols;
-constexpr int kGridHalo = kGridWin - kGridRows * kWave;  // 2 * 66 + 2 * kGridRows
+template <int R>
+constexpr int grid_win() { return (R + 2) * kGridWinCols; }
+template <int R>
+constexpr int grid_halo() { return grid_win<R>() - R * kWave; }  // 2 * 66 + 2 R
 
-// Window slot of (row r, column l), r in [-1, kGridRows], l in [-1, 64]; the offsets
-// 1, L-1, L, L+1 move the slot by 1, 65, 66, 67.
+// Window slot of (row r, column l), r in [-1, R], l in [-1, 64]; the offsets 1, L-1, L,
+// L+1 move the slot by 1, 65, 66, 67.
 __device__ __forceinline__ int grid_slot(int r, int l) { return (r + 1) * kGridWinCols + l + 1; }
 
 // One tile of the 2D iteration (values of type VT); adds the tile's sums to acc.
-template <typename VT, int Prec>
+template <typename VT, int Prec, int R>
 __device__ __forceinline__ void grid_tile(const PcgBuffers& b, const StencilView& v, const VT* __restrict__ planes,
                                           const dpair* __restrict__ diag, int np_d, int np, int kb,
                                           const double* __restrict__ r_in, double* __restrict__ r_out,
@@ -612,12 +640,12 @@ __device__ __forceinline__ void grid_tile(const PcgBuffers& b, const StencilView
                                           double (&acc)[5]) {
     const int n = v.n_nodes, L = v.row_len;
     const int col_tiles = (L + kWave - 1) / kWave;
-    const int row_tiles = ((n + L - 1) / L + kGridRows - 1) / kGridRows;
+    const int row_tiles = ((n + L - 1) / L + R - 1) / R;
     if (tile < row_tiles * col_tiles) {
         const int rt = tile / col_tiles;
         const int c0 = (tile - rt * col_tiles) * kWave;
         const int r = static_cast<int>(threadIdx.x) / kWave, l = static_cast<int>(threadIdx.x) & (kWave - 1);
-        const int i0 = rt * kGridRows * L + c0;
+        const int i0 = rt * R * L + c0;
         const int i = i0 + r * L + l;
         const bool own = c0 + l < L && i < n;
         const int ic = own ? i : 0;
@@ -634,15 +662,20 @@ __device__ __forceinline__ void grid_tile(const PcgBuffers& b, const StencilView
             xo = x2[ic];
             if (x_mode == 2) pp = reinterpret_cast<const dpair*>(p_out)[ic];
         }
-        // halo position of this thread: row -1, row kGridRows, then columns -1 and 64
-        const int h = static_cast<int>(threadIdx.x);
+        // halo position of this thread: row -1, row R, then columns -1 and 64, spread evenly
+        // over the wavefronts (kHaloW positions each, in order) so that none waits at the
+        // barrier for another's extra loads
+        constexpr int kHaloN = grid_halo<R>();
+        constexpr int kHaloW = (kHaloN + R - 1) / R;
+        static_assert(kHaloW <= kWave, "one halo position per thread");
+        const int h = (l < kHaloW) ? r * kHaloW + l : kHaloN;
         int hr = 0, hc = 0, hg = -1;
-        if (h < kGridHalo) {
+        if (h < kHaloN) {
             if (h < kGridWinCols) {
                 hr = -1;
                 hc = h - 1;
             } else if (h < 2 * kGridWinCols) {
-                hr = kGridRows;
+                hr = R;
                 hc = h - kGridWinCols - 1;
             } else {
                 hr = (h - 2 * kGridWinCols) >> 1;
@@ -654,15 +687,15 @@ __device__ __forceinline__ void grid_tile(const PcgBuffers& b, const StencilView
         const int hgc = hg >= 0 ? hg : 0;
         dpair hrv = {0.0, 0.0}, hqv = {0.0, 0.0}, hpv = {0.0, 0.0};
         NodePrecond<Prec> hm;
-        if (h < kGridHalo) {
+        if (h < kHaloN) {
             hrv = r2[hgc];
             hqv = q2[hgc];
             hpv = p2[hgc];
             hm.load(b.m, hgc);
         }
-        // the node's planes
+        // the node's own planes (upper and diagonal blocks)
         SymRows<VT, 4> rows;
-        load_sym_rows<VT, 4>(v, diag, np_d, planes, np, kb, ic, rows);
+        load_sym_upper<VT, 4>(diag, np_d, planes, np, kb, ic, rows);
         // 1. own node: x += alpha p (deferred), r' = r - alpha q, z = M^-1 r', p' = z + beta p
         dpair pn = {0.0, 0.0}, zn = {0.0, 0.0};
         if (own) {
@@ -686,8 +719,11 @@ __device__ __forceinline__ void grid_tile(const PcgBuffers& b, const StencilView
             acc[1] += rn.x * z0 + rn.y * z1;
         }
         win[grid_slot(r, l)] = pn;
+        // the transposed lower blocks, once the own vectors' registers are free
+        __builtin_amdgcn_sched_barrier(0);
+        load_sym_lower<VT, 4>(v, planes, np, kb, ic, rows);
         // 2. the halo ring
-        if (h < kGridHalo) {
+        if (h < kHaloN) {
             dpair pj = {0.0, 0.0};
             if (hg >= 0) {
                 double z0, z1;
@@ -722,15 +758,15 @@ __device__ __forceinline__ void grid_tile(const PcgBuffers& b, const StencilView
 // One fused PCG iteration per launch on 2D tiles (launch_pcg_fused_grid): the FP32 or FP64
 // planes by the state's phase, then the iteration's sums and state update as in
 // pcg_fused_stencil_kernel.
-template <int Prec>
-__global__ __launch_bounds__(kFusedTile) void pcg_fused_grid_kernel(
+template <int Prec, int R>
+__global__ __launch_bounds__(R * kWave) void pcg_fused_grid_kernel(
     PcgBuffers b, StencilView v, const double* __restrict__ planes64, const float* __restrict__ planes32,
     const double* __restrict__ diag_c, const double* __restrict__ r_in, double* __restrict__ r_out,
     const double* __restrict__ p_in, double* __restrict__ p_out, const double* __restrict__ q_in,
     double* __restrict__ q_out) {
     if (b.st->done) return;
-    __shared__ dpair win[kGridWin];
-    __shared__ double sh[5 * (kFusedTile / kWave)];
+    __shared__ dpair win[grid_win<R>()];
+    __shared__ double sh[5 * R];
     const double alpha = b.st->alpha, beta = b.st->beta, alpha_prev = b.st->alpha_prev;
     const long long launch = b.st->iter + 1;  // x update deferred as in pcg_fused_stencil_kernel
     const int x_mode = (launch & 1) ? 0 : (launch >= 2 ? 2 : 1);
@@ -738,14 +774,14 @@ __global__ __launch_bounds__(kFusedTile) void pcg_fused_grid_kernel(
     const int tile = xcd_block(blockIdx.x, gridDim.x);  // row bands stay on one XCD
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // r'.r', r'.z, p'.q', q'.z, q'.M^-1 q'
     if (use32)
-        grid_tile<float, Prec>(b, v, planes32, reinterpret_cast<const dpair*>(diag_c), 1, v.n_off - 1, 0, r_in, r_out,
+        grid_tile<float, Prec, R>(b, v, planes32, reinterpret_cast<const dpair*>(diag_c), 1, v.n_off - 1, 0, r_in, r_out,
                                p_in, p_out, q_in, q_out, alpha, beta, alpha_prev, x_mode, tile, win, acc);
     else
-        grid_tile<double, Prec>(b, v, planes64, reinterpret_cast<const dpair*>(planes64), v.n_off, v.n_off, 1, r_in,
+        grid_tile<double, Prec, R>(b, v, planes64, reinterpret_cast<const dpair*>(planes64), v.n_off, v.n_off, 1, r_in,
                                 r_out, p_in, p_out, q_in, q_out, alpha, beta, alpha_prev, x_mode, tile, win, acc);
-    block_sums<kFusedTile, 5>(acc, sh);
+    block_sums<R * kWave, 5>(acc, sh);
     double tot[5];
-    if (!finish_reduction<kFusedTile, 5>(b.tickets + 1, b.partials, acc, tot, sh)) return;
+    if (!finish_reduction<R * kWave, 5>(b.tickets + 1, b.partials, acc, tot, sh)) return;
     if (threadIdx.x == 0) fused_state_update(b.st, tot, alpha);
 }
 
@@ -778,16 +814,17 @@ void launch_pcg_fused_flush(hipStream_t s, const PcgBuffers& b, const double* p_
 void launch_pcg_fused_grid(hipStream_t s, const PcgBuffers& b, const StencilView& v, const double* planes64,
                            const float* planes32, const double* diag_c, const double* r_in, double* r_out,
                            const double* p_in, double* p_out, const double* q_in, double* q_out) {
+    constexpr int R = kGridRows;
     const int L = v.row_len;
-    const unsigned int tiles = static_cast<unsigned int>(((v.n_nodes + L - 1) / L + kGridRows - 1) / kGridRows) *
+    const unsigned int tiles = static_cast<unsigned int>(((v.n_nodes + L - 1) / L + R - 1) / R) *
                                static_cast<unsigned int>((L + kWave - 1) / kWave);
     const unsigned int grid = round8(tiles);
     if (b.precond == LFG_PRECOND_BLOCK2)
-        pcg_fused_grid_kernel<LFG_PRECOND_BLOCK2><<<grid, kFusedTile, 0, s>>>(b, v, planes64, planes32, diag_c, r_in,
-                                                                              r_out, p_in, p_out, q_in, q_out);
+        pcg_fused_grid_kernel<LFG_PRECOND_BLOCK2, R><<<grid, R * kWave, 0, s>>>(b, v, planes64, planes32, diag_c,
+                                                                                r_in, r_out, p_in, p_out, q_in, q_out);
     else
-        pcg_fused_grid_kernel<LFG_PRECOND_JACOBI><<<grid, kFusedTile, 0, s>>>(b, v, planes64, planes32, diag_c, r_in,
-                                                                              r_out, p_in, p_out, q_in, q_out);
+        pcg_fused_grid_kernel<LFG_PRECOND_JACOBI, R><<<grid, R * kWave, 0, s>>>(b, v, planes64, planes32, diag_c,
+                                                                                r_in, r_out, p_in, p_out, q_in, q_out);
 }
 
 void launch_pack_stencil(hipStream_t s, const CsrView& a, const StencilView& v, const double* values, double* out64,
