@@ -249,7 +249,11 @@ def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict
     return out
 
 
-def run_preconditioners_200(dev, tol: float) -> dict:
+PRECOND_CASES = (("jacobi", "jacobi", "csr"), ("block_1_node", "block", "csr"), ("block_rcb_64", "subdomain", "csr"),
+                 ("ic0", "ic0", "csr"), ("jacobi_bsr", "jacobi", "bsr"))
+
+
+def run_preconditioners_200(dev, tol: float, names=None) -> dict:
     """The 200x200 cantilever in RCM order with coloring assembly, every preconditioner and
     storage of the reference: Jacobi, 2x2 nodal block, the default block (rcb, 64 nodes per
     subdomain), IC(0), and Jacobi with block storage. Device solve time (hipEvents) to the
@@ -260,9 +264,9 @@ def run_preconditioners_200(dev, tol: float) -> dict:
     dev.set_subdomains(nodes, count)
     out = {"workload": "q4-cantilever-200x200-plane-stress-rcm-coloring", "dofs": int(dev.n), "nnz": int(dev.nnz),
            "stop_rule": f"reference ||r|| <= {tol:g} max(1, ||r0||)"}
-    cases = (("jacobi", "jacobi", "csr"), ("block_1_node", "block", "csr"), ("block_rcb_64", "subdomain", "csr"),
-             ("ic0", "ic0", "csr"), ("jacobi_bsr", "jacobi", "bsr"))
-    for name, precond, storage in cases:
+    for name, precond, storage in PRECOND_CASES:
+        if names and name not in names:
+            continue
         dev.pcg(precond, "fp64", "reference", tol, storage=storage, raise_on_error=False)  # set-up and warm-up
         _, rep = dev.pcg(precond, "fp64", "reference", tol, storage=storage, raise_on_error=False)
         out[name] = {"device_solve_ms": round(rep["solve_ms"], 3), "iterations": rep["iterations"],
@@ -293,6 +297,7 @@ def main() -> int:
     ap.add_argument("--batch-forms", default="fp64,mixed", help="comma-separated batch precisions to run")
     ap.add_argument("--skip-precond", action="store_true", help="skip the 200x200 preconditioner comparison")
     ap.add_argument("--precond-only", action="store_true", help="only the 200x200 preconditioner comparison (profiling)")
+    ap.add_argument("--precond-cases", default="", help="comma-separated subset of the 200x200 preconditioner cases")
     ap.add_argument("--assembly-only", action="store_true", help="only the assembly strategies (profiling)")
     ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--profile", type=int, default=0,
@@ -319,7 +324,8 @@ def main() -> int:
         return 0
     if args.precond_only:
         dev = native.Device(0)
-        print(json.dumps({"preconditioners_200x200": run_preconditioners_200(dev, args.tol)}))
+        names = [c for c in args.precond_cases.split(",") if c]
+        print(json.dumps({"preconditioners_200x200": run_preconditioners_200(dev, args.tol, names)}))
         dev.close()
         return 0
     if args.batch_only:

@@ -178,6 +178,7 @@ struct lfg_ctx {
     lfg::BatchShape batch_shape{256, 1};
     double batch_tune_ms = 0.0;
     DevArray<int> batch_slice_problem, batch_pslice_ptr, batch_problem_done, batch_phase32;
+    DevArray<int> batch_active, batch_active_first;  // active slice list and its compaction offsets
     DevArray<unsigned int> batch_tickets;
     DevArray<double> batch_b, batch_x, batch_r, batch_p, batch_q, batch_res, batch_minv, batch_partials, batch_tmp;
     DevArray<lfg::BatchSystem> batch_sys;
@@ -2097,6 +2098,8 @@ int lfg_batch_set_problems(lfg_ctx* ctx, int32_t n_problems, const int32_t* node
         ctx->batch_pslice_ptr.upload(pslice.data(), pslice.size(), ctx->stream);
         ctx->batch_problem_done.alloc(static_cast<size_t>(n_problems));
         ctx->batch_phase32.alloc(static_cast<size_t>(n_problems));
+        ctx->batch_active.alloc(static_cast<size_t>(n_slices));
+        ctx->batch_active_first.alloc(static_cast<size_t>(n_problems));
         ctx->batch_tickets.alloc(32 * static_cast<size_t>(n_problems));
         LFG_HIP(hipMemsetAsync(ctx->batch_tickets.p, 0, ctx->batch_tickets.n * sizeof(unsigned int), ctx->stream));
         ctx->batch_partials.alloc(8 * static_cast<size_t>(n_slices));
@@ -2188,6 +2191,8 @@ lfg::BatchView batch_view(lfg_ctx* c, const lfg_batch_options* opt) {
     bv.global = c->batch_global.p;
     const bool mixed = opt->precision == LFG_PRECISION_MIXED;
     bv.problem_phase32 = mixed ? c->batch_phase32.p : nullptr;
+    bv.active = c->batch_active.p;
+    bv.active_first = c->batch_active_first.p;
     bv.switch_tau = opt->mixed_switch != 0.0 ? opt->mixed_switch : 1.0e-2;
     bv.tol = opt->tolerance;
     bv.stop_rule = opt->stop_rule;
@@ -2233,8 +2238,13 @@ void batch_autotune(lfg_ctx* c, const lfg::BatchView& bv, bool mixed) {
 long long batch_solve(lfg_ctx* c, const lfg_batch_options* opt, const lfg::BatchView& bv, bool mixed) {
     const lfg::SliceView v = slice_view(c);
     const lfg::BatchShape shape = c->batch_shape;
-    const int per_graph = opt->check_every > 0 ? opt->check_every : 32;
-    LFG_HIP(hipMemsetAsync(c->batch_global.p, 0, sizeof(lfg::BatchGlobal), c->stream));
+    // five kernels per iteration (two multiply-and-fold, the direction): 16 iterations and
+    // the compaction per captured graph
+    const int per_graph = opt->check_every > 0 ? opt->check_every : 16;
+    // every slice active at the start (the init kernel writes the identity list)
+    lfg::BatchGlobal start{};
+    start.n_active = c->n_slices;
+    LFG_HIP(hipMemcpyAsync(c->batch_global.p, &start, sizeof(start), hipMemcpyHostToDevice, c->stream));
     lfg::launch_batch_init(c->stream, bv, shape);
     auto iteration = [&] {
         lfg::launch_batch_spmv(c->stream, bv, shape, v, c->slices64.p, mixed ? c->slices32.p : nullptr, 0);
@@ -2245,6 +2255,9 @@ long long batch_solve(lfg_ctx* c, const lfg_batch_options* opt, const lfg::Batch
     hipGraphExec_t exec = nullptr;
     LFG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < per_graph; ++i) iteration();
+    // drop the problems that stopped during these iterations from the active list (until
+    // then every kernel skips them by their done flag)
+    lfg::launch_batch_compact(c->stream, bv);
     LFG_HIP(hipStreamEndCapture(c->stream, &graph));
     struct GraphGuard {
         hipGraph_t g;

@@ -42,6 +42,18 @@ struct SliceRange {
     int first, stop, step;
 };
 
+// This block's share of the active slice list (XCD-remapped contiguous ranges, as
+// block_slices): the iteration kernels walk only the running problems' slices, so a
+// launch costs what the remaining work costs, however many problems have stopped.
+__device__ __forceinline__ SliceRange active_range(const BatchView& bv) {
+    const int na = bv.global->n_active;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = (na + gridDim.x - 1) / gridDim.x;
+    const int start = lb * per_block;
+    const int stop = min(start + per_block, na);
+    return SliceRange{start + static_cast<int>(threadIdx.x) / kWave, stop, static_cast<int>(blockDim.x) / kWave};
+}
+
 __device__ __forceinline__ SliceRange block_slices(int n_slices) {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int per_block = (n_slices + gridDim.x - 1) / gridDim.x;
@@ -84,6 +96,7 @@ __device__ __forceinline__ void close_problem_if_done(const BatchView& bv, int p
     for (int c = 0; c < bv.n_cases; ++c)
         if (!bv.sys[p * bv.n_cases + c].done) return;
     bv.problem_done[p] = 1;
+    __hip_atomic_store(&bv.global->changed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int before =
         __hip_atomic_fetch_add(&bv.global->problems_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (before + 1 == static_cast<unsigned int>(bv.n_problems))
@@ -100,6 +113,7 @@ __global__ __launch_bounds__(512) void batch_init_kernel(BatchView bv) {
     const dpair* __restrict__ m2 = reinterpret_cast<const dpair*>(bv.minv);
     for (int s = range.first; s < range.stop; s += range.step) {
         const int p = bv.slice_problem[s];
+        if (lane_id() == 0) bv.active[s] = s;  // every problem starts running
         double rr = 0.0, rz = 0.0;
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
@@ -164,7 +178,7 @@ __global__ __launch_bounds__(512) void batch_init_kernel(BatchView bv) {
 // 0.0 as the CSR row loop (sparse.cpp:104-120). Returns the per-case p.q of the lane.
 template <int NC, typename VT, int kW>
 __device__ __forceinline__ void slice_multiply(const BatchView& bv, const SliceView& v, const VT* __restrict__ planes,
-                                               int s, double* pq) {
+                                               int s, double* pq, dpair* qout) {
     using P = typename PairOf<VT>::type;
     const int node = s * kWave + lane_id();
     const int begin = v.ptr[s];
@@ -215,7 +229,6 @@ __device__ __forceinline__ void slice_multiply(const BatchView& bv, const SliceV
             }
         }
     }
-    dpair* q2 = reinterpret_cast<dpair*>(bv.q) + static_cast<long long>(node) * NC;
     const dpair* pn = p2 + static_cast<long long>(node) * NC;
     if (sizeof(VT) == sizeof(float)) {
         // FP32 planes hold a zero diagonal block: add the FP64 compensated one
@@ -232,46 +245,84 @@ __device__ __forceinline__ void slice_multiply(const BatchView& bv, const SliceV
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        const dpair qv = {a0[c], a1[c]};
-        q2[c] = qv;
+        qout[c] = dpair{a0[c], a1[c]};
         const dpair pv = pn[c];
         pq[c] = pv.x * a0[c] + pv.y * a1[c];
     }
 }
 
-// q = K p, then per system alpha = rho / p.q (solver.cpp:585-595). kMixed: each
-// problem's phase picks its values (FP64 planes, or FP32 planes + compensated diagonal).
+// q of slice s (the NC cases of each node) into dst.
+template <int NC>
+__device__ __forceinline__ void store_slice(double* dst, int s, const dpair* qv) {
+    dpair* d = reinterpret_cast<dpair*>(dst) + static_cast<long long>(s * kWave + lane_id()) * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) d[c] = qv[c];
+}
+
+// q = K p and the slice's per-case p.q partials (slot s of the partials); the problem's
+// sums and alpha follow in batch_spmv_finish_kernel. kMixed: each problem's phase picks
+// its values (FP64 planes, or FP32 planes + compensated diagonal).
 template <int NC, int kW, bool kMixed>
 __global__ __launch_bounds__(512) void batch_spmv_kernel(BatchView bv, SliceView v, const double* __restrict__ planes64,
                                                          const float* __restrict__ planes32) {
     if (bv.global->done_all) return;
-    const SliceRange range = block_slices(bv.n_slices);
-    for (int s = range.first; s < range.stop; s += range.step) {
+    const SliceRange range = active_range(bv);
+    for (int a = range.first; a < range.stop; a += range.step) {
+        const int s = bv.active[a];
         const int p = bv.slice_problem[s];
         if (bv.problem_done[p]) continue;
         double pq[NC];
-        if (kMixed && bv.problem_phase32[p]) slice_multiply<NC, float, kW>(bv, v, planes32, s, pq);
-        else slice_multiply<NC, double, kW>(bv, v, planes64, s, pq);
+        dpair qv[NC];
+        if (kMixed && bv.problem_phase32[p]) slice_multiply<NC, float, kW>(bv, v, planes32, s, pq, qv);
+        else slice_multiply<NC, double, kW>(bv, v, planes64, s, pq, qv);
+        store_slice<NC>(bv.q, s, qv);
         double vals[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) vals[c] = wave_sum(pq[c]);
-        double tot[NC];
-        if (!finish_problem<NC>(bv, p, s, vals, tot)) continue;
         if (lane_id() == 0) {
-            for (int c = 0; c < NC; ++c) {
-                BatchSystem* st = bv.sys + p * NC + c;
-                if (st->done) continue;
-                const double tpq = tot[c];
-                if (!(tpq > 0.0) || !isfinite(tpq)) {
-                    st->status = LFG_ERR_NOT_SPD;
-                    st->done = 1;
-                } else {
-                    st->alpha = st->rho / tpq;
-                }
-            }
-            close_problem_if_done(bv, p);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) bv.partials[static_cast<long long>(s) * NC + c] = vals[c];
         }
     }
+}
+
+// The sum over problem p's slices of K values per slice (slots s * K + k of the partials),
+// folded in slice order by one wavefront (lane l takes slices l, l + 64, ...), the fold
+// of finish_problem: run-to-run and launch-shape identical.
+template <int K>
+__device__ __forceinline__ void fold_problem(const BatchView& bv, int p, double* out) {
+    const int first = bv.pslice_ptr[p];
+    const int count = bv.pslice_ptr[p + 1] - first;
+    const double* part = bv.partials + static_cast<long long>(first) * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double acc = 0.0;
+        for (int i = lane_id(); i < count; i += kWave) acc += part[i * K + k];
+        out[k] = wave_sum(acc);
+    }
+}
+
+// Per system alpha = rho / p.q (solver.cpp:585-595), one wavefront per running problem.
+template <int NC>
+__global__ __launch_bounds__(256) void batch_spmv_finish_kernel(BatchView bv) {
+    if (bv.global->done_all) return;
+    const int p = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    if (p >= bv.n_problems || bv.problem_done[p]) return;
+    double tot[NC];
+    fold_problem<NC>(bv, p, tot);
+    if (lane_id() != 0) return;
+    for (int c = 0; c < NC; ++c) {
+        BatchSystem* st = bv.sys + p * NC + c;
+        if (st->done) continue;
+        const double tpq = tot[c];
+        if (!(tpq > 0.0) || !isfinite(tpq)) {
+            st->status = LFG_ERR_NOT_SPD;
+            st->done = 1;
+        } else {
+            st->alpha = st->rho / tpq;
+        }
+    }
+    close_problem_if_done(bv, p);
 }
 
 // Plain q = K p over all slices (no reductions, no state): timing the launch shapes.
@@ -280,25 +331,29 @@ __global__ __launch_bounds__(512) void batch_multiply_kernel(BatchView bv, Slice
     const SliceRange range = block_slices(bv.n_slices);
     for (int s = range.first; s < range.stop; s += range.step) {
         double pq[NC];
-        slice_multiply<NC, VT, kW>(bv, v, planes, s, pq);
+        dpair qv[NC];
+        slice_multiply<NC, VT, kW>(bv, v, planes, s, pq, qv);
+        store_slice<NC>(bv.q, s, qv);
     }
 }
 
 // ------------------------------------------------------------ update and direction
 
-// x += alpha p, r -= alpha q, z = M^-1 r; per system ||r||^2 and r.z, then the
-// stop test on the recurrence residual and beta (solver.cpp:597-621).
+// x += alpha p, r -= alpha q, z = M^-1 r; the slice's per-case ||r||^2 and r.z partials
+// (slots 2 NC s .. of the partials); the stop test and beta follow in
+// batch_update_finish_kernel (solver.cpp:597-621).
 template <int NC>
 __global__ __launch_bounds__(512) void batch_update_kernel(BatchView bv) {
     if (bv.global->done_all) return;
-    const SliceRange range = block_slices(bv.n_slices);
+    const SliceRange range = active_range(bv);
     const int my_case = lane_id() % NC;
     const dpair* __restrict__ m2 = reinterpret_cast<const dpair*>(bv.minv);
     dpair* __restrict__ x2 = reinterpret_cast<dpair*>(bv.x);
     dpair* __restrict__ r2 = reinterpret_cast<dpair*>(bv.r);
     const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(bv.p);
     const dpair* __restrict__ q2 = reinterpret_cast<const dpair*>(bv.q);
-    for (int s = range.first; s < range.stop; s += range.step) {
+    for (int a = range.first; a < range.stop; a += range.step) {
+        const int s = bv.active[a];
         const int p = bv.slice_problem[s];
         if (bv.problem_done[p]) continue;
         const BatchSystem* st = bv.sys + p * NC + my_case;
@@ -331,53 +386,66 @@ __global__ __launch_bounds__(512) void batch_update_kernel(BatchView bv) {
             vals[2 * k] = __shfl(rr, k, kWave);
             vals[2 * k + 1] = __shfl(rz, k, kWave);
         }
-        double tot[2 * NC];
-        if (!finish_problem<2 * NC>(bv, p, s, vals, tot)) continue;
         if (lane_id() == 0) {
-            for (int c = 0; c < NC; ++c) {
-                BatchSystem* st = bv.sys + p * NC + c;
-                if (st->done) continue;
-                st->iter += 1;
-                st->rr = tot[2 * c];
-                const double rz = tot[2 * c + 1];
-                if (sqrt(max0(st->rr)) <= st->target) {
-                    st->converged = 1;
-                    st->done = 1;
-                } else if (!(rz > 0.0) || !isfinite(rz)) {
-                    st->status = LFG_ERR_NONFINITE;
-                    st->done = 1;
-                } else {
-                    st->beta = rz / st->rho;
-                    st->rho = rz;
-                    if (st->iter >= bv.max_iter) st->done = 1;
-                }
-            }
-            if (bv.problem_phase32 && !bv.problem_phase32[p]) {
-                bool reached = true;
-                for (int c = 0; c < NC; ++c) {
-                    const BatchSystem* st = bv.sys + p * NC + c;
-                    reached = reached && (st->done || sqrt(max0(st->rr)) <= bv.switch_tau * st->rnorm0);
-                }
-                if (reached) {
-                    bv.problem_phase32[p] = 1;
-                    for (int c = 0; c < NC; ++c) bv.sys[p * NC + c].switch_iter = bv.sys[p * NC + c].iter;
-                }
-            }
-            close_problem_if_done(bv, p);
+#pragma unroll
+            for (int k = 0; k < 2 * NC; ++k) bv.partials[static_cast<long long>(s) * 2 * NC + k] = vals[k];
         }
     }
+}
+
+// Per system the stop test on the recurrence residual and beta, the mixed-precision
+// phase switch, one wavefront per running problem.
+template <int NC>
+__global__ __launch_bounds__(256) void batch_update_finish_kernel(BatchView bv) {
+    if (bv.global->done_all) return;
+    const int p = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    if (p >= bv.n_problems || bv.problem_done[p]) return;
+    double tot[2 * NC];
+    fold_problem<2 * NC>(bv, p, tot);
+    if (lane_id() != 0) return;
+    for (int c = 0; c < NC; ++c) {
+        BatchSystem* st = bv.sys + p * NC + c;
+        if (st->done) continue;
+        st->iter += 1;
+        st->rr = tot[2 * c];
+        const double rz = tot[2 * c + 1];
+        if (sqrt(max0(st->rr)) <= st->target) {
+            st->converged = 1;
+            st->done = 1;
+        } else if (!(rz > 0.0) || !isfinite(rz)) {
+            st->status = LFG_ERR_NONFINITE;
+            st->done = 1;
+        } else {
+            st->beta = rz / st->rho;
+            st->rho = rz;
+            if (st->iter >= bv.max_iter) st->done = 1;
+        }
+    }
+    if (bv.problem_phase32 && !bv.problem_phase32[p]) {
+        bool reached = true;
+        for (int c = 0; c < NC; ++c) {
+            const BatchSystem* st = bv.sys + p * NC + c;
+            reached = reached && (st->done || sqrt(max0(st->rr)) <= bv.switch_tau * st->rnorm0);
+        }
+        if (reached) {
+            bv.problem_phase32[p] = 1;
+            for (int c = 0; c < NC; ++c) bv.sys[p * NC + c].switch_iter = bv.sys[p * NC + c].iter;
+        }
+    }
+    close_problem_if_done(bv, p);
 }
 
 // p = z + beta p for the systems still running.
 template <int NC>
 __global__ __launch_bounds__(512) void batch_direction_kernel(BatchView bv) {
     if (bv.global->done_all) return;
-    const SliceRange range = block_slices(bv.n_slices);
+    const SliceRange range = active_range(bv);
     const int my_case = lane_id() % NC;
     const dpair* __restrict__ m2 = reinterpret_cast<const dpair*>(bv.minv);
     const dpair* __restrict__ r2 = reinterpret_cast<const dpair*>(bv.r);
     dpair* __restrict__ p2 = reinterpret_cast<dpair*>(bv.p);
-    for (int s = range.first; s < range.stop; s += range.step) {
+    for (int a = range.first; a < range.stop; a += range.step) {
+        const int s = bv.active[a];
         const int p = bv.slice_problem[s];
         if (bv.problem_done[p]) continue;
         const BatchSystem* st = bv.sys + p * NC + my_case;
@@ -397,6 +465,53 @@ __global__ __launch_bounds__(512) void batch_direction_kernel(BatchView bv) {
     }
 }
 
+// ------------------------------------------------------------ active slice list
+
+constexpr int kScanThreads = 256;
+
+// Positions of the running problems' slices in the new list (an exclusive scan of their
+// slice counts in problem order) and its length; nothing when no problem stopped since
+// the last compaction.
+__global__ __launch_bounds__(kScanThreads) void batch_compact_scan_kernel(BatchView bv) {
+    BatchGlobal* g = bv.global;
+    const int t = static_cast<int>(threadIdx.x);
+    if (g->done_all || !g->changed) {
+        if (t == 0) g->compact_now = 0;
+        return;
+    }
+    __shared__ int sh[kScanThreads];
+    int total = 0;
+    for (int base = 0; base < bv.n_problems; base += kScanThreads) {
+        const int p = base + t;
+        const int cnt = (p < bv.n_problems && !bv.problem_done[p]) ? bv.pslice_ptr[p + 1] - bv.pslice_ptr[p] : 0;
+        sh[t] = cnt;
+        __syncthreads();
+        for (int o = 1; o < kScanThreads; o <<= 1) {  // inclusive scan (Hillis-Steele)
+            const int add = t >= o ? sh[t - o] : 0;
+            __syncthreads();
+            sh[t] += add;
+            __syncthreads();
+        }
+        if (p < bv.n_problems) bv.active_first[p] = total + sh[t] - cnt;
+        total += sh[kScanThreads - 1];
+        __syncthreads();
+    }
+    if (t == 0) {
+        g->n_active = total;
+        g->changed = 0;
+        g->compact_now = 1;
+    }
+}
+
+// Writes the new list: slice s of running problem p at active_first[p] + (s - first slice of p).
+__global__ __launch_bounds__(256) void batch_compact_fill_kernel(BatchView bv) {
+    if (!bv.global->compact_now) return;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < bv.n_slices; s += gridDim.x * blockDim.x) {
+        const int p = bv.slice_problem[s];
+        if (!bv.problem_done[p]) bv.active[bv.active_first[p] + s - bv.pslice_ptr[p]] = s;
+    }
+}
+
 // ------------------------------------------------------------ FP64 residual
 
 // res = b - K x with FP64 values, per system ||res||^2 into true_rr.
@@ -412,7 +527,9 @@ __global__ __launch_bounds__(512) void batch_residual_kernel(BatchView bv, Slice
         tmp.p = bv.x;
         tmp.q = res;
         double unused[NC];
-        slice_multiply<NC, double, kW>(tmp, v, planes, s, unused);
+        dpair kx[NC];
+        slice_multiply<NC, double, kW>(tmp, v, planes, s, unused, kx);
+        store_slice<NC>(res, s, kx);
         double vals[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
@@ -505,10 +622,11 @@ static void spmv_nc(hipStream_t s, const BatchView& bv, const SliceView& v, cons
     if (plain) {
         if (planes64) batch_multiply_kernel<NC, double, kW><<<grid, threads, 0, s>>>(bv, v, planes64);
         else batch_multiply_kernel<NC, float, kW><<<grid, threads, 0, s>>>(bv, v, planes32);
-    } else if (planes32) {
-        batch_spmv_kernel<NC, kW, true><<<grid, threads, 0, s>>>(bv, v, planes64, planes32);
     } else {
-        batch_spmv_kernel<NC, kW, false><<<grid, threads, 0, s>>>(bv, v, planes64, nullptr);
+        if (planes32) batch_spmv_kernel<NC, kW, true><<<grid, threads, 0, s>>>(bv, v, planes64, planes32);
+        else batch_spmv_kernel<NC, kW, false><<<grid, threads, 0, s>>>(bv, v, planes64, nullptr);
+        const unsigned int fin = blocks_for(static_cast<long long>(bv.n_problems) * kWave, 256);
+        batch_spmv_finish_kernel<NC><<<fin, 256, 0, s>>>(bv);
     }
 }
 
@@ -525,12 +643,20 @@ void launch_batch_spmv(hipStream_t s, const BatchView& bv, const BatchShape& sha
 
 void launch_batch_update(hipStream_t s, const BatchView& bv, const BatchShape& shape) {
     const unsigned int grid = batch_grid(shape, bv.n_slices);
-    LFG_BATCH_NC(bv.n_cases, (batch_update_kernel<NC><<<grid, shape.block_threads, 0, s>>>(bv)));
+    const unsigned int fin = blocks_for(static_cast<long long>(bv.n_problems) * kWave, 256);
+    LFG_BATCH_NC(bv.n_cases, (batch_update_kernel<NC><<<grid, shape.block_threads, 0, s>>>(bv),
+                              batch_update_finish_kernel<NC><<<fin, 256, 0, s>>>(bv)));
 }
 
 void launch_batch_direction(hipStream_t s, const BatchView& bv, const BatchShape& shape) {
     const unsigned int grid = batch_grid(shape, bv.n_slices);
     LFG_BATCH_NC(bv.n_cases, (batch_direction_kernel<NC><<<grid, shape.block_threads, 0, s>>>(bv)));
+}
+
+void launch_batch_compact(hipStream_t s, const BatchView& bv) {
+    batch_compact_scan_kernel<<<1, kScanThreads, 0, s>>>(bv);
+    const unsigned int need = blocks_for(bv.n_slices, 256);
+    batch_compact_fill_kernel<<<need < 1024u ? (need > 0 ? need : 1u) : 1024u, 256, 0, s>>>(bv);
 }
 
 void launch_batch_residual(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,

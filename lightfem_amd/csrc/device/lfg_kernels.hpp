@@ -329,7 +329,10 @@ struct BatchSystem {      // one (problem, case)
 struct BatchGlobal {
     unsigned int problems_done;
     int done_all;
-    int pad[14];
+    int n_active;     // entries of the active slice list the iteration kernels walk
+    int changed;      // a problem stopped since the list was last compacted
+    int compact_now;  // the scan found changes: the fill kernel rewrites the list
+    int pad[11];
 };
 struct BatchView {
     int n_problems, n_cases, n_slices;
@@ -347,6 +350,8 @@ struct BatchView {
     double* partials;          // slices x 2 * n_cases
     BatchGlobal* global;
     int* problem_phase32;      // mixed: 1 once problem p multiplies with FP32 values; null for FP64
+    int* active;               // slices of the running problems (problem order), global->n_active of them
+    int* active_first;         // compaction: position of problem p's first slice in the new list
     double switch_tau;         // mixed: relative residual that moves a problem to FP32 (< 0: from the start)
     double tol;
     int stop_rule;
@@ -365,6 +370,9 @@ void launch_batch_spmv(hipStream_t s, const BatchView& bv, const BatchShape& sha
                        const double* planes64, const float* planes32, int plain);
 void launch_batch_update(hipStream_t s, const BatchView& bv, const BatchShape& shape);
 void launch_batch_direction(hipStream_t s, const BatchView& bv, const BatchShape& shape);
+// Rebuilds the active slice list without the problems that stopped since the last
+// compaction (two kernels; both return at once when nothing changed).
+void launch_batch_compact(hipStream_t s, const BatchView& bv);
 // res = b - K x (FP64 values), ||res||^2 per system into true_rr.
 void launch_batch_residual(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
                            const double* planes64, double* res);

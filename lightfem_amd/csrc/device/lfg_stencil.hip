@@ -637,7 +637,7 @@ __device__ __forceinline__ void grid_tile(const PcgBuffers& b, const StencilView
                                           const double* __restrict__ p_in, double* __restrict__ p_out,
                                           const double* __restrict__ q_in, double* __restrict__ q_out, double alpha,
                                           double beta, double alpha_prev, int x_mode, int tile, dpair* win,
-                                          double (&acc)[5]) {
+                                          double (&acc)[5], int& qi, dpair& qv) {
     const int n = v.n_nodes, L = v.row_len;
     const int col_tiles = (L + kWave - 1) / kWave;
     const int row_tiles = ((n + L - 1) / L + R - 1) / R;
@@ -745,7 +745,8 @@ __device__ __forceinline__ void grid_tile(const PcgBuffers& b, const StencilView
             add_upper(rows.du, rows.dl, win[c], a0, a1);
 #pragma unroll
             for (int k = 0; k < 4; ++k) add_upper(rows.uu[k], rows.ul[k], win[c + dw[k]], a0, a1);
-            reinterpret_cast<dpair*>(q_out)[i] = dpair{a0, a1};
+            qi = i;  // q' is stored after the reduction's ticket (pcg_fused_grid_kernel)
+            qv = dpair{a0, a1};
             double m0, m1;
             mo.apply(a0, a1, m0, m1);
             acc[2] += pn.x * a0 + pn.y * a1;
@@ -773,15 +774,22 @@ __global__ __launch_bounds__(R * kWave) void pcg_fused_grid_kernel(
     const bool use32 = planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
     const int tile = xcd_block(blockIdx.x, gridDim.x);  // row bands stay on one XCD
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // r'.r', r'.z, p'.q', q'.z, q'.M^-1 q'
+    int qi = -1;
+    dpair qv = {0.0, 0.0};
     if (use32)
         grid_tile<float, Prec, R>(b, v, planes32, reinterpret_cast<const dpair*>(diag_c), 1, v.n_off - 1, 0, r_in, r_out,
-                               p_in, p_out, q_in, q_out, alpha, beta, alpha_prev, x_mode, tile, win, acc);
+                                  p_in, p_out, q_in, q_out, alpha, beta, alpha_prev, x_mode, tile, win, acc, qi, qv);
     else
         grid_tile<double, Prec, R>(b, v, planes64, reinterpret_cast<const dpair*>(planes64), v.n_off, v.n_off, 1, r_in,
-                                r_out, p_in, p_out, q_in, q_out, alpha, beta, alpha_prev, x_mode, tile, win, acc);
+                                   r_out, p_in, p_out, q_in, q_out, alpha, beta, alpha_prev, x_mode, tile, win, acc, qi,
+                                   qv);
     block_sums<R * kWave, 5>(acc, sh);
     double tot[5];
-    if (!finish_reduction<R * kWave, 5>(b.tickets + 1, b.partials, acc, tot, sh)) return;
+    // q' is stored after the block's ticket: the ticket waits for the partial's store, and
+    // would otherwise also wait for q''s (the next launch reads q' after this one ends)
+    const bool last = finish_reduction<R * kWave, 5>(b.tickets + 1, b.partials, acc, tot, sh);
+    if (qi >= 0) reinterpret_cast<dpair*>(q_out)[qi] = qv;
+    if (!last) return;
     if (threadIdx.x == 0) fused_state_update(b.st, tot, alpha);
 }
 
