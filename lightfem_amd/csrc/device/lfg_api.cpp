@@ -116,6 +116,7 @@ struct lfg_ctx {
     // pattern
     int n = 0;
     int max_row = 0;
+    int max_block_slots = 0;  // most node-list entries of one 64-node block (block-row assembly)
     long long nnz = 0;
     std::vector<int> h_row_ptr, h_col;
     DevArray<int> row_ptr, col, emap, cptr, contrib;
@@ -233,6 +234,13 @@ void build_node_elements(lfg_ctx* c) {
         for (int l = 0; l < 2; ++l) list[cur[c->h_bar_nodes[2 * t + l]]++] = ((n_q4 + t) << 2) | l;
     c->node_elem_ptr.upload(ptr.data(), ptr.size(), c->stream);
     c->node_elem.upload(list.data(), list.size(), c->stream);
+    // the most (element, corner) entries of one block of the block-row assembly (the
+    // colour-ordered lists hold the same entries per node)
+    c->max_block_slots = 0;
+    for (int n0 = 0; n0 < n_nodes; n0 += lfg::kAssembleBlockNodes) {
+        const int n1 = std::min(n_nodes, n0 + lfg::kAssembleBlockNodes);
+        c->max_block_slots = std::max(c->max_block_slots, ptr[n1] - ptr[n0]);
+    }
 }
 
 lfg::CsrView csr_view(const lfg_ctx* c) { return lfg::CsrView{c->n, c->nnz, c->row_ptr.p, c->col.p, c->max_row}; }
@@ -1391,11 +1399,15 @@ int lfg_assemble(lfg_ctx* ctx, int32_t strategy, lfg_assembly_report* report) {
                     "validated model has an unknown assembly strategy");
             require(!colored || !ctx->h_color_ptr.empty(), LFG_ERR_STATE,
                     "lfg_set_colors must precede coloring assembly");
+            const int* node_ptr = colored ? ctx->color_node_ptr.p : ctx->node_elem_ptr.p;
+            const int* node_list = colored ? ctx->color_node.p : ctx->node_elem.p;
             LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-            lfg::launch_assemble_rows(ctx->stream, m, ctx->n_nodes, ctx->row_ptr.p,
-                                      colored ? ctx->color_node_ptr.p : ctx->node_elem_ptr.p,
-                                      colored ? ctx->color_node.p : ctx->node_elem.p, ctx->emap.p, ctx->K.p,
-                                      ctx->flags.p);
+            if (ctx->have_slices && ctx->max_block_slots <= lfg::kAssembleBlockSlots)
+                lfg::launch_assemble_block_rows(ctx->stream, m, ctx->n_nodes, ctx->row_ptr.p, ctx->col.p, node_ptr,
+                                                node_list, ctx->K.p, ctx->flags.p);
+            else
+                lfg::launch_assemble_rows(ctx->stream, m, ctx->n_nodes, ctx->row_ptr.p, node_ptr, node_list,
+                                          ctx->emap.p, ctx->K.p, ctx->flags.p);
             LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
             sync(ctx);
             rep.assembly_ms = elapsed(ctx->ev_start, ctx->ev_stop);  // element matrices formed in the pass

@@ -20,7 +20,7 @@ struct Coords {
     double y[4];
 };
 
-__device__ __forceinline__ void constitutive(double e, double nu, int plane_mode, double d[9]) {
+__host__ __device__ __forceinline__ void constitutive(double e, double nu, int plane_mode, double d[9]) {
     // element.cpp:16-32
     if (plane_mode == 0) {
         const double factor = e / (1.0 - nu * nu);
@@ -35,8 +35,10 @@ __device__ __forceinline__ void constitutive(double e, double nu, int plane_mode
     }
 }
 
-// element.cpp:34-73. b is 3x8 row-major; returns false on a non-positive Jacobian.
-__device__ __forceinline__ bool kinematics(const Coords& c, double xi, double eta, double b[24], double& det) {
+// element.cpp:34-73: the shape-function gradients dN_i/dx, dN_i/dy and det J at (xi, eta);
+// returns false on a non-positive Jacobian.
+__host__ __device__ __forceinline__ bool shape_gradients(const Coords& c, double xi, double eta, double dnx[4],
+                                                         double dny[4], double& det) {
     const double dxi[4] = {-0.25 * (1.0 - eta), +0.25 * (1.0 - eta), +0.25 * (1.0 + eta), -0.25 * (1.0 + eta)};
     const double deta[4] = {-0.25 * (1.0 - xi), -0.25 * (1.0 + xi), +0.25 * (1.0 + xi), +0.25 * (1.0 - xi)};
     double dx_dxi = 0.0, dx_deta = 0.0, dy_dxi = 0.0, dy_deta = 0.0;
@@ -48,23 +50,34 @@ __device__ __forceinline__ bool kinematics(const Coords& c, double xi, double et
         dy_deta += deta[i] * c.y[i];
     }
     det = dx_dxi * dy_deta - dx_deta * dy_dxi;
-    if (!(det > 1.0e-14) || !isfinite(det)) return false;
+    const bool ok = det > 1.0e-14 && __builtin_isfinite(det);
     const double inv00 = dy_deta / det;
     const double inv01 = -dx_deta / det;
     const double inv10 = -dy_dxi / det;
     const double inv11 = dx_dxi / det;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const double dnx = inv00 * dxi[i] + inv10 * deta[i];
-        const double dny = inv01 * dxi[i] + inv11 * deta[i];
-        b[2 * i] = dnx;
+        dnx[i] = inv00 * dxi[i] + inv10 * deta[i];
+        dny[i] = inv01 * dxi[i] + inv11 * deta[i];
+    }
+    return ok;
+}
+
+// B (3x8 row-major) at (xi, eta): columns 2i = (dN_i/dx, 0, dN_i/dy), 2i+1 = (0, dN_i/dy, dN_i/dx).
+__host__ __device__ __forceinline__ bool kinematics(const Coords& c, double xi, double eta, double b[24],
+                                                    double& det) {
+    double dnx[4], dny[4];
+    const bool ok = shape_gradients(c, xi, eta, dnx, dny, det);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        b[2 * i] = dnx[i];
         b[2 * i + 1] = 0.0;
         b[8 + 2 * i] = 0.0;
-        b[8 + 2 * i + 1] = dny;
-        b[16 + 2 * i] = dny;
-        b[16 + 2 * i + 1] = dnx;
+        b[8 + 2 * i + 1] = dny[i];
+        b[16 + 2 * i] = dny[i];
+        b[16 + 2 * i + 1] = dnx[i];
     }
-    return true;
+    return ok;
 }
 
 __device__ __forceinline__ void load_element(const MeshView& m, int e, Coords& c, double& young, double& nu,
@@ -88,7 +101,7 @@ __device__ __forceinline__ void load_element(const MeshView& m, int e, Coords& c
 // ke[j][c] = sum_gp (sum_s b[s][row0 + j] db[s][c]) * detJ * t; every row with the same
 // operations whatever NR (rows of one Gauss point share its B and DB).
 template <int NR>
-__device__ __forceinline__ bool ke_rows(const Coords& c, double young, double nu, double thick, int plane_mode,
+__host__ __device__ __forceinline__ bool ke_rows(const Coords& c, double young, double nu, double thick, int plane_mode,
                                         int row0, double (&ke)[NR][8]) {
     double d[9];
     constitutive(young, nu, plane_mode, d);
@@ -148,6 +161,51 @@ __device__ __forceinline__ bool ke_row(const Coords& c, double young, double nu,
     const bool ok = ke_rows<1>(c, young, nu, thick, plane_mode, row, one);
 #pragma unroll
     for (int k = 0; k < 8; ++k) ke[k] = one[0][k];
+    return ok;
+}
+
+// Rows 2a, 2a + 1 of Ke with the operations of ke_rows<2>(..., 2a, ...) minus the products
+// with the structural zeros of B and D (D's (0,2), (1,2), (2,0), (2,1) entries; B's zero in
+// each column). Every sum of ke_rows starts from +0.0, and an omitted product is an exact
+// +-0 (its other factor is finite): adding +-0 to a sum that started from +0.0 never
+// changes it, so each `0.0 + x` below is the sum ke_rows forms and the rows are
+// bit-identical to it, with about half the floating-point operations.
+__host__ __device__ __forceinline__ bool ke_corner_rows(const Coords& c, double young, double nu, double thick,
+                                                        int plane_mode, int a, double (&ke)[2][8]) {
+    double d[9];
+    constitutive(young, nu, plane_mode, d);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ke[0][k] = ke[1][k] = 0.0;
+    bool ok = true;
+#pragma unroll 1
+    for (int p = 0; p < 4; ++p) {
+        const double xi = (p == 1 || p == 2) ? kGauss : -kGauss;
+        const double eta = p >= 2 ? kGauss : -kGauss;
+        double dnx[4], dny[4], det;
+        ok = shape_gradients(c, xi, eta, dnx, dny, det) && ok;
+        const double weight = det * thick;
+        // B column 2a = (ax, 0, ay), column 2a + 1 = (0, ay, ax)
+        double ax = dnx[0], ay = dny[0];
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            ax = a == k ? dnx[k] : ax;
+            ay = a == k ? dny[k] : ay;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            // DB columns 2i and 2i + 1 (D rows (d0 d1 0), (d3 d4 0), (0 0 d8))
+            const double db0x = 0.0 + d[0] * dnx[i];
+            const double db1x = 0.0 + d[3] * dnx[i];
+            const double db2x = 0.0 + d[8] * dny[i];
+            const double db0y = 0.0 + d[1] * dny[i];
+            const double db1y = 0.0 + d[4] * dny[i];
+            const double db2y = 0.0 + d[8] * dnx[i];
+            ke[0][2 * i] += ((0.0 + ax * db0x) + ay * db2x) * weight;
+            ke[0][2 * i + 1] += ((0.0 + ax * db0y) + ay * db2y) * weight;
+            ke[1][2 * i] += ((0.0 + ay * db1x) + ax * db2x) * weight;
+            ke[1][2 * i + 1] += ((0.0 + ay * db1y) + ax * db2y) * weight;
+        }
+    }
     return ok;
 }
 
@@ -281,6 +339,152 @@ __global__ __launch_bounds__(256) void assemble_rows_kernel(MeshView m, int n_no
             const int b = rp[2 * node + hh], len = rp[2 * node + hh + 1] - b;
             const double* row = acc + (g * 2 + hh) * kRowSlots;
             for (int k = slot; k < len; k += 4) values[b + k] = row[k];
+        }
+    }
+}
+
+// Owner-computes assembly with each (element, corner) row pair formed once per block: a
+// block owns nodes n0 .. n0 + 63 and so rows 2n0 .. 2n0 + 127, whose (element, corner)
+// slots are one contiguous range of the node lists (the same element and colour orders as
+// above). Phase 1: one thread per slot forms rows 2a, 2a + 1 of its element's Ke
+// (ke_corner_rows, the bits of ke_kernel) into LDS, column by column with the two rows'
+// entries side by side, and keeps the element's four nodes. Phase 2: a half-wavefront per
+// node, one lane per column dof of the node's rows (node-blocked pattern: rows 2n and
+// 2n + 1 hold the same columns): the lane reads its column from the pattern and sums, for
+// both rows at once, over the node's slots in list order, the entry of the corner whose
+// node is the column node: the order in which serial_assembly (solver.cpp:135-142) and
+// coloring_assembly (:163-179) add into the value, from 0.0. No element map is read (a
+// value's position is where the pattern holds its column), every value is written once,
+// coalesced, and the launch needs no zeroed K. The host uses it for node-blocked
+// patterns whose rows hold at most kAssembleRowSlots values and whose every block fits
+// kBlockSlots slots.
+constexpr int kBlockNodes = kAssembleBlockNodes;
+constexpr int kBlockSlots = kAssembleBlockSlots;
+
+__global__ __launch_bounds__(256) void assemble_block_rows_kernel(MeshView m, int n_nodes,
+                                                                  const int* __restrict__ rp,
+                                                                  const int* __restrict__ col,
+                                                                  const int* __restrict__ node_ptr,
+                                                                  const int* __restrict__ node_list,
+                                                                  double* __restrict__ values,
+                                                                  int* __restrict__ failures) {
+    constexpr int kHalf = kWave / 2;                       // lanes per node (columns of a row)
+    constexpr int kNodesPerRound = 256 / kHalf;            // nodes the block handles at once
+    constexpr int kRounds = kBlockNodes / kNodesPerRound;  // per half-wavefront
+    // column c of slot k at ke_lds[c * kStride + k]: a wavefront's phase-1 stores are
+    // consecutive, and the odd stride puts a node's columns in different LDS banks
+    constexpr int kStride = kBlockSlots + 1;
+    constexpr int kZero = kBlockSlots;                     // (+0.0, +0.0) in column 0's spare place
+    static_assert(kAssembleRowSlots <= kHalf, "a row's columns fit a half-wavefront");
+    __shared__ double2 ke_lds[8 * kStride];          // slot k, column c: (row 2a, row 2a + 1)
+    __shared__ int4 en_lds[kBlockSlots + 1];         // slot k: its element's nodes; a sentinel
+    __shared__ int sp[kBlockNodes + 1];              // node list offsets of the block's nodes
+    __shared__ int rps[2 * kBlockNodes + 1];         // row offsets of the block's rows
+    const int t = static_cast<int>(threadIdx.x);
+    const int n0 = static_cast<int>(blockIdx.x) * kBlockNodes;
+    const int nb = min(kBlockNodes, n_nodes - n0);
+    if (t <= nb) sp[t] = node_ptr[n0 + t];
+    if (t <= 2 * nb) rps[t] = rp[2 * n0 + t];
+    if (t == 0) {
+        ke_lds[kZero] = make_double2(0.0, 0.0);
+        en_lds[kBlockSlots] = make_int4(-2, -2, -2, -2);  // matches no column node
+    }
+    __syncthreads();
+    // node g = t / kHalf + kNodesPerRound * q, column lane l = t % kHalf: the columns are
+    // loaded now so their latency overlaps phase 1
+    const int gl = t / kHalf;
+    const int l = t % kHalf;
+    int cols[kRounds];
+#pragma unroll
+    for (int q = 0; q < kRounds; ++q) {
+        const int g = gl + kNodesPerRound * q;
+        cols[q] = -1;
+        if (g < nb && l < rps[2 * g + 1] - rps[2 * g]) cols[q] = __builtin_nontemporal_load(col + rps[2 * g] + l);
+    }
+    const int s0 = sp[0];
+    const int ns = min(sp[nb] - s0, kBlockSlots);  // the host guarantees sp[nb] - s0 <= kBlockSlots
+    for (int k = t; k < ns; k += 256) {
+        const int code = node_list[s0 + k];
+        const int e = code >> 2;
+        const int a = code & 3;
+        const int4 nodes = reinterpret_cast<const int4*>(m.enodes)[e];
+        const int ids[4] = {nodes.x, nodes.y, nodes.z, nodes.w};
+        Coords c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double2 p = reinterpret_cast<const double2*>(m.xy)[ids[i]];
+            c.x[i] = p.x;
+            c.y[i] = p.y;
+        }
+        const int mat = m.elem_mat[e];
+        const double young = m.materials[3 * mat];
+        const double nu = m.materials[3 * mat + 1];
+        const double thick = m.materials[3 * mat + 2];
+        const int mode = m.mat_mode ? m.mat_mode[mat] : m.plane_mode;
+        double ke[2][8];
+        if (!ke_corner_rows(c, young, nu, thick, mode, a, ke)) {
+            if (a == 0) atomicAdd(failures, 1);  // once per element
+#pragma unroll
+            for (int j = 0; j < 8; ++j) ke[0][j] = ke[1][j] = 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ke_lds[j * kStride + k] = make_double2(ke[0][j], ke[1][j]);
+        en_lds[k] = nodes;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int q = 0; q < kRounds; ++q) {
+        const int g = gl + kNodesPerRound * q;
+        if (g >= nb) break;  // the same for every lane of a half-wavefront; later rounds too
+        const int cv = cols[q];
+        const int k0 = sp[g] - s0;
+        const int cnt = min(sp[g + 1] - s0, ns) - k0;
+        const int j = cv >> 1;
+        const int cp = cv & 1;
+        double2 sum = make_double2(0.0, 0.0);
+        // each slot adds the entries of the corner whose node is the column node (+0.0
+        // where none is, and for the unused places of a node with fewer slots: adding
+        // +0.0 to a sum that started from +0.0 changes nothing); branch-free, so the reads
+        // of all the slots are in flight together. A node with more slots, or an element
+        // that repeats a corner node, takes the general loop below.
+        constexpr int kFast = 6;
+        bool repeated = false;
+        if (cnt <= kFast) {
+            int at[kFast];
+#pragma unroll
+            for (int r = 0; r < kFast; ++r) {
+                const int k = r < cnt ? k0 + r : kBlockSlots;
+                const int4 en = en_lds[k];
+                const int mx = en.x == j, my = en.y == j, mz = en.z == j, mw = en.w == j;
+                repeated |= mx + my + mz + mw > 1;
+                const int bsel = mx ? 0 : (my ? 1 : (mz ? 2 : 3));
+                at[r] = (mx | my | mz | mw) ? (2 * bsel + cp) * kStride + k : kZero;
+            }
+#pragma unroll
+            for (int r = 0; r < kFast; ++r) {
+                const double2 v = ke_lds[at[r]];
+                sum.x += v.x;
+                sum.y += v.y;
+            }
+        }
+        if (cnt > kFast || repeated) {
+            sum = make_double2(0.0, 0.0);
+            for (int k = k0; k < k0 + cnt; ++k) {
+                const int4 en = en_lds[k];
+                const int ids[4] = {en.x, en.y, en.z, en.w};
+#pragma unroll
+                for (int bb = 0; bb < 4; ++bb) {
+                    if (ids[bb] == j) {
+                        const double2 v = ke_lds[(2 * bb + cp) * kStride + k];
+                        sum.x += v.x;
+                        sum.y += v.y;
+                    }
+                }
+            }
+        }
+        if (cv >= 0) {
+            values[rps[2 * g] + l] = sum.x;
+            values[rps[2 * g + 1] + l] = sum.y;
         }
     }
 }
@@ -560,6 +764,13 @@ void launch_assemble_rows(hipStream_t s, const MeshView& m, int n_nodes, const i
     if (n_nodes <= 0) return;
     assemble_rows_kernel<<<blocks_for(n_nodes, kAsmNodes), 256, 0, s>>>(m, n_nodes, row_ptr, node_ptr, node_list,
                                                                         emap, values, jacobian_failures);
+}
+
+void launch_assemble_block_rows(hipStream_t s, const MeshView& m, int n_nodes, const int* row_ptr, const int* col,
+                                const int* node_ptr, const int* node_list, double* values, int* jacobian_failures) {
+    if (n_nodes <= 0) return;
+    assemble_block_rows_kernel<<<blocks_for(n_nodes, kBlockNodes), 256, 0, s>>>(m, n_nodes, row_ptr, col, node_ptr,
+                                                                               node_list, values, jacobian_failures);
 }
 
 void launch_gather(hipStream_t s, long long nnz, const int* cptr, const int* contrib, const double* ke,

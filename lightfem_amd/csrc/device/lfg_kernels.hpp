@@ -116,6 +116,13 @@ void launch_ke(hipStream_t s, const MeshView& m, KeMode mode, const int* elems, 
 constexpr int kAssembleRowSlots = 32;
 void launch_assemble_rows(hipStream_t s, const MeshView& m, int n_nodes, const int* row_ptr, const int* node_ptr,
                           const int* node_list, const int* emap, double* values, int* jacobian_failures);
+// The same sums with each (element, corner) row pair formed once per block of 64 nodes
+// and the value positions taken from the pattern's columns; for meshes whose every block
+// of 64 consecutive nodes has at most kAssembleBlockSlots (element, corner) entries.
+constexpr int kAssembleBlockNodes = 64;
+constexpr int kAssembleBlockSlots = 272;
+void launch_assemble_block_rows(hipStream_t s, const MeshView& m, int n_nodes, const int* row_ptr, const int* col,
+                                const int* node_ptr, const int* node_list, double* values, int* jacobian_failures);
 // values[k] = sum of ke[contrib[j]] for j in [cptr[k], cptr[k+1]) in element order.
 void launch_gather(hipStream_t s, long long nnz, const int* cptr, const int* contrib, const double* ke,
                    double* values);

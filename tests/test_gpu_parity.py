@@ -95,6 +95,44 @@ def test_assembly_orders_on_long_rows(dev):
         assert np.array_equal(bits(dev.stiffness()), bits(colored_sum(ke, emap2, ptr, elems, nnz))), sectors
 
 
+def grid_mesh(nx: int, ny: int, copies: int = 1):
+    """nx x ny unit quads numbered row by row, each listed `copies` times in a row."""
+    x, y = np.meshgrid(np.arange(nx + 1, dtype=float), np.arange(ny + 1, dtype=float))
+    xy = np.column_stack([x.ravel() + 0.05 * np.sin(y.ravel()), y.ravel()])
+    quads = []
+    for j in range(ny):
+        for i in range(nx):
+            n0 = j * (nx + 1) + i
+            quads += [[n0, n0 + 1, n0 + nx + 2, n0 + nx + 1]] * copies
+    return xy, np.array(quads, dtype=np.int32)
+
+
+@pytest.mark.parametrize("case", ["fan7", "grid_doubled"])
+def test_block_row_assembly_paths(dev, case):
+    """The block-row assembly's general path (a node with 7 elements: more slots than the
+    branch-free sum takes) and the per-node-row kernel it falls back to (every element
+    listed twice: 64-node blocks with more (element, corner) slots than its LDS holds) keep
+    the reference's serial and coloured sums bit for bit."""
+    print(f"case {case}", flush=True)
+    xy, enode = fan_mesh(7) if case == "fan7" else grid_mesh(12, 9, copies=2)
+    n_nodes = xy.shape[0]
+    young = 1.0 + 0.1 * np.arange(enode.shape[0])
+    materials = np.column_stack([young, np.full_like(young, 0.3), np.ones_like(young)])
+    dev.set_mesh(xy, enode, np.arange(enode.shape[0]), materials, native.PLANE_STRAIN)
+    row_ptr, col, emap = native.csr_pattern(n_nodes, enode)
+    colors, ptr, elems = native.element_colors(n_nodes, enode)
+    dev.set_pattern(row_ptr, col, emap)
+    dev.set_colors(ptr, elems)
+    assert np.max(np.diff(row_ptr)) <= 32
+    ke = dev.element_stiffness()
+    nnz = col.shape[0]
+    emap2 = np.asarray(emap).reshape(-1, 64)
+    dev.assemble("serial")
+    assert np.array_equal(bits(dev.stiffness()), bits(lfo.assemble_serial(ke, emap2, nnz)))
+    dev.assemble("coloring")
+    assert np.array_equal(bits(dev.stiffness()), bits(colored_sum(ke, emap2, ptr, elems, nnz)))
+
+
 @pytest.mark.parametrize("name", FULL_CASES)
 def test_stages_match_reference(dev, golden_index, name):
     g = load_golden(name)
