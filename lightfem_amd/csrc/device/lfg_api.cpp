@@ -117,6 +117,7 @@ struct lfg_ctx {
     int n = 0;
     int max_row = 0;
     int max_block_slots = 0;  // most node-list entries of one 64-node block (block-row assembly)
+    int grid_w = 0, grid_h = 0;  // elements per row and rows of a row-numbered element grid, else 0
     long long nnz = 0;
     std::vector<int> h_row_ptr, h_col;
     DevArray<int> row_ptr, col, emap, cptr, contrib;
@@ -241,6 +242,27 @@ void build_node_elements(lfg_ctx* c) {
         const int n1 = std::min(n_nodes, n0 + lfg::kAssembleBlockNodes);
         c->max_block_slots = std::max(c->max_block_slots, ptr[n1] - ptr[n0]);
     }
+}
+
+// A Q4 mesh whose elements form a W x H grid numbered row by row with counter-clockwise
+// corners from the lower left (element ey W + ex: nodes n, n + 1, n + W + 2, n + W + 1,
+// n = ey (W + 1) + ex) sets grid_w / grid_h; any other mesh leaves them 0.
+void detect_element_grid(lfg_ctx* c) {
+    c->grid_w = c->grid_h = 0;
+    const long long n_e = c->n_elems;
+    if (n_e == 0) return;
+    const int* en = c->h_enodes.data();
+    const long long w = static_cast<long long>(en[3]) - en[0] - 1;
+    if (w <= 0 || n_e % w != 0 || en[0] != 0) return;
+    const long long h = n_e / w;
+    if ((w + 1) * (h + 1) != c->n_nodes) return;
+    for (long long e = 0; e < n_e; ++e) {
+        const long long n = (e / w) * (w + 1) + e % w;
+        if (en[4 * e] != n || en[4 * e + 1] != n + 1 || en[4 * e + 2] != n + w + 2 || en[4 * e + 3] != n + w + 1)
+            return;
+    }
+    c->grid_w = static_cast<int>(w);
+    c->grid_h = static_cast<int>(h);
 }
 
 lfg::CsrView csr_view(const lfg_ctx* c) { return lfg::CsrView{c->n, c->nnz, c->row_ptr.p, c->col.p, c->max_row}; }
@@ -1159,6 +1181,7 @@ int lfg_set_mesh(lfg_ctx* ctx, int64_t n_nodes, const double* xy, int64_t n_elem
         ctx->materials.upload(materials, 3 * static_cast<size_t>(n_materials), ctx->stream);
         ctx->mat_mode.release();
         ctx->h_enodes.assign(enodes, enodes + 4 * n_elems);
+        detect_element_grid(ctx);
         ctx->n_bars = 0;
         ctx->h_bar_nodes.clear();
         build_node_elements(ctx);
@@ -1430,8 +1453,12 @@ int lfg_assemble(lfg_ctx* ctx, int32_t strategy, lfg_assembly_report* report) {
         } else if (strategy == LFG_ASSEMBLY_ATOMIC) {
             LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
             lfg::launch_fill(ctx->stream, ctx->K.p, ctx->nnz, 0.0);
-            lfg::launch_ke(ctx->stream, m, lfg::KE_ATOMIC, nullptr, ctx->n_elems, ctx->emap.p, ctx->K.p, nullptr,
-                           ctx->flags.p);
+            if (ctx->grid_w > 0 && ctx->n_bars == 0 && ctx->max_row <= lfg::kAssembleGridRow)
+                lfg::launch_assemble_atomic_tiles(ctx->stream, m, ctx->grid_w, ctx->grid_h, ctx->row_ptr.p,
+                                                  ctx->emap.p, ctx->K.p, ctx->flags.p);
+            else
+                lfg::launch_ke(ctx->stream, m, lfg::KE_ATOMIC, nullptr, ctx->n_elems, ctx->emap.p, ctx->K.p, nullptr,
+                               ctx->flags.p);
             lfg::launch_bar_ke(ctx->stream, m, bv, lfg::KE_ATOMIC, nullptr, ctx->n_bars, ctx->K.p, nullptr,
                                ctx->flags.p + 3);
             LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));

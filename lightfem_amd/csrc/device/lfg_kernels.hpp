@@ -121,6 +121,13 @@ void launch_assemble_rows(hipStream_t s, const MeshView& m, int n_nodes, const i
 // of 64 consecutive nodes has at most kAssembleBlockSlots (element, corner) entries.
 constexpr int kAssembleBlockNodes = 64;
 constexpr int kAssembleBlockSlots = 272;
+// Atomic assembly of a W x H element grid numbered row by row (element ey W + ex, nodes
+// n, n + 1, n + W + 2, n + W + 1 with n = ey (W + 1) + ex; rows of at most
+// kAssembleGridRow values), with K zeroed: 8 x 8-element tiles summed in LDS, stores for
+// the rows a tile completes and FP64 atomics for the others.
+constexpr int kAssembleGridRow = 18;
+void launch_assemble_atomic_tiles(hipStream_t s, const MeshView& m, int grid_w, int grid_h, const int* row_ptr,
+                                  const int* emap, double* values, int* jacobian_failures);
 void launch_assemble_block_rows(hipStream_t s, const MeshView& m, int n_nodes, const int* row_ptr, const int* col,
                                 const int* node_ptr, const int* node_list, double* values, int* jacobian_failures);
 // values[k] = sum of ke[contrib[j]] for j in [cptr[k], cptr[k+1]) in element order.

@@ -107,14 +107,17 @@ def grid_mesh(nx: int, ny: int, copies: int = 1):
     return xy, np.array(quads, dtype=np.int32)
 
 
-@pytest.mark.parametrize("case", ["fan7", "grid_doubled"])
+@pytest.mark.parametrize("case", ["fan7", "grid_doubled", "grid_tiles"])
 def test_block_row_assembly_paths(dev, case):
     """The block-row assembly's general path (a node with 7 elements: more slots than the
     branch-free sum takes) and the per-node-row kernel it falls back to (every element
     listed twice: 64-node blocks with more (element, corner) slots than its LDS holds) keep
-    the reference's serial and coloured sums bit for bit."""
+    the reference's serial and coloured sums bit for bit; on a 13 x 11 element grid (tiles
+    of 8 x 8 elements cut at both edges) the tiled atomic assembly is within 1e-13 of the
+    serial sums, as is the element-wise atomic kernel on the doubled grid and the fan."""
     print(f"case {case}", flush=True)
-    xy, enode = fan_mesh(7) if case == "fan7" else grid_mesh(12, 9, copies=2)
+    xy, enode = {"fan7": lambda: fan_mesh(7), "grid_doubled": lambda: grid_mesh(12, 9, copies=2),
+                 "grid_tiles": lambda: grid_mesh(13, 11)}[case]()
     n_nodes = xy.shape[0]
     young = 1.0 + 0.1 * np.arange(enode.shape[0])
     materials = np.column_stack([young, np.full_like(young, 0.3), np.ones_like(young)])
@@ -131,6 +134,10 @@ def test_block_row_assembly_paths(dev, case):
     assert np.array_equal(bits(dev.stiffness()), bits(lfo.assemble_serial(ke, emap2, nnz)))
     dev.assemble("coloring")
     assert np.array_equal(bits(dev.stiffness()), bits(colored_sum(ke, emap2, ptr, elems, nnz)))
+    serial = lfo.assemble_serial(ke, emap2, nnz)
+    dev.assemble("atomic")
+    atomic = dev.stiffness()
+    assert np.max(np.abs(atomic - serial)) <= 1e-13 * np.max(np.abs(serial))
 
 
 @pytest.mark.parametrize("name", FULL_CASES)
