@@ -39,7 +39,8 @@ KERNELS = {1: "pcg_spmv_kernel<4>", 2: "pcg_spmv_kernel<8>", 3: "pcg_spmv_kernel
            4: "pcg_spmv_stream_kernel<24, 64>", 5: "pcg_spmv_stream_kernel<24, 32>",
            6: "pcg_spmv_slices_kernel<9>", 7: "pcg_spmv_slices_dir_kernel<9, 0>",
            8: "pcg_spmv_stencil_kernel<4, true>", 9: "pcg_spmv_stencil_kernel<9, false>",
-           10: "pcg_fused_stencil_kernel<4, 0>", 11: "pcg_fused_grid_kernel<0, 4>"}
+           10: "pcg_fused_stencil_kernel<4, 0>", 11: "pcg_fused_grid_kernel<0, 4>",
+           12: "pcg_fused_slices_kernel<9, 0>", 13: "pcg_slices_multiply_kernel<9, 0>"}
 
 
 def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int, layout: dict) -> int:
@@ -66,9 +67,17 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int, layout: dict) -
         return matrix + 16 * n
     if variant == 9:
         return n_nodes * layout["stencil_exact_planes"] * 32 + 16 * n
-    if variant in (6, 7):
+    if variant in (6, 7, 12, 13):
         n_slices = (n // 2 + 63) // 64
         matrix = nnz * value_bytes + 4 * (nnz // 4) + 4 * (n_slices + 1)
+        if variant == 12:
+            # the whole Jacobi iteration as in 10 and 11 (the column nodes' r, q, p are
+            # gathered from the XCD's L2; each counted once): 136 B per node
+            return matrix + n_nodes * 136
+        if variant == 13:
+            # the vector launch (r, q, p, M^-1 read, r', p' written, x every second launch:
+            # 120 B per node) and the multiply (p' gathered, r', M^-1 read, q' written: 64 B)
+            return matrix + n_nodes * 184
         return matrix + (40 * n if variant == 7 else 16 * n)
     return nnz * (value_bytes + 4) + 4 * (n + 1) + 16 * n
 
@@ -363,7 +372,8 @@ def main() -> int:
     layout = dev.layout_info()
     # the library's default Jacobi iteration: fused stencil, stencil multiply, node slices
     default_variant = 11 if layout["grid_row_len"] else (
-        10 if layout["fused_window_pairs"] else (8 if layout["stencil_sym_planes"] else 6))
+        10 if layout["fused_window_pairs"] else (8 if layout["stencil_sym_planes"] else (
+            12 if layout["n_slices"] <= 1024 else 13)))
     if args.profile:
         # Profiling run: the FP64 SpMV of the headline variant alone (args.profile launches),
         # then FP64 Jacobi-PCG iterations (a whole number of captured graphs), no other kernels.
@@ -392,8 +402,9 @@ def main() -> int:
     spmv, sweep, iteration_kernel = {}, {}, {}
     for precision, vbytes in (("fp64", 8), ("mixed", 4)):
         sweep[precision] = {}
-        for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11):
-            if (variant == 8 and not layout["stencil_sym_planes"]) or (variant == 9 and (
+        for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13):
+            if (variant in (6, 7, 12, 13) and not layout["node_blocked"]) or (
+                    variant == 8 and not layout["stencil_sym_planes"]) or (variant == 9 and (
                     precision == "mixed" or not layout["stencil_exact_planes"])) or (
                     variant == 10 and not layout["fused_window_pairs"]) or (
                     variant == 11 and not layout["grid_row_len"]):
@@ -404,7 +415,7 @@ def main() -> int:
                                          "GBs": b / (ms * 1e-3) / 1e9}
         # the multiply alone (the stand-alone kernel of the solve's layout) and the kernel the
         # PCG iteration spends its time in (the same, or the fused iteration)
-        multiply = 8 if solve_variant in (10, 11) else solve_variant
+        multiply = 8 if solve_variant in (10, 11) else (6 if solve_variant in (12, 13) else solve_variant)
         spmv[precision] = dict(sweep[precision][multiply], variant=multiply)
         iteration_kernel[precision] = dict(sweep[precision][solve_variant], variant=solve_variant)
     print(f"bench: spmv {json.dumps(spmv)}", file=sys.stderr, flush=True)

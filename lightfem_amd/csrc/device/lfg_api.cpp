@@ -285,15 +285,26 @@ constexpr int kStencilSym = 8;    // node-stencil planes, symmetric form (diagon
 constexpr int kStencilExact = 9;  // node-stencil planes, every offset (CSR column order)
 constexpr int kStencilFused = 10; // symmetric node stencil, whole Jacobi / block PCG iteration in one launch
 constexpr int kStencilGrid = 11;  // the iteration of 10 on 2D tiles (row-structured stencils)
+constexpr int kSliceIter = 12;    // node slices, whole Jacobi / block PCG iteration in one launch
+constexpr int kSliceSplit = 13;   // the iteration of 12 in two launches (vector update, multiply)
+constexpr int kMaxVariant = kSliceSplit;
+constexpr int kSliceIterMaxSlices = 1024;  // above this the split iteration (13) is the faster default
 
-bool slice_layout(int variant) { return variant == kSliceVariant || variant == kSliceFused; }
+bool slice_layout(int variant) {
+    return variant == kSliceVariant || variant == kSliceFused || variant == kSliceIter || variant == kSliceSplit;
+}
 bool stencil_layout(int variant) {
     return variant == kStencilSym || variant == kStencilExact || variant == kStencilFused || variant == kStencilGrid;
 }
-// The whole Jacobi / block iteration in one launch (1D or 2D tiles).
-bool fused_iteration(int variant) { return variant == kStencilFused || variant == kStencilGrid; }
+// The whole Jacobi / block iteration in one launch (1D or 2D stencil tiles, node slices).
+bool fused_iteration(int variant) {
+    return variant == kStencilFused || variant == kStencilGrid || variant == kSliceIter || variant == kSliceSplit;
+}
 bool grid_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stencil_sym.row_len > 0; }
 bool fused_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stencil_sym.win_total > 0; }
+// The fused node-slice iteration for Jacobi / block on a node-blocked pattern: one launch
+// per iteration for small and medium systems, two for large ones.
+int slice_iteration(const lfg_ctx* c) { return c->n_slices <= kSliceIterMaxSlices ? kSliceIter : kSliceSplit; }
 
 // 0 selects, in order: the fused stencil iteration (Jacobi and block preconditioners),
 // the symmetric node-stencil multiply, the node-slice multiply, row-segment streaming.
@@ -307,18 +318,24 @@ int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JAC
                                           ? kStencilGrid
                                           : (fused_stencil_ok(c) && !sweep_precond(precond)
                                                  ? kStencilFused
-                                                 : (c->have_stencil_sym ? kStencilSym : kSliceVariant)))
+                                                 : (c->have_stencil_sym ? kStencilSym
+                                                                        : (sweep_precond(precond) ? kSliceVariant
+                                                                                                  : slice_iteration(c)))))
                                    : variant;
-        require(v >= kSliceVariant && v <= kStencilGrid, LFG_ERR_UNSUPPORTED,
-                "block storage runs on the node-slice (6, 7) and node-stencil (8-10) SpMVs");
+        require(v >= kSliceVariant && v <= kMaxVariant, LFG_ERR_UNSUPPORTED,
+                "block storage runs on the node-slice (6, 7, 12) and node-stencil (8-11) SpMVs");
         return resolve_variant(c, v, precond, false);
     }
     if (variant == 0) {
         if (grid_stencil_ok(c) && !sweep_precond(precond)) return kStencilGrid;
         if (fused_stencil_ok(c) && !sweep_precond(precond)) return kStencilFused;
-        return c->have_stencil_sym ? kStencilSym : (c->have_slices ? kSliceVariant : 4);
+        if (c->have_stencil_sym) return kStencilSym;
+        if (c->have_slices) return sweep_precond(precond) ? kSliceVariant : slice_iteration(c);
+        return 4;
     }
-    require(variant >= 1 && variant <= kStencilGrid, LFG_ERR_ARG, "SpMV variant must be 0-11");
+    require(variant >= 1 && variant <= kMaxVariant, LFG_ERR_ARG, "SpMV variant must be 0-13");
+    require((variant != kSliceIter && variant != kSliceSplit) || !sweep_precond(precond), LFG_ERR_UNSUPPORTED,
+            "the fused node-slice iteration applies the jacobi and 2x2 block preconditioners, not ic0 or subdomains");
     require(variant != kStencilGrid || grid_stencil_ok(c), LFG_ERR_UNSUPPORTED,
             "the 2D-tiled fused iteration needs a symmetric node stencil with positive offsets 1, L-1, L, L+1 "
             "(L >= 64 nodes per mesh row)");
@@ -1068,6 +1085,12 @@ void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int repl
         double* const rb[2] = {c->r.p, c->r_alt.p};
         double* const pb[2] = {c->p.p, c->p2.p};
         double* const qb[2] = {c->q.p, c->q_alt.p};
+        if (variant == kSliceIter || variant == kSliceSplit) {
+            (variant == kSliceIter ? lfg::launch_pcg_fused_slices : lfg::launch_pcg_split_slices)(
+                c->stream, b, slice_view(c), c->slices64.p, mixed ? c->slices32.p : nullptr, rb[parity],
+                rb[parity ^ 1], pb[parity], pb[parity ^ 1], qb[parity], qb[parity ^ 1]);
+            return;
+        }
         (variant == kStencilGrid ? lfg::launch_pcg_fused_grid : lfg::launch_pcg_fused_stencil)(
             c->stream, b, c->stencil_sym, c->stencil64.p, mixed ? c->stencil32.p : nullptr,
             mixed ? c->stencil_diag.p : nullptr, rb[parity], rb[parity ^ 1], pb[parity], pb[parity ^ 1], qb[parity],
@@ -2074,7 +2097,10 @@ int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t rep
             ensure_vectors(ctx);
             if (ctx->precond_ready != LFG_PRECOND_JACOBI) setup_preconditioner(ctx, LFG_PRECOND_JACOBI, &degenerate);
             const bool f32 = precision == LFG_PRECISION_MIXED;
-            ensure_stencil(ctx, v, !f32, f32);
+            if (slice_layout(v))
+                ensure_slices(ctx, !f32, f32);
+            else
+                ensure_stencil(ctx, v, !f32, f32);
             ctx->p2.alloc(static_cast<size_t>(ctx->n));
             ctx->r_alt.alloc(static_cast<size_t>(ctx->n));
             ctx->q_alt.alloc(static_cast<size_t>(ctx->n));

@@ -27,6 +27,7 @@ struct PcgState {
     double switch_rr;  // mixed: FP32 values once ||r||^2 <= switch_rr
     double true_rr;    // mixed: ||r||^2 at the last replacement
     double alpha_prev; // fused stencil iteration: the alpha before `alpha` (deferred x update)
+    double rr_new;     // split node-slice iteration: ||r'||^2 of the vector launch (its r'.z in rz_new)
     long long iter;
     long long max_iter;
     long long replacements;
@@ -96,12 +97,40 @@ __device__ __forceinline__ double read_partial(const double* slot) {
 // and the last of the eight group finishers folds those. Returns true in every
 // thread of that final block, with the totals in `out` of thread 0. The fold
 // order is fixed, so results are bit-reproducible run to run. `partials` holds
-// (gridDim.x + 8) * K doubles.
+// (gridDim.x + 8) * K doubles. A grid of at most kOneLevelGrid blocks takes one level
+// (one counter, the last block folds every partial): three dependent memory round trips
+// fewer, which is what a small system's iteration waits for, and contention on one
+// counter that so few blocks cannot make costly.
+constexpr int kOneLevelGrid = 256;
 template <int NT, int K>
 __device__ __forceinline__ bool finish_reduction(TicketSet* t, double* partials, const double* vals, double* out,
                                                  double* sh) {
     __shared__ int flag;
     const int nb = gridDim.x;
+    if (nb <= kOneLevelGrid) {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) publish_partial(partials + blockIdx.x * K + k, vals[k]);
+            stores_complete();
+            const unsigned int a = __hip_atomic_fetch_add(&t->top[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            flag = (a == static_cast<unsigned int>(nb - 1)) ? 1 : 0;
+        }
+        __syncthreads();
+        if (!flag) return false;
+        double s[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double acc = 0.0;
+            for (int i = threadIdx.x; i < nb; i += NT) acc += read_partial(partials + i * K + k);
+            s[k] = block_sum<NT>(acc, sh);
+        }
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) out[k] = s[k];
+            __hip_atomic_store(&t->top[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return true;
+    }
     const int x = blockIdx.x & 7;
     const int per = nb >> 3;
     double* group = partials + static_cast<long long>(nb) * K;

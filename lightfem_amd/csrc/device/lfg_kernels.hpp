@@ -240,6 +240,16 @@ void launch_pcg_fused_flush(hipStream_t s, const PcgBuffers& b, const double* p_
 void launch_pcg_fused_grid(hipStream_t s, const PcgBuffers& b, const StencilView& v, const double* planes64,
                            const float* planes32, const double* diag_c, const double* r_in, double* r_out,
                            const double* p_in, double* p_out, const double* q_in, double* q_out);
+// Variant 12: the same iteration on the node-slice layout (any node-blocked pattern); p' of
+// the column nodes is formed from r_in, q_in, p_in as the owner forms it.
+void launch_pcg_fused_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
+                             const float* planes32, const double* r_in, double* r_out, const double* p_in,
+                             double* p_out, const double* q_in, double* q_out);
+// Variant 13: the same iteration in two launches on the node slices (the vector update,
+// then the multiply with the state update); same buffers.
+void launch_pcg_split_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
+                             const float* planes32, const double* r_in, double* r_out, const double* p_in,
+                             double* p_out, const double* q_in, double* q_out);
 // x += alpha p, r -= alpha q, ||r||^2, r.z; convergence / replacement decision, beta.
 void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed);
 // mixed only: r = b - K x in FP64 when a replacement is due; then r.z and beta.

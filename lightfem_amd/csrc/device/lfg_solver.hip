@@ -12,6 +12,7 @@
 // kernel returns immediately once the device state says the solve is done.
 #include "lfg_kernels.hpp"
 #include "lfg_precond.hpp"
+#include "lfg_slices.hpp"
 
 #include "lightfem_gpu.h"
 
@@ -559,16 +560,6 @@ __global__ __launch_bounds__(256) void pack_slices_kernel(int n_nodes, int n_sli
     }
 }
 
-// FP32 copy: the diagonal block (FP64, compensated) times the node's own pair, added
-// after the off-diagonal blocks.
-__device__ __forceinline__ void add_diag32(const SliceView& v, int node, dpair x, double& a0, double& a1) {
-    const dpair du = reinterpret_cast<const dpair*>(v.diag32)[2 * node];
-    const dpair dl = reinterpret_cast<const dpair*>(v.diag32)[2 * node + 1];
-    a0 += du.x * x.x;
-    a0 += du.y * x.y;
-    a1 += dl.x * x.x;
-    a1 += dl.y * x.y;
-}
 
 // Rows 2i, 2i+1 of node i = 64s + lane: every load instruction of the wave moves 64
 // consecutive blocks of a plane (or columns), p is gathered as one pair per block,

@@ -10,6 +10,7 @@
 // L2 (each XCD walks one contiguous node range), so HBM sees each value once.
 #include "lfg_kernels.hpp"
 #include "lfg_precond.hpp"
+#include "lfg_slices.hpp"
 
 #include "lightfem_gpu.h"
 
@@ -333,17 +334,6 @@ __device__ __forceinline__ void block_sums(double (&v)[K], double* sh) {
         }
     }
     __syncthreads();
-}
-
-// p' = M^-1 (r - alpha q) + beta p for node j (solver.cpp:599-604, 612-620).
-template <int Prec>
-__device__ __forceinline__ dpair next_direction(const double* __restrict__ m, const dpair* __restrict__ r2,
-                                                const dpair* __restrict__ q2, const dpair* __restrict__ p2, int j,
-                                                double alpha, double beta) {
-    const dpair ro = r2[j], qo = q2[j], po = p2[j];
-    double z0, z1;
-    precond_node<Prec>(m, j, ro.x - alpha * qo.x, ro.y - alpha * qo.y, z0, z1);
-    return dpair{z0 + beta * po.x, z1 + beta * po.y};
 }
 
 // The matrix values one node's rows need in the symmetric form (kM positive offsets).
@@ -793,6 +783,312 @@ __global__ __launch_bounds__(R * kWave) void pcg_fused_grid_kernel(
     if (threadIdx.x == 0) fused_state_update(b.st, tot, alpha);
 }
 
+// ---- the fused iteration on the node-slice layout (variant 12)
+//
+// The iteration of pcg_fused_stencil_kernel for any node-blocked pattern (RCM or other
+// numberings, where the blocks lie at no fixed node offsets). A wavefront owns one slice of
+// 64 nodes (lane l: node 64 s + l). The lane updates its node (x deferred, r' = r - alpha q,
+// z = M^-1 r', p' = z + beta p), then forms q' = A p' for the node's two rows over the
+// slice's blocks in column order, as slice_rows does: p' of each column node j is formed
+// from the launch's inputs r, q, p and the preconditioner of j (no block writes them during
+// the launch) with the owner's arithmetic, so it has the bits the owner writes and no
+// exchange between blocks is needed. The column gathers hit the XCD's L2: slices are
+// XCD-remapped in contiguous ranges and a node's columns lie within the numbering's
+// bandwidth.
+template <typename VT, int kW, int Prec>
+__device__ __forceinline__ void fused_slice(const PcgBuffers& b, const SliceView& v, const VT* __restrict__ planes,
+                                            int s, const double* __restrict__ r_in, double* __restrict__ r_out,
+                                            const double* __restrict__ p_in, double* __restrict__ p_out,
+                                            const double* __restrict__ q_in, double alpha, double beta,
+                                            double alpha_prev, int x_mode, double (&acc)[5], int& qi, dpair& qv) {
+    using P = typename PairOf<VT>::type;
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int node = s * kWave + lane;
+    const bool own = node < (b.n >> 1);
+    const int ic = own ? node : 0;
+    dpair* __restrict__ x2 = reinterpret_cast<dpair*>(b.x);
+    const dpair ro = reinterpret_cast<const dpair*>(r_in)[ic], qo = reinterpret_cast<const dpair*>(q_in)[ic],
+                po = reinterpret_cast<const dpair*>(p_in)[ic];
+    NodePrecond<Prec> mo;
+    mo.load(b.m, ic);
+    dpair xo = {0.0, 0.0}, pp = {0.0, 0.0};
+    if (x_mode != 0) {
+        xo = x2[ic];
+        if (x_mode == 2) pp = reinterpret_cast<const dpair*>(p_out)[ic];
+    }
+    const int begin = v.ptr[s];
+    const int width = (v.ptr[s + 1] - begin) / kWave;
+    const P* __restrict__ upper = reinterpret_cast<const P*>(planes);
+    const P* __restrict__ lower = upper + v.blocks;
+    // p' of column node j from the launch's inputs, with the owner's arithmetic
+    auto column = [&](int j) {
+        const dpair rj = reinterpret_cast<const dpair*>(r_in)[j], qj = reinterpret_cast<const dpair*>(q_in)[j],
+                    pj = reinterpret_cast<const dpair*>(p_in)[j];
+        double z0, z1;
+        precond_node<Prec>(b.m, j, rj.x - alpha * qj.x, rj.y - alpha * qj.y, z0, z1);
+        return dpair{z0 + beta * pj.x, z1 + beta * pj.y};
+    };
+    // 1. own node (solver.cpp:599-604, 612-620), x two iterations at a time
+    dpair pn = {0.0, 0.0}, zn = {0.0, 0.0};
+    if (own) {
+        if (x_mode != 0) {
+            if (x_mode == 2) {
+                xo.x += alpha_prev * pp.x;
+                xo.y += alpha_prev * pp.y;
+            }
+            xo.x += alpha * po.x;
+            xo.y += alpha * po.y;
+            x2[node] = xo;
+        }
+        const dpair rn = {ro.x - alpha * qo.x, ro.y - alpha * qo.y};
+        double z0, z1;
+        mo.apply(rn.x, rn.y, z0, z1);
+        zn = dpair{z0, z1};
+        pn = dpair{z0 + beta * po.x, z1 + beta * po.y};
+        reinterpret_cast<dpair*>(r_out)[node] = rn;
+        reinterpret_cast<dpair*>(p_out)[node] = pn;
+        acc[0] += rn.x * rn.x + rn.y * rn.y;
+        acc[1] += rn.x * z0 + rn.y * z1;
+    }
+    // 2. q' = A p' in column order (sparse.cpp:104-120; block order with v.pair_sum)
+    double a0 = 0.0, a1 = 0.0;
+    if (kW > 0 && width > 0 && width <= kW) {
+        P u[kW > 0 ? kW : 1], l[kW > 0 ? kW : 1];
+        int c[kW > 0 ? kW : 1];
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            const int at = begin + min(k, width - 1) * kWave + lane;
+            u[k] = __builtin_nontemporal_load(upper + at);
+            l[k] = __builtin_nontemporal_load(lower + at);
+            c[k] = __builtin_nontemporal_load(v.col + at);
+        }
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            if (k < width) {
+                const dpair x = column(c[k]);
+                add_block(a0, a1, static_cast<double>(u[k].x), static_cast<double>(u[k].y),
+                          static_cast<double>(l[k].x), static_cast<double>(l[k].y), x, v.pair_sum != 0);
+            }
+        }
+    } else {
+        for (int k = 0; k < width; ++k) {
+            const int at = begin + k * kWave + lane;
+            const P uk = __builtin_nontemporal_load(upper + at);
+            const P lk = __builtin_nontemporal_load(lower + at);
+            const dpair x = column(__builtin_nontemporal_load(v.col + at));
+            add_block(a0, a1, static_cast<double>(uk.x), static_cast<double>(uk.y), static_cast<double>(lk.x),
+                      static_cast<double>(lk.y), x, v.pair_sum != 0);
+        }
+    }
+    if (own) {
+        if (sizeof(VT) == sizeof(float)) add_diag32(v, node, pn, a0, a1);
+        if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;  // an earlier slice of this wavefront
+        qi = node;
+        qv = dpair{a0, a1};
+        double m0, m1;
+        mo.apply(a0, a1, m0, m1);
+        acc[2] += pn.x * a0 + pn.y * a1;
+        acc[3] += a0 * zn.x + a1 * zn.y;
+        acc[4] += a0 * m0 + a1 * m1;
+    }
+}
+
+// One fused PCG iteration per launch on the node slices (launch_pcg_fused_slices): each
+// block takes a contiguous, XCD-remapped range of slices, one per wavefront and round; the
+// sums, state update and q' store as in pcg_fused_grid_kernel (b.q is q_out).
+template <int kW, int Prec>
+__global__ __launch_bounds__(kThreads) void pcg_fused_slices_kernel(
+    PcgBuffers b, SliceView v, const double* __restrict__ planes64, const float* __restrict__ planes32,
+    const double* __restrict__ r_in, double* __restrict__ r_out, const double* __restrict__ p_in,
+    double* __restrict__ p_out, const double* __restrict__ q_in) {
+    if (b.st->done) return;
+    __shared__ double sh[5 * (kThreads / kWave)];
+    const double alpha = b.st->alpha, beta = b.st->beta, alpha_prev = b.st->alpha_prev;
+    const long long launch = b.st->iter + 1;  // x update deferred as in pcg_fused_stencil_kernel
+    const int x_mode = (launch & 1) ? 0 : (launch >= 2 ? 2 : 1);
+    const bool use32 = planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
+    constexpr int kWaves = kThreads / kWave;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = (v.n_slices + gridDim.x - 1) / gridDim.x;
+    const int start = lb * per_block;
+    const int stop = min(start + per_block, v.n_slices);
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // r'.r', r'.z, p'.q', q'.z, q'.M^-1 q'
+    int qi = -1;
+    dpair qv = {0.0, 0.0};
+    for (int s = start + static_cast<int>(threadIdx.x) / kWave; s < stop; s += kWaves) {
+        if (use32)
+            fused_slice<float, kW, Prec>(b, v, planes32, s, r_in, r_out, p_in, p_out, q_in, alpha, beta, alpha_prev,
+                                         x_mode, acc, qi, qv);
+        else
+            fused_slice<double, kW, Prec>(b, v, planes64, s, r_in, r_out, p_in, p_out, q_in, alpha, beta, alpha_prev,
+                                          x_mode, acc, qi, qv);
+    }
+    block_sums<kThreads, 5>(acc, sh);
+    double tot[5];
+    const bool last = finish_reduction<kThreads, 5>(b.tickets + 1, b.partials, acc, tot, sh);
+    if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;
+    if (!last) return;
+    if (threadIdx.x == 0) fused_state_update(b.st, tot, alpha);
+}
+
+// ---- the same iteration in two launches (variant 13): for node slices of large systems,
+// where forming every column's p' in the multiply (variant 12) costs more L2 traffic than
+// one more launch.
+//
+// Vector launch: x (deferred), r' = r - alpha q, z = M^-1 r', p' = z + beta p for every node,
+// ||r'||^2 and r'.z into the state. Its blocks take the nodes of the multiply launch's blocks
+// (256 per block, the same XCD remapping), so a block's r', p' stay in its XCD's L2.
+template <int Prec>
+__global__ __launch_bounds__(kThreads) void pcg_slices_vector_kernel(PcgBuffers b, const double* __restrict__ r_in,
+                                                                     double* __restrict__ r_out,
+                                                                     const double* __restrict__ p_in,
+                                                                     double* __restrict__ p_out,
+                                                                     const double* __restrict__ q_in) {
+    if (b.st->done) return;
+    __shared__ double sh[2 * (kThreads / kWave)];
+    const double alpha = b.st->alpha, beta = b.st->beta, alpha_prev = b.st->alpha_prev;
+    const long long launch = b.st->iter + 1;  // x update deferred as in pcg_fused_stencil_kernel
+    const int x_mode = (launch & 1) ? 0 : (launch >= 2 ? 2 : 1);
+    const int n_nodes = b.n >> 1;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = ((n_nodes + kThreads - 1) / kThreads + gridDim.x - 1) / gridDim.x * kThreads;
+    const int stop = min((lb + 1) * per_block, n_nodes);
+    double acc[2] = {0.0, 0.0};
+    for (int node = lb * per_block + static_cast<int>(threadIdx.x); node < stop; node += kThreads) {
+        const dpair ro = reinterpret_cast<const dpair*>(r_in)[node];
+        const dpair qo = reinterpret_cast<const dpair*>(q_in)[node];
+        const dpair po = reinterpret_cast<const dpair*>(p_in)[node];
+        NodePrecond<Prec> mo;
+        mo.load(b.m, node);
+        if (x_mode != 0) {  // solver.cpp:599-601, two iterations' terms in their order
+            dpair xo = reinterpret_cast<const dpair*>(b.x)[node];
+            if (x_mode == 2) {
+                const dpair pp = reinterpret_cast<const dpair*>(p_out)[node];
+                xo.x += alpha_prev * pp.x;
+                xo.y += alpha_prev * pp.y;
+            }
+            xo.x += alpha * po.x;
+            xo.y += alpha * po.y;
+            reinterpret_cast<dpair*>(b.x)[node] = xo;
+        }
+        const dpair rn = {ro.x - alpha * qo.x, ro.y - alpha * qo.y};
+        double z0, z1;
+        mo.apply(rn.x, rn.y, z0, z1);
+        reinterpret_cast<dpair*>(r_out)[node] = rn;
+        reinterpret_cast<dpair*>(p_out)[node] = dpair{z0 + beta * po.x, z1 + beta * po.y};
+        acc[0] += rn.x * rn.x + rn.y * rn.y;
+        acc[1] += rn.x * z0 + rn.y * z1;
+    }
+    block_sums<kThreads, 2>(acc, sh);
+    double tot[2];
+    if (!finish_reduction<kThreads, 2>(b.tickets + 2, b.partials, acc, tot, sh)) return;
+    if (threadIdx.x == 0) {
+        b.st->rr_new = tot[0];
+        b.st->rz_new = tot[1];
+    }
+}
+
+// Multiply launch: q' = A p' over the node slices (the rows of slice_rows, p' gathered from
+// p_out), p'.q', q'.z and q'.M^-1 q' with z = M^-1 r' formed again from r' (the vector
+// launch's arithmetic), then the fused iteration's state update.
+template <typename VT, int kW, int Prec>
+__device__ __forceinline__ void split_slice(const PcgBuffers& b, const SliceView& v, const VT* __restrict__ planes,
+                                            int s, const double* __restrict__ r_new,
+                                            const double* __restrict__ p_new, double (&acc)[3], int& qi, dpair& qv) {
+    using P = typename PairOf<VT>::type;
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int node = s * kWave + lane;
+    const bool own = node < (b.n >> 1);
+    const int ic = own ? node : 0;
+    const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(p_new);
+    const dpair pn = p2[ic];
+    const dpair rn = reinterpret_cast<const dpair*>(r_new)[ic];
+    NodePrecond<Prec> mo;
+    mo.load(b.m, ic);
+    const int begin = v.ptr[s];
+    const int width = (v.ptr[s + 1] - begin) / kWave;
+    const P* __restrict__ upper = reinterpret_cast<const P*>(planes);
+    const P* __restrict__ lower = upper + v.blocks;
+    double a0 = 0.0, a1 = 0.0;
+    if (kW > 0 && width > 0 && width <= kW) {
+        P u[kW > 0 ? kW : 1], l[kW > 0 ? kW : 1];
+        int c[kW > 0 ? kW : 1];
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            const int at = begin + min(k, width - 1) * kWave + lane;
+            u[k] = __builtin_nontemporal_load(upper + at);
+            l[k] = __builtin_nontemporal_load(lower + at);
+            c[k] = __builtin_nontemporal_load(v.col + at);
+        }
+        dpair x[kW > 0 ? kW : 1];
+#pragma unroll
+        for (int k = 0; k < kW; ++k) x[k] = p2[c[k]];
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            if (k < width) {
+                add_block(a0, a1, static_cast<double>(u[k].x), static_cast<double>(u[k].y),
+                          static_cast<double>(l[k].x), static_cast<double>(l[k].y), x[k], v.pair_sum != 0);
+            }
+        }
+    } else {
+        for (int k = 0; k < width; ++k) {
+            const int at = begin + k * kWave + lane;
+            const P uk = __builtin_nontemporal_load(upper + at);
+            const P lk = __builtin_nontemporal_load(lower + at);
+            const dpair xk = p2[__builtin_nontemporal_load(v.col + at)];
+            add_block(a0, a1, static_cast<double>(uk.x), static_cast<double>(uk.y), static_cast<double>(lk.x),
+                      static_cast<double>(lk.y), xk, v.pair_sum != 0);
+        }
+    }
+    if (own) {
+        if (sizeof(VT) == sizeof(float)) add_diag32(v, node, pn, a0, a1);
+        if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;  // an earlier slice of this wavefront
+        qi = node;
+        qv = dpair{a0, a1};
+        double z0, z1, m0, m1;
+        mo.apply(rn.x, rn.y, z0, z1);
+        mo.apply(a0, a1, m0, m1);
+        acc[0] += pn.x * a0 + pn.y * a1;
+        acc[1] += a0 * z0 + a1 * z1;
+        acc[2] += a0 * m0 + a1 * m1;
+    }
+}
+
+template <int kW, int Prec>
+__global__ __launch_bounds__(kThreads) void pcg_slices_multiply_kernel(PcgBuffers b, SliceView v,
+                                                                       const double* __restrict__ planes64,
+                                                                       const float* __restrict__ planes32,
+                                                                       const double* __restrict__ r_new,
+                                                                       const double* __restrict__ p_new) {
+    if (b.st->done) return;
+    __shared__ double sh[3 * (kThreads / kWave)];
+    const double alpha = b.st->alpha;
+    const bool use32 = planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
+    constexpr int kWaves = kThreads / kWave;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = (v.n_slices + gridDim.x - 1) / gridDim.x;
+    const int start = lb * per_block;
+    const int stop = min(start + per_block, v.n_slices);
+    double acc[3] = {0.0, 0.0, 0.0};  // p'.q', q'.z, q'.M^-1 q'
+    int qi = -1;
+    dpair qv = {0.0, 0.0};
+    for (int s = start + static_cast<int>(threadIdx.x) / kWave; s < stop; s += kWaves) {
+        if (use32)
+            split_slice<float, kW, Prec>(b, v, planes32, s, r_new, p_new, acc, qi, qv);
+        else
+            split_slice<double, kW, Prec>(b, v, planes64, s, r_new, p_new, acc, qi, qv);
+    }
+    block_sums<kThreads, 3>(acc, sh);
+    double tot[3];
+    const bool last = finish_reduction<kThreads, 3>(b.tickets + 1, b.partials, acc, tot, sh);
+    if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;
+    if (!last) return;
+    if (threadIdx.x == 0) {
+        const double all[5] = {b.st->rr_new, b.st->rz_new, tot[0], tot[1], tot[2]};
+        fused_state_update(b.st, all, alpha);
+    }
+}
+
 // The x term an odd final launch deferred: x += alpha p_in of that launch. A solve
 // that stopped there (done) left alpha unchanged; otherwise the launch moved it to
 // alpha_prev.
@@ -833,6 +1129,58 @@ void launch_pcg_fused_grid(hipStream_t s, const PcgBuffers& b, const StencilView
     else
         pcg_fused_grid_kernel<LFG_PRECOND_JACOBI, R><<<grid, R * kWave, 0, s>>>(b, v, planes64, planes32, diag_c,
                                                                                 r_in, r_out, p_in, p_out, q_in, q_out);
+}
+
+void launch_pcg_fused_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
+                             const float* planes32, const double* r_in, double* r_out, const double* p_in,
+                             double* p_out, const double* q_in, double* q_out) {
+    PcgBuffers bq = b;
+    bq.q = q_out;
+    const unsigned int need = blocks_for(v.n_slices, kThreads / kWave);
+    const unsigned int grid = round8(need < 8192u ? need : 8192u);
+    const bool narrow = v.max_width <= 9;
+    if (b.precond == LFG_PRECOND_BLOCK2) {
+        if (narrow)
+            pcg_fused_slices_kernel<9, LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32, r_in,
+                                                                                   r_out, p_in, p_out, q_in);
+        else
+            pcg_fused_slices_kernel<0, LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32, r_in,
+                                                                                   r_out, p_in, p_out, q_in);
+    } else {
+        if (narrow)
+            pcg_fused_slices_kernel<9, LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32, r_in,
+                                                                                   r_out, p_in, p_out, q_in);
+        else
+            pcg_fused_slices_kernel<0, LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32, r_in,
+                                                                                   r_out, p_in, p_out, q_in);
+    }
+}
+
+void launch_pcg_split_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
+                             const float* planes32, const double* r_in, double* r_out, const double* p_in,
+                             double* p_out, const double* q_in, double* q_out) {
+    PcgBuffers bq = b;
+    bq.q = q_out;
+    const unsigned int need = blocks_for(v.n_slices, kThreads / kWave);
+    const unsigned int grid = round8(need < 8192u ? need : 8192u);
+    const bool narrow = v.max_width <= 9;
+    if (b.precond == LFG_PRECOND_BLOCK2) {
+        pcg_slices_vector_kernel<LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(b, r_in, r_out, p_in, p_out, q_in);
+        if (narrow)
+            pcg_slices_multiply_kernel<9, LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32,
+                                                                                      r_out, p_out);
+        else
+            pcg_slices_multiply_kernel<0, LFG_PRECOND_BLOCK2><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32,
+                                                                                      r_out, p_out);
+    } else {
+        pcg_slices_vector_kernel<LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(b, r_in, r_out, p_in, p_out, q_in);
+        if (narrow)
+            pcg_slices_multiply_kernel<9, LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32,
+                                                                                      r_out, p_out);
+        else
+            pcg_slices_multiply_kernel<0, LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32,
+                                                                                      r_out, p_out);
+    }
 }
 
 void launch_pack_stencil(hipStream_t s, const CsrView& a, const StencilView& v, const double* values, double* out64,

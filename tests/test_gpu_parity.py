@@ -338,7 +338,8 @@ def test_cantilever200_subdomain_block_and_block_storage(dev, golden_index):
 
 @pytest.mark.parametrize("precond,switch,variant,drop", [
     ("jacobi", 0.0, 0, 0.0), ("block", 0.0, 0, 0.0), ("jacobi", 0.0, 8, 0.0), ("jacobi", 0.0, 8, 0.1),
-    ("jacobi", -1.0, 8, 0.1), ("jacobi", 0.0, 6, 0.1), ("jacobi", -1.0, 6, 0.1)])
+    ("jacobi", -1.0, 8, 0.1), ("jacobi", 0.0, 6, 0.1), ("jacobi", -1.0, 6, 0.1), ("jacobi", 0.0, 12, 0.0),
+    ("block", -1.0, 13, 0.0)])
 def test_mixed_precision_matches_fp64(dev, precond, switch, variant, drop):
     """FP32 values (after an FP64 phase, or from the start), FP64 vectors and dots,
     with or without FP64 residual replacements: the same answer as FP64 and a direct
@@ -375,11 +376,38 @@ def test_deterministic_and_graph_equals_direct(dev):
     u3, r3 = dev.pcg("jacobi", direct_launch=True, check_every=7)
     assert np.array_equal(bits(u1), bits(u2)) and np.array_equal(bits(u1), bits(u3))
     assert r1["iterations"] == r2["iterations"] == r3["iterations"]
-    for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10):
+    for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13):
         u4, r4 = dev.pcg("jacobi", spmv_variant=variant)
         assert r4["converged"] and np.linalg.norm(u4 - u1) <= 1e-12 * np.linalg.norm(u1), variant
-    u10, r10 = dev.pcg("jacobi", spmv_variant=10, direct_launch=True, check_every=7)
-    assert np.array_equal(bits(u10), bits(dev.pcg("jacobi", spmv_variant=10)[0]))
+    for variant in (10, 12, 13):
+        u10, r10 = dev.pcg("jacobi", spmv_variant=variant, direct_launch=True, check_every=7)
+        assert np.array_equal(bits(u10), bits(dev.pcg("jacobi", spmv_variant=variant)[0])), variant
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "block"])
+def test_fused_slice_iterations(dev, precond):
+    """The fused node-slice iterations on an RCM-numbered log-E mesh (no fixed node offsets):
+    one launch (12) and two launches (13) per iteration give the same bits (the same sums over
+    the same nodes per block, p' of column nodes formed with the owner's arithmetic); both
+    match the three-launch node-slice iteration (6) to 1e-9 with its iteration count within
+    1% + 2, in CSR and in block storage order; they refuse the sweep preconditioners."""
+    print(f"case fused node slices 48x40 rcm log-E {precond}", flush=True)
+    m = meshgen.structured_cantilever(48, 40, young=meshgen.loguniform_young(48 * 40, 7), plane_mode="plane_strain")
+    pipeline.setup(m, device=dev, assembly="coloring", reordering="rcm")
+    assert dev.layout_info()["stencil_sym_planes"] == 0  # RCM: node slices only
+    for storage in ("csr", "bsr"):
+        u6, r6 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=6, storage=storage)
+        u12, r12 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=12, storage=storage)
+        u13, r13 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=13, storage=storage)
+        u0, r0 = dev.pcg(precond, "fp64", "relative", 1e-10, storage=storage)  # the default: 12 at this size
+        assert np.array_equal(bits(u12), bits(u13)) and r12["iterations"] == r13["iterations"], storage
+        assert np.array_equal(bits(u0), bits(u12)), storage
+        assert r12["converged"] and abs(r12["iterations"] - r6["iterations"]) <= 2 + 0.01 * r6["iterations"]
+        assert np.linalg.norm(u12 - u6) <= 1e-9 * np.linalg.norm(u6), storage
+    for variant in (12, 13):
+        with pytest.raises(native.LightFemError) as err:
+            dev.pcg("ic0", spmv_variant=variant)
+        assert err.value.code == native.ERR_UNSUPPORTED
 
 
 def test_stencil_layouts(dev):
