@@ -60,6 +60,21 @@ int lfh_subdomain_split(int64_t n_nodes, const double* xy, int32_t split, int64_
 /* csr_profile (sparse.cpp:227-242): out = {nonzeros, bandwidth, profile}. */
 int lfh_csr_profile(int64_t n_rows, const int32_t* row_ptr, const int32_t* col, int64_t out[3]);
 
+/* The node-level schedule of the device IC(0) sweeps (lfg_pcg with ic0 on a node-blocked
+ * pattern) for the IC(0) factor of the CSR pattern row_ptr/col (n_dofs rows): no
+ * counterpart in the reference (whose sweeps run row by row, precond.cpp:386-412); for
+ * inspection and tests of the schedule the device waits on. backward 0: the forward
+ * sweep over L's rows, 1: the backward sweep over its transposed columns. Fills
+ * ord[n_dofs / 2] (position -> node), qptr[n_dofs / 2 + 1] and key[< key_cap] (each
+ * position's dependency positions), parts[< n_dofs / 2 + 2] (part boundaries: levels
+ * split into parts of at most 64 nodes; *n_parts boundaries written) and *reach (the
+ * largest distance back a position reads). Returns 0; 1 for invalid arguments; 2 when
+ * the factor's rows 2i, 2i+1 do not share their dependency nodes (the device then uses
+ * the row form); 3 when key_cap is too small. */
+int lfh_ic0_node_schedule(int64_t n_dofs, const int32_t* row_ptr, const int32_t* col, int32_t backward,
+                          int32_t* ord, int32_t* qptr, int32_t* key, int64_t key_cap, int32_t* parts,
+                          int64_t* n_parts, int32_t* reach);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,7 @@
 #include <thread>
 #include <vector>
 
+#include "lfg_ic0_plan.hpp"
 #include "lfg_kernels.hpp"
 #include "lightfem_gpu.h"
 
@@ -602,113 +603,50 @@ void ensure_gather_map(lfg_ctx* c) {
     build_gather_map(c, emap.data(), bmap.data(), c->n_bars);
 }
 
-// IC(0) of the lower triangle (precond.cpp:298-384), level sets for the device
-// sweeps, and the transposed strict lower part with rows in descending order.
-// Level parts of at most 64 positions (rows of one level are independent, so any split
-// keeps the order valid).
-std::vector<int> split_levels(const std::vector<int>& level_ptr) {
-    std::vector<int> parts(1, 0);
-    for (size_t l = 0; l + 1 < level_ptr.size(); ++l) {
-        const int b = level_ptr[l], e = level_ptr[l + 1];
-        const int pieces = (e - b + 63) / 64;
-        for (int k = 1; k <= pieces; ++k)
-            parts.push_back(b + static_cast<int>((static_cast<long long>(e - b) * k) / pieces));
-    }
-    return parts;
-}
+using lfg_ic0::split_levels;
 
-// The node form of the two sweeps (Ic0NodeSweep), when rows 2i, 2i+1 of the factor
-// have the same dependency nodes: row 2i's strict lower part is the pairs (2j, 2j+1)
-// of its nodes j in ascending order and row 2i+1's is the same followed by 2i (the
-// transposed columns alike: column 2i+1 holds the pairs (2j+1, 2j) in descending j,
-// column 2i the same followed by row 2i+1). Otherwise the row form serves.
+// The node form of the two sweeps (Ic0NodeSweep, lfg_ic0::node_plan) when rows 2i, 2i+1
+// of the factor have the same dependency nodes; otherwise the row form serves.
 void build_ic0_nodes(lfg_ctx* c, const std::vector<int>& lrow, const std::vector<int>& lcol,
                      const std::vector<double>& lval, const std::vector<int>& urow, const std::vector<int>& ucol,
                      const std::vector<double>& uval, const std::vector<double>& inv_diag) {
     Ic0Device& d = c->ic0;
     d.nfwd.ready = d.nbwd.ready = false;
     const int nn = c->n / 2;
-    // rows (forward) or columns (backward) f = first, s = second of node i: entries
-    // [b_f, e_f) must be pairs over nodes, [b_s, e_s) the same pairs plus one cross entry
     auto node_form = [&](bool backward, const std::vector<int>& ptr, const std::vector<int>& col,
                          const std::vector<double>& val, Ic0NodeSweepDevice& out) {
-        std::vector<int> deps_ptr(static_cast<size_t>(nn) + 1, 0), deps;
-        std::vector<double> vf, vs, cross(static_cast<size_t>(nn)), sf(static_cast<size_t>(nn)),
-            ss(static_cast<size_t>(nn));
-        for (int i = 0; i < nn; ++i) {
-            const int first = backward ? 2 * i + 1 : 2 * i, second = backward ? 2 * i : 2 * i + 1;
-            const int bf = ptr[first], ef = backward ? ptr[first + 1] : ptr[first + 1] - 1;
-            const int bs = ptr[second], es = backward ? ptr[second + 1] : ptr[second + 1] - 1;
-            if ((ef - bf) % 2 != 0 || es - bs != ef - bf + 1 || col[es - 1] != first) return false;
-            for (int k = 0; k < ef - bf; k += 2) {
-                const int c0 = col[bf + k], c1 = col[bf + k + 1];
-                // forward: (2j, 2j+1) ascending; backward: (2j+1, 2j) descending
-                const int j = backward ? c1 / 2 : c0 / 2;
-                const bool pair_ok = backward ? (c0 == 2 * j + 1 && c1 == 2 * j) : (c0 == 2 * j && c1 == 2 * j + 1);
-                if (!pair_ok || col[bs + k] != c0 || col[bs + k + 1] != c1) return false;
-                deps.push_back(j);
-                vf.push_back(val[bf + k]);
-                vf.push_back(val[bf + k + 1]);
-                vs.push_back(val[bs + k]);
-                vs.push_back(val[bs + k + 1]);
-            }
-            deps_ptr[i + 1] = static_cast<int>(deps.size());
-            cross[i] = val[es - 1];
-            sf[i] = inv_diag[first];
-            ss[i] = inv_diag[second];
-        }
-        // node levels (dependencies come before in the sweep's order), positions by level
-        std::vector<int> level(static_cast<size_t>(nn), 0);
-        int n_levels = 0;
-        for (int t = 0; t < nn; ++t) {
-            const int i = backward ? nn - 1 - t : t;
-            int lv = 0;
-            for (int k = deps_ptr[i]; k < deps_ptr[i + 1]; ++k) lv = std::max(lv, level[deps[k]] + 1);
-            level[i] = lv;
-            n_levels = std::max(n_levels, lv + 1);
-        }
-        std::vector<int> lptr(static_cast<size_t>(n_levels) + 1, 0), ord(static_cast<size_t>(nn));
-        for (int i = 0; i < nn; ++i) lptr[level[i] + 1]++;
-        for (int l = 0; l < n_levels; ++l) lptr[l + 1] += lptr[l];
-        {
-            std::vector<int> cur(lptr.begin(), lptr.end() - 1);
-            for (int i = 0; i < nn; ++i) ord[cur[level[i]]++] = i;
-        }
-        std::vector<int> pos(static_cast<size_t>(nn));
-        for (int q = 0; q < nn; ++q) pos[ord[q]] = q;
-        std::vector<int> qptr(static_cast<size_t>(nn) + 1, 0), key;
+        lfg_ic0::NodePlan plan;
+        if (!lfg_ic0::node_plan(nn, backward, ptr.data(), col.data(), plan)) return false;
         std::vector<double> qvf, qvs, qcross(static_cast<size_t>(nn)), qsf(static_cast<size_t>(nn)),
             qss(static_cast<size_t>(nn));
-        int reach = 0;
+        qvf.reserve(2 * plan.key.size());
+        qvs.reserve(2 * plan.key.size());
         for (int q = 0; q < nn; ++q) {
-            const int i = ord[q];
-            for (int k = deps_ptr[i]; k < deps_ptr[i + 1]; ++k) {
-                key.push_back(pos[deps[k]]);
-                reach = std::max(reach, q - pos[deps[k]]);
-                qvf.push_back(vf[2 * static_cast<size_t>(k)]);
-                qvf.push_back(vf[2 * static_cast<size_t>(k) + 1]);
-                qvs.push_back(vs[2 * static_cast<size_t>(k)]);
-                qvs.push_back(vs[2 * static_cast<size_t>(k) + 1]);
+            const int i = plan.ord[q];
+            const int count = plan.qptr[q + 1] - plan.qptr[q];
+            const int f = plan.first_at[q], s2 = plan.second_at[q];
+            for (int u = 0; u < count; ++u) {
+                qvf.push_back(val[f + 2 * u]);
+                qvf.push_back(val[f + 2 * u + 1]);
+                qvs.push_back(val[s2 + 2 * u]);
+                qvs.push_back(val[s2 + 2 * u + 1]);
             }
-            qptr[q + 1] = static_cast<int>(key.size());
-            qcross[q] = cross[i];
-            qsf[q] = sf[i];
-            qss[q] = ss[i];
+            qcross[q] = val[s2 + 2 * count];
+            qsf[q] = inv_diag[backward ? 2 * i + 1 : 2 * i];
+            qss[q] = inv_diag[backward ? 2 * i : 2 * i + 1];
         }
-        const std::vector<int> parts = split_levels(lptr);
-        out.ord.upload(ord.data(), ord.size(), c->stream);
-        out.ptr.upload(qptr.data(), qptr.size(), c->stream);
-        out.key.upload(key.data(), key.size(), c->stream);
+        out.ord.upload(plan.ord.data(), plan.ord.size(), c->stream);
+        out.ptr.upload(plan.qptr.data(), plan.qptr.size(), c->stream);
+        out.key.upload(plan.key.data(), plan.key.size(), c->stream);
         out.v_first.upload(qvf.data(), qvf.size(), c->stream);
         out.v_second.upload(qvs.data(), qvs.size(), c->stream);
         out.cross.upload(qcross.data(), qcross.size(), c->stream);
         out.scale_first.upload(qsf.data(), qsf.size(), c->stream);
         out.scale_second.upload(qss.data(), qss.size(), c->stream);
-        out.level_ptr.upload(parts.data(), parts.size(), c->stream);
-        out.levels = static_cast<int>(parts.size()) - 1;
-        out.reach = reach;
-        out.widest = 0;
-        for (int l = 0; l < out.levels; ++l) out.widest = std::max(out.widest, parts[l + 1] - parts[l]);
+        out.level_ptr.upload(plan.parts.data(), plan.parts.size(), c->stream);
+        out.levels = static_cast<int>(plan.parts.size()) - 1;
+        out.reach = plan.reach;
+        out.widest = plan.widest;
         out.ready = true;
         return true;
     };
@@ -716,34 +654,19 @@ void build_ic0_nodes(lfg_ctx* c, const std::vector<int>& lrow, const std::vector
         d.nfwd.ready = d.nbwd.ready = false;
 }
 
+// IC(0) of the lower triangle (precond.cpp:298-384), level sets for the device
+// sweeps, and the transposed strict lower part with rows in descending order.
 void build_ic0(lfg_ctx* c) {
     const int n = c->n;
     std::vector<double> kc(static_cast<size_t>(c->nnz));
     c->Kc.download(kc.data(), kc.size(), c->stream);
     sync(c);
-    std::vector<int> lrow(static_cast<size_t>(n) + 1, 0), lcol, diag_index(static_cast<size_t>(n), 0);
-    std::vector<double> lval, inv_diag(static_cast<size_t>(n), 1.0);
-    lcol.reserve(static_cast<size_t>(c->nnz / 2 + n));
-    lval.reserve(static_cast<size_t>(c->nnz / 2 + n));
-    for (int row = 0; row < n; ++row) {
-        bool found = false;
-        for (int k = c->h_row_ptr[row]; k < c->h_row_ptr[row + 1]; ++k) {
-            const int colk = c->h_col[k];
-            if (colk > row) break;
-            lcol.push_back(colk);
-            lval.push_back(kc[static_cast<size_t>(k)]);
-            if (colk == row) {
-                diag_index[row] = static_cast<int>(lval.size()) - 1;
-                found = true;
-            }
-        }
-        if (!found) {
-            lcol.push_back(row);
-            lval.push_back(1.0);
-            diag_index[row] = static_cast<int>(lval.size()) - 1;
-        }
-        lrow[row + 1] = static_cast<int>(lval.size());
-    }
+    const lfg_ic0::Pattern pat = lfg_ic0::pattern(n, c->h_row_ptr.data(), c->h_col.data());
+    const std::vector<int>& lrow = pat.lrow;
+    const std::vector<int>& lcol = pat.lcol;
+    const std::vector<int>& diag_index = pat.diag;
+    std::vector<double> lval(pat.lcol.size()), inv_diag(static_cast<size_t>(n), 1.0);
+    for (size_t k = 0; k < lval.size(); ++k) lval[k] = pat.lsrc[k] >= 0 ? kc[static_cast<size_t>(pat.lsrc[k])] : 1.0;
     double scale = 0.0;
     for (int row = 0; row < n; ++row) scale = std::max(scale, std::abs(lval[diag_index[row]]));
     const double floor_value = std::max(1.0e-30, 1.0e-14 * scale);
@@ -794,22 +717,10 @@ void build_ic0(lfg_ctx* c) {
         for (int row = 0; row < n; ++row) frows[cur[level[row]]++] = row;
     }
     // transpose of the strict lower part, rows descending per column
-    std::vector<int> urow(static_cast<size_t>(n) + 1, 0);
-    for (int row = 0; row < n; ++row)
-        for (int k = lrow[row]; k < lrow[row + 1] - 1; ++k) urow[lcol[k] + 1]++;
-    for (int i = 0; i < n; ++i) urow[i + 1] += urow[i];
-    std::vector<int> ucol(static_cast<size_t>(urow[n]));
-    std::vector<double> uval(static_cast<size_t>(urow[n]));
-    {
-        std::vector<int> cur(urow.begin(), urow.end() - 1);
-        for (int row = n - 1; row >= 0; --row)
-            for (int k = lrow[row]; k < lrow[row + 1] - 1; ++k) {
-                const int cc = lcol[k];
-                ucol[cur[cc]] = row;
-                uval[cur[cc]] = lval[k];
-                ++cur[cc];
-            }
-    }
+    const std::vector<int>& urow = pat.urow;
+    const std::vector<int>& ucol = pat.ucol;
+    std::vector<double> uval(pat.usrc.size());
+    for (size_t k = 0; k < uval.size(); ++k) uval[k] = lval[static_cast<size_t>(pat.usrc[k])];
     // backward levels
     std::vector<int> blevel(static_cast<size_t>(n), 0);
     int bwd_levels = 0;

@@ -10,6 +10,8 @@
 // seconds at a million elements.
 #include "lightfem_host.h"
 
+#include "../device/lfg_ic0_plan.hpp"
+
 #include <algorithm>
 #include <cstdint>
 #include <queue>
@@ -365,6 +367,28 @@ int lfh_csr_profile(int64_t n_rows, const int32_t* row_ptr, const int32_t* col, 
     out[0] = row_ptr[n_rows];
     out[1] = bandwidth;
     out[2] = profile;
+    return 0;
+}
+
+int lfh_ic0_node_schedule(int64_t n_dofs, const int32_t* row_ptr, const int32_t* col, int32_t backward,
+                          int32_t* ord, int32_t* qptr, int32_t* key, int64_t key_cap, int32_t* parts,
+                          int64_t* n_parts, int32_t* reach) {
+    if (n_dofs < 2 || n_dofs % 2 != 0 || n_dofs >= (int64_t{1} << 30) || !row_ptr || !col || !ord || !qptr ||
+        !key || !parts || !n_parts || !reach)
+        return 1;
+    const int n = static_cast<int>(n_dofs);
+    const lfg_ic0::Pattern pat = lfg_ic0::pattern(n, row_ptr, col);
+    lfg_ic0::NodePlan plan;
+    const bool ok = backward ? lfg_ic0::node_plan(n / 2, true, pat.urow.data(), pat.ucol.data(), plan)
+                             : lfg_ic0::node_plan(n / 2, false, pat.lrow.data(), pat.lcol.data(), plan);
+    if (!ok) return 2;
+    if (static_cast<int64_t>(plan.key.size()) > key_cap) return 3;
+    std::copy(plan.ord.begin(), plan.ord.end(), ord);
+    std::copy(plan.qptr.begin(), plan.qptr.end(), qptr);
+    std::copy(plan.key.begin(), plan.key.end(), key);
+    std::copy(plan.parts.begin(), plan.parts.end(), parts);
+    *n_parts = static_cast<int64_t>(plan.parts.size());
+    *reach = plan.reach;
     return 0;
 }
 

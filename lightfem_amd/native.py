@@ -36,7 +36,7 @@ GPU_SYMBOLS = ["lfg_create", "lfg_destroy", "lfg_last_error", "lfg_status_string
                "lfg_pin_zero_pivots", "lfg_spmv_noalloc"]
 HOST_SYMBOLS = ["lfh_csr_pattern_nnz", "lfh_csr_pattern", "lfh_rcm", "lfh_element_colors", "lfh_csr_profile",
                 "lfh_subdomain_split", "lfh_csr_pattern_nnz_bars", "lfh_csr_pattern_bars", "lfh_rcm_bars",
-                "lfh_element_colors_bars"]
+                "lfh_element_colors_bars", "lfh_ic0_node_schedule"]
 
 
 class LightFemError(RuntimeError):
@@ -189,6 +189,7 @@ def host_lib() -> ctypes.CDLL:
         lib.lfh_csr_pattern_nnz_bars.argtypes = [_I64, _I64, _P, _I64, _P]
         lib.lfh_csr_pattern_bars.argtypes = [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P]
         lib.lfh_rcm_bars.argtypes = [_I64, _P, _I64, _P, _I64, _P, _P]
+        lib.lfh_ic0_node_schedule.argtypes = [_I64, _P, _P, ctypes.c_int32, _P, _P, _P, _I64, _P, _P, _P]
         lib.lfh_element_colors_bars.argtypes = [_I64, _I64, _P, _I64, _P, _P, _P, _P, _P]
         _host = lib
     return _host
@@ -259,6 +260,30 @@ def csr_profile(row_ptr: np.ndarray, col: np.ndarray) -> dict:
     out = np.empty(3, dtype=np.int64)
     host_lib().lfh_csr_profile(row_ptr.shape[0] - 1, _ptr(row_ptr), _ptr(col), _ptr(out))
     return {"nonzeros": int(out[0]), "bandwidth": int(out[1]), "profile": int(out[2])}
+
+
+def ic0_node_schedule(row_ptr: np.ndarray, col: np.ndarray, backward: bool = False):
+    """The level schedule of the device IC(0) sweeps for the factor of this CSR pattern
+    (lfh_ic0_node_schedule): dict with ord, qptr, key, parts, reach, or None when the
+    factor's rows 2i, 2i+1 do not share their dependency nodes."""
+    row_ptr, col = _i32(row_ptr), _i32(col)
+    n = row_ptr.shape[0] - 1
+    nn = n // 2
+    ord_ = np.empty(nn, dtype=np.int32)
+    qptr = np.empty(nn + 1, dtype=np.int32)
+    key = np.empty(max(1, col.shape[0]), dtype=np.int32)
+    parts = np.empty(nn + 2, dtype=np.int32)
+    n_parts = ctypes.c_int64(0)
+    reach = ctypes.c_int32(0)
+    rc = host_lib().lfh_ic0_node_schedule(n, _ptr(row_ptr), _ptr(col), int(backward), _ptr(ord_), _ptr(qptr),
+                                          _ptr(key), key.shape[0], _ptr(parts), ctypes.byref(n_parts),
+                                          ctypes.byref(reach))
+    if rc == 2:
+        return None
+    if rc != 0:
+        raise LightFemError(rc, "lfh_ic0_node_schedule failed")
+    return {"ord": ord_, "qptr": qptr, "key": key[:qptr[-1]].copy(), "parts": parts[:n_parts.value].copy(),
+            "reach": int(reach.value)}
 
 
 def subdomain_split(xy: np.ndarray, split: str = "rcb", nodes_per_subdomain: int = 64):
