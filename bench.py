@@ -254,6 +254,20 @@ def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict
                      "launch_shape": {"block_threads": summary["block_threads"],
                                       "slices_per_wave": summary["slices_per_wave"],
                                       "tune_ms": round(summary["tune_ms"], 2)}}
+        # the dominant kernel: the batch multiply (and its per-problem fold) with every problem
+        # running, timed by the launch-shape tuner. Algorithmic bytes: the node-slice values
+        # (FP64, or FP32 plus the FP64 compensated diagonal, 32 B per node), one column per 2x2
+        # block, the slice offsets, and per node the 4 cases' p read and q written (128 B)
+        nodes = dev.n // 2
+        n_slices = (nodes + 63) // 64
+        values = 8 * dev.nnz if form == "fp64" else 4 * dev.nnz + 32 * nodes
+        moved = values + 4 * (dev.nnz // 4) + 4 * (n_slices + 1) + 128 * nodes
+        ms = summary["multiply_ms"]
+        out[form]["roofline"] = {"kernel": "batch_spmv_kernel + batch_spmv_finish_kernel", "bound": "hbm",
+                                 "bytes_per_launch": int(moved), "avg_launch_ms": round(ms, 4),
+                                 "achieved": round(moved / (ms * 1e-3) / 1e9, 1) if ms > 0 else None,
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(moved / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms > 0 else None}
         print(f"bench: batch {form} {json.dumps(out[form])}", file=sys.stderr, flush=True)
     return out
 
@@ -483,6 +497,17 @@ def main() -> int:
             "true_relative_residual": rep6["true_residual"] / rep6["initial_residual"],
             "rel_u_diff_vs_fp64": float(np.linalg.norm(u6 - u64) / np.linalg.norm(u64))}
         print(f"bench: fp64 node slices {forms['fp64_node_slices']}", file=sys.stderr, flush=True)
+        # the same layout with the iteration fused into two launches (variant 13: the default
+        # for node-blocked patterns without fixed offsets at this size, e.g. RCM numbering)
+        t = time.perf_counter()
+        u13, rep13 = dev.pcg("jacobi", "fp64", "relative", args.tol, 10 ** 6, args.check_every, spmv_variant=13,
+                             raise_on_error=False)
+        forms["fp64_node_slices_fused"] = {
+            "ms_per_solve": (time.perf_counter() - t) * 1e3, "device_solve_ms": rep13["solve_ms"],
+            "iterations": rep13["iterations"], "converged": bool(rep13["converged"]), "spmv_variant": 13,
+            "true_relative_residual": rep13["true_residual"] / rep13["initial_residual"],
+            "rel_u_diff_vs_fp64": float(np.linalg.norm(u13 - u64) / np.linalg.norm(u64))}
+        print(f"bench: fp64 node slices fused {forms['fp64_node_slices_fused']}", file=sys.stderr, flush=True)
     per_iter_ms = {k: v["device_solve_ms"] / max(1, v["iterations"]) for k, v in forms.items()}
     cpu, protocol = None, None
     if not args.skip_cpu:

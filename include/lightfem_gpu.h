@@ -298,6 +298,8 @@ typedef struct {
     int32_t slices_per_wave;
     int32_t systems_converged;
     int64_t max_iterations;   /* over systems */
+    double multiply_ms;       /* the tuned batch multiply (with its per-problem fold), one launch with
+                                 every problem running: device time of the fastest launch shape */
 } lfg_batch_summary;
 
 /* Solves every (problem, case) system; u_out[c * n_dofs + i] (NULL to skip), reports

@@ -180,6 +180,7 @@ struct lfg_ctx {
     int batch_tuned_for = -1;  // precision the cached launch shape was timed with
     lfg::BatchShape batch_shape{256, 1};
     double batch_tune_ms = 0.0;
+    double batch_multiply_ms = 0.0;  // the chosen launch shape's multiply, per launch
     DevArray<int> batch_slice_problem, batch_pslice_ptr, batch_problem_done, batch_phase32;
     DevArray<int> batch_active, batch_active_first;  // active slice list and its compaction offsets
     DevArray<unsigned int> batch_tickets;
@@ -2205,6 +2206,7 @@ void batch_autotune(lfg_ctx* c, const lfg::BatchView& bv, bool mixed) {
     LFG_HIP(hipEventRecord(c->ev_ring[1], c->stream));
     sync(c);
     c->batch_tune_ms = elapsed(c->ev_ring[0], c->ev_ring[1]);
+    c->batch_multiply_ms = best / 5.0;
     c->batch_shape = chosen;
     c->batch_tuned_for = mixed ? LFG_PRECISION_MIXED : LFG_PRECISION_FP64;
 }
@@ -2343,6 +2345,7 @@ int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg
             summary->slices_per_wave = ctx->batch_shape.slices_per_wave;
             summary->systems_converged = converged;
             summary->max_iterations = max_iter;
+            summary->multiply_ms = ctx->batch_multiply_ms;
         }
         if (first_status != LFG_OK) throw LfgError(first_status, status_text(first_status));
         if (converged != static_cast<int>(n_sys))

@@ -111,7 +111,8 @@ class BatchSummary(ctypes.Structure):
     _fields_ = [("solve_ms", ctypes.c_double), ("tune_ms", ctypes.c_double), ("graph_launches", ctypes.c_int64),
                 ("problems_switched", ctypes.c_int32), ("block_threads", ctypes.c_int32),
                 ("slices_per_wave", ctypes.c_int32),
-                ("systems_converged", ctypes.c_int32), ("max_iterations", ctypes.c_int64)]
+                ("systems_converged", ctypes.c_int32), ("max_iterations", ctypes.c_int64),
+                ("multiply_ms", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
