@@ -800,7 +800,8 @@ __device__ __forceinline__ void fused_slice(const PcgBuffers& b, const SliceView
                                             int s, const double* __restrict__ r_in, double* __restrict__ r_out,
                                             const double* __restrict__ p_in, double* __restrict__ p_out,
                                             const double* __restrict__ q_in, double alpha, double beta,
-                                            double alpha_prev, int x_mode, double (&acc)[5], int& qi, dpair& qv) {
+                                            double alpha_prev, int x_mode, int done, double (&acc)[5], int& qi,
+                                            dpair& qv) {
     using P = typename PairOf<VT>::type;
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
     const int node = s * kWave + lane;
@@ -811,15 +812,31 @@ __device__ __forceinline__ void fused_slice(const PcgBuffers& b, const SliceView
                 po = reinterpret_cast<const dpair*>(p_in)[ic];
     NodePrecond<Prec> mo;
     mo.load(b.m, ic);
+    const int begin = v.ptr[s];
+    const int width = (v.ptr[s + 1] - begin) / kWave;
+    const P* __restrict__ upper = reinterpret_cast<const P*>(planes);
+    const P* __restrict__ lower = upper + v.blocks;
+    // the slice's blocks (up to kW: every load issued here, indices clamped to the last block)
+    const bool unrolled = kW > 0 && width > 0 && width <= kW;
+    P u[kW > 0 ? kW : 1], l[kW > 0 ? kW : 1];
+    int c[kW > 0 ? kW : 1];
+    if (unrolled) {
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            const int at = begin + min(k, width - 1) * kWave + lane;
+            u[k] = __builtin_nontemporal_load(upper + at);
+            l[k] = __builtin_nontemporal_load(lower + at);
+            c[k] = __builtin_nontemporal_load(v.col + at);
+        }
+    }
+    // x and the older p (the launch parity comes from the state, read at the kernel's start
+    // and waited for only here); a finished solve's launch stops here, before any store
     dpair xo = {0.0, 0.0}, pp = {0.0, 0.0};
     if (x_mode != 0) {
         xo = x2[ic];
         if (x_mode == 2) pp = reinterpret_cast<const dpair*>(p_out)[ic];
     }
-    const int begin = v.ptr[s];
-    const int width = (v.ptr[s + 1] - begin) / kWave;
-    const P* __restrict__ upper = reinterpret_cast<const P*>(planes);
-    const P* __restrict__ lower = upper + v.blocks;
+    if (done) return;
     // p' of column node j from the launch's inputs, with the owner's arithmetic
     auto column = [&](int j) {
         const dpair rj = reinterpret_cast<const dpair*>(r_in)[j], qj = reinterpret_cast<const dpair*>(q_in)[j],
@@ -852,16 +869,7 @@ __device__ __forceinline__ void fused_slice(const PcgBuffers& b, const SliceView
     }
     // 2. q' = A p' in column order (sparse.cpp:104-120; block order with v.pair_sum)
     double a0 = 0.0, a1 = 0.0;
-    if (kW > 0 && width > 0 && width <= kW) {
-        P u[kW > 0 ? kW : 1], l[kW > 0 ? kW : 1];
-        int c[kW > 0 ? kW : 1];
-#pragma unroll
-        for (int k = 0; k < kW; ++k) {
-            const int at = begin + min(k, width - 1) * kWave + lane;
-            u[k] = __builtin_nontemporal_load(upper + at);
-            l[k] = __builtin_nontemporal_load(lower + at);
-            c[k] = __builtin_nontemporal_load(v.col + at);
-        }
+    if (unrolled) {
 #pragma unroll
         for (int k = 0; k < kW; ++k) {
             if (k < width) {
@@ -901,8 +909,8 @@ __global__ __launch_bounds__(kThreads) void pcg_fused_slices_kernel(
     PcgBuffers b, SliceView v, const double* __restrict__ planes64, const float* __restrict__ planes32,
     const double* __restrict__ r_in, double* __restrict__ r_out, const double* __restrict__ p_in,
     double* __restrict__ p_out, const double* __restrict__ q_in) {
-    if (b.st->done) return;
     __shared__ double sh[5 * (kThreads / kWave)];
+    const int done = b.st->done;  // checked once the slice's loads are issued
     const double alpha = b.st->alpha, beta = b.st->beta, alpha_prev = b.st->alpha_prev;
     const long long launch = b.st->iter + 1;  // x update deferred as in pcg_fused_stencil_kernel
     const int x_mode = (launch & 1) ? 0 : (launch >= 2 ? 2 : 1);
@@ -918,11 +926,12 @@ __global__ __launch_bounds__(kThreads) void pcg_fused_slices_kernel(
     for (int s = start + static_cast<int>(threadIdx.x) / kWave; s < stop; s += kWaves) {
         if (use32)
             fused_slice<float, kW, Prec>(b, v, planes32, s, r_in, r_out, p_in, p_out, q_in, alpha, beta, alpha_prev,
-                                         x_mode, acc, qi, qv);
+                                         x_mode, done, acc, qi, qv);
         else
             fused_slice<double, kW, Prec>(b, v, planes64, s, r_in, r_out, p_in, p_out, q_in, alpha, beta, alpha_prev,
-                                          x_mode, acc, qi, qv);
+                                          x_mode, done, acc, qi, qv);
     }
+    if (done) return;
     block_sums<kThreads, 5>(acc, sh);
     double tot[5];
     const bool last = finish_reduction<kThreads, 5>(b.tickets + 1, b.partials, acc, tot, sh);
