@@ -1920,7 +1920,9 @@ int lfg_spmv_noalloc(lfg_ctx* ctx, int32_t precision, int32_t variant, const dou
         if (stencil_layout(v)) {
             ready = v == kStencilExact ? (!f32 && ctx->stencil_exact_valid)
                                        : (f32 ? ctx->stencil32_valid : ctx->stencil64_valid);
-            grid = lfg::stencil_spmv_blocks(ctx->n / 2);
+            grid = v != kStencilExact && lfg::stencil_grid_multiply(ctx->stencil_sym)
+                       ? lfg::stencil_grid_blocks(ctx->stencil_sym)
+                       : lfg::stencil_spmv_blocks(ctx->n / 2);
         } else if (slice_layout(v)) {
             ready = f32 ? ctx->slices32_valid : ctx->slices64_valid;
             grid = static_cast<unsigned int>(lfg::slice_spmv_blocks(ctx->n_slices));

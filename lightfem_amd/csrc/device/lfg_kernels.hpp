@@ -427,6 +427,16 @@ inline unsigned int stencil_spmv_blocks(int n_nodes) {
     return r8 < 4096u ? r8 : 4096u;
 }
 
+// The 2D-tiled stencil multiply (row-structured symmetric stencils, v.row_len > 0): one
+// block per tile of kGridRows x 64 nodes, at most 4096 blocks (a block then takes several).
+inline unsigned int stencil_grid_blocks(const StencilView& v) {
+    const long long L = v.row_len;
+    const long long tiles = ((v.n_nodes + L - 1) / L + kGridRows - 1) / kGridRows * ((L + 63) / 64);
+    const unsigned int r8 = static_cast<unsigned int>((tiles + 7) & ~7LL);
+    return r8 < 4096u ? r8 : 4096u;
+}
+inline bool stencil_grid_multiply(const StencilView& v) { return v.sym && v.n_off == 5 && v.row_len > 0; }
+
 // ||b - K x||^2 in FP64 into out2 (reporting, iterative refinement); r_out, when not
 // null, receives r = b - K x.
 void launch_residual_norm(hipStream_t s, const CsrView& a, const double* vals64, const double* x, const double* bvec,

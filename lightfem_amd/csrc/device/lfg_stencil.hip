@@ -1098,6 +1098,115 @@ __global__ __launch_bounds__(kThreads) void pcg_slices_multiply_kernel(PcgBuffer
     }
 }
 
+// ---- the stencil multiply alone on 2D tiles (variant 8 where the stencil is row-structured)
+//
+// q = A p for one tile of the 2D iteration's shape: p of the tile's 256 nodes and of its
+// 140-position ring into the LDS window (one ring position per thread, as grid_tile), the
+// node's planes (upper and diagonal blocks, then the transposed lower blocks), and the rows
+// in column order from the window: the sums of stencil_sym_node, so q is bit-identical to
+// the 1D multiply. p is read once per tile instead of once per row block.
+template <typename VT, int R>
+__device__ __forceinline__ void grid_spmv_tile(const PcgBuffers& b, const StencilView& v, const VT* __restrict__ planes,
+                                               const dpair* __restrict__ diag, int np_d, int np, int kb, int tile,
+                                               dpair* win, double& pq, int& qi, dpair& qv) {
+    const int n = v.n_nodes, L = v.row_len;
+    const int col_tiles = (L + kWave - 1) / kWave;
+    const int rt = tile / col_tiles;
+    const int c0 = (tile - rt * col_tiles) * kWave;
+    const int r = static_cast<int>(threadIdx.x) / kWave, l = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int i0 = rt * R * L + c0;
+    const int i = i0 + r * L + l;
+    const bool own = c0 + l < L && i < n;
+    const int ic = own ? i : 0;
+    const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(b.p);
+    const dpair po = p2[ic];
+    constexpr int kHaloN = grid_halo<R>();
+    constexpr int kHaloW = (kHaloN + R - 1) / R;
+    static_assert(kHaloW <= kWave, "one halo position per thread");
+    const int h = (l < kHaloW) ? r * kHaloW + l : kHaloN;
+    int hr = 0, hc = 0, hg = -1;
+    if (h < kHaloN) {
+        if (h < kGridWinCols) {
+            hr = -1;
+            hc = h - 1;
+        } else if (h < 2 * kGridWinCols) {
+            hr = R;
+            hc = h - kGridWinCols - 1;
+        } else {
+            hr = (h - 2 * kGridWinCols) >> 1;
+            hc = ((h - 2 * kGridWinCols) & 1) ? kWave : -1;
+        }
+        const int g = i0 + hr * L + hc;
+        hg = (g >= 0 && g < n) ? g : -1;
+    }
+    const dpair hp = hg >= 0 ? p2[hg] : dpair{0.0, 0.0};
+    SymRows<VT, 4> rows;
+    load_sym_upper<VT, 4>(diag, np_d, planes, np, kb, ic, rows);
+    win[grid_slot(r, l)] = own ? po : dpair{0.0, 0.0};
+    __builtin_amdgcn_sched_barrier(0);
+    load_sym_lower<VT, 4>(v, planes, np, kb, ic, rows);
+    if (h < kHaloN) win[grid_slot(hr, hc)] = hp;
+    __syncthreads();
+    if (own) {
+        const int c = grid_slot(r, l);
+        const int dw[4] = {1, kGridWinCols - 1, kGridWinCols, kGridWinCols + 1};
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int k = 3; k >= 0; --k)
+            if (rows.okl[k]) add_lower(rows.lu[k], rows.ll[k], win[c - dw[k]], a0, a1);
+        add_upper(rows.du, rows.dl, win[c], a0, a1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) add_upper(rows.uu[k], rows.ul[k], win[c + dw[k]], a0, a1);
+        if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;  // an earlier tile of this block
+        qi = i;
+        qv = dpair{a0, a1};
+        pq += po.x * a0 + po.y * a1;
+    }
+    __syncthreads();  // the window is reused by the block's next tile
+}
+
+// q = A p, p.q and alpha = rho / p.q (solver.cpp:589-597) on 2D tiles; q is stored after
+// the block's reduction ticket, as in pcg_fused_grid_kernel.
+template <int R>
+__global__ __launch_bounds__(R * kWave) void pcg_spmv_grid_kernel(PcgBuffers b, StencilView v,
+                                                                  const double* __restrict__ planes64,
+                                                                  const float* __restrict__ planes32,
+                                                                  const double* __restrict__ diag_c) {
+    if (b.st->done) return;
+    __shared__ dpair win[grid_win<R>()];
+    __shared__ double sh[R];
+    const bool use32 = planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
+    const int L = v.row_len;
+    const int n_tiles = ((v.n_nodes + L - 1) / L + R - 1) / R * ((L + kWave - 1) / kWave);
+    double pq = 0.0;
+    int qi = -1;
+    dpair qv = {0.0, 0.0};
+    for (int tile = xcd_block(blockIdx.x, gridDim.x); tile < n_tiles; tile += gridDim.x) {
+        if (use32)
+            grid_spmv_tile<float, R>(b, v, planes32, reinterpret_cast<const dpair*>(diag_c), 1, v.n_off - 1, 0, tile,
+                                     win, pq, qi, qv);
+        else
+            grid_spmv_tile<double, R>(b, v, planes64, reinterpret_cast<const dpair*>(planes64), v.n_off, v.n_off, 1,
+                                      tile, win, pq, qi, qv);
+    }
+    const double vals[1] = {block_sum<R * kWave>(pq, sh)};
+    double tot[1];
+    const bool last = finish_reduction<R * kWave, 1>(b.tickets + 1, b.partials, vals, tot, sh);
+    if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;
+    if (!last) return;
+    if (threadIdx.x == 0) {
+        const double tpq = tot[0];
+        PcgState* st = b.st;
+        st->pq = tpq;
+        if (!(tpq > 0.0) || !isfinite(tpq)) {  // solver.cpp:591-593
+            st->status = LFG_ERR_NOT_SPD;
+            st->done = 1;
+        } else {
+            st->alpha = st->rho / tpq;
+        }
+    }
+}
+
 // The x term an odd final launch deferred: x += alpha p_in of that launch. A solve
 // that stopped there (done) left alpha unchanged; otherwise the launch moved it to
 // alpha_prev.
@@ -1201,6 +1310,11 @@ void launch_pack_stencil(hipStream_t s, const CsrView& a, const StencilView& v, 
 
 void launch_pcg_spmv_stencil(hipStream_t s, const PcgBuffers& b, const StencilView& v, const double* planes64,
                              const float* planes32, const double* diag_c) {
+    if (stencil_grid_multiply(v)) {
+        pcg_spmv_grid_kernel<kGridRows><<<stencil_grid_blocks(v), kGridRows * kWave, 0, s>>>(b, v, planes64, planes32,
+                                                                                          diag_c);
+        return;
+    }
     const unsigned int grid = stencil_spmv_blocks(v.n_nodes);
     if (v.sym) {
         // Q4 meshes numbered row by row have 4 positive offsets; other counts take the

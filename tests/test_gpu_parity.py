@@ -465,12 +465,15 @@ def test_grid_tiles_on_uneven_meshes(dev, nx, ny):
     summed as the reference) within 1e-9 for both fused preconditioners, in FP64 and with
     FP32 values (mixed, with its FP64 refinement); the iteration count of the 1D-tiled
     fused iteration (variant 10: the same arithmetic, partial sums grouped by other tiles)
-    within 1% + 2 in FP64; two runs bit-identical."""
+    within 1% + 2 in FP64; two runs bit-identical. The stand-alone symmetric multiply (variant
+    8, on the same 2D tiles here) against the CSR product and its FP32 copy's bound."""
     print(f"case grid tiles {nx}x{ny}", flush=True)
     m = meshgen.structured_cantilever(nx, ny, lx=float(nx), ly=float(ny),
                                       young=meshgen.loguniform_young(nx * ny, nx + ny), plane_mode="plane_strain")
-    pipeline.setup(m, device=dev, assembly="coloring")
+    sys_ = pipeline.setup(m, device=dev, assembly="coloring")
     assert dev.layout_info()["grid_row_len"] == nx + 1
+    kc, _ = dev.system()
+    check_stencil_spmv(dev, sys_.row_ptr, sys_.col, kc, np.random.default_rng(nx).standard_normal(dev.n))
     for precond in ("jacobi", "block"):
         u6, r6 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=6)
         _, r10 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=10)
