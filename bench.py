@@ -43,6 +43,14 @@ KERNELS = {1: "pcg_spmv_kernel<4>", 2: "pcg_spmv_kernel<8>", 3: "pcg_spmv_kernel
            12: "pcg_fused_slices_kernel<9, 0>", 13: "pcg_slices_multiply_kernel<9, 0>"}
 
 
+def kernel_name(variant: int, layout: dict) -> str:
+    """The kernel a launch of `variant` runs on this layout (variant 8 takes the 2D tiles of
+    the fused grid iteration where the stencil is row-structured)."""
+    if variant == 8 and layout.get("grid_row_len"):
+        return "pcg_spmv_grid_kernel<4>"
+    return KERNELS[variant]
+
+
 def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int, layout: dict) -> int:
     """Algorithmic bytes of one SpMV launch: the values, the column indices of the
     layout, its row (or slice) offsets, x read once and y written once. CSR variants
@@ -394,7 +402,7 @@ def main() -> int:
         variant = args.variant or default_variant  # the multiply the default PCG iteration runs
         ms = dev.bench_spmv("fp64", variant, args.profile)
         b = spmv_bytes(n, nnz, 8, variant, layout)
-        prof = {"kernel": KERNELS[variant], "launches": args.profile, "avg_launch_ms": ms, "bytes": b,
+        prof = {"kernel": kernel_name(variant, layout), "launches": args.profile, "avg_launch_ms": ms, "bytes": b,
                 "GBs": b / (ms * 1e-3) / 1e9}
         print(f"bench: profile spmv {json.dumps(prof)}", file=sys.stderr, flush=True)
         per_graph = args.check_every or 32
@@ -425,7 +433,7 @@ def main() -> int:
                 continue
             ms = dev.bench_spmv(precision, variant, 50)
             b = spmv_bytes(n, nnz, vbytes, variant, layout)
-            sweep[precision][variant] = {"kernel": KERNELS[variant], "ms": ms, "bytes": b,
+            sweep[precision][variant] = {"kernel": kernel_name(variant, layout), "ms": ms, "bytes": b,
                                          "GBs": b / (ms * 1e-3) / 1e9}
         # the multiply alone (the stand-alone kernel of the solve's layout) and the kernel the
         # PCG iteration spends its time in (the same, or the fused iteration)
