@@ -40,7 +40,8 @@ KERNELS = {1: "pcg_spmv_kernel<4>", 2: "pcg_spmv_kernel<8>", 3: "pcg_spmv_kernel
            6: "pcg_spmv_slices_kernel<9>", 7: "pcg_spmv_slices_dir_kernel<9, 0>",
            8: "pcg_spmv_stencil_kernel<4, true>", 9: "pcg_spmv_stencil_kernel<9, false>",
            10: "pcg_fused_stencil_kernel<4, 0>", 11: "pcg_fused_grid_kernel<0, 4>",
-           12: "pcg_fused_slices_kernel<9, 0>", 13: "pcg_slices_multiply_kernel<9, 0>"}
+           12: "pcg_fused_slices_kernel<9, 0>", 13: "pcg_slices_multiply_kernel<9, 0>",
+           14: "pcg_fold_slices_kernel<9, 0>"}
 
 
 def kernel_name(variant: int, layout: dict) -> str:
@@ -75,10 +76,10 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int, layout: dict) -
         return matrix + 16 * n
     if variant == 9:
         return n_nodes * layout["stencil_exact_planes"] * 32 + 16 * n
-    if variant in (6, 7, 12, 13):
+    if variant in (6, 7, 12, 13, 14):
         n_slices = (n // 2 + 63) // 64
         matrix = nnz * value_bytes + 4 * (nnz // 4) + 4 * (n_slices + 1)
-        if variant == 12:
+        if variant in (12, 14):
             # the whole Jacobi iteration as in 10 and 11 (the column nodes' r, q, p are
             # gathered from the XCD's L2; each counted once): 136 B per node
             return matrix + n_nodes * 136
@@ -395,7 +396,7 @@ def main() -> int:
     # the library's default Jacobi iteration: fused stencil, stencil multiply, node slices
     default_variant = 11 if layout["grid_row_len"] else (
         10 if layout["fused_window_pairs"] else (8 if layout["stencil_sym_planes"] else (
-            12 if layout["n_slices"] <= 1024 else 13)))
+            14 if layout["n_slices"] <= 1024 else 13)))
     if args.profile:
         # Profiling run: the FP64 SpMV of the headline variant alone (args.profile launches),
         # then FP64 Jacobi-PCG iterations (a whole number of captured graphs), no other kernels.
@@ -424,8 +425,9 @@ def main() -> int:
     spmv, sweep, iteration_kernel = {}, {}, {}
     for precision, vbytes in (("fp64", 8), ("mixed", 4)):
         sweep[precision] = {}
-        for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13):
-            if (variant in (6, 7, 12, 13) and not layout["node_blocked"]) or (
+        for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14):
+            if (variant in (6, 7, 12, 13, 14) and not layout["node_blocked"]) or (
+                    variant == 14 and layout["n_slices"] > 1024) or (
                     variant == 8 and not layout["stencil_sym_planes"]) or (variant == 9 and (
                     precision == "mixed" or not layout["stencil_exact_planes"])) or (
                     variant == 10 and not layout["fused_window_pairs"]) or (
@@ -437,7 +439,7 @@ def main() -> int:
                                          "GBs": b / (ms * 1e-3) / 1e9}
         # the multiply alone (the stand-alone kernel of the solve's layout) and the kernel the
         # PCG iteration spends its time in (the same, or the fused iteration)
-        multiply = 8 if solve_variant in (10, 11) else (6 if solve_variant in (12, 13) else solve_variant)
+        multiply = 8 if solve_variant in (10, 11) else (6 if solve_variant in (12, 13, 14) else solve_variant)
         spmv[precision] = dict(sweep[precision][multiply], variant=multiply)
         iteration_kernel[precision] = dict(sweep[precision][solve_variant], variant=solve_variant)
     print(f"bench: spmv {json.dumps(spmv)}", file=sys.stderr, flush=True)

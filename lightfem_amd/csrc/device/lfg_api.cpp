@@ -159,6 +159,8 @@ struct lfg_ctx {
     DevArray<double> lam, adj_rhs, dE, grad;  // adjoint solution and right-hand side, sensitivities
     bool have_adjoint = false;
     DevArray<lfg::PcgState> state;
+    DevArray<lfg::PcgState> state_alt;  // variant 14: the state of odd launches
+    DevArray<double> fold_partials;     // variant 14: two sets of 5 sums per block
     DevArray<lfg::TicketSet> tickets;  // 0 init, 1 SpMV, 2 update, 3 replacement, 4 residual, 5 subdomains
     DevArray<int> flags;
     lfg::PcgState* h_state = nullptr;  // pinned, 2 slots
@@ -289,24 +291,34 @@ constexpr int kStencilFused = 10; // symmetric node stencil, whole Jacobi / bloc
 constexpr int kStencilGrid = 11;  // the iteration of 10 on 2D tiles (row-structured stencils)
 constexpr int kSliceIter = 12;    // node slices, whole Jacobi / block PCG iteration in one launch
 constexpr int kSliceSplit = 13;   // the iteration of 12 in two launches (vector update, multiply)
-constexpr int kMaxVariant = kSliceSplit;
+constexpr int kSliceFold = 14;    // the iteration of 12, its reduction folded by the next launch
+constexpr int kMaxVariant = kSliceFold;
 constexpr int kSliceIterMaxSlices = 1024;  // above this the split iteration (13) is the faster default
+static_assert(kSliceIterMaxSlices <= lfg::kFoldMaxSlices, "variant 14 serves the one-launch default");
+constexpr int kFoldPartials = 5 * lfg::kFoldMaxBlocks;  // one set of variant 14's partial sums
 
 bool slice_layout(int variant) {
-    return variant == kSliceVariant || variant == kSliceFused || variant == kSliceIter || variant == kSliceSplit;
+    return variant == kSliceVariant || variant == kSliceFused || variant == kSliceIter || variant == kSliceSplit ||
+           variant == kSliceFold;
 }
 bool stencil_layout(int variant) {
     return variant == kStencilSym || variant == kStencilExact || variant == kStencilFused || variant == kStencilGrid;
 }
 // The whole Jacobi / block iteration in one launch (1D or 2D stencil tiles, node slices).
 bool fused_iteration(int variant) {
-    return variant == kStencilFused || variant == kStencilGrid || variant == kSliceIter || variant == kSliceSplit;
+    return variant == kStencilFused || variant == kStencilGrid || variant == kSliceIter || variant == kSliceSplit ||
+           variant == kSliceFold;
+}
+// The node-slice iterations of one or two launches per iteration (Jacobi and block only).
+bool slice_iteration_variant(int variant) {
+    return variant == kSliceIter || variant == kSliceSplit || variant == kSliceFold;
 }
 bool grid_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stencil_sym.row_len > 0; }
 bool fused_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stencil_sym.win_total > 0; }
 // The fused node-slice iteration for Jacobi / block on a node-blocked pattern: one launch
-// per iteration for small and medium systems, two for large ones.
-int slice_iteration(const lfg_ctx* c) { return c->n_slices <= kSliceIterMaxSlices ? kSliceIter : kSliceSplit; }
+// per iteration, its sums folded by the next launch, for small and medium systems; two
+// launches for large ones.
+int slice_iteration(const lfg_ctx* c) { return c->n_slices <= kSliceIterMaxSlices ? kSliceFold : kSliceSplit; }
 
 // 0 selects, in order: the fused stencil iteration (Jacobi and block preconditioners),
 // the symmetric node-stencil multiply, the node-slice multiply, row-segment streaming.
@@ -325,7 +337,7 @@ int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JAC
                                                                                                   : slice_iteration(c)))))
                                    : variant;
         require(v >= kSliceVariant && v <= kMaxVariant, LFG_ERR_UNSUPPORTED,
-                "block storage runs on the node-slice (6, 7, 12) and node-stencil (8-11) SpMVs");
+                "block storage runs on the node-slice (6, 7, 12-14) and node-stencil (8-11) SpMVs");
         return resolve_variant(c, v, precond, false);
     }
     if (variant == 0) {
@@ -335,8 +347,8 @@ int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JAC
         if (c->have_slices) return sweep_precond(precond) ? kSliceVariant : slice_iteration(c);
         return 4;
     }
-    require(variant >= 1 && variant <= kMaxVariant, LFG_ERR_ARG, "SpMV variant must be 0-13");
-    require((variant != kSliceIter && variant != kSliceSplit) || !sweep_precond(precond), LFG_ERR_UNSUPPORTED,
+    require(variant >= 1 && variant <= kMaxVariant, LFG_ERR_ARG, "SpMV variant must be 0-14");
+    require(!slice_iteration_variant(variant) || !sweep_precond(precond), LFG_ERR_UNSUPPORTED,
             "the fused node-slice iteration applies the jacobi and 2x2 block preconditioners, not ic0 or subdomains");
     require(variant != kStencilGrid || grid_stencil_ok(c), LFG_ERR_UNSUPPORTED,
             "the 2D-tiled fused iteration needs a symmetric node stencil with positive offsets 1, L-1, L, L+1 "
@@ -350,6 +362,8 @@ int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JAC
             "the fused stencil iteration applies the jacobi and 2x2 block preconditioners, not ic0 or subdomains");
     require(!slice_layout(variant) || c->have_slices, LFG_ERR_UNSUPPORTED,
             "the node-slice SpMV needs rows 2i, 2i+1 sharing one column list of dof pairs");
+    require(variant != kSliceFold || c->n_slices <= kSliceIterMaxSlices, LFG_ERR_UNSUPPORTED,
+            "the folded node-slice iteration (14) takes at most 1,024 slices of 64 nodes; use 12 or 13");
     require(variant != kStencilSym || c->have_stencil_sym, LFG_ERR_UNSUPPORTED,
             "the symmetric node-stencil SpMV needs a symmetric node-blocked pattern with at most 8 positive node "
             "offsets");
@@ -926,6 +940,8 @@ void ensure_vectors(lfg_ctx* c) {
     c->scratch.alloc(4096 + 8);
     if (!c->state.p) {
         c->state.alloc(1);
+        c->state_alt.alloc(1);
+        c->fold_partials.alloc(2 * kFoldPartials);
         LFG_HIP(hipMemsetAsync(c->state.p, 0, sizeof(lfg::PcgState), c->stream));
         c->tickets.alloc(6);
         LFG_HIP(hipMemsetAsync(c->tickets.p, 0, 6 * sizeof(lfg::TicketSet), c->stream));
@@ -997,6 +1013,14 @@ void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int repl
         double* const rb[2] = {c->r.p, c->r_alt.p};
         double* const pb[2] = {c->p.p, c->p2.p};
         double* const qb[2] = {c->q.p, c->q_alt.p};
+        if (variant == kSliceFold) {  // state and partials alternate with the launch parity
+            lfg::PcgState* const sb[2] = {b.st, c->state_alt.p};
+            double* const fb[2] = {c->fold_partials.p, c->fold_partials.p + kFoldPartials};
+            lfg::launch_pcg_fold_slices(c->stream, b, slice_view(c), c->slices64.p, mixed ? c->slices32.p : nullptr,
+                                        rb[parity], rb[parity ^ 1], pb[parity], pb[parity ^ 1], qb[parity],
+                                        qb[parity ^ 1], sb[parity], sb[parity ^ 1], fb[parity ^ 1], fb[parity]);
+            return;
+        }
         if (variant == kSliceIter || variant == kSliceSplit) {
             (variant == kSliceIter ? lfg::launch_pcg_fused_slices : lfg::launch_pcg_split_slices)(
                 c->stream, b, slice_view(c), c->slices64.p, mixed ? c->slices32.p : nullptr, rb[parity],
@@ -1612,9 +1636,10 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
             lfg::launch_pcg_subdomain_step(ctx->stream, b, subdomain_view(ctx));
             lfg::launch_pcg_direction(ctx->stream, b);
         }
-        // every iteration the cap allows fits in this many graph launches
-        // (the fused stencil iteration spends its first launch on p0 and A p0)
-        const long long launches_needed = max_iter + (stencil_fused ? 1 : 0);
+        // every iteration the cap allows fits in this many graph launches (a fused iteration
+        // spends its first launch on p0 and A p0; variant 14 folds the last sums one launch
+        // later)
+        const long long launches_needed = max_iter + (stencil_fused ? 1 : 0) + (variant == kSliceFold ? 1 : 0);
         const long long launch_cap = std::max<long long>(1, (launches_needed + per_graph - 1) / per_graph);
         long long launched = 0;
         auto launch_one = [&](int slot) {

@@ -37,6 +37,7 @@ struct PcgState {
     int need_replace;  // mixed: the replacement kernel runs this iteration
     int phase32;       // mixed: SpMVs read the FP32 values
     int replaced;      // mixed: this iteration's residual is the FP64 one in z (accepted replacement)
+    int pending;       // variant 14: the last launch's partial sums are still to be folded
 };
 
 // Completion counters of one reducing kernel: one per XCD and one on top, each on

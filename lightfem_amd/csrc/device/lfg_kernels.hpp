@@ -245,6 +245,19 @@ void launch_pcg_fused_grid(hipStream_t s, const PcgBuffers& b, const StencilView
 void launch_pcg_fused_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
                              const float* planes32, const double* r_in, double* r_out, const double* p_in,
                              double* p_out, const double* q_in, double* q_out);
+// Variant 14's blocks: kFoldThreads threads, one node slice per wavefront, at most
+// kFoldMaxBlocks blocks (1,024 slices); part_in / part_out hold 5 doubles per block.
+constexpr int kFoldThreads = 256;
+constexpr int kFoldMaxSlices = 1024;
+constexpr int kFoldMaxBlocks = kFoldMaxSlices / (kFoldThreads / 64);
+// Variant 14: the iteration of variant 12 with the reduction finished at the start of the
+// next launch (same bits). The state and the partial sums alternate by launch parity: a
+// launch reads st_in and part_in (5 doubles per block of the previous launch) and writes
+// st_out and part_out. At most 1,024 slices (one slice per wavefront, one-level fold).
+void launch_pcg_fold_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
+                            const float* planes32, const double* r_in, double* r_out, const double* p_in,
+                            double* p_out, const double* q_in, double* q_out, const PcgState* st_in,
+                            PcgState* st_out, const double* part_in, double* part_out);
 // Variant 13: the same iteration in two launches on the node slices (the vector update,
 // then the multiply with the state update); same buffers.
 void launch_pcg_split_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,

@@ -282,6 +282,7 @@ __global__ __launch_bounds__(kThreads) void pcg_init_kernel(PcgBuffers b, int st
         st->drop = drop;
         st->need_replace = 0;
         st->replaced = 0;
+        st->pending = 0;
         st->phase32 = switch_tau < 0.0 ? 1 : 0;
         st->true_rr = 0.0;
         st->switch_rr = switch_tau > 0.0 ? (switch_tau * rnorm0) * (switch_tau * rnorm0) : -1.0;

@@ -8,6 +8,8 @@
 // offsets and reads block (i, i - d) as the transpose of block (i - d, i): the
 // second read of a plane trails the first by d nodes and is served from the XCD's
 // L2 (each XCD walks one contiguous node range), so HBM sees each value once.
+#include <stdexcept>
+
 #include "lfg_kernels.hpp"
 #include "lfg_precond.hpp"
 #include "lfg_slices.hpp"
@@ -783,7 +785,7 @@ __global__ __launch_bounds__(R * kWave) void pcg_fused_grid_kernel(
     if (threadIdx.x == 0) fused_state_update(b.st, tot, alpha);
 }
 
-// ---- the fused iteration on the node-slice layout (variant 12)
+// ---- the fused iteration on the node-slice layout (variants 12 and 14)
 //
 // The iteration of pcg_fused_stencil_kernel for any node-blocked pattern (RCM or other
 // numberings, where the blocks lie at no fixed node offsets). A wavefront owns one slice of
@@ -794,7 +796,132 @@ __global__ __launch_bounds__(R * kWave) void pcg_fused_grid_kernel(
 // the launch) with the owner's arithmetic, so it has the bits the owner writes and no
 // exchange between blocks is needed. The column gathers hit the XCD's L2: slices are
 // XCD-remapped in contiguous ranges and a node's columns lie within the numbering's
-// bandwidth.
+// bandwidth. issue() starts every load that does not need the launch's scalars; finish()
+// waits for them and does the arithmetic, so the scalars can arrive in between.
+template <typename VT, int kW, int Prec>
+struct SliceStep {
+    using P = typename PairOf<VT>::type;
+    int node, ic, begin, width;
+    bool own, unrolled;
+    dpair ro, qo, po, xo, pp;
+    NodePrecond<Prec> mo;
+    P u[kW > 0 ? kW : 1], l[kW > 0 ? kW : 1];
+    int c[kW > 0 ? kW : 1];
+
+    // The own node's r, q, p and preconditioner, the slice's blocks (up to kW: indices
+    // clamped to the last block), and x and the older p when the launch updates x.
+    __device__ __forceinline__ void issue(const PcgBuffers& b, const SliceView& v, const VT* __restrict__ planes,
+                                          int s, const double* __restrict__ r_in, const double* __restrict__ p_in,
+                                          const double* __restrict__ p_out, const double* __restrict__ q_in,
+                                          int x_mode) {
+        const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+        node = s * kWave + lane;
+        own = node < (b.n >> 1);
+        ic = own ? node : 0;
+        ro = reinterpret_cast<const dpair*>(r_in)[ic];
+        qo = reinterpret_cast<const dpair*>(q_in)[ic];
+        po = reinterpret_cast<const dpair*>(p_in)[ic];
+        mo.load(b.m, ic);
+        begin = v.ptr[s];
+        width = (v.ptr[s + 1] - begin) / kWave;
+        const P* __restrict__ upper = reinterpret_cast<const P*>(planes);
+        const P* __restrict__ lower = upper + v.blocks;
+        unrolled = kW > 0 && width > 0 && width <= kW;
+        if (unrolled) {
+#pragma unroll
+            for (int k = 0; k < kW; ++k) {
+                const int at = begin + min(k, width - 1) * kWave + lane;
+                u[k] = __builtin_nontemporal_load(upper + at);
+                l[k] = __builtin_nontemporal_load(lower + at);
+                c[k] = __builtin_nontemporal_load(v.col + at);
+            }
+        }
+        xo = dpair{0.0, 0.0};
+        pp = dpair{0.0, 0.0};
+        if (x_mode != 0) {
+            xo = reinterpret_cast<const dpair*>(b.x)[ic];
+            if (x_mode == 2) pp = reinterpret_cast<const dpair*>(p_out)[ic];
+        }
+    }
+
+    // The own node's update (stores r', p', x) and q' = A p' with the sums into acc; q' is
+    // kept in (qi, qv) and stored by the caller, an earlier slice's here.
+    __device__ __forceinline__ void finish(const PcgBuffers& b, const SliceView& v, const VT* __restrict__ planes,
+                                           const double* __restrict__ r_in, double* __restrict__ r_out,
+                                           const double* __restrict__ p_in, double* __restrict__ p_out,
+                                           const double* __restrict__ q_in, double alpha, double beta,
+                                           double alpha_prev, int x_mode, double (&acc)[5], int& qi, dpair& qv) {
+        const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+        // p' of column node j from the launch's inputs, with the owner's arithmetic
+        auto column = [&](int j) {
+            const dpair rj = reinterpret_cast<const dpair*>(r_in)[j], qj = reinterpret_cast<const dpair*>(q_in)[j],
+                        pj = reinterpret_cast<const dpair*>(p_in)[j];
+            double z0, z1;
+            precond_node<Prec>(b.m, j, rj.x - alpha * qj.x, rj.y - alpha * qj.y, z0, z1);
+            return dpair{z0 + beta * pj.x, z1 + beta * pj.y};
+        };
+        // 1. own node (solver.cpp:599-604, 612-620), x two iterations at a time
+        dpair pn = {0.0, 0.0}, zn = {0.0, 0.0};
+        if (own) {
+            if (x_mode != 0) {
+                if (x_mode == 2) {
+                    xo.x += alpha_prev * pp.x;
+                    xo.y += alpha_prev * pp.y;
+                }
+                xo.x += alpha * po.x;
+                xo.y += alpha * po.y;
+                reinterpret_cast<dpair*>(b.x)[node] = xo;
+            }
+            const dpair rn = {ro.x - alpha * qo.x, ro.y - alpha * qo.y};
+            double z0, z1;
+            mo.apply(rn.x, rn.y, z0, z1);
+            zn = dpair{z0, z1};
+            pn = dpair{z0 + beta * po.x, z1 + beta * po.y};
+            reinterpret_cast<dpair*>(r_out)[node] = rn;
+            reinterpret_cast<dpair*>(p_out)[node] = pn;
+            acc[0] += rn.x * rn.x + rn.y * rn.y;
+            acc[1] += rn.x * z0 + rn.y * z1;
+        }
+        // 2. q' = A p' in column order (sparse.cpp:104-120; block order with v.pair_sum)
+        double a0 = 0.0, a1 = 0.0;
+        if (unrolled) {
+#pragma unroll
+            for (int k = 0; k < kW; ++k) {
+                if (k < width) {
+                    const dpair x = column(c[k]);
+                    add_block(a0, a1, static_cast<double>(u[k].x), static_cast<double>(u[k].y),
+                              static_cast<double>(l[k].x), static_cast<double>(l[k].y), x, v.pair_sum != 0);
+                }
+            }
+        } else {
+            const P* __restrict__ upper = reinterpret_cast<const P*>(planes);
+            const P* __restrict__ lower = upper + v.blocks;
+            for (int k = 0; k < width; ++k) {
+                const int at = begin + k * kWave + lane;
+                const P uk = __builtin_nontemporal_load(upper + at);
+                const P lk = __builtin_nontemporal_load(lower + at);
+                const dpair x = column(__builtin_nontemporal_load(v.col + at));
+                add_block(a0, a1, static_cast<double>(uk.x), static_cast<double>(uk.y), static_cast<double>(lk.x),
+                          static_cast<double>(lk.y), x, v.pair_sum != 0);
+            }
+        }
+        if (own) {
+            if (sizeof(VT) == sizeof(float)) add_diag32(v, node, pn, a0, a1);
+            if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;  // an earlier slice of this wavefront
+            qi = node;
+            qv = dpair{a0, a1};
+            double m0, m1;
+            mo.apply(a0, a1, m0, m1);
+            acc[2] += pn.x * a0 + pn.y * a1;
+            acc[3] += a0 * zn.x + a1 * zn.y;
+            acc[4] += a0 * m0 + a1 * m1;
+        }
+    }
+};
+
+// One slice of variant 12: the scalars are read at the kernel's start; the launch parity
+// (x_mode) is known before the loads, and a finished solve's launch stops after issuing
+// them, before any store.
 template <typename VT, int kW, int Prec>
 __device__ __forceinline__ void fused_slice(const PcgBuffers& b, const SliceView& v, const VT* __restrict__ planes,
                                             int s, const double* __restrict__ r_in, double* __restrict__ r_out,
@@ -802,103 +929,10 @@ __device__ __forceinline__ void fused_slice(const PcgBuffers& b, const SliceView
                                             const double* __restrict__ q_in, double alpha, double beta,
                                             double alpha_prev, int x_mode, int done, double (&acc)[5], int& qi,
                                             dpair& qv) {
-    using P = typename PairOf<VT>::type;
-    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-    const int node = s * kWave + lane;
-    const bool own = node < (b.n >> 1);
-    const int ic = own ? node : 0;
-    dpair* __restrict__ x2 = reinterpret_cast<dpair*>(b.x);
-    const dpair ro = reinterpret_cast<const dpair*>(r_in)[ic], qo = reinterpret_cast<const dpair*>(q_in)[ic],
-                po = reinterpret_cast<const dpair*>(p_in)[ic];
-    NodePrecond<Prec> mo;
-    mo.load(b.m, ic);
-    const int begin = v.ptr[s];
-    const int width = (v.ptr[s + 1] - begin) / kWave;
-    const P* __restrict__ upper = reinterpret_cast<const P*>(planes);
-    const P* __restrict__ lower = upper + v.blocks;
-    // the slice's blocks (up to kW: every load issued here, indices clamped to the last block)
-    const bool unrolled = kW > 0 && width > 0 && width <= kW;
-    P u[kW > 0 ? kW : 1], l[kW > 0 ? kW : 1];
-    int c[kW > 0 ? kW : 1];
-    if (unrolled) {
-#pragma unroll
-        for (int k = 0; k < kW; ++k) {
-            const int at = begin + min(k, width - 1) * kWave + lane;
-            u[k] = __builtin_nontemporal_load(upper + at);
-            l[k] = __builtin_nontemporal_load(lower + at);
-            c[k] = __builtin_nontemporal_load(v.col + at);
-        }
-    }
-    // x and the older p (the launch parity comes from the state, read at the kernel's start
-    // and waited for only here); a finished solve's launch stops here, before any store
-    dpair xo = {0.0, 0.0}, pp = {0.0, 0.0};
-    if (x_mode != 0) {
-        xo = x2[ic];
-        if (x_mode == 2) pp = reinterpret_cast<const dpair*>(p_out)[ic];
-    }
+    SliceStep<VT, kW, Prec> step;
+    step.issue(b, v, planes, s, r_in, p_in, p_out, q_in, x_mode);
     if (done) return;
-    // p' of column node j from the launch's inputs, with the owner's arithmetic
-    auto column = [&](int j) {
-        const dpair rj = reinterpret_cast<const dpair*>(r_in)[j], qj = reinterpret_cast<const dpair*>(q_in)[j],
-                    pj = reinterpret_cast<const dpair*>(p_in)[j];
-        double z0, z1;
-        precond_node<Prec>(b.m, j, rj.x - alpha * qj.x, rj.y - alpha * qj.y, z0, z1);
-        return dpair{z0 + beta * pj.x, z1 + beta * pj.y};
-    };
-    // 1. own node (solver.cpp:599-604, 612-620), x two iterations at a time
-    dpair pn = {0.0, 0.0}, zn = {0.0, 0.0};
-    if (own) {
-        if (x_mode != 0) {
-            if (x_mode == 2) {
-                xo.x += alpha_prev * pp.x;
-                xo.y += alpha_prev * pp.y;
-            }
-            xo.x += alpha * po.x;
-            xo.y += alpha * po.y;
-            x2[node] = xo;
-        }
-        const dpair rn = {ro.x - alpha * qo.x, ro.y - alpha * qo.y};
-        double z0, z1;
-        mo.apply(rn.x, rn.y, z0, z1);
-        zn = dpair{z0, z1};
-        pn = dpair{z0 + beta * po.x, z1 + beta * po.y};
-        reinterpret_cast<dpair*>(r_out)[node] = rn;
-        reinterpret_cast<dpair*>(p_out)[node] = pn;
-        acc[0] += rn.x * rn.x + rn.y * rn.y;
-        acc[1] += rn.x * z0 + rn.y * z1;
-    }
-    // 2. q' = A p' in column order (sparse.cpp:104-120; block order with v.pair_sum)
-    double a0 = 0.0, a1 = 0.0;
-    if (unrolled) {
-#pragma unroll
-        for (int k = 0; k < kW; ++k) {
-            if (k < width) {
-                const dpair x = column(c[k]);
-                add_block(a0, a1, static_cast<double>(u[k].x), static_cast<double>(u[k].y),
-                          static_cast<double>(l[k].x), static_cast<double>(l[k].y), x, v.pair_sum != 0);
-            }
-        }
-    } else {
-        for (int k = 0; k < width; ++k) {
-            const int at = begin + k * kWave + lane;
-            const P uk = __builtin_nontemporal_load(upper + at);
-            const P lk = __builtin_nontemporal_load(lower + at);
-            const dpair x = column(__builtin_nontemporal_load(v.col + at));
-            add_block(a0, a1, static_cast<double>(uk.x), static_cast<double>(uk.y), static_cast<double>(lk.x),
-                      static_cast<double>(lk.y), x, v.pair_sum != 0);
-        }
-    }
-    if (own) {
-        if (sizeof(VT) == sizeof(float)) add_diag32(v, node, pn, a0, a1);
-        if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;  // an earlier slice of this wavefront
-        qi = node;
-        qv = dpair{a0, a1};
-        double m0, m1;
-        mo.apply(a0, a1, m0, m1);
-        acc[2] += pn.x * a0 + pn.y * a1;
-        acc[3] += a0 * zn.x + a1 * zn.y;
-        acc[4] += a0 * m0 + a1 * m1;
-    }
+    step.finish(b, v, planes, r_in, r_out, p_in, p_out, q_in, alpha, beta, alpha_prev, x_mode, acc, qi, qv);
 }
 
 // One fused PCG iteration per launch on the node slices (launch_pcg_fused_slices): each
@@ -938,6 +972,125 @@ __global__ __launch_bounds__(kThreads) void pcg_fused_slices_kernel(
     if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;
     if (!last) return;
     if (threadIdx.x == 0) fused_state_update(b.st, tot, alpha);
+}
+
+// ---- variant 14: the iteration of variant 12 with its reduction finished at the start of
+// the next launch.
+//
+// Launch L stores its blocks' partial sums and ends: no ticket, no wait for the stores, no
+// last-block fold, which is the chain of dependent memory round trips a small system's
+// iteration waits for. Every block of launch L + 1 reads those partials while its slice
+// loads are in flight, folds them in the order of finish_reduction's one-level path (so the
+// totals, and every iterate, have the bits of variant 12), and applies the state update of
+// variant 12 itself; block 0 writes the updated state for the host and the next launch. The
+// state and the partials alternate between two buffers by launch parity (st_in / st_out,
+// part_in / part_out): no block overwrites what another block of the same launch reads.
+// `pending` in the state says whether part_in holds sums still to fold. The host sees a
+// finished solve one launch later than with variant 12. Blocks of kFoldThreads threads, one
+// slice per wavefront, at most kFoldMaxBlocks blocks (launch_pcg_fold_slices).
+constexpr int kFoldParts = (kFoldMaxBlocks + kFoldThreads - 1) / kFoldThreads;  // partials per thread
+
+// Folds the previous launch's partials into `cur` (the state that launch used, in
+// registers): every thread gets the totals (its wave's sums through LDS, added in wave
+// order as block_sums adds them for thread 0) and applies the state update itself, so no
+// broadcast is needed. part[i] holds block t + i kFoldThreads's five sums (thread t; n_part
+// of them exist); `sh` holds 5 doubles per wave. Block 0 publishes the result.
+__device__ __forceinline__ void fold_previous(PcgState& cur, PcgState* __restrict__ st_out,
+                                              const double (&part)[kFoldParts][5], int n_part, double* sh) {
+    constexpr int W = kFoldThreads / kWave;
+    if (cur.pending) {
+        double tot[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            double acc = 0.0;  // finish_reduction's order: blocks t, t + NT, ...
+#pragma unroll
+            for (int i = 0; i < kFoldParts; ++i)
+                if (i < n_part) acc += part[i][k];
+            tot[k] = wave_sum(acc);
+        }
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) sh[k * W + threadIdx.x / kWave] = tot[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            double acc = 0.0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) acc += sh[k * W + w];
+            tot[k] = acc;
+        }
+        fused_state_update(&cur, tot, cur.alpha);
+    }
+    cur.pending = cur.done ? 0 : 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // word by word: a struct copy keeps `cur` out of registers
+        static_assert(sizeof(PcgState) % 8 == 0, "state words");
+        const long long* src = reinterpret_cast<const long long*>(&cur);
+        long long* dst = reinterpret_cast<long long*>(st_out);
+#pragma unroll
+        for (int i = 0; i < static_cast<int>(sizeof(PcgState) / 8); ++i) dst[i] = src[i];
+    }
+}
+
+template <int kW, int Prec>
+__global__ __launch_bounds__(kFoldThreads) void pcg_fold_slices_kernel(
+    PcgBuffers b, SliceView v, const double* __restrict__ planes64, const float* __restrict__ planes32,
+    const double* __restrict__ r_in, double* __restrict__ r_out, const double* __restrict__ p_in,
+    double* __restrict__ p_out, const double* __restrict__ q_in, const PcgState* __restrict__ st_in,
+    PcgState* __restrict__ st_out, const double* __restrict__ part_in, double* __restrict__ part_out) {
+    __shared__ double sh[5 * (kFoldThreads / kWave)];
+    __shared__ double sh_fold[5 * (kFoldThreads / kWave)];
+    const int t = static_cast<int>(threadIdx.x);
+    const int nb = static_cast<int>(gridDim.x);
+    // the previous launch's partials, read whether or not they are pending: no wait on the state
+    const int n_part = t < nb ? (nb - 1 - t) / kFoldThreads + 1 : 0;
+    double part[kFoldParts][5];
+#pragma unroll
+    for (int i = 0; i < kFoldParts; ++i) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) part[i][k] = i < n_part ? part_in[(t + i * kFoldThreads) * 5 + k] : 0.0;
+    }
+    PcgState cur = *st_in;
+    const long long launch = cur.iter + cur.pending + 1;  // the iteration count after the fold, plus one
+    const int x_mode = (launch & 1) ? 0 : (launch >= 2 ? 2 : 1);
+    const bool use32_in = planes32 != nullptr && (planes64 == nullptr || cur.phase32 != 0);
+    const int lb = xcd_block(blockIdx.x, nb);
+    const int per_block = (v.n_slices + nb - 1) / nb;  // at most one slice per wavefront
+    const int s = lb * per_block + t / kWave;
+    const bool has = t / kWave < per_block && s < v.n_slices;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // r'.r', r'.z, p'.q', q'.z, q'.M^-1 q'
+    int qi = -1;
+    dpair qv = {0.0, 0.0};
+    if (use32_in) {  // the FP32 phase never ends within a solve
+        SliceStep<float, kW, Prec> step;
+        if (has) step.issue(b, v, planes32, s, r_in, p_in, p_out, q_in, x_mode);
+        fold_previous(cur, st_out, part, n_part, sh_fold);
+        if (has && !cur.done)
+            step.finish(b, v, planes32, r_in, r_out, p_in, p_out, q_in, cur.alpha, cur.beta, cur.alpha_prev, x_mode,
+                        acc, qi, qv);
+    } else {
+        SliceStep<double, kW, Prec> step;
+        if (has) step.issue(b, v, planes64, s, r_in, p_in, p_out, q_in, x_mode);
+        fold_previous(cur, st_out, part, n_part, sh_fold);
+        if (has && !cur.done) {
+            if (planes32 == nullptr || cur.phase32 == 0) {
+                step.finish(b, v, planes64, r_in, r_out, p_in, p_out, q_in, cur.alpha, cur.beta, cur.alpha_prev,
+                            x_mode, acc, qi, qv);
+            } else {  // the fold switched to FP32 values: this launch's blocks again, as FP32
+                SliceStep<float, kW, Prec> step32;
+                step32.issue(b, v, planes32, s, r_in, p_in, p_out, q_in, x_mode);
+                step32.finish(b, v, planes32, r_in, r_out, p_in, p_out, q_in, cur.alpha, cur.beta, cur.alpha_prev,
+                              x_mode, acc, qi, qv);
+            }
+        }
+    }
+    if (cur.done) return;
+    block_sums<kFoldThreads, 5>(acc, sh);
+    if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) part_out[blockIdx.x * 5 + k] = acc[k];
+    }
+    if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;
 }
 
 // ---- the same iteration in two launches (variant 13): for node slices of large systems,
@@ -1271,6 +1424,33 @@ void launch_pcg_fused_slices(hipStream_t s, const PcgBuffers& b, const SliceView
         else
             pcg_fused_slices_kernel<0, LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32, r_in,
                                                                                    r_out, p_in, p_out, q_in);
+    }
+}
+
+void launch_pcg_fold_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
+                            const float* planes32, const double* r_in, double* r_out, const double* p_in,
+                            double* p_out, const double* q_in, double* q_out, const PcgState* st_in,
+                            PcgState* st_out, const double* part_in, double* part_out) {
+    PcgBuffers bq = b;
+    bq.q = q_out;
+    const unsigned int grid = round8(blocks_for(v.n_slices, kFoldThreads / kWave));  // one slice per wavefront
+    if (grid > static_cast<unsigned int>(kFoldMaxBlocks))
+        throw std::runtime_error("the folded node-slice iteration takes at most 1,024 slices");
+    const bool narrow = v.max_width <= 9;
+    if (b.precond == LFG_PRECOND_BLOCK2) {
+        if (narrow)
+            pcg_fold_slices_kernel<9, LFG_PRECOND_BLOCK2><<<grid, kFoldThreads, 0, s>>>(
+                bq, v, planes64, planes32, r_in, r_out, p_in, p_out, q_in, st_in, st_out, part_in, part_out);
+        else
+            pcg_fold_slices_kernel<0, LFG_PRECOND_BLOCK2><<<grid, kFoldThreads, 0, s>>>(
+                bq, v, planes64, planes32, r_in, r_out, p_in, p_out, q_in, st_in, st_out, part_in, part_out);
+    } else {
+        if (narrow)
+            pcg_fold_slices_kernel<9, LFG_PRECOND_JACOBI><<<grid, kFoldThreads, 0, s>>>(
+                bq, v, planes64, planes32, r_in, r_out, p_in, p_out, q_in, st_in, st_out, part_in, part_out);
+        else
+            pcg_fold_slices_kernel<0, LFG_PRECOND_JACOBI><<<grid, kFoldThreads, 0, s>>>(
+                bq, v, planes64, planes32, r_in, r_out, p_in, p_out, q_in, st_in, st_out, part_in, part_out);
     }
 }
 

@@ -376,10 +376,10 @@ def test_deterministic_and_graph_equals_direct(dev):
     u3, r3 = dev.pcg("jacobi", direct_launch=True, check_every=7)
     assert np.array_equal(bits(u1), bits(u2)) and np.array_equal(bits(u1), bits(u3))
     assert r1["iterations"] == r2["iterations"] == r3["iterations"]
-    for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13):
+    for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13, 14):
         u4, r4 = dev.pcg("jacobi", spmv_variant=variant)
         assert r4["converged"] and np.linalg.norm(u4 - u1) <= 1e-12 * np.linalg.norm(u1), variant
-    for variant in (10, 12, 13):
+    for variant in (10, 12, 13, 14):
         u10, r10 = dev.pcg("jacobi", spmv_variant=variant, direct_launch=True, check_every=7)
         assert np.array_equal(bits(u10), bits(dev.pcg("jacobi", spmv_variant=variant)[0])), variant
 
@@ -387,10 +387,12 @@ def test_deterministic_and_graph_equals_direct(dev):
 @pytest.mark.parametrize("precond", ["jacobi", "block"])
 def test_fused_slice_iterations(dev, precond):
     """The fused node-slice iterations on an RCM-numbered log-E mesh (no fixed node offsets):
-    one launch (12) and two launches (13) per iteration give the same bits (the same sums over
-    the same nodes per block, p' of column nodes formed with the owner's arithmetic); both
-    match the three-launch node-slice iteration (6) to 1e-9 with its iteration count within
-    1% + 2, in CSR and in block storage order; they refuse the sweep preconditioners."""
+    one launch (12), two launches (13) and one launch with the sums folded by the next launch
+    (14) per iteration give the same bits (the same sums over the same nodes per block, folded
+    in the same order, p' of column nodes formed with the owner's arithmetic), also with FP32
+    values after the switch and from the first iteration; all match the three-launch
+    node-slice iteration (6) to 1e-9 with its iteration count within 1% + 2, in CSR and in
+    block storage order; they honour the iteration cap and refuse the sweep preconditioners."""
     print(f"case fused node slices 48x40 rcm log-E {precond}", flush=True)
     m = meshgen.structured_cantilever(48, 40, young=meshgen.loguniform_young(48 * 40, 7), plane_mode="plane_strain")
     pipeline.setup(m, device=dev, assembly="coloring", reordering="rcm")
@@ -399,12 +401,28 @@ def test_fused_slice_iterations(dev, precond):
         u6, r6 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=6, storage=storage)
         u12, r12 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=12, storage=storage)
         u13, r13 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=13, storage=storage)
-        u0, r0 = dev.pcg(precond, "fp64", "relative", 1e-10, storage=storage)  # the default: 12 at this size
+        u14, r14 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=14, storage=storage)
+        u0, r0 = dev.pcg(precond, "fp64", "relative", 1e-10, storage=storage)  # the default: 14 at this size
         assert np.array_equal(bits(u12), bits(u13)) and r12["iterations"] == r13["iterations"], storage
+        assert np.array_equal(bits(u12), bits(u14)) and r12["iterations"] == r14["iterations"], storage
+        assert r12["true_residual"] == r14["true_residual"], storage
         assert np.array_equal(bits(u0), bits(u12)), storage
         assert r12["converged"] and abs(r12["iterations"] - r6["iterations"]) <= 2 + 0.01 * r6["iterations"]
         assert np.linalg.norm(u12 - u6) <= 1e-9 * np.linalg.norm(u6), storage
-    for variant in (12, 13):
+    for switch in (0.0, -1.0):  # FP32 values after the default switch / from the first iteration
+        mixed = [dev.pcg(precond, "mixed", "relative", 1e-10, mixed_switch=switch, spmv_variant=v)
+                 for v in (12, 13, 14)]
+        for um, rm in mixed[1:]:
+            assert np.array_equal(bits(um), bits(mixed[0][0])), switch
+            assert rm["iterations"] == mixed[0][1]["iterations"] and rm["refine_passes"] == mixed[0][1]["refine_passes"]
+        assert mixed[0][1]["converged"] and np.linalg.norm(mixed[0][0] - u6) <= 1e-9 * np.linalg.norm(u6)
+    for variant in (12, 13, 14):
+        with pytest.raises(native.LightFemError) as err:
+            dev.pcg(precond, "fp64", "relative", 1e-10, max_iterations=5, spmv_variant=variant)
+        assert err.value.code == native.ERR_NOT_CONVERGED
+        _, capped = dev.pcg(precond, "fp64", "relative", 1e-10, max_iterations=5, spmv_variant=variant,
+                            raise_on_error=False)
+        assert capped["iterations"] == 5 and not capped["converged"], variant
         with pytest.raises(native.LightFemError) as err:
             dev.pcg("ic0", spmv_variant=variant)
         assert err.value.code == native.ERR_UNSUPPORTED
