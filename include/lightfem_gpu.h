@@ -74,7 +74,7 @@ typedef struct {
                                 values; 0 = default 1e-2, negative = FP32 values from the first iteration */
     int32_t direct_launch;   /* 1: launch each kernel directly instead of replaying a captured hipGraph */
     int32_t spmv_variant;    /* 0 = default: for jacobi / block 11, else 10, where the node stencil
-                                applies, else 12 (up to 1024 slices) or 13 on a node-blocked
+                                applies, else 14 (up to 1024 slices) or 13 on a node-blocked
                                 pattern; for ic0 / subdomains 8 where the stencil applies, else 6;
                                 else 4. 1-3: node
                                 groups of 4/8/16 lanes per row;
@@ -101,7 +101,11 @@ typedef struct {
                                 launch per iteration; 13: the same iteration in two launches (the
                                 vector update, then the multiply with the sums and state update),
                                 cheaper than 12 where the column gathers outweigh a launch (large
-                                systems). 10-13 apply the jacobi and block preconditioners only. */
+                                systems); 14: the iteration of 12 with its sums folded at the start
+                                of the next launch instead of by the last block (no completion
+                                counter, no wait for the stores: bit-identical to 12, the stop seen
+                                one launch later; up to 1024 slices). 10-14 apply the jacobi and
+                                block preconditioners only. */
     int32_t storage;         /* LFG_STORAGE_*: BSR sums each 2x2 block's products first (variants 6, 7
                                 and 9 follow it bit for bit; 8 and 10 read the transposed blocks as
                                 always; 1-5 are CSR only). Needs a node-blocked pattern. */
@@ -333,7 +337,7 @@ int lfg_set_subdomains(lfg_ctx* ctx, int64_t n_subdomains, const int32_t* node_s
 /* Checks and measurement hooks on the constrained system. */
 int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, double* out);
 /* y = Kc x with the PCG SpMV kernel of `variant` (values of `precision`; 10 and 11 multiply
- * as 8, 12 and 13 as 6; variant | LFG_SPMV_BLOCK_ORDER for 6, 7, 9: the block storage order). */
+ * as 8, 12-14 as 6; variant | LFG_SPMV_BLOCK_ORDER for 6, 7, 9: the block storage order). */
 int lfg_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x, double* y);
 /* Allocation-free form of lfg_spmv, for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops): no counterpart in the reference. y_dev =
@@ -349,7 +353,7 @@ int lfg_spmv_noalloc(lfg_ctx* ctx, int32_t precision, int32_t variant, const dou
                      double* xy_dev);
 /* Average device time (ms) of `reps` SpMV launches of the given precision and
  * variant on resident data (variant 7: the multiply with the Jacobi direction formed
- * on the fly, as its PCG iteration runs it; variants 10-13: one whole fused Jacobi PCG
+ * on the fly, as its PCG iteration runs it; variants 10-14: one whole fused Jacobi PCG
  * iteration, its launches included). */
 int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t reps, double* avg_ms);
 /* Average device time (ms) of a device-to-device copy of `bytes` bytes: the
