@@ -232,7 +232,7 @@ def oracle_errors(dev, u: np.ndarray, oracle: dict) -> dict:
 def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict:
     """The uneven batch: n_problems Q4 cantilevers (nx, ny ~ U{32..256}, 4 load cases each)
     solved together by the batched Jacobi-PCG to ||r|| <= tol ||b|| per system, with FP64
-    values and mixed (each problem moves to compensated FP32 values at 1e-2 ||b||)."""
+    values and mixed (each problem moves to compensated FP32 values at 1e-3 ||b||)."""
     from lightfem_amd import meshgen, pipeline
     t0 = time.perf_counter()
     specs = meshgen.batch_problems(n_problems)

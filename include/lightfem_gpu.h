@@ -278,7 +278,7 @@ typedef struct {
     double mixed_switch;      /* mixed: a problem multiplies with FP64 values until every running case's
                                  residual is at most mixed_switch * ||b||, then with FP32 off-diagonal
                                  values and an FP64 diagonal carrying their rounding (vectors and scalars
-                                 stay FP64); 0 = 1e-2, negative = FP32 from the first iteration */
+                                 stay FP64); 0 = 1e-3, negative = FP32 from the first iteration */
     int32_t block_threads;    /* launch shape when not tuning: 64-512 (0 = 256) */
     int32_t slices_per_wave;  /* slices each wavefront walks per launch (0 = 1) */
 } lfg_batch_options;

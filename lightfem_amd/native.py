@@ -377,7 +377,7 @@ class Device:
                   tune: bool = True, block_threads: int = 0, slices_per_wave: int = 0, raise_on_error: bool = True):
         """All (problem, case) systems; returns u (cases x dofs), per-system reports, summary.
         mixed_switch (mixed only): relative residual at which a problem moves to FP32
-        values; 0 = 1e-2, negative = FP32 from the first iteration."""
+        values; 0 = 1e-3, negative = FP32 from the first iteration."""
         opt = BatchOptions(PRECISION[precision], STOP[stop_rule], check_every, int(tune), tolerance, max_iterations,
                            mixed_switch, block_threads, slices_per_wave)
         n_sys = self.batch_problems * self.batch_cases

@@ -104,7 +104,7 @@ def test_batch_equals_single_problem_solves(dev):
 
 @pytest.mark.parametrize("switch", [0.0, -1.0])
 def test_batch_mixed_precision(dev, switch):
-    """Mixed batches: FP64 values until a problem's residuals fall to 1e-2 ||b|| (or FP32
+    """Mixed batches: FP64 values until a problem's residuals fall to 1e-3 ||b|| (the default; or FP32
     from the start), then FP32 off-diagonal values with the compensated FP64 diagonal
     (K~ = K + dK). Every system converges on its recurrence residual r; the FP64
     residual b - K u = r + dK (u - u_switch) is bounded by ||r|| + ||dK u|| with
