@@ -166,10 +166,13 @@ __device__ __forceinline__ bool ke_row(const Coords& c, double young, double nu,
 
 // Rows 2a, 2a + 1 of Ke with the operations of ke_rows<2>(..., 2a, ...) minus the products
 // with the structural zeros of B and D (D's (0,2), (1,2), (2,0), (2,1) entries; B's zero in
-// each column). Every sum of ke_rows starts from +0.0, and an omitted product is an exact
-// +-0 (its other factor is finite): adding +-0 to a sum that started from +0.0 never
-// changes it, so each `0.0 + x` below is the sum ke_rows forms and the rows are
-// bit-identical to it, with about half the floating-point operations.
+// each column) and minus the additions of those sums' starting +0.0. An omitted product is
+// an exact +-0 (its other factor is finite) and adding +-0 to a nonzero value returns it,
+// so every intermediate value here equals ke_rows's, except that a zero may carry the other
+// sign. Zero signs cannot reach the rows: each entry is a sum that starts from +0.0, and an
+// IEEE sum is -0.0 only when both addends are -0.0, so an entry is never -0.0 and adding a
+// zero of either sign leaves it unchanged. The rows are bit-identical to ke_rows's, with
+// less than half its floating-point operations.
 __host__ __device__ __forceinline__ bool ke_corner_rows(const Coords& c, double young, double nu, double thick,
                                                         int plane_mode, int a, double (&ke)[2][8]) {
     double d[9];
@@ -194,16 +197,16 @@ __host__ __device__ __forceinline__ bool ke_corner_rows(const Coords& c, double 
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             // DB columns 2i and 2i + 1 (D rows (d0 d1 0), (d3 d4 0), (0 0 d8))
-            const double db0x = 0.0 + d[0] * dnx[i];
-            const double db1x = 0.0 + d[3] * dnx[i];
-            const double db2x = 0.0 + d[8] * dny[i];
-            const double db0y = 0.0 + d[1] * dny[i];
-            const double db1y = 0.0 + d[4] * dny[i];
-            const double db2y = 0.0 + d[8] * dnx[i];
-            ke[0][2 * i] += ((0.0 + ax * db0x) + ay * db2x) * weight;
-            ke[0][2 * i + 1] += ((0.0 + ax * db0y) + ay * db2y) * weight;
-            ke[1][2 * i] += ((0.0 + ay * db1x) + ax * db2x) * weight;
-            ke[1][2 * i + 1] += ((0.0 + ay * db1y) + ax * db2y) * weight;
+            const double db0x = d[0] * dnx[i];
+            const double db1x = d[3] * dnx[i];
+            const double db2x = d[8] * dny[i];
+            const double db0y = d[1] * dny[i];
+            const double db1y = d[4] * dny[i];
+            const double db2y = d[8] * dnx[i];
+            ke[0][2 * i] += (ax * db0x + ay * db2x) * weight;
+            ke[0][2 * i + 1] += (ax * db0y + ay * db2y) * weight;
+            ke[1][2 * i] += (ay * db1x + ax * db2x) * weight;
+            ke[1][2 * i + 1] += (ay * db1y + ax * db2y) * weight;
         }
     }
     return ok;
@@ -360,6 +363,35 @@ __global__ __launch_bounds__(256) void assemble_rows_kernel(MeshView m, int n_no
 // kBlockSlots slots.
 constexpr int kBlockNodes = kAssembleBlockNodes;
 constexpr int kBlockSlots = kAssembleBlockSlots;
+constexpr int kBlockStride = kBlockSlots + 1;  // LDS column stride: odd, so a node's columns differ in bank
+
+// Phase 2's sum for column node j (column dof cp) of a node with cnt <= N slots from k0:
+// each slot adds the entries of the corner whose node is j (+0.0 where none is, and for the
+// unused places of a node with fewer slots: adding +0.0 to a sum that started from +0.0
+// changes nothing); branch-free, so the reads of all the slots are in flight together.
+// `repeated` is set when an element holds j at two corners (the caller's general loop).
+template <int N>
+__device__ __forceinline__ double2 block_row_sum(const int4* en_lds, const double2* ke_lds, int k0, int cnt, int j,
+                                                 int cp, bool& repeated) {
+    int at[N];
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        const int k = r < cnt ? k0 + r : kBlockSlots;
+        const int4 en = en_lds[k];
+        const int mx = en.x == j, my = en.y == j, mz = en.z == j, mw = en.w == j;
+        repeated |= mx + my + mz + mw > 1;
+        const int bsel = mx ? 0 : (my ? 1 : (mz ? 2 : 3));
+        at[r] = (mx | my | mz | mw) ? (2 * bsel + cp) * kBlockStride + k : kBlockSlots;
+    }
+    double2 sum = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        const double2 v = ke_lds[at[r]];
+        sum.x += v.x;
+        sum.y += v.y;
+    }
+    return sum;
+}
 
 __global__ __launch_bounds__(256) void assemble_block_rows_kernel(MeshView m, int n_nodes,
                                                                   const int* __restrict__ rp,
@@ -373,7 +405,7 @@ __global__ __launch_bounds__(256) void assemble_block_rows_kernel(MeshView m, in
     constexpr int kRounds = kBlockNodes / kNodesPerRound;  // per half-wavefront
     // column c of slot k at ke_lds[c * kStride + k]: a wavefront's phase-1 stores are
     // consecutive, and the odd stride puts a node's columns in different LDS banks
-    constexpr int kStride = kBlockSlots + 1;
+    constexpr int kStride = kBlockStride;
     constexpr int kZero = kBlockSlots;                     // (+0.0, +0.0) in column 0's spare place
     static_assert(kAssembleRowSlots <= kHalf, "a row's columns fit a half-wavefront");
     __shared__ double2 ke_lds[8 * kStride];          // slot k, column c: (row 2a, row 2a + 1)
@@ -441,32 +473,15 @@ __global__ __launch_bounds__(256) void assemble_block_rows_kernel(MeshView m, in
         const int cnt = min(sp[g + 1] - s0, ns) - k0;
         const int j = cv >> 1;
         const int cp = cv & 1;
-        double2 sum = make_double2(0.0, 0.0);
-        // each slot adds the entries of the corner whose node is the column node (+0.0
-        // where none is, and for the unused places of a node with fewer slots: adding
-        // +0.0 to a sum that started from +0.0 changes nothing); branch-free, so the reads
-        // of all the slots are in flight together. A node with more slots, or an element
-        // that repeats a corner node, takes the general loop below.
+        // up to 4 slots (a structured mesh's nodes) or 6 branch-free; a node with more
+        // slots, or an element that repeats a corner node, takes the general loop below
         constexpr int kFast = 6;
         bool repeated = false;
-        if (cnt <= kFast) {
-            int at[kFast];
-#pragma unroll
-            for (int r = 0; r < kFast; ++r) {
-                const int k = r < cnt ? k0 + r : kBlockSlots;
-                const int4 en = en_lds[k];
-                const int mx = en.x == j, my = en.y == j, mz = en.z == j, mw = en.w == j;
-                repeated |= mx + my + mz + mw > 1;
-                const int bsel = mx ? 0 : (my ? 1 : (mz ? 2 : 3));
-                at[r] = (mx | my | mz | mw) ? (2 * bsel + cp) * kStride + k : kZero;
-            }
-#pragma unroll
-            for (int r = 0; r < kFast; ++r) {
-                const double2 v = ke_lds[at[r]];
-                sum.x += v.x;
-                sum.y += v.y;
-            }
-        }
+        double2 sum = make_double2(0.0, 0.0);
+        if (cnt <= 4)
+            sum = block_row_sum<4>(en_lds, ke_lds, k0, cnt, j, cp, repeated);
+        else if (cnt <= kFast)
+            sum = block_row_sum<kFast>(en_lds, ke_lds, k0, cnt, j, cp, repeated);
         if (cnt > kFast || repeated) {
             sum = make_double2(0.0, 0.0);
             for (int k = k0; k < k0 + cnt; ++k) {
