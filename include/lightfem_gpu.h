@@ -73,10 +73,10 @@ typedef struct {
     double mixed_switch;     /* mixed: SpMVs use FP64 values until ||r|| <= mixed_switch*||r0||, then FP32
                                 values; 0 = default 1e-2, negative = FP32 values from the first iteration */
     int32_t direct_launch;   /* 1: launch each kernel directly instead of replaying a captured hipGraph */
-    int32_t spmv_variant;    /* 0 = default: for jacobi / block 11, else 10, where the node stencil
-                                applies, else 14 (up to 1024 slices) or 13 on a node-blocked
-                                pattern; for ic0 / subdomains 8 where the stencil applies, else 6;
-                                else 4. 1-3: node
+    int32_t spmv_variant;    /* 0 = default: for jacobi / block 14 on a node-blocked pattern of at
+                                most 1024 slices (any numbering), else 11, else 10 where the node
+                                stencil applies, else 13 on a node-blocked pattern; for ic0 /
+                                subdomains 8 where the stencil applies, else 6; else 4. 1-3: node
                                 groups of 4/8/16 lanes per row;
                                 4, 5: row-segment streaming with 64 / 32 rows per wavefront; 6:
                                 node-slice layout (one column per 2x2 block, 64 nodes per

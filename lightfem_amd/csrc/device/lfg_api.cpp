@@ -320,13 +320,19 @@ bool fused_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stenc
 // launches for large ones.
 int slice_iteration(const lfg_ctx* c) { return c->n_slices <= kSliceIterMaxSlices ? kSliceFold : kSliceSplit; }
 
-// 0 selects, in order: the fused stencil iteration (Jacobi and block preconditioners),
-// the symmetric node-stencil multiply, the node-slice multiply, row-segment streaming.
+// 0 selects, in order: for the Jacobi and block preconditioners the folded node-slice
+// iteration up to kSliceIterMaxSlices slices (the fastest at those sizes whatever the
+// numbering: 7.7-9.9 us per iteration from 50x50 to 250x250 against 8.5-13 us for the
+// stencil iterations), then the fused stencil iterations; the symmetric node-stencil
+// multiply, the node-slice multiply, row-segment streaming.
 // Preconditioners applied by a separate sweep kernel into z (not per node in the
 // update / direction / fused kernels).
 bool sweep_precond(int precond) { return precond == LFG_PRECOND_IC0 || precond == LFG_PRECOND_SUBDOMAIN; }
 
+bool small_slices(const lfg_ctx* c) { return c->have_slices && c->n_slices <= kSliceIterMaxSlices; }
+
 int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JACOBI, bool block_order = false) {
+    if (variant == 0 && small_slices(c) && !sweep_precond(precond)) return resolve_variant(c, kSliceFold, precond, block_order);
     if (block_order) {
         const int v = variant == 0 ? (!sweep_precond(precond) && grid_stencil_ok(c)
                                           ? kStencilGrid
