@@ -229,7 +229,7 @@ def oracle_errors(dev, u: np.ndarray, oracle: dict) -> dict:
     return out
 
 
-def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict:
+def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed"), max_iterations: int = 1_000_000) -> dict:
     """The uneven batch: n_problems Q4 cantilevers (nx, ny ~ U{32..256}, 4 load cases each)
     solved together by the batched Jacobi-PCG to ||r|| <= tol ||b|| per system, with FP64
     values and mixed (each problem moves to compensated FP32 values at 1e-3 ||b||)."""
@@ -248,7 +248,7 @@ def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed")) -> dict
           flush=True)
     for form in forms:
         t = time.perf_counter()
-        _, rep, summary = dev.batch_pcg(form, "relative", tol, raise_on_error=False)
+        _, rep, summary = dev.batch_pcg(form, "relative", tol, max_iterations, raise_on_error=False)
         wall = (time.perf_counter() - t) * 1e3
         true_rel = rep["true_residual"] / np.maximum(rep["initial_residual"], 1e-300)
         out[form] = {"ms": round(wall, 2), "device_ms": round(summary["solve_ms"], 2),
@@ -285,7 +285,7 @@ PRECOND_CASES = (("jacobi", "jacobi", "csr"), ("block_1_node", "block", "csr"), 
                  ("ic0", "ic0", "csr"), ("jacobi_bsr", "jacobi", "bsr"))
 
 
-def run_preconditioners_200(dev, tol: float, names=None) -> dict:
+def run_preconditioners_200(dev, tol: float, names=None, max_iterations: int = 100000) -> dict:
     """The 200x200 cantilever in RCM order with coloring assembly, every preconditioner and
     storage of the reference: Jacobi, 2x2 nodal block, the default block (rcb, 64 nodes per
     subdomain), IC(0), and Jacobi with block storage. Device solve time (hipEvents) to the
@@ -299,8 +299,9 @@ def run_preconditioners_200(dev, tol: float, names=None) -> dict:
     for name, precond, storage in PRECOND_CASES:
         if names and name not in names:
             continue
-        dev.pcg(precond, "fp64", "reference", tol, storage=storage, raise_on_error=False)  # set-up and warm-up
-        _, rep = dev.pcg(precond, "fp64", "reference", tol, storage=storage, raise_on_error=False)
+        dev.pcg(precond, "fp64", "reference", tol, max_iterations, storage=storage,
+                raise_on_error=False)  # set-up and warm-up
+        _, rep = dev.pcg(precond, "fp64", "reference", tol, max_iterations, storage=storage, raise_on_error=False)
         out[name] = {"device_solve_ms": round(rep["solve_ms"], 3), "iterations": rep["iterations"],
                      "converged": bool(rep["converged"]), "setup_ms": round(rep["setup_ms"], 3),
                      "us_per_iteration": round(1e3 * rep["solve_ms"] / max(1, rep["iterations"]), 2)}
@@ -332,6 +333,9 @@ def main() -> int:
     ap.add_argument("--precond-cases", default="", help="comma-separated subset of the 200x200 preconditioner cases")
     ap.add_argument("--assembly-only", action="store_true", help="only the assembly strategies (profiling)")
     ap.add_argument("--check-every", type=int, default=0)
+    ap.add_argument("--max-iterations", type=int, default=0,
+                    help="with --precond-only / --batch-only: cap every solve at this many iterations and end "
+                         "with two 1 GiB copies (a short run for kernel traces and traffic counters)")
     ap.add_argument("--profile", type=int, default=0,
                     help="profiling run: this many FP64 SpMV launches and FP64 PCG iterations, nothing else")
     args = ap.parse_args()
@@ -357,12 +361,20 @@ def main() -> int:
     if args.precond_only:
         dev = native.Device(0)
         names = [c for c in args.precond_cases.split(",") if c]
-        print(json.dumps({"preconditioners_200x200": run_preconditioners_200(dev, args.tol, names)}))
+        cap = args.max_iterations or 100000
+        out = {"preconditioners_200x200": run_preconditioners_200(dev, args.tol, names, cap)}
+        if args.max_iterations:
+            out["copy_bytes"], out["copy_ms"] = 1 << 30, dev.bench_copy(1 << 30, 2)
+        print(json.dumps(out))
         dev.close()
         return 0
     if args.batch_only:
         dev = native.Device(0)
-        print(json.dumps({"batch": run_batch(dev, args.batch_problems, args.tol, args.batch_forms.split(","))}))
+        cap = args.max_iterations or 1_000_000
+        out = {"batch": run_batch(dev, args.batch_problems, args.tol, args.batch_forms.split(","), cap)}
+        if args.max_iterations:
+            out["copy_bytes"], out["copy_ms"] = 1 << 30, dev.bench_copy(1 << 30, 2)
+        print(json.dumps(out))
         dev.close()
         return 0
     n_mesh = args.n

@@ -7,7 +7,7 @@
 // of each sweep by level and rewrites every dependency as a level-order position. The
 // sweep then runs level after level, one wavefront per level in rotation: the
 // wavefront that owns level l loaded its rows' data (factor values, dependency
-// positions, inputs) while the 15 levels before it ran, waits for the LDS counter of
+// positions, inputs) during the rotations before, waits for the LDS counter of
 // finished levels to reach l, reads its dependencies from an LDS window of the last
 // 4096 positions and advances the counter. Every wait has a bound: past it the sweep
 // stops and sets the fault flag. Factors whose dependencies reach back further than
@@ -33,106 +33,157 @@ constexpr long long kIcSpinLimit = 1LL << 22;
 constexpr int kIcNodeWin = kIcWin / 2;  // LDS window of the node form (node positions)
 constexpr int kIcNodeDeps = 8;          // dependency nodes per node held in registers
 
+// The window; its last element is a zero slot that padded dependencies read.
 struct Ic0Lds {
     union {
-        double z[kIcWin];
-        dpair zn[kIcNodeWin];
+        double z[kIcWin + 2];
+        dpair zn[kIcNodeWin + 1];
     };
     int done_pos;  // every position below it is finished (levels finish in order)
     int fault;
 };
 
-// A lane's row of the level it processes next, loaded a round of levels ahead.
-struct Ic0Row {
-    int row, cnt;
-    double in, scale;
+// Loads of a level run a pipeline of three rotations (a rotation: the four wavefronts
+// each running one level). After running its level, a wavefront issues the body
+// (dependency positions, factor values, input) of its next level, whose head it loaded
+// one rotation earlier, the head (row or node index, factor offset and count, pivots)
+// of the level after that, whose bounds it loaded one rotation earlier, and the bounds
+// of the level after that, so none of the three dependent loads waits for another on
+// the wavefront's way to its next level. Timestamps of a 200x200 RCM sweep (1,088 node
+// levels) put the per-level time at ~1,100 cycles: ~250 from the counter's store to the
+// next wavefront seeing it, ~580 for the window reads, ~160 for the arithmetic, ~120 to
+// store the values and advance the counter; the loads have arrived when the wait ends.
+
+// The positions [begin, end) of level l.
+struct Ic0Bounds {
+    int begin, end;
+};
+
+__device__ __forceinline__ Ic0Bounds ic0_bounds(const int* level_ptr, int levels, int l) {
+    if (l >= levels) return Ic0Bounds{0, 0};
+    return Ic0Bounds{level_ptr[l], level_ptr[l + 1]};
+}
+
+// kend - kb dependencies at kb: the count is formed where it is used, so issuing the
+// head's loads never waits for one of them.
+struct Ic0RowHead {
+    int row, kb, kend;
+    double scale;
+};
+
+struct Ic0RowBody {
+    double in;
     int key[kIcDeps];
     double val[kIcDeps];
 };
 
-__device__ __forceinline__ void load_ic0_row(const Ic0Sweep& sw, const double* in, int p, Ic0Row& r) {
-    r.row = sw.ord[p];
-    const int kb = sw.ptr[p];
-    r.cnt = sw.ptr[p + 1] - kb;
-    r.scale = sw.scale[p];
+__device__ __forceinline__ void load_ic0_row_head(const Ic0Sweep& sw, int p, Ic0RowHead& h) {
+    h.row = sw.ord[p];
+    h.kb = sw.ptr[p];
+    h.kend = sw.ptr[p + 1];
+    h.scale = sw.scale[p];
+}
+
+__device__ __forceinline__ void load_ic0_row_body(const Ic0Sweep& sw, const double* in, const Ic0RowHead& h,
+                                                  Ic0RowBody& r) {
+    const int cnt = h.kend - h.kb;
 #pragma unroll
     for (int u = 0; u < kIcDeps; ++u) {
-        const bool has = u < r.cnt;
-        r.key[u] = has ? sw.key[kb + u] : 0;
-        r.val[u] = has ? sw.val[kb + u] : 0.0;
+        const bool has = u < cnt;
+        r.key[u] = has ? sw.key[h.kb + u] : 0;
+        r.val[u] = has ? sw.val[h.kb + u] : 0.0;
     }
-    r.in = in[r.row];
+    r.in = in[h.row];
 }
 
 // The reference's row: sum = in - val_0 z_0 - val_1 z_1 - ... (dependency order), times
-// the reciprocal pivot.
-__device__ __forceinline__ double ic0_row_value(const Ic0Sweep& sw, const Ic0Lds& s, const Ic0Row& r, int p) {
+// the reciprocal pivot. Every dependency's value is read from the window before the
+// first is used, so a row costs one LDS round trip instead of one per dependency, and
+// the terms run straight through all kIcDeps slots: a padded dependency has a zero
+// value and reads the zero slot, and sum - (+0 * +0) is sum in every case (-0
+// included), so each row's result is the reference's.
+__device__ __forceinline__ double ic0_row_value(const Ic0Sweep& sw, const Ic0Lds& s, const Ic0RowHead& h,
+                                                const Ic0RowBody& r) {
+    const int cnt = h.kend - h.kb;
+    double zd[kIcDeps];
+#pragma unroll
+    for (int u = 0; u < kIcDeps; ++u) zd[u] = s.z[u < cnt ? (r.key[u] & (kIcWin - 1)) : kIcWin];
     double sum = r.in;
 #pragma unroll
-    for (int u = 0; u < kIcDeps; ++u)
-        if (u < r.cnt) sum -= r.val[u] * s.z[r.key[u] & (kIcWin - 1)];
-    if (r.cnt > kIcDeps) {  // rows with more dependencies than registers
-        const int kb = sw.ptr[p];
-        for (int u = kIcDeps; u < r.cnt; ++u) sum -= sw.val[kb + u] * s.z[sw.key[kb + u] & (kIcWin - 1)];
+    for (int u = 0; u < kIcDeps; ++u) sum -= r.val[u] * zd[u];
+    if (cnt > kIcDeps) {  // rows with more dependencies than registers
+        for (int u = kIcDeps; u < cnt; ++u) sum -= sw.val[h.kb + u] * s.z[sw.key[h.kb + u] & (kIcWin - 1)];
     }
-    return sum * r.scale;
+    return sum * h.scale;
 }
 
-// One sweep: wavefront w takes levels w, w + 16, w + 32, ... (the host splits levels
+// Waits until every position before `pos` is finished; false when the wait ran out.
+__device__ __forceinline__ bool ic0_wait(const Ic0Lds& s, int pos) {
+    long long spins = 0;
+    while (__hip_atomic_load(&s.done_pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < pos) {
+        if (++spins > kIcSpinLimit) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+// One sweep: wavefront w takes levels w, w + 4, w + 8, ... (the host splits levels
 // into parts of at most 64 rows, one row per lane); it waits until every earlier level
 // is done (done_pos has reached its first position), computes its rows from the data
 // it prefetched, writes each value into the window slot of its position, advances
-// done_pos to its level's end, then stores the values to `out` and prefetches its
-// next level. A level reads earlier levels only, within `reach` positions, and a slot
-// is overwritten only by a position more than reach + 64 later, so every value read
-// is still in the window. LDS serves each wavefront's accesses in issue order: values
-// stored before the counter are visible to any wavefront that has seen it
+// done_pos to its level's end, then stores the values to `out` and issues the loads
+// of its next levels. A level reads earlier levels only, within `reach` positions, and
+// a slot is overwritten only by a position more than reach + 64 later, so every value
+// read is still in the window. LDS serves each wavefront's accesses in issue order:
+// values stored before the counter are visible to any wavefront that has seen it
 // (compiler-only fences keep that order in the code; a hardware fence would also wait
-// for the sweep's global loads and stores). Adding the already finished terms before
-// the wait was tried and measured slower (6.0 against 3.8 ms per application at
-// 200x200): the waiting wavefronts then compete with the one on the critical path.
+// for the sweep's global loads and stores). A row's input is read one rotation before
+// the row runs; in the backward sweep (in == out) no other row writes it before then.
+// Adding the already finished terms before the wait was tried and measured slower
+// (6.0 against 3.8 ms per application at 200x200): the waiting wavefronts then compete
+// with the one on the critical path.
 __device__ void ic0_sweep_pipelined(const Ic0Sweep& sw, Ic0Lds& s, const double* in, double* out) {
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
-    if (threadIdx.x == 0) s.done_pos = 0;
-    __syncthreads();
-    Ic0Row next;
-    int next_begin = 0, next_end = 0;
-    if (w < sw.levels) {
-        next_begin = sw.level_ptr[w];
-        next_end = sw.level_ptr[w + 1];
-        if (next_begin + lane < next_end) load_ic0_row(sw, in, next_begin + lane, next);
+    if (threadIdx.x == 0) {
+        s.done_pos = 0;
+        s.z[kIcWin] = 0.0;
     }
+    __syncthreads();
+    Ic0Bounds bc = ic0_bounds(sw.level_ptr, sw.levels, w);
+    Ic0RowHead hc{0, 0, 0, 0.0}, hn{0, 0, 0, 0.0};
+    Ic0RowBody cur;
+    if (bc.begin + lane < bc.end) {
+        load_ic0_row_head(sw, bc.begin + lane, hc);
+        load_ic0_row_body(sw, in, hc, cur);
+    }
+    Ic0Bounds bn = ic0_bounds(sw.level_ptr, sw.levels, w + kIcWaves);
+    if (bn.begin + lane < bn.end) load_ic0_row_head(sw, bn.begin + lane, hn);
+    Ic0Bounds bnn = ic0_bounds(sw.level_ptr, sw.levels, w + 2 * kIcWaves);
     bool faulted = false;
     for (int l = w; l < sw.levels; l += kIcWaves) {
-        const Ic0Row cur = next;
-        const int begin = next_begin, end = next_end;
-        const int p = begin + lane;
-        const bool mine = p < end;
-        long long spins = 0;
-        while (__hip_atomic_load(&s.done_pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < begin) {
-            if (++spins > kIcSpinLimit) {
-                faulted = true;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
+        const int p = bc.begin + lane;
+        const bool mine = p < bc.end;
+        if (!ic0_wait(s, bc.begin)) {
+            faulted = true;
+            break;
         }
-        if (faulted) break;
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         double z = 0.0;
         if (mine) {
-            z = ic0_row_value(sw, s, cur, p);
+            z = ic0_row_value(sw, s, hc, cur);
             s.z[p & (kIcWin - 1)] = z;
         }
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (lane == 0) __hip_atomic_store(&s.done_pos, end, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (mine) out[cur.row] = z;
-        const int ln = l + kIcWaves;  // this wavefront's next level, loaded while the next 15 run
-        if (ln < sw.levels) {
-            next_begin = sw.level_ptr[ln];
-            next_end = sw.level_ptr[ln + 1];
-            if (next_begin + lane < next_end) load_ic0_row(sw, in, next_begin + lane, next);
-        }
+        if (lane == 0) __hip_atomic_store(&s.done_pos, bc.end, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (mine) out[hc.row] = z;
+        const bool next_mine = bn.begin + lane < bn.end;
+        if (next_mine) load_ic0_row_body(sw, in, hn, cur);
+        hc = hn;
+        bc = bn;
+        if (bnn.begin + lane < bnn.end) load_ic0_row_head(sw, bnn.begin + lane, hn);
+        bn = bnn;
+        bnn = ic0_bounds(sw.level_ptr, sw.levels, l + 3 * kIcWaves);
     }
     if (faulted) s.fault = 1;
     __syncthreads();
@@ -140,31 +191,41 @@ __device__ void ic0_sweep_pipelined(const Ic0Sweep& sw, Ic0Lds& s, const double*
 
 // Node form (Ic0NodeSweep): a lane computes both rows of its node, the second after the
 // first, from dependency nodes read as value pairs from the window.
-struct Ic0NodeRow {
-    int node, cnt;
-    double in_first, in_second, cross, scale_first, scale_second;
+struct Ic0NodeHead {
+    int node, kb, kend;
+    double cross, scale_first, scale_second;
+};
+
+struct Ic0NodeBody {
+    double in_first, in_second;
     int key[kIcNodeDeps];
     double vf[2 * kIcNodeDeps], vs[2 * kIcNodeDeps];
 };
 
+__device__ __forceinline__ void load_ic0_node_head(const Ic0NodeSweep& sw, int p, Ic0NodeHead& h) {
+    h.node = sw.ord[p];
+    h.kb = sw.ptr[p];
+    h.kend = sw.ptr[p + 1];
+    h.cross = sw.cross[p];
+    h.scale_first = sw.scale_first[p];
+    h.scale_second = sw.scale_second[p];
+}
+
 template <bool kBackward>
-__device__ __forceinline__ void load_ic0_node(const Ic0NodeSweep& sw, const double* in, int p, Ic0NodeRow& r) {
-    r.node = sw.ord[p];
-    const int kb = sw.ptr[p];
-    r.cnt = sw.ptr[p + 1] - kb;
-    r.cross = sw.cross[p];
-    r.scale_first = sw.scale_first[p];
-    r.scale_second = sw.scale_second[p];
+__device__ __forceinline__ void load_ic0_node_body(const Ic0NodeSweep& sw, const double* in, const Ic0NodeHead& h,
+                                                   Ic0NodeBody& r) {
+    const int cnt = h.kend - h.kb;
 #pragma unroll
     for (int u = 0; u < kIcNodeDeps; ++u) {
-        const bool has = u < r.cnt;
-        r.key[u] = has ? sw.key[kb + u] : 0;
-        r.vf[2 * u] = has ? sw.v_first[2 * (kb + u)] : 0.0;
-        r.vf[2 * u + 1] = has ? sw.v_first[2 * (kb + u) + 1] : 0.0;
-        r.vs[2 * u] = has ? sw.v_second[2 * (kb + u)] : 0.0;
-        r.vs[2 * u + 1] = has ? sw.v_second[2 * (kb + u) + 1] : 0.0;
+        const bool has = u < cnt;
+        const int k = h.kb + u;
+        r.key[u] = has ? sw.key[k] : 0;
+        r.vf[2 * u] = has ? sw.v_first[2 * k] : 0.0;
+        r.vf[2 * u + 1] = has ? sw.v_first[2 * k + 1] : 0.0;
+        r.vs[2 * u] = has ? sw.v_second[2 * k] : 0.0;
+        r.vs[2 * u + 1] = has ? sw.v_second[2 * k + 1] : 0.0;
     }
-    const dpair x = reinterpret_cast<const dpair*>(in)[r.node];
+    const dpair x = reinterpret_cast<const dpair*>(in)[h.node];
     r.in_first = kBackward ? x.y : x.x;
     r.in_second = kBackward ? x.x : x.y;
 }
@@ -172,78 +233,86 @@ __device__ __forceinline__ void load_ic0_node(const Ic0NodeSweep& sw, const doub
 // Forward: first = 2i, second = 2i+1, a dependency pair in the order (2j, 2j+1);
 // backward: first = 2i+1, second = 2i, the order (2j+1, 2j). The second row's cross
 // term (its coupling to the first) comes last, as in the reference's rows / columns.
+// Every dependency pair is read before the first is used (one LDS round trip per node)
+// and the terms run straight through all kIcNodeDeps slots, as in ic0_row_value: a
+// padded dependency has zero values and reads the zero slot's pair.
 template <bool kBackward>
-__device__ __forceinline__ dpair ic0_node_value(const Ic0NodeSweep& sw, const Ic0Lds& s, const Ic0NodeRow& r,
-                                                int p) {
-    double a = r.in_first, b = r.in_second;
-    auto term = [&](int key, double f0, double f1, double s0, double s1) {
-        const dpair zp = s.zn[key & (kIcNodeWin - 1)];
-        const double z0 = kBackward ? zp.y : zp.x, z1 = kBackward ? zp.x : zp.y;
-        a -= f0 * z0;
-        a -= f1 * z1;
-        b -= s0 * z0;
-        b -= s1 * z1;
-    };
+__device__ __forceinline__ dpair ic0_node_value(const Ic0NodeSweep& sw, const Ic0Lds& s, const Ic0NodeHead& h,
+                                                const Ic0NodeBody& r) {
+    const int cnt = h.kend - h.kb;
+    dpair zd[kIcNodeDeps];
 #pragma unroll
-    for (int u = 0; u < kIcNodeDeps; ++u)
-        if (u < r.cnt) term(r.key[u], r.vf[2 * u], r.vf[2 * u + 1], r.vs[2 * u], r.vs[2 * u + 1]);
-    if (r.cnt > kIcNodeDeps) {  // nodes with more dependency nodes than registers
-        const int kb = sw.ptr[p];
-        for (int u = kIcNodeDeps; u < r.cnt; ++u) {
-            const int k = kb + u;
-            term(sw.key[k], sw.v_first[2 * k], sw.v_first[2 * k + 1], sw.v_second[2 * k], sw.v_second[2 * k + 1]);
+    for (int u = 0; u < kIcNodeDeps; ++u) zd[u] = s.zn[u < cnt ? (r.key[u] & (kIcNodeWin - 1)) : kIcNodeWin];
+    double a = r.in_first, b = r.in_second;
+#pragma unroll
+    for (int u = 0; u < kIcNodeDeps; ++u) {
+        const double z0 = kBackward ? zd[u].y : zd[u].x, z1 = kBackward ? zd[u].x : zd[u].y;
+        a -= r.vf[2 * u] * z0;
+        a -= r.vf[2 * u + 1] * z1;
+        b -= r.vs[2 * u] * z0;
+        b -= r.vs[2 * u + 1] * z1;
+    }
+    if (cnt > kIcNodeDeps) {  // nodes with more dependency nodes than registers
+        for (int u = kIcNodeDeps; u < cnt; ++u) {
+            const int k = h.kb + u;
+            const dpair zp = s.zn[sw.key[k] & (kIcNodeWin - 1)];
+            const double z0 = kBackward ? zp.y : zp.x, z1 = kBackward ? zp.x : zp.y;
+            a -= sw.v_first[2 * k] * z0;
+            a -= sw.v_first[2 * k + 1] * z1;
+            b -= sw.v_second[2 * k] * z0;
+            b -= sw.v_second[2 * k + 1] * z1;
         }
     }
-    const double zf = a * r.scale_first;
-    b -= r.cross * zf;
-    const double zs = b * r.scale_second;
+    const double zf = a * h.scale_first;
+    b -= h.cross * zf;
+    const double zs = b * h.scale_second;
     return kBackward ? dpair{zs, zf} : dpair{zf, zs};
 }
 
-// The node form of ic0_sweep_pipelined (same schedule, window and counter).
+// The node form of ic0_sweep_pipelined (same schedule, window, counter and loads).
 template <bool kBackward>
 __device__ void ic0_node_sweep(const Ic0NodeSweep& sw, Ic0Lds& s, const double* in, double* out) {
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
-    if (threadIdx.x == 0) s.done_pos = 0;
-    __syncthreads();
-    Ic0NodeRow next;
-    int next_begin = 0, next_end = 0;
-    if (w < sw.levels) {
-        next_begin = sw.level_ptr[w];
-        next_end = sw.level_ptr[w + 1];
-        if (next_begin + lane < next_end) load_ic0_node<kBackward>(sw, in, next_begin + lane, next);
+    if (threadIdx.x == 0) {
+        s.done_pos = 0;
+        s.zn[kIcNodeWin] = dpair{0.0, 0.0};
     }
+    __syncthreads();
+    Ic0Bounds bc = ic0_bounds(sw.level_ptr, sw.levels, w);
+    Ic0NodeHead hc{0, 0, 0, 0.0, 0.0, 0.0}, hn{0, 0, 0, 0.0, 0.0, 0.0};
+    Ic0NodeBody cur;
+    if (bc.begin + lane < bc.end) {
+        load_ic0_node_head(sw, bc.begin + lane, hc);
+        load_ic0_node_body<kBackward>(sw, in, hc, cur);
+    }
+    Ic0Bounds bn = ic0_bounds(sw.level_ptr, sw.levels, w + kIcWaves);
+    if (bn.begin + lane < bn.end) load_ic0_node_head(sw, bn.begin + lane, hn);
+    Ic0Bounds bnn = ic0_bounds(sw.level_ptr, sw.levels, w + 2 * kIcWaves);
     bool faulted = false;
     for (int l = w; l < sw.levels; l += kIcWaves) {
-        const Ic0NodeRow cur = next;
-        const int begin = next_begin, end = next_end;
-        const int p = begin + lane;
-        const bool mine = p < end;
-        long long spins = 0;
-        while (__hip_atomic_load(&s.done_pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < begin) {
-            if (++spins > kIcSpinLimit) {
-                faulted = true;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
+        const int p = bc.begin + lane;
+        const bool mine = p < bc.end;
+        if (!ic0_wait(s, bc.begin)) {
+            faulted = true;
+            break;
         }
-        if (faulted) break;
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         dpair z = {0.0, 0.0};
         if (mine) {
-            z = ic0_node_value<kBackward>(sw, s, cur, p);
+            z = ic0_node_value<kBackward>(sw, s, hc, cur);
             s.zn[p & (kIcNodeWin - 1)] = z;
         }
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (lane == 0) __hip_atomic_store(&s.done_pos, end, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (mine) reinterpret_cast<dpair*>(out)[cur.node] = z;
-        const int ln = l + kIcWaves;
-        if (ln < sw.levels) {
-            next_begin = sw.level_ptr[ln];
-            next_end = sw.level_ptr[ln + 1];
-            if (next_begin + lane < next_end) load_ic0_node<kBackward>(sw, in, next_begin + lane, next);
-        }
+        if (lane == 0) __hip_atomic_store(&s.done_pos, bc.end, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (mine) reinterpret_cast<dpair*>(out)[hc.node] = z;
+        const bool next_mine = bn.begin + lane < bn.end;
+        if (next_mine) load_ic0_node_body<kBackward>(sw, in, hn, cur);
+        hc = hn;
+        bc = bn;
+        if (bnn.begin + lane < bnn.end) load_ic0_node_head(sw, bnn.begin + lane, hn);
+        bn = bnn;
+        bnn = ic0_bounds(sw.level_ptr, sw.levels, l + 3 * kIcWaves);
     }
     if (faulted) s.fault = 1;
     __syncthreads();
