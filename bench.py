@@ -41,7 +41,7 @@ KERNELS = {1: "pcg_spmv_kernel<4>", 2: "pcg_spmv_kernel<8>", 3: "pcg_spmv_kernel
            8: "pcg_spmv_stencil_kernel<4, true>", 9: "pcg_spmv_stencil_kernel<9, false>",
            10: "pcg_fused_stencil_kernel<4, 0>", 11: "pcg_fused_grid_kernel<0, 4>",
            12: "pcg_fused_slices_kernel<9, 0>", 13: "pcg_slices_multiply_kernel<9, 0>",
-           14: "pcg_fold_slices_kernel<9, 0>"}
+           14: "pcg_fold_slices_kernel<9, 0>", 15: "pcg_sym_slices_multiply_kernel<4, 5, 0>"}
 
 
 def kernel_name(variant: int, layout: dict) -> str:
@@ -88,6 +88,16 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int, layout: dict) -
             # 120 B per node) and the multiply (p' gathered, r', M^-1 read, q' written: 64 B)
             return matrix + n_nodes * 184
         return matrix + (40 * n if variant == 7 else 16 * n)
+    if variant == 15:
+        # symmetric node slices (FP64): each node's blocks from the diagonal on hold values
+        # (32 B) and a column, the blocks before it a column and the position of the mirrored
+        # block (read from the L2, not counted), per slice two offsets; the vector and
+        # multiply launches of variant 13 (184 B per node)
+        n_slices = (n // 2 + 63) // 64
+        blocks = nnz // 4
+        upper = (blocks + n_nodes) // 2
+        lower = blocks - upper
+        return upper * 36 + lower * 8 + 8 * (n_slices + 1) + n_nodes * 184
     return nnz * (value_bytes + 4) + 4 * (n + 1) + 16 * n
 
 
@@ -437,8 +447,9 @@ def main() -> int:
     spmv, sweep, iteration_kernel = {}, {}, {}
     for precision, vbytes in (("fp64", 8), ("mixed", 4)):
         sweep[precision] = {}
-        for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14):
+        for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15):
             if (variant in (6, 7, 12, 13, 14) and not layout["node_blocked"]) or (
+                    variant == 15 and (precision == "mixed" or not layout["sym_slices"])) or (
                     variant == 14 and layout["n_slices"] > 1024) or (
                     variant == 8 and not layout["stencil_sym_planes"]) or (variant == 9 and (
                     precision == "mixed" or not layout["stencil_exact_planes"])) or (
@@ -460,6 +471,13 @@ def main() -> int:
     copy_ms = dev.bench_copy(copy_bytes, 20)
     copy = {"bytes_per_copy": copy_bytes, "ms": copy_ms, "GBs": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9}
     print(f"bench: device copy {json.dumps(copy)}", file=sys.stderr, flush=True)
+    # streaming calibration at the dominant kernel's launch size: four buffers read and one
+    # written, 16-B pairs, the same read / write mix and bytes as one fused iteration
+    stream_bytes = iteration_kernel["fp64"]["bytes"] // 5 // 16 * 16
+    stream_ms = dev.bench_stream(stream_bytes, 50)
+    stream = {"bytes_per_launch": 5 * stream_bytes, "ms": stream_ms,
+              "GBs": 5 * stream_bytes / (stream_ms * 1e-3) / 1e9}
+    print(f"bench: stream calibration {json.dumps(stream)}", file=sys.stderr, flush=True)
     forms = {}
     cap = 10 ** 6
     oracle = load_c3_oracle(n_mesh)
@@ -530,6 +548,19 @@ def main() -> int:
             "true_relative_residual": rep13["true_residual"] / rep13["initial_residual"],
             "rel_u_diff_vs_fp64": float(np.linalg.norm(u13 - u64) / np.linalg.norm(u64))}
         print(f"bench: fp64 node slices fused {forms['fp64_node_slices_fused']}", file=sys.stderr, flush=True)
+    if solve_variant != 15 and layout["sym_slices"]:
+        # the symmetric node slices (variant 15: the FP64 default for large node-blocked
+        # patterns without fixed offsets, e.g. RCM numbering): lower blocks read as the
+        # transposes of the column nodes' blocks
+        t = time.perf_counter()
+        u15, rep15 = dev.pcg("jacobi", "fp64", "relative", args.tol, 10 ** 6, args.check_every, spmv_variant=15,
+                             raise_on_error=False)
+        forms["fp64_sym_slices"] = {
+            "ms_per_solve": (time.perf_counter() - t) * 1e3, "device_solve_ms": rep15["solve_ms"],
+            "iterations": rep15["iterations"], "converged": bool(rep15["converged"]), "spmv_variant": 15,
+            "true_relative_residual": rep15["true_residual"] / rep15["initial_residual"],
+            "rel_u_diff_vs_fp64": float(np.linalg.norm(u15 - u64) / np.linalg.norm(u64))}
+        print(f"bench: fp64 symmetric node slices {forms['fp64_sym_slices']}", file=sys.stderr, flush=True)
     per_iter_ms = {k: v["device_solve_ms"] / max(1, v["iterations"]) for k, v in forms.items()}
     cpu, protocol = None, None
     if not args.skip_cpu:
@@ -577,7 +608,7 @@ def main() -> int:
         "forms": forms, "per_iteration_ms": per_iter_ms,
         "spmv": spmv, "iteration_kernel": iteration_kernel,
         "spmv_variants": {p: {str(k): round(v["GBs"], 1) for k, v in d.items()} for p, d in sweep.items()},
-        "device_copy": copy, "assembly": assembly, "setup_s": round(setup_s, 2),
+        "device_copy": copy, "stream_calibration": stream, "assembly": assembly, "setup_s": round(setup_s, 2),
         "roofline": {"bound": "hbm", "kernel": f"{dominant['kernel']} (FP64 values)",
                      "achieved": round(dominant["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dominant["GBs"] / HBM_PEAK_GBS, 4),
@@ -587,7 +618,9 @@ def main() -> int:
                      "spmv_frac": round(spmv["fp64"]["GBs"] / HBM_PEAK_GBS, 4),
                      "fp32_values_spmv_GBs": round(spmv["mixed"]["GBs"], 1),
                      "fp32_values_iteration_GBs": round(iteration_kernel["mixed"]["GBs"], 1),
-                     "copy_ceiling_GBs": round(copy["GBs"], 1)},
+                     "stream_same_bytes_GBs": round(stream["GBs"], 1),
+                     "frac_of_stream": round(dominant["GBs"] / stream["GBs"], 4),
+                     "runtime_copy_GBs": round(copy["GBs"], 1)},
         "cpu_baseline": cpu,
         "cpu_protocol_c1_c2": protocol,
         "batch": batch,

@@ -75,7 +75,9 @@ typedef struct {
     int32_t direct_launch;   /* 1: launch each kernel directly instead of replaying a captured hipGraph */
     int32_t spmv_variant;    /* 0 = default: for jacobi / block 14 on a node-blocked pattern of at
                                 most 1024 slices (any numbering), else 11, else 10 where the node
-                                stencil applies, else 13 on a node-blocked pattern; for ic0 /
+                                stencil applies, else on a node-blocked pattern 15 with FP64 values
+                                (13 with mixed precision, or where the pattern is not structurally
+                                symmetric); for ic0 /
                                 subdomains 8 where the stencil applies, else 6; else 4. 1-3: node
                                 groups of 4/8/16 lanes per row;
                                 4, 5: row-segment streaming with 64 / 32 rows per wavefront; 6:
@@ -104,8 +106,11 @@ typedef struct {
                                 systems); 14: the iteration of 12 with its sums folded at the start
                                 of the next launch instead of by the last block (no completion
                                 counter, no wait for the stores: bit-identical to 12, the stop seen
-                                one launch later; up to 1024 slices). 10-14 apply the jacobi and
-                                block preconditioners only. */
+                                one launch later; up to 1024 slices); 15: the iteration of 13 on the
+                                symmetric node slices (per node the blocks from the diagonal on hold
+                                values, those before it are read as the transposes of the column
+                                nodes' blocks: each mirrored pair streams once; FP64 values, CSR
+                                order). 10-15 apply the jacobi and block preconditioners only. */
     int32_t storage;         /* LFG_STORAGE_*: BSR sums each 2x2 block's products first (variants 6, 7
                                 and 9 follow it bit for bit; 8 and 10 read the transposed blocks as
                                 always; 1-5 are CSR only). Needs a node-blocked pattern. */
@@ -324,6 +329,7 @@ typedef struct {
     int32_t fused_window_pairs;    /* variant 10: LDS pairs per 256-node tile (0 = not available) */
     int32_t stencil_offsets[17];   /* the exact form's offsets, ascending */
     int32_t grid_row_len;          /* variant 11: nodes per mesh row L (0 = not available) */
+    int32_t sym_slices;            /* variant 15: node-blocked and structurally symmetric (0 = not available) */
 } lfg_layout_info;
 int lfg_layout_info_get(lfg_ctx* ctx, lfg_layout_info* info);
 
@@ -337,7 +343,8 @@ int lfg_set_subdomains(lfg_ctx* ctx, int64_t n_subdomains, const int32_t* node_s
 /* Checks and measurement hooks on the constrained system. */
 int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, double* out);
 /* y = Kc x with the PCG SpMV kernel of `variant` (values of `precision`; 10 and 11 multiply
- * as 8, 12-14 as 6; variant | LFG_SPMV_BLOCK_ORDER for 6, 7, 9: the block storage order). */
+ * as 8, 12-14 as 6, 15 on the symmetric node slices (FP64); variant | LFG_SPMV_BLOCK_ORDER
+ * for 6, 7, 9: the block storage order). */
 int lfg_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x, double* y);
 /* Allocation-free form of lfg_spmv, for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops): no counterpart in the reference. y_dev =
@@ -353,13 +360,18 @@ int lfg_spmv_noalloc(lfg_ctx* ctx, int32_t precision, int32_t variant, const dou
                      double* xy_dev);
 /* Average device time (ms) of `reps` SpMV launches of the given precision and
  * variant on resident data (variant 7: the multiply with the Jacobi direction formed
- * on the fly, as its PCG iteration runs it; variants 10-14: one whole fused Jacobi PCG
+ * on the fly, as its PCG iteration runs it; variants 10-15: one whole fused Jacobi PCG
  * iteration, its launches included). */
 int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t reps, double* avg_ms);
-/* Average device time (ms) of a device-to-device copy of `bytes` bytes: the
- * streaming-bandwidth ceiling of this device, and the calibration kernel for
- * hardware traffic counters. */
+/* Average device time (ms) of a device-to-device copy of `bytes` bytes (the runtime's
+ * copy): the calibration dispatch for hardware traffic counters. */
 int lfg_bench_copy(lfg_ctx* ctx, int64_t bytes, int32_t reps, double* avg_ms);
+/* Average device time (ms) of a streaming kernel over five device buffers of `bytes`
+ * bytes each: four read, one written, one 16-byte pair per thread (the read / write mix
+ * of the fused PCG iterations, 5 x bytes moved per launch). The calibration a fused
+ * iteration's achieved bandwidth is compared with at its own launch size. No counterpart
+ * in the reference (a measurement helper, like lfg_bench_copy). */
+int lfg_bench_stream(lfg_ctx* ctx, int64_t bytes, int32_t reps, double* avg_ms);
 
 #ifdef __cplusplus
 }

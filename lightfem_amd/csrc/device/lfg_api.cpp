@@ -131,6 +131,15 @@ struct lfg_ctx {
     DevArray<int> slice_ptr, slice_col;
     DevArray<double> slices64, slice_diag;  // slice_diag: compensated FP64 diagonal of slices32
     DevArray<float> slices32;
+    // symmetric node-slice copy (variant 15); its lists are a cache of the pattern, built on
+    // first use (also from const contexts): sym_state 0 = not built yet, 1 = built, -1 = the
+    // pattern is not structurally symmetric
+    mutable int sym_state = 0;
+    bool sym64_valid = false;
+    mutable int sym_max_uwidth = 0, sym_max_lwidth = 0;
+    mutable long long sym_ublocks = 0;
+    mutable DevArray<int> sym_uptr, sym_ucol, sym_lptr, sym_lcol, sym_ltpos;
+    DevArray<double> sym64;
     // node-stencil copies of the constrained values (SpMV variants 8 and 9), when every
     // block column lies at one of a few fixed node offsets (structured meshes numbered
     // row by row)
@@ -276,6 +285,11 @@ lfg::SliceView slice_view(const lfg_ctx* c) {
                           c->slice_max_width, c->slice_diag.p, c->pair_sum};
 }
 
+lfg::SymSliceView sym_view(const lfg_ctx* c) {
+    return lfg::SymSliceView{c->n_slices,   c->sym_ublocks,  c->sym_uptr.p,      c->sym_ucol.p,    c->sym_lptr.p,
+                             c->sym_lcol.p, c->sym_ltpos.p,  c->sym_max_uwidth, c->sym_max_lwidth};
+}
+
 // Sets the block storage summation order for the products enqueued in its scope.
 struct PairScope {
     lfg_ctx* c;
@@ -292,7 +306,8 @@ constexpr int kStencilGrid = 11;  // the iteration of 10 on 2D tiles (row-struct
 constexpr int kSliceIter = 12;    // node slices, whole Jacobi / block PCG iteration in one launch
 constexpr int kSliceSplit = 13;   // the iteration of 12 in two launches (vector update, multiply)
 constexpr int kSliceFold = 14;    // the iteration of 12, its reduction folded by the next launch
-constexpr int kMaxVariant = kSliceFold;
+constexpr int kSliceSym = 15;     // the iteration of 13 on the symmetric node slices (FP64 values)
+constexpr int kMaxVariant = kSliceSym;
 constexpr int kSliceIterMaxSlices = 1024;  // above this the split iteration (13) is the faster default
 static_assert(kSliceIterMaxSlices <= lfg::kFoldMaxSlices, "variant 14 serves the one-launch default");
 constexpr int kFoldPartials = 5 * lfg::kFoldMaxBlocks;  // one set of variant 14's partial sums
@@ -304,14 +319,15 @@ bool slice_layout(int variant) {
 bool stencil_layout(int variant) {
     return variant == kStencilSym || variant == kStencilExact || variant == kStencilFused || variant == kStencilGrid;
 }
-// The whole Jacobi / block iteration in one launch (1D or 2D stencil tiles, node slices).
+// The whole Jacobi / block iteration in one launch (1D or 2D stencil tiles, node slices)
+// or two (variants 13, 15).
 bool fused_iteration(int variant) {
     return variant == kStencilFused || variant == kStencilGrid || variant == kSliceIter || variant == kSliceSplit ||
-           variant == kSliceFold;
+           variant == kSliceFold || variant == kSliceSym;
 }
 // The node-slice iterations of one or two launches per iteration (Jacobi and block only).
 bool slice_iteration_variant(int variant) {
-    return variant == kSliceIter || variant == kSliceSplit || variant == kSliceFold;
+    return variant == kSliceIter || variant == kSliceSplit || variant == kSliceFold || variant == kSliceSym;
 }
 bool grid_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stencil_sym.row_len > 0; }
 bool fused_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stencil_sym.win_total > 0; }
@@ -319,6 +335,98 @@ bool fused_stencil_ok(const lfg_ctx* c) { return c->have_stencil_sym && c->stenc
 // per iteration, its sums folded by the next launch, for small and medium systems; two
 // launches for large ones.
 int slice_iteration(const lfg_ctx* c) { return c->n_slices <= kSliceIterMaxSlices ? kSliceFold : kSliceSplit; }
+
+// The host lists of the symmetric node-slice layout (SymSliceView), built from the
+// pattern on first use: per node the blocks before its diagonal ("lower") and from it on
+// ("upper"), per slice as wide as its widest node, and for every lower block (i, j) the
+// position of the upper block (j, i) it is read as. False (remembered) when the pattern
+// is not node-blocked or some block has no mirror.
+bool sym_slices_ok(const lfg_ctx* c) {
+    if (c->sym_state != 0) return c->sym_state > 0;
+    c->sym_state = -1;
+    if (!c->have_slices) return false;
+    const int n_nodes = c->n / 2;
+    const std::vector<int>& rp = c->h_row_ptr;
+    const std::vector<int>& col = c->h_col;
+    std::vector<int> deg(static_cast<size_t>(n_nodes)), dlo(static_cast<size_t>(n_nodes), 0);
+    for (int i = 0; i < n_nodes; ++i) {
+        const int r0 = rp[2 * i], r1 = rp[2 * i + 1];
+        if (r1 - r0 == 1) {  // a node in no element: its diagonal block alone (build_slices checked it)
+            deg[i] = 1;
+            continue;
+        }
+        deg[i] = (r1 - r0) / 2;
+        int d = 0;
+        while (d < deg[i] && col[r0 + 2 * d] < 2 * i) ++d;
+        dlo[i] = d;
+    }
+    const int n_slices = c->n_slices;
+    std::vector<int> uptr(static_cast<size_t>(n_slices) + 1, 0), lptr(static_cast<size_t>(n_slices) + 1, 0);
+    long long ut = 0, lt = 0;
+    int max_u = 0, max_l = 0;
+    for (int s = 0; s < n_slices; ++s) {
+        int uw = 0, lw = 0;
+        for (int i = s * 64; i < std::min(n_nodes, s * 64 + 64); ++i) {
+            uw = std::max(uw, deg[i] - dlo[i]);
+            lw = std::max(lw, dlo[i]);
+        }
+        max_u = std::max(max_u, uw);
+        max_l = std::max(max_l, lw);
+        ut += 64LL * uw;
+        lt += 64LL * lw;
+        if (ut >= (1LL << 31) - 1 || lt >= (1LL << 31)) return false;  // positions are int32
+        uptr[s + 1] = static_cast<int>(ut);
+        lptr[s + 1] = static_cast<int>(lt);
+    }
+    std::vector<int> ucol(static_cast<size_t>(ut)), lcol(static_cast<size_t>(lt)), ltpos(static_cast<size_t>(lt));
+    for (int s = 0; s < n_slices; ++s) {
+        const int uw = (uptr[s + 1] - uptr[s]) / 64, lw = (lptr[s + 1] - lptr[s]) / 64;
+        for (int lane = 0; lane < 64; ++lane) {
+            const int node = s * 64 + lane;
+            const int self = std::min(node, n_nodes - 1);
+            const bool real = node < n_nodes;
+            const int r0 = real ? rp[2 * node] : 0;
+            const bool lone = real && rp[2 * node + 1] - r0 == 1;
+            for (int k = 0; k < uw; ++k) {
+                int j = self;  // padding: a zero block times the node's own pair
+                if (real && k < deg[node] - dlo[node]) j = lone ? node : col[r0 + 2 * (dlo[node] + k)] >> 1;
+                ucol[static_cast<size_t>(uptr[s]) + 64 * k + lane] = j;
+            }
+            for (int k = 0; k < lw; ++k) {
+                const size_t at = static_cast<size_t>(lptr[s]) + 64 * k + lane;
+                lcol[at] = self;
+                ltpos[at] = static_cast<int>(ut);  // padding: the zero block
+                if (!real || k >= dlo[node]) continue;
+                const int j = col[r0 + 2 * k] >> 1;  // j < node
+                // block (j, node) among node j's column pairs (ascending)
+                int lo = 0, hi = deg[j];
+                const int rj = rp[2 * j];
+                while (lo < hi) {
+                    const int mid = (lo + hi) / 2;
+                    if (col[rj + 2 * mid] < 2 * node)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                if (lo >= deg[j] || rp[2 * j + 1] - rj == 1 || col[rj + 2 * lo] != 2 * node) return false;
+                const int u = lo - dlo[j];  // >= 0: node > j
+                lcol[at] = j;
+                ltpos[at] = uptr[j / 64] + 64 * u + j % 64;
+            }
+        }
+    }
+    c->sym_uptr.upload(uptr.data(), uptr.size(), c->stream);
+    c->sym_lptr.upload(lptr.data(), lptr.size(), c->stream);
+    c->sym_ucol.upload(ucol.data(), ucol.size(), c->stream);
+    c->sym_lcol.upload(lcol.data(), lcol.size(), c->stream);
+    c->sym_ltpos.upload(ltpos.data(), ltpos.size(), c->stream);
+    LFG_HIP(hipStreamSynchronize(c->stream));  // the host lists end with this scope
+    c->sym_ublocks = ut;
+    c->sym_max_uwidth = max_u;
+    c->sym_max_lwidth = max_l;
+    c->sym_state = 1;
+    return true;
+}
 
 // 0 selects, in order: for the Jacobi and block preconditioners the folded node-slice
 // iteration up to kSliceIterMaxSlices slices (the fastest at those sizes whatever the
@@ -342,7 +450,7 @@ int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JAC
                                                                         : (sweep_precond(precond) ? kSliceVariant
                                                                                                   : slice_iteration(c)))))
                                    : variant;
-        require(v >= kSliceVariant && v <= kMaxVariant, LFG_ERR_UNSUPPORTED,
+        require(v >= kSliceVariant && v <= kSliceFold, LFG_ERR_UNSUPPORTED,
                 "block storage runs on the node-slice (6, 7, 12-14) and node-stencil (8-11) SpMVs");
         return resolve_variant(c, v, precond, false);
     }
@@ -353,7 +461,7 @@ int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JAC
         if (c->have_slices) return sweep_precond(precond) ? kSliceVariant : slice_iteration(c);
         return 4;
     }
-    require(variant >= 1 && variant <= kMaxVariant, LFG_ERR_ARG, "SpMV variant must be 0-14");
+    require(variant >= 1 && variant <= kMaxVariant, LFG_ERR_ARG, "SpMV variant must be 0-15");
     require(!slice_iteration_variant(variant) || !sweep_precond(precond), LFG_ERR_UNSUPPORTED,
             "the fused node-slice iteration applies the jacobi and 2x2 block preconditioners, not ic0 or subdomains");
     require(variant != kStencilGrid || grid_stencil_ok(c), LFG_ERR_UNSUPPORTED,
@@ -370,6 +478,8 @@ int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JAC
             "the node-slice SpMV needs rows 2i, 2i+1 sharing one column list of dof pairs");
     require(variant != kSliceFold || c->n_slices <= kSliceIterMaxSlices, LFG_ERR_UNSUPPORTED,
             "the folded node-slice iteration (14) takes at most 1,024 slices of 64 nodes; use 12 or 13");
+    require(variant != kSliceSym || sym_slices_ok(c), LFG_ERR_UNSUPPORTED,
+            "the symmetric node-slice layout (15) needs a node-blocked, structurally symmetric pattern");
     require(variant != kStencilSym || c->have_stencil_sym, LFG_ERR_UNSUPPORTED,
             "the symmetric node-stencil SpMV needs a symmetric node-blocked pattern with at most 8 positive node "
             "offsets");
@@ -384,6 +494,8 @@ int resolve_variant(const lfg_ctx* c, int variant, int precond = LFG_PRECOND_JAC
 void build_slices(lfg_ctx* c) {
     c->have_slices = false;
     c->slices64_valid = c->slices32_valid = false;
+    c->sym_state = 0;
+    c->sym64_valid = false;
     const int n_nodes = c->n / 2;
     const std::vector<int>& rp = c->h_row_ptr;
     const std::vector<int>& col = c->h_col;
@@ -583,6 +695,16 @@ void ensure_slices(lfg_ctx* c, bool need64, bool need32) {
                                 c->slice_diag.p);
         c->slices32_valid = true;
     }
+}
+
+// Packs the constrained values into the symmetric slice planes (variant 15).
+void ensure_sym_slices(lfg_ctx* c) {
+    require(sym_slices_ok(c), LFG_ERR_UNSUPPORTED,
+            "the symmetric node-slice layout (15) needs a node-blocked, structurally symmetric pattern");
+    if (c->sym64_valid) return;
+    c->sym64.alloc(4 * static_cast<size_t>(c->sym_ublocks + 1));
+    lfg::launch_pack_sym_slices(c->stream, csr_view(c), sym_view(c), c->Kc.p, c->sym64.p);
+    c->sym64_valid = true;
 }
 
 void sync(lfg_ctx* c) { LFG_HIP(hipStreamSynchronize(c->stream)); }
@@ -1027,6 +1149,11 @@ void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int repl
                                         qb[parity ^ 1], sb[parity], sb[parity ^ 1], fb[parity ^ 1], fb[parity]);
             return;
         }
+        if (variant == kSliceSym) {
+            lfg::launch_pcg_sym_split_slices(c->stream, b, sym_view(c), c->sym64.p, rb[parity], rb[parity ^ 1],
+                                             pb[parity], pb[parity ^ 1], qb[parity], qb[parity ^ 1]);
+            return;
+        }
         if (variant == kSliceIter || variant == kSliceSplit) {
             (variant == kSliceIter ? lfg::launch_pcg_fused_slices : lfg::launch_pcg_split_slices)(
                 c->stream, b, slice_view(c), c->slices64.p, mixed ? c->slices32.p : nullptr, rb[parity],
@@ -1299,7 +1426,7 @@ int lfg_pin_zero_pivots(lfg_ctx* ctx, lfg_pivot_report* report) {
         if (report) *report = rep;
         if (rep.pinned > 0) {  // the packed copies and factors of Kc are stale
             ctx->k32_valid = false;
-            ctx->slices64_valid = ctx->slices32_valid = false;
+            ctx->slices64_valid = ctx->slices32_valid = ctx->sym64_valid = false;
             ctx->stencil64_valid = ctx->stencil32_valid = ctx->stencil_exact_valid = false;
             ctx->precond_ready = -1;
         }
@@ -1522,7 +1649,7 @@ int lfg_dirichlet(lfg_ctx* ctx, int64_t n_constraints, const int32_t* dofs, cons
         ctx->have_system = true;
         ctx->have_solution = false;
         ctx->k32_valid = false;
-        ctx->slices64_valid = ctx->slices32_valid = false;
+        ctx->slices64_valid = ctx->slices32_valid = ctx->sym64_valid = false;
         ctx->stencil64_valid = ctx->stencil32_valid = ctx->stencil_exact_valid = false;
         ctx->precond_ready = -1;
         return LFG_OK;
@@ -1572,11 +1699,18 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     if (previous == kind && kind == LFG_PRECOND_BLOCK2) degenerate = -1;  // factor reused; count unknown here
     if (kind == LFG_PRECOND_SUBDOMAIN) degenerate = ctx->sub.degenerate;
     const bool block_order = opt->storage == LFG_STORAGE_BSR;
-    const int variant = resolve_variant(ctx, opt->spmv_variant, kind, block_order);
+    int variant = resolve_variant(ctx, opt->spmv_variant, kind, block_order);
+    // FP64 values on a large pattern without a stencil: the two-launch iteration on the
+    // symmetric node slices, which streams each mirrored pair of blocks once
+    if (opt->spmv_variant == 0 && variant == kSliceSplit && !mixed && sym_slices_ok(ctx)) variant = kSliceSym;
     const PairScope pair_scope(ctx, block_order);
     require(!(mixed && variant == kStencilExact), LFG_ERR_UNSUPPORTED,
             "mixed precision uses the symmetric stencil form (variant 8), not the exact one");
-    if (stencil_layout(variant)) {
+    require(!(mixed && variant == kSliceSym), LFG_ERR_UNSUPPORTED,
+            "the symmetric node slices (variant 15) hold FP64 values; mixed precision runs on 6, 7 or 12-14");
+    if (variant == kSliceSym) {
+        ensure_sym_slices(ctx);
+    } else if (stencil_layout(variant)) {
         ensure_stencil(ctx, variant, true, mixed != 0);
     } else if (slice_layout(variant)) {
         ensure_slices(ctx, true, mixed != 0);
@@ -1904,6 +2038,8 @@ static void run_spmv(lfg_ctx* ctx, int precision, int variant, bool fused_dir = 
     if (fused_dir && variant == kSliceFused)
         lfg::launch_pcg_spmv_slices_dir(ctx->stream, b, slice_view(ctx), f32 ? nullptr : ctx->slices64.p,
                                         f32 ? ctx->slices32.p : nullptr, ctx->p.p, ctx->p2.p);
+    else if (variant == kSliceSym)
+        lfg::launch_pcg_spmv_sym_slices(ctx->stream, b, sym_view(ctx), ctx->sym64.p);
     else if (stencil_layout(variant))
         launch_stencil_spmv(ctx, b, variant, !f32, f32);
     else if (slice_layout(variant))
@@ -1918,7 +2054,11 @@ static void prepare_spmv(lfg_ctx* ctx, int precision, int variant) {
     ensure_vectors(ctx);
     lfg::PcgState zero{};
     LFG_HIP(hipMemcpyAsync(ctx->state.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
-    if (stencil_layout(variant)) {
+    if (variant == kSliceSym) {
+        require(precision != LFG_PRECISION_MIXED, LFG_ERR_UNSUPPORTED,
+                "the symmetric node slices (variant 15) hold FP64 values only");
+        ensure_sym_slices(ctx);
+    } else if (stencil_layout(variant)) {
         require(!(variant == kStencilExact && precision == LFG_PRECISION_MIXED), LFG_ERR_UNSUPPORTED,
                 "the exact stencil form has no FP32 copy");
         ensure_stencil(ctx, variant, precision != LFG_PRECISION_MIXED, precision == LFG_PRECISION_MIXED);
@@ -1948,7 +2088,10 @@ int lfg_spmv_noalloc(lfg_ctx* ctx, int32_t precision, int32_t variant, const dou
         const bool f32 = precision == LFG_PRECISION_MIXED;
         bool ready = false;
         unsigned int grid = 0;
-        if (stencil_layout(v)) {
+        if (v == kSliceSym) {
+            ready = !f32 && ctx->sym64_valid;
+            grid = static_cast<unsigned int>(lfg::slice_spmv_blocks(ctx->n_slices));
+        } else if (stencil_layout(v)) {
             ready = v == kStencilExact ? (!f32 && ctx->stencil_exact_valid)
                                        : (f32 ? ctx->stencil32_valid : ctx->stencil64_valid);
             grid = v != kStencilExact && lfg::stencil_grid_multiply(ctx->stencil_sym)
@@ -1980,7 +2123,9 @@ int lfg_spmv_noalloc(lfg_ctx* ctx, int32_t precision, int32_t variant, const dou
         b.st = reinterpret_cast<lfg::PcgState*>(base);
         b.tickets = reinterpret_cast<lfg::TicketSet*>(base + lfg::kNoallocTicketsAt);
         b.partials = reinterpret_cast<double*>(base + lfg::kNoallocPartialsAt);
-        if (stencil_layout(v))
+        if (v == kSliceSym)
+            lfg::launch_pcg_spmv_sym_slices(ctx->stream, b, sym_view(ctx), ctx->sym64.p);
+        else if (stencil_layout(v))
             launch_stencil_spmv(ctx, b, v, !f32, f32);
         else if (slice_layout(v))
             lfg::launch_pcg_spmv_slices(ctx->stream, b, slice_view(ctx), f32 ? nullptr : ctx->slices64.p,
@@ -2006,6 +2151,7 @@ int lfg_layout_info_get(lfg_ctx* ctx, lfg_layout_info* info) {
         out.fused_window_pairs = ctx->have_stencil_sym ? ctx->stencil_sym.win_total : 0;
         for (int k = 0; k < out.stencil_exact_planes; ++k) out.stencil_offsets[k] = ctx->stencil_exact.off[k];
         out.grid_row_len = ctx->have_stencil_sym ? ctx->stencil_sym.row_len : 0;
+        out.sym_slices = sym_slices_ok(ctx) ? 1 : 0;
         *info = out;
         return LFG_OK;
     });
@@ -2042,10 +2188,14 @@ int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t rep
             ensure_vectors(ctx);
             if (ctx->precond_ready != LFG_PRECOND_JACOBI) setup_preconditioner(ctx, LFG_PRECOND_JACOBI, &degenerate);
             const bool f32 = precision == LFG_PRECISION_MIXED;
-            if (slice_layout(v))
+            if (v == kSliceSym) {
+                require(!f32, LFG_ERR_UNSUPPORTED, "the symmetric node slices (variant 15) hold FP64 values only");
+                ensure_sym_slices(ctx);
+            } else if (slice_layout(v)) {
                 ensure_slices(ctx, !f32, f32);
-            else
+            } else {
                 ensure_stencil(ctx, v, !f32, f32);
+            }
             ctx->p2.alloc(static_cast<size_t>(ctx->n));
             ctx->r_alt.alloc(static_cast<size_t>(ctx->n));
             ctx->q_alt.alloc(static_cast<size_t>(ctx->n));
@@ -2169,7 +2319,7 @@ int lfg_batch_set_loads(lfg_ctx* ctx, int32_t n_cases, const double* loads, int6
         ctx->have_system = true;
         ctx->have_solution = ctx->have_adjoint = false;
         ctx->k32_valid = false;
-        ctx->slices64_valid = ctx->slices32_valid = false;
+        ctx->slices64_valid = ctx->slices32_valid = ctx->sym64_valid = false;
         ctx->stencil64_valid = ctx->stencil32_valid = ctx->stencil_exact_valid = false;
         ctx->precond_ready = -1;
         ctx->have_batch_loads = true;
@@ -2409,6 +2559,27 @@ int lfg_bench_copy(lfg_ctx* ctx, int64_t bytes, int32_t reps, double* avg_ms) {
         LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
         for (int i = 0; i < reps; ++i)
             LFG_HIP(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToDevice, ctx->stream));
+        LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+        sync(ctx);
+        *avg_ms = elapsed(ctx->ev_start, ctx->ev_stop) / reps;
+        return LFG_OK;
+    });
+}
+
+int lfg_bench_stream(lfg_ctx* ctx, int64_t bytes, int32_t reps, double* avg_ms) {
+    return guarded(ctx, [&] {
+        require(bytes >= 16 && reps > 0 && avg_ms, LFG_ERR_ARG, "stream size and repetitions must be positive");
+        LFG_HIP(hipSetDevice(ctx->device));
+        const long long pairs = bytes / 16;
+        DevArray<double> buf[5];
+        for (auto& a : buf) {
+            a.alloc(static_cast<size_t>(2 * pairs));
+            LFG_HIP(hipMemsetAsync(a.p, 0, static_cast<size_t>(16 * pairs), ctx->stream));
+        }
+        lfg::launch_stream_probe(ctx->stream, pairs, buf[0].p, buf[1].p, buf[2].p, buf[3].p, buf[4].p);
+        LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+        for (int i = 0; i < reps; ++i)
+            lfg::launch_stream_probe(ctx->stream, pairs, buf[0].p, buf[1].p, buf[2].p, buf[3].p, buf[4].p);
         LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
         sync(ctx);
         *avg_ms = elapsed(ctx->ev_start, ctx->ev_stop) / reps;

@@ -263,6 +263,35 @@ void launch_pcg_fold_slices(hipStream_t s, const PcgBuffers& b, const SliceView&
 void launch_pcg_split_slices(hipStream_t s, const PcgBuffers& b, const SliceView& v, const double* planes64,
                              const float* planes32, const double* r_in, double* r_out, const double* p_in,
                              double* p_out, const double* q_in, double* q_out);
+// Symmetric node-slice layout (variant 15) of a node-blocked matrix whose pattern is
+// symmetric: per node i, the "upper" blocks (i, j) with j >= i (the diagonal first) keep
+// their values and node column, the "lower" blocks (j < i) keep their node column and
+// the position of block (j, i) among the upper blocks, whose transpose they are read as.
+// Both lists are stored per 64-node slice column-major, as in SliceView (entry k of node
+// 64 s + l at ptr[s] + 64 k + l). The upper values are two pair planes (upper pairs, then
+// lower pairs), each with a zero block at index ublocks that padded lower entries point
+// to; padded upper entries are zero blocks. FP64 values only.
+struct SymSliceView {
+    int n_slices;
+    long long ublocks;  // padded upper entries, uptr[n_slices]
+    const int* uptr;    // n_slices + 1 upper entry offsets (multiples of 64)
+    const int* ucol;    // node column per upper entry
+    const int* lptr;    // n_slices + 1 lower entry offsets (multiples of 64)
+    const int* lcol;    // node column per lower entry
+    const int* ltpos;   // position of the transposed upper block per lower entry
+    int max_uwidth, max_lwidth;
+};
+// Copies every node's upper blocks from its two CSR rows into the planes
+// (4 * (ublocks + 1) doubles), and the zero block.
+void launch_pack_sym_slices(hipStream_t s, const CsrView& a, const SymSliceView& v, const double* values,
+                            double* planes);
+// q = A p on the symmetric node slices, p.q and alpha (as launch_pcg_spmv_slices).
+void launch_pcg_spmv_sym_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes);
+// Variant 15: the two-launch iteration of variant 13 with the multiply on the symmetric
+// node slices.
+void launch_pcg_sym_split_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes,
+                                 const double* r_in, double* r_out, const double* p_in, double* p_out,
+                                 const double* q_in, double* q_out);
 // x += alpha p, r -= alpha q, ||r||^2, r.z; convergence / replacement decision, beta.
 void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed);
 // mixed only: r = b - K x in FP64 when a replacement is due; then r.z and beta.
@@ -456,5 +485,8 @@ void launch_residual_norm(hipStream_t s, const CsrView& a, const double* vals64,
                           double* partials, TicketSet* tickets, double* out2, double* r_out = nullptr);
 // y += a x
 void launch_axpy(hipStream_t s, long long n, double a, const double* x, double* y);
+// e = a + b + c + d over n double pairs (streaming-bandwidth calibration).
+void launch_stream_probe(hipStream_t s, long long n, const double* a, const double* b, const double* c,
+                         const double* d, double* e);
 
 }  // namespace lfg

@@ -959,6 +959,19 @@ __global__ __launch_bounds__(kThreads) void axpy_kernel(long long n, double a, c
         y[i] += a * x[i];
 }
 
+// Streaming calibration: e = a + b + c + d over n pairs, one 16-B pair per thread (four
+// streams read, one written, the read / write mix of the fused PCG iterations).
+__global__ __launch_bounds__(kThreads) void stream_probe_kernel(long long n, const dpair* __restrict__ a,
+                                                                const dpair* __restrict__ b,
+                                                                const dpair* __restrict__ c,
+                                                                const dpair* __restrict__ d, dpair* __restrict__ e) {
+    const long long i = blockIdx.x * static_cast<long long>(kThreads) + threadIdx.x;
+    if (i < n) {
+        const dpair u = a[i], v = b[i], w = c[i], z = d[i];
+        e[i] = dpair{u.x + v.x + w.x + z.x, u.y + v.y + w.y + z.y};
+    }
+}
+
 __global__ void mask_constrained_kernel(int n, const int* __restrict__ constrained, const double* __restrict__ in,
                                         double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1183,6 +1196,14 @@ void launch_residual_norm(hipStream_t s, const CsrView& a, const double* vals64,
     const unsigned int grid = need < 2048u ? need : 2048u;
     residual_norm_kernel<W>
         <<<grid, kThreads, 0, s>>>(a.n, a.row_ptr, a.col, vals64, x, bvec, partials, tickets, out2, r_out);
+}
+
+void launch_stream_probe(hipStream_t s, long long n, const double* a, const double* b, const double* c,
+                         const double* d, double* e) {
+    const long long blocks = (n + kThreads - 1) / kThreads;
+    stream_probe_kernel<<<static_cast<unsigned>(blocks > 0 ? blocks : 1), kThreads, 0, s>>>(
+        n, reinterpret_cast<const dpair*>(a), reinterpret_cast<const dpair*>(b), reinterpret_cast<const dpair*>(c),
+        reinterpret_cast<const dpair*>(d), reinterpret_cast<dpair*>(e));
 }
 
 void launch_axpy(hipStream_t s, long long n, double a, const double* x, double* y) {

@@ -1379,6 +1379,223 @@ __global__ __launch_bounds__(kThreads) void fused_flush_kernel(PcgBuffers b, con
     }
 }
 
+// ---- the symmetric node-slice layout (variant 15)
+//
+// A node's rows read its upper blocks (j >= i) as stored and its lower blocks (j < i) as
+// the transposes of the column nodes' upper blocks (j, i), so each pair of mirrored blocks
+// streams its values once: at 1000 x 1000 about 5 blocks of 32 B and 5 columns plus 4
+// (column, position) pairs per node instead of 9 blocks and 9 columns. A transposed block
+// was streamed by its own node's wavefront shortly before (the slices of a launch are
+// XCD-remapped in contiguous ranges, and a numbering's bandwidth is small next to a
+// range), so it is read from the L2; the upper planes are therefore read with ordinary
+// loads, not nontemporal ones. Each row sums in column order (lower blocks ascending, the
+// diagonal block, the upper blocks ascending), as the CSR row loop; a transposed block's
+// values are block (j, i)'s, which assembly makes equal to block (i, j)'s up to rounding
+// (Ke(a, b) and Ke(b, a) are formed by different operation orders).
+
+// One thread per (slice, lane): node 64 s + l's upper blocks from its two CSR rows (the
+// blocks from the first column pair >= 2i on), zero padding; thread 0 also writes the zero
+// block the padded lower entries point to.
+__global__ __launch_bounds__(256) void pack_sym_slices_kernel(int n_nodes, const int* __restrict__ rp,
+                                                              const int* __restrict__ col, SymSliceView v,
+                                                              const double* __restrict__ values,
+                                                              double* __restrict__ planes) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int s = static_cast<int>(t >> 6);
+    const int lane = static_cast<int>(t & (kWave - 1));
+    if (t == 0) {
+        for (int h = 0; h < 2; ++h) {
+            planes[2 * (h * (v.ublocks + 1) + v.ublocks)] = 0.0;
+            planes[2 * (h * (v.ublocks + 1) + v.ublocks) + 1] = 0.0;
+        }
+    }
+    if (s >= v.n_slices) return;
+    const int node = s * kWave + lane;
+    const int begin = v.uptr[s];
+    const int width = (v.uptr[s + 1] - begin) / kWave;
+    int r0 = 0, r1 = 0, deg = 0, dlo = 0;
+    bool lone = false;  // a node in no element: rows hold only their diagonals
+    if (node < n_nodes) {
+        r0 = rp[2 * node];
+        r1 = rp[2 * node + 1];
+        lone = r1 - r0 == 1;
+        deg = lone ? 1 : (r1 - r0) >> 1;
+        if (!lone)
+            while (dlo < deg && col[r0 + 2 * dlo] < 2 * node) ++dlo;
+    }
+    for (int k = 0; k < width; ++k) {
+        const long long at = begin + static_cast<long long>(k) * kWave + lane;
+        double u0 = 0.0, u1 = 0.0, l0 = 0.0, l1 = 0.0;
+        const int kk = dlo + k;
+        if (lone && k == 0) {
+            u0 = values[r0];
+            l1 = values[r1];
+        } else if (!lone && kk < deg) {
+            u0 = values[r0 + 2 * kk];
+            u1 = values[r0 + 2 * kk + 1];
+            l0 = values[r1 + 2 * kk];
+            l1 = values[r1 + 2 * kk + 1];
+        }
+        planes[2 * at] = u0;
+        planes[2 * at + 1] = u1;
+        planes[2 * (v.ublocks + 1 + at)] = l0;
+        planes[2 * (v.ublocks + 1 + at) + 1] = l1;
+    }
+}
+
+// Rows 2i, 2i+1 of node i = 64 s + lane over both lists, p from p2. kWL / kWU > 0: lists up
+// to that wide take unrolled paths whose loads are unconditional (entries clamped to the
+// last one) and whose sums skip the clamped entries.
+template <int kWL, int kWU>
+__device__ __forceinline__ void sym_slice_rows(const SymSliceView& v, const double* __restrict__ planes,
+                                               const dpair* __restrict__ p2, int s, double& a0, double& a1) {
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const dpair* __restrict__ up = reinterpret_cast<const dpair*>(planes);
+    const dpair* __restrict__ lo = up + (v.ublocks + 1);
+    const int lb = v.lptr[s];
+    const int lw = (v.lptr[s + 1] - lb) / kWave;
+    const int ub = v.uptr[s];
+    const int uw = (v.uptr[s + 1] - ub) / kWave;
+    if (kWL > 0 && kWU > 0 && lw > 0 && lw <= kWL && uw > 0 && uw <= kWU) {
+        int lc[kWL > 0 ? kWL : 1], lt[kWL > 0 ? kWL : 1], uc[kWU > 0 ? kWU : 1];
+#pragma unroll
+        for (int k = 0; k < kWL; ++k) {
+            const int at = lb + min(k, lw - 1) * kWave + lane;
+            lc[k] = __builtin_nontemporal_load(v.lcol + at);
+            lt[k] = __builtin_nontemporal_load(v.ltpos + at);
+        }
+        dpair uu[kWU > 0 ? kWU : 1], ul[kWU > 0 ? kWU : 1];
+#pragma unroll
+        for (int k = 0; k < kWU; ++k) {
+            const int at = ub + min(k, uw - 1) * kWave + lane;
+            uc[k] = __builtin_nontemporal_load(v.ucol + at);
+            uu[k] = up[at];
+            ul[k] = lo[at];
+        }
+        dpair tu[kWL > 0 ? kWL : 1], tl[kWL > 0 ? kWL : 1], xl[kWL > 0 ? kWL : 1];
+#pragma unroll
+        for (int k = 0; k < kWL; ++k) {
+            tu[k] = up[lt[k]];
+            tl[k] = lo[lt[k]];
+            xl[k] = p2[lc[k]];
+        }
+        dpair xu[kWU > 0 ? kWU : 1];
+#pragma unroll
+        for (int k = 0; k < kWU; ++k) xu[k] = p2[uc[k]];
+#pragma unroll
+        for (int k = 0; k < kWL; ++k)
+            if (k < lw) add_block(a0, a1, tu[k].x, tl[k].x, tu[k].y, tl[k].y, xl[k], false);
+#pragma unroll
+        for (int k = 0; k < kWU; ++k)
+            if (k < uw) add_block(a0, a1, uu[k].x, uu[k].y, ul[k].x, ul[k].y, xu[k], false);
+        return;
+    }
+    for (int k = 0; k < lw; ++k) {
+        const int at = lb + k * kWave + lane;
+        const int tp = __builtin_nontemporal_load(v.ltpos + at);
+        const dpair tu = up[tp], tl = lo[tp];
+        const dpair x = p2[__builtin_nontemporal_load(v.lcol + at)];
+        add_block(a0, a1, tu.x, tl.x, tu.y, tl.y, x, false);
+    }
+    for (int k = 0; k < uw; ++k) {
+        const int at = ub + k * kWave + lane;
+        const dpair uu = up[at], ul = lo[at];
+        const dpair x = p2[__builtin_nontemporal_load(v.ucol + at)];
+        add_block(a0, a1, uu.x, uu.y, ul.x, ul.y, x, false);
+    }
+}
+
+// q = A p over the symmetric node slices, fused p.q and alpha = rho / p.q
+// (as pcg_spmv_slices_kernel).
+template <int kWL, int kWU>
+__global__ __launch_bounds__(kThreads) void pcg_spmv_sym_slices_kernel(PcgBuffers b, SymSliceView v,
+                                                                       const double* __restrict__ planes) {
+    if (b.st->done) return;
+    __shared__ double sh[kThreads / kWave];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = (v.n_slices + gridDim.x - 1) / gridDim.x;
+    const int start = lb * per_block;
+    const int stop = min(start + per_block, v.n_slices);
+    const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(b.p);
+    double pq = 0.0;
+    for (int s = start + static_cast<int>(threadIdx.x) / kWave; s < stop; s += kThreads / kWave) {
+        double a0 = 0.0, a1 = 0.0;
+        sym_slice_rows<kWL, kWU>(v, planes, p2, s, a0, a1);
+        const int node = s * kWave + (static_cast<int>(threadIdx.x) & (kWave - 1));
+        if (node < (b.n >> 1)) {
+            const dpair pv = p2[node];
+            reinterpret_cast<dpair*>(b.q)[node] = dpair{a0, a1};
+            pq += pv.x * a0 + pv.y * a1;
+        }
+    }
+    const double vals[1] = {block_sum<kThreads>(pq, sh)};
+    double tot[1];
+    if (!finish_reduction<kThreads, 1>(b.tickets + 1, b.partials, vals, tot, sh)) return;
+    if (threadIdx.x == 0) {
+        const double tpq = tot[0];
+        PcgState* st = b.st;
+        st->pq = tpq;
+        if (!(tpq > 0.0) || !isfinite(tpq)) {  // solver.cpp:591-593
+            st->status = LFG_ERR_NOT_SPD;
+            st->done = 1;
+        } else {
+            st->alpha = st->rho / tpq;
+        }
+    }
+}
+
+// The multiply launch of variant 15: pcg_slices_multiply_kernel's sums and state update
+// with q' = A p' on the symmetric node slices.
+template <int kWL, int kWU, int Prec>
+__global__ __launch_bounds__(kThreads) void pcg_sym_slices_multiply_kernel(PcgBuffers b, SymSliceView v,
+                                                                           const double* __restrict__ planes,
+                                                                           const double* __restrict__ r_new,
+                                                                           const double* __restrict__ p_new) {
+    if (b.st->done) return;
+    __shared__ double sh[3 * (kThreads / kWave)];
+    const double alpha = b.st->alpha;
+    constexpr int kWaves = kThreads / kWave;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int per_block = (v.n_slices + gridDim.x - 1) / gridDim.x;
+    const int start = lb * per_block;
+    const int stop = min(start + per_block, v.n_slices);
+    const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(p_new);
+    double acc[3] = {0.0, 0.0, 0.0};  // p'.q', q'.z, q'.M^-1 q'
+    int qi = -1;
+    dpair qv = {0.0, 0.0};
+    for (int s = start + static_cast<int>(threadIdx.x) / kWave; s < stop; s += kWaves) {
+        const int node = s * kWave + (static_cast<int>(threadIdx.x) & (kWave - 1));
+        const bool own = node < (b.n >> 1);
+        const int ic = own ? node : 0;
+        const dpair pn = p2[ic];
+        const dpair rn = reinterpret_cast<const dpair*>(r_new)[ic];
+        NodePrecond<Prec> mo;
+        mo.load(b.m, ic);
+        double a0 = 0.0, a1 = 0.0;
+        sym_slice_rows<kWL, kWU>(v, planes, p2, s, a0, a1);
+        if (own) {
+            if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;  // an earlier slice of this wavefront
+            qi = node;
+            qv = dpair{a0, a1};
+            double z0, z1, m0, m1;
+            mo.apply(rn.x, rn.y, z0, z1);
+            mo.apply(a0, a1, m0, m1);
+            acc[0] += pn.x * a0 + pn.y * a1;
+            acc[1] += a0 * z0 + a1 * z1;
+            acc[2] += a0 * m0 + a1 * m1;
+        }
+    }
+    block_sums<kThreads, 3>(acc, sh);
+    double tot[3];
+    const bool last = finish_reduction<kThreads, 3>(b.tickets + 1, b.partials, acc, tot, sh);
+    if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;
+    if (!last) return;
+    if (threadIdx.x == 0) {
+        const double all[5] = {b.st->rr_new, b.st->rz_new, tot[0], tot[1], tot[2]};
+        fused_state_update(b.st, all, alpha);
+    }
+}
+
 }  // namespace
 
 void launch_pcg_fused_flush(hipStream_t s, const PcgBuffers& b, const double* p_odd) {
@@ -1479,6 +1696,50 @@ void launch_pcg_split_slices(hipStream_t s, const PcgBuffers& b, const SliceView
             pcg_slices_multiply_kernel<0, LFG_PRECOND_JACOBI><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32,
                                                                                       r_out, p_out);
     }
+}
+
+void launch_pack_sym_slices(hipStream_t s, const CsrView& a, const SymSliceView& v, const double* values,
+                            double* planes) {
+    const long long threads = static_cast<long long>(v.n_slices) * kWave;
+    pack_sym_slices_kernel<<<blocks_for(threads > 0 ? threads : 1, 256), 256, 0, s>>>(a.n >> 1, a.row_ptr, a.col, v,
+                                                                                      values, planes);
+}
+
+void launch_pcg_spmv_sym_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes) {
+    const unsigned int grid = static_cast<unsigned int>(slice_spmv_blocks(v.n_slices));
+    if (v.max_lwidth <= 4 && v.max_uwidth <= 5)
+        pcg_spmv_sym_slices_kernel<4, 5><<<grid, kThreads, 0, s>>>(b, v, planes);
+    else if (v.max_lwidth <= 8 && v.max_uwidth <= 9)
+        pcg_spmv_sym_slices_kernel<8, 9><<<grid, kThreads, 0, s>>>(b, v, planes);
+    else
+        pcg_spmv_sym_slices_kernel<0, 0><<<grid, kThreads, 0, s>>>(b, v, planes);
+}
+
+template <int Prec>
+static void launch_sym_split(hipStream_t s, unsigned int grid, const PcgBuffers& b, const PcgBuffers& bq,
+                             const SymSliceView& v, const double* planes, const double* r_in, double* r_out,
+                             const double* p_in, double* p_out, const double* q_in) {
+    pcg_slices_vector_kernel<Prec><<<grid, kThreads, 0, s>>>(b, r_in, r_out, p_in, p_out, q_in);
+    if (v.max_lwidth <= 4 && v.max_uwidth <= 5)
+        pcg_sym_slices_multiply_kernel<4, 5, Prec><<<grid, kThreads, 0, s>>>(bq, v, planes, r_out, p_out);
+    else if (v.max_lwidth <= 8 && v.max_uwidth <= 9)
+        pcg_sym_slices_multiply_kernel<8, 9, Prec><<<grid, kThreads, 0, s>>>(bq, v, planes, r_out, p_out);
+    else
+        pcg_sym_slices_multiply_kernel<0, 0, Prec><<<grid, kThreads, 0, s>>>(bq, v, planes, r_out, p_out);
+}
+
+void launch_pcg_sym_split_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes,
+                                 const double* r_in, double* r_out, const double* p_in, double* p_out,
+                                 const double* q_in, double* q_out) {
+    PcgBuffers bq = b;
+    bq.q = q_out;
+    // the vector launch's 256-node ranges are the multiply's 4-slice ranges (same grid)
+    const unsigned int need = blocks_for(v.n_slices, kThreads / kWave);
+    const unsigned int grid = round8(need < 8192u ? need : 8192u);
+    if (b.precond == LFG_PRECOND_BLOCK2)
+        launch_sym_split<LFG_PRECOND_BLOCK2>(s, grid, b, bq, v, planes, r_in, r_out, p_in, p_out, q_in);
+    else
+        launch_sym_split<LFG_PRECOND_JACOBI>(s, grid, b, bq, v, planes, r_in, r_out, p_in, p_out, q_in);
 }
 
 void launch_pack_stencil(hipStream_t s, const CsrView& a, const StencilView& v, const double* values, double* out64,

@@ -30,7 +30,7 @@ STOP = {"reference": 0, "relative": 1}
 GPU_SYMBOLS = ["lfg_create", "lfg_destroy", "lfg_last_error", "lfg_status_string", "lfg_set_mesh", "lfg_set_pattern",
                "lfg_set_colors", "lfg_element_stiffness", "lfg_assemble", "lfg_get_stiffness", "lfg_set_stiffness",
                "lfg_set_load", "lfg_dirichlet", "lfg_get_system", "lfg_pcg", "lfg_reactions", "lfg_postprocess",
-               "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv", "lfg_bench_copy", "lfg_set_materials",
+               "lfg_precond_apply", "lfg_spmv", "lfg_bench_spmv", "lfg_bench_copy", "lfg_bench_stream", "lfg_set_materials",
                "lfg_adjoint_solve", "lfg_sensitivity", "lfg_set_material_modes", "lfg_batch_set_problems",
                "lfg_batch_set_loads", "lfg_batch_pcg", "lfg_layout_info_get", "lfg_set_subdomains", "lfg_set_bars",
                "lfg_pin_zero_pivots", "lfg_spmv_noalloc"]
@@ -79,12 +79,13 @@ class LayoutInfo(ctypes.Structure):
     _fields_ = [("node_blocked", ctypes.c_int32), ("n_slices", ctypes.c_int32),
                 ("stencil_exact_planes", ctypes.c_int32), ("stencil_sym_planes", ctypes.c_int32),
                 ("fused_window_pairs", ctypes.c_int32), ("stencil_offsets", ctypes.c_int32 * 17),
-                ("grid_row_len", ctypes.c_int32)]
+                ("grid_row_len", ctypes.c_int32), ("sym_slices", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         out = {name: getattr(self, name) for name, _ in self._fields_[:5]}
         out["stencil_offsets"] = list(self.stencil_offsets[:self.stencil_exact_planes])
         out["grid_row_len"] = self.grid_row_len
+        out["sym_slices"] = self.sym_slices
         return out
 
 
@@ -159,6 +160,7 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lfg_bench_spmv.argtypes = [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.POINTER(ctypes.c_double)]
         lib.lfg_bench_copy.argtypes = [_P, _I64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        lib.lfg_bench_stream.argtypes = [_P, _I64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
         lib.lfg_set_materials.argtypes = [_P, _I64, _P]
         lib.lfg_adjoint_solve.argtypes = [_P, ctypes.POINTER(PcgOptions), _P, _P, ctypes.POINTER(PcgReport)]
         lib.lfg_sensitivity.argtypes = [_P, _P, _P, _P]
@@ -541,6 +543,12 @@ class Device:
     def bench_copy(self, nbytes: int, reps: int = 20) -> float:
         ms = ctypes.c_double(0.0)
         self._check(self._lib.lfg_bench_copy(self._ctx, nbytes, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def bench_stream(self, nbytes: int, reps: int = 20) -> float:
+        """Average ms of the streaming calibration over five buffers of nbytes (4 read, 1 written)."""
+        ms = ctypes.c_double(0.0)
+        self._check(self._lib.lfg_bench_stream(self._ctx, nbytes, reps, ctypes.byref(ms)))
         return ms.value
 
     def bench_spmv(self, precision: str = "fp64", variant: int = 0, reps: int = 20) -> float:

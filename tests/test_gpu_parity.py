@@ -257,6 +257,8 @@ def check_stencil_spmv(dev, row_ptr, col, kc, x):
         assert np.all(np.abs(dev.spmv(x, "fp64", 8) - y_ref) <= 1e-14 * scale)
         assert check_compensated(dev.spmv(x, "mixed", 8), y_ref, scale, row_abs, x)
         check_translations(dev, row_ptr, col, kc, 8)
+    if info["sym_slices"]:  # symmetric node slices: lower blocks read as transposed upper ones
+        assert np.all(np.abs(dev.spmv(x, "fp64", 15) - y_ref) <= 1e-14 * scale)
     return info
 
 
@@ -376,10 +378,10 @@ def test_deterministic_and_graph_equals_direct(dev):
     u3, r3 = dev.pcg("jacobi", direct_launch=True, check_every=7)
     assert np.array_equal(bits(u1), bits(u2)) and np.array_equal(bits(u1), bits(u3))
     assert r1["iterations"] == r2["iterations"] == r3["iterations"]
-    for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13, 14):
+    for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13, 14, 15):
         u4, r4 = dev.pcg("jacobi", spmv_variant=variant)
         assert r4["converged"] and np.linalg.norm(u4 - u1) <= 1e-12 * np.linalg.norm(u1), variant
-    for variant in (10, 12, 13, 14):
+    for variant in (10, 12, 13, 14, 15):
         u10, r10 = dev.pcg("jacobi", spmv_variant=variant, direct_launch=True, check_every=7)
         assert np.array_equal(bits(u10), bits(dev.pcg("jacobi", spmv_variant=variant)[0])), variant
 
@@ -426,6 +428,47 @@ def test_fused_slice_iterations(dev, precond):
         with pytest.raises(native.LightFemError) as err:
             dev.pcg("ic0", spmv_variant=variant)
         assert err.value.code == native.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "block"])
+def test_symmetric_slice_iteration(dev, precond):
+    """Variant 15, the two-launch iteration on the symmetric node slices (each node's
+    blocks from its diagonal on hold values, the blocks before it are read as the
+    transposes of the column nodes' blocks), on an RCM-numbered mesh of more than 1,024
+    slices, where it is the FP64 default: the multiply within 1e-14 of |K||x| per row of
+    the CSR product, the PCG against the node-slice iteration (6) to 1e-9 with its
+    iteration count within 1% + 2, graph and direct launches bit-identical, the iteration
+    cap honoured; mixed precision keeps variant 13, and 15 refuses FP32 values, block
+    storage and the sweep preconditioners."""
+    print(f"case symmetric node slices 262x260 rcm {precond}", flush=True)
+    sys_ = pipeline.setup(meshgen.structured_cantilever(262, 260, young=meshgen.loguniform_young(262 * 260, 11)),
+                          device=dev, assembly="coloring", reordering="rcm")
+    info = dev.layout_info()
+    assert info["stencil_sym_planes"] == 0 and info["n_slices"] > 1024 and info["sym_slices"] == 1
+    kc, _ = dev.system()
+    x = np.random.default_rng(5).standard_normal(dev.n)
+    y_ref = lfo.spmv(sys_.row_ptr, sys_.col, kc, x)
+    scale = lfo.spmv(sys_.row_ptr, sys_.col, np.abs(kc), np.abs(x))
+    assert np.all(np.abs(dev.spmv(x, "fp64", 15) - y_ref) <= 1e-14 * scale)
+    u6, r6 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=6)
+    u15, r15 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=15)
+    u0, r0 = dev.pcg(precond, "fp64", "relative", 1e-10)  # the FP64 default at this size
+    ud, rd = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=15, direct_launch=True, check_every=7)
+    assert r15["converged"] and abs(r15["iterations"] - r6["iterations"]) <= 2 + 0.01 * r6["iterations"]
+    assert np.linalg.norm(u15 - u6) <= 1e-9 * np.linalg.norm(u6)
+    assert np.array_equal(bits(u0), bits(u15)) and np.array_equal(bits(ud), bits(u15))
+    assert r15["true_residual"] <= 1e-9 * r15["initial_residual"]
+    um, rm = dev.pcg(precond, "mixed", "relative", 1e-10)  # default: the FP32 node slices of 13
+    um13, rm13 = dev.pcg(precond, "mixed", "relative", 1e-10, spmv_variant=13)
+    assert np.array_equal(bits(um), bits(um13)) and rm["iterations"] == rm13["iterations"]
+    _, capped = dev.pcg(precond, "fp64", "relative", 1e-10, max_iterations=5, spmv_variant=15, raise_on_error=False)
+    assert capped["iterations"] == 5 and not capped["converged"]
+    for kwargs in ({"precision": "mixed"}, {"storage": "bsr"}, {"preconditioner": "ic0"}):
+        args = {"preconditioner": precond, "precision": "fp64", **kwargs}
+        with pytest.raises(native.LightFemError) as err:
+            dev.pcg(args["preconditioner"], args["precision"], "relative", 1e-10, spmv_variant=15,
+                    storage=kwargs.get("storage", "csr"))
+        assert err.value.code == native.ERR_UNSUPPORTED, kwargs
 
 
 def test_stencil_layouts(dev):
