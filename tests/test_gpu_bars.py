@@ -81,6 +81,10 @@ def test_bar_models_match_reference(dev, golden_index, name):
     assert bad == -1 and piv["dof"] == -1 and piv["pinned"] == pinned
     kc, fc = dev.system()
     assert np.array_equal(bits(kc), bits(kp)) and np.array_equal(bits(fc), bits(fp))
+    if dev.layout_info()["sym_slices"]:  # bar blocks mirrored like the Q4 ones (variant 15)
+        x = np.random.default_rng(3).standard_normal(row_ptr.shape[0] - 1)
+        scale = lfo.spmv(row_ptr, col, np.abs(kp), np.abs(x))
+        assert np.all(np.abs(dev.spmv(x, "fp64", 15) - lfo.spmv(row_ptr, col, kp, x)) <= 1e-14 * scale)
     u, rep = dev.pcg("jacobi", "fp64", "reference", meta["tolerance"])
     assert rep["converged"] == 1 and abs(rep["iterations"] - meta["iterations"]) <= 2 + 0.01 * meta["iterations"]
     assert np.linalg.norm(u - g["u"]) <= 1e-9 * np.linalg.norm(g["u"])
