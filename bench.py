@@ -279,9 +279,18 @@ def run_batch(dev, n_problems: int, tol: float, forms=("fp64", "mixed"), max_ite
         # block, the slice offsets, and per node the 4 cases' p read and q written (128 B)
         nodes = dev.n // 2
         n_slices = (nodes + 63) // 64
-        values = 8 * dev.nnz if form == "fp64" else 4 * dev.nnz + 32 * nodes
-        moved = values + 4 * (dev.nnz // 4) + 4 * (n_slices + 1) + 128 * nodes
+        if form == "fp64" and summary.get("sym_slices"):
+            # symmetric node slices: per node its blocks from the diagonal on (32 B of values
+            # and a column each) and the blocks before it (a column and the mirrored block's
+            # position: the mirrored values come from the L2), two offset arrays
+            blocks = dev.nnz // 4
+            upper = (blocks + nodes) // 2
+            moved = upper * 36 + (blocks - upper) * 8 + 8 * (n_slices + 1) + 128 * nodes
+        else:
+            values = 8 * dev.nnz if form == "fp64" else 4 * dev.nnz + 32 * nodes
+            moved = values + 4 * (dev.nnz // 4) + 4 * (n_slices + 1) + 128 * nodes
         ms = summary["multiply_ms"]
+        out[form]["sym_slices"] = bool(summary.get("sym_slices"))
         out[form]["roofline"] = {"kernel": "batch_spmv_kernel + batch_spmv_finish_kernel", "bound": "hbm",
                                  "bytes_per_launch": int(moved), "avg_launch_ms": round(ms, 4),
                                  "achieved": round(moved / (ms * 1e-3) / 1e9, 1) if ms > 0 else None,

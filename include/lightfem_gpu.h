@@ -309,6 +309,8 @@ typedef struct {
     int64_t max_iterations;   /* over systems */
     double multiply_ms;       /* the tuned batch multiply (with its per-problem fold), one launch with
                                  every problem running: device time of the fastest launch shape */
+    int32_t sym_slices;       /* 1: FP64 products on the symmetric node slices (lower blocks read as the
+                                 mirrored upper ones), 0: on the node slices */
 } lfg_batch_summary;
 
 /* Solves every (problem, case) system; u_out[c * n_dofs + i] (NULL to skip), reports

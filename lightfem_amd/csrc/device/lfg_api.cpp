@@ -2364,22 +2364,24 @@ lfg::BatchView batch_view(lfg_ctx* c, const lfg_batch_options* opt) {
 
 // Launch-shape autotuner: times the batch multiply (the dominant kernel) for each
 // block size and slices-per-wavefront candidate on this batch and keeps the fastest.
-void batch_autotune(lfg_ctx* c, const lfg::BatchView& bv, bool mixed) {
+void batch_autotune(lfg_ctx* c, const lfg::BatchView& bv, bool mixed, bool sym) {
     const int threads[] = {64, 128, 256, 512};
     const int per_wave[] = {1, 2, 4};
     lfg::launch_fill(c->stream, c->batch_p.p, static_cast<long long>(c->batch_p.n), 1.0);
     const lfg::SliceView v = slice_view(c);
-    const double* p64 = mixed ? nullptr : c->slices64.p;
+    const lfg::SymSliceView sv = sym_view(c);
+    const double* p64 = mixed ? nullptr : (sym ? c->sym64.p : c->slices64.p);
     const float* p32 = mixed ? c->slices32.p : nullptr;
+    const lfg::SymSliceView* svp = (sym && !mixed) ? &sv : nullptr;
     float best = 0.0f;
     lfg::BatchShape chosen{256, 1};
     LFG_HIP(hipEventRecord(c->ev_ring[0], c->stream));
     for (int t : threads)
         for (int w : per_wave) {
             const lfg::BatchShape shape{t, w};
-            lfg::launch_batch_spmv(c->stream, bv, shape, v, p64, p32, 1);  // warm-up
+            lfg::launch_batch_spmv(c->stream, bv, shape, v, p64, p32, 1, svp);  // warm-up
             LFG_HIP(hipEventRecord(c->ev_start, c->stream));
-            for (int i = 0; i < 5; ++i) lfg::launch_batch_spmv(c->stream, bv, shape, v, p64, p32, 1);
+            for (int i = 0; i < 5; ++i) lfg::launch_batch_spmv(c->stream, bv, shape, v, p64, p32, 1, svp);
             LFG_HIP(hipEventRecord(c->ev_stop, c->stream));
             sync(c);
             const float ms = elapsed(c->ev_start, c->ev_stop);
@@ -2398,8 +2400,9 @@ void batch_autotune(lfg_ctx* c, const lfg::BatchView& bv, bool mixed) {
 
 // PCG over all systems (right-hand sides batch_b, solutions in batch_x); returns the
 // number of graph launches.
-long long batch_solve(lfg_ctx* c, const lfg_batch_options* opt, const lfg::BatchView& bv, bool mixed) {
+long long batch_solve(lfg_ctx* c, const lfg_batch_options* opt, const lfg::BatchView& bv, bool mixed, bool sym) {
     const lfg::SliceView v = slice_view(c);
+    const lfg::SymSliceView sv = sym_view(c);
     const lfg::BatchShape shape = c->batch_shape;
     // five kernels per iteration (two multiply-and-fold, the direction): 16 iterations and
     // the compaction per captured graph
@@ -2410,7 +2413,8 @@ long long batch_solve(lfg_ctx* c, const lfg_batch_options* opt, const lfg::Batch
     LFG_HIP(hipMemcpyAsync(c->batch_global.p, &start, sizeof(start), hipMemcpyHostToDevice, c->stream));
     lfg::launch_batch_init(c->stream, bv, shape);
     auto iteration = [&] {
-        lfg::launch_batch_spmv(c->stream, bv, shape, v, c->slices64.p, mixed ? c->slices32.p : nullptr, 0);
+        lfg::launch_batch_spmv(c->stream, bv, shape, v, sym ? c->sym64.p : c->slices64.p,
+                               mixed ? c->slices32.p : nullptr, 0, sym ? &sv : nullptr);
         lfg::launch_batch_update(c->stream, bv, shape);
         lfg::launch_batch_direction(c->stream, bv, shape);
     };
@@ -2468,9 +2472,14 @@ int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg
         const bool mixed = opt->precision == LFG_PRECISION_MIXED;
         LFG_HIP(hipSetDevice(ctx->device));
         ensure_slices(ctx, true, mixed);
+        // FP64 products on the symmetric node slices where the pattern admits them (each
+        // mirrored pair of blocks streamed once); the FP32 ones and the final FP64 residual
+        // on the node slices
+        const bool sym = sym_slices_ok(ctx);
+        if (sym) ensure_sym_slices(ctx);
         lfg::BatchView bv = batch_view(ctx, opt);
         if (opt->tune) {
-            if (ctx->batch_tuned_for != opt->precision) batch_autotune(ctx, bv, mixed);
+            if (ctx->batch_tuned_for != opt->precision) batch_autotune(ctx, bv, mixed, sym);
         } else {
             const int t = opt->block_threads > 0 ? opt->block_threads : 256;
             const int w = opt->slices_per_wave > 0 ? opt->slices_per_wave : 1;
@@ -2483,7 +2492,7 @@ int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg
         std::vector<lfg::BatchSystem> sys(n_sys);
 
         LFG_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-        const long long launches = batch_solve(ctx, opt, bv, mixed);
+        const long long launches = batch_solve(ctx, opt, bv, mixed, sym);
         // FP64 residual of every system's solution
         lfg::launch_batch_residual(ctx->stream, bv, ctx->batch_shape, v, ctx->slices64.p, ctx->batch_res.p);
         LFG_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
@@ -2531,6 +2540,7 @@ int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg
             summary->systems_converged = converged;
             summary->max_iterations = max_iter;
             summary->multiply_ms = ctx->batch_multiply_ms;
+            summary->sym_slices = sym ? 1 : 0;
         }
         if (first_status != LFG_OK) throw LfgError(first_status, status_text(first_status));
         if (converged != static_cast<int>(n_sys))

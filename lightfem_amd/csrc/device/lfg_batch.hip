@@ -259,11 +259,106 @@ __device__ __forceinline__ void store_slice(double* dst, int s, const dpair* qv)
     for (int c = 0; c < NC; ++c) d[c] = qv[c];
 }
 
+// The same product on the symmetric node slices (SymSliceView, FP64 values): each row
+// adds its lower blocks (the transposes of the column nodes' upper blocks, ascending
+// columns) and then its upper blocks, every block applied to the NC case pairs of its
+// column node; within rounding of slice_multiply (a mirrored block differs only by Ke's
+// (a, b) / (b, a) rounding), with each pair of mirrored blocks streamed once. kWL, kWU > 0:
+// lists up to that wide take an unrolled path with unconditional (clamped) loads.
+template <int NC, int kWL, int kWU>
+__device__ __forceinline__ void sym_slice_multiply(const BatchView& bv, const SymSliceView& v,
+                                                   const double* __restrict__ planes, int s, double* pq,
+                                                   dpair* qout) {
+    const int node = s * kWave + lane_id();
+    const dpair* __restrict__ up = reinterpret_cast<const dpair*>(planes);
+    const dpair* __restrict__ lo = up + (v.ublocks + 1);
+    const dpair* __restrict__ p2 = reinterpret_cast<const dpair*>(bv.p);
+    double a0[NC], a1[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a0[c] = a1[c] = 0.0;
+    // block (rows u, l) times the NC pairs of node col
+    auto apply = [&](double ux, double uy, double lx, double ly, int col) {
+        const dpair* pc = p2 + static_cast<long long>(col) * NC;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const dpair x = pc[c];
+            a0[c] += ux * x.x;
+            a0[c] += uy * x.y;
+            a1[c] += lx * x.x;
+            a1[c] += ly * x.y;
+        }
+    };
+    const int lb = v.lptr[s];
+    const int lw = (v.lptr[s + 1] - lb) / kWave;
+    const int ub = v.uptr[s];
+    const int uw = (v.uptr[s + 1] - ub) / kWave;
+    if (kWL > 0 && kWU > 0 && lw > 0 && lw <= kWL && uw > 0 && uw <= kWU) {
+        int lc[kWL > 0 ? kWL : 1], lt[kWL > 0 ? kWL : 1], uc[kWU > 0 ? kWU : 1];
+        dpair tu[kWL > 0 ? kWL : 1], tl[kWL > 0 ? kWL : 1], uu[kWU > 0 ? kWU : 1], ul[kWU > 0 ? kWU : 1];
+#pragma unroll
+        for (int k = 0; k < kWL; ++k) {
+            const int at = lb + min(k, lw - 1) * kWave + lane_id();
+            lc[k] = __builtin_nontemporal_load(v.lcol + at);
+            lt[k] = __builtin_nontemporal_load(v.ltpos + at);
+        }
+#pragma unroll
+        for (int k = 0; k < kWU; ++k) {
+            const int at = ub + min(k, uw - 1) * kWave + lane_id();
+            uc[k] = __builtin_nontemporal_load(v.ucol + at);
+            uu[k] = up[at];
+            ul[k] = lo[at];
+        }
+#pragma unroll
+        for (int k = 0; k < kWL; ++k) {
+            tu[k] = up[lt[k]];
+            tl[k] = lo[lt[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kWL; ++k)
+            if (k < lw) apply(tu[k].x, tl[k].x, tu[k].y, tl[k].y, lc[k]);
+#pragma unroll
+        for (int k = 0; k < kWU; ++k)
+            if (k < uw) apply(uu[k].x, uu[k].y, ul[k].x, ul[k].y, uc[k]);
+    } else {
+        for (int k = 0; k < lw; ++k) {
+            const int at = lb + k * kWave + lane_id();
+            const int tp = __builtin_nontemporal_load(v.ltpos + at);
+            const dpair tu = up[tp], tl = lo[tp];
+            apply(tu.x, tl.x, tu.y, tl.y, __builtin_nontemporal_load(v.lcol + at));
+        }
+        for (int k = 0; k < uw; ++k) {
+            const int at = ub + k * kWave + lane_id();
+            const dpair uu = up[at], ul = lo[at];
+            apply(uu.x, uu.y, ul.x, ul.y, __builtin_nontemporal_load(v.ucol + at));
+        }
+    }
+    const dpair* pn = p2 + static_cast<long long>(node) * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        qout[c] = dpair{a0[c], a1[c]};
+        const dpair pv = pn[c];
+        pq[c] = pv.x * a0[c] + pv.y * a1[c];
+    }
+}
+
+// The FP64 product of one slice: the symmetric node slices where given (kSym: 1 = the
+// unrolled lists of a Q4 mesh, up to 4 lower and 5 upper blocks; 2 = any widths), else
+// the node slices.
+template <int NC, int kW, int kSym>
+__device__ __forceinline__ void fp64_multiply(const BatchView& bv, const SliceView& v, const SymSliceView& sv,
+                                              const double* __restrict__ planes64, int s, double* pq, dpair* qv) {
+    if (kSym == 1) sym_slice_multiply<NC, 4, 5>(bv, sv, planes64, s, pq, qv);
+    else if (kSym == 2) sym_slice_multiply<NC, 0, 0>(bv, sv, planes64, s, pq, qv);
+    else slice_multiply<NC, double, kW>(bv, v, planes64, s, pq, qv);
+}
+
 // q = K p and the slice's per-case p.q partials (slot s of the partials); the problem's
 // sums and alpha follow in batch_spmv_finish_kernel. kMixed: each problem's phase picks
-// its values (FP64 planes, or FP32 planes + compensated diagonal).
-template <int NC, int kW, bool kMixed>
-__global__ __launch_bounds__(512) void batch_spmv_kernel(BatchView bv, SliceView v, const double* __restrict__ planes64,
+// its values (FP64 planes, or FP32 planes + compensated diagonal); kSym > 0: the FP64
+// values are the symmetric node slices' (planes64 = their planes).
+template <int NC, int kW, bool kMixed, int kSym>
+__global__ __launch_bounds__(512) void batch_spmv_kernel(BatchView bv, SliceView v, SymSliceView sv,
+                                                         const double* __restrict__ planes64,
                                                          const float* __restrict__ planes32) {
     if (bv.global->done_all) return;
     const SliceRange range = active_range(bv);
@@ -274,7 +369,7 @@ __global__ __launch_bounds__(512) void batch_spmv_kernel(BatchView bv, SliceView
         double pq[NC];
         dpair qv[NC];
         if (kMixed && bv.problem_phase32[p]) slice_multiply<NC, float, kW>(bv, v, planes32, s, pq, qv);
-        else slice_multiply<NC, double, kW>(bv, v, planes64, s, pq, qv);
+        else fp64_multiply<NC, kW, kSym>(bv, v, sv, planes64, s, pq, qv);
         store_slice<NC>(bv.q, s, qv);
         double vals[NC];
 #pragma unroll
@@ -333,6 +428,19 @@ __global__ __launch_bounds__(512) void batch_multiply_kernel(BatchView bv, Slice
         double pq[NC];
         dpair qv[NC];
         slice_multiply<NC, VT, kW>(bv, v, planes, s, pq, qv);
+        store_slice<NC>(bv.q, s, qv);
+    }
+}
+
+// The same on the symmetric node slices (FP64).
+template <int NC, int kSym>
+__global__ __launch_bounds__(512) void batch_sym_multiply_kernel(BatchView bv, SliceView v, SymSliceView sv,
+                                                                 const double* __restrict__ planes) {
+    const SliceRange range = block_slices(bv.n_slices);
+    for (int s = range.first; s < range.stop; s += range.step) {
+        double pq[NC];
+        dpair qv[NC];
+        fp64_multiply<NC, 0, kSym>(bv, v, sv, planes, s, pq, qv);
         store_slice<NC>(bv.q, s, qv);
     }
 }
@@ -616,28 +724,38 @@ void launch_batch_init(hipStream_t s, const BatchView& bv, const BatchShape& sha
     LFG_BATCH_NC(bv.n_cases, (batch_init_kernel<NC><<<grid, shape.block_threads, 0, s>>>(bv)));
 }
 
-template <int NC, int kW>
-static void spmv_nc(hipStream_t s, const BatchView& bv, const SliceView& v, const double* planes64,
-                    const float* planes32, unsigned int grid, int threads, bool plain) {
+template <int NC, int kW, int kSym>
+static void spmv_nc(hipStream_t s, const BatchView& bv, const SliceView& v, const SymSliceView& sv,
+                    const double* planes64, const float* planes32, unsigned int grid, int threads, bool plain) {
     if (plain) {
-        if (planes64) batch_multiply_kernel<NC, double, kW><<<grid, threads, 0, s>>>(bv, v, planes64);
+        if (planes64 && kSym) batch_sym_multiply_kernel<NC, kSym><<<grid, threads, 0, s>>>(bv, v, sv, planes64);
+        else if (planes64) batch_multiply_kernel<NC, double, kW><<<grid, threads, 0, s>>>(bv, v, planes64);
         else batch_multiply_kernel<NC, float, kW><<<grid, threads, 0, s>>>(bv, v, planes32);
     } else {
-        if (planes32) batch_spmv_kernel<NC, kW, true><<<grid, threads, 0, s>>>(bv, v, planes64, planes32);
-        else batch_spmv_kernel<NC, kW, false><<<grid, threads, 0, s>>>(bv, v, planes64, nullptr);
+        if (planes32) batch_spmv_kernel<NC, kW, true, kSym><<<grid, threads, 0, s>>>(bv, v, sv, planes64, planes32);
+        else batch_spmv_kernel<NC, kW, false, kSym><<<grid, threads, 0, s>>>(bv, v, sv, planes64, nullptr);
         const unsigned int fin = blocks_for(static_cast<long long>(bv.n_problems) * kWave, 256);
         batch_spmv_finish_kernel<NC><<<fin, 256, 0, s>>>(bv);
     }
 }
 
 void launch_batch_spmv(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
-                       const double* planes64, const float* planes32, int plain) {
+                       const double* planes64, const float* planes32, int plain, const SymSliceView* sym) {
     const unsigned int grid = batch_grid(shape, bv.n_slices);
     const int t = shape.block_threads;
-    if (v.max_width <= 9) {
-        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 9>(s, bv, v, planes64, planes32, grid, t, plain != 0)));
+    const SymSliceView sv = sym ? *sym : SymSliceView{};
+    const int ksym = !sym ? 0 : (sym->max_lwidth <= 4 && sym->max_uwidth <= 5 ? 1 : 2);
+    // kW is the FP32 node slices' unrolled width (mixed), whatever the FP64 layout
+    if (ksym == 1 && v.max_width <= 9) {
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 9, 1>(s, bv, v, sv, planes64, planes32, grid, t, plain != 0)));
+    } else if (ksym == 1) {
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 0, 1>(s, bv, v, sv, planes64, planes32, grid, t, plain != 0)));
+    } else if (ksym == 2) {
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 0, 2>(s, bv, v, sv, planes64, planes32, grid, t, plain != 0)));
+    } else if (v.max_width <= 9) {
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 9, 0>(s, bv, v, sv, planes64, planes32, grid, t, plain != 0)));
     } else {
-        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 0>(s, bv, v, planes64, planes32, grid, t, plain != 0)));
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 0, 0>(s, bv, v, sv, planes64, planes32, grid, t, plain != 0)));
     }
 }
 

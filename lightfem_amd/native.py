@@ -113,7 +113,7 @@ class BatchSummary(ctypes.Structure):
                 ("problems_switched", ctypes.c_int32), ("block_threads", ctypes.c_int32),
                 ("slices_per_wave", ctypes.c_int32),
                 ("systems_converged", ctypes.c_int32), ("max_iterations", ctypes.c_int64),
-                ("multiply_ms", ctypes.c_double)]
+                ("multiply_ms", ctypes.c_double), ("sym_slices", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

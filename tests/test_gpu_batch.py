@@ -58,6 +58,7 @@ def test_batch_matches_direct_solves(dev):
     u, rep, summary = dev.batch_pcg("fp64", "relative", 1e-12)
     assert rep["converged"].all() and summary["systems_converged"] == 28
     assert summary["block_threads"] in (64, 128, 256, 512) and summary["slices_per_wave"] in (1, 2, 4)
+    assert summary["sym_slices"] == 1  # FP64 products on the symmetric node slices
     ref = direct_solutions(bs, loads)
     for p, up in enumerate(bs.split(u)):
         for c in range(4):
