@@ -89,15 +89,17 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int, variant: int, layout: dict) -
             return matrix + n_nodes * 184
         return matrix + (40 * n if variant == 7 else 16 * n)
     if variant == 15:
-        # symmetric node slices (FP64): each node's blocks from the diagonal on hold values
-        # (32 B) and a column, the blocks before it a column and the position of the mirrored
-        # block (read from the L2, not counted), per slice two offsets; the vector and
-        # multiply launches of variant 13 (184 B per node)
+        # symmetric node slices: each node's blocks from the diagonal on hold values (32 B;
+        # FP32 copy: 16 B, the diagonal entry zero, plus the compensated FP64 diagonal block,
+        # 32 B per node) and a column, the blocks before it a column and the position of the
+        # mirrored block (read from the L2, not counted), per slice two offsets; the vector
+        # and multiply launches of variant 13 (184 B per node)
         n_slices = (n // 2 + 63) // 64
         blocks = nnz // 4
         upper = (blocks + n_nodes) // 2
         lower = blocks - upper
-        return upper * 36 + lower * 8 + 8 * (n_slices + 1) + n_nodes * 184
+        values = upper * 36 if value_bytes == 8 else upper * 20 + n_nodes * 32
+        return values + lower * 8 + 8 * (n_slices + 1) + n_nodes * 184
     return nnz * (value_bytes + 4) + 4 * (n + 1) + 16 * n
 
 
@@ -458,7 +460,7 @@ def main() -> int:
         sweep[precision] = {}
         for variant in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15):
             if (variant in (6, 7, 12, 13, 14) and not layout["node_blocked"]) or (
-                    variant == 15 and (precision == "mixed" or not layout["sym_slices"])) or (
+                    variant == 15 and not layout["sym_slices"]) or (
                     variant == 14 and layout["n_slices"] > 1024) or (
                     variant == 8 and not layout["stencil_sym_planes"]) or (variant == 9 and (
                     precision == "mixed" or not layout["stencil_exact_planes"])) or (
@@ -570,6 +572,19 @@ def main() -> int:
             "true_relative_residual": rep15["true_residual"] / rep15["initial_residual"],
             "rel_u_diff_vs_fp64": float(np.linalg.norm(u15 - u64) / np.linalg.norm(u64))}
         print(f"bench: fp64 symmetric node slices {forms['fp64_sym_slices']}", file=sys.stderr, flush=True)
+        # mixed precision on the same layout (its FP32 copy with the compensated diagonal),
+        # the default mixed iteration of such patterns
+        t = time.perf_counter()
+        um15, repm15 = dev.pcg("jacobi", "mixed", "relative", args.tol, 10 ** 6, args.check_every, spmv_variant=15,
+                               raise_on_error=False)
+        forms["mixed_sym_slices"] = {
+            "ms_per_solve": (time.perf_counter() - t) * 1e3, "device_solve_ms": repm15["solve_ms"],
+            "iterations": repm15["iterations"], "converged": bool(repm15["converged"]), "spmv_variant": 15,
+            "refine_passes": repm15["refine_passes"],
+            "true_relative_residual": repm15["true_residual"] / repm15["initial_residual"],
+            "rel_u_diff_vs_fp64": float(np.linalg.norm(um15 - u64) / np.linalg.norm(u64)),
+            "speedup_vs_fp64_sym_slices": rep15["solve_ms"] / max(repm15["solve_ms"], 1e-9)}
+        print(f"bench: mixed symmetric node slices {forms['mixed_sym_slices']}", file=sys.stderr, flush=True)
     per_iter_ms = {k: v["device_solve_ms"] / max(1, v["iterations"]) for k, v in forms.items()}
     cpu, protocol = None, None
     if not args.skip_cpu:

@@ -75,9 +75,8 @@ typedef struct {
     int32_t direct_launch;   /* 1: launch each kernel directly instead of replaying a captured hipGraph */
     int32_t spmv_variant;    /* 0 = default: for jacobi / block 14 on a node-blocked pattern of at
                                 most 1024 slices (any numbering), else 11, else 10 where the node
-                                stencil applies, else on a node-blocked pattern 15 with FP64 values
-                                (13 with mixed precision, or where the pattern is not structurally
-                                symmetric); for ic0 /
+                                stencil applies, else on a node-blocked pattern 15 (13 where the
+                                pattern is not structurally symmetric); for ic0 /
                                 subdomains 8 where the stencil applies, else 6; else 4. 1-3: node
                                 groups of 4/8/16 lanes per row;
                                 4, 5: row-segment streaming with 64 / 32 rows per wavefront; 6:
@@ -109,8 +108,9 @@ typedef struct {
                                 one launch later; up to 1024 slices); 15: the iteration of 13 on the
                                 symmetric node slices (per node the blocks from the diagonal on hold
                                 values, those before it are read as the transposes of the column
-                                nodes' blocks: each mirrored pair streams once; FP64 values, CSR
-                                order). 10-15 apply the jacobi and block preconditioners only. */
+                                nodes' blocks: each mirrored pair streams once; CSR order; its FP32
+                                copy keeps a rounding-compensated FP64 diagonal). 10-15 apply the
+                                jacobi and block preconditioners only. */
     int32_t storage;         /* LFG_STORAGE_*: BSR sums each 2x2 block's products first (variants 6, 7
                                 and 9 follow it bit for bit; 8 and 10 read the transposed blocks as
                                 always; 1-5 are CSR only). Needs a node-blocked pattern. */
@@ -345,7 +345,7 @@ int lfg_set_subdomains(lfg_ctx* ctx, int64_t n_subdomains, const int32_t* node_s
 /* Checks and measurement hooks on the constrained system. */
 int lfg_precond_apply(lfg_ctx* ctx, int32_t preconditioner, const double* in, double* out);
 /* y = Kc x with the PCG SpMV kernel of `variant` (values of `precision`; 10 and 11 multiply
- * as 8, 12-14 as 6, 15 on the symmetric node slices (FP64); variant | LFG_SPMV_BLOCK_ORDER
+ * as 8, 12-14 as 6, 15 on the symmetric node slices; variant | LFG_SPMV_BLOCK_ORDER
  * for 6, 7, 9: the block storage order). */
 int lfg_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, const double* x, double* y);
 /* Allocation-free form of lfg_spmv, for callers that cannot allocate device memory (code

@@ -135,11 +135,12 @@ struct lfg_ctx {
     // first use (also from const contexts): sym_state 0 = not built yet, 1 = built, -1 = the
     // pattern is not structurally symmetric
     mutable int sym_state = 0;
-    bool sym64_valid = false;
+    bool sym64_valid = false, sym32_valid = false;
     mutable int sym_max_uwidth = 0, sym_max_lwidth = 0;
     mutable long long sym_ublocks = 0;
     mutable DevArray<int> sym_uptr, sym_ucol, sym_lptr, sym_lcol, sym_ltpos;
-    DevArray<double> sym64;
+    DevArray<double> sym64, sym_diag;  // sym_diag: compensated FP64 diagonal of sym32
+    DevArray<float> sym32;
     // node-stencil copies of the constrained values (SpMV variants 8 and 9), when every
     // block column lies at one of a few fixed node offsets (structured meshes numbered
     // row by row)
@@ -287,7 +288,7 @@ lfg::SliceView slice_view(const lfg_ctx* c) {
 
 lfg::SymSliceView sym_view(const lfg_ctx* c) {
     return lfg::SymSliceView{c->n_slices,   c->sym_ublocks,  c->sym_uptr.p,      c->sym_ucol.p,    c->sym_lptr.p,
-                             c->sym_lcol.p, c->sym_ltpos.p,  c->sym_max_uwidth, c->sym_max_lwidth};
+                             c->sym_lcol.p, c->sym_ltpos.p,  c->sym_max_uwidth, c->sym_max_lwidth, c->sym_diag.p};
 }
 
 // Sets the block storage summation order for the products enqueued in its scope.
@@ -306,7 +307,7 @@ constexpr int kStencilGrid = 11;  // the iteration of 10 on 2D tiles (row-struct
 constexpr int kSliceIter = 12;    // node slices, whole Jacobi / block PCG iteration in one launch
 constexpr int kSliceSplit = 13;   // the iteration of 12 in two launches (vector update, multiply)
 constexpr int kSliceFold = 14;    // the iteration of 12, its reduction folded by the next launch
-constexpr int kSliceSym = 15;     // the iteration of 13 on the symmetric node slices (FP64 values)
+constexpr int kSliceSym = 15;     // the iteration of 13 on the symmetric node slices
 constexpr int kMaxVariant = kSliceSym;
 constexpr int kSliceIterMaxSlices = 1024;  // above this the split iteration (13) is the faster default
 static_assert(kSliceIterMaxSlices <= lfg::kFoldMaxSlices, "variant 14 serves the one-launch default");
@@ -495,7 +496,7 @@ void build_slices(lfg_ctx* c) {
     c->have_slices = false;
     c->slices64_valid = c->slices32_valid = false;
     c->sym_state = 0;
-    c->sym64_valid = false;
+    c->sym64_valid = c->sym32_valid = false;
     const int n_nodes = c->n / 2;
     const std::vector<int>& rp = c->h_row_ptr;
     const std::vector<int>& col = c->h_col;
@@ -697,14 +698,22 @@ void ensure_slices(lfg_ctx* c, bool need64, bool need32) {
     }
 }
 
-// Packs the constrained values into the symmetric slice planes (variant 15).
-void ensure_sym_slices(lfg_ctx* c) {
+// Packs the constrained values into the symmetric slice planes (variant 15): FP64, and the
+// FP32 copy with its compensated diagonal.
+void ensure_sym_slices(lfg_ctx* c, bool need64, bool need32) {
     require(sym_slices_ok(c), LFG_ERR_UNSUPPORTED,
             "the symmetric node-slice layout (15) needs a node-blocked, structurally symmetric pattern");
-    if (c->sym64_valid) return;
-    c->sym64.alloc(4 * static_cast<size_t>(c->sym_ublocks + 1));
-    lfg::launch_pack_sym_slices(c->stream, csr_view(c), sym_view(c), c->Kc.p, c->sym64.p);
-    c->sym64_valid = true;
+    if (need64 && !c->sym64_valid) {
+        c->sym64.alloc(4 * static_cast<size_t>(c->sym_ublocks + 1));
+        lfg::launch_pack_sym_slices(c->stream, csr_view(c), sym_view(c), c->Kc.p, c->sym64.p, nullptr);
+        c->sym64_valid = true;
+    }
+    if (need32 && !c->sym32_valid) {
+        c->sym32.alloc(4 * static_cast<size_t>(c->sym_ublocks + 1));
+        c->sym_diag.alloc(2 * static_cast<size_t>(c->n));
+        lfg::launch_pack_sym_slices(c->stream, csr_view(c), sym_view(c), c->Kc.p, nullptr, c->sym32.p);
+        c->sym32_valid = true;
+    }
 }
 
 void sync(lfg_ctx* c) { LFG_HIP(hipStreamSynchronize(c->stream)); }
@@ -1150,7 +1159,8 @@ void enqueue_iteration(lfg_ctx* c, const lfg::PcgBuffers& b, int mixed, int repl
             return;
         }
         if (variant == kSliceSym) {
-            lfg::launch_pcg_sym_split_slices(c->stream, b, sym_view(c), c->sym64.p, rb[parity], rb[parity ^ 1],
+            lfg::launch_pcg_sym_split_slices(c->stream, b, sym_view(c), c->sym64.p, mixed ? c->sym32.p : nullptr,
+                                             rb[parity], rb[parity ^ 1],
                                              pb[parity], pb[parity ^ 1], qb[parity], qb[parity ^ 1]);
             return;
         }
@@ -1426,7 +1436,7 @@ int lfg_pin_zero_pivots(lfg_ctx* ctx, lfg_pivot_report* report) {
         if (report) *report = rep;
         if (rep.pinned > 0) {  // the packed copies and factors of Kc are stale
             ctx->k32_valid = false;
-            ctx->slices64_valid = ctx->slices32_valid = ctx->sym64_valid = false;
+            ctx->slices64_valid = ctx->slices32_valid = ctx->sym64_valid = ctx->sym32_valid = false;
             ctx->stencil64_valid = ctx->stencil32_valid = ctx->stencil_exact_valid = false;
             ctx->precond_ready = -1;
         }
@@ -1649,7 +1659,7 @@ int lfg_dirichlet(lfg_ctx* ctx, int64_t n_constraints, const int32_t* dofs, cons
         ctx->have_system = true;
         ctx->have_solution = false;
         ctx->k32_valid = false;
-        ctx->slices64_valid = ctx->slices32_valid = ctx->sym64_valid = false;
+        ctx->slices64_valid = ctx->slices32_valid = ctx->sym64_valid = ctx->sym32_valid = false;
         ctx->stencil64_valid = ctx->stencil32_valid = ctx->stencil_exact_valid = false;
         ctx->precond_ready = -1;
         return LFG_OK;
@@ -1700,16 +1710,14 @@ void run_pcg(lfg_ctx* ctx, const lfg_pcg_options* opt, const double* rhs, double
     if (kind == LFG_PRECOND_SUBDOMAIN) degenerate = ctx->sub.degenerate;
     const bool block_order = opt->storage == LFG_STORAGE_BSR;
     int variant = resolve_variant(ctx, opt->spmv_variant, kind, block_order);
-    // FP64 values on a large pattern without a stencil: the two-launch iteration on the
-    // symmetric node slices, which streams each mirrored pair of blocks once
-    if (opt->spmv_variant == 0 && variant == kSliceSplit && !mixed && sym_slices_ok(ctx)) variant = kSliceSym;
+    // a large pattern without a stencil: the two-launch iteration on the symmetric node
+    // slices, which streams each mirrored pair of blocks once
+    if (opt->spmv_variant == 0 && variant == kSliceSplit && sym_slices_ok(ctx)) variant = kSliceSym;
     const PairScope pair_scope(ctx, block_order);
     require(!(mixed && variant == kStencilExact), LFG_ERR_UNSUPPORTED,
             "mixed precision uses the symmetric stencil form (variant 8), not the exact one");
-    require(!(mixed && variant == kSliceSym), LFG_ERR_UNSUPPORTED,
-            "the symmetric node slices (variant 15) hold FP64 values; mixed precision runs on 6, 7 or 12-14");
     if (variant == kSliceSym) {
-        ensure_sym_slices(ctx);
+        ensure_sym_slices(ctx, true, mixed != 0);
     } else if (stencil_layout(variant)) {
         ensure_stencil(ctx, variant, true, mixed != 0);
     } else if (slice_layout(variant)) {
@@ -2039,7 +2047,8 @@ static void run_spmv(lfg_ctx* ctx, int precision, int variant, bool fused_dir = 
         lfg::launch_pcg_spmv_slices_dir(ctx->stream, b, slice_view(ctx), f32 ? nullptr : ctx->slices64.p,
                                         f32 ? ctx->slices32.p : nullptr, ctx->p.p, ctx->p2.p);
     else if (variant == kSliceSym)
-        lfg::launch_pcg_spmv_sym_slices(ctx->stream, b, sym_view(ctx), ctx->sym64.p);
+        lfg::launch_pcg_spmv_sym_slices(ctx->stream, b, sym_view(ctx), f32 ? nullptr : ctx->sym64.p,
+                                        f32 ? ctx->sym32.p : nullptr);
     else if (stencil_layout(variant))
         launch_stencil_spmv(ctx, b, variant, !f32, f32);
     else if (slice_layout(variant))
@@ -2055,9 +2064,7 @@ static void prepare_spmv(lfg_ctx* ctx, int precision, int variant) {
     lfg::PcgState zero{};
     LFG_HIP(hipMemcpyAsync(ctx->state.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
     if (variant == kSliceSym) {
-        require(precision != LFG_PRECISION_MIXED, LFG_ERR_UNSUPPORTED,
-                "the symmetric node slices (variant 15) hold FP64 values only");
-        ensure_sym_slices(ctx);
+        ensure_sym_slices(ctx, precision != LFG_PRECISION_MIXED, precision == LFG_PRECISION_MIXED);
     } else if (stencil_layout(variant)) {
         require(!(variant == kStencilExact && precision == LFG_PRECISION_MIXED), LFG_ERR_UNSUPPORTED,
                 "the exact stencil form has no FP32 copy");
@@ -2089,7 +2096,7 @@ int lfg_spmv_noalloc(lfg_ctx* ctx, int32_t precision, int32_t variant, const dou
         bool ready = false;
         unsigned int grid = 0;
         if (v == kSliceSym) {
-            ready = !f32 && ctx->sym64_valid;
+            ready = f32 ? ctx->sym32_valid : ctx->sym64_valid;
             grid = static_cast<unsigned int>(lfg::slice_spmv_blocks(ctx->n_slices));
         } else if (stencil_layout(v)) {
             ready = v == kStencilExact ? (!f32 && ctx->stencil_exact_valid)
@@ -2124,7 +2131,8 @@ int lfg_spmv_noalloc(lfg_ctx* ctx, int32_t precision, int32_t variant, const dou
         b.tickets = reinterpret_cast<lfg::TicketSet*>(base + lfg::kNoallocTicketsAt);
         b.partials = reinterpret_cast<double*>(base + lfg::kNoallocPartialsAt);
         if (v == kSliceSym)
-            lfg::launch_pcg_spmv_sym_slices(ctx->stream, b, sym_view(ctx), ctx->sym64.p);
+            lfg::launch_pcg_spmv_sym_slices(ctx->stream, b, sym_view(ctx), f32 ? nullptr : ctx->sym64.p,
+                                            f32 ? ctx->sym32.p : nullptr);
         else if (stencil_layout(v))
             launch_stencil_spmv(ctx, b, v, !f32, f32);
         else if (slice_layout(v))
@@ -2189,8 +2197,7 @@ int lfg_bench_spmv(lfg_ctx* ctx, int32_t precision, int32_t variant, int32_t rep
             if (ctx->precond_ready != LFG_PRECOND_JACOBI) setup_preconditioner(ctx, LFG_PRECOND_JACOBI, &degenerate);
             const bool f32 = precision == LFG_PRECISION_MIXED;
             if (v == kSliceSym) {
-                require(!f32, LFG_ERR_UNSUPPORTED, "the symmetric node slices (variant 15) hold FP64 values only");
-                ensure_sym_slices(ctx);
+                ensure_sym_slices(ctx, !f32, f32);
             } else if (slice_layout(v)) {
                 ensure_slices(ctx, !f32, f32);
             } else {
@@ -2319,7 +2326,7 @@ int lfg_batch_set_loads(lfg_ctx* ctx, int32_t n_cases, const double* loads, int6
         ctx->have_system = true;
         ctx->have_solution = ctx->have_adjoint = false;
         ctx->k32_valid = false;
-        ctx->slices64_valid = ctx->slices32_valid = ctx->sym64_valid = false;
+        ctx->slices64_valid = ctx->slices32_valid = ctx->sym64_valid = ctx->sym32_valid = false;
         ctx->stencil64_valid = ctx->stencil32_valid = ctx->stencil_exact_valid = false;
         ctx->precond_ready = -1;
         ctx->have_batch_loads = true;
@@ -2476,7 +2483,7 @@ int lfg_batch_pcg(lfg_ctx* ctx, const lfg_batch_options* opt, double* u_out, lfg
         // mirrored pair of blocks streamed once); the FP32 ones and the final FP64 residual
         // on the node slices
         const bool sym = sym_slices_ok(ctx);
-        if (sym) ensure_sym_slices(ctx);
+        if (sym) ensure_sym_slices(ctx, true, false);
         lfg::BatchView bv = batch_view(ctx, opt);
         if (opt->tune) {
             if (ctx->batch_tuned_for != opt->precision) batch_autotune(ctx, bv, mixed, sym);

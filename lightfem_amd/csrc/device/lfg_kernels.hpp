@@ -270,7 +270,11 @@ void launch_pcg_split_slices(hipStream_t s, const PcgBuffers& b, const SliceView
 // Both lists are stored per 64-node slice column-major, as in SliceView (entry k of node
 // 64 s + l at ptr[s] + 64 k + l). The upper values are two pair planes (upper pairs, then
 // lower pairs), each with a zero block at index ublocks that padded lower entries point
-// to; padded upper entries are zero blocks. FP64 values only.
+// to; padded upper entries are zero blocks. The FP32 copy holds the off-diagonal upper
+// blocks in FP32 (the diagonal entry zero) and the diagonal block in FP64 (diag32), minus
+// the rounding error of every FP32 block the node's rows read (its own upper blocks and
+// the mirrored blocks of its lower ones), so each row keeps its FP64 sum and nodal
+// translations multiply exactly, as in the other FP32 layouts.
 struct SymSliceView {
     int n_slices;
     long long ublocks;  // padded upper entries, uptr[n_slices]
@@ -280,18 +284,22 @@ struct SymSliceView {
     const int* lcol;    // node column per lower entry
     const int* ltpos;   // position of the transposed upper block per lower entry
     int max_uwidth, max_lwidth;
+    const double* diag32;  // FP32 copy: compensated FP64 diagonal block per node (two pairs)
 };
-// Copies every node's upper blocks from its two CSR rows into the planes
-// (4 * (ublocks + 1) doubles), and the zero block.
+// Copies every node's upper blocks from its two CSR rows into the FP64 planes
+// (4 * (ublocks + 1) doubles) and / or the FP32 planes (as many floats; then the
+// compensated diagonal into v.diag32), with the zero block.
 void launch_pack_sym_slices(hipStream_t s, const CsrView& a, const SymSliceView& v, const double* values,
-                            double* planes);
-// q = A p on the symmetric node slices, p.q and alpha (as launch_pcg_spmv_slices).
-void launch_pcg_spmv_sym_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes);
+                            double* planes64, float* planes32);
+// q = A p on the symmetric node slices (FP64 planes, or FP32 ones when planes64 is null or
+// the state's phase says so), p.q and alpha (as launch_pcg_spmv_slices).
+void launch_pcg_spmv_sym_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes64,
+                                const float* planes32);
 // Variant 15: the two-launch iteration of variant 13 with the multiply on the symmetric
 // node slices.
-void launch_pcg_sym_split_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes,
-                                 const double* r_in, double* r_out, const double* p_in, double* p_out,
-                                 const double* q_in, double* q_out);
+void launch_pcg_sym_split_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes64,
+                                 const float* planes32, const double* r_in, double* r_out, const double* p_in,
+                                 double* p_out, const double* q_in, double* q_out);
 // x += alpha p, r -= alpha q, ||r||^2, r.z; convergence / replacement decision, beta.
 void launch_pcg_update(hipStream_t s, const PcgBuffers& b, int mixed);
 // mixed only: r = b - K x in FP64 when a replacement is due; then r.z and beta.

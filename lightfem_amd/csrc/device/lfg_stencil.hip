@@ -1395,18 +1395,20 @@ __global__ __launch_bounds__(kThreads) void fused_flush_kernel(PcgBuffers b, con
 
 // One thread per (slice, lane): node 64 s + l's upper blocks from its two CSR rows (the
 // blocks from the first column pair >= 2i on), zero padding; thread 0 also writes the zero
-// block the padded lower entries point to.
+// block the padded lower entries point to. VT = float: the FP32 copy, whose diagonal entry
+// is zero (the compensated FP64 diagonal block is added separately).
+template <typename VT>
 __global__ __launch_bounds__(256) void pack_sym_slices_kernel(int n_nodes, const int* __restrict__ rp,
                                                               const int* __restrict__ col, SymSliceView v,
                                                               const double* __restrict__ values,
-                                                              double* __restrict__ planes) {
+                                                              VT* __restrict__ planes) {
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int s = static_cast<int>(t >> 6);
     const int lane = static_cast<int>(t & (kWave - 1));
     if (t == 0) {
         for (int h = 0; h < 2; ++h) {
-            planes[2 * (h * (v.ublocks + 1) + v.ublocks)] = 0.0;
-            planes[2 * (h * (v.ublocks + 1) + v.ublocks) + 1] = 0.0;
+            planes[2 * (h * (v.ublocks + 1) + v.ublocks)] = VT(0);
+            planes[2 * (h * (v.ublocks + 1) + v.ublocks) + 1] = VT(0);
         }
     }
     if (s >= v.n_slices) return;
@@ -1423,11 +1425,14 @@ __global__ __launch_bounds__(256) void pack_sym_slices_kernel(int n_nodes, const
         if (!lone)
             while (dlo < deg && col[r0 + 2 * dlo] < 2 * node) ++dlo;
     }
+    constexpr bool kF32 = sizeof(VT) == sizeof(float);
     for (int k = 0; k < width; ++k) {
         const long long at = begin + static_cast<long long>(k) * kWave + lane;
         double u0 = 0.0, u1 = 0.0, l0 = 0.0, l1 = 0.0;
         const int kk = dlo + k;
-        if (lone && k == 0) {
+        if (kF32 && k == 0) {
+            // the diagonal block: FP64 in diag32
+        } else if (lone && k == 0) {
             u0 = values[r0];
             l1 = values[r1];
         } else if (!lone && kk < deg) {
@@ -1436,26 +1441,79 @@ __global__ __launch_bounds__(256) void pack_sym_slices_kernel(int n_nodes, const
             l0 = values[r1 + 2 * kk];
             l1 = values[r1 + 2 * kk + 1];
         }
-        planes[2 * at] = u0;
-        planes[2 * at + 1] = u1;
-        planes[2 * (v.ublocks + 1 + at)] = l0;
-        planes[2 * (v.ublocks + 1 + at) + 1] = l1;
+        planes[2 * at] = static_cast<VT>(u0);
+        planes[2 * at + 1] = static_cast<VT>(u1);
+        planes[2 * (v.ublocks + 1 + at)] = static_cast<VT>(l0);
+        planes[2 * (v.ublocks + 1 + at) + 1] = static_cast<VT>(l1);
     }
 }
 
-// Rows 2i, 2i+1 of node i = 64 s + lane over both lists, p from p2. kWL / kWU > 0: lists up
-// to that wide take unrolled paths whose loads are unconditional (entries clamped to the
-// last one) and whose sums skip the clamped entries.
-template <int kWL, int kWU>
-__device__ __forceinline__ void sym_slice_rows(const SymSliceView& v, const double* __restrict__ planes,
+// The FP32 copy's diagonal block per node, after its planes are packed: the FP64 block
+// minus, entry by entry, the rounding error of every FP32 value the node's rows read —
+// its upper blocks' own values, and for a lower block (i, j) the mirrored FP32 block (j, i)
+// in place of the FP64 values of block (i, j) — so every row keeps its FP64 sum.
+__global__ __launch_bounds__(256) void sym_diag32_kernel(int n_nodes, const int* __restrict__ rp,
+                                                         const int* __restrict__ col, SymSliceView v,
+                                                         const double* __restrict__ values,
+                                                         const float* __restrict__ planes32,
+                                                         double* __restrict__ diag32) {
+    const int node = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (node >= n_nodes) return;
+    const int r0 = rp[2 * node], r1 = rp[2 * node + 1];
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    if (r1 - r0 == 1) {  // a node in no element
+        d0 = values[r0];
+        d3 = values[r1];
+    } else {
+        const int deg = (r1 - r0) >> 1;
+        int dlo = 0;
+        while (dlo < deg && col[r0 + 2 * dlo] < 2 * node) ++dlo;
+        const fpair* __restrict__ up = reinterpret_cast<const fpair*>(planes32);
+        const fpair* __restrict__ lo = up + (v.ublocks + 1);
+        const int s = node / kWave, lane = node % kWave;
+        for (int k = 0; k < deg; ++k) {
+            const double u0 = values[r0 + 2 * k], u1 = values[r0 + 2 * k + 1];
+            const double l0 = values[r1 + 2 * k], l1 = values[r1 + 2 * k + 1];
+            if (k == dlo) {  // the diagonal block
+                d0 += u0;
+                d1 += u1;
+                d2 += l0;
+                d3 += l1;
+            } else if (k > dlo) {  // an upper block: its own FP32 values
+                d0 -= static_cast<double>(static_cast<float>(u0)) - u0;
+                d1 -= static_cast<double>(static_cast<float>(u1)) - u1;
+                d2 -= static_cast<double>(static_cast<float>(l0)) - l0;
+                d3 -= static_cast<double>(static_cast<float>(l1)) - l1;
+            } else {  // a lower block: read as the transpose of the mirrored FP32 block
+                const int tp = v.ltpos[v.lptr[s] + k * kWave + lane];
+                const fpair tu = up[tp], tl = lo[tp];
+                d0 -= static_cast<double>(tu.x) - u0;
+                d1 -= static_cast<double>(tl.x) - u1;
+                d2 -= static_cast<double>(tu.y) - l0;
+                d3 -= static_cast<double>(tl.y) - l1;
+            }
+        }
+    }
+    reinterpret_cast<dpair*>(diag32)[2 * node] = dpair{d0, d1};
+    reinterpret_cast<dpair*>(diag32)[2 * node + 1] = dpair{d2, d3};
+}
+
+// Rows 2i, 2i+1 of node i = 64 s + lane over both lists, p from p2 (values of type VT; the
+// FP32 copy adds its compensated FP64 diagonal block last). kWL / kWU > 0: lists up to that
+// wide take unrolled paths whose loads are unconditional (entries clamped to the last one)
+// and whose sums skip the clamped entries.
+template <typename VT, int kWL, int kWU>
+__device__ __forceinline__ void sym_slice_rows(const SymSliceView& v, const VT* __restrict__ planes,
                                                const dpair* __restrict__ p2, int s, double& a0, double& a1) {
+    using P = typename PairOf<VT>::type;
     const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
-    const dpair* __restrict__ up = reinterpret_cast<const dpair*>(planes);
-    const dpair* __restrict__ lo = up + (v.ublocks + 1);
+    const P* __restrict__ up = reinterpret_cast<const P*>(planes);
+    const P* __restrict__ lo = up + (v.ublocks + 1);
     const int lb = v.lptr[s];
     const int lw = (v.lptr[s + 1] - lb) / kWave;
     const int ub = v.uptr[s];
     const int uw = (v.uptr[s + 1] - ub) / kWave;
+    auto add = [&](double ux, double uy, double lx, double ly, dpair x) { add_block(a0, a1, ux, uy, lx, ly, x, false); };
     if (kWL > 0 && kWU > 0 && lw > 0 && lw <= kWL && uw > 0 && uw <= kWU) {
         int lc[kWL > 0 ? kWL : 1], lt[kWL > 0 ? kWL : 1], uc[kWU > 0 ? kWU : 1];
 #pragma unroll
@@ -1464,7 +1522,7 @@ __device__ __forceinline__ void sym_slice_rows(const SymSliceView& v, const doub
             lc[k] = __builtin_nontemporal_load(v.lcol + at);
             lt[k] = __builtin_nontemporal_load(v.ltpos + at);
         }
-        dpair uu[kWU > 0 ? kWU : 1], ul[kWU > 0 ? kWU : 1];
+        P uu[kWU > 0 ? kWU : 1], ul[kWU > 0 ? kWU : 1];
 #pragma unroll
         for (int k = 0; k < kWU; ++k) {
             const int at = ub + min(k, uw - 1) * kWave + lane;
@@ -1472,7 +1530,8 @@ __device__ __forceinline__ void sym_slice_rows(const SymSliceView& v, const doub
             uu[k] = up[at];
             ul[k] = lo[at];
         }
-        dpair tu[kWL > 0 ? kWL : 1], tl[kWL > 0 ? kWL : 1], xl[kWL > 0 ? kWL : 1];
+        P tu[kWL > 0 ? kWL : 1], tl[kWL > 0 ? kWL : 1];
+        dpair xl[kWL > 0 ? kWL : 1];
 #pragma unroll
         for (int k = 0; k < kWL; ++k) {
             tu[k] = up[lt[k]];
@@ -1484,24 +1543,32 @@ __device__ __forceinline__ void sym_slice_rows(const SymSliceView& v, const doub
         for (int k = 0; k < kWU; ++k) xu[k] = p2[uc[k]];
 #pragma unroll
         for (int k = 0; k < kWL; ++k)
-            if (k < lw) add_block(a0, a1, tu[k].x, tl[k].x, tu[k].y, tl[k].y, xl[k], false);
+            if (k < lw) add(tu[k].x, tl[k].x, tu[k].y, tl[k].y, xl[k]);
 #pragma unroll
         for (int k = 0; k < kWU; ++k)
-            if (k < uw) add_block(a0, a1, uu[k].x, uu[k].y, ul[k].x, ul[k].y, xu[k], false);
-        return;
+            if (k < uw) add(uu[k].x, uu[k].y, ul[k].x, ul[k].y, xu[k]);
+    } else {
+        for (int k = 0; k < lw; ++k) {
+            const int at = lb + k * kWave + lane;
+            const int tp = __builtin_nontemporal_load(v.ltpos + at);
+            const P tu = up[tp], tl = lo[tp];
+            add(tu.x, tl.x, tu.y, tl.y, p2[__builtin_nontemporal_load(v.lcol + at)]);
+        }
+        for (int k = 0; k < uw; ++k) {
+            const int at = ub + k * kWave + lane;
+            const P uu = up[at], ul = lo[at];
+            add(uu.x, uu.y, ul.x, ul.y, p2[__builtin_nontemporal_load(v.ucol + at)]);
+        }
     }
-    for (int k = 0; k < lw; ++k) {
-        const int at = lb + k * kWave + lane;
-        const int tp = __builtin_nontemporal_load(v.ltpos + at);
-        const dpair tu = up[tp], tl = lo[tp];
-        const dpair x = p2[__builtin_nontemporal_load(v.lcol + at)];
-        add_block(a0, a1, tu.x, tl.x, tu.y, tl.y, x, false);
-    }
-    for (int k = 0; k < uw; ++k) {
-        const int at = ub + k * kWave + lane;
-        const dpair uu = up[at], ul = lo[at];
-        const dpair x = p2[__builtin_nontemporal_load(v.ucol + at)];
-        add_block(a0, a1, uu.x, uu.y, ul.x, ul.y, x, false);
+    if (sizeof(VT) == sizeof(float)) {
+        const int node = s * kWave + lane;
+        const dpair du = reinterpret_cast<const dpair*>(v.diag32)[2 * node];
+        const dpair dl = reinterpret_cast<const dpair*>(v.diag32)[2 * node + 1];
+        const dpair x = p2[node];
+        a0 += du.x * x.x;
+        a0 += du.y * x.y;
+        a1 += dl.x * x.x;
+        a1 += dl.y * x.y;
     }
 }
 
@@ -1509,9 +1576,11 @@ __device__ __forceinline__ void sym_slice_rows(const SymSliceView& v, const doub
 // (as pcg_spmv_slices_kernel).
 template <int kWL, int kWU>
 __global__ __launch_bounds__(kThreads) void pcg_spmv_sym_slices_kernel(PcgBuffers b, SymSliceView v,
-                                                                       const double* __restrict__ planes) {
+                                                                       const double* __restrict__ planes64,
+                                                                       const float* __restrict__ planes32) {
     if (b.st->done) return;
     __shared__ double sh[kThreads / kWave];
+    const bool use32 = planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int per_block = (v.n_slices + gridDim.x - 1) / gridDim.x;
     const int start = lb * per_block;
@@ -1520,7 +1589,10 @@ __global__ __launch_bounds__(kThreads) void pcg_spmv_sym_slices_kernel(PcgBuffer
     double pq = 0.0;
     for (int s = start + static_cast<int>(threadIdx.x) / kWave; s < stop; s += kThreads / kWave) {
         double a0 = 0.0, a1 = 0.0;
-        sym_slice_rows<kWL, kWU>(v, planes, p2, s, a0, a1);
+        if (use32)
+            sym_slice_rows<float, kWL, kWU>(v, planes32, p2, s, a0, a1);
+        else
+            sym_slice_rows<double, kWL, kWU>(v, planes64, p2, s, a0, a1);
         const int node = s * kWave + (static_cast<int>(threadIdx.x) & (kWave - 1));
         if (node < (b.n >> 1)) {
             const dpair pv = p2[node];
@@ -1545,15 +1617,18 @@ __global__ __launch_bounds__(kThreads) void pcg_spmv_sym_slices_kernel(PcgBuffer
 }
 
 // The multiply launch of variant 15: pcg_slices_multiply_kernel's sums and state update
-// with q' = A p' on the symmetric node slices.
-template <int kWL, int kWU, int Prec>
+// with q' = A p' on the symmetric node slices. kMixed: the FP32 copy after the switch (an
+// FP64 solve launches the instantiation without that path, which holds fewer registers).
+template <int kWL, int kWU, int Prec, bool kMixed>
 __global__ __launch_bounds__(kThreads) void pcg_sym_slices_multiply_kernel(PcgBuffers b, SymSliceView v,
-                                                                           const double* __restrict__ planes,
+                                                                           const double* __restrict__ planes64,
+                                                                           const float* __restrict__ planes32,
                                                                            const double* __restrict__ r_new,
                                                                            const double* __restrict__ p_new) {
     if (b.st->done) return;
     __shared__ double sh[3 * (kThreads / kWave)];
     const double alpha = b.st->alpha;
+    const bool use32 = kMixed && planes32 != nullptr && (planes64 == nullptr || b.st->phase32 != 0);
     constexpr int kWaves = kThreads / kWave;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int per_block = (v.n_slices + gridDim.x - 1) / gridDim.x;
@@ -1572,7 +1647,10 @@ __global__ __launch_bounds__(kThreads) void pcg_sym_slices_multiply_kernel(PcgBu
         NodePrecond<Prec> mo;
         mo.load(b.m, ic);
         double a0 = 0.0, a1 = 0.0;
-        sym_slice_rows<kWL, kWU>(v, planes, p2, s, a0, a1);
+        if (use32)
+            sym_slice_rows<float, kWL, kWU>(v, planes32, p2, s, a0, a1);
+        else
+            sym_slice_rows<double, kWL, kWU>(v, planes64, p2, s, a0, a1);
         if (own) {
             if (qi >= 0) reinterpret_cast<dpair*>(b.q)[qi] = qv;  // an earlier slice of this wavefront
             qi = node;
@@ -1699,47 +1777,69 @@ void launch_pcg_split_slices(hipStream_t s, const PcgBuffers& b, const SliceView
 }
 
 void launch_pack_sym_slices(hipStream_t s, const CsrView& a, const SymSliceView& v, const double* values,
-                            double* planes) {
+                            double* planes64, float* planes32) {
     const long long threads = static_cast<long long>(v.n_slices) * kWave;
-    pack_sym_slices_kernel<<<blocks_for(threads > 0 ? threads : 1, 256), 256, 0, s>>>(a.n >> 1, a.row_ptr, a.col, v,
-                                                                                      values, planes);
+    const unsigned int grid = blocks_for(threads > 0 ? threads : 1, 256);
+    if (planes64)
+        pack_sym_slices_kernel<double><<<grid, 256, 0, s>>>(a.n >> 1, a.row_ptr, a.col, v, values, planes64);
+    if (planes32) {
+        pack_sym_slices_kernel<float><<<grid, 256, 0, s>>>(a.n >> 1, a.row_ptr, a.col, v, values, planes32);
+        if (a.n > 0)
+            sym_diag32_kernel<<<blocks_for(a.n >> 1, 256), 256, 0, s>>>(a.n >> 1, a.row_ptr, a.col, v, values,
+                                                                        planes32, const_cast<double*>(v.diag32));
+    }
 }
 
-void launch_pcg_spmv_sym_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes) {
+void launch_pcg_spmv_sym_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes64,
+                                const float* planes32) {
     const unsigned int grid = static_cast<unsigned int>(slice_spmv_blocks(v.n_slices));
     if (v.max_lwidth <= 4 && v.max_uwidth <= 5)
-        pcg_spmv_sym_slices_kernel<4, 5><<<grid, kThreads, 0, s>>>(b, v, planes);
+        pcg_spmv_sym_slices_kernel<4, 5><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32);
     else if (v.max_lwidth <= 8 && v.max_uwidth <= 9)
-        pcg_spmv_sym_slices_kernel<8, 9><<<grid, kThreads, 0, s>>>(b, v, planes);
+        pcg_spmv_sym_slices_kernel<8, 9><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32);
     else
-        pcg_spmv_sym_slices_kernel<0, 0><<<grid, kThreads, 0, s>>>(b, v, planes);
+        pcg_spmv_sym_slices_kernel<0, 0><<<grid, kThreads, 0, s>>>(b, v, planes64, planes32);
+}
+
+template <int Prec, bool kMixed>
+static void launch_sym_multiply(hipStream_t s, unsigned int grid, const PcgBuffers& bq, const SymSliceView& v,
+                                const double* planes64, const float* planes32, const double* r_out,
+                                const double* p_out) {
+    if (v.max_lwidth <= 4 && v.max_uwidth <= 5)
+        pcg_sym_slices_multiply_kernel<4, 5, Prec, kMixed><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32, r_out,
+                                                                                     p_out);
+    else if (v.max_lwidth <= 8 && v.max_uwidth <= 9)
+        pcg_sym_slices_multiply_kernel<8, 9, Prec, kMixed><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32, r_out,
+                                                                                     p_out);
+    else
+        pcg_sym_slices_multiply_kernel<0, 0, Prec, kMixed><<<grid, kThreads, 0, s>>>(bq, v, planes64, planes32, r_out,
+                                                                                     p_out);
 }
 
 template <int Prec>
 static void launch_sym_split(hipStream_t s, unsigned int grid, const PcgBuffers& b, const PcgBuffers& bq,
-                             const SymSliceView& v, const double* planes, const double* r_in, double* r_out,
-                             const double* p_in, double* p_out, const double* q_in) {
+                             const SymSliceView& v, const double* planes64, const float* planes32,
+                             const double* r_in, double* r_out, const double* p_in, double* p_out,
+                             const double* q_in) {
     pcg_slices_vector_kernel<Prec><<<grid, kThreads, 0, s>>>(b, r_in, r_out, p_in, p_out, q_in);
-    if (v.max_lwidth <= 4 && v.max_uwidth <= 5)
-        pcg_sym_slices_multiply_kernel<4, 5, Prec><<<grid, kThreads, 0, s>>>(bq, v, planes, r_out, p_out);
-    else if (v.max_lwidth <= 8 && v.max_uwidth <= 9)
-        pcg_sym_slices_multiply_kernel<8, 9, Prec><<<grid, kThreads, 0, s>>>(bq, v, planes, r_out, p_out);
+    if (planes32)
+        launch_sym_multiply<Prec, true>(s, grid, bq, v, planes64, planes32, r_out, p_out);
     else
-        pcg_sym_slices_multiply_kernel<0, 0, Prec><<<grid, kThreads, 0, s>>>(bq, v, planes, r_out, p_out);
+        launch_sym_multiply<Prec, false>(s, grid, bq, v, planes64, planes32, r_out, p_out);
 }
 
-void launch_pcg_sym_split_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes,
-                                 const double* r_in, double* r_out, const double* p_in, double* p_out,
-                                 const double* q_in, double* q_out) {
+void launch_pcg_sym_split_slices(hipStream_t s, const PcgBuffers& b, const SymSliceView& v, const double* planes64,
+                                 const float* planes32, const double* r_in, double* r_out, const double* p_in,
+                                 double* p_out, const double* q_in, double* q_out) {
     PcgBuffers bq = b;
     bq.q = q_out;
     // the vector launch's 256-node ranges are the multiply's 4-slice ranges (same grid)
     const unsigned int need = blocks_for(v.n_slices, kThreads / kWave);
     const unsigned int grid = round8(need < 8192u ? need : 8192u);
     if (b.precond == LFG_PRECOND_BLOCK2)
-        launch_sym_split<LFG_PRECOND_BLOCK2>(s, grid, b, bq, v, planes, r_in, r_out, p_in, p_out, q_in);
+        launch_sym_split<LFG_PRECOND_BLOCK2>(s, grid, b, bq, v, planes64, planes32, r_in, r_out, p_in, p_out, q_in);
     else
-        launch_sym_split<LFG_PRECOND_JACOBI>(s, grid, b, bq, v, planes, r_in, r_out, p_in, p_out, q_in);
+        launch_sym_split<LFG_PRECOND_JACOBI>(s, grid, b, bq, v, planes64, planes32, r_in, r_out, p_in, p_out, q_in);
 }
 
 void launch_pack_stencil(hipStream_t s, const CsrView& a, const StencilView& v, const double* values, double* out64,

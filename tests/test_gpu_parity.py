@@ -259,6 +259,8 @@ def check_stencil_spmv(dev, row_ptr, col, kc, x):
         check_translations(dev, row_ptr, col, kc, 8)
     if info["sym_slices"]:  # symmetric node slices: lower blocks read as transposed upper ones
         assert np.all(np.abs(dev.spmv(x, "fp64", 15) - y_ref) <= 1e-14 * scale)
+        assert check_compensated(dev.spmv(x, "mixed", 15), y_ref, scale, row_abs, x)
+        check_translations(dev, row_ptr, col, kc, 15)
     return info
 
 
@@ -435,11 +437,11 @@ def test_symmetric_slice_iteration(dev, precond):
     """Variant 15, the two-launch iteration on the symmetric node slices (each node's
     blocks from its diagonal on hold values, the blocks before it are read as the
     transposes of the column nodes' blocks), on an RCM-numbered mesh of more than 1,024
-    slices, where it is the FP64 default: the multiply within 1e-14 of |K||x| per row of
-    the CSR product, the PCG against the node-slice iteration (6) to 1e-9 with its
-    iteration count within 1% + 2, graph and direct launches bit-identical, the iteration
-    cap honoured; mixed precision keeps variant 13, and 15 refuses FP32 values, block
-    storage and the sweep preconditioners."""
+    slices, where it is the default: the multiply within 1e-14 of |K||x| per row of the CSR
+    product (its FP32 copy within the compensated bound, nodal translations exact), the PCG
+    against the node-slice iteration (6) to 1e-9 with its iteration count within 1% + 2,
+    graph and direct launches bit-identical, the iteration cap honoured; mixed precision
+    on it within 1e-9 of FP64; 15 refuses block storage and the sweep preconditioners."""
     print(f"case symmetric node slices 262x260 rcm {precond}", flush=True)
     sys_ = pipeline.setup(meshgen.structured_cantilever(262, 260, young=meshgen.loguniform_young(262 * 260, 11)),
                           device=dev, assembly="coloring", reordering="rcm")
@@ -450,6 +452,9 @@ def test_symmetric_slice_iteration(dev, precond):
     y_ref = lfo.spmv(sys_.row_ptr, sys_.col, kc, x)
     scale = lfo.spmv(sys_.row_ptr, sys_.col, np.abs(kc), np.abs(x))
     assert np.all(np.abs(dev.spmv(x, "fp64", 15) - y_ref) <= 1e-14 * scale)
+    row_abs = lfo.spmv(sys_.row_ptr, sys_.col, np.abs(kc), np.ones_like(x))
+    assert check_compensated(dev.spmv(x, "mixed", 15), y_ref, scale, row_abs, x)
+    check_translations(dev, sys_.row_ptr, sys_.col, kc, 15)
     u6, r6 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=6)
     u15, r15 = dev.pcg(precond, "fp64", "relative", 1e-10, spmv_variant=15)
     u0, r0 = dev.pcg(precond, "fp64", "relative", 1e-10)  # the FP64 default at this size
@@ -458,12 +463,15 @@ def test_symmetric_slice_iteration(dev, precond):
     assert np.linalg.norm(u15 - u6) <= 1e-9 * np.linalg.norm(u6)
     assert np.array_equal(bits(u0), bits(u15)) and np.array_equal(bits(ud), bits(u15))
     assert r15["true_residual"] <= 1e-9 * r15["initial_residual"]
-    um, rm = dev.pcg(precond, "mixed", "relative", 1e-10)  # default: the FP32 node slices of 13
-    um13, rm13 = dev.pcg(precond, "mixed", "relative", 1e-10, spmv_variant=13)
-    assert np.array_equal(bits(um), bits(um13)) and rm["iterations"] == rm13["iterations"]
+    um, rm = dev.pcg(precond, "mixed", "relative", 1e-10)  # default: 15 with its FP32 copy
+    um15, rm15 = dev.pcg(precond, "mixed", "relative", 1e-10, spmv_variant=15)
+    assert np.array_equal(bits(um), bits(um15)) and rm["iterations"] == rm15["iterations"]
+    assert rm["converged"] and np.linalg.norm(um - u6) <= 1e-9 * np.linalg.norm(u6)
+    um32, rm32 = dev.pcg(precond, "mixed", "relative", 1e-10, spmv_variant=15, mixed_switch=-1.0)
+    assert rm32["converged"] and np.linalg.norm(um32 - u6) <= 1e-9 * np.linalg.norm(u6)
     _, capped = dev.pcg(precond, "fp64", "relative", 1e-10, max_iterations=5, spmv_variant=15, raise_on_error=False)
     assert capped["iterations"] == 5 and not capped["converged"]
-    for kwargs in ({"precision": "mixed"}, {"storage": "bsr"}, {"preconditioner": "ic0"}):
+    for kwargs in ({"storage": "bsr"}, {"preconditioner": "ic0"}):
         args = {"preconditioner": precond, "precision": "fp64", **kwargs}
         with pytest.raises(native.LightFemError) as err:
             dev.pcg(args["preconditioner"], args["precision"], "relative", 1e-10, spmv_variant=15,

@@ -70,7 +70,7 @@ def check_system(hip, n_mesh: int, cases):
         dev.close()
 
 
-CASES = [("fp64", 4), ("fp64", 6), ("fp64", 8), ("fp64", 9), ("mixed", 4), ("mixed", 6), ("mixed", 8)]
+CASES = [("fp64", 4), ("fp64", 6), ("fp64", 8), ("fp64", 9), ("fp64", 15), ("mixed", 4), ("mixed", 6), ("mixed", 8), ("mixed", 15)]
 
 
 def test_noalloc_matches_spmv_then_smaller_system(hip):
