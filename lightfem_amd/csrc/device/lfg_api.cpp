@@ -2369,26 +2369,41 @@ lfg::BatchView batch_view(lfg_ctx* c, const lfg_batch_options* opt) {
     return bv;
 }
 
-// Launch-shape autotuner: times the batch multiply (the dominant kernel) for each
-// block size and slices-per-wavefront candidate on this batch and keeps the fastest.
-void batch_autotune(lfg_ctx* c, const lfg::BatchView& bv, bool mixed, bool sym) {
+// Launch-shape autotuner: for each block size and slices-per-wavefront candidate, runs
+// a few iterations of this batch from its initial state (which the solve then restores)
+// and keeps the fastest shape; the multiply alone is a poor proxy, since the vector
+// kernels share the shape and a shape's register and cache pressure differs per kernel.
+// Mixed precision times the FP32-value iteration, where the mixed solve spends most of its
+// iterations. Then times the multiply with its per-problem fold at the chosen shape.
+void batch_autotune(lfg_ctx* c, const lfg::BatchView& bv_in, bool mixed, bool sym) {
     const int threads[] = {64, 128, 256, 512};
     const int per_wave[] = {1, 2, 4};
-    lfg::launch_fill(c->stream, c->batch_p.p, static_cast<long long>(c->batch_p.n), 1.0);
+    constexpr int kReps = 4;
+    lfg::BatchView bv = bv_in;
+    if (mixed) bv.switch_tau = -1.0;  // every problem on FP32 values from the first iteration
     const lfg::SliceView v = slice_view(c);
     const lfg::SymSliceView sv = sym_view(c);
-    const double* p64 = mixed ? nullptr : (sym ? c->sym64.p : c->slices64.p);
+    const double* p64 = sym ? c->sym64.p : c->slices64.p;
     const float* p32 = mixed ? c->slices32.p : nullptr;
-    const lfg::SymSliceView* svp = (sym && !mixed) ? &sv : nullptr;
+    const lfg::SymSliceView* svp = sym ? &sv : nullptr;
+    lfg::BatchGlobal start{};
+    start.n_active = c->n_slices;
+    auto iteration = [&](const lfg::BatchShape& shape) {
+        lfg::launch_batch_spmv(c->stream, bv, shape, v, p64, p32, svp);
+        lfg::launch_batch_update(c->stream, bv, shape);
+        lfg::launch_batch_direction(c->stream, bv, shape);
+    };
     float best = 0.0f;
     lfg::BatchShape chosen{256, 1};
     LFG_HIP(hipEventRecord(c->ev_ring[0], c->stream));
     for (int t : threads)
         for (int w : per_wave) {
             const lfg::BatchShape shape{t, w};
-            lfg::launch_batch_spmv(c->stream, bv, shape, v, p64, p32, 1, svp);  // warm-up
+            LFG_HIP(hipMemcpyAsync(c->batch_global.p, &start, sizeof(start), hipMemcpyHostToDevice, c->stream));
+            lfg::launch_batch_init(c->stream, bv, shape);
+            iteration(shape);  // warm-up
             LFG_HIP(hipEventRecord(c->ev_start, c->stream));
-            for (int i = 0; i < 5; ++i) lfg::launch_batch_spmv(c->stream, bv, shape, v, p64, p32, 1, svp);
+            for (int i = 0; i < kReps; ++i) iteration(shape);
             LFG_HIP(hipEventRecord(c->ev_stop, c->stream));
             sync(c);
             const float ms = elapsed(c->ev_start, c->ev_stop);
@@ -2400,7 +2415,16 @@ void batch_autotune(lfg_ctx* c, const lfg::BatchView& bv, bool mixed, bool sym) 
     LFG_HIP(hipEventRecord(c->ev_ring[1], c->stream));
     sync(c);
     c->batch_tune_ms = elapsed(c->ev_ring[0], c->ev_ring[1]);
-    c->batch_multiply_ms = best / 5.0;
+    // the multiply and its fold at the chosen shape from the initial state (every problem
+    // running; the repeated products recompute the same q and alpha)
+    LFG_HIP(hipMemcpyAsync(c->batch_global.p, &start, sizeof(start), hipMemcpyHostToDevice, c->stream));
+    lfg::launch_batch_init(c->stream, bv, chosen);
+    lfg::launch_batch_spmv(c->stream, bv, chosen, v, p64, p32, svp);
+    LFG_HIP(hipEventRecord(c->ev_start, c->stream));
+    for (int i = 0; i < 5; ++i) lfg::launch_batch_spmv(c->stream, bv, chosen, v, p64, p32, svp);
+    LFG_HIP(hipEventRecord(c->ev_stop, c->stream));
+    sync(c);
+    c->batch_multiply_ms = elapsed(c->ev_start, c->ev_stop) / 5.0;
     c->batch_shape = chosen;
     c->batch_tuned_for = mixed ? LFG_PRECISION_MIXED : LFG_PRECISION_FP64;
 }
@@ -2419,34 +2443,49 @@ long long batch_solve(lfg_ctx* c, const lfg_batch_options* opt, const lfg::Batch
     start.n_active = c->n_slices;
     LFG_HIP(hipMemcpyAsync(c->batch_global.p, &start, sizeof(start), hipMemcpyHostToDevice, c->stream));
     lfg::launch_batch_init(c->stream, bv, shape);
-    auto iteration = [&] {
-        lfg::launch_batch_spmv(c->stream, bv, shape, v, sym ? c->sym64.p : c->slices64.p,
-                               mixed ? c->slices32.p : nullptr, 0, sym ? &sv : nullptr);
-        lfg::launch_batch_update(c->stream, bv, shape);
-        lfg::launch_batch_direction(c->stream, bv, shape);
+    auto iteration = [&](const lfg::BatchShape& sh) {
+        lfg::launch_batch_spmv(c->stream, bv, sh, v, sym ? c->sym64.p : c->slices64.p,
+                               mixed ? c->slices32.p : nullptr, sym ? &sv : nullptr);
+        lfg::launch_batch_update(c->stream, bv, sh);
+        lfg::launch_batch_direction(c->stream, bv, sh);
     };
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    LFG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < per_graph; ++i) iteration();
-    // drop the problems that stopped during these iterations from the active list (until
-    // then every kernel skips them by their done flag)
-    lfg::launch_batch_compact(c->stream, bv);
-    LFG_HIP(hipStreamEndCapture(c->stream, &graph));
-    struct GraphGuard {
-        hipGraph_t g;
-        hipGraphExec_t e;
-        ~GraphGuard() {
-            if (e) (void)hipGraphExecDestroy(e);
-            if (g) (void)hipGraphDestroy(g);
+    // One graph per grid span: every slice, then a half, a quarter, ... (down to 256
+    // slices). The host launches the smallest whose grid still gives each slice of the
+    // active list a wavefront, by the active count it last read (the list only shrinks, so
+    // the count when the graph runs is never larger): a launch in the tail then dispatches
+    // blocks for the running problems' slices, not for every slice of the batch.
+    std::vector<int> spans{c->n_slices};
+    for (int sp = c->n_slices / 2; sp >= 256; sp /= 2) spans.push_back(sp);
+    struct Graphs {
+        std::vector<hipGraph_t> g;
+        std::vector<hipGraphExec_t> e;
+        ~Graphs() {
+            for (hipGraphExec_t x : e) (void)hipGraphExecDestroy(x);
+            for (hipGraph_t x : g) (void)hipGraphDestroy(x);
         }
-    } guard{graph, nullptr};
-    LFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    guard.e = exec;
+    } graphs;
+    for (int sp : spans) {
+        lfg::BatchShape sh = shape;
+        sh.span = sp;
+        hipGraph_t graph = nullptr;
+        LFG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < per_graph; ++i) iteration(sh);
+        // drop the problems that stopped during these iterations from the active list (until
+        // then every kernel skips them by their done flag)
+        lfg::launch_batch_compact(c->stream, bv);
+        LFG_HIP(hipStreamEndCapture(c->stream, &graph));
+        graphs.g.push_back(graph);
+        hipGraphExec_t exec = nullptr;
+        LFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        graphs.e.push_back(exec);
+    }
     const long long launch_cap = std::max<long long>(1, (opt->max_iterations + per_graph - 1) / per_graph);
     long long launched = 0;
+    int active = c->n_slices;  // the active slice count the host read last
     auto launch_one = [&](int slot) {
-        LFG_HIP(hipGraphLaunch(exec, c->stream));
+        size_t k = 0;
+        while (k + 1 < spans.size() && spans[k + 1] >= active) ++k;
+        LFG_HIP(hipGraphLaunch(graphs.e[k], c->stream));
         LFG_HIP(hipMemcpyAsync(c->h_batch_global + slot, c->batch_global.p, sizeof(lfg::BatchGlobal),
                                hipMemcpyDeviceToHost, c->stream));
         LFG_HIP(hipEventRecord(c->ev_ring[slot], c->stream));
@@ -2458,6 +2497,7 @@ long long batch_solve(lfg_ctx* c, const lfg_batch_options* opt, const lfg::Batch
         if (launched < launch_cap) launch_one(1 - slot);
         LFG_HIP(hipEventSynchronize(c->ev_ring[slot]));
         if (c->h_batch_global[slot].done_all || launched >= launch_cap) break;
+        active = c->h_batch_global[slot].n_active;
     }
     sync(c);
     return launched;

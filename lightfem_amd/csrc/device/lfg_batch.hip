@@ -6,8 +6,8 @@
 // (x[node][case][2]): one load of a 2x2 block of K serves all NC right-hand sides, and
 // the p entries the block multiplies arrive as NC consecutive pairs. Every system
 // (problem, case) has its own CG scalars and convergence flag; per-problem sums
-// (p.Ap, r.r, r.z) are finished in-kernel by the problem's last wavefront (a ticket
-// per problem, partials folded in slice order, so results are run-to-run identical).
+// (p.Ap, r.r, r.z) are written as one partial per slice and folded in slice order by
+// small finish kernels (one wavefront per problem), so results are run-to-run identical.
 // A problem whose cases have all stopped is skipped by every later kernel, and every
 // kernel returns at once when all problems have stopped, so a captured graph of many
 // iterations costs almost nothing after convergence.
@@ -420,31 +420,6 @@ __global__ __launch_bounds__(256) void batch_spmv_finish_kernel(BatchView bv) {
     close_problem_if_done(bv, p);
 }
 
-// Plain q = K p over all slices (no reductions, no state): timing the launch shapes.
-template <int NC, typename VT, int kW>
-__global__ __launch_bounds__(512) void batch_multiply_kernel(BatchView bv, SliceView v, const VT* __restrict__ planes) {
-    const SliceRange range = block_slices(bv.n_slices);
-    for (int s = range.first; s < range.stop; s += range.step) {
-        double pq[NC];
-        dpair qv[NC];
-        slice_multiply<NC, VT, kW>(bv, v, planes, s, pq, qv);
-        store_slice<NC>(bv.q, s, qv);
-    }
-}
-
-// The same on the symmetric node slices (FP64).
-template <int NC, int kSym>
-__global__ __launch_bounds__(512) void batch_sym_multiply_kernel(BatchView bv, SliceView v, SymSliceView sv,
-                                                                 const double* __restrict__ planes) {
-    const SliceRange range = block_slices(bv.n_slices);
-    for (int s = range.first; s < range.stop; s += range.step) {
-        double pq[NC];
-        dpair qv[NC];
-        fp64_multiply<NC, 0, kSym>(bv, v, sv, planes, s, pq, qv);
-        store_slice<NC>(bv.q, s, qv);
-    }
-}
-
 // ------------------------------------------------------------ update and direction
 
 // x += alpha p, r -= alpha q, z = M^-1 r; the slice's per-case ||r||^2 and r.z partials
@@ -706,6 +681,7 @@ inline unsigned int blocks_for(long long work, int per_block) {
 
 unsigned int batch_grid(const BatchShape& shape, int n_slices) {
     const int waves = shape.block_threads / kWave;
+    if (shape.span > 0 && shape.span < n_slices) n_slices = shape.span;
     const long long need = (static_cast<long long>(n_slices) + waves * shape.slices_per_wave - 1) /
                            (static_cast<long long>(waves) * shape.slices_per_wave);
     const unsigned int g = static_cast<unsigned int>(need < 1 ? 1 : need);
@@ -726,36 +702,30 @@ void launch_batch_init(hipStream_t s, const BatchView& bv, const BatchShape& sha
 
 template <int NC, int kW, int kSym>
 static void spmv_nc(hipStream_t s, const BatchView& bv, const SliceView& v, const SymSliceView& sv,
-                    const double* planes64, const float* planes32, unsigned int grid, int threads, bool plain) {
-    if (plain) {
-        if (planes64 && kSym) batch_sym_multiply_kernel<NC, kSym><<<grid, threads, 0, s>>>(bv, v, sv, planes64);
-        else if (planes64) batch_multiply_kernel<NC, double, kW><<<grid, threads, 0, s>>>(bv, v, planes64);
-        else batch_multiply_kernel<NC, float, kW><<<grid, threads, 0, s>>>(bv, v, planes32);
-    } else {
-        if (planes32) batch_spmv_kernel<NC, kW, true, kSym><<<grid, threads, 0, s>>>(bv, v, sv, planes64, planes32);
-        else batch_spmv_kernel<NC, kW, false, kSym><<<grid, threads, 0, s>>>(bv, v, sv, planes64, nullptr);
-        const unsigned int fin = blocks_for(static_cast<long long>(bv.n_problems) * kWave, 256);
-        batch_spmv_finish_kernel<NC><<<fin, 256, 0, s>>>(bv);
-    }
+                    const double* planes64, const float* planes32, unsigned int grid, int threads) {
+    if (planes32) batch_spmv_kernel<NC, kW, true, kSym><<<grid, threads, 0, s>>>(bv, v, sv, planes64, planes32);
+    else batch_spmv_kernel<NC, kW, false, kSym><<<grid, threads, 0, s>>>(bv, v, sv, planes64, nullptr);
+    const unsigned int fin = blocks_for(static_cast<long long>(bv.n_problems) * kWave, 256);
+    batch_spmv_finish_kernel<NC><<<fin, 256, 0, s>>>(bv);
 }
 
 void launch_batch_spmv(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
-                       const double* planes64, const float* planes32, int plain, const SymSliceView* sym) {
+                       const double* planes64, const float* planes32, const SymSliceView* sym) {
     const unsigned int grid = batch_grid(shape, bv.n_slices);
     const int t = shape.block_threads;
     const SymSliceView sv = sym ? *sym : SymSliceView{};
     const int ksym = !sym ? 0 : (sym->max_lwidth <= 4 && sym->max_uwidth <= 5 ? 1 : 2);
     // kW is the FP32 node slices' unrolled width (mixed), whatever the FP64 layout
     if (ksym == 1 && v.max_width <= 9) {
-        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 9, 1>(s, bv, v, sv, planes64, planes32, grid, t, plain != 0)));
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 9, 1>(s, bv, v, sv, planes64, planes32, grid, t)));
     } else if (ksym == 1) {
-        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 0, 1>(s, bv, v, sv, planes64, planes32, grid, t, plain != 0)));
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 0, 1>(s, bv, v, sv, planes64, planes32, grid, t)));
     } else if (ksym == 2) {
-        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 0, 2>(s, bv, v, sv, planes64, planes32, grid, t, plain != 0)));
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 0, 2>(s, bv, v, sv, planes64, planes32, grid, t)));
     } else if (v.max_width <= 9) {
-        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 9, 0>(s, bv, v, sv, planes64, planes32, grid, t, plain != 0)));
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 9, 0>(s, bv, v, sv, planes64, planes32, grid, t)));
     } else {
-        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 0, 0>(s, bv, v, sv, planes64, planes32, grid, t, plain != 0)));
+        LFG_BATCH_NC(bv.n_cases, (spmv_nc<NC, 0, 0>(s, bv, v, sv, planes64, planes32, grid, t)));
     }
 }
 

@@ -434,16 +434,16 @@ struct BatchView {
 struct BatchShape {
     int block_threads;    // 64 .. 512
     int slices_per_wave;  // slices each wavefront walks per launch
+    int span = 0;         // grid sized for this many slices (0: every slice); the iteration
+                          // kernels walk the active list with any grid
 };
 unsigned int batch_grid(const BatchShape& shape, int n_slices);
 void launch_batch_init(hipStream_t s, const BatchView& bv, const BatchShape& shape);
-// plain = 1: the multiply alone (autotuning) with one of planes64 / planes32, no
-// reductions or state. Otherwise planes32 non-null selects the mixed kernel (both
-// plane sets, per-problem phase). sym non-null: the FP64 values are the symmetric node
-// slices' (planes64 = their planes), the FP32 ones stay the node slices'.
+// q = K p with the per-problem p.q fold and alpha. planes32 non-null selects the mixed
+// kernel (both plane sets, per-problem phase). sym non-null: the FP64 values are the
+// symmetric node slices' (planes64 = their planes), the FP32 ones stay the node slices'.
 void launch_batch_spmv(hipStream_t s, const BatchView& bv, const BatchShape& shape, const SliceView& v,
-                       const double* planes64, const float* planes32, int plain,
-                       const SymSliceView* sym = nullptr);
+                       const double* planes64, const float* planes32, const SymSliceView* sym = nullptr);
 void launch_batch_update(hipStream_t s, const BatchView& bv, const BatchShape& shape);
 void launch_batch_direction(hipStream_t s, const BatchView& bv, const BatchShape& shape);
 // Rebuilds the active slice list without the problems that stopped since the last
