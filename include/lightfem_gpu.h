@@ -276,8 +276,9 @@ int lfg_batch_set_loads(lfg_ctx* ctx, int32_t n_cases, const double* loads, int6
 typedef struct {
     int32_t precision;        /* LFG_PRECISION_FP64, or MIXED (see mixed_switch) */
     int32_t stop_rule;        /* LFG_STOP_* on each system's own ||b|| */
-    int32_t check_every;      /* iterations per captured graph; 0 = 32 */
-    int32_t tune;             /* 1: choose the launch shape by timing candidates (cached per context) */
+    int32_t check_every;      /* iterations per captured graph; 0 = 16 */
+    int32_t tune;             /* 1: choose the launch shape by timing a few iterations of this batch per
+                                 candidate shape (cached per context and precision) */
     double tolerance;
     int64_t max_iterations;   /* per system */
     double mixed_switch;      /* mixed: a problem multiplies with FP64 values until every running case's
@@ -307,8 +308,8 @@ typedef struct {
     int32_t slices_per_wave;
     int32_t systems_converged;
     int64_t max_iterations;   /* over systems */
-    double multiply_ms;       /* the tuned batch multiply (with its per-problem fold), one launch with
-                                 every problem running: device time of the fastest launch shape */
+    double multiply_ms;       /* the batch multiply (with its per-problem fold) at the tuned launch shape,
+                                 one launch with every problem running, timed by the last tuning */
     int32_t sym_slices;       /* 1: FP64 products on the symmetric node slices (lower blocks read as the
                                  mirrored upper ones), 0: on the node slices */
 } lfg_batch_summary;
